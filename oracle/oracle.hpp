@@ -1,0 +1,266 @@
+// oracle.hpp — FP64 CPU restatement of the reference voxel-SIMP inner loop.
+//
+// TEST INFRASTRUCTURE ONLY. This is the checker the HIP path is compared
+// against (and the CPU baseline that is timed beside it). Nothing in the
+// product library links, loads or calls it; only tests/, the smoke check and
+// the bench's cpu_baseline leg do.
+//
+// Every function names the reference file:line it restates. Paths are relative
+// to the reference checkout (core/...). Parity status of each part (which
+// golden vector / closed form / invariant pins it) is listed in the test files
+// under tests/test_oracle_*.py.
+#pragma once
+
+#include <array>
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace oracle {
+
+constexpr int kDof = 24;
+constexpr int kNumColors = 8;
+
+// Voxel grid: x-fastest, then y, then z (core/include/topopt/voxel.hpp:64-69).
+// `solid` is one byte per voxel (non-zero = solid); empty vector = all solid.
+struct Grid {
+  int nx = 0, ny = 0, nz = 0;
+  double h = 1.0;
+  std::vector<std::uint8_t> solid;
+
+  std::size_t voxels() const {
+    return static_cast<std::size_t>(nx) * static_cast<std::size_t>(ny) *
+           static_cast<std::size_t>(nz);
+  }
+  std::size_t index(int i, int j, int k) const {
+    return (static_cast<std::size_t>(k) * static_cast<std::size_t>(ny) +
+            static_cast<std::size_t>(j)) *
+               static_cast<std::size_t>(nx) +
+           static_cast<std::size_t>(i);
+  }
+  bool is_solid(int i, int j, int k) const {
+    return solid.empty() || solid[index(i, j, k)] != 0;
+  }
+  std::size_t solid_count() const;
+  int nnx() const { return nx + 1; }
+  int nny() const { return ny + 1; }
+  int nnz() const { return nz + 1; }
+  int nodes() const { return nnx() * nny() * nnz(); }
+  // core/src/fea/assembly.cpp:41-43
+  int node_index(int a, int b, int c) const { return (c * nny() + b) * nnx() + a; }
+};
+
+struct DirichletBC { int node = 0; int comp = 0; double value = 0.0; };
+struct NodalLoad { int node = 0; int comp = 0; double value = 0.0; };
+
+// core/include/topopt/fea.hpp:418-426
+struct SolverNonConvergence : public std::runtime_error {
+  int iterations;
+  double residual;
+  SolverNonConvergence(const std::string& w, int it, double r)
+      : std::runtime_error(w), iterations(it), residual(r) {}
+};
+
+// Subset of CgInfo (core/include/topopt/fea.hpp:272-381).
+struct CgInfo {
+  bool converged = false;
+  int iterations = 0;
+  double residual = 0.0;
+  bool used_multigrid = false;
+  int mg_levels = 0;
+  bool hier_built = false;
+  int mg_cycles_attempted = 0;
+  long long matvecs = 0;
+  double t_build_ms = 0.0, t_mg_build_ms = 0.0, t_mg_ms = 0.0, t_cg_ms = 0.0,
+         t_total_ms = 0.0;
+};
+
+// ---- L1 element ------------------------------------------------------------
+// hex8_stiffness(E, nu, h): core/src/fea/hex_element.cpp:23-114. Row-major 24x24.
+void hex8_stiffness(double E, double nu, double h, double* K576);
+// Hex8 node order (bottom CCW then top CCW): core/src/fea/assembly.cpp:45-54.
+std::array<int, 8> element_nodes(const Grid& g, int i, int j, int k);
+// core/src/fea/assembly.cpp:70-116 (self-weight, 1/8 lumped per corner node).
+std::vector<NodalLoad> self_weight_loads(const Grid& g, double density,
+                                         double gravity, const double dir[3]);
+
+// ---- L2 matrix-free operator ----------------------------------------------
+// MfElem (core/src/fea/fea_matfree.hpp:56-59).
+struct MfElem {
+  double factor = 1.0;
+  int edof[kDof];
+};
+
+void set_threads(int n);  // 0 = hardware concurrency
+int threads();
+
+// Colour-sorted element table: core/src/fea/matfree.cpp:422-472.
+std::vector<MfElem> build_elems(const Grid& g, const std::vector<double>* youngs,
+                                std::vector<int>* color_offsets);
+// y = K x over the full stiffness, colour 0..7 in fixed order, column-AXPY
+// mul+add per element: core/src/fea/matfree.cpp:75-123, 535-553.
+void apply_full(const std::vector<MfElem>& elems, const std::vector<int>& coff,
+                const double* Ke, const double* x, double* y, int ndof);
+// fea_matfree_apply graded overload: core/src/fea/matfree.cpp:1085-1104,1190.
+std::vector<double> matfree_apply(const Grid& g, const std::vector<double>& youngs,
+                                  double nu, const std::vector<double>& u);
+
+// Reduced, void-gated system: core/src/fea/matfree.cpp:670-834.
+struct Reduced {
+  int ndof = 0;
+  int ng = 0;
+  double Ke[kDof * kDof];
+  std::vector<MfElem> elems;
+  std::vector<int> color_offsets;
+  std::vector<double> up;           // prescribed field (full)
+  std::vector<int> kept_global;     // kept id -> global dof
+  std::vector<double> rg;           // reduced RHS
+  std::vector<double> invdiag;      // 1 / diag
+  mutable std::vector<double> xfull, yfull;
+  mutable long long matvecs = 0;
+  // core/src/fea/matfree.cpp:595-608
+  void apply_kgg(const double* xg, double* yg) const;
+};
+Reduced build_reduced(const Grid& g, const std::vector<double>& youngs, double nu,
+                      const std::vector<DirichletBC>& bcs,
+                      const std::vector<NodalLoad>& loads, CgInfo* info);
+
+// Jacobi-PCG (Eigen ConjugateGradient algorithm): core/src/fea/matfree.cpp:841-1076.
+void cg_solve(const Reduced& m, double tol, int max_it, std::vector<double>& x,
+              int& iters, double& err, bool& converged);
+
+// ---- L2 multigrid ---------------------------------------------------------
+// Compressed sparse row matrix (the restatement's stand-in for Eigen::SparseMatrix).
+struct Csr {
+  int rows = 0, cols = 0;
+  std::vector<int> ptr, col;
+  std::vector<double> val;
+  void spmv(const double* x, double* y) const;   // y = A x
+  void spmv_t(const double* x, double* y) const; // y = A^T x (y sized cols)
+};
+
+// One coarse level: core/src/fea/multigrid.cpp:474-488.
+struct Level {
+  int nx = 0, ny = 0, nz = 0;  // node dims
+  int n = 0;
+  Csr A;
+  std::vector<double> Dinv;
+  Csr P;  // prolongation coarse(level+1) -> this
+  std::vector<int> active;
+  bool coarsest = false;
+  // Envelope Cholesky factor of the coarsest operator (oracle.cpp).
+  struct Factor {
+    int n = 0;
+    std::vector<int> perm, first;
+    std::vector<std::size_t> off;
+    std::vector<double> L;
+  } chol;
+};
+
+struct Hierarchy {
+  const Reduced* m = nullptr;
+  Csr P0;                     // fine kept rows x level-1 cols
+  std::vector<Level> coarse;  // levels 1..L
+  int levels() const { return 1 + static_cast<int>(coarse.size()); }
+};
+
+// build_mf_hierarchy: core/src/fea/multigrid.cpp:1344-1515 (geometric branch,
+// unpadded), galerkin_a1_from_prolong :1543-1868, build_hierarchy :557-716.
+bool build_mf_hierarchy(const Reduced& m, int nnx, int nny, int nnz, Hierarchy& H);
+// mf_v_cycle: core/src/fea/multigrid.cpp:1143-1160 (+ v_cycle :798-819).
+void mf_v_cycle(const Hierarchy& H, const std::vector<double>& b,
+                std::vector<double>& x);
+// mf_mgpcg: core/src/fea/multigrid.cpp:1211-1318.
+bool mf_mgpcg(const Hierarchy& H, const std::vector<double>& b, double tol,
+              int max_it, std::vector<double>& x, int& iters, double& resid);
+
+// solve_mgcg_matfree / fea_solve_mgcg_matfree (graded):
+// core/src/fea/multigrid.cpp:2098-2319, 2540-2570. Returns full u (3*nodes).
+std::vector<double> solve_mgcg_matfree(const Grid& g, const std::vector<double>& youngs,
+                                       double nu, const std::vector<DirichletBC>& bcs,
+                                       const std::vector<NodalLoad>& loads,
+                                       double tol, int max_it, CgInfo* info,
+                                       const std::vector<double>* initial_guess = nullptr);
+// fea_solve_cg_matfree (graded, Jacobi-CG only): core/src/fea/matfree.cpp:1190-1290.
+std::vector<double> solve_cg_matfree(const Grid& g, const std::vector<double>& youngs,
+                                     double nu, const std::vector<DirichletBC>& bcs,
+                                     const std::vector<NodalLoad>& loads, double tol,
+                                     int max_it, CgInfo* info);
+
+// ---- L3 SIMP ----------------------------------------------------------------
+struct SimpParams {
+  double E0 = 1.0, nu = 0.3, penalty = 3.0, density_min = 1e-3;
+};
+// clamp_density + simp_youngs: core/src/simp/simp.cpp:44-46, 299-303.
+double simp_youngs(const SimpParams& p, double rho);
+
+struct SimpCompliance {
+  double compliance = 0.0;
+  std::vector<double> dc;  // per voxel
+  std::vector<double> u;   // full displacement
+  CgInfo cg;
+};
+// simp_compliance (MultigridCG_Matfree route + sensitivity sweep):
+// core/src/simp/simp.cpp:567-828.
+SimpCompliance simp_compliance(const Grid& g, const SimpParams& p,
+                               const std::vector<double>& density,
+                               const std::vector<DirichletBC>& bcs,
+                               const std::vector<NodalLoad>& loads, double tol,
+                               int max_it, bool use_mg = true);
+
+// DensityFilter (core/include/topopt/simp.hpp:387-406; core/src/simp/simp.cpp:834-944).
+struct DensityFilter {
+  std::size_t voxel_count = 0;
+  std::vector<std::size_t> row_start, neighbor;
+  std::vector<double> weight, weight_sum;
+  std::vector<double> filter_density(const std::vector<double>& x) const;
+  std::vector<double> filter_sensitivity(const std::vector<double>& g) const;
+};
+DensityFilter make_density_filter(const Grid& g, double radius);
+
+// oc_update: core/src/simp/simp.cpp:948-1032. `bisections` (optional) reports
+// the number of bisection steps taken and `lambda` the multiplier used.
+std::vector<double> oc_update(const Grid& g, const DensityFilter& f,
+                              const std::vector<double>& x,
+                              const std::vector<double>& dcompliance,
+                              double volfrac, double move, double density_min,
+                              int* bisections = nullptr, double* lambda = nullptr);
+
+// penalty_continuation_ramp / penalty_at_iteration: core/src/simp/simp.cpp:381-427.
+struct PenaltyStage { double penalty; int iterations; };
+double penalty_at_iteration(const std::vector<PenaltyStage>& s, int iteration);
+std::vector<PenaltyStage> penalty_continuation_ramp(double p0, double p1, double step,
+                                                    int per_stage);
+
+struct SimpOptions {
+  double volume_fraction = 0.5;
+  double filter_radius = 1.5;
+  double move = 0.2;
+  int max_iterations = 200;
+  double change_tol = 0.01;
+  double cg_tolerance = 1e-8;
+  int cg_max_iterations = 0;
+  bool use_mg = true;
+  std::vector<PenaltyStage> penalty_continuation;
+};
+struct SimpIteration { double compliance, change, volume_fraction; int cg_iterations; };
+struct SimpResult {
+  std::vector<double> design, physical_density;
+  std::vector<SimpIteration> history;
+  double compliance = 0.0, initial_compliance = 0.0, volume_fraction = 0.0;
+  int iterations = 0;
+  bool converged = false, non_convergent = false;
+};
+// simp_optimize plain OC loop: core/src/simp/simp.cpp:2018-2608 (single
+// unprojected stage, OC updater, no active domain, cold solves).
+SimpResult simp_optimize(const Grid& g, const SimpParams& p,
+                         const std::vector<DirichletBC>& bcs,
+                         const std::vector<NodalLoad>& loads, const SimpOptions& o);
+
+// ---- Heaviside projection (core/src/simp/simp.cpp:352-379) ---------------
+double heaviside_project(double rt, double beta, double eta);
+double heaviside_project_derivative(double rt, double beta, double eta);
+
+}  // namespace oracle
