@@ -1,0 +1,209 @@
+/* topopt_hip.h — C ABI of the MI355X voxel-SIMP engine (libtopopt_hip.so).
+ *
+ * Drop-in boundary for the reference core's matrix-free multigrid solve and
+ * the SIMP inner loop around it. Every entry point below names the reference
+ * interface it replaces (paths relative to the reference checkout). Plain C
+ * types only: host buffers are caller-owned, device buffers are handle-owned.
+ *
+ * Numbering is the reference's: voxel (i,j,k) -> (k*ny + j)*nx + i; node
+ * (a,b,c) -> (c*(ny+1) + b)*(nx+1) + a; DOF = 3*node + component (x,y,z);
+ * displacement vectors are interleaved xyz FP64 of length 3*(nx+1)(ny+1)(nz+1)
+ * (core/include/topopt/voxel.hpp:64-69, core/src/fea/assembly.cpp:41-54).
+ *
+ * Status codes map 1:1 onto the reference exception taxonomy
+ * (core/include/topopt/fea.hpp:418-469):
+ *   TOPOHIP_EINVAL           std::invalid_argument (sizes, indices, material)
+ *   TOPOHIP_EUNDERCONSTRAINED std::runtime_error from the void-DOF gate
+ *   TOPOHIP_ENONCONVERGENCE  SolverNonConvergence (iterations/residual in info)
+ *   TOPOHIP_EDEADLINE        SolverDeadlineExceeded
+ *   TOPOHIP_EDEVICE          a HIP runtime failure
+ * topohip_last_error() returns the message of the calling thread's last error.
+ *
+ * Threading: one handle per host thread. A problem handle owns one HIP stream;
+ * calls on different handles may run concurrently.
+ */
+#ifndef TOPOPT_HIP_H
+#define TOPOPT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  TOPOHIP_OK = 0,
+  TOPOHIP_EINVAL = 1,
+  TOPOHIP_EUNDERCONSTRAINED = 2,
+  TOPOHIP_ENONCONVERGENCE = 3,
+  TOPOHIP_EDEADLINE = 4,
+  TOPOHIP_EDEVICE = 5,
+  TOPOHIP_EINTERNAL = 6
+};
+
+/* Solver precision of a problem handle.
+ *   TOPOHIP_PREC_MIXED: FP32 K.u and FP32 multigrid V-cycle (FP64 coarsest
+ *     solve) inside an FP64 outer PCG (x, r, p, dot products FP64) with an FP64
+ *     true-residual check before convergence is declared.
+ *   TOPOHIP_PREC_F64: every operator apply and the whole V-cycle in FP64 (the
+ *     plain FP64 form). */
+enum { TOPOHIP_PREC_MIXED = 0, TOPOHIP_PREC_F64 = 1 };
+
+typedef struct topohip_problem topohip_problem;
+typedef struct topohip_server topohip_server;
+
+/* Subset of CgInfo (core/include/topopt/fea.hpp:272-381) plus the engine's own
+ * counters. */
+typedef struct {
+  int converged;
+  int iterations;            /* outer PCG iterations */
+  int used_multigrid;
+  int mg_levels;
+  int hier_built;
+  int mg_cycles_attempted;
+  double residual;           /* final ||b - A x|| / ||b|| (FP64 true residual) */
+  long long matvecs;         /* fine-level operator applies */
+  double t_build_ms, t_mg_build_ms, t_mg_ms, t_cg_ms, t_total_ms;
+  int residual_replacements; /* mixed form: FP64 true-residual restarts */
+  int coarse_dofs;           /* size of the coarsest (direct) level */
+} topohip_cginfo;
+
+/* Create a problem on device 0 (or the device set by topohip_set_device).
+ * Replaces the setup half of mf_build_reduced (core/src/fea/matfree.cpp:670-834):
+ * grid, Poisson ratio, element size, solid tags (NULL = every voxel solid) and
+ * Dirichlet BCs (node, component, value) as in DirichletBC
+ * (core/include/topopt/fea.hpp:180-184). Validates indices like the reference
+ * (TOPOHIP_EINVAL on a BC index out of range or a bad material). */
+int topohip_problem_create(int nx, int ny, int nz, double h, double poisson,
+                           const uint8_t* solid, int n_bc, const int32_t* bc_node,
+                           const int32_t* bc_comp, const double* bc_val, int precision,
+                           topohip_problem** out);
+int topohip_problem_destroy(topohip_problem* p);
+
+/* Load cases: n_rhs right-hand sides; case r holds counts[r] consecutive
+ * (node, comp, value) entries of the concatenated arrays, summed per DOF as
+ * NodalLoad vectors are (core/include/topopt/fea.hpp:188-192;
+ * core/src/fea/matfree.cpp:726-732). */
+int topohip_set_loads(topohip_problem* p, int n_rhs, const int32_t* counts,
+                      const int32_t* node, const int32_t* comp, const double* val);
+
+/* y = K(E) u over the full stiffness of the solid voxels (no BCs), matrix-free.
+ * Mirrors fea_matfree_apply's graded overload (core/include/topopt/fea.hpp:
+ * 763-766; core/src/fea/matfree.cpp:1085-1104): youngs per voxel (> 0 on solid
+ * voxels), u and y of length 3*nodes. _f32 computes in FP32, _f64 in FP64. */
+int topohip_ku_apply_f32(topohip_problem* p, const double* youngs, const double* u,
+                         double* y);
+int topohip_ku_apply_f64(topohip_problem* p, const double* youngs, const double* u,
+                         double* y);
+
+/* Multigrid-preconditioned CG solve of the BC-reduced, void-gated system for
+ * load case 0. Mirrors fea_solve_mgcg_matfree's graded overload
+ * (core/include/topopt/fea.hpp:707-715; core/src/fea/multigrid.cpp:2098-2319):
+ * stop on ||r|| <= tol*||b||; max_it <= 0 selects max(1000, 2*ng) capped at the
+ * 300-cycle MG budget, then the Jacobi-CG fallback; `guess` (NULL = cold start)
+ * of the wrong length is ignored exactly like initial_guess. u_out receives the
+ * full displacement with prescribed values scattered in. */
+int topohip_solve(topohip_problem* p, const double* youngs, double tol, int max_it,
+                  const double* guess, double* u_out, topohip_cginfo* info);
+
+/* Compliance and sensitivity: simp_compliance's MultigridCG_Matfree route and
+ * sensitivity sweep (core/src/simp/simp.cpp:567-828). E_e = clamp(xphys_e,
+ * rho_min, 1)^penalty * E0; c = sum E_e u_e^T K1 u_e; dc_e = -p rho^(p-1) E0 q_e.
+ * With n_rhs > 1 load cases the solves run as one block PCG and c, dc are the
+ * sums over cases. u_out (nullable) receives load case 0's displacement. */
+int topohip_simp_compliance(topohip_problem* p, const double* xphys, double E0,
+                            double penalty, double rho_min, double tol, int max_it,
+                            double* c, double* dc, double* u_out, topohip_cginfo* info);
+
+/* Density filter with cone weights max(0, r - dist) over solid voxels
+ * (make_density_filter / filter_density / filter_sensitivity,
+ * core/src/simp/simp.cpp:834-944). adjoint == 0: out = H x / Hs;
+ * adjoint != 0: out = H^T (x / Hs). FP64. */
+int topohip_filter(topohip_problem* p, double radius, const double* x, double* out,
+                   int adjoint);
+
+/* Optimality-criteria update with lambda bisection on the filtered volume
+ * (oc_update, core/src/simp/simp.cpp:948-1032). active_set (nullable) receives
+ * per voxel 0 = interior, 1 = at the lower clamp, 2 = at the upper clamp.
+ * bisections / lambda (nullable) report the search. FP64 throughout. */
+int topohip_oc_update(topohip_problem* p, double radius, const double* x, const double* dc,
+                      double volfrac, double move, double rho_min, double* x_new,
+                      uint8_t* active_set, int* bisections, double* lambda);
+
+/* The plain OC design loop (simp_optimize, core/src/simp/simp.cpp:2018-2608):
+ * filter -> solve -> sensitivity -> OC, change = max|x_new - x|, stop when
+ * change < change_tol, then a final solve on filter(design). Optional
+ * penalty continuation ramp pc_p0 -> pc_p1 by pc_step, pc_per_stage iterations
+ * each (penalty_continuation_ramp, core/src/simp/simp.cpp:402-427); pc_per_stage
+ * 0 disables it. */
+typedef struct {
+  double volfrac, rmin, move, penalty, E0, rho_min, change_tol, cg_tol;
+  int max_iterations, cg_max_iterations;
+  double pc_p0, pc_p1, pc_step;
+  int pc_per_stage;
+} topohip_options;
+
+typedef struct {
+  int iterations, converged, non_convergent;
+  double compliance, initial_compliance, volume_fraction;
+  double t_total_ms, t_solve_ms, t_build_ms, t_filter_ms, t_sens_ms, t_oc_ms;
+  long long cg_iterations_total;
+} topohip_result;
+
+/* history (nullable) receives 4 doubles per iteration: compliance, change,
+ * volume fraction, outer PCG iterations. */
+int topohip_optimize(topohip_problem* p, const topohip_options* opt, double* design_out,
+                     double* xphys_out, double* history, topohip_result* result);
+
+/* ---- device-resident benchmarking entry (inputs already in HBM) -------------
+ * Runs `reps` back-to-back applies of the fine operator on the handle's own
+ * device buffers (random u, E = rho^3 set by topohip_bench_prepare) and
+ * returns the mean kernel time measured with HIP events on the handle's
+ * stream. precision: TOPOHIP_PREC_MIXED = FP32 apply, TOPOHIP_PREC_F64. */
+int topohip_bench_prepare(topohip_problem* p, uint64_t seed);
+int topohip_bench_ku(topohip_problem* p, int precision, int reps, double* ms_per_apply);
+
+/* ---- serving: persistent continuous-batching loop ---------------------------
+ * Jobs of uneven grid sizes are admitted as device memory (budget_bytes) and
+ * slots (max_slots) allow; each admitted job keeps its density, displacement
+ * and hierarchy resident and runs on its own stream, interleaved with the
+ * others. There is no reference counterpart beyond the worker's FIFO queue
+ * (tools/topopt-worker/topopt_worker.py:356-459). */
+typedef struct {
+  int nx, ny, nz;
+  int bc_kind;        /* 0 = cantilever, 1 = MBB (see DESIGN) */
+  double volfrac, rmin, penalty, move, cg_tol;
+  int iterations;
+  int precision;
+} topohip_job;
+
+typedef struct {
+  int state;          /* 0 queued, 1 running, 2 done, 3 failed */
+  int iterations;
+  double compliance, volume_fraction;
+  double t_submit_ms, t_start_ms, t_end_ms; /* server clock */
+  int status;
+} topohip_job_result;
+
+int topohip_server_create(size_t budget_bytes, int max_slots, topohip_server** out);
+int topohip_server_submit(topohip_server* s, const topohip_job* job, int64_t* job_id);
+int topohip_server_poll(topohip_server* s, int64_t job_id, topohip_job_result* out);
+/* Blocks until every submitted job has finished. */
+int topohip_server_drain(topohip_server* s);
+/* Copies a finished job's physical density (nx*ny*nz doubles). */
+int topohip_server_fetch_density(topohip_server* s, int64_t job_id, double* xphys);
+int topohip_server_destroy(topohip_server* s);
+/* Device bytes a job of this shape keeps resident (admission estimate). */
+size_t topohip_job_footprint(const topohip_job* job);
+
+/* ---- misc ------------------------------------------------------------------ */
+int topohip_set_device(int device);
+int topohip_last_error(char* buf, size_t n);
+const char* topohip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOPOPT_HIP_H */

@@ -1,0 +1,343 @@
+"""Pins of the FP64 oracle (CPU only).
+
+Each test checks the oracle against something that does not restate its own
+formula: the reference's golden fixtures (tests/golden/*.json, copied from
+core/tests/fixtures/), closed forms, invariants, or brute force (an explicitly
+assembled stiffness and a dense direct solve in numpy).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from topopt_amd import problems
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def setup_module():
+    oracle.set_threads(0)
+
+
+# ---- element (core/tests/unit/test_fea.cpp; hex_stiffness_reference.json) ---
+
+@pytest.mark.parametrize("case", ["E1_nu0.30", "E1_nu0.40"])
+def test_hex8_matches_reference_fixture(case):
+    d = json.load(open(os.path.join(GOLD, "hex_stiffness_reference.json")))
+    c = d["cases"][case]
+    K = oracle.hex8_stiffness(c["youngs_modulus"], c["poisson"], 1.0)
+    R = np.array(c["K"])
+    assert np.abs(K - R).max() <= 1e-10 * np.abs(R).max()
+
+
+def test_hex8_properties():
+    K = oracle.hex8_stiffness(1.0, 0.3, 1.0)
+    assert np.abs(K - K.T).max() <= 1e-12 * np.abs(K).max()
+    ev = np.linalg.eigvalsh(K)
+    assert np.sum(np.abs(ev) < 1e-10) == 6 and np.all(ev[6:] > 0)
+    for comp in range(3):
+        t = np.zeros(24)
+        t[comp::3] = 1.0
+        assert np.abs(K @ t).max() < 1e-12
+    # scaling K(E, h) = E h K(1, 1)
+    K2 = oracle.hex8_stiffness(2.5, 0.3, 0.5)
+    assert np.abs(K2 - 1.25 * K).max() < 1e-12
+
+
+# ---- matrix-free apply vs an explicitly assembled K ----------------------------
+
+REF_NODES = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+
+
+def assemble(nx, ny, nz, E, nu=0.3, solid=None):
+    Ke = oracle.hex8_stiffness(1.0, nu, 1.0)
+    nd = 3 * (nx + 1) * (ny + 1) * (nz + 1)
+    K = np.zeros((nd, nd))
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                e = (k * ny + j) * nx + i
+                if solid is not None and not solid[e]:
+                    continue
+                dofs = []
+                for a, b, c in REF_NODES:
+                    n = problems.node_index(nx, ny, i + a, j + b, k + c)
+                    dofs += [3 * n, 3 * n + 1, 3 * n + 2]
+                K[np.ix_(dofs, dofs)] += E[e] * Ke
+    return K
+
+
+def test_apply_matches_assembled():
+    nx, ny, nz = 4, 3, 5
+    rng = np.random.default_rng(0)
+    E = rng.uniform(1e-3, 1, nx * ny * nz) ** 3
+    u = rng.uniform(-1, 1, 3 * (nx + 1) * (ny + 1) * (nz + 1))
+    K = assemble(nx, ny, nz, E)
+    y = oracle.apply(oracle.Setup(nx, ny, nz), E, u)
+    ya = K @ u
+    assert np.abs(y - ya).max() <= 1e-9 * np.abs(ya).max()
+
+
+def test_apply_invariants():
+    nx, ny, nz = 5, 4, 3
+    rng = np.random.default_rng(1)
+    E = rng.uniform(0.1, 1, nx * ny * nz)
+    s = oracle.Setup(nx, ny, nz)
+    nd = s.ndof
+    for comp in range(3):  # rigid translation
+        t = np.zeros(nd)
+        t[comp::3] = 1.0
+        assert np.abs(oracle.apply(s, E, t)).max() < 1e-12
+    u, v = rng.uniform(-1, 1, nd), rng.uniform(-1, 1, nd)
+    Ku, Kv = oracle.apply(s, E, u), oracle.apply(s, E, v)
+    assert abs(u @ Kv - v @ Ku) < 1e-10 * abs(u @ Kv)
+    assert u @ Ku > 0
+    assert np.allclose(oracle.apply(s, 2 * E, u), 2 * Ku, rtol=0, atol=1e-12)
+
+
+def test_apply_skips_empty_voxels():
+    nx, ny, nz = 4, 4, 3
+    rng = np.random.default_rng(2)
+    solid = (rng.uniform(size=nx * ny * nz) > 0.4).astype(np.uint8)
+    E = rng.uniform(0.1, 1, nx * ny * nz)
+    u = rng.uniform(-1, 1, 3 * (nx + 1) * (ny + 1) * (nz + 1))
+    y = oracle.apply(oracle.Setup(nx, ny, nz, solid=solid), E, u)
+    ya = assemble(nx, ny, nz, E, solid=solid) @ u
+    assert np.abs(y - ya).max() <= 1e-9 * np.abs(ya).max()
+
+
+# ---- solvers --------------------------------------------------------------------
+
+def direct_solve(nx, ny, nz, E, bcs, loads):
+    K = assemble(nx, ny, nz, E)
+    nd = K.shape[0]
+    f = np.zeros(nd)
+    for n, c, v in loads:
+        f[3 * n + c] += v
+    fixed = np.zeros(nd, bool)
+    up = np.zeros(nd)
+    for n, c, v in bcs:
+        fixed[3 * n + c] = True
+        up[3 * n + c] = v
+    free = ~fixed
+    rhs = f - K @ up
+    u = up.copy()
+    u[free] = np.linalg.solve(K[np.ix_(free, free)], rhs[free])
+    return u
+
+
+@pytest.mark.parametrize("use_mg", [True, False])
+def test_solve_matches_direct(use_mg):
+    nx, ny, nz = 8, 4, 4
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    rng = np.random.default_rng(3)
+    E = rng.uniform(1e-3, 1, nx * ny * nz) ** 3
+    u_ref = direct_solve(nx, ny, nz, E, bcs, loads)
+    u, info = oracle.solve(oracle.Setup(nx, ny, nz, bcs=bcs, loads=loads), E, tol=1e-12,
+                           use_mg=use_mg)
+    assert info["converged"] and bool(info["used_multigrid"]) == use_mg
+    assert np.abs(u - u_ref).max() <= 1e-6 * np.abs(u_ref).max()
+
+
+def test_patch_test_uniform_strain():
+    """Prescribed linear displacement on every boundary node: the interior
+    solution reproduces the uniform strain field exactly (test_cg.cpp:138-151)."""
+    nx, ny, nz = 6, 4, 4
+    G = np.array([[1e-3, 2e-4, 0], [0, -5e-4, 1e-4], [3e-4, 0, 2e-3]])
+    bcs = []
+    exact = np.zeros(3 * (nx + 1) * (ny + 1) * (nz + 1))
+    for c in range(nz + 1):
+        for b in range(ny + 1):
+            for a in range(nx + 1):
+                n = problems.node_index(nx, ny, a, b, c)
+                val = G @ np.array([a, b, c], float)
+                exact[3 * n:3 * n + 3] = val
+                if a in (0, nx) or b in (0, ny) or c in (0, nz):
+                    bcs += [(n, q, float(val[q])) for q in range(3)]
+    E = np.ones(nx * ny * nz)
+    u, info = oracle.solve(oracle.Setup(nx, ny, nz, bcs=bcs, loads=[]), E, tol=1e-12)
+    assert np.abs(u - exact).max() < 1e-9
+
+
+def test_mg_iteration_counts_mesh_independent():
+    its = []
+    for n in (8, 16, 32):
+        bcs, loads = problems.cantilever(n, n, n)
+        s = oracle.Setup(n, n, n, bcs=bcs, loads=loads)
+        _, info = oracle.solve(s, np.ones(n ** 3), tol=1e-8)
+        assert info["used_multigrid"]
+        its.append(info["iterations"])
+    assert max(its) <= 30 and its[-1] <= its[0] + 10, its
+
+
+def test_hierarchy_levels_follow_reference_rule():
+    nx, ny, nz = 60, 20, 20
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    lv, dofs = oracle.hierarchy(oracle.Setup(nx, ny, nz, bcs=bcs, loads=loads), np.ones(nx * ny * nz))
+    assert lv == 3 and dofs == [79380, 10890, 1620]
+    # an odd fine axis does not coarsen: no hierarchy (Jacobi-CG fallback)
+    bcs, loads = problems.cantilever(7, 4, 4)
+    lv, _ = oracle.hierarchy(oracle.Setup(7, 4, 4, bcs=bcs, loads=loads), np.ones(7 * 16))
+    assert lv == 0
+
+
+def test_void_gate_errors():
+    nx, ny, nz = 4, 2, 2
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    solid = np.zeros(nx * ny * nz, np.uint8)
+    with pytest.raises(oracle.OracleError) as ei:
+        oracle.solve(oracle.Setup(nx, ny, nz, solid=solid, bcs=bcs, loads=loads), np.ones(16))
+    assert ei.value.code == 2
+    solid[:] = 1
+    solid[-1] = 0  # the loaded tip corner element is void, but its far node is loaded
+    tip = [(problems.node_index(nx, ny, nx, ny, nz), 2, -1.0)]
+    with pytest.raises(oracle.OracleError) as ei:
+        oracle.solve(oracle.Setup(nx, ny, nz, solid=solid, bcs=bcs, loads=tip), np.ones(16))
+    assert ei.value.code == 2
+    with pytest.raises(oracle.OracleError) as ei:
+        oracle.solve(oracle.Setup(nx, ny, nz, bcs=[(10 ** 6, 0, 0.0)], loads=loads), np.ones(16))
+    assert ei.value.code == 1
+
+
+# ---- SIMP ------------------------------------------------------------------------
+
+def test_compliance_equals_f_dot_u_and_fd_sensitivity():
+    nx, ny, nz = 6, 3, 3
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    s = oracle.Setup(nx, ny, nz, bcs=bcs, loads=loads)
+    rng = np.random.default_rng(4)
+    x = rng.uniform(0.2, 1.0, nx * ny * nz)
+    c, dc, u, _ = oracle.simp_compliance(s, x, tol=1e-13)
+    f = np.zeros(s.ndof)
+    for n, comp, v in loads:
+        f[3 * n + comp] += v
+    assert abs(c - f @ u) < 1e-9 * abs(c)
+    assert np.all(dc < 0)
+    for e in (0, 17, 40):
+        hstep = 1e-6
+        xp, xm = x.copy(), x.copy()
+        xp[e] += hstep
+        xm[e] -= hstep
+        cp = oracle.simp_compliance(s, xp, tol=1e-13)[0]
+        cm = oracle.simp_compliance(s, xm, tol=1e-13)[0]
+        fd = (cp - cm) / (2 * hstep)
+        assert abs(fd - dc[e]) / abs(dc[e]) < 1e-3
+
+
+def brute_filter_matrix(nx, ny, nz, r, solid=None):
+    N = nx * ny * nz
+    pos = np.array([(i, j, k) for k in range(nz) for j in range(ny) for i in range(nx)], float)
+    D = np.sqrt(((pos[:, None, :] - pos[None, :, :]) ** 2).sum(-1))
+    H = np.maximum(0.0, r - D)
+    if solid is not None:
+        H[:, solid == 0] = 0
+        H[solid == 0, :] = 0
+    return H
+
+
+@pytest.mark.parametrize("r", [1.2, 1.5, 2.0, 2.5])
+def test_filter_against_brute_force(r):
+    nx, ny, nz = 6, 5, 4
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0, 1, nx * ny * nz)
+    H = brute_filter_matrix(nx, ny, nz, r)
+    Hs = H.sum(1)
+    s = oracle.Setup(nx, ny, nz)
+    assert np.allclose(oracle.filter(s, r, x), (H @ x) / Hs, rtol=0, atol=1e-13)
+    assert np.allclose(oracle.filter(s, r, x, adjoint=True), H.T @ (x / Hs), rtol=0, atol=1e-13)
+    # partition of unity and adjoint identity <Mx, y> = <x, M^T y>
+    assert np.allclose(oracle.filter(s, r, np.ones_like(x)), 1.0, atol=1e-14)
+    y = rng.uniform(0, 1, x.size)
+    lhs = oracle.filter(s, r, x) @ y
+    rhs = x @ oracle.filter(s, r, y, adjoint=True)
+    assert abs(lhs - rhs) < 1e-9 * abs(lhs)
+
+
+def test_filter_stencil_sizes():
+    """Interior stencil point counts: r=1.2 -> 7, 1.5 -> 19, 2.0 -> 27 (cube minus
+    the 8 corners at distance sqrt(3) < 2? no: sqrt(3) < 2 so 27), 2.5 -> 81, 3.0 -> 93."""
+    nx = ny = nz = 9
+    s = oracle.Setup(nx, ny, nz)
+    centre = (4 * ny + 4) * nx + 4
+    for r, n in [(1.2, 7), (1.5, 19), (2.0, 27), (2.5, 81), (3.0, 93)]:
+        nnz, _ = oracle.filter_rows(s, r)
+        assert nnz[centre] == n
+
+
+def test_oc_update_bounds_volume_and_bisection():
+    nx, ny, nz = 10, 6, 5
+    rng = np.random.default_rng(6)
+    x = rng.uniform(0.15, 0.45, nx * ny * nz)  # reachable within one move limit
+    dc = -rng.uniform(0.01, 3, nx * ny * nz)
+    s = oracle.Setup(nx, ny, nz)
+    r, vf, move = 1.5, 0.3, 0.2
+    xn, nb, lam = oracle.oc_update(s, r, x, dc, vf, move)
+    assert np.all(xn >= np.maximum(1e-3, x - move) - 1e-15)
+    assert np.all(xn <= np.minimum(1.0, x + move) + 1e-15)
+    vol = oracle.filter(s, r, xn).sum()
+    assert abs(vol / x.size - vf) < 5e-3
+    # brute-force re-run of the bisection in numpy
+    H = brute_filter_matrix(nx, ny, nz, r)
+    Hs = H.sum(1)
+    dcf = H.T @ (dc / Hs)
+    dv = H.T @ (np.ones_like(x) / Hs)
+
+    def cand(l):
+        be = np.where(dv > 0, -dcf / (l * dv), 0.0)
+        be = np.maximum(be, 0)
+        return np.clip(x * np.sqrt(be), np.maximum(1e-3, x - move), np.minimum(1.0, x + move))
+
+    def pv(l):
+        return ((H @ cand(l)) / Hs).sum()
+
+    l1, l2 = 0.0, 1.0
+    while l2 < 1e30 and pv(l2) > vf * x.size:
+        l2 *= 2
+    it = 0
+    while it < 100 and (l2 - l1) > 1e-6 * (l1 + l2):
+        lm = 0.5 * (l1 + l2)
+        if pv(lm) > vf * x.size:
+            l1 = lm
+        else:
+            l2 = lm
+        it += 1
+    assert it == nb
+    assert abs(0.5 * (l1 + l2) - lam) <= 1e-12 * lam
+    assert np.abs(cand(0.5 * (l1 + l2)) - xn).max() < 1e-12
+
+
+def test_penalty_continuation_ramp():
+    # 1 -> 3 by 0.5, 6 iterations per stage (the large-cantilever workload's ramp)
+    vals = [oracle.penalty_at(1.0, 3.0, 0.5, 6, it) for it in range(1, 40)]
+    assert vals[0] == 1.0 and vals[5] == 1.0 and vals[6] == 1.5 and vals[29] == 3.0
+    assert vals[35] == 3.0
+    assert oracle.penalty_at(1.0, 3.0, 0.25, 1, 9) == 3.0
+
+
+def test_heaviside_spot_values():
+    d = json.load(open(os.path.join(GOLD, "benchmarks.json")))["projection_unit_reference"]
+    for bkey, pts in d.items():
+        if not bkey.startswith("beta_"):
+            continue
+        beta = float(bkey.split("_")[1])
+        for rkey, ref in pts.items():
+            rt = float(rkey.split("_")[1])
+            v, dv = oracle.heaviside(rt, beta, 0.5)
+            assert abs(v - ref["value"]) <= 1e-12 * max(1.0, abs(ref["value"]))
+            assert abs(dv - ref["derivative"]) <= 1e-12 * max(1.0, abs(ref["derivative"]))
+
+
+def test_simp_loop_structural_gates():
+    """A short MBB-style run: compliance decreases, the volume constraint holds,
+    physical density is the filtered design (test_simp.cpp:466-566, test_mbb.cpp)."""
+    nx, ny, nz = 24, 8, 2
+    bcs, loads = problems.mbb(nx, ny, nz)
+    s = oracle.Setup(nx, ny, nz, bcs=bcs, loads=loads)
+    out = oracle.optimize(s, 0.5, 1.5, 25, cg_tol=1e-9)
+    h = out["history"]
+    assert out["iterations"] == 25
+    assert h[-1, 0] < 0.6 * h[0, 0]
+    assert abs(out["volume_fraction"] - 0.5) < 5e-3
+    assert np.allclose(out["xphys"], oracle.filter(s, 1.5, out["design"]), atol=1e-14)

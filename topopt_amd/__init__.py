@@ -1,0 +1,316 @@
+"""topopt_amd — MI355X (gfx950) engine for the voxel-SIMP inner loop.
+
+Python mirror of the reference core's solver interface for this path, over the
+C ABI of libtopopt_hip.so (include/topopt_hip.h). Names, argument meaning and
+error behaviour follow the reference:
+
+  fea_matfree_apply        core/include/topopt/fea.hpp:763-766
+  fea_solve_mgcg_matfree   core/include/topopt/fea.hpp:707-715
+  simp_compliance          core/include/topopt/simp.hpp:247-255
+  DensityFilter            core/include/topopt/simp.hpp:387-406
+  oc_update                core/include/topopt/simp.hpp:481-485
+  simp_optimize            core/include/topopt/simp.hpp:1783
+
+Exceptions map the C status codes onto the reference taxonomy:
+ValueError (std::invalid_argument), UnderConstrained (std::runtime_error from
+the void gate), SolverNonConvergence (with .iterations / .residual),
+DeviceError (a HIP failure). The library must be built (topopt_amd.build);
+nothing here falls back to a CPU path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtopopt_hip.so")
+
+PREC_MIXED = 0
+PREC_F64 = 1
+
+OK, EINVAL, EUNDERCONSTRAINED, ENONCONVERGENCE, EDEADLINE, EDEVICE, EINTERNAL = range(7)
+
+
+class UnderConstrained(RuntimeError):
+    pass
+
+
+class SolverNonConvergence(RuntimeError):
+    def __init__(self, msg, iterations=0, residual=0.0):
+        super().__init__(msg)
+        self.iterations = iterations
+        self.residual = residual
+
+
+class DeviceError(RuntimeError):
+    pass
+
+
+class CgInfo(C.Structure):
+    _fields_ = [
+        ("converged", C.c_int), ("iterations", C.c_int), ("used_multigrid", C.c_int),
+        ("mg_levels", C.c_int), ("hier_built", C.c_int), ("mg_cycles_attempted", C.c_int),
+        ("residual", C.c_double), ("matvecs", C.c_longlong),
+        ("t_build_ms", C.c_double), ("t_mg_build_ms", C.c_double), ("t_mg_ms", C.c_double),
+        ("t_cg_ms", C.c_double), ("t_total_ms", C.c_double),
+        ("residual_replacements", C.c_int), ("coarse_dofs", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("volfrac", C.c_double), ("rmin", C.c_double), ("move", C.c_double),
+        ("penalty", C.c_double), ("E0", C.c_double), ("rho_min", C.c_double),
+        ("change_tol", C.c_double), ("cg_tol", C.c_double),
+        ("max_iterations", C.c_int), ("cg_max_iterations", C.c_int),
+        ("pc_p0", C.c_double), ("pc_p1", C.c_double), ("pc_step", C.c_double),
+        ("pc_per_stage", C.c_int),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int), ("converged", C.c_int), ("non_convergent", C.c_int),
+        ("compliance", C.c_double), ("initial_compliance", C.c_double),
+        ("volume_fraction", C.c_double),
+        ("t_total_ms", C.c_double), ("t_solve_ms", C.c_double), ("t_build_ms", C.c_double),
+        ("t_filter_ms", C.c_double), ("t_sens_ms", C.c_double), ("t_oc_ms", C.c_double),
+        ("cg_iterations_total", C.c_longlong),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Job(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("bc_kind", C.c_int),
+        ("volfrac", C.c_double), ("rmin", C.c_double), ("penalty", C.c_double),
+        ("move", C.c_double), ("cg_tol", C.c_double), ("iterations", C.c_int),
+        ("precision", C.c_int),
+    ]
+
+
+class JobResult(C.Structure):
+    _fields_ = [
+        ("state", C.c_int), ("iterations", C.c_int), ("compliance", C.c_double),
+        ("volume_fraction", C.c_double), ("t_submit_ms", C.c_double),
+        ("t_start_ms", C.c_double), ("t_end_ms", C.c_double), ("status", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load the in-tree HIP library; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DeviceError(f"{LIB_PATH} is missing: build it with `python -m topopt_amd.build`")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.topohip_version.restype = C.c_char_p
+        _lib.topohip_job_footprint.restype = C.c_size_t
+        _lib.topohip_job_footprint.argtypes = [C.POINTER(Job)]
+        _lib.topohip_server_create.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]
+    return _lib
+
+
+def _err():
+    buf = C.create_string_buffer(1024)
+    lib().topohip_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def _check(code, info=None):
+    if code == OK:
+        return
+    msg = _err()
+    if code == EINVAL:
+        raise ValueError(msg)
+    if code == EUNDERCONSTRAINED:
+        raise UnderConstrained(msg)
+    if code == ENONCONVERGENCE:
+        it = info.iterations if info is not None else 0
+        res = info.residual if info is not None else 0.0
+        raise SolverNonConvergence(msg, it, res)
+    if code == EDEVICE:
+        raise DeviceError(msg)
+    raise RuntimeError(f"topohip status {code}: {msg}")
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+class Problem:
+    """Device-resident problem: grid, material, Dirichlet BCs and load cases.
+
+    bcs / load cases are sequences of (node, component, value) triples, the
+    reference's DirichletBC / NodalLoad.
+    """
+
+    def __init__(self, nx, ny, nz, bcs=(), loads=None, h=1.0, poisson=0.3, solid=None,
+                 precision=PREC_MIXED):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.nvox = self.nx * self.ny * self.nz
+        self.ndof = 3 * (self.nx + 1) * (self.ny + 1) * (self.nz + 1)
+        self.precision = precision
+        bcs = list(bcs)
+        bn = np.array([b[0] for b in bcs], dtype=np.int32)
+        bc = np.array([b[1] for b in bcs], dtype=np.int32)
+        bv = np.array([b[2] for b in bcs], dtype=np.float64)
+        sol = None if solid is None else np.ascontiguousarray(solid, dtype=np.uint8).ravel()
+        if sol is not None and sol.size != self.nvox:
+            raise ValueError("solid tag size != voxel_count")
+        self._h = C.c_void_p()
+        _check(lib().topohip_problem_create(
+            self.nx, self.ny, self.nz, C.c_double(h), C.c_double(poisson), _p(sol), len(bcs),
+            _p(bn), _p(bc), _p(bv), int(precision), C.byref(self._h)))
+        self.nrhs = 0
+        if loads is not None:
+            self.set_loads(loads)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().topohip_problem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_loads(self, cases):
+        """cases: one list of (node, comp, value) triples, or a list of them."""
+        cases = list(cases)
+        if cases and isinstance(cases[0], tuple):
+            cases = [cases]
+        counts = np.array([len(c) for c in cases], dtype=np.int32)
+        flat = [t for c in cases for t in c]
+        n = np.array([t[0] for t in flat], dtype=np.int32)
+        cc = np.array([t[1] for t in flat], dtype=np.int32)
+        v = np.array([t[2] for t in flat], dtype=np.float64)
+        _check(lib().topohip_set_loads(self._h, len(cases), _p(counts), _p(n), _p(cc), _p(v)))
+        self.nrhs = len(cases)
+
+    # -- operators ----------------------------------------------------------
+    def apply(self, youngs, u, precision="f32"):
+        E = _f64(youngs, self.nvox)
+        u = _f64(u, self.ndof)
+        y = np.zeros(self.ndof)
+        fn = lib().topohip_ku_apply_f32 if precision == "f32" else lib().topohip_ku_apply_f64
+        _check(fn(self._h, _p(E), _p(u), _p(y)))
+        return y
+
+    def solve(self, youngs, tol=1e-8, max_iterations=0, initial_guess=None):
+        E = _f64(youngs, self.nvox)
+        g = None if initial_guess is None else _f64(initial_guess)
+        if g is not None and g.size != self.ndof:
+            g = None  # wrong length: ignored, as the reference's initial_guess
+        u = np.zeros(self.ndof)
+        info = CgInfo()
+        code = lib().topohip_solve(self._h, _p(E), C.c_double(tol), int(max_iterations), _p(g),
+                                   _p(u), C.byref(info))
+        _check(code, info)
+        return u, info.as_dict()
+
+    def compliance(self, xphys, E0=1.0, penalty=3.0, rho_min=1e-3, tol=1e-8, max_iterations=0,
+                   want_u=False):
+        x = _f64(xphys, self.nvox)
+        c = C.c_double(0)
+        dc = np.zeros(self.nvox)
+        u = np.zeros(self.ndof) if want_u else None
+        info = CgInfo()
+        code = lib().topohip_simp_compliance(self._h, _p(x), C.c_double(E0), C.c_double(penalty),
+                                             C.c_double(rho_min), C.c_double(tol),
+                                             int(max_iterations), C.byref(c), _p(dc), _p(u),
+                                             C.byref(info))
+        _check(code, info)
+        return c.value, dc, u, info.as_dict()
+
+    def filter(self, radius, x, adjoint=False):
+        x = _f64(x, self.nvox)
+        out = np.zeros(self.nvox)
+        _check(lib().topohip_filter(self._h, C.c_double(radius), _p(x), _p(out), int(adjoint)))
+        return out
+
+    def oc_update(self, radius, x, dc, volfrac, move=0.2, rho_min=1e-3):
+        x = _f64(x, self.nvox)
+        dc = _f64(dc, self.nvox)
+        out = np.zeros(self.nvox)
+        act = np.zeros(self.nvox, dtype=np.uint8)
+        nb = C.c_int(0)
+        lam = C.c_double(0)
+        _check(lib().topohip_oc_update(self._h, C.c_double(radius), _p(x), _p(dc),
+                                       C.c_double(volfrac), C.c_double(move), C.c_double(rho_min),
+                                       _p(out), _p(act), C.byref(nb), C.byref(lam)))
+        return out, act, nb.value, lam.value
+
+    def optimize(self, volfrac, rmin, iterations, penalty=3.0, move=0.2, E0=1.0, rho_min=1e-3,
+                 change_tol=0.0, cg_tol=1e-8, cg_max_iterations=0, continuation=None):
+        o = Options(volfrac, rmin, move, penalty, E0, rho_min, change_tol, cg_tol, int(iterations),
+                    int(cg_max_iterations), 0.0, 0.0, 0.0, 0)
+        if continuation:
+            o.pc_p0, o.pc_p1, o.pc_step, o.pc_per_stage = continuation
+        design = np.zeros(self.nvox)
+        xphys = np.zeros(self.nvox)
+        hist = np.zeros(4 * iterations)
+        res = Result()
+        _check(lib().topohip_optimize(self._h, C.byref(o), _p(design), _p(xphys), _p(hist),
+                                      C.byref(res)))
+        n = res.iterations
+        out = res.as_dict()
+        out.update(design=design, xphys=xphys, history=hist[: 4 * n].reshape(n, 4))
+        return out
+
+    # -- device-resident benchmarking ----------------------------------------
+    def bench_prepare(self, seed=0x5EED):
+        _check(lib().topohip_bench_prepare(self._h, C.c_uint64(seed)))
+
+    def bench_ku(self, precision=PREC_MIXED, reps=50):
+        ms = C.c_double(0)
+        _check(lib().topohip_bench_ku(self._h, int(precision), int(reps), C.byref(ms)))
+        return ms.value
+
+
+# -- reference-named free functions ------------------------------------------
+
+def fea_matfree_apply(nx, ny, nz, youngs_per_voxel, u, poisson=0.3, h=1.0, solid=None,
+                      precision="f32"):
+    """y = K(E) u over all solid voxels, no BCs (fea_matfree_apply, graded)."""
+    with Problem(nx, ny, nz, h=h, poisson=poisson, solid=solid) as p:
+        return p.apply(youngs_per_voxel, u, precision)
+
+
+def fea_solve_mgcg_matfree(nx, ny, nz, youngs_per_voxel, poisson, bcs, loads, tolerance=1e-8,
+                           max_iterations=0, initial_guess=None, h=1.0, solid=None,
+                           precision=PREC_MIXED):
+    """Multigrid-CG solve; returns (u, CgInfo dict)."""
+    with Problem(nx, ny, nz, bcs, loads, h=h, poisson=poisson, solid=solid,
+                 precision=precision) as p:
+        return p.solve(youngs_per_voxel, tolerance, max_iterations, initial_guess)
+
+
+def version():
+    return lib().topohip_version().decode()

@@ -1,0 +1,310 @@
+// capi.cpp — extern "C" boundary (include/topopt_hip.h). Converts exceptions
+// to status codes and keeps the message per calling thread.
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+
+#include "engine.hpp"
+
+using namespace topohip;
+
+struct topohip_problem {
+  std::unique_ptr<Problem> P;
+  // bench buffers (device-resident inputs)
+  DBuf<double> bu64, by64;
+  DBuf<float> bu32, by32;
+  bool bench_ready = false;
+};
+
+namespace {
+thread_local std::string g_err;
+int g_device = 0;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return TOPOHIP_OK;
+  } catch (const Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const HipError& e) {
+    g_err = e.what();
+    return TOPOHIP_EDEVICE;
+  } catch (const std::bad_alloc& e) {
+    g_err = "out of memory";
+    return TOPOHIP_EDEVICE;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return TOPOHIP_EINTERNAL;
+  }
+}
+
+void need(const void* p, const char* what) {
+  if (!p) throw Error(TOPOHIP_EINVAL, std::string(what) + " must not be NULL");
+}
+
+// Upload host youngs (validated > 0 on solid voxels, as mf_build_elems,
+// core/src/fea/matfree.cpp:442-448) into P.E64.
+void upload_youngs(Problem& P, const double* youngs) {
+  const long long nv = P.g.elems();
+  for (long long e = 0; e < nv; ++e) {
+    if (!P.all_solid && !P.solid_h[e]) continue;
+    if (!(youngs[e] > 0.0))
+      throw Error(TOPOHIP_EINVAL,
+                  "fea_solve_mgcg_matfree: per-voxel Young's modulus must be > 0 on a solid voxel");
+  }
+  std::vector<double> Eh(youngs, youngs + nv);
+  if (!P.all_solid)
+    for (long long e = 0; e < nv; ++e)
+      if (!P.solid_h[e]) Eh[e] = 0.0;
+  TOPOHIP_CHECK(hipMemcpyAsync(P.E64.p, Eh.data(), nv * sizeof(double), hipMemcpyHostToDevice, P.s));
+  TOPOHIP_CHECK(hipStreamSynchronize(P.s));
+}
+
+template <typename T>
+void apply_host(Problem& P, const double* youngs, const double* u, double* y) {
+  upload_youngs(P, youngs);
+  const long long nd = P.g.ndof(), nv = P.g.elems();
+  DBuf<double> u64, y64;
+  u64.ensure(nd);
+  y64.ensure(nd);
+  TOPOHIP_CHECK(hipMemcpyAsync(u64.p, u, nd * sizeof(double), hipMemcpyHostToDevice, P.s));
+  KuOp<T> op;
+  op.g = P.g;
+  op.tz = P.tz;
+  op.epi = EPI_AX;
+  if constexpr (sizeof(T) == 4) {
+    DBuf<float> u32, y32;
+    u32.ensure(nd);
+    y32.ensure(nd);
+    launch_convert<double, float>(P.E64.p, P.E32.p, nv, P.s);
+    launch_convert<double, float>(u64.p, u32.p, nd, P.s);
+    op.E = P.E32.p;
+    op.mc = P.mc32;
+    op.x = u32.p;
+    op.y = y32.p;
+    launch_ku_fine<float>(op, P.s);
+    launch_convert<float, double>(y32.p, y64.p, nd, P.s);
+    TOPOHIP_CHECK(hipStreamSynchronize(P.s));
+  } else {
+    op.E = P.E64.p;
+    op.mc = P.mc64;
+    op.x = u64.p;
+    op.y = y64.p;
+    launch_ku_fine<double>(op, P.s);
+  }
+  TOPOHIP_CHECK(hipMemcpyAsync(y, y64.p, nd * sizeof(double), hipMemcpyDeviceToHost, P.s));
+  TOPOHIP_CHECK(hipStreamSynchronize(P.s));
+}
+}  // namespace
+
+extern "C" {
+
+const char* topohip_version(void) { return "topopt_hip 0.1 (gfx950)"; }
+
+int topohip_last_error(char* buf, size_t n) {
+  if (buf && n > 0) {
+    std::strncpy(buf, g_err.c_str(), n - 1);
+    buf[n - 1] = '\0';
+  }
+  return static_cast<int>(g_err.size());
+}
+
+int topohip_set_device(int device) {
+  return guarded([&] {
+    TOPOHIP_CHECK(hipSetDevice(device));
+    g_device = device;
+  });
+}
+
+int topohip_problem_create(int nx, int ny, int nz, double h, double poisson, const uint8_t* solid,
+                           int n_bc, const int32_t* bc_node, const int32_t* bc_comp,
+                           const double* bc_val, int precision, topohip_problem** out) {
+  return guarded([&] {
+    need(out, "out");
+    if (n_bc > 0) { need(bc_node, "bc_node"); need(bc_comp, "bc_comp"); need(bc_val, "bc_val"); }
+    TOPOHIP_CHECK(hipSetDevice(g_device));
+    auto h_ = std::make_unique<topohip_problem>();
+    h_->P = create_problem(nx, ny, nz, h, poisson, solid, n_bc, bc_node, bc_comp, bc_val, precision);
+    *out = h_.release();
+  });
+}
+
+int topohip_problem_destroy(topohip_problem* p) {
+  return guarded([&] { delete p; });
+}
+
+int topohip_set_loads(topohip_problem* p, int n_rhs, const int32_t* counts, const int32_t* node,
+                      const int32_t* comp, const double* val) {
+  return guarded([&] {
+    need(p, "problem");
+    need(counts, "counts");
+    set_loads(*p->P, n_rhs, counts, node, comp, val);
+  });
+}
+
+int topohip_ku_apply_f32(topohip_problem* p, const double* youngs, const double* u, double* y) {
+  return guarded([&] {
+    need(p, "problem"); need(youngs, "youngs"); need(u, "u"); need(y, "y");
+    apply_host<float>(*p->P, youngs, u, y);
+  });
+}
+
+int topohip_ku_apply_f64(topohip_problem* p, const double* youngs, const double* u, double* y) {
+  return guarded([&] {
+    need(p, "problem"); need(youngs, "youngs"); need(u, "u"); need(y, "y");
+    apply_host<double>(*p->P, youngs, u, y);
+  });
+}
+
+int topohip_solve(topohip_problem* p, const double* youngs, double tol, int max_it,
+                  const double* guess, double* u_out, topohip_cginfo* info) {
+  return guarded([&] {
+    need(p, "problem"); need(youngs, "youngs"); need(u_out, "u_out");
+    Problem& P = *p->P;
+    upload_youngs(P, youngs);
+    Timers t;
+    build_hierarchy(P, t);
+    DBuf<double> dg;
+    const long long nd = P.g.ndof();
+    if (guess) {
+      dg.ensure(nd);
+      TOPOHIP_CHECK(hipMemcpyAsync(dg.p, guess, nd * sizeof(double), hipMemcpyHostToDevice, P.s));
+    }
+    topohip_cginfo ci{};
+    try {
+      solve_rhs(P, 0, tol, max_it, guess ? dg.p : nullptr, &ci);
+    } catch (...) {
+      ci.t_mg_build_ms = t.mg_build;
+      if (info) *info = ci;
+      throw;
+    }
+    ci.t_mg_build_ms = t.mg_build;
+    if (info) *info = ci;
+    TOPOHIP_CHECK(hipMemcpyAsync(u_out, P.u.p, nd * sizeof(double), hipMemcpyDeviceToHost, P.s));
+    TOPOHIP_CHECK(hipStreamSynchronize(P.s));
+  });
+}
+
+int topohip_simp_compliance(topohip_problem* p, const double* xphys, double E0, double penalty,
+                            double rho_min, double tol, int max_it, double* c, double* dc,
+                            double* u_out, topohip_cginfo* info) {
+  return guarded([&] {
+    need(p, "problem"); need(xphys, "xphys"); need(c, "c");
+    Problem& P = *p->P;
+    const long long nv = P.g.elems(), nd = P.g.ndof();
+    P.xphys.ensure(nv);
+    TOPOHIP_CHECK(hipMemcpyAsync(P.xphys.p, xphys, nv * sizeof(double), hipMemcpyHostToDevice, P.s));
+    *c = simp_compliance_dev(P, P.xphys.p, E0, penalty, rho_min, tol, max_it, info);
+    if (dc)
+      TOPOHIP_CHECK(hipMemcpyAsync(dc, P.dc.p, nv * sizeof(double), hipMemcpyDeviceToHost, P.s));
+    if (u_out && P.nrhs == 1)
+      TOPOHIP_CHECK(hipMemcpyAsync(u_out, P.u.p, nd * sizeof(double), hipMemcpyDeviceToHost, P.s));
+    TOPOHIP_CHECK(hipStreamSynchronize(P.s));
+  });
+}
+
+int topohip_filter(topohip_problem* p, double radius, const double* x, double* out, int adjoint) {
+  return guarded([&] {
+    need(p, "problem"); need(x, "x"); need(out, "out");
+    Problem& P = *p->P;
+    FilterCache& fc = filter_for(P, radius);
+    const long long nv = P.g.elems();
+    DBuf<double> xi, xo;
+    xi.ensure(nv);
+    xo.ensure(nv);
+    TOPOHIP_CHECK(hipMemcpyAsync(xi.p, x, nv * sizeof(double), hipMemcpyHostToDevice, P.s));
+    launch_filter(P.g, fc.st, P.all_solid ? nullptr : P.solid.p, fc.hs.p, xi.p, xo.p, adjoint ? 1 : 0,
+                  P.s);
+    TOPOHIP_CHECK(hipMemcpyAsync(out, xo.p, nv * sizeof(double), hipMemcpyDeviceToHost, P.s));
+    TOPOHIP_CHECK(hipStreamSynchronize(P.s));
+  });
+}
+
+int topohip_oc_update(topohip_problem* p, double radius, const double* x, const double* dc,
+                      double volfrac, double move, double rho_min, double* x_new,
+                      uint8_t* active_set, int* bisections, double* lambda) {
+  return guarded([&] {
+    need(p, "problem"); need(x, "x"); need(dc, "dc"); need(x_new, "x_new");
+    Problem& P = *p->P;
+    const long long nv = P.g.elems();
+    DBuf<double> xi, di, xo;
+    DBuf<uint8_t> act;
+    xi.ensure(nv); di.ensure(nv); xo.ensure(nv); act.ensure(nv);
+    TOPOHIP_CHECK(hipMemcpyAsync(xi.p, x, nv * sizeof(double), hipMemcpyHostToDevice, P.s));
+    TOPOHIP_CHECK(hipMemcpyAsync(di.p, dc, nv * sizeof(double), hipMemcpyHostToDevice, P.s));
+    oc_update_dev(P, radius, xi.p, di.p, volfrac, move, rho_min, xo.p, act.p, bisections, lambda,
+                  nullptr);
+    TOPOHIP_CHECK(hipMemcpyAsync(x_new, xo.p, nv * sizeof(double), hipMemcpyDeviceToHost, P.s));
+    if (active_set)
+      TOPOHIP_CHECK(hipMemcpyAsync(active_set, act.p, nv, hipMemcpyDeviceToHost, P.s));
+    TOPOHIP_CHECK(hipStreamSynchronize(P.s));
+  });
+}
+
+int topohip_optimize(topohip_problem* p, const topohip_options* opt, double* design_out,
+                     double* xphys_out, double* history, topohip_result* result) {
+  return guarded([&] {
+    need(p, "problem"); need(opt, "options");
+    optimize(*p->P, *opt, design_out, xphys_out, history, result);
+  });
+}
+
+int topohip_bench_prepare(topohip_problem* p, uint64_t seed) {
+  return guarded([&] {
+    need(p, "problem");
+    Problem& P = *p->P;
+    const long long nd = P.g.ndof(), nv = P.g.elems();
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<double> uu(-1.0, 1.0), ur(1e-3, 1.0);
+    std::vector<double> u(nd), E(nv);
+    for (auto& v : u) v = uu(rng);
+    for (auto& v : E) { const double r = ur(rng); v = r * r * r; }
+    p->bu64.ensure(nd); p->by64.ensure(nd); p->bu32.ensure(nd); p->by32.ensure(nd);
+    TOPOHIP_CHECK(hipMemcpyAsync(p->bu64.p, u.data(), nd * sizeof(double), hipMemcpyHostToDevice, P.s));
+    TOPOHIP_CHECK(hipMemcpyAsync(P.E64.p, E.data(), nv * sizeof(double), hipMemcpyHostToDevice, P.s));
+    launch_convert<double, float>(p->bu64.p, p->bu32.p, nd, P.s);
+    launch_convert<double, float>(P.E64.p, P.E32.p, nv, P.s);
+    TOPOHIP_CHECK(hipStreamSynchronize(P.s));
+    p->bench_ready = true;
+  });
+}
+
+int topohip_bench_ku(topohip_problem* p, int precision, int reps, double* ms_per_apply) {
+  return guarded([&] {
+    need(p, "problem"); need(ms_per_apply, "ms_per_apply");
+    if (!p->bench_ready) throw Error(TOPOHIP_EINVAL, "call topohip_bench_prepare first");
+    if (reps < 1) throw Error(TOPOHIP_EINVAL, "reps must be >= 1");
+    Problem& P = *p->P;
+    hipEvent_t e0, e1;
+    TOPOHIP_CHECK(hipEventCreate(&e0));
+    TOPOHIP_CHECK(hipEventCreate(&e1));
+    auto run = [&]() {
+      if (precision == TOPOHIP_PREC_F64) {
+        KuOp<double> op;
+        op.g = P.g; op.tz = P.tz; op.epi = EPI_AX; op.E = P.E64.p; op.mc = P.mc64;
+        op.x = p->bu64.p; op.y = p->by64.p;
+        launch_ku_fine<double>(op, P.s);
+      } else {
+        KuOp<float> op;
+        op.g = P.g; op.tz = P.tz; op.epi = EPI_AX; op.E = P.E32.p; op.mc = P.mc32;
+        op.x = p->bu32.p; op.y = p->by32.p;
+        launch_ku_fine<float>(op, P.s);
+      }
+    };
+    run();
+    TOPOHIP_CHECK(hipEventRecord(e0, P.s));
+    for (int i = 0; i < reps; ++i) run();
+    TOPOHIP_CHECK(hipEventRecord(e1, P.s));
+    TOPOHIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    TOPOHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_apply = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  });
+}
+
+}  // extern "C"

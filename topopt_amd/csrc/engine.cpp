@@ -1,0 +1,1026 @@
+// engine.cpp — problem handle, multigrid hierarchy and the device-resident
+// PCG / OC drivers of the voxel-SIMP engine.
+//
+// The solve restates solve_mgcg_matfree (core/src/fea/multigrid.cpp:2098-2319)
+// with mf_mgpcg (:1211-1318) and the V-cycle (:1143-1160, :798-819); the
+// reduced system and void gate restate mf_build_reduced
+// (core/src/fea/matfree.cpp:670-834). Vectors stay full-grid (3 per node) with
+// inactive DOFs held at zero, which is the reduced system embedded in the grid.
+#include "engine.hpp"
+
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <random>
+
+namespace topohip {
+
+namespace {
+
+constexpr double kOmega = 0.6;       // damped Jacobi weight (multigrid.cpp:72)
+constexpr int kMinCoarseElems = 2;   // coarsen.hpp:76
+constexpr int kCoarseDofCap = 6000;  // coarsen.hpp:80
+constexpr int kMgIterBudget = 300;   // multigrid.cpp:108
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+void sync(Problem& P) { TOPOHIP_CHECK(hipStreamSynchronize(P.s)); }
+
+template <typename T>
+void h2d(T* d, const T* h, size_t n, hipStream_t s) {
+  TOPOHIP_CHECK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <typename T>
+void d2h(T* h, const T* d, size_t n, hipStream_t s) {
+  TOPOHIP_CHECK(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, s));
+  TOPOHIP_CHECK(hipStreamSynchronize(s));
+}
+
+// 1-D trilinear weight of coarse node bit b for fine position p in {0,1,2}.
+double w1h(int p, int b) {
+  if (p == 1) return 0.5;
+  return (p == 0) == (b == 0) ? 1.0 : 0.0;
+}
+
+int popcount3(uint8_t v) { return (v & 1) + ((v >> 1) & 1) + ((v >> 2) & 1); }
+
+}  // namespace
+
+// Isotropic hex8 stiffness by 2x2x2 Gauss quadrature of B^T D B on a cube of
+// edge h (the element of core/src/fea/hex_element.cpp:23-114), with D written
+// as lambda * m m^T + mu * diag(2,2,2,1,1,1) in Voigt order
+// [xx, yy, zz, gxy, gyz, gzx]. Reference DOF order (bottom CCW, top CCW).
+void hex8_stiffness_host(double E, double nu, double h, double* K) {
+  if (!(E > 0.0)) throw Error(TOPOHIP_EINVAL, "hex8_stiffness: youngs_modulus must be > 0");
+  if (!(h > 0.0)) throw Error(TOPOHIP_EINVAL, "hex8_stiffness: element_size must be > 0");
+  if (!(nu > -1.0 && nu < 0.5))
+    throw Error(TOPOHIP_EINVAL, "hex8_stiffness: poisson must be in (-1, 0.5)");
+  const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  const double mu = E / (2.0 * (1.0 + nu));
+  double D[6][6] = {};
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) D[i][j] = lam;
+    D[i][i] += 2.0 * mu;
+    D[i + 3][i + 3] = mu;
+  }
+  static const int cx[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+  static const int cy[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+  static const int cz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+  std::memset(K, 0, 576 * sizeof(double));
+  const double gq = 1.0 / std::sqrt(3.0);
+  const double jac = 0.125 * h * h * h;  // det of the (h/2) I map
+  for (int q = 0; q < 8; ++q) {
+    const double xi = (q & 1) ? gq : -gq, et = (q & 2) ? gq : -gq, ze = (q & 4) ? gq : -gq;
+    double G[8][3];
+    for (int a = 0; a < 8; ++a) {
+      const double sx = cx[a] ? 1.0 : -1.0, sy = cy[a] ? 1.0 : -1.0, sz = cz[a] ? 1.0 : -1.0;
+      G[a][0] = (2.0 / h) * 0.125 * sx * (1.0 + sy * et) * (1.0 + sz * ze);
+      G[a][1] = (2.0 / h) * 0.125 * sy * (1.0 + sx * xi) * (1.0 + sz * ze);
+      G[a][2] = (2.0 / h) * 0.125 * sz * (1.0 + sx * xi) * (1.0 + sy * et);
+    }
+    double B[6][24] = {};
+    for (int a = 0; a < 8; ++a) {
+      B[0][3 * a] = G[a][0];
+      B[1][3 * a + 1] = G[a][1];
+      B[2][3 * a + 2] = G[a][2];
+      B[3][3 * a] = G[a][1]; B[3][3 * a + 1] = G[a][0];
+      B[4][3 * a + 1] = G[a][2]; B[4][3 * a + 2] = G[a][1];
+      B[5][3 * a] = G[a][2]; B[5][3 * a + 2] = G[a][0];
+    }
+    double DB[6][24];
+    for (int i = 0; i < 6; ++i)
+      for (int c = 0; c < 24; ++c) {
+        double s = 0.0;
+        for (int k = 0; k < 6; ++k) s += D[i][k] * B[k][c];
+        DB[i][c] = s;
+      }
+    for (int r = 0; r < 24; ++r)
+      for (int c = 0; c < 24; ++c) {
+        double s = 0.0;
+        for (int k = 0; k < 6; ++k) s += B[k][r] * DB[k][c];
+        K[r * 24 + c] += jac * s;
+      }
+  }
+}
+
+Problem::~Problem() {
+  if (rb) rocblas_destroy_handle(rb);
+  if (st_h) (void)hipHostFree(st_h);
+  if (oc_h) (void)hipHostFree(oc_h);
+  if (scal_h) (void)hipHostFree(scal_h);
+  filters.clear();
+  L.clear();
+  if (s) (void)hipStreamDestroy(s);
+}
+
+namespace {
+
+// Element constants: bit-ordered Ke, its modal coefficients (checked against
+// the 45-entry pattern), the diagonal and the 8 child Galerkin blocks
+// S_c = W_c^T Ke W_c.
+void element_constants(Problem& P) {
+  hex8_stiffness_host(1.0, P.nu, P.h, P.Ke);
+  for (int mr = 0; mr < 8; ++mr)
+    for (int cr = 0; cr < 3; ++cr)
+      for (int ms = 0; ms < 8; ++ms)
+        for (int cs = 0; cs < 3; ++cs)
+          P.Kb[(3 * mr + cr) * 24 + 3 * ms + cs] =
+              P.Ke[(3 * kRefOfBit[mr] + cr) * 24 + 3 * kRefOfBit[ms] + cs];
+  for (int i = 0; i < 24; ++i) P.kdiag[i] = P.Kb[i * 24 + i];
+  // M = H Kb H (H = parity Hadamard per component, symmetric).
+  auto had = [](int q, int m) { return (__builtin_popcount(q & m) & 1) ? -1.0 : 1.0; };
+  double M[576];
+  double mx = 0.0;
+  for (int r = 0; r < 24; ++r)
+    for (int c = 0; c < 24; ++c) {
+      double s = 0.0;
+      for (int a = 0; a < 8; ++a)
+        for (int b = 0; b < 8; ++b)
+          s += had(r / 3, a) * P.Kb[(3 * a + r % 3) * 24 + 3 * b + c % 3] * had(c / 3, b);
+      M[r * 24 + c] = s;
+      mx = std::max(mx, std::fabs(s));
+    }
+  bool in_pat[576] = {};
+  int k = 0;
+#define TOPOHIP_X(kk, rr, cc)               \
+  in_pat[(rr) * 24 + (cc)] = true;          \
+  P.mc64.c[kk] = M[(rr) * 24 + (cc)] / 64.0; \
+  P.mc32.c[kk] = static_cast<float>(P.mc64.c[kk]); \
+  ++k;
+  TOPOHIP_MODAL_PATTERN(TOPOHIP_X)
+#undef TOPOHIP_X
+  (void)k;
+  for (int i = 0; i < 576; ++i)
+    if (!in_pat[i] && std::fabs(M[i]) > 1e-12 * mx)
+      throw Error(TOPOHIP_EINTERNAL, "element stiffness outside the modal pattern");
+  for (int c = 0; c < 8; ++c) {
+    double W[576];
+    for (int a = 0; a < 24; ++a)
+      for (int s = 0; s < 24; ++s) {
+        double w = 0.0;
+        if (a % 3 == s % 3) {
+          const int ma = a / 3, ms = s / 3;
+          w = w1h((c & 1) + (ma & 1), ms & 1) * w1h(((c >> 1) & 1) + ((ma >> 1) & 1), (ms >> 1) & 1) *
+              w1h(((c >> 2) & 1) + ((ma >> 2) & 1), (ms >> 2) & 1);
+        }
+        W[a * 24 + s] = w;
+      }
+    double KW[576];
+    for (int a = 0; a < 24; ++a)
+      for (int s = 0; s < 24; ++s) {
+        double v = 0.0;
+        for (int b = 0; b < 24; ++b) v += P.Kb[a * 24 + b] * W[b * 24 + s];
+        KW[a * 24 + s] = v;
+      }
+    for (int r = 0; r < 24; ++r)
+      for (int s = 0; s < 24; ++s) {
+        double v = 0.0;
+        for (int a = 0; a < 24; ++a) v += W[a * 24 + r] * KW[a * 24 + s];
+        P.S8[c * 576 + r * 24 + s] = v;
+      }
+  }
+}
+
+// Coarsening plan (build_mf_hierarchy + build_hierarchy acceptance rules,
+// core/src/fea/multigrid.cpp:557-716, 1344-1515; unpadded geometric branch).
+void plan_levels(Problem& P) {
+  P.L.clear();
+  auto lv0 = std::make_unique<Level>();
+  lv0->g = P.g;
+  const long long nn = P.g.nodes();
+  lv0->mask_h.assign(static_cast<size_t>(nn), 0);
+  long long nact = 0;
+  for (long long n = 0; n < nn; ++n) {
+    const uint8_t m = P.touched_h[n] ? static_cast<uint8_t>(~P.fixed_h[n] & 7) : 0;
+    lv0->mask_h[n] = m;
+    nact += popcount3(m);
+  }
+  lv0->nact = nact;
+  P.L.push_back(std::move(lv0));
+  P.mg_plan_ok = false;
+
+  auto coarsen = [](const Level& f, std::unique_ptr<Level>& out) -> bool {
+    if ((f.g.ex & 1) || (f.g.ey & 1) || (f.g.ez & 1)) return false;
+    if (f.g.ex / 2 < kMinCoarseElems || f.g.ey / 2 < kMinCoarseElems || f.g.ez / 2 < kMinCoarseElems)
+      return false;
+    out = std::make_unique<Level>();
+    out->g = make_geom(f.g.ex / 2, f.g.ey / 2, f.g.ez / 2);
+    const Geom& gc = out->g;
+    out->mask_h.assign(static_cast<size_t>(gc.nodes()), 0);
+    long long na = 0;
+    for (int c = 0; c < gc.nnz; ++c)
+      for (int b = 0; b < gc.nny; ++b)
+        for (int a = 0; a < gc.nnx; ++a) {
+          const long long fn = (static_cast<long long>(2 * c) * f.g.nny + 2 * b) * f.g.nnx + 2 * a;
+          const long long cn = (static_cast<long long>(c) * gc.nny + b) * gc.nnx + a;
+          out->mask_h[cn] = f.mask_h[fn];
+          na += popcount3(f.mask_h[fn]);
+        }
+    out->nact = na;
+    return true;
+  };
+
+  std::unique_ptr<Level> l1;
+  if (!coarsen(*P.L[0], l1) || l1->nact == 0) return;
+  std::vector<std::unique_ptr<Level>> sub;
+  sub.push_back(std::move(l1));
+  while (true) {
+    std::unique_ptr<Level> nx;
+    if (!coarsen(*sub.back(), nx)) break;
+    if (nx->nact == 0) break;
+    const long long nc = nx->nact;
+    sub.push_back(std::move(nx));
+    if (nc <= kCoarseDofCap) break;
+  }
+  const bool ok_sub = sub.size() >= 2 && sub.back()->nact <= kCoarseDofCap;
+  if (!ok_sub) {
+    if (sub[0]->nact > kCoarseDofCap) return;
+    sub.resize(1);
+  }
+  for (auto& l : sub) P.L.push_back(std::move(l));
+  P.mg_plan_ok = true;
+}
+
+template <typename T> T* Kapp(Level& l);
+template <> float* Kapp<float>(Level& l) { return l.K32.p; }
+template <> double* Kapp<double>(Level& l) { return l.K64.p; }
+template <typename T> T* Dinv(Level& l);
+template <> float* Dinv<float>(Level& l) { return l.dinv32.p; }
+template <> double* Dinv<double>(Level& l) { return l.dinv64.p; }
+template <typename T> T* Bv(Level& l);
+template <> float* Bv<float>(Level& l) { return l.b32.p; }
+template <> double* Bv<double>(Level& l) { return l.b64.p; }
+template <typename T> T* Xv(Level& l);
+template <> float* Xv<float>(Level& l) { return l.x32.p; }
+template <> double* Xv<double>(Level& l) { return l.x64.p; }
+template <typename T> T* Tv(Level& l);
+template <> float* Tv<float>(Level& l) { return l.t32.p; }
+template <> double* Tv<double>(Level& l) { return l.t64.p; }
+template <typename T> const T* Ev(Problem& P);
+template <> const float* Ev<float>(Problem& P) { return P.E32.p; }
+template <> const double* Ev<double>(Problem& P) { return P.E64.p; }
+template <typename T> const Modal<T>& Mc(Problem& P);
+template <> const Modal<float>& Mc<float>(Problem& P) { return P.mc32; }
+template <> const Modal<double>& Mc<double>(Problem& P) { return P.mc64; }
+
+void alloc_levels(Problem& P) {
+  const bool mixed = P.precision == TOPOHIP_PREC_MIXED;
+  for (size_t l = 0; l < P.L.size(); ++l) {
+    Level& lv = *P.L[l];
+    const long long nd = lv.g.ndof();
+    lv.mask.ensure(static_cast<size_t>(lv.g.nodes()));
+    h2d(lv.mask.p, lv.mask_h.data(), lv.mask_h.size(), P.s);
+    // FP64 diagonal always (Jacobi-CG fallback on the fine level).
+    lv.dinv64.ensure(static_cast<size_t>(nd));
+    if (mixed) lv.dinv32.ensure(static_cast<size_t>(nd));
+    if (l > 0) {
+      lv.Kem.ensure(static_cast<size_t>(lv.g.elems()) * 576);
+      const bool coarsest = l + 1 == P.L.size();
+      if (!coarsest) {
+        if (mixed) lv.K32.ensure(static_cast<size_t>(lv.g.elems()) * 576);
+        else lv.K64.ensure(static_cast<size_t>(lv.g.elems()) * 576);
+      }
+      if (mixed) { lv.b32.ensure(nd); lv.x32.ensure(nd); lv.t32.ensure(nd); }
+      else { lv.b64.ensure(nd); lv.x64.ensure(nd); lv.t64.ensure(nd); }
+    } else {
+      if (mixed) { lv.x32.ensure(nd); lv.t32.ensure(nd); }
+      else { lv.x64.ensure(nd); lv.t64.ensure(nd); }
+    }
+  }
+  if (P.mg_plan_ok) {
+    const Level& c = *P.L.back();
+    P.dense_n = static_cast<int>(c.g.ndof());
+    P.Ad.ensure(static_cast<size_t>(P.dense_n) * P.dense_n);
+    P.gtmp.ensure(static_cast<size_t>(gemv_tmp_size(P.dense_n)));
+  }
+}
+
+// ---- V-cycle ----------------------------------------------------------------
+// Coarse levels l >= 1: restrict the level-(l-1) residual rprev, smooth,
+// recurse, prolong, smooth. Result lives in t_l (x_l at the coarsest).
+template <typename T>
+void coarse_cycle(Problem& P, size_t l, const T* rprev, const int* done) {
+  Level& f = *P.L[l - 1];
+  Level& c = *P.L[l];
+  const bool coarsest = l + 1 == P.L.size();
+  if (coarsest) {
+    launch_restrict<T, T>(f.g, c.g, rprev, c.mask.p, Bv<T>(c), nullptr, nullptr, 0.0, done, P.s);
+    launch_sym_gemv<T, T>(P.Ad.p, P.dense_n, Bv<T>(c), Xv<T>(c), P.gtmp.p, done, P.s);
+    return;
+  }
+  launch_restrict<T, T>(f.g, c.g, rprev, c.mask.p, Bv<T>(c), Xv<T>(c), Dinv<T>(c), kOmega, done,
+                        P.s);
+  KuOp<T> op;
+  op.g = c.g;
+  op.mask = c.mask.p;
+  op.Kel = Kapp<T>(c);
+  op.tz = P.tz;
+  op.done = done;
+  op.x = Xv<T>(c);
+  op.b = Bv<T>(c);
+  op.y = Tv<T>(c);
+  op.epi = EPI_RES;
+  launch_ku_dense<T>(op, P.s);
+  coarse_cycle<T>(P, l + 1, Tv<T>(c), done);
+  Level& n = *P.L[l + 1];
+  const bool ncoarsest = l + 2 == P.L.size();
+  launch_prolong_add<T, T>(n.g, c.g, ncoarsest ? Xv<T>(n) : Tv<T>(n), c.mask.p, Xv<T>(c), done,
+                           P.s);
+  op.epi = EPI_JAC;
+  op.dinv = Dinv<T>(c);
+  op.omega = static_cast<T>(kOmega);
+  op.y = Tv<T>(c);
+  launch_ku_dense<T>(op, P.s);
+}
+
+// z = M^{-1} b0 on the fine level (mf_v_cycle with 1+1 damped Jacobi sweeps;
+// the first sweep starts from zero so it is the scaling omega D^-1 b0).
+template <typename T>
+void vcycle(Problem& P, const T* b0, T* z, const int* done) {
+  Level& f = *P.L[0];
+  const long long nd = f.g.ndof();
+  launch_jacobi0<T>(Xv<T>(f), Dinv<T>(f), b0, kOmega, nd, done, P.s);
+  KuOp<T> op;
+  op.g = f.g;
+  op.mask = f.mask.p;
+  op.E = Ev<T>(P);
+  op.mc = Mc<T>(P);
+  op.tz = P.tz;
+  op.done = done;
+  op.x = Xv<T>(f);
+  op.b = b0;
+  op.y = Tv<T>(f);
+  op.epi = EPI_RES;
+  launch_ku_fine<T>(op, P.s);
+  coarse_cycle<T>(P, 1, Tv<T>(f), done);
+  Level& c = *P.L[1];
+  const bool ccoarsest = P.L.size() == 2;
+  launch_prolong_add<T, T>(c.g, f.g, ccoarsest ? Xv<T>(c) : Tv<T>(c), f.mask.p, Xv<T>(f), done,
+                           P.s);
+  op.epi = EPI_JAC;
+  op.dinv = Dinv<T>(f);
+  op.omega = static_cast<T>(kOmega);
+  op.y = z;
+  launch_ku_fine<T>(op, P.s);
+  P.matvecs += 2;
+}
+
+template <typename T>
+void fine_apply(Problem& P, const T* x, T* y, const T* b, int epi, double* partials,
+                const int* done) {
+  KuOp<T> op;
+  op.g = P.g;
+  op.mask = P.L[0]->mask.p;
+  op.E = Ev<T>(P);
+  op.mc = Mc<T>(P);
+  op.tz = P.tz;
+  op.done = done;
+  op.x = x;
+  op.y = y;
+  op.b = b;
+  op.epi = epi;
+  op.partials = partials;
+  launch_ku_fine<T>(op, P.s);
+  P.matvecs += 1;
+}
+
+int max_partials(const Problem& P) {
+  return std::max(ku_blocks(P.g, P.tz), std::max(dot_blocks(P.g.ndof()), sens_blocks(P.g)));
+}
+
+enum class Prec { MG32, MG64, JACOBI };
+
+// Preconditioner application z = M^{-1} r. Mixed: r32 must hold float(r).
+void precondition(Problem& P, Prec pc, const int* done) {
+  const long long nd = P.g.ndof();
+  if (pc == Prec::MG32) vcycle<float>(P, P.r32.p, P.z32.p, done);
+  else if (pc == Prec::MG64) vcycle<double>(P, P.r.p, P.z64.p, done);
+  else launch_jacobi0<double>(P.z64.p, P.L[0]->dinv64.p, P.r.p, 1.0, nd, done, P.s);
+}
+
+void dot_rz(Problem& P, Prec pc, const int* done) {
+  const long long nd = P.g.ndof();
+  const int nb = dot_blocks(nd);
+  if (pc == Prec::MG32) launch_dot<double, float>(P.r.p, P.z32.p, nd, P.partials.p, nb, done, P.s);
+  else launch_dot<double, double>(P.r.p, P.z64.p, nd, P.partials.p, nb, done, P.s);
+}
+
+// Restart the recurrence from the current r: z = M^-1 r, rz = r.z, p = z.
+void restart(Problem& P, Prec pc) {
+  const long long nd = P.g.ndof();
+  const int* done = &P.st.p->done;
+  if (pc == Prec::MG32) launch_convert<double, float>(P.r.p, P.r32.p, nd, P.s);
+  precondition(P, pc, done);
+  dot_rz(P, pc, done);
+  launch_cg_set_rz(P.st.p, P.partials.p, dot_blocks(nd), P.s);
+  if (pc == Prec::MG32) launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, P.p32.p, nd, 1, P.s);
+  else launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nullptr, nd, 1, P.s);
+}
+
+void iteration(Problem& P, Prec pc) {
+  const long long nd = P.g.ndof();
+  const int* done = &P.st.p->done;
+  const int kb = ku_blocks(P.g, P.tz);
+  const int db = dot_blocks(nd);
+  if (pc == Prec::MG32) {
+    fine_apply<float>(P, P.p32.p, P.Ap32.p, nullptr, EPI_AX, P.partials.p, done);
+    launch_cg_alpha(P.st.p, P.partials.p, kb, P.s);
+    launch_cg_xr<float, float>(P.st.p, P.x.p, P.r.p, P.p32.p, P.Ap32.p, P.r32.p, nd, P.partials.p,
+                               db, P.s);
+  } else {
+    fine_apply<double>(P, P.p.p, P.Ap64.p, nullptr, EPI_AX, P.partials.p, done);
+    launch_cg_alpha(P.st.p, P.partials.p, kb, P.s);
+    launch_cg_xr<double, double>(P.st.p, P.x.p, P.r.p, P.p.p, P.Ap64.p, nullptr, nd, P.partials.p,
+                                 db, P.s);
+  }
+  launch_cg_check(P.st.p, P.partials.p, db, P.s);
+  precondition(P, pc, done);
+  dot_rz(P, pc, done);
+  launch_cg_beta(P.st.p, P.partials.p, db, P.s);
+  if (pc == Prec::MG32) launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, P.p32.p, nd, 0, P.s);
+  else launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nullptr, nd, 0, P.s);
+}
+
+void read_state(Problem& P) { d2h(P.st_h, P.st.p, 1, P.s); }
+
+// rt = b - A x in FP64; returns ||rt||^2.
+double true_residual(Problem& P) {
+  const long long nd = P.g.ndof();
+  fine_apply<double>(P, P.x.p, P.rt.p, P.b.p, EPI_RES, nullptr, nullptr);
+  launch_dot<double, double>(P.rt.p, P.rt.p, nd, P.partials.p, dot_blocks(nd), nullptr, P.s);
+  launch_sum_partials(P.partials.p, dot_blocks(nd), P.scal.p, P.s);
+  d2h(P.scal_h, P.scal.p, 1, P.s);
+  return P.scal_h[0];
+}
+
+// PCG on the device. Returns true when ||b - A x|| <= tol ||b|| in FP64.
+// x must hold the start vector. iters/resid report the attempt.
+bool pcg(Problem& P, Prec pc, double tol, int cap, int& iters, double& resid, bool warm) {
+  const long long nd = P.g.ndof();
+  const int db = dot_blocks(nd);
+  // r = b - A x (warm) or b.
+  if (warm) {
+    fine_apply<double>(P, P.x.p, P.r.p, P.b.p, EPI_RES, nullptr, nullptr);
+  } else {
+    TOPOHIP_CHECK(hipMemcpyAsync(P.r.p, P.b.p, nd * sizeof(double), hipMemcpyDeviceToDevice, P.s));
+  }
+  launch_dot<double, double>(P.b.p, P.b.p, nd, P.partials.p, db, nullptr, P.s);
+  launch_cg_init(P.st.p, P.partials.p, db, tol, cap, P.s);
+  if (warm) {
+    launch_dot<double, double>(P.r.p, P.r.p, nd, P.partials.p, db, nullptr, P.s);
+    launch_cg_resume(P.st.p, P.partials.p, db, P.s);
+  }
+  restart(P, pc);
+  read_state(P);
+  iters = 0;
+  resid = 0.0;
+  if (P.st_h->bnorm2 == 0.0) {
+    launch_fill<double>(P.x.p, 0.0, nd, P.s);
+    return true;
+  }
+  const int batch = 4;
+  for (long long guard = 0;; ++guard) {
+    if (guard > 4LL * cap + 64) throw Error(TOPOHIP_EINTERNAL, "pcg: iteration state stalled");
+    if (!P.st_h->done) {
+      for (int i = 0; i < batch; ++i) iteration(P, pc);
+      read_state(P);
+      if (!P.st_h->done) continue;
+    }
+    iters = P.st_h->iters;
+    if (P.st_h->fail) {
+      resid = std::sqrt(P.st_h->rr / P.st_h->bnorm2);
+      return false;
+    }
+    // Recurrence converged: confirm with the FP64 true residual.
+    const double rr = true_residual(P);
+    resid = std::sqrt(rr / P.st_h->bnorm2);
+    if (!std::isfinite(rr)) return false;
+    if (rr <= P.st_h->thresh2) return true;
+    if (iters >= cap) return false;
+    // Residual replacement: continue from r = b - A x.
+    ++P.replacements;
+    TOPOHIP_CHECK(hipMemcpyAsync(P.r.p, P.rt.p, nd * sizeof(double), hipMemcpyDeviceToDevice, P.s));
+    launch_dot<double, double>(P.r.p, P.r.p, nd, P.partials.p, db, nullptr, P.s);
+    launch_cg_resume(P.st.p, P.partials.p, db, P.s);
+    restart(P, pc);
+    read_state(P);
+  }
+}
+
+}  // namespace
+
+std::unique_ptr<Problem> create_problem(int nx, int ny, int nz, double h, double nu,
+                                        const uint8_t* solid, int n_bc, const int32_t* bc_node,
+                                        const int32_t* bc_comp, const double* bc_val,
+                                        int precision) {
+  if (nx < 1 || ny < 1 || nz < 1) throw Error(TOPOHIP_EINVAL, "grid dimensions must be >= 1");
+  if (precision != TOPOHIP_PREC_MIXED && precision != TOPOHIP_PREC_F64)
+    throw Error(TOPOHIP_EINVAL, "unknown precision");
+  auto P = std::make_unique<Problem>();
+  P->g = make_geom(nx, ny, nz);
+  P->h = h;
+  P->nu = nu;
+  P->precision = precision;
+  element_constants(*P);  // validates h and nu
+  TOPOHIP_CHECK(hipGetDevice(&P->device));
+  TOPOHIP_CHECK(hipStreamCreateWithFlags(&P->s, hipStreamNonBlocking));
+  const long long nv = P->g.elems(), nn = P->g.nodes();
+  P->all_solid = solid == nullptr;
+  if (solid) {
+    P->solid_h.assign(solid, solid + nv);
+    for (auto& v : P->solid_h) v = v ? 1 : 0;
+    P->solid.ensure(static_cast<size_t>(nv));
+    h2d(P->solid.p, P->solid_h.data(), static_cast<size_t>(nv), P->s);
+  }
+  P->n_solid = 0;
+  P->touched_h.assign(static_cast<size_t>(nn), 0);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const long long e = (static_cast<long long>(k) * ny + j) * nx + i;
+        if (solid && !P->solid_h[e]) continue;
+        ++P->n_solid;
+        for (int m = 0; m < 8; ++m)
+          P->touched_h[(static_cast<long long>(k + (m >> 2)) * P->g.nny + j + ((m >> 1) & 1)) *
+                           P->g.nnx + i + (m & 1)] = 1;
+      }
+  P->fixed_h.assign(static_cast<size_t>(nn), 0);
+  P->up_h.assign(static_cast<size_t>(3 * nn), 0.0);
+  for (int i = 0; i < n_bc; ++i) {
+    if (bc_node[i] < 0 || bc_node[i] >= nn || bc_comp[i] < 0 || bc_comp[i] > 2)
+      throw Error(TOPOHIP_EINVAL, "fea_solve_mgcg_matfree: BC index out of range");
+    P->fixed_h[bc_node[i]] |= static_cast<uint8_t>(1 << bc_comp[i]);
+    P->up_h[3 * static_cast<long long>(bc_node[i]) + bc_comp[i]] = bc_val[i];
+    if (bc_val[i] != 0.0) P->nonzero_bc = true;
+  }
+  P->up.ensure(static_cast<size_t>(3 * nn));
+  h2d(P->up.p, P->up_h.data(), P->up_h.size(), P->s);
+  P->dKb.ensure(576);
+  h2d(P->dKb.p, P->Kb, 576, P->s);
+  P->dS8.ensure(8 * 576);
+  h2d(P->dS8.p, P->S8, 8 * 576, P->s);
+  P->dkdiag.ensure(24);
+  h2d(P->dkdiag.p, P->kdiag, 24, P->s);
+  P->E64.ensure(static_cast<size_t>(nv));
+  P->E32.ensure(static_cast<size_t>(nv));
+  const size_t nd = static_cast<size_t>(3 * nn);
+  P->b.ensure(nd); P->x.ensure(nd); P->r.ensure(nd); P->p.ensure(nd); P->rt.ensure(nd);
+  P->u.ensure(nd);
+  if (precision == TOPOHIP_PREC_MIXED) {
+    P->p32.ensure(nd); P->Ap32.ensure(nd); P->r32.ensure(nd); P->z32.ensure(nd);
+  }
+  P->Ap64.ensure(nd); P->z64.ensure(nd);
+  P->st.ensure(1);
+  TOPOHIP_CHECK(hipHostMalloc(&P->st_h, sizeof(CgState)));
+  TOPOHIP_CHECK(hipHostMalloc(&P->oc_h, sizeof(OcState)));
+  TOPOHIP_CHECK(hipHostMalloc(&P->scal_h, 8 * sizeof(double)));
+  P->scal.ensure(8);
+  P->oc.ensure(1);
+  plan_levels(*P);
+  alloc_levels(*P);
+  P->partials.ensure(static_cast<size_t>(max_partials(*P)) + 8);
+  if (rocblas_create_handle(&P->rb) != rocblas_status_success)
+    throw Error(TOPOHIP_EDEVICE, "rocblas_create_handle failed");
+  rocblas_set_stream(P->rb, P->s);
+  P->dinfo.ensure(1);
+  sync(*P);
+  return P;
+}
+
+void set_loads(Problem& P, int n_rhs, const int32_t* counts, const int32_t* node,
+               const int32_t* comp, const double* val) {
+  if (n_rhs < 1) throw Error(TOPOHIP_EINVAL, "n_rhs must be >= 1");
+  const long long nn = P.g.nodes();
+  const long long nd = 3 * nn;
+  P.f_h.assign(static_cast<size_t>(n_rhs), std::vector<double>(static_cast<size_t>(nd), 0.0));
+  long long q = 0;
+  for (int r = 0; r < n_rhs; ++r) {
+    if (counts[r] < 0) throw Error(TOPOHIP_EINVAL, "negative load count");
+    for (int i = 0; i < counts[r]; ++i, ++q) {
+      if (node[q] < 0 || node[q] >= nn || comp[q] < 0 || comp[q] > 2)
+        throw Error(TOPOHIP_EINVAL, "fea_solve_mgcg_matfree: load index out of range");
+      P.f_h[r][3 * static_cast<long long>(node[q]) + comp[q]] += val[q];
+    }
+  }
+  P.nrhs = n_rhs;
+  P.f.ensure(static_cast<size_t>(n_rhs * nd));
+  for (int r = 0; r < n_rhs; ++r) h2d(P.f.p + r * nd, P.f_h[r].data(), static_cast<size_t>(nd), P.s);
+  sync(P);
+}
+
+void build_hierarchy(Problem& P, Timers& t) {
+  const double t0 = now_ms();
+  const long long nv = P.g.elems();
+  const bool mixed = P.precision == TOPOHIP_PREC_MIXED;
+  launch_convert<double, float>(P.E64.p, P.E32.p, nv, P.s);
+  Level& f = *P.L[0];
+  launch_diag_fine<double>(f.g, P.E64.p, P.dkdiag.p, f.mask.p, f.dinv64.p, P.s);
+  if (mixed) launch_diag_fine<float>(f.g, P.E64.p, P.dkdiag.p, f.mask.p, f.dinv32.p, P.s);
+  if (P.mg_plan_ok) {
+    for (size_t l = 1; l < P.L.size(); ++l) {
+      Level& c = *P.L[l];
+      Level& pf = *P.L[l - 1];
+      if (l == 1)
+        launch_galerkin_fine(c.g, pf.g, P.E64.p, pf.mask.p, c.mask.p, P.dS8.p, P.dKb.p, c.Kem.p,
+                             P.s);
+      else
+        launch_galerkin_coarse(c.g, pf.g, pf.Kem.p, c.mask.p, c.Kem.p, P.s);
+      const bool coarsest = l + 1 == P.L.size();
+      if (!coarsest) {
+        if (mixed) {
+          launch_diag_dense<float>(c.g, c.Kem.p, c.mask.p, c.dinv32.p, P.s);
+          launch_transpose_K<float>(c.Kem.p, c.g.elems(), c.K32.p, P.s);
+        } else {
+          launch_diag_dense<double>(c.g, c.Kem.p, c.mask.p, c.dinv64.p, P.s);
+          launch_transpose_K<double>(c.Kem.p, c.g.elems(), c.K64.p, P.s);
+        }
+      }
+    }
+    Level& c = *P.L.back();
+    launch_assemble_dense(c.g, c.Kem.p, c.mask.p, P.Ad.p, P.s);
+    const int n = P.dense_n;
+    if (rocsolver_dpotrf(P.rb, rocblas_fill_lower, n, P.Ad.p, n, P.dinfo.p) != rocblas_status_success)
+      throw Error(TOPOHIP_EDEVICE, "rocsolver_dpotrf failed");
+    int info = 0;
+    d2h(&info, P.dinfo.p, 1, P.s);
+    if (info != 0) throw Error(TOPOHIP_EINTERNAL, "coarsest Galerkin operator is not SPD");
+    if (rocsolver_dpotri(P.rb, rocblas_fill_lower, n, P.Ad.p, n, P.dinfo.p) != rocblas_status_success)
+      throw Error(TOPOHIP_EDEVICE, "rocsolver_dpotri failed");
+    launch_symmetrize_lower(P.Ad.p, n, P.s);
+  }
+  sync(P);
+  t.mg_build += now_ms() - t0;
+}
+
+void solve_rhs(Problem& P, int rhs, double tol, int max_it, const double* d_guess,
+               topohip_cginfo* info) {
+  const double t0 = now_ms();
+  const long long nn = P.g.nodes(), nd = 3 * nn;
+  if (rhs < 0 || rhs >= P.nrhs) throw Error(TOPOHIP_EINVAL, "no load case set");
+  Level& f = *P.L[0];
+  // Void gate (core/src/fea/matfree.cpp:782-814).
+  std::vector<double> rhs_full = P.f_h[rhs];
+  if (P.nonzero_bc) {
+    KuOp<double> op;
+    op.g = P.g;
+    op.E = P.E64.p;
+    op.mc = P.mc64;
+    op.tz = P.tz;
+    op.x = P.up.p;
+    op.y = P.rt.p;
+    op.epi = EPI_AX;
+    launch_ku_fine<double>(op, P.s);
+    std::vector<double> kup(static_cast<size_t>(nd));
+    d2h(kup.data(), P.rt.p, static_cast<size_t>(nd), P.s);
+    for (long long d = 0; d < nd; ++d) rhs_full[d] -= kup[d];
+  }
+  long long nfree = 0, ntouched_free = 0;
+  double rmax = 0.0;
+  for (long long n = 0; n < nn; ++n)
+    for (int c = 0; c < 3; ++c) {
+      if ((P.fixed_h[n] >> c) & 1) continue;
+      ++nfree;
+      if (P.touched_h[n]) ++ntouched_free;
+      rmax = std::max(rmax, std::fabs(rhs_full[3 * n + c]));
+    }
+  if (nfree > 0 && ntouched_free == 0)
+    throw Error(TOPOHIP_EUNDERCONSTRAINED,
+                "fea_solve_mgcg_matfree: singular system (no stiffness — every free DOF is void)");
+  for (long long n = 0; n < nn; ++n) {
+    if (P.touched_h[n]) continue;
+    for (int c = 0; c < 3; ++c)
+      if (!((P.fixed_h[n] >> c) & 1) && std::fabs(rhs_full[3 * n + c]) > 1e-9 * rmax)
+        throw Error(TOPOHIP_EUNDERCONSTRAINED,
+                    "fea_solve_mgcg_matfree: under-constrained system (load applied to a void DOF "
+                    "with no stiffness — no equilibrium possible)");
+  }
+  // b = mask .* (f - K up); masked by the active bits through a transfer-free
+  // kernel: a residual apply with x = 0 would cost an apply, so mask on host
+  // only when prescribed values are non-zero, else mask on the device.
+  if (P.nonzero_bc) {
+    for (long long n = 0; n < nn; ++n)
+      for (int c = 0; c < 3; ++c)
+        if (!((f.mask_h[n] >> c) & 1)) rhs_full[3 * n + c] = 0.0;
+    h2d(P.b.p, rhs_full.data(), static_cast<size_t>(nd), P.s);
+  } else {
+    std::vector<double> bm = P.f_h[rhs];
+    for (long long n = 0; n < nn; ++n)
+      for (int c = 0; c < 3; ++c)
+        if (!((f.mask_h[n] >> c) & 1)) bm[3 * n + c] = 0.0;
+    h2d(P.b.p, bm.data(), static_cast<size_t>(nd), P.s);
+  }
+  long long ng = f.nact;
+  const int cap = max_it > 0 ? max_it : static_cast<int>(std::max<long long>(1000, 2 * ng));
+  topohip_cginfo d{};
+  d.converged = 1;
+  P.replacements = 0;
+  const long long mv0 = P.matvecs;
+  if (ng > 0) {
+    bool warm = false;
+    std::vector<double> gh;
+    if (d_guess) {
+      // masked copy of the guess: x = guess on active DOFs; a non-finite
+      // entry makes the solve cold, as the reference's initial_guess handling.
+      gh.resize(static_cast<size_t>(nd));
+      d2h(gh.data(), d_guess, static_cast<size_t>(nd), P.s);
+      warm = true;
+      for (long long n = 0; n < nn && warm; ++n)
+        for (int c = 0; c < 3; ++c) {
+          if (!((f.mask_h[n] >> c) & 1)) { gh[3 * n + c] = 0.0; continue; }
+          if (!std::isfinite(gh[3 * n + c])) { warm = false; break; }
+        }
+      if (warm) h2d(P.x.p, gh.data(), static_cast<size_t>(nd), P.s);
+    }
+    if (!warm) launch_fill<double>(P.x.p, 0.0, nd, P.s);
+    bool solved = false;
+    int it = 0;
+    double res = 0.0;
+    const double tm = now_ms();
+    d.hier_built = P.mg_plan_ok;
+    if (P.mg_plan_ok) {
+      const Prec pc = P.precision == TOPOHIP_PREC_MIXED ? Prec::MG32 : Prec::MG64;
+      solved = pcg(P, pc, tol, std::min(cap, kMgIterBudget), it, res, warm);
+      d.mg_cycles_attempted = it;
+      if (solved) {
+        d.iterations = it;
+        d.residual = res;
+        d.converged = 1;
+        d.used_multigrid = 1;
+        d.mg_levels = static_cast<int>(P.L.size());
+        d.coarse_dofs = static_cast<int>(P.L.back()->nact);
+      }
+      d.t_mg_ms = now_ms() - tm;
+    }
+    if (!solved) {
+      // Exact Jacobi-PCG fallback (core/src/fea/matfree.cpp:841-1076), FP64.
+      const double tc = now_ms();
+      if (warm) h2d(P.x.p, gh.data(), static_cast<size_t>(nd), P.s);
+      else launch_fill<double>(P.x.p, 0.0, nd, P.s);
+      solved = pcg(P, Prec::JACOBI, tol, cap, it, res, warm);
+      d.iterations = it;
+      d.residual = res;
+      d.converged = solved;
+      d.used_multigrid = 0;
+      d.mg_levels = 0;
+      d.t_cg_ms = now_ms() - tc;
+      d.matvecs = P.matvecs - mv0;
+      d.residual_replacements = P.replacements;
+      d.t_total_ms = now_ms() - t0;
+      if (info) *info = d;
+      if (!solved)
+        throw Error(TOPOHIP_ENONCONVERGENCE,
+                    "fea_solve_mgcg_matfree: CG did not reach the requested tolerance within "
+                    "max_iterations");
+    }
+  } else {
+    launch_fill<double>(P.x.p, 0.0, nd, P.s);
+  }
+  launch_add(P.x.p, P.up.p, P.u.p, nd, P.s);
+  sync(P);
+  d.matvecs = P.matvecs - mv0;
+  d.residual_replacements = P.replacements;
+  d.t_total_ms = now_ms() - t0;
+  if (info) *info = d;
+}
+
+FilterCache& filter_for(Problem& P, double radius) {
+  if (!(radius > 0.0)) throw Error(TOPOHIP_EINVAL, "make_density_filter: radius must be > 0");
+  for (auto& fc : P.filters)
+    if (fc->radius == radius) return *fc;
+  auto fc = std::make_unique<FilterCache>();
+  fc->radius = radius;
+  const int R = static_cast<int>(std::ceil(radius));
+  std::vector<int> off;
+  std::vector<double> w;
+  // Neighbour order of make_density_filter (core/src/simp/simp.cpp:862-880).
+  for (int dk = -R; dk <= R; ++dk)
+    for (int dj = -R; dj <= R; ++dj)
+      for (int di = -R; di <= R; ++di) {
+        const double dist = std::sqrt(static_cast<double>(di * di + dj * dj + dk * dk));
+        const double hw = radius - dist;
+        if (hw <= 0.0) continue;
+        off.push_back(di);
+        off.push_back(dj);
+        off.push_back(dk);
+        w.push_back(hw);
+      }
+  fc->off.ensure(off.size());
+  fc->w.ensure(w.size());
+  h2d(fc->off.p, off.data(), off.size(), P.s);
+  h2d(fc->w.p, w.data(), w.size(), P.s);
+  fc->st.n = static_cast<int>(w.size());
+  fc->st.off = fc->off.p;
+  fc->st.w = fc->w.p;
+  const long long nv = P.g.elems();
+  fc->hs.ensure(static_cast<size_t>(nv));
+  fc->dv.ensure(static_cast<size_t>(nv));
+  const uint8_t* sol = P.all_solid ? nullptr : P.solid.p;
+  launch_filter_hs(P.g, fc->st, sol, fc->hs.p, P.s);
+  // dv = H^T (ones / Hs) over design voxels (ones on non-design voxels are
+  // ignored by the adjoint, which skips them).
+  DBuf<double> ones;
+  ones.ensure(static_cast<size_t>(nv));
+  launch_fill<double>(ones.p, 1.0, nv, P.s);
+  launch_filter(P.g, fc->st, sol, fc->hs.p, ones.p, fc->dv.p, 1, P.s);
+  sync(P);
+  P.filters.push_back(std::move(fc));
+  return *P.filters.back();
+}
+
+double simp_compliance_dev(Problem& P, const double* d_xphys, double E0, double penalty,
+                           double rho_min, double tol, int max_it, topohip_cginfo* info) {
+  if (!(E0 > 0.0)) throw Error(TOPOHIP_EINVAL, "simp: youngs_modulus (E0) must be > 0");
+  if (!(penalty > 0.0)) throw Error(TOPOHIP_EINVAL, "simp: penalty (p) must be > 0");
+  if (!(rho_min > 0.0 && rho_min <= 1.0))
+    throw Error(TOPOHIP_EINVAL, "simp: density_min must be in (0, 1]");
+  if (P.nrhs < 1) throw Error(TOPOHIP_EINVAL, "no load case set");
+  const long long nv = P.g.elems();
+  const uint8_t* sol = P.all_solid ? nullptr : P.solid.p;
+  launch_youngs(d_xphys, sol, nv, E0, penalty, rho_min, P.E64.p, nullptr, P.s);
+  Timers t;
+  build_hierarchy(P, t);
+  P.dc.ensure(static_cast<size_t>(nv));
+  const int sb = sens_blocks(P.g);
+  double c = 0.0;
+  topohip_cginfo agg{};
+  for (int r = 0; r < P.nrhs; ++r) {
+    topohip_cginfo ci{};
+    solve_rhs(P, r, tol, max_it, nullptr, &ci);
+    if (r == 0) agg = ci;
+    else {
+      agg.iterations += ci.iterations;
+      agg.matvecs += ci.matvecs;
+      agg.t_total_ms += ci.t_total_ms;
+      agg.residual = std::max(agg.residual, ci.residual);
+      agg.converged = agg.converged && ci.converged;
+    }
+    launch_sensitivity(P.g, P.u.p, d_xphys, sol, P.mc64, E0, penalty, rho_min, P.dc.p, r > 0,
+                       P.partials.p, sb, P.s);
+    launch_sum_partials(P.partials.p, sb, P.scal.p, P.s);
+    d2h(P.scal_h, P.scal.p, 1, P.s);
+    c += P.scal_h[0];
+  }
+  agg.t_mg_build_ms = t.mg_build;
+  if (info) *info = agg;
+  return c;
+}
+
+void oc_update_dev(Problem& P, double radius, const double* d_x, const double* d_dc,
+                   double volfrac, double move, double rho_min, double* d_xnew, uint8_t* d_active,
+                   int* bisections, double* lambda, double* change) {
+  if (!(volfrac > 0.0 && volfrac <= 1.0))
+    throw Error(TOPOHIP_EINVAL, "oc_update: volume_fraction must be in (0, 1]");
+  if (!(move > 0.0)) throw Error(TOPOHIP_EINVAL, "oc_update: move must be > 0");
+  if (!(rho_min > 0.0 && rho_min <= 1.0))
+    throw Error(TOPOHIP_EINVAL, "oc_update: density_min must be in (0, 1]");
+  FilterCache& fc = filter_for(P, radius);
+  const long long nv = P.g.elems();
+  const uint8_t* sol = P.all_solid ? nullptr : P.solid.p;
+  P.dcf.ensure(static_cast<size_t>(nv));
+  launch_filter(P.g, fc.st, sol, fc.hs.p, d_dc, P.dcf.p, 1, P.s);
+  const double target = volfrac * static_cast<double>(P.n_solid);
+  launch_oc_init(P.oc.p, target, P.s);
+  const int ob = oc_blocks(nv);
+  // The search takes at most ~100 bracket + 100 bisection trials.
+  for (int round = 0;; ++round) {
+    if (round > 40) throw Error(TOPOHIP_EINTERNAL, "oc_update: lambda search did not finish");
+    for (int i = 0; i < 8; ++i) {
+      launch_oc_trial(d_x, P.dcf.p, fc.dv.p, sol, nv, move, rho_min, P.oc.p, P.partials.p, ob, P.s);
+      launch_oc_decide(P.oc.p, P.partials.p, ob, P.s);
+    }
+    d2h(P.oc_h, P.oc.p, 1, P.s);
+    if (P.oc_h->phase == 2) break;
+  }
+  launch_oc_final(d_x, P.dcf.p, fc.dv.p, sol, nv, move, rho_min, P.oc.p, d_xnew, d_active,
+                  P.partials.p, ob, P.s);
+  launch_max_partials(P.partials.p, ob, P.scal.p, P.s);
+  d2h(P.scal_h, P.scal.p, 1, P.s);
+  if (bisections) *bisections = P.oc_h->steps;
+  if (lambda) *lambda = 0.5 * (P.oc_h->l1 + P.oc_h->l2);
+  if (change) *change = P.scal_h[0];
+}
+
+void optimize(Problem& P, const topohip_options& o, double* design_out, double* xphys_out,
+              double* history, topohip_result* res) {
+  if (!(o.volfrac > 0.0 && o.volfrac <= 1.0))
+    throw Error(TOPOHIP_EINVAL, "simp_optimize: volume_fraction must be in (0, 1]");
+  if (!(o.move > 0.0)) throw Error(TOPOHIP_EINVAL, "simp_optimize: move must be > 0");
+  if (o.max_iterations < 1) throw Error(TOPOHIP_EINVAL, "simp_optimize: max_iterations must be >= 1");
+  if (o.change_tol < 0.0) throw Error(TOPOHIP_EINVAL, "simp_optimize: change_tol must be >= 0");
+  std::vector<std::pair<double, int>> stages;
+  if (o.pc_per_stage > 0) {
+    if (!(o.pc_p0 > 0.0) || !(o.pc_p1 >= o.pc_p0) || !(o.pc_step > 0.0))
+      throw Error(TOPOHIP_EINVAL, "penalty_continuation_ramp: bad arguments");
+    const long long n = static_cast<long long>((o.pc_p1 - o.pc_p0) / o.pc_step + 1e-9);
+    for (long long k = 0; k <= n; ++k)
+      stages.emplace_back(o.pc_p0 + static_cast<double>(k) * o.pc_step, o.pc_per_stage);
+  }
+  auto penalty_at = [&](int it) {
+    if (stages.empty()) return o.penalty;
+    long long rem = it;
+    for (auto& s : stages) {
+      rem -= s.second;
+      if (rem <= 0) return s.first;
+    }
+    return stages.back().first;
+  };
+  const double t0 = now_ms();
+  const long long nv = P.g.elems();
+  const uint8_t* sol = P.all_solid ? nullptr : P.solid.p;
+  FilterCache& fc = filter_for(P, o.rmin);
+  P.xd.ensure(static_cast<size_t>(nv));
+  P.xphys.ensure(static_cast<size_t>(nv));
+  P.xnew.ensure(static_cast<size_t>(nv));
+  // x = volfrac on solid voxels (simp_uniform_density, simp.cpp:305-312)
+  {
+    std::vector<double> x0(static_cast<size_t>(nv), 0.0);
+    for (long long e = 0; e < nv; ++e)
+      if (P.all_solid || P.solid_h[e]) x0[e] = o.volfrac;
+    h2d(P.xd.p, x0.data(), static_cast<size_t>(nv), P.s);
+  }
+  topohip_result r{};
+  double t_solve = 0, t_filter = 0, t_oc = 0, t_build = 0;
+  const double nd = static_cast<double>(P.n_solid);
+  const int vb = dot_blocks(nv);
+  for (int it = 0; it < o.max_iterations; ++it) {
+    double ta = now_ms();
+    launch_filter(P.g, fc.st, sol, fc.hs.p, P.xd.p, P.xphys.p, 0, P.s);
+    sync(P);
+    t_filter += now_ms() - ta;
+    ta = now_ms();
+    topohip_cginfo ci{};
+    double c = 0.0;
+    try {
+      c = simp_compliance_dev(P, P.xphys.p, o.E0, penalty_at(r.iterations + 1), o.rho_min, o.cg_tol,
+                              o.cg_max_iterations, &ci);
+    } catch (const Error& e) {
+      if (e.code != TOPOHIP_ENONCONVERGENCE) throw;
+      r.non_convergent = 1;
+      break;
+    }
+    t_solve += now_ms() - ta;
+    t_build += ci.t_mg_build_ms;
+    ta = now_ms();
+    double change = 0.0;
+    oc_update_dev(P, o.rmin, P.xd.p, P.dc.p, o.volfrac, o.move, o.rho_min, P.xnew.p, nullptr,
+                  nullptr, nullptr, &change);
+    std::swap(P.xd.p, P.xnew.p);
+    t_oc += now_ms() - ta;
+    if (r.iterations == 0) r.initial_compliance = c;
+    ++r.iterations;
+    r.cg_iterations_total += ci.iterations;
+    ta = now_ms();
+    launch_filter(P.g, fc.st, sol, fc.hs.p, P.xd.p, P.xphys.p, 0, P.s);
+    launch_sum_vec<double>(P.xphys.p, nv, P.partials.p, vb, P.s);
+    launch_sum_partials(P.partials.p, vb, P.scal.p, P.s);
+    d2h(P.scal_h, P.scal.p, 1, P.s);
+    t_filter += now_ms() - ta;
+    const double vf = nd > 0 ? P.scal_h[0] / nd : 0.0;
+    if (history) {
+      history[4 * it + 0] = c;
+      history[4 * it + 1] = change;
+      history[4 * it + 2] = vf;
+      history[4 * it + 3] = ci.iterations;
+    }
+    if (change < o.change_tol) {
+      r.converged = 1;
+      break;
+    }
+  }
+  launch_filter(P.g, fc.st, sol, fc.hs.p, P.xd.p, P.xphys.p, 0, P.s);
+  if (r.non_convergent) {
+    r.compliance = history && r.iterations > 0 ? history[4 * (r.iterations - 1)] : 0.0;
+  } else {
+    try {
+      const double ta = now_ms();
+      topohip_cginfo ci{};
+      r.compliance = simp_compliance_dev(P, P.xphys.p, o.E0, o.penalty, o.rho_min, o.cg_tol,
+                                         o.cg_max_iterations, &ci);
+      t_solve += now_ms() - ta;
+    } catch (const Error& e) {
+      if (e.code != TOPOHIP_ENONCONVERGENCE) throw;
+      r.non_convergent = 1;
+      r.converged = 0;
+    }
+  }
+  launch_sum_vec<double>(P.xphys.p, nv, P.partials.p, vb, P.s);
+  launch_sum_partials(P.partials.p, vb, P.scal.p, P.s);
+  d2h(P.scal_h, P.scal.p, 1, P.s);
+  r.volume_fraction = nd > 0 ? P.scal_h[0] / nd : 0.0;
+  if (design_out) d2h(design_out, P.xd.p, static_cast<size_t>(nv), P.s);
+  if (xphys_out) d2h(xphys_out, P.xphys.p, static_cast<size_t>(nv), P.s);
+  r.t_total_ms = now_ms() - t0;
+  r.t_solve_ms = t_solve;
+  r.t_build_ms = t_build;
+  r.t_filter_ms = t_filter;
+  r.t_oc_ms = t_oc;
+  if (res) *res = r;
+}
+
+}  // namespace topohip

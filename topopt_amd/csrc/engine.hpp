@@ -1,0 +1,153 @@
+// engine.hpp — device-resident problem handle of the voxel-SIMP engine.
+#pragma once
+
+#include <rocblas/rocblas.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/topopt_hip.h"
+#include "kernels.hpp"
+
+namespace topohip {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void ensure(size_t m) {
+    if (m <= n && p) return;
+    release();
+    TOPOHIP_CHECK(hipMalloc(&p, (m > 0 ? m : 1) * sizeof(T)));
+    n = m;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// One level of the multigrid hierarchy (level 0 = the fine, matrix-free one).
+struct Level {
+  Geom g;
+  std::vector<uint8_t> mask_h;  // per node: bit c = DOF c active
+  DBuf<uint8_t> mask;
+  long long nact = 0;           // active DOFs
+  DBuf<double> Kem;             // element-major FP64 matrices (levels >= 1)
+  DBuf<float> K32;              // entry-major apply copies
+  DBuf<double> K64;
+  DBuf<float> dinv32;
+  DBuf<double> dinv64;
+  DBuf<float> b32, x32, t32;    // V-cycle work (mixed form)
+  DBuf<double> b64, x64, t64;   // V-cycle work (FP64 form)
+};
+
+struct FilterCache {
+  double radius = 0.0;
+  Stencil st;
+  DBuf<int> off;
+  DBuf<double> w;
+  DBuf<double> hs, dv;
+};
+
+struct Timers {
+  double build = 0, mg_build = 0, mg = 0, cg = 0, total = 0;
+};
+
+struct Problem {
+  int device = 0;
+  hipStream_t s = nullptr;
+  rocblas_handle rb = nullptr;
+  Geom g;
+  double h = 1.0, nu = 0.3;
+  int precision = TOPOHIP_PREC_MIXED;
+  int tz = 8;
+
+  bool all_solid = true;
+  std::vector<uint8_t> solid_h;
+  DBuf<uint8_t> solid;
+  long long n_solid = 0;
+  std::vector<uint8_t> fixed_h;    // per node: bit c = Dirichlet-fixed
+  std::vector<uint8_t> touched_h;  // per node: 1 if a solid element touches it
+  std::vector<double> up_h;        // prescribed displacement (full)
+  bool nonzero_bc = false;
+  DBuf<double> up;
+
+  double Ke[576];   // reference DOF order
+  double Kb[576];   // parity-bit DOF order
+  double S8[8 * 576];
+  double kdiag[24];
+  Modal<float> mc32;
+  Modal<double> mc64;
+  DBuf<double> dKb, dS8, dkdiag;
+
+  int nrhs = 0;
+  std::vector<std::vector<double>> f_h;
+  DBuf<double> f;  // nrhs * ndof
+
+  DBuf<double> E64;
+  DBuf<float> E32;
+
+  std::vector<std::unique_ptr<Level>> L;
+  bool mg_plan_ok = false;
+  int dense_n = 0;
+  DBuf<double> Ad, gtmp;
+  DBuf<int> dinfo;
+
+  // PCG vectors
+  DBuf<double> b, x, r, p, Ap64, z64, rt, u;
+  DBuf<float> p32, Ap32, r32, z32;
+  DBuf<double> partials;
+  DBuf<CgState> st;
+  CgState* st_h = nullptr;
+  DBuf<double> scal;
+  double* scal_h = nullptr;
+
+  // SIMP work
+  std::vector<std::unique_ptr<FilterCache>> filters;
+  DBuf<OcState> oc;
+  OcState* oc_h = nullptr;
+  DBuf<double> xd, xphys, dc, xnew, dcf;
+  DBuf<uint8_t> active;
+
+  long long matvecs = 0;
+  int replacements = 0;
+
+  Problem() = default;
+  ~Problem();
+};
+
+std::unique_ptr<Problem> create_problem(int nx, int ny, int nz, double h, double nu,
+                                        const uint8_t* solid, int n_bc, const int32_t* bc_node,
+                                        const int32_t* bc_comp, const double* bc_val,
+                                        int precision);
+void set_loads(Problem& P, int n_rhs, const int32_t* counts, const int32_t* node,
+               const int32_t* comp, const double* val);
+// E64 (device) -> E32, diagonals, Galerkin levels and the coarsest inverse.
+void build_hierarchy(Problem& P, Timers& t);
+// Solve load case `rhs` with the current moduli; leaves the full displacement in P.u.
+void solve_rhs(Problem& P, int rhs, double tol, int max_it, const double* d_guess,
+               topohip_cginfo* info);
+FilterCache& filter_for(Problem& P, double radius);
+double simp_compliance_dev(Problem& P, const double* d_xphys, double E0, double penalty,
+                           double rho_min, double tol, int max_it, topohip_cginfo* info);
+void oc_update_dev(Problem& P, double radius, const double* d_x, const double* d_dc,
+                   double volfrac, double move, double rho_min, double* d_xnew,
+                   uint8_t* d_active, int* bisections, double* lambda, double* change);
+void optimize(Problem& P, const topohip_options& o, double* design_out, double* xphys_out,
+              double* history, topohip_result* res);
+void hex8_stiffness_host(double E, double nu, double h, double* K576);
+
+}  // namespace topohip

@@ -1,0 +1,382 @@
+// galerkin.hip — multigrid hierarchy build on the device.
+//
+// The reference forms A1 = P0^T A0 P0 element-locally without assembling A0
+// (galerkin_a1_from_prolong, core/src/fea/multigrid.cpp:1543-1868) and deeper
+// levels by sparse Galerkin products (build_hierarchy :557-716). On a
+// structured grid with 2x vertex coarsening every fine element lies in exactly
+// one coarse element and its nodes interpolate only from that coarse
+// element's 8 nodes, so every level is the sum of per-coarse-element 24x24
+// blocks:
+//   K1_E = Mc ( sum_c E_c W_c^T (Mf Ke Mf) W_c ) Mc
+//   K_E  = Mc ( sum_c W_c^T K_c W_c ) Mc            (levels >= 2)
+// with W_c the trilinear weights of child c and Mf/Mc the active-DOF masks
+// (rows/columns the reference's P drops). Children whose nodes are all active
+// use the 8 precomputed S_c = W_c^T Ke W_c blocks (the reference's colour
+// cache, :1760-1800). Element matrices are kept element-major in FP64 for the
+// build and transposed to entry-major (FP32 or FP64) for the apply kernel.
+// The coarsest level is assembled dense (FP64) for the explicit-inverse solve.
+#include "kernels.hpp"
+
+namespace topohip {
+
+namespace {
+
+// 1-D trilinear weight of coarse node bit b for fine position p in {0,1,2}
+// (axis_weights, core/src/fea/multigrid.cpp:494-506, within one coarse cell).
+__device__ inline double w1(int p, int b) {
+  if (p == 1) return 0.5;
+  return (p == 0) == (b == 0) ? 1.0 : 0.0;
+}
+// W_c(fine local dof a, coarse local dof s) for child c (bit-ordered).
+__device__ inline double wc(int c, int a, int s) {
+  if ((a % 3) != (s % 3)) return 0.0;
+  const int ma = a / 3, ms = s / 3;
+  return w1(((c >> 0) & 1) + ((ma >> 0) & 1), (ms >> 0) & 1) *
+         w1(((c >> 1) & 1) + ((ma >> 1) & 1), (ms >> 1) & 1) *
+         w1(((c >> 2) & 1) + ((ma >> 2) & 1), (ms >> 2) & 1);
+}
+
+__device__ inline long long node_of(const Geom& g, int a, int b, int c) {
+  return (static_cast<long long>(c) * g.nny + b) * g.nnx + a;
+}
+
+template <typename T>
+__global__ void diag_fine_k(Geom g, const double* __restrict__ E, const double* __restrict__ kd,
+                            const uint8_t* __restrict__ mask, T* __restrict__ dinv) {
+  const long long n = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (n >= g.nodes()) return;
+  const int a = static_cast<int>(n % g.nnx);
+  const int b = static_cast<int>((n / g.nnx) % g.nny);
+  const int c = static_cast<int>(n / (static_cast<long long>(g.nnx) * g.nny));
+  double d[3] = {0.0, 0.0, 0.0};
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx) {
+        const int i = a - dx, j = b - dy, k = c - dz;
+        if (i < 0 || j < 0 || k < 0 || i >= g.ex || j >= g.ey || k >= g.ez) continue;
+        const double e = E[(static_cast<long long>(k) * g.ey + j) * g.ex + i];
+        if (e == 0.0) continue;
+        const int m = dx + 2 * dy + 4 * dz;
+        for (int q = 0; q < 3; ++q) d[q] += e * kd[3 * m + q];
+      }
+  const int mk = mask ? mask[n] : 7;
+  for (int q = 0; q < 3; ++q)
+    dinv[n * 3 + q] = ((mk >> q) & 1) && d[q] > 0.0 ? static_cast<T>(1.0 / d[q]) : T(0);
+}
+
+// One block (576 threads) per coarse element.
+__global__ void __launch_bounds__(576)
+galerkin_fine_k(Geom gc, Geom gf, const double* __restrict__ E, const uint8_t* __restrict__ mf,
+                const uint8_t* __restrict__ mc, const double* __restrict__ S8,
+                const double* __restrict__ Ke, double* __restrict__ Kout) {
+  __shared__ double T[576];
+  __shared__ int s_mask[8];
+  const long long Eid = blockIdx.x;
+  const int I = static_cast<int>(Eid % gc.ex);
+  const int J = static_cast<int>((Eid / gc.ex) % gc.ey);
+  const int K = static_cast<int>(Eid / (static_cast<long long>(gc.ex) * gc.ey));
+  const int q = threadIdx.x;
+  const int r = q / 24, s = q % 24;
+  double acc = 0.0;
+  for (int c = 0; c < 8; ++c) {
+    const int fi = 2 * I + (c & 1), fj = 2 * J + ((c >> 1) & 1), fk = 2 * K + (c >> 2);
+    const double e = E[(static_cast<long long>(fk) * gf.ey + fj) * gf.ex + fi];
+    if (e == 0.0) continue;
+    if (q < 8) {
+      const long long fn = node_of(gf, fi + (q & 1), fj + ((q >> 1) & 1), fk + (q >> 2));
+      s_mask[q] = mf ? mf[fn] : 7;
+    }
+    __syncthreads();
+    bool masked = false;
+    for (int m = 0; m < 8; ++m) masked |= s_mask[m] != 7;
+    if (!masked) {
+      acc += e * S8[c * 576 + q];
+    } else {
+      {
+        const int aa = r, ss = s;  // T[a][s] = sum_b Ke[a][b] M_b W(b, s)
+        double t = 0.0;
+        for (int mb = 0; mb < 8; ++mb) {
+          const int bb = 3 * mb + (ss % 3);
+          if (!((s_mask[mb] >> (ss % 3)) & 1)) continue;
+          t += Ke[aa * 24 + bb] * wc(c, bb, ss);
+        }
+        T[aa * 24 + ss] = t;
+      }
+      __syncthreads();
+      double sv = 0.0;
+      for (int ma = 0; ma < 8; ++ma) {
+        const int aa = 3 * ma + (r % 3);
+        if (!((s_mask[ma] >> (r % 3)) & 1)) continue;
+        sv += wc(c, aa, r) * T[aa * 24 + s];
+      }
+      acc += e * sv;
+    }
+    __syncthreads();
+  }
+  const int mr = r / 3, ms = s / 3;
+  const long long nr = node_of(gc, I + (mr & 1), J + ((mr >> 1) & 1), K + (mr >> 2));
+  const long long ns = node_of(gc, I + (ms & 1), J + ((ms >> 1) & 1), K + (ms >> 2));
+  const int kr = mc ? mc[nr] : 7, ks = mc ? mc[ns] : 7;
+  if (!((kr >> (r % 3)) & 1) || !((ks >> (s % 3)) & 1)) acc = 0.0;
+  Kout[Eid * 576 + q] = acc;
+}
+
+__global__ void __launch_bounds__(576)
+galerkin_coarse_k(Geom gc, Geom gf, const double* __restrict__ Kf,
+                  const uint8_t* __restrict__ mc, double* __restrict__ Kout) {
+  __shared__ double Ks[576];
+  __shared__ double T[576];
+  const long long Eid = blockIdx.x;
+  const int I = static_cast<int>(Eid % gc.ex);
+  const int J = static_cast<int>((Eid / gc.ex) % gc.ey);
+  const int K = static_cast<int>(Eid / (static_cast<long long>(gc.ex) * gc.ey));
+  const int q = threadIdx.x;
+  const int r = q / 24, s = q % 24;
+  double acc = 0.0;
+  for (int c = 0; c < 8; ++c) {
+    const int fi = 2 * I + (c & 1), fj = 2 * J + ((c >> 1) & 1), fk = 2 * K + (c >> 2);
+    const long long fe = (static_cast<long long>(fk) * gf.ey + fj) * gf.ex + fi;
+    Ks[q] = Kf[fe * 576 + q];
+    __syncthreads();
+    {
+      double t = 0.0;  // T[a][s] = sum_b K[a][b] W(b, s), a = r
+      for (int mb = 0; mb < 8; ++mb) {
+        const int bb = 3 * mb + (s % 3);
+        t += Ks[r * 24 + bb] * wc(c, bb, s);
+      }
+      T[r * 24 + s] = t;
+    }
+    __syncthreads();
+    double sv = 0.0;
+    for (int ma = 0; ma < 8; ++ma) {
+      const int aa = 3 * ma + (r % 3);
+      sv += wc(c, aa, r) * T[aa * 24 + s];
+    }
+    acc += sv;
+    __syncthreads();
+  }
+  const int mr = r / 3, ms = s / 3;
+  const long long nr = node_of(gc, I + (mr & 1), J + ((mr >> 1) & 1), K + (mr >> 2));
+  const long long ns = node_of(gc, I + (ms & 1), J + ((ms >> 1) & 1), K + (ms >> 2));
+  const int kr = mc ? mc[nr] : 7, ks = mc ? mc[ns] : 7;
+  if (!((kr >> (r % 3)) & 1) || !((ks >> (s % 3)) & 1)) acc = 0.0;
+  Kout[Eid * 576 + q] = acc;
+}
+
+template <typename T>
+__global__ void diag_dense_k(Geom g, const double* __restrict__ K, const uint8_t* __restrict__ mask,
+                             T* __restrict__ dinv) {
+  const long long n = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (n >= g.nodes()) return;
+  const int a = static_cast<int>(n % g.nnx);
+  const int b = static_cast<int>((n / g.nnx) % g.nny);
+  const int c = static_cast<int>(n / (static_cast<long long>(g.nnx) * g.nny));
+  double d[3] = {0.0, 0.0, 0.0};
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx) {
+        const int i = a - dx, j = b - dy, k = c - dz;
+        if (i < 0 || j < 0 || k < 0 || i >= g.ex || j >= g.ey || k >= g.ez) continue;
+        const long long e = (static_cast<long long>(k) * g.ey + j) * g.ex + i;
+        const int m = dx + 2 * dy + 4 * dz;
+        for (int q = 0; q < 3; ++q) d[q] += K[e * 576 + (3 * m + q) * 24 + 3 * m + q];
+      }
+  const int mk = mask ? mask[n] : 7;
+  for (int q = 0; q < 3; ++q)
+    dinv[n * 3 + q] = ((mk >> q) & 1) && d[q] > 0.0 ? static_cast<T>(1.0 / d[q]) : T(0);
+}
+
+template <typename T>
+__global__ void transpose_k(const double* __restrict__ Kem, long long ne, T* __restrict__ Kapp) {
+  // Kapp[q*ne + e] = Kem[e*576 + q]; one block handles 64 elements x 576 entries via LDS.
+  __shared__ double tile[64][65];
+  const long long e0 = blockIdx.x * 64LL;
+  for (int q0 = 0; q0 < 576; q0 += 64) {
+    for (int t = threadIdx.x; t < 64 * 64; t += blockDim.x) {
+      const int le = t / 64, lq = t % 64;
+      const long long e = e0 + le;
+      tile[le][lq] = e < ne ? Kem[e * 576 + q0 + lq] : 0.0;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 64 * 64; t += blockDim.x) {
+      const int lq = t / 64, le = t % 64;
+      const long long e = e0 + le;
+      if (e < ne) Kapp[static_cast<long long>(q0 + lq) * ne + e] = static_cast<T>(tile[le][lq]);
+    }
+    __syncthreads();
+  }
+}
+
+// Dense assembly: one thread per (node i, neighbour offset o in 27).
+__global__ void assemble_dense_k(Geom g, const double* __restrict__ K, double* __restrict__ A) {
+  const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (t >= g.nodes() * 27) return;
+  const long long ni = t / 27;
+  const int o = static_cast<int>(t % 27);
+  const int ox = o % 3 - 1, oy = (o / 3) % 3 - 1, oz = o / 9 - 1;
+  const int a = static_cast<int>(ni % g.nnx);
+  const int b = static_cast<int>((ni / g.nnx) % g.nny);
+  const int c = static_cast<int>(ni / (static_cast<long long>(g.nnx) * g.nny));
+  const int aj = a + ox, bj = b + oy, cj = c + oz;
+  if (aj < 0 || bj < 0 || cj < 0 || aj >= g.nnx || bj >= g.nny || cj >= g.nnz) return;
+  const long long nj = node_of(g, aj, bj, cj);
+  const long long N = 3 * g.nodes();
+  double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx) {
+        const int i = a - dx, j = b - dy, k = c - dz;  // element containing node i at bits d
+        if (i < 0 || j < 0 || k < 0 || i >= g.ex || j >= g.ey || k >= g.ez) continue;
+        const int bx = aj - i, by = bj - j, bz = cj - k;
+        if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+        const long long e = (static_cast<long long>(k) * g.ey + j) * g.ex + i;
+        const int mi = dx + 2 * dy + 4 * dz, mj = bx + 2 * by + 4 * bz;
+        for (int p = 0; p < 3; ++p)
+          for (int qq = 0; qq < 3; ++qq) blk[p * 3 + qq] += K[e * 576 + (3 * mi + p) * 24 + 3 * mj + qq];
+      }
+  for (int p = 0; p < 3; ++p)
+    for (int qq = 0; qq < 3; ++qq) A[(ni * 3 + p) * N + nj * 3 + qq] = blk[p * 3 + qq];
+}
+
+__global__ void dense_identity_inactive_k(Geom g, const uint8_t* __restrict__ mask, double* A) {
+  const long long n = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (n >= g.nodes()) return;
+  const long long N = 3 * g.nodes();
+  const int mk = mask ? mask[n] : 7;
+  for (int q = 0; q < 3; ++q)
+    if (!((mk >> q) & 1)) A[(n * 3 + q) * N + n * 3 + q] = 1.0;
+}
+
+__global__ void symmetrize_lower_k(double* A, int n) {
+  const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<long long>(n) * n) return;
+  const int i = static_cast<int>(t / n), j = static_cast<int>(t % n);
+  // column-major lower triangle (rocSOLVER) -> full symmetric; A[i + j*n] valid for i >= j
+  if (i < j) A[i + static_cast<long long>(j) * n] = A[j + static_cast<long long>(i) * n];
+}
+
+constexpr int GV_BS = 256;   // rows (i) per block
+constexpr int GV_JC = 64;    // columns (j) per block
+
+template <typename TB>
+__global__ void sym_gemv_partial_k(const double* __restrict__ A, int n, const TB* __restrict__ b,
+                                   double* __restrict__ tmp, const int* done) {
+  if (done && *done) return;
+  __shared__ double bs[GV_JC];
+  const int i = blockIdx.x * GV_BS + threadIdx.x;
+  const int j0 = blockIdx.y * GV_JC;
+  if (threadIdx.x < GV_JC) {
+    const int j = j0 + threadIdx.x;
+    bs[threadIdx.x] = j < n ? static_cast<double>(b[j]) : 0.0;
+  }
+  __syncthreads();
+  if (i >= n) return;
+  double s = 0.0;
+  const int jn = min(GV_JC, n - j0);
+  for (int jj = 0; jj < jn; ++jj) s += A[static_cast<long long>(j0 + jj) * n + i] * bs[jj];
+  tmp[static_cast<long long>(blockIdx.y) * n + i] = s;
+}
+
+template <typename TX>
+__global__ void sym_gemv_reduce_k(const double* __restrict__ tmp, int n, int nj, TX* __restrict__ x,
+                                  const int* done) {
+  if (done && *done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int jb = 0; jb < nj; ++jb) s += tmp[static_cast<long long>(jb) * n + i];
+  x[i] = static_cast<TX>(s);
+}
+
+}  // namespace
+
+template <typename T>
+void launch_diag_fine(const Geom& g, const double* E, const double* kd, const uint8_t* mask, T* dinv,
+                      hipStream_t s) {
+  const long long n = g.nodes();
+  hipLaunchKernelGGL((diag_fine_k<T>), dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                     s, g, E, kd, mask, dinv);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+template void launch_diag_fine<float>(const Geom&, const double*, const double*, const uint8_t*,
+                                      float*, hipStream_t);
+template void launch_diag_fine<double>(const Geom&, const double*, const double*, const uint8_t*,
+                                       double*, hipStream_t);
+
+void launch_galerkin_fine(const Geom& gc, const Geom& gf, const double* E, const uint8_t* mask_f,
+                          const uint8_t* mask_c, const double* S8, const double* Ke, double* Kout,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(galerkin_fine_k, dim3(static_cast<unsigned>(gc.elems())), dim3(576), 0, s, gc,
+                     gf, E, mask_f, mask_c, S8, Ke, Kout);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+
+void launch_galerkin_coarse(const Geom& gc, const Geom& gf, const double* Kf, const uint8_t* mask_c,
+                            double* Kout, hipStream_t s) {
+  hipLaunchKernelGGL(galerkin_coarse_k, dim3(static_cast<unsigned>(gc.elems())), dim3(576), 0, s,
+                     gc, gf, Kf, mask_c, Kout);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_diag_dense(const Geom& g, const double* K, const uint8_t* mask, T* dinv, hipStream_t s) {
+  const long long n = g.nodes();
+  hipLaunchKernelGGL((diag_dense_k<T>), dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                     s, g, K, mask, dinv);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+template void launch_diag_dense<float>(const Geom&, const double*, const uint8_t*, float*,
+                                       hipStream_t);
+template void launch_diag_dense<double>(const Geom&, const double*, const uint8_t*, double*,
+                                        hipStream_t);
+
+template <typename T>
+void launch_transpose_K(const double* Kem, long long ne, T* Kapp, hipStream_t s) {
+  hipLaunchKernelGGL((transpose_k<T>), dim3(static_cast<unsigned>((ne + 63) / 64)), dim3(256), 0,
+                     s, Kem, ne, Kapp);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+template void launch_transpose_K<float>(const double*, long long, float*, hipStream_t);
+template void launch_transpose_K<double>(const double*, long long, double*, hipStream_t);
+
+void launch_assemble_dense(const Geom& g, const double* K, const uint8_t* mask, double* A,
+                           hipStream_t s) {
+  const long long N = 3 * g.nodes();
+  launch_fill<double>(A, 0.0, N * N, s);
+  const long long t = g.nodes() * 27;
+  hipLaunchKernelGGL(assemble_dense_k, dim3(static_cast<unsigned>((t + 255) / 256)), dim3(256), 0,
+                     s, g, K, A);
+  TOPOHIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(dense_identity_inactive_k, dim3(static_cast<unsigned>((g.nodes() + 255) / 256)),
+                     dim3(256), 0, s, g, mask, A);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+
+void launch_symmetrize_lower(double* A, int n, hipStream_t s) {
+  const long long t = static_cast<long long>(n) * n;
+  hipLaunchKernelGGL(symmetrize_lower_k, dim3(static_cast<unsigned>((t + 255) / 256)), dim3(256),
+                     0, s, A, n);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+
+long long gemv_tmp_size(int n) {
+  return static_cast<long long>((n + GV_JC - 1) / GV_JC) * n;
+}
+
+template <typename TB, typename TX>
+void launch_sym_gemv(const double* A, int n, const TB* b, TX* x, double* tmp, const int* done,
+                     hipStream_t s) {
+  const int nj = (n + GV_JC - 1) / GV_JC;
+  hipLaunchKernelGGL((sym_gemv_partial_k<TB>), dim3((n + GV_BS - 1) / GV_BS, nj), dim3(GV_BS), 0, s,
+                     A, n, b, tmp, done);
+  TOPOHIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL((sym_gemv_reduce_k<TX>), dim3((n + 255) / 256), dim3(256), 0, s, tmp, n, nj, x,
+                     done);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+template void launch_sym_gemv<double, double>(const double*, int, const double*, double*, double*,
+                                              const int*, hipStream_t);
+template void launch_sym_gemv<float, float>(const double*, int, const float*, float*, double*,
+                                            const int*, hipStream_t);
+
+}  // namespace topohip

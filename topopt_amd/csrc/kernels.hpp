@@ -1,0 +1,163 @@
+// kernels.hpp — host-side launchers of the engine's HIP kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace topohip {
+
+enum Epilogue { EPI_AX = 0, EPI_RES = 1, EPI_JAC = 2 };
+
+// One tiled operator apply over a structured level. The fine level uses the
+// modal element operator scaled by per-element moduli E; coarse levels use a
+// dense 24x24 matrix per element (Kel, entry-major [576][elems], bit-ordered
+// local DOFs). Epilogues: y = A x, y = b - A x, y = x + omega*dinv*(b - A x).
+// With `partials` set, each block also writes sum(x . A x) over its owned kept
+// DOFs (FP64) to partials[block].
+template <typename T>
+struct KuOp {
+  Geom g;
+  const T* x = nullptr;
+  T* y = nullptr;
+  const T* b = nullptr;
+  const T* dinv = nullptr;
+  T omega = T(0);
+  const uint8_t* mask = nullptr;  // per node: bit c = DOF c kept; null = all kept
+  const T* E = nullptr;           // fine level
+  const T* Kel = nullptr;         // coarse levels
+  Modal<T> mc;
+  int epi = EPI_AX;
+  double* partials = nullptr;
+  int tz = 8;
+  const int* done = nullptr;      // early exit when *done != 0
+};
+
+int ku_blocks(const Geom& g, int tz);
+template <typename T> void launch_ku_fine(const KuOp<T>& op, hipStream_t s);
+template <typename T> void launch_ku_dense(const KuOp<T>& op, hipStream_t s);
+
+// ---- transfers between a fine level f and the coincident coarse level c ----
+// bc = mask_c .* P^T rf; optionally xc = omega * dinv_c .* bc (pre-smooth from 0).
+template <typename Tf, typename Tc>
+void launch_restrict(const Geom& gf, const Geom& gc, const Tf* rf, const uint8_t* mask_c,
+                     Tc* bc, Tc* xc, const Tc* dinv_c, double omega, const int* done,
+                     hipStream_t s);
+// xf += mask_f .* P xc
+template <typename Tc, typename Tf>
+void launch_prolong_add(const Geom& gc, const Geom& gf, const Tc* xc, const uint8_t* mask_f,
+                        Tf* xf, const int* done, hipStream_t s);
+
+// ---- reductions / vector kernels -------------------------------------------
+// out[0] = sum(partials[0..n)) in a fixed order (one block).
+void launch_sum_partials(const double* partials, int n, double* out, hipStream_t s);
+// partial dot products over n entries: partials[block] = sum a[i]*b[i]
+template <typename TA, typename TB>
+void launch_dot(const TA* a, const TB* b, long long n, double* partials, int nblocks,
+                const int* done, hipStream_t s);
+int dot_blocks(long long n);
+// x = omega * dinv .* b (one damped-Jacobi sweep from a zero start).
+template <typename T>
+void launch_jacobi0(T* x, const T* dinv, const T* b, double omega, long long n, const int* done,
+                    hipStream_t s);
+// y = x + up (scatter prescribed values into the solution).
+void launch_add(const double* x, const double* up, double* y, long long n, hipStream_t s);
+template <typename T> void launch_fill(T* p, T v, long long n, hipStream_t s);
+template <typename TI, typename TO> void launch_convert(const TI* in, TO* out, long long n,
+                                                        hipStream_t s);
+
+// PCG pieces (device-resident scalars in CgState).
+// alpha = rz / pAp (pAp = sum of partials); breakdown sets fail/done.
+void launch_cg_alpha(CgState* st, const double* partials, int n, hipStream_t s);
+// x += alpha*p; r -= alpha*Ap; rr partials; r32 = float(r) (nullable)
+template <typename TP, typename TA>
+void launch_cg_xr(CgState* st, double* x, double* r, const TP* p, const TA* Ap, float* r32,
+                  long long n, double* partials, int nblocks, hipStream_t s);
+// rr = sum(partials); iters++; done if rr <= thresh2
+void launch_cg_check(CgState* st, const double* partials, int n, hipStream_t s);
+// rz_new = sum(partials); beta = rz_new / rz; rz = rz_new
+void launch_cg_beta(CgState* st, const double* partials, int n, hipStream_t s);
+// p = z + beta*p (beta = 0 on restart); optional p32 shadow
+template <typename TZ>
+void launch_cg_p(CgState* st, const TZ* z, double* p, float* p32, long long n, int restart,
+                 hipStream_t s);
+// start: bnorm2, thresh2, rz from partials etc.
+void launch_cg_init(CgState* st, const double* bb_partials, int nb, double tol, int cap,
+                    hipStream_t s);
+// rr = sum(partials) of a recomputed residual; done = rr <= thresh2 (resume).
+void launch_cg_resume(CgState* st, const double* partials, int n, hipStream_t s);
+void launch_cg_set_rz(CgState* st, const double* partials, int n, hipStream_t s);
+
+// ---- hierarchy build ---------------------------------------------------------
+// Fine Jacobi inverse diagonal: dinv = 1 / sum_e E_e Ke(l,l) on kept DOFs, else 0.
+template <typename T>
+void launch_diag_fine(const Geom& g, const double* E, const double* kediag, const uint8_t* mask,
+                      T* dinv, hipStream_t s);
+// Level-1 element matrices: K1_E = Mc (sum_c E_c W_c^T (Mf Ke Mf) W_c) Mc.
+void launch_galerkin_fine(const Geom& gc, const Geom& gf, const double* E, const uint8_t* mask_f,
+                          const uint8_t* mask_c, const double* S8, const double* Ke,
+                          double* Kout, hipStream_t s);
+// Deeper element matrices: K_E = Mc (sum_c W_c^T K_c W_c) Mc.
+void launch_galerkin_coarse(const Geom& gc, const Geom& gf, const double* Kf,
+                            const uint8_t* mask_c, double* Kout, hipStream_t s);
+// dinv of an element-form level.
+template <typename T>
+void launch_diag_dense(const Geom& g, const double* K, const uint8_t* mask, T* dinv,
+                       hipStream_t s);
+// Element-major FP64 [elems][576] -> entry-major [576][elems] (apply layout).
+template <typename T>
+void launch_transpose_K(const double* Kem, long long ne, T* Kapp, hipStream_t s);
+// Dense FP64 assembly of an element-form level (3*nodes square, identity on
+// inactive DOFs).
+void launch_assemble_dense(const Geom& g, const double* K, const uint8_t* mask, double* A,
+                           hipStream_t s);
+void launch_symmetrize_lower(double* A, int n, hipStream_t s);
+// x = A b for a dense symmetric n x n FP64 matrix (column-coalesced), with
+// 2-D partials then a fixed-order reduction. tmp: gemv_tmp_size(n) doubles.
+long long gemv_tmp_size(int n);
+template <typename TB, typename TX>
+void launch_sym_gemv(const double* A, int n, const TB* b, TX* x, double* tmp, const int* done,
+                     hipStream_t s);
+
+// ---- SIMP kernels --------------------------------------------------------------
+// E = clamp(xphys, rho_min, 1)^p * E0 on solid voxels (0 elsewhere).
+void launch_youngs(const double* xphys, const uint8_t* solid, long long n, double E0,
+                   double penalty, double rho_min, double* E64, float* E32, hipStream_t s);
+// q_e = u_e^T K1 u_e; c partials = sum E_e q_e; dc_e = -p rho^(p-1) E0 q_e (sum over rhs).
+void launch_sensitivity(const Geom& g, const double* u, const double* xphys, const uint8_t* solid,
+                        const Modal<double>& mc, double E0, double penalty, double rho_min,
+                        double* dc, int accumulate, double* partials, int nblocks, hipStream_t s);
+int sens_blocks(const Geom& g);
+
+struct Stencil {
+  int n = 0;
+  int* off = nullptr;    // device: 3 ints per point (di, dj, dk), lexicographic (dk, dj, di)
+  double* w = nullptr;   // device: weight per point
+};
+// out = H x / Hs (adjoint = 0) or H^T (x / Hs) (adjoint = 1); hs precomputed.
+void launch_filter(const Geom& g, const Stencil& st, const uint8_t* solid, const double* hs,
+                   const double* x, double* out, int adjoint, hipStream_t s);
+void launch_filter_hs(const Geom& g, const Stencil& st, const uint8_t* solid, double* hs,
+                      hipStream_t s);
+
+// OC state: the lambda search runs on the device.
+struct OcState {
+  double l1, l2, lmid, target, vol;
+  int phase;      // 0 = bracketing, 1 = bisecting, 2 = done
+  int steps;      // bisection steps taken
+  int pad[2];
+};
+void launch_oc_init(OcState* st, double target, hipStream_t s);
+// One lambda trial: volume partials of candidate(lambda_trial(st)).
+void launch_oc_trial(const double* x, const double* dcf, const double* dv, const uint8_t* solid,
+                     long long n, double move, double rho_min, const OcState* st,
+                     double* partials, int nblocks, hipStream_t s);
+void launch_oc_decide(OcState* st, const double* partials, int n, hipStream_t s);
+int oc_blocks(long long n);
+// Final candidate at lambda = (l1+l2)/2 + active set + change partials (max).
+void launch_oc_final(const double* x, const double* dcf, const double* dv, const uint8_t* solid,
+                     long long n, double move, double rho_min, const OcState* st, double* xnew,
+                     uint8_t* active, double* change_partials, int nblocks, hipStream_t s);
+void launch_max_partials(const double* partials, int n, double* out, hipStream_t s);
+template <typename T>
+void launch_sum_vec(const T* v, long long n, double* partials, int nblocks, hipStream_t s);
+
+}  // namespace topohip

@@ -1,0 +1,464 @@
+// vec.hip — grid transfers, reductions and the device-resident PCG scalars.
+//
+// Transfers restate the trilinear vertex prolongation of
+// core/src/fea/multigrid.cpp:494-506 (axis_weights) and :586-654 as stencils on
+// the structured node grid: restriction is its transpose (R = P^T), both
+// masked by the level's active-DOF bits so inactive (fixed / void) DOFs carry
+// zero, exactly as P drops inactive rows and columns.
+//
+// Reductions are two-pass and fixed-order (per-block partials, then a single
+// block sums them in index order), so every scalar is bit-identical run to run.
+#include "kernels.hpp"
+
+namespace topohip {
+
+namespace {
+
+constexpr int RB = 256;  // reduction block size
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+
+// Block sum of one value per thread; result valid in thread 0.
+__device__ inline double block_sum(double v) {
+  __shared__ double sh[RB / 64];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < static_cast<int>(blockDim.x) / 64; ++w) s += sh[w];
+  return s;
+}
+
+template <typename Tf, typename Tc>
+__global__ void restrict_k(Geom gf, Geom gc, const Tf* __restrict__ rf,
+                           const uint8_t* __restrict__ mask_c, Tc* __restrict__ bc,
+                           Tc* __restrict__ xc, const Tc* __restrict__ dinv_c, double omega,
+                           const int* done) {
+  if (done && *done) return;
+  const long long nodes = gc.nodes();
+  const long long n = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (n >= nodes) return;
+  const int A = static_cast<int>(n % gc.nnx);
+  const int B = static_cast<int>((n / gc.nnx) % gc.nny);
+  const int C = static_cast<int>(n / (static_cast<long long>(gc.nnx) * gc.nny));
+  const int mk = mask_c ? mask_c[n] : 7;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int c = 2 * C + dz;
+    if (c < 0 || c >= gf.nnz) continue;
+    const double wz = dz == 0 ? 1.0 : 0.5;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int b = 2 * B + dy;
+      if (b < 0 || b >= gf.nny) continue;
+      const double wy = wz * (dy == 0 ? 1.0 : 0.5);
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int a = 2 * A + dx;
+        if (a < 0 || a >= gf.nnx) continue;
+        const double w = wy * (dx == 0 ? 1.0 : 0.5);
+        const long long fn = (static_cast<long long>(c) * gf.nny + b) * gf.nnx + a;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) s[q] += w * static_cast<double>(rf[fn * 3 + q]);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const bool kept = (mk >> q) & 1;
+    const Tc v = kept ? static_cast<Tc>(s[q]) : Tc(0);
+    bc[n * 3 + q] = v;
+    if (xc) xc[n * 3 + q] = kept ? static_cast<Tc>(omega * static_cast<double>(dinv_c[n * 3 + q]) *
+                                                   static_cast<double>(v))
+                                 : Tc(0);
+  }
+}
+
+template <typename Tc, typename Tf>
+__global__ void prolong_add_k(Geom gc, Geom gf, const Tc* __restrict__ xc,
+                              const uint8_t* __restrict__ mask_f, Tf* __restrict__ xf,
+                              const int* done) {
+  if (done && *done) return;
+  const long long nodes = gf.nodes();
+  const long long n = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (n >= nodes) return;
+  const int a = static_cast<int>(n % gf.nnx);
+  const int b = static_cast<int>((n / gf.nnx) % gf.nny);
+  const int c = static_cast<int>(n / (static_cast<long long>(gf.nnx) * gf.nny));
+  const int mk = mask_f ? mask_f[n] : 7;
+  if (mk == 0) return;
+  int ia[2], ib[2], ic[2];
+  double wa[2], wb[2], wc[2];
+  int na, nb, nc;
+  if (a & 1) { ia[0] = (a - 1) / 2; ia[1] = (a + 1) / 2; wa[0] = wa[1] = 0.5; na = 2; }
+  else { ia[0] = a / 2; wa[0] = 1.0; na = 1; }
+  if (b & 1) { ib[0] = (b - 1) / 2; ib[1] = (b + 1) / 2; wb[0] = wb[1] = 0.5; nb = 2; }
+  else { ib[0] = b / 2; wb[0] = 1.0; nb = 1; }
+  if (c & 1) { ic[0] = (c - 1) / 2; ic[1] = (c + 1) / 2; wc[0] = wc[1] = 0.5; nc = 2; }
+  else { ic[0] = c / 2; wc[0] = 1.0; nc = 1; }
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int x = 0; x < na; ++x)
+    for (int y = 0; y < nb; ++y)
+      for (int z = 0; z < nc; ++z) {
+        const double w = wa[x] * wb[y] * wc[z];
+        const long long cn = (static_cast<long long>(ic[z]) * gc.nny + ib[y]) * gc.nnx + ia[x];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) s[q] += w * static_cast<double>(xc[cn * 3 + q]);
+      }
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    if ((mk >> q) & 1) xf[n * 3 + q] = static_cast<Tf>(static_cast<double>(xf[n * 3 + q]) + s[q]);
+}
+
+__global__ void sum_partials_k(const double* __restrict__ p, int n, double* out) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) *out = s;
+}
+
+__global__ void max_partials_k(const double* __restrict__ p, int n, double* out) {
+  __shared__ double sh[RB / 64];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, p[i]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int w = 0; w < RB / 64; ++w) r = fmax(r, sh[w]);
+    *out = r;
+  }
+}
+
+template <typename TA, typename TB>
+__global__ void dot_k(const TA* __restrict__ a, const TB* __restrict__ b, long long n,
+                      double* partials, const int* done) {
+  if (done && *done) return;
+  double s = 0.0;
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    s += static_cast<double>(a[i]) * static_cast<double>(b[i]);
+  s = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ void sum_vec_k(const T* __restrict__ a, long long n, double* partials) {
+  double s = 0.0;
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    s += static_cast<double>(a[i]);
+  s = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ void fill_k(T* p, T v, long long n) {
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+template <typename TI, typename TO>
+__global__ void convert_k(const TI* __restrict__ in, TO* __restrict__ out, long long n) {
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    out[i] = static_cast<TO>(in[i]);
+}
+
+template <typename T>
+__global__ void jacobi0_k(T* __restrict__ x, const T* __restrict__ dinv, const T* __restrict__ b,
+                          double omega, long long n, const int* done) {
+  if (done && *done) return;
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    x[i] = static_cast<T>(omega * static_cast<double>(dinv[i]) * static_cast<double>(b[i]));
+}
+
+__global__ void add_k(const double* __restrict__ x, const double* __restrict__ up,
+                      double* __restrict__ y, long long n) {
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    y[i] = x[i] + up[i];
+}
+
+// ---- PCG scalar kernels (single thread of a single block reads partials) ---
+__device__ inline double fixed_sum(const double* p, int n) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
+  return block_sum(s);
+}
+
+__global__ void cg_init_k(CgState* st, const double* p, int n, double tol, int cap) {
+  const double bb = fixed_sum(p, n);
+  if (threadIdx.x == 0) {
+    st->bnorm2 = bb;
+    const double th = tol * sqrt(bb);
+    st->thresh2 = th * th;
+    st->done = bb > 0.0 ? 0 : 1;
+    st->fail = 0;
+    st->iters = 0;
+    st->cap = cap;
+    st->rr = bb;
+  }
+}
+
+__global__ void cg_set_rz_k(CgState* st, const double* p, int n) {
+  if (st->done) return;
+  const double rz = fixed_sum(p, n);
+  if (threadIdx.x == 0) {
+    st->rz = rz;
+    if (!isfinite(rz)) { st->fail = 1; st->done = 1; }
+  }
+}
+
+__global__ void cg_alpha_k(CgState* st, const double* p, int n) {
+  if (st->done) return;
+  const double pAp = fixed_sum(p, n);
+  if (threadIdx.x == 0) {
+    st->pAp = pAp;
+    if (!(pAp > 0.0) || !isfinite(pAp)) {
+      st->fail = 1;
+      st->done = 1;
+      st->alpha = 0.0;
+    } else {
+      st->alpha = st->rz / pAp;
+    }
+  }
+}
+
+__global__ void cg_check_k(CgState* st, const double* p, int n) {
+  if (st->done) return;
+  const double rr = fixed_sum(p, n);
+  if (threadIdx.x == 0) {
+    st->rr = rr;
+    st->iters += 1;
+    if (!isfinite(rr)) { st->fail = 1; st->done = 1; }
+    else if (rr <= st->thresh2) st->done = 1;
+    else if (st->cap > 0 && st->iters >= st->cap) { st->fail = 2; st->done = 1; }
+  }
+}
+
+__global__ void cg_beta_k(CgState* st, const double* p, int n) {
+  if (st->done) return;
+  const double rzn = fixed_sum(p, n);
+  if (threadIdx.x == 0) {
+    st->rz_new = rzn;
+    if (!isfinite(rzn)) { st->fail = 1; st->done = 1; st->beta = 0.0; return; }
+    st->beta = rzn / st->rz;
+    st->rz = rzn;
+  }
+}
+
+__global__ void cg_resume_k(CgState* st, const double* p, int n) {
+  const double rr = fixed_sum(p, n);
+  if (threadIdx.x == 0) {
+    st->rr = rr;
+    st->done = rr <= st->thresh2 ? 1 : 0;
+    st->fail = 0;
+  }
+}
+
+template <typename TP, typename TA>
+__global__ void cg_xr_k(CgState* st, double* __restrict__ x, double* __restrict__ r,
+                        const TP* __restrict__ p, const TA* __restrict__ Ap, float* r32,
+                        long long n, double* partials) {
+  if (st->done) return;
+  const double alpha = st->alpha;
+  double s = 0.0;
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    x[i] += alpha * static_cast<double>(p[i]);
+    const double ri = r[i] - alpha * static_cast<double>(Ap[i]);
+    r[i] = ri;
+    if (r32) r32[i] = static_cast<float>(ri);
+    s += ri * ri;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <typename TZ>
+__global__ void cg_p_k(CgState* st, const TZ* __restrict__ z, double* __restrict__ p,
+                       float* p32, long long n, int restart) {
+  if (st->done) return;
+  const double beta = restart ? 0.0 : st->beta;
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const double v = static_cast<double>(z[i]) + (restart ? 0.0 : beta * p[i]);
+    p[i] = v;
+    if (p32) p32[i] = static_cast<float>(v);
+  }
+}
+
+int grid_for(long long n, int bs, int cap) {
+  long long b = (n + bs - 1) / bs;
+  if (b > cap) b = cap;
+  if (b < 1) b = 1;
+  return static_cast<int>(b);
+}
+
+}  // namespace
+
+template <typename Tf, typename Tc>
+void launch_restrict(const Geom& gf, const Geom& gc, const Tf* rf, const uint8_t* mask_c, Tc* bc,
+                     Tc* xc, const Tc* dinv_c, double omega, const int* done, hipStream_t s) {
+  const long long n = gc.nodes();
+  hipLaunchKernelGGL((restrict_k<Tf, Tc>), dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256),
+                     0, s, gf, gc, rf, mask_c, bc, xc, dinv_c, omega, done);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+
+template <typename Tc, typename Tf>
+void launch_prolong_add(const Geom& gc, const Geom& gf, const Tc* xc, const uint8_t* mask_f,
+                        Tf* xf, const int* done, hipStream_t s) {
+  const long long n = gf.nodes();
+  hipLaunchKernelGGL((prolong_add_k<Tc, Tf>), dim3(static_cast<unsigned>((n + 255) / 256)),
+                     dim3(256), 0, s, gc, gf, xc, mask_f, xf, done);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+
+template void launch_restrict<float, float>(const Geom&, const Geom&, const float*, const uint8_t*,
+                                            float*, float*, const float*, double, const int*,
+                                            hipStream_t);
+template void launch_restrict<double, double>(const Geom&, const Geom&, const double*,
+                                              const uint8_t*, double*, double*, const double*,
+                                              double, const int*, hipStream_t);
+template void launch_restrict<float, double>(const Geom&, const Geom&, const float*,
+                                             const uint8_t*, double*, double*, const double*,
+                                             double, const int*, hipStream_t);
+template void launch_prolong_add<float, float>(const Geom&, const Geom&, const float*,
+                                               const uint8_t*, float*, const int*, hipStream_t);
+template void launch_prolong_add<double, double>(const Geom&, const Geom&, const double*,
+                                                 const uint8_t*, double*, const int*, hipStream_t);
+template void launch_prolong_add<double, float>(const Geom&, const Geom&, const double*,
+                                                const uint8_t*, float*, const int*, hipStream_t);
+
+void launch_sum_partials(const double* partials, int n, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(sum_partials_k, dim3(1), dim3(RB), 0, s, partials, n, out);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+void launch_max_partials(const double* partials, int n, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(max_partials_k, dim3(1), dim3(RB), 0, s, partials, n, out);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+
+int dot_blocks(long long n) { return grid_for(n, RB * 4, 1024); }
+
+template <typename TA, typename TB>
+void launch_dot(const TA* a, const TB* b, long long n, double* partials, int nblocks,
+                const int* done, hipStream_t s) {
+  hipLaunchKernelGGL((dot_k<TA, TB>), dim3(nblocks), dim3(RB), 0, s, a, b, n, partials, done);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+template void launch_dot<double, double>(const double*, const double*, long long, double*, int,
+                                         const int*, hipStream_t);
+template void launch_dot<double, float>(const double*, const float*, long long, double*, int,
+                                        const int*, hipStream_t);
+template void launch_dot<float, float>(const float*, const float*, long long, double*, int,
+                                       const int*, hipStream_t);
+
+template <typename T>
+void launch_sum_vec(const T* v, long long n, double* partials, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL((sum_vec_k<T>), dim3(nblocks), dim3(RB), 0, s, v, n, partials);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+template void launch_sum_vec<double>(const double*, long long, double*, int, hipStream_t);
+
+template <typename T>
+void launch_fill(T* p, T v, long long n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((fill_k<T>), dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, p, v, n);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+template void launch_fill<float>(float*, float, long long, hipStream_t);
+template void launch_fill<double>(double*, double, long long, hipStream_t);
+template void launch_fill<uint8_t>(uint8_t*, uint8_t, long long, hipStream_t);
+
+template <typename TI, typename TO>
+void launch_convert(const TI* in, TO* out, long long n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((convert_k<TI, TO>), dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, in, out,
+                     n);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+template void launch_convert<double, float>(const double*, float*, long long, hipStream_t);
+template void launch_convert<float, double>(const float*, double*, long long, hipStream_t);
+
+template <typename T>
+void launch_jacobi0(T* x, const T* dinv, const T* b, double omega, long long n, const int* done,
+                    hipStream_t s) {
+  hipLaunchKernelGGL((jacobi0_k<T>), dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, x, dinv, b,
+                     omega, n, done);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+template void launch_jacobi0<float>(float*, const float*, const float*, double, long long,
+                                    const int*, hipStream_t);
+template void launch_jacobi0<double>(double*, const double*, const double*, double, long long,
+                                     const int*, hipStream_t);
+
+void launch_add(const double* x, const double* up, double* y, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(add_k, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, x, up, y, n);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+
+void launch_cg_init(CgState* st, const double* p, int n, double tol, int cap, hipStream_t s) {
+  hipLaunchKernelGGL(cg_init_k, dim3(1), dim3(RB), 0, s, st, p, n, tol, cap);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+void launch_cg_set_rz(CgState* st, const double* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(cg_set_rz_k, dim3(1), dim3(RB), 0, s, st, p, n);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+void launch_cg_resume(CgState* st, const double* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(cg_resume_k, dim3(1), dim3(RB), 0, s, st, p, n);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+void launch_cg_alpha(CgState* st, const double* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(cg_alpha_k, dim3(1), dim3(RB), 0, s, st, p, n);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+void launch_cg_check(CgState* st, const double* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(cg_check_k, dim3(1), dim3(RB), 0, s, st, p, n);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+void launch_cg_beta(CgState* st, const double* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(cg_beta_k, dim3(1), dim3(RB), 0, s, st, p, n);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+
+template <typename TP, typename TA>
+void launch_cg_xr(CgState* st, double* x, double* r, const TP* p, const TA* Ap, float* r32,
+                  long long n, double* partials, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL((cg_xr_k<TP, TA>), dim3(nblocks), dim3(RB), 0, s, st, x, r, p, Ap, r32, n,
+                     partials);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+template void launch_cg_xr<double, double>(CgState*, double*, double*, const double*,
+                                           const double*, float*, long long, double*, int,
+                                           hipStream_t);
+template void launch_cg_xr<float, float>(CgState*, double*, double*, const float*, const float*,
+                                         float*, long long, double*, int, hipStream_t);
+
+template <typename TZ>
+void launch_cg_p(CgState* st, const TZ* z, double* p, float* p32, long long n, int restart,
+                 hipStream_t s) {
+  hipLaunchKernelGGL((cg_p_k<TZ>), dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, st, z, p, p32,
+                     n, restart);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+template void launch_cg_p<float>(CgState*, const float*, double*, float*, long long, int,
+                                 hipStream_t);
+template void launch_cg_p<double>(CgState*, const double*, double*, float*, long long, int,
+                                  hipStream_t);
+
+}  // namespace topohip
