@@ -1,0 +1,163 @@
+"""Benchmark: voxel-SIMP design iterations/s on one MI355X.
+
+Workload (default): MBB half-beam 192x64x64 hex8 (2.45 M DOF), volfrac 0.2,
+rmin 2.0, p = 3, OC move 0.2, MG-PCG tol 1e-6, FP32 K.u + FP32 4-level V-cycle
+(FP64 coarsest) inside FP64-outer PCG. A step is one design iteration
+(filter -> hierarchy rebuild + solve -> sensitivity -> OC update); the timed
+region is one optimisation run of --steps iterations from the uniform design,
+final certification solve included. Inputs are generated on the host before
+the timed region and live in HBM throughout.
+
+Also reported on the same JSON line:
+  fp64_form    the same run in the plain FP64 GPU form (every apply and the
+               whole V-cycle in FP64)
+  ku           the fine K.u apply (FP32 and FP64) timed with HIP events over
+               back-to-back launches on the problem's stream: GDOF/s and HBM
+               fraction of the algorithmic bytes
+  roofline     that FP32 K.u kernel against MI355X's 8 TB/s
+  cpu_baseline the FP64 oracle (a C++ restatement of the reference CPU path,
+               8-colour threaded apply) timed on this host for one design
+               iteration of the same workload
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def ku_bytes(nx, ny, nz, word):
+    ndof = 3 * (nx + 1) * (ny + 1) * (nz + 1)
+    return 2 * word * ndof + word * nx * ny * nz, ndof
+
+
+def run_design(ta, wl, precision, steps, warmup):
+    nx, ny, nz = wl["grid"]
+    with ta.Problem(nx, ny, nz, wl["bcs"], wl["loads"], precision=precision) as p:
+        if warmup > 0:
+            p.optimize(wl["volfrac"], wl["rmin"], warmup, penalty=wl["penalty"], move=wl["move"],
+                       cg_tol=wl["tol"])
+        t0 = time.perf_counter()
+        out = p.optimize(wl["volfrac"], wl["rmin"], steps, penalty=wl["penalty"], move=wl["move"],
+                         cg_tol=wl["tol"])
+        dt = time.perf_counter() - t0
+    return out, dt
+
+
+def ku_measure(ta, nx, ny, nz, reps=100):
+    with ta.Problem(nx, ny, nz) as p:
+        p.bench_prepare(0x5EED)
+        ms32 = p.bench_ku(ta.PREC_MIXED, reps)
+        ms64 = p.bench_ku(ta.PREC_F64, reps)
+    b32, ndof = ku_bytes(nx, ny, nz, 4)
+    b64, _ = ku_bytes(nx, ny, nz, 8)
+    res = {}
+    for name, ms, b in (("f32", ms32, b32), ("f64", ms64, b64)):
+        gbs = b / (ms * 1e-3) / 1e9
+        res[name] = {"ms": ms, "gdof_s": ndof / (ms * 1e-3) / 1e9, "GBs": gbs,
+                     "hbm_frac": gbs / HBM_PEAK_GBS, "bytes": b,
+                     "gflops": 1152.0 * nx * ny * nz / (ms * 1e-3) / 1e9}
+    return res
+
+
+def cpu_baseline(wl, threads):
+    """One design iteration of the workload on the FP64 oracle (bounded sample)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    oracle.set_threads(threads)
+    nx, ny, nz = wl["grid"]
+    s = oracle.Setup(nx, ny, nz, bcs=wl["bcs"], loads=wl["loads"][0])
+    x = np.full(nx * ny * nz, wl["volfrac"])
+    t0 = time.perf_counter()
+    xp = oracle.filter(s, wl["rmin"], x)
+    c, dc, _, info = oracle.simp_compliance(s, xp, penalty=wl["penalty"], tol=wl["tol"])
+    oracle.oc_update(s, wl["rmin"], x, dc, wl["volfrac"], wl["move"])
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "design iters/s", "cores": threads, "kind": "port",
+            "sample": f"1 design iteration (filter + MG-PCG solve incl. hierarchy build + "
+                      f"sensitivity + OC) of {nx}x{ny}x{nz} from the uniform design, FP64 oracle, "
+                      f"{info['iterations']} PCG iterations, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="mbb_192x64x64")
+    ap.add_argument("--no-fp64-form", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this engine has no multi-GPU path (each job and each "
+              "problem runs on a single MI355X); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import topopt_amd as ta
+    from topopt_amd import problems
+
+    wl = problems.workload(a.workload)
+    nx, ny, nz = wl["grid"]
+    print(f"# workload {a.workload}: warmup {a.warmup}, timed {a.steps} design iterations",
+          file=sys.stderr, flush=True)
+    out, dt = run_design(ta, wl, ta.PREC_MIXED, a.steps, a.warmup)
+    value = out["iterations"] / dt
+    print(f"# mixed form: {value:.2f} design iters/s, compliance {out['compliance']:.6g}, "
+          f"cg iters {out['cg_iterations_total']}", file=sys.stderr, flush=True)
+    fp64 = None
+    if not a.no_fp64_form:
+        out64, dt64 = run_design(ta, wl, ta.PREC_F64, a.steps, min(a.warmup, 1))
+        v64 = out64["iterations"] / dt64
+        rel = abs(out["compliance"] - out64["compliance"]) / abs(out64["compliance"])
+        fp64 = {"value": v64, "ratio": value / v64, "compliance": out64["compliance"],
+                "compliance_rel_diff_vs_mixed": rel, "cg_iterations_total": out64["cg_iterations_total"]}
+        print(f"# fp64 form: {v64:.2f} design iters/s", file=sys.stderr, flush=True)
+    ku = ku_measure(ta, nx, ny, nz)
+    print(f"# K.u f32 {ku['f32']['ms']*1e3:.1f} us {ku['f32']['GBs']:.0f} GB/s; "
+          f"f64 {ku['f64']['ms']*1e3:.1f} us", file=sys.stderr, flush=True)
+    cpu = None
+    if not a.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(wl, threads)
+        print(f"# cpu baseline {cpu['value']:.4f} design iters/s on {threads} threads",
+              file=sys.stderr, flush=True)
+    line = {
+        "metric": "design iterations/s (MBB 192x64x64, FP32 K.u + FP32 MG in FP64-outer PCG)",
+        "value": value,
+        "unit": "design iters/s",
+        "n_gpus": 1,
+        "steps": out["iterations"],
+        "warmup": a.warmup,
+        "ms_per_step": dt * 1e3 / max(1, out["iterations"]),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 operator/V-cycle, f64 outer PCG",
+        "data": "synthetic (analytic MBB box, uniform initial density)",
+        "config": {"workload": a.workload, "grid": [nx, ny, nz], "dof": ku["f32"]["bytes"] and
+                   3 * (nx + 1) * (ny + 1) * (nz + 1), "volfrac": wl["volfrac"], "rmin": wl["rmin"],
+                   "penalty": wl["penalty"], "cg_tol": wl["tol"], "mg_levels": 4,
+                   "compliance": out["compliance"], "cg_iterations_total": out["cg_iterations_total"],
+                   "t_solve_ms": out["t_solve_ms"], "t_build_ms": out["t_build_ms"],
+                   "t_oc_ms": out["t_oc_ms"], "t_filter_ms": out["t_filter_ms"]},
+        "fp64_form": fp64,
+        "ku": ku,
+        "roofline": {"bound": "hbm", "achieved": ku["f32"]["GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": ku["f32"]["hbm_frac"], "traffic": None,
+                     "kernel": "ku_tiled<float> fine modal apply",
+                     "bytes_per_launch": ku["f32"]["bytes"], "us_per_launch": ku["f32"]["ms"] * 1e3},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "libtopopt_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["ku.hip", "vec.hip", "galerkin.hip", "simp.hip", "engine.cpp", "capi.cpp", "server.cpp"]
+SOURCES = ["ku.hip", "vec.hip", "galerkin.hip", "simp.hip", "stencil.hip", "coarse.hip", "engine.cpp", "capi.cpp", "server.cpp"]
 FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
          # packed FP32 arithmetic stays off: plain scalar FP32 only
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -42,7 +42,7 @@ def build(verbose=False):
     objs = [o for o, _ in results]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
-           "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+           "-Wl,-rpath,/opt/rocm/lib"]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError("link failed:\n" + p.stdout + p.stderr)

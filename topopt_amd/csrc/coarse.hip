@@ -1,0 +1,530 @@
+// coarse.hip — exact FP64 solve of the coarsest multigrid level by block
+// cyclic reduction over grid planes.
+//
+// The reference factors its coarsest level (<= 6000 DOF) with a sparse LDL^T
+// (core/src/fea/multigrid.cpp:690-694) and solves it exactly in every V-cycle
+// (:798-801). On the device the level is ordered plane by plane along its
+// longest node axis, which makes it block tridiagonal with m x m blocks
+// (m = 3 * nodes per plane):  A = tridiag(L_i, D_i, L_{i+1}^T).
+// Block cyclic reduction eliminates the odd planes, leaving a block
+// tridiagonal Schur complement on the even ones, and recurses:
+//   Dinv_i = D_i^-1,  P_i = Dinv_i L_i,  Q_i = Dinv_i L_{i+1}^T      (i odd)
+//   D'_j = D_j - L_j Q_{j-1} - L_{j+1}^T P_{j+1}
+//   L'_j = -L_j P_{j-1}                                               (j even)
+// and the solve runs
+//   b'_j = b_j - Q_{j-1}^T b_{j-1} - P_{j+1}^T b_{j+1}        (forward)
+//   x_i  = Dinv_i b_i - P_i x_{i-1} - Q_i x_{i+1}             (backward)
+// It is an exact direct solve (block Cholesky in a nested-dissection plane
+// order), with log2(planes) sequential steps, each a batch of independent
+// m x m FP64 inversions / GEMMs / GEMVs.
+#include <algorithm>
+#include <vector>
+
+#include "coarse.hpp"
+
+namespace topohip {
+
+namespace {
+
+constexpr int GJ_T = 16;           // 16 x 16 lanes per inversion (one wave per SIMD, so a
+constexpr int GJ_MAXR = 16;        // lane may use the whole register file); m <= 256
+
+// Batched inversion of SPD m x m blocks with the symmetric sweep operator.
+// Sweeping an index block B maps
+//   A_BB -> Sw = -A_BB^-1,  A_iB -> U_i = A_iB A_BB^-1,
+//   A_ij -> A_ij - U_i A_jB^T                      (i, j outside B),
+// which keeps the matrix symmetric; sweeping every block leaves -A^-1.
+// Pivots go in blocks of 16 (one 16 x 16 sweep in LDS, then a rank-16 update),
+// so an inversion takes m/16 barrier-separated steps. Each lane holds only
+// lower-triangle entries of its 16-strided row/column slots in registers:
+// lane (ti, tj) owns rows ti+16r and columns tj+16c for r > c, and r == c when
+// ti >= tj. offs holds (source, destination) offset pairs per block; the
+// destination receives the full symmetric inverse.
+constexpr int SW_NB = GJ_T;
+__global__ void __launch_bounds__(256) gj_inverse_k(double* W, const long long* offs, int m) {
+  __shared__ double colB[GJ_T * GJ_MAXR][SW_NB + 1];
+  __shared__ double U[GJ_T * GJ_MAXR][SW_NB + 1];
+  __shared__ double Sw[SW_NB][SW_NB + 1];
+  const double* A = W + offs[2 * blockIdx.x];
+  double* Ainv = W + offs[2 * blockIdx.x + 1];
+  const int t = threadIdx.x;
+  const int ti = t / GJ_T, tj = t % GJ_T;
+  const bool diag_ok = ti >= tj;
+  double a[GJ_MAXR][GJ_MAXR];
+#pragma unroll
+  for (int r = 0; r < GJ_MAXR; ++r)
+#pragma unroll
+    for (int c = 0; c <= r; ++c) {
+      const int i = ti + GJ_T * r, j = tj + GJ_T * c;
+      a[r][c] = (i < m && j < m && (r > c || diag_ok)) ? A[static_cast<long long>(i) * m + j] : 0.0;
+    }
+  for (int k0 = 0; k0 < m; k0 += SW_NB) {
+    const int kc = k0 / SW_NB;
+    const int nb = min(SW_NB, m - k0);
+    // (a) column block colB[i][b] = A[i][k0+b] for every row i (symmetric storage);
+    // columns past a short last block are zero so they drop out of (d).
+    if (nb < SW_NB)
+      for (int q = t; q < m * SW_NB; q += 256)
+        if (q % SW_NB >= nb) colB[q / SW_NB][q % SW_NB] = 0.0;
+#pragma unroll
+    for (int r = 0; r < GJ_MAXR; ++r)
+#pragma unroll
+      for (int cc = 0; cc <= r; ++cc) {
+        if (!(r > cc || diag_ok)) continue;
+        const int i = ti + GJ_T * r, j = tj + GJ_T * cc;
+        if (i >= m || j >= m) continue;
+        if (cc == kc && tj < nb) colB[i][tj] = a[r][cc];   // (i, k0+tj), i >= k0+tj
+        if (r == kc && ti < nb) colB[j][ti] = a[r][cc];    // (k0+ti, j) = (j, k0+ti)
+      }
+    __syncthreads();
+    // (b) Sw = sweep of the pivot block (16 single-pivot sweeps in LDS).
+    const int sa = t / SW_NB, sb = t % SW_NB;
+    if (sa < nb && sb < nb) Sw[sa][sb] = colB[k0 + sa][sb];
+    __syncthreads();
+    for (int p = 0; p < nb; ++p) {
+      double nv = 0.0;
+      if (sa < nb && sb < nb) {
+        const double inv = 1.0 / Sw[p][p];
+        const double v = Sw[sa][sb];
+        if (sa == p && sb == p) nv = -inv;
+        else if (sa == p || sb == p) nv = v * inv;
+        else nv = v - Sw[sa][p] * (Sw[p][sb] * inv);
+      }
+      __syncthreads();
+      if (sa < nb && sb < nb) Sw[sa][sb] = nv;
+      __syncthreads();
+    }
+    // (c) U_i = A_iB A_BB^-1 = -colB[i] Sw
+    for (int q = t; q < m * SW_NB; q += 256) {
+      const int i = q / SW_NB, b = q % SW_NB;
+      double s = 0.0;
+      if (b < nb)
+        for (int e = 0; e < nb; ++e) s -= colB[i][e] * Sw[e][b];
+      U[i][b] = s;
+    }
+    __syncthreads();
+    // (d) rank-16 update a_ij -= sum_b U[i][b] colB[j][b] as register-blocked
+    // outer products, then the entries in row/column block B are replaced.
+#pragma unroll 1
+    for (int b = 0; b < SW_NB; ++b) {
+      double ur[GJ_MAXR], cb[GJ_MAXR];
+#pragma unroll
+      for (int r = 0; r < GJ_MAXR; ++r) {
+        const int i = min(ti + GJ_T * r, GJ_T * GJ_MAXR - 1);
+        ur[r] = U[i][b];
+        cb[r] = colB[min(tj + GJ_T * r, GJ_T * GJ_MAXR - 1)][b];
+      }
+#pragma unroll
+      for (int r = 0; r < GJ_MAXR; ++r)
+#pragma unroll
+        for (int cc = 0; cc <= r; ++cc) a[r][cc] -= ur[r] * cb[cc];
+    }
+#pragma unroll
+    for (int r = 0; r < GJ_MAXR; ++r) {
+      const int i = ti + GJ_T * r;
+      const bool ib = r == kc;  // i in B (rows k0..k0+15 are row slot kc)
+#pragma unroll
+      for (int cc = 0; cc <= r; ++cc) {
+        const bool jb = cc == kc;
+        if (!(ib || jb)) continue;
+        const int j = tj + GJ_T * cc;
+        if (i >= m || j >= m || !(r > cc || diag_ok)) continue;
+        if (ib && jb) a[r][cc] = Sw[ti][tj];
+        else if (jb) a[r][cc] = U[i][tj];
+        else a[r][cc] = U[j][ti];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < GJ_MAXR; ++r)
+#pragma unroll
+    for (int c = 0; c <= r; ++c) {
+      const int i = ti + GJ_T * r, j = tj + GJ_T * c;
+      if (i < m && j < m && (r > c || diag_ok)) {
+        Ainv[static_cast<long long>(i) * m + j] = -a[r][c];
+        Ainv[static_cast<long long>(j) * m + i] = -a[r][c];
+      }
+    }
+}
+
+// Batched C = beta*Din + sum_t alpha_t op(A_t) op(B_t), all m x m row-major.
+constexpr int GT = 32;
+__global__ void __launch_bounds__(256) gemm_jobs_k(double* W, const CrGemm* jobs, int m) {
+  __shared__ double As[GT][GT + 1];
+  __shared__ double Bs[GT][GT + 1];
+  const CrGemm& jb = jobs[blockIdx.y];  // read in place: indexing a local copy by term would spill
+  const int tiles = (m + GT - 1) / GT;
+  const int tr = blockIdx.x / tiles, tc = blockIdx.x % tiles;
+  const int lr = threadIdx.x / 16, lc = threadIdx.x % 16;  // 16 x 16 lanes, 2 x 2 outputs each
+  double acc[2][2] = {{0, 0}, {0, 0}};
+  for (int t = 0; t < jb.nterms; ++t) {
+    const double* A = W + jb.a[t];
+    const double* B = W + jb.b[t];
+    double tacc[2][2] = {{0, 0}, {0, 0}};
+    for (int k0 = 0; k0 < m; k0 += GT) {
+      for (int q = threadIdx.x; q < GT * GT; q += 256) {
+        const int r = q / GT, c = q % GT;
+        const int gr = tr * GT + r, gk = k0 + c;  // op(A)[gr][gk]
+        double va = 0.0;
+        if (gr < m && gk < m) va = jb.ta[t] ? A[static_cast<long long>(gk) * m + gr] : A[static_cast<long long>(gr) * m + gk];
+        As[r][c] = va;
+        const int bk = k0 + r, bc = tc * GT + c;  // op(B)[bk][bc]
+        double vb = 0.0;
+        if (bk < m && bc < m) vb = jb.tb[t] ? B[static_cast<long long>(bc) * m + bk] : B[static_cast<long long>(bk) * m + bc];
+        Bs[r][c] = vb;
+      }
+      __syncthreads();
+#pragma unroll 8
+      for (int kk = 0; kk < GT; ++kk) {
+        const double a0 = As[lr * 2][kk], a1 = As[lr * 2 + 1][kk];
+        const double b0 = Bs[kk][lc * 2], b1 = Bs[kk][lc * 2 + 1];
+        tacc[0][0] += a0 * b0; tacc[0][1] += a0 * b1;
+        tacc[1][0] += a1 * b0; tacc[1][1] += a1 * b1;
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y) acc[x][y] += jb.alpha[t] * tacc[x][y];
+  }
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+      const int gr = tr * GT + lr * 2 + x, gc = tc * GT + lc * 2 + y;
+      if (gr < m && gc < m) {
+        const long long o = static_cast<long long>(gr) * m + gc;
+        double v = acc[x][y];
+        if (jb.beta != 0.0) v += jb.beta * W[jb.d + o];
+        W[jb.c + o] = v;
+      }
+    }
+}
+
+// Batched y = beta*y0 + sum_t alpha_t M_t^T v_t (length-m vectors, FP64; every
+// matrix is stored so that the product reads it row by row, coalesced). Each
+// job's k range is split over GV_KS workgroups (partials to tmp), then a
+// second kernel sums the partials in a fixed order.
+constexpr int GV_KS = 32;
+__global__ void __launch_bounds__(256) gemv_partial_k(const double* W, const CrGemv* jobs, int m,
+                                                      double* tmp, const int* done) {
+  if (done && *done) return;
+  const CrGemv& jb = jobs[blockIdx.x];
+  const int ks = blockIdx.y;
+  const int kc = (m + GV_KS - 1) / GV_KS;
+  const int k0 = ks * kc, k1 = min(m, k0 + kc);
+  const int o = threadIdx.x;
+  double s = 0.0;
+  if (o < m) {
+    for (int t = 0; t < jb.nterms; ++t) {
+      const double* M = W + jb.m[t];
+      const double* v = W + jb.v[t];
+      double st = 0.0;
+#pragma unroll 8
+      for (int k = k0; k < k1; ++k) st += M[static_cast<long long>(k) * m + o] * v[k];
+      s += jb.alpha[t] * st;
+    }
+  }
+  tmp[(static_cast<long long>(blockIdx.x) * GV_KS + ks) * 256 + o] = s;
+}
+
+__global__ void __launch_bounds__(256) gemv_reduce_k(double* W, const CrGemv* jobs, int m,
+                                                     const double* tmp, const int* done) {
+  if (done && *done) return;
+  const CrGemv& jb = jobs[blockIdx.x];
+  const int o = threadIdx.x;
+  if (o >= m) return;
+  double y = 0.0;
+  for (int ks = 0; ks < GV_KS; ++ks) y += tmp[(static_cast<long long>(blockIdx.x) * GV_KS + ks) * 256 + o];
+  if (jb.beta != 0.0) y += jb.beta * W[jb.y0 + o];
+  W[jb.out + o] = y;
+}
+
+// Blocks of the coarsest operator in plane order, from the element matrices.
+__global__ void cr_assemble_k(Geom g, const double* __restrict__ K, const uint8_t* __restrict__ mask,
+                              const int* __restrict__ plane_of, const int* __restrict__ local_of,
+                              int m, double* W, long long offD, long long offL) {
+  const long long nodes = g.nodes();
+  const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (t >= nodes * 27) return;
+  const long long ni = t % nodes;
+  const int o = static_cast<int>(t / nodes);
+  const int ox = o % 3 - 1, oy = (o / 3) % 3 - 1, oz = o / 9 - 1;
+  const int a = static_cast<int>(ni % g.nnx);
+  const int b = static_cast<int>((ni / g.nnx) % g.nny);
+  const int c = static_cast<int>(ni / (static_cast<long long>(g.nnx) * g.nny));
+  const int aj = a + ox, bj = b + oy, cj = c + oz;
+  if (aj < 0 || bj < 0 || cj < 0 || aj >= g.nnx || bj >= g.nny || cj >= g.nnz) return;
+  const long long nj = (static_cast<long long>(cj) * g.nny + bj) * g.nnx + aj;
+  const int pi = plane_of[ni], pj = plane_of[nj];
+  long long base;
+  if (pj == pi) base = offD + static_cast<long long>(pi) * m * m;
+  else if (pj == pi - 1) base = offL + static_cast<long long>(pi) * m * m;
+  else return;
+  double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx) {
+        const int i = a - dx, j = b - dy, k = c - dz;
+        if (i < 0 || j < 0 || k < 0 || i >= g.ex || j >= g.ey || k >= g.ez) continue;
+        const int bx = aj - i, by = bj - j, bz = cj - k;
+        if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+        const long long e = (static_cast<long long>(k) * g.ey + j) * g.ex + i;
+        const int mi = dx + 2 * dy + 4 * dz, mj = bx + 2 * by + 4 * bz;
+        for (int p = 0; p < 3; ++p)
+          for (int q = 0; q < 3; ++q) blk[p * 3 + q] += K[e * 576 + (3 * mi + p) * 24 + 3 * mj + q];
+      }
+  const int li = local_of[ni], lj = local_of[nj];
+  const int mki = mask ? mask[ni] : 7;
+  for (int p = 0; p < 3; ++p)
+    for (int q = 0; q < 3; ++q) {
+      double v = blk[p * 3 + q];
+      if (ni == nj && p == q && !((mki >> p) & 1)) v = 1.0;  // identity on inactive DOFs
+      W[base + static_cast<long long>(li + p) * m + lj + q] = v;
+    }
+}
+
+// Node-ordered level vector (T) <-> plane-ordered FP64 vector.
+template <typename T>
+__global__ void cr_gather_k(const T* __restrict__ v, const int* __restrict__ dst, long long n,
+                            double* W, long long off, const int* done) {
+  if (done && *done) return;
+  const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (i < n) W[off + dst[i]] = static_cast<double>(v[i]);
+}
+template <typename T>
+__global__ void cr_scatter_k(const double* W, long long off, const int* __restrict__ dst,
+                             long long n, T* __restrict__ v, const int* done) {
+  if (done && *done) return;
+  const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (i < n) v[i] = static_cast<T>(W[off + dst[i]]);
+}
+
+}  // namespace
+
+bool CoarseCR::plan(const Geom& g) {
+  geom = g;
+  const int dims[3] = {g.nnx, g.nny, g.nnz};
+  axis = 0;
+  for (int d = 1; d < 3; ++d)
+    if (dims[d] > dims[axis]) axis = d;
+  planes = dims[axis];
+  const int u = axis == 0 ? 1 : 0, w = axis == 2 ? 1 : 2;
+  m = 3 * dims[u] * dims[w];
+  if (m > GJ_T * GJ_MAXR) return false;
+  const long long nodes = g.nodes();
+  std::vector<int> plane_of(static_cast<size_t>(nodes)), local_of(static_cast<size_t>(nodes));
+  std::vector<int> dst(static_cast<size_t>(3 * nodes));
+  for (int c = 0; c < g.nnz; ++c)
+    for (int b = 0; b < g.nny; ++b)
+      for (int a = 0; a < g.nnx; ++a) {
+        const int idx[3] = {a, b, c};
+        const long long n = (static_cast<long long>(c) * g.nny + b) * g.nnx + a;
+        plane_of[n] = idx[axis];
+        local_of[n] = 3 * (idx[w] * dims[u] + idx[u]);
+        for (int q = 0; q < 3; ++q)
+          dst[3 * n + q] = plane_of[n] * m + local_of[n] + q;
+      }
+  // Level sizes and workspace layout.
+  lv.clear();
+  long long off = 0;
+  const long long mm = static_cast<long long>(m) * m;
+  int n = planes;
+  while (true) {
+    CrLevel L;
+    L.n = n;
+    L.D = off; off += n * mm;
+    L.L = off; off += n * mm;
+    L.Dinv = off; off += n * mm;
+    L.P = off; off += n * mm;
+    L.Q = off; off += n * mm;
+    L.Pt = off; off += n * mm;
+    L.Qt = off; off += n * mm;
+    L.b = off; off += static_cast<long long>(n) * m;
+    L.x = off; off += static_cast<long long>(n) * m;
+    lv.push_back(L);
+    if (n == 1) break;
+    n = (n + 1) / 2;
+  }
+  wsize = off;
+  // Job tables.
+  std::vector<long long> inv_offs;
+  std::vector<CrGemm> gemm_p, gemm_d;
+  std::vector<CrGemv> fwd, bwd;
+  lv_inv.clear(); lv_gp.clear(); lv_gd.clear(); lv_fw.clear(); lv_bw.clear();
+  for (size_t r = 0; r < lv.size(); ++r) {
+    const CrLevel& L = lv[r];
+    const int nr = L.n;
+    lv_inv.push_back(static_cast<int>(inv_offs.size() / 2));
+    lv_gp.push_back(static_cast<int>(gemm_p.size()));
+    lv_gd.push_back(static_cast<int>(gemm_d.size()));
+    lv_fw.push_back(static_cast<int>(fwd.size()));
+    lv_bw.push_back(static_cast<int>(bwd.size()));
+    if (nr == 1) {
+      inv_offs.push_back(L.D);  // final block: Dinv = D^-1
+      inv_offs.push_back(L.Dinv);
+      CrGemv j{};
+      j.out = L.x; j.nterms = 1; j.m[0] = L.Dinv; j.v[0] = L.b; j.tm[0] = 1; j.alpha[0] = 1.0;
+      bwd.push_back(j);
+      continue;
+    }
+    const CrLevel& N = lv[r + 1];
+    for (int i = 1; i < nr; i += 2) {
+      inv_offs.push_back(L.D + i * mm);
+      inv_offs.push_back(L.Dinv + i * mm);
+    }
+    for (int i = 1; i < nr; i += 2) {
+      CrGemm g1{};  // P_i = Dinv_i L_i
+      g1.c = L.P + i * mm; g1.nterms = 1; g1.a[0] = L.Dinv + i * mm; g1.b[0] = L.L + i * mm;
+      g1.alpha[0] = 1.0;
+      gemm_p.push_back(g1);
+      CrGemm g1t{};  // P_i^T = L_i^T Dinv_i
+      g1t.c = L.Pt + i * mm; g1t.nterms = 1; g1t.a[0] = L.L + i * mm; g1t.ta[0] = 1;
+      g1t.b[0] = L.Dinv + i * mm; g1t.alpha[0] = 1.0;
+      gemm_p.push_back(g1t);
+      if (i + 1 < nr) {
+        CrGemm g2{};  // Q_i = Dinv_i L_{i+1}^T
+        g2.c = L.Q + i * mm; g2.nterms = 1; g2.a[0] = L.Dinv + i * mm;
+        g2.b[0] = L.L + (i + 1) * mm; g2.tb[0] = 1; g2.alpha[0] = 1.0;
+        gemm_p.push_back(g2);
+        CrGemm g2t{};  // Q_i^T = L_{i+1} Dinv_i
+        g2t.c = L.Qt + i * mm; g2t.nterms = 1; g2t.a[0] = L.L + (i + 1) * mm;
+        g2t.b[0] = L.Dinv + i * mm; g2t.alpha[0] = 1.0;
+        gemm_p.push_back(g2t);
+      }
+    }
+    for (int j = 0; j < nr; j += 2) {
+      CrGemm gd{};  // D'_j = D_j - L_j Q_{j-1} - L_{j+1}^T P_{j+1}
+      gd.c = N.D + (j / 2) * mm; gd.d = L.D + j * mm; gd.beta = 1.0;
+      if (j >= 1) {
+        gd.a[gd.nterms] = L.L + j * mm; gd.b[gd.nterms] = L.Q + (j - 1) * mm;
+        gd.alpha[gd.nterms++] = -1.0;
+      }
+      if (j + 1 < nr) {
+        gd.a[gd.nterms] = L.L + (j + 1) * mm; gd.ta[gd.nterms] = 1;
+        gd.b[gd.nterms] = L.P + (j + 1) * mm; gd.alpha[gd.nterms++] = -1.0;
+      }
+      gemm_d.push_back(gd);
+      if (j >= 2) {
+        CrGemm gl{};  // L'_j = -L_j P_{j-1}
+        gl.c = N.L + (j / 2) * mm; gl.nterms = 1; gl.a[0] = L.L + j * mm;
+        gl.b[0] = L.P + (j - 1) * mm; gl.alpha[0] = -1.0;
+        gemm_d.push_back(gl);
+      }
+      CrGemv f{};  // b'_j = b_j - Q_{j-1}^T b_{j-1} - P_{j+1}^T b_{j+1}
+      f.out = N.b + static_cast<long long>(j / 2) * m; f.y0 = L.b + static_cast<long long>(j) * m;
+      f.beta = 1.0;
+      if (j >= 1) {
+        f.m[f.nterms] = L.Q + (j - 1) * mm; f.v[f.nterms] = L.b + static_cast<long long>(j - 1) * m;
+        f.tm[f.nterms] = 1; f.alpha[f.nterms++] = -1.0;
+      }
+      if (j + 1 < nr) {
+        f.m[f.nterms] = L.P + (j + 1) * mm; f.v[f.nterms] = L.b + static_cast<long long>(j + 1) * m;
+        f.tm[f.nterms] = 1; f.alpha[f.nterms++] = -1.0;
+      }
+      fwd.push_back(f);
+      CrGemv c{};  // x_j = x'_{j/2}
+      c.out = L.x + static_cast<long long>(j) * m; c.y0 = N.x + static_cast<long long>(j / 2) * m;
+      c.beta = 1.0;
+      bwd.push_back(c);
+    }
+    for (int i = 1; i < nr; i += 2) {
+      CrGemv bi{};  // x_i = Dinv_i b_i - P_i x_{i-1} - Q_i x_{i+1}; x_{i+-1} read from the coarser level
+      bi.out = L.x + static_cast<long long>(i) * m;
+      bi.m[0] = L.Dinv + i * mm; bi.v[0] = L.b + static_cast<long long>(i) * m; bi.tm[0] = 1;
+      bi.alpha[0] = 1.0; bi.nterms = 1;
+      bi.m[bi.nterms] = L.Pt + i * mm; bi.v[bi.nterms] = N.x + static_cast<long long>((i - 1) / 2) * m;
+      bi.tm[bi.nterms] = 1; bi.alpha[bi.nterms++] = -1.0;
+      if (i + 1 < nr) {
+        bi.m[bi.nterms] = L.Qt + i * mm; bi.v[bi.nterms] = N.x + static_cast<long long>((i + 1) / 2) * m;
+        bi.tm[bi.nterms] = 1; bi.alpha[bi.nterms++] = -1.0;
+      }
+      bwd.push_back(bi);
+    }
+  }
+  lv_inv.push_back(static_cast<int>(inv_offs.size() / 2));
+  lv_gp.push_back(static_cast<int>(gemm_p.size()));
+  lv_gd.push_back(static_cast<int>(gemm_d.size()));
+  lv_fw.push_back(static_cast<int>(fwd.size()));
+  lv_bw.push_back(static_cast<int>(bwd.size()));
+  auto up = [](auto& buf, const auto& host) {
+    buf.ensure(host.size());
+    if (!host.empty())
+      TOPOHIP_CHECK(hipMemcpy(buf.p, host.data(), host.size() * sizeof(host[0]), hipMemcpyHostToDevice));
+  };
+  W.ensure(static_cast<size_t>(wsize));
+  size_t maxjobs = 1;
+  for (size_t r = 0; r + 1 < lv_fw.size(); ++r) {
+    maxjobs = std::max(maxjobs, static_cast<size_t>(lv_fw[r + 1] - lv_fw[r]));
+    maxjobs = std::max(maxjobs, static_cast<size_t>(lv_bw[r + 1] - lv_bw[r]));
+  }
+  tmp.ensure(maxjobs * GV_KS * 256);
+  up(d_inv, inv_offs);
+  up(d_gp, gemm_p);
+  up(d_gd, gemm_d);
+  up(d_fw, fwd);
+  up(d_bw, bwd);
+  up(d_plane, plane_of);
+  up(d_local, local_of);
+  up(d_dst, dst);
+  return true;
+}
+
+void CoarseCR::build(const double* Kem, const uint8_t* mask, hipStream_t s) {
+  const Geom& g = geom;
+  const long long mm = static_cast<long long>(m) * m;
+  launch_fill<double>(W.p + lv[0].D, 0.0, 2LL * planes * mm, s);  // D and L of level 0
+  const long long t = g.nodes() * 27;
+  hipLaunchKernelGGL(cr_assemble_k, dim3(static_cast<unsigned>((t + 255) / 256)), dim3(256), 0, s, g,
+                     Kem, mask, d_plane.p, d_local.p, m, W.p, lv[0].D, lv[0].L);
+  TOPOHIP_CHECK(hipGetLastError());
+  const int tiles = (m + 31) / 32;
+  for (size_t r = 0; r < lv.size(); ++r) {
+    const int ni = lv_inv[r + 1] - lv_inv[r];
+    if (ni > 0) {
+      hipLaunchKernelGGL(gj_inverse_k, dim3(ni), dim3(GJ_T * GJ_T), 0, s, W.p, d_inv.p + 2 * lv_inv[r], m);
+      TOPOHIP_CHECK(hipGetLastError());
+    }
+    const int np = lv_gp[r + 1] - lv_gp[r];
+    if (np > 0) {
+      hipLaunchKernelGGL(gemm_jobs_k, dim3(tiles * tiles, np), dim3(256), 0, s, W.p,
+                         d_gp.p + lv_gp[r], m);
+      TOPOHIP_CHECK(hipGetLastError());
+    }
+    const int nd = lv_gd[r + 1] - lv_gd[r];
+    if (nd > 0) {
+      hipLaunchKernelGGL(gemm_jobs_k, dim3(tiles * tiles, nd), dim3(256), 0, s, W.p,
+                         d_gd.p + lv_gd[r], m);
+      TOPOHIP_CHECK(hipGetLastError());
+    }
+  }
+}
+
+void CoarseCR::gemv(const CrGemv* jobs, int n, const int* done, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(gemv_partial_k, dim3(n, GV_KS), dim3(256), 0, s, W.p, jobs, m, tmp.p, done);
+  hipLaunchKernelGGL(gemv_reduce_k, dim3(n), dim3(256), 0, s, W.p, jobs, m, tmp.p, done);
+}
+
+template <typename T>
+void CoarseCR::solve(const T* b, T* x, const int* done, hipStream_t s) {
+  const long long n = 3 * geom.nodes();
+  const unsigned nb = static_cast<unsigned>((n + 255) / 256);
+  hipLaunchKernelGGL((cr_gather_k<T>), dim3(nb), dim3(256), 0, s, b, d_dst.p, n, W.p, lv[0].b, done);
+  for (size_t r = 0; r + 1 < lv.size(); ++r) {
+    const int nf = lv_fw[r + 1] - lv_fw[r];
+    gemv(d_fw.p + lv_fw[r], nf, done, s);
+  }
+  for (int r = static_cast<int>(lv.size()) - 1; r >= 0; --r) {
+    const int nbw = lv_bw[r + 1] - lv_bw[r];
+    gemv(d_bw.p + lv_bw[r], nbw, done, s);
+  }
+  hipLaunchKernelGGL((cr_scatter_k<T>), dim3(nb), dim3(256), 0, s, W.p, lv[0].x, d_dst.p, n, x, done);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+template void CoarseCR::solve<float>(const float*, float*, const int*, hipStream_t);
+template void CoarseCR::solve<double>(const double*, double*, const int*, hipStream_t);
+
+}  // namespace topohip

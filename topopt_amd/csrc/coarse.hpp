@@ -1,0 +1,63 @@
+// coarse.hpp — block cyclic reduction solver of the coarsest level.
+#pragma once
+
+#include <vector>
+
+#include "engine_buf.hpp"
+#include "kernels.hpp"
+
+namespace topohip {
+
+// C = beta*D + sum_t alpha_t op(A_t) op(B_t) on m x m blocks of a workspace
+// (offsets in doubles).
+struct CrGemm {
+  long long c, d;
+  long long a[2], b[2];
+  int ta[2], tb[2];
+  double alpha[2];
+  double beta;
+  int nterms;
+};
+// out = beta*y0 + sum_t alpha_t op(M_t) v_t on length-m vectors.
+struct CrGemv {
+  long long out, y0;
+  long long m[3], v[3];
+  int tm[3];
+  double alpha[3];
+  double beta;
+  int nterms;
+};
+struct CrLevel {
+  int n = 0;  // planes
+  long long D = 0, L = 0, Dinv = 0, P = 0, Q = 0, Pt = 0, Qt = 0, b = 0, x = 0;
+};
+
+class CoarseCR {
+ public:
+  // Plans the reduction for a level of geometry g (false if a plane is wider
+  // than the register-resident inversion supports).
+  bool plan(const Geom& g);
+  // Factor from the level's element-major FP64 element matrices.
+  void build(const double* Kem, const uint8_t* mask, hipStream_t s);
+  // x = A^-1 b (node-ordered level vectors, FP64 arithmetic inside).
+  template <typename T>
+  void solve(const T* b, T* x, const int* done, hipStream_t s);
+  int block() const { return m; }
+  int nplanes() const { return planes; }
+  size_t bytes() const { return static_cast<size_t>(wsize) * sizeof(double); }
+
+ private:
+  Geom geom;
+  int axis = 0, planes = 0, m = 0;
+  long long wsize = 0;
+  std::vector<CrLevel> lv;
+  std::vector<int> lv_inv, lv_gp, lv_gd, lv_fw, lv_bw;
+  void gemv(const CrGemv* jobs, int n, const int* done, hipStream_t s);
+  DBuf<double> W, tmp;
+  DBuf<long long> d_inv;
+  DBuf<CrGemm> d_gp, d_gd;
+  DBuf<CrGemv> d_fw, d_bw;
+  DBuf<int> d_plane, d_local, d_dst;
+};
+
+}  // namespace topohip

@@ -8,12 +8,13 @@
 // inactive DOFs held at zero, which is the reduced system embedded in the grid.
 #include "engine.hpp"
 
-#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <cstdio>
+#include <cstdlib>
 #include <random>
 
 namespace topohip {
@@ -110,7 +111,8 @@ void hex8_stiffness_host(double E, double nu, double h, double* K) {
 }
 
 Problem::~Problem() {
-  if (rb) rocblas_destroy_handle(rb);
+  for (auto& g : cg_graph)
+    if (g) (void)hipGraphExecDestroy(g);
   if (st_h) (void)hipHostFree(st_h);
   if (oc_h) (void)hipHostFree(oc_h);
   if (scal_h) (void)hipHostFree(scal_h);
@@ -283,8 +285,8 @@ void alloc_levels(Problem& P) {
       lv.Kem.ensure(static_cast<size_t>(lv.g.elems()) * 576);
       const bool coarsest = l + 1 == P.L.size();
       if (!coarsest) {
-        if (mixed) lv.K32.ensure(static_cast<size_t>(lv.g.elems()) * 576);
-        else lv.K64.ensure(static_cast<size_t>(lv.g.elems()) * 576);
+        if (mixed) lv.K32.ensure(static_cast<size_t>(lv.g.nodes()) * 243);
+        else lv.K64.ensure(static_cast<size_t>(lv.g.nodes()) * 243);
       }
       if (mixed) { lv.b32.ensure(nd); lv.x32.ensure(nd); lv.t32.ensure(nd); }
       else { lv.b64.ensure(nd); lv.x64.ensure(nd); lv.t64.ensure(nd); }
@@ -293,12 +295,7 @@ void alloc_levels(Problem& P) {
       else { lv.x64.ensure(nd); lv.t64.ensure(nd); }
     }
   }
-  if (P.mg_plan_ok) {
-    const Level& c = *P.L.back();
-    P.dense_n = static_cast<int>(c.g.ndof());
-    P.Ad.ensure(static_cast<size_t>(P.dense_n) * P.dense_n);
-    P.gtmp.ensure(static_cast<size_t>(gemv_tmp_size(P.dense_n)));
-  }
+  if (P.mg_plan_ok && !P.cr.plan(P.L.back()->g)) P.mg_plan_ok = false;
 }
 
 // ---- V-cycle ----------------------------------------------------------------
@@ -311,32 +308,20 @@ void coarse_cycle(Problem& P, size_t l, const T* rprev, const int* done) {
   const bool coarsest = l + 1 == P.L.size();
   if (coarsest) {
     launch_restrict<T, T>(f.g, c.g, rprev, c.mask.p, Bv<T>(c), nullptr, nullptr, 0.0, done, P.s);
-    launch_sym_gemv<T, T>(P.Ad.p, P.dense_n, Bv<T>(c), Xv<T>(c), P.gtmp.p, done, P.s);
+    P.cr.solve<T>(Bv<T>(c), Xv<T>(c), done, P.s);
     return;
   }
   launch_restrict<T, T>(f.g, c.g, rprev, c.mask.p, Bv<T>(c), Xv<T>(c), Dinv<T>(c), kOmega, done,
                         P.s);
-  KuOp<T> op;
-  op.g = c.g;
-  op.mask = c.mask.p;
-  op.Kel = Kapp<T>(c);
-  op.tz = P.tz;
-  op.done = done;
-  op.x = Xv<T>(c);
-  op.b = Bv<T>(c);
-  op.y = Tv<T>(c);
-  op.epi = EPI_RES;
-  launch_ku_dense<T>(op, P.s);
+  launch_stencil_apply<T>(c.g, Kapp<T>(c), c.mask.p, Xv<T>(c), Tv<T>(c), Bv<T>(c), nullptr, 0.0,
+                          EPI_RES, done, P.s);
   coarse_cycle<T>(P, l + 1, Tv<T>(c), done);
   Level& n = *P.L[l + 1];
   const bool ncoarsest = l + 2 == P.L.size();
   launch_prolong_add<T, T>(n.g, c.g, ncoarsest ? Xv<T>(n) : Tv<T>(n), c.mask.p, Xv<T>(c), done,
                            P.s);
-  op.epi = EPI_JAC;
-  op.dinv = Dinv<T>(c);
-  op.omega = static_cast<T>(kOmega);
-  op.y = Tv<T>(c);
-  launch_ku_dense<T>(op, P.s);
+  launch_stencil_apply<T>(c.g, Kapp<T>(c), c.mask.p, Xv<T>(c), Tv<T>(c), Bv<T>(c), Dinv<T>(c),
+                          kOmega, EPI_JAC, done, P.s);
 }
 
 // z = M^{-1} b0 on the fine level (mf_v_cycle with 1+1 damped Jacobi sweeps;
@@ -352,6 +337,7 @@ void vcycle(Problem& P, const T* b0, T* z, const int* done) {
   op.E = Ev<T>(P);
   op.mc = Mc<T>(P);
   op.tz = P.tz;
+  op.tx = P.tx;
   op.done = done;
   op.x = Xv<T>(f);
   op.b = b0;
@@ -368,7 +354,6 @@ void vcycle(Problem& P, const T* b0, T* z, const int* done) {
   op.omega = static_cast<T>(kOmega);
   op.y = z;
   launch_ku_fine<T>(op, P.s);
-  P.matvecs += 2;
 }
 
 template <typename T>
@@ -380,6 +365,7 @@ void fine_apply(Problem& P, const T* x, T* y, const T* b, int epi, double* parti
   op.E = Ev<T>(P);
   op.mc = Mc<T>(P);
   op.tz = P.tz;
+  op.tx = P.tx;
   op.done = done;
   op.x = x;
   op.y = y;
@@ -387,11 +373,10 @@ void fine_apply(Problem& P, const T* x, T* y, const T* b, int epi, double* parti
   op.epi = epi;
   op.partials = partials;
   launch_ku_fine<T>(op, P.s);
-  P.matvecs += 1;
 }
 
 int max_partials(const Problem& P) {
-  return std::max(ku_blocks(P.g, P.tz), std::max(dot_blocks(P.g.ndof()), sens_blocks(P.g)));
+  return std::max(ku_blocks(P.g, P.tz, P.tx), std::max(dot_blocks(P.g.ndof()), sens_blocks(P.g)));
 }
 
 enum class Prec { MG32, MG64, JACOBI };
@@ -414,6 +399,7 @@ void dot_rz(Problem& P, Prec pc, const int* done) {
 // Restart the recurrence from the current r: z = M^-1 r, rz = r.z, p = z.
 void restart(Problem& P, Prec pc) {
   const long long nd = P.g.ndof();
+  if (pc != Prec::JACOBI) P.matvecs += 2;
   const int* done = &P.st.p->done;
   if (pc == Prec::MG32) launch_convert<double, float>(P.r.p, P.r32.p, nd, P.s);
   precondition(P, pc, done);
@@ -426,7 +412,7 @@ void restart(Problem& P, Prec pc) {
 void iteration(Problem& P, Prec pc) {
   const long long nd = P.g.ndof();
   const int* done = &P.st.p->done;
-  const int kb = ku_blocks(P.g, P.tz);
+  const int kb = ku_blocks(P.g, P.tz, P.tx);
   const int db = dot_blocks(nd);
   if (pc == Prec::MG32) {
     fine_apply<float>(P, P.p32.p, P.Ap32.p, nullptr, EPI_AX, P.partials.p, done);
@@ -452,6 +438,7 @@ void read_state(Problem& P) { d2h(P.st_h, P.st.p, 1, P.s); }
 // rt = b - A x in FP64; returns ||rt||^2.
 double true_residual(Problem& P) {
   const long long nd = P.g.ndof();
+  P.matvecs += 1;
   fine_apply<double>(P, P.x.p, P.rt.p, P.b.p, EPI_RES, nullptr, nullptr);
   launch_dot<double, double>(P.rt.p, P.rt.p, nd, P.partials.p, dot_blocks(nd), nullptr, P.s);
   launch_sum_partials(P.partials.p, dot_blocks(nd), P.scal.p, P.s);
@@ -485,11 +472,25 @@ bool pcg(Problem& P, Prec pc, double tol, int cap, int& iters, double& resid, bo
     return true;
   }
   const int batch = 4;
+  // The iteration body is captured once per preconditioner kind into a HIP
+  // graph (all pointers are fixed for the problem's lifetime) and replayed;
+  // every kernel in it returns at once when the device-side done flag is set.
+  hipGraphExec_t& gx = P.cg_graph[static_cast<int>(pc)];
+  if (!gx) {
+    hipGraph_t gr;
+    TOPOHIP_CHECK(hipStreamBeginCapture(P.s, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < batch; ++i) iteration(P, pc);
+    TOPOHIP_CHECK(hipStreamEndCapture(P.s, &gr));
+    TOPOHIP_CHECK(hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0));
+    TOPOHIP_CHECK(hipGraphDestroy(gr));
+  }
   for (long long guard = 0;; ++guard) {
     if (guard > 4LL * cap + 64) throw Error(TOPOHIP_EINTERNAL, "pcg: iteration state stalled");
     if (!P.st_h->done) {
-      for (int i = 0; i < batch; ++i) iteration(P, pc);
+      const int before = P.st_h->iters;
+      TOPOHIP_CHECK(hipGraphLaunch(gx, P.s));
       read_state(P);
+      P.matvecs += static_cast<long long>(P.st_h->iters - before) * (pc == Prec::JACOBI ? 1 : 3);
       if (!P.st_h->done) continue;
     }
     iters = P.st_h->iters;
@@ -514,6 +515,57 @@ bool pcg(Problem& P, Prec pc, double tol, int cap, int& iters, double& resid, bo
 }
 
 }  // namespace
+
+// Tile autotuner: time the fine-level apply for each (tile width, z-chunk)
+// candidate on the problem's own buffers and keep the fastest. The choice only
+// changes the launch shape; every candidate computes bit-identical results
+// (the per-node summation order does not depend on the tile).
+void autotune_ku(Problem& P) {
+  if (const char* env = std::getenv("TOPOHIP_KU_TILE")) {
+    int tx = 16, tz = 8;
+    if (std::sscanf(env, "%d,%d", &tx, &tz) == 2 && tz > 0) { P.tx = tx; P.tz = tz; }
+    return;
+  }
+  if (P.g.elems() < 4096) return;
+  const bool mixed = P.precision == TOPOHIP_PREC_MIXED;
+  const long long nd = P.g.ndof();
+  launch_fill<double>(P.E64.p, 0.5, P.g.elems(), P.s);
+  launch_convert<double, float>(P.E64.p, P.E32.p, P.g.elems(), P.s);
+  launch_fill<double>(P.x.p, 1.0, nd, P.s);
+  if (mixed) launch_fill<float>(P.p32.p, 1.0f, nd, P.s);
+  hipEvent_t e0, e1;
+  TOPOHIP_CHECK(hipEventCreate(&e0));
+  TOPOHIP_CHECK(hipEventCreate(&e1));
+  float best = 1e30f;
+  const int txs[3] = {16, 32, 64};
+  const int tzs[4] = {2, 4, 8, 16};
+  for (int tx : txs)
+    for (int tz : tzs) {
+      auto run = [&]() {
+        if (mixed) {
+          KuOp<float> op;
+          op.g = P.g; op.E = P.E32.p; op.mc = P.mc32; op.tz = tz; op.tx = tx;
+          op.x = P.p32.p; op.y = P.Ap32.p; op.epi = EPI_AX;
+          launch_ku_fine<float>(op, P.s);
+        } else {
+          KuOp<double> op;
+          op.g = P.g; op.E = P.E64.p; op.mc = P.mc64; op.tz = tz; op.tx = tx;
+          op.x = P.x.p; op.y = P.Ap64.p; op.epi = EPI_AX;
+          launch_ku_fine<double>(op, P.s);
+        }
+      };
+      run();
+      TOPOHIP_CHECK(hipEventRecord(e0, P.s));
+      for (int i = 0; i < 5; ++i) run();
+      TOPOHIP_CHECK(hipEventRecord(e1, P.s));
+      TOPOHIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      TOPOHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) { best = ms; P.tx = tx; P.tz = tz; }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+}
 
 std::unique_ptr<Problem> create_problem(int nx, int ny, int nz, double h, double nu,
                                         const uint8_t* solid, int n_bc, const int32_t* bc_node,
@@ -584,13 +636,36 @@ std::unique_ptr<Problem> create_problem(int nx, int ny, int nz, double h, double
   P->oc.ensure(1);
   plan_levels(*P);
   alloc_levels(*P);
+  autotune_ku(*P);
   P->partials.ensure(static_cast<size_t>(max_partials(*P)) + 8);
-  if (rocblas_create_handle(&P->rb) != rocblas_status_success)
-    throw Error(TOPOHIP_EDEVICE, "rocblas_create_handle failed");
-  rocblas_set_stream(P->rb, P->s);
-  P->dinfo.ensure(1);
   sync(*P);
   return P;
+}
+
+// Void-DOF gate of mf_build_reduced (core/src/fea/matfree.cpp:782-814): every
+// free DOF void -> singular; a load on an untouched free DOF -> no equilibrium.
+void void_gate(const Problem& P, const std::vector<double>& rhs_full) {
+  const long long nn = P.g.nodes();
+  long long nfree = 0, ntouched_free = 0;
+  double rmax = 0.0;
+  for (long long n = 0; n < nn; ++n)
+    for (int c = 0; c < 3; ++c) {
+      if ((P.fixed_h[n] >> c) & 1) continue;
+      ++nfree;
+      if (P.touched_h[n]) ++ntouched_free;
+      rmax = std::max(rmax, std::fabs(rhs_full[3 * n + c]));
+    }
+  if (nfree > 0 && ntouched_free == 0)
+    throw Error(TOPOHIP_EUNDERCONSTRAINED,
+                "fea_solve_mgcg_matfree: singular system (no stiffness — every free DOF is void)");
+  for (long long n = 0; n < nn; ++n) {
+    if (P.touched_h[n]) continue;
+    for (int c = 0; c < 3; ++c)
+      if (!((P.fixed_h[n] >> c) & 1) && std::fabs(rhs_full[3 * n + c]) > 1e-9 * rmax)
+        throw Error(TOPOHIP_EUNDERCONSTRAINED,
+                    "fea_solve_mgcg_matfree: under-constrained system (load applied to a void DOF "
+                    "with no stiffness — no equilibrium possible)");
+  }
 }
 
 void set_loads(Problem& P, int n_rhs, const int32_t* counts, const int32_t* node,
@@ -608,9 +683,21 @@ void set_loads(Problem& P, int n_rhs, const int32_t* counts, const int32_t* node
       P.f_h[r][3 * static_cast<long long>(node[q]) + comp[q]] += val[q];
     }
   }
+  // With zero prescribed values the gate and the masked RHS depend only on the
+  // loads: check and upload them once here.
+  if (!P.nonzero_bc)
+    for (int r = 0; r < n_rhs; ++r) void_gate(P, P.f_h[r]);
   P.nrhs = n_rhs;
   P.f.ensure(static_cast<size_t>(n_rhs * nd));
-  for (int r = 0; r < n_rhs; ++r) h2d(P.f.p + r * nd, P.f_h[r].data(), static_cast<size_t>(nd), P.s);
+  const Level& f0 = *P.L[0];
+  for (int r = 0; r < n_rhs; ++r) {
+    std::vector<double> bm = P.f_h[r];
+    if (!P.nonzero_bc)
+      for (long long n = 0; n < nn; ++n)
+        for (int c = 0; c < 3; ++c)
+          if (!((f0.mask_h[n] >> c) & 1)) bm[3 * n + c] = 0.0;
+    h2d(P.f.p + r * nd, bm.data(), static_cast<size_t>(nd), P.s);
+  }
   sync(P);
 }
 
@@ -635,24 +722,15 @@ void build_hierarchy(Problem& P, Timers& t) {
       if (!coarsest) {
         if (mixed) {
           launch_diag_dense<float>(c.g, c.Kem.p, c.mask.p, c.dinv32.p, P.s);
-          launch_transpose_K<float>(c.Kem.p, c.g.elems(), c.K32.p, P.s);
+          launch_assemble_stencil<float>(c.g, c.Kem.p, c.K32.p, P.s);
         } else {
           launch_diag_dense<double>(c.g, c.Kem.p, c.mask.p, c.dinv64.p, P.s);
-          launch_transpose_K<double>(c.Kem.p, c.g.elems(), c.K64.p, P.s);
+          launch_assemble_stencil<double>(c.g, c.Kem.p, c.K64.p, P.s);
         }
       }
     }
     Level& c = *P.L.back();
-    launch_assemble_dense(c.g, c.Kem.p, c.mask.p, P.Ad.p, P.s);
-    const int n = P.dense_n;
-    if (rocsolver_dpotrf(P.rb, rocblas_fill_lower, n, P.Ad.p, n, P.dinfo.p) != rocblas_status_success)
-      throw Error(TOPOHIP_EDEVICE, "rocsolver_dpotrf failed");
-    int info = 0;
-    d2h(&info, P.dinfo.p, 1, P.s);
-    if (info != 0) throw Error(TOPOHIP_EINTERNAL, "coarsest Galerkin operator is not SPD");
-    if (rocsolver_dpotri(P.rb, rocblas_fill_lower, n, P.Ad.p, n, P.dinfo.p) != rocblas_status_success)
-      throw Error(TOPOHIP_EDEVICE, "rocsolver_dpotri failed");
-    launch_symmetrize_lower(P.Ad.p, n, P.s);
+    P.cr.build(c.Kem.p, c.mask.p, P.s);
   }
   sync(P);
   t.mg_build += now_ms() - t0;
@@ -664,14 +742,16 @@ void solve_rhs(Problem& P, int rhs, double tol, int max_it, const double* d_gues
   const long long nn = P.g.nodes(), nd = 3 * nn;
   if (rhs < 0 || rhs >= P.nrhs) throw Error(TOPOHIP_EINVAL, "no load case set");
   Level& f = *P.L[0];
-  // Void gate (core/src/fea/matfree.cpp:782-814).
-  std::vector<double> rhs_full = P.f_h[rhs];
   if (P.nonzero_bc) {
+    // rhs = f - K(E) up depends on the moduli: form it on the device, check
+    // the void gate, and mask it (core/src/fea/matfree.cpp:745-834).
+    std::vector<double> rhs_full = P.f_h[rhs];
     KuOp<double> op;
     op.g = P.g;
     op.E = P.E64.p;
     op.mc = P.mc64;
     op.tz = P.tz;
+    op.tx = P.tx;
     op.x = P.up.p;
     op.y = P.rt.p;
     op.epi = EPI_AX;
@@ -679,41 +759,15 @@ void solve_rhs(Problem& P, int rhs, double tol, int max_it, const double* d_gues
     std::vector<double> kup(static_cast<size_t>(nd));
     d2h(kup.data(), P.rt.p, static_cast<size_t>(nd), P.s);
     for (long long d = 0; d < nd; ++d) rhs_full[d] -= kup[d];
-  }
-  long long nfree = 0, ntouched_free = 0;
-  double rmax = 0.0;
-  for (long long n = 0; n < nn; ++n)
-    for (int c = 0; c < 3; ++c) {
-      if ((P.fixed_h[n] >> c) & 1) continue;
-      ++nfree;
-      if (P.touched_h[n]) ++ntouched_free;
-      rmax = std::max(rmax, std::fabs(rhs_full[3 * n + c]));
-    }
-  if (nfree > 0 && ntouched_free == 0)
-    throw Error(TOPOHIP_EUNDERCONSTRAINED,
-                "fea_solve_mgcg_matfree: singular system (no stiffness — every free DOF is void)");
-  for (long long n = 0; n < nn; ++n) {
-    if (P.touched_h[n]) continue;
-    for (int c = 0; c < 3; ++c)
-      if (!((P.fixed_h[n] >> c) & 1) && std::fabs(rhs_full[3 * n + c]) > 1e-9 * rmax)
-        throw Error(TOPOHIP_EUNDERCONSTRAINED,
-                    "fea_solve_mgcg_matfree: under-constrained system (load applied to a void DOF "
-                    "with no stiffness — no equilibrium possible)");
-  }
-  // b = mask .* (f - K up); masked by the active bits through a transfer-free
-  // kernel: a residual apply with x = 0 would cost an apply, so mask on host
-  // only when prescribed values are non-zero, else mask on the device.
-  if (P.nonzero_bc) {
+    void_gate(P, rhs_full);
     for (long long n = 0; n < nn; ++n)
       for (int c = 0; c < 3; ++c)
         if (!((f.mask_h[n] >> c) & 1)) rhs_full[3 * n + c] = 0.0;
     h2d(P.b.p, rhs_full.data(), static_cast<size_t>(nd), P.s);
   } else {
-    std::vector<double> bm = P.f_h[rhs];
-    for (long long n = 0; n < nn; ++n)
-      for (int c = 0; c < 3; ++c)
-        if (!((f.mask_h[n] >> c) & 1)) bm[3 * n + c] = 0.0;
-    h2d(P.b.p, bm.data(), static_cast<size_t>(nd), P.s);
+    // Zero prescribed values: the masked RHS was formed and gated once by set_loads.
+    TOPOHIP_CHECK(hipMemcpyAsync(P.b.p, P.f.p + rhs * nd, nd * sizeof(double),
+                                 hipMemcpyDeviceToDevice, P.s));
   }
   long long ng = f.nact;
   const int cap = max_it > 0 ? max_it : static_cast<int>(std::max<long long>(1000, 2 * ng));

@@ -1,43 +1,16 @@
 // engine.hpp — device-resident problem handle of the voxel-SIMP engine.
 #pragma once
 
-#include <rocblas/rocblas.h>
-
 #include <memory>
 #include <string>
 #include <vector>
 
 #include "../../include/topopt_hip.h"
+#include "coarse.hpp"
+#include "engine_buf.hpp"
 #include "kernels.hpp"
 
 namespace topohip {
-
-struct Error : std::runtime_error {
-  int code;
-  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-template <typename T>
-struct DBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  DBuf() = default;
-  DBuf(const DBuf&) = delete;
-  DBuf& operator=(const DBuf&) = delete;
-  ~DBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  void ensure(size_t m) {
-    if (m <= n && p) return;
-    release();
-    TOPOHIP_CHECK(hipMalloc(&p, (m > 0 ? m : 1) * sizeof(T)));
-    n = m;
-  }
-  size_t bytes() const { return n * sizeof(T); }
-};
 
 // One level of the multigrid hierarchy (level 0 = the fine, matrix-free one).
 struct Level {
@@ -69,11 +42,11 @@ struct Timers {
 struct Problem {
   int device = 0;
   hipStream_t s = nullptr;
-  rocblas_handle rb = nullptr;
   Geom g;
   double h = 1.0, nu = 0.3;
   int precision = TOPOHIP_PREC_MIXED;
-  int tz = 8;
+  int tz = 8;   // fine K.u z-chunk (autotuned)
+  int tx = 16;  // fine K.u tile width (autotuned)
 
   bool all_solid = true;
   std::vector<uint8_t> solid_h;
@@ -102,9 +75,7 @@ struct Problem {
 
   std::vector<std::unique_ptr<Level>> L;
   bool mg_plan_ok = false;
-  int dense_n = 0;
-  DBuf<double> Ad, gtmp;
-  DBuf<int> dinfo;
+  CoarseCR cr;  // exact coarsest-level solver
 
   // PCG vectors
   DBuf<double> b, x, r, p, Ap64, z64, rt, u;
@@ -124,8 +95,11 @@ struct Problem {
 
   long long matvecs = 0;
   int replacements = 0;
+  hipGraphExec_t cg_graph[3] = {nullptr, nullptr, nullptr};  // per preconditioner kind
 
   Problem() = default;
+  Problem(const Problem&) = delete;
+  Problem& operator=(const Problem&) = delete;
   ~Problem();
 };
 

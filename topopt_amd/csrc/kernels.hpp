@@ -27,11 +27,12 @@ struct KuOp {
   Modal<T> mc;
   int epi = EPI_AX;
   double* partials = nullptr;
-  int tz = 8;
+  int tz = 8;                     // node planes per workgroup (z-march length)
+  int tx = 16;                    // workgroup tile: tx x (256/tx) element columns
   const int* done = nullptr;      // early exit when *done != 0
 };
 
-int ku_blocks(const Geom& g, int tz);
+int ku_blocks(const Geom& g, int tz, int tx);
 template <typename T> void launch_ku_fine(const KuOp<T>& op, hipStream_t s);
 template <typename T> void launch_ku_dense(const KuOp<T>& op, hipStream_t s);
 
@@ -160,4 +161,17 @@ void launch_max_partials(const double* partials, int n, double* out, hipStream_t
 template <typename T>
 void launch_sum_vec(const T* v, long long n, double* partials, int nblocks, hipStream_t s);
 
+}  // namespace topohip
+
+namespace topohip {
+// ---- assembled coarse levels: 27-point node stencil of 3x3 blocks ----------
+// Coefficient (o, p, q) of node n (neighbour o = (dz+1)*9 + (dy+1)*3 + (dx+1),
+// row component p, column component q) at S[(o*9 + p*3 + q) * nodes + n].
+template <typename T>
+void launch_assemble_stencil(const Geom& g, const double* Kem, T* S, hipStream_t s);
+// y = A x / b - A x / x + omega*dinv*(b - A x) on a stencil level (masked).
+template <typename T>
+void launch_stencil_apply(const Geom& g, const T* S, const uint8_t* mask, const T* x, T* y,
+                          const T* b, const T* dinv, double omega, int epi, const int* done,
+                          hipStream_t s);
 }  // namespace topohip

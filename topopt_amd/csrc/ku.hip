@@ -22,10 +22,7 @@ namespace topohip {
 
 namespace {
 
-constexpr int TX = 16;
-constexpr int TY = 16;
-constexpr int NT = TX * TY;
-constexpr int PLANE = (TX + 1) * (TY + 1) * 3;
+constexpr int NT = 256;
 
 template <typename T>
 __device__ inline void hadamard3(T (&a)[24]) {
@@ -77,8 +74,11 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-template <typename T, bool DENSE, int EPI, bool DOT>
+template <typename T, int TX, bool DENSE, int EPI, bool DOT>
 __global__ void __launch_bounds__(NT) ku_tiled(KuOp<T> a) {
+  constexpr int TY = NT / TX;
+  constexpr int PLANE = (TX + 1) * (TY + 1) * 3;
+  constexpr int NPF = (PLANE + NT - 1) / NT;  // prefetch registers per lane
   if (a.done && *a.done) return;
   __shared__ T s_u[2][PLANE];
   __shared__ T s_c[8][NT * 3];
@@ -95,31 +95,48 @@ __global__ void __launch_bounds__(NT) ku_tiled(KuOp<T> a) {
   const uint8_t* __restrict__ mask = a.mask;
   const T* __restrict__ x = a.x;
 
-  auto load_plane = [&](int buf, int c) {
-    for (int q = t; q < PLANE; q += NT) {
-      const int comp = q % 3;
-      const int n = q / 3;
-      const int lx = n % (TX + 1), ly = n / (TX + 1);
-      const int ga = i0 - 1 + lx, gb = j0 - 1 + ly;
+  // Node plane c is fetched into registers one layer ahead (software
+  // pipelining) and written to LDS at the top of the next layer.
+  T pf[NPF];
+  auto fetch_plane = [&](int c) {
+#pragma unroll
+    for (int i = 0; i < NPF; ++i) {
+      const int q = t + i * NT;
       T v = T(0);
-      if (c >= 0 && c < g.nnz && ga >= 0 && ga < g.nnx && gb >= 0 && gb < g.nny) {
-        const long long node = (static_cast<long long>(c) * g.nny + gb) * g.nnx + ga;
-        const bool kept = !mask || ((mask[node] >> comp) & 1);
-        if (kept) v = x[node * 3 + comp];
+      if (q < PLANE) {
+        const int comp = q % 3;
+        const int n = q / 3;
+        const int lx = n % (TX + 1), ly = n / (TX + 1);
+        const int ga = i0 - 1 + lx, gb = j0 - 1 + ly;
+        if (c >= 0 && c < g.nnz && ga >= 0 && ga < g.nnx && gb >= 0 && gb < g.nny) {
+          const long long node = (static_cast<long long>(c) * g.nny + gb) * g.nnx + ga;
+          const bool kept = !mask || ((mask[node] >> comp) & 1);
+          if (kept) v = x[node * 3 + comp];
+        }
       }
-      s_u[buf][q] = v;
+      pf[i] = v;
+    }
+  };
+  auto store_plane = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < NPF; ++i) {
+      const int q = t + i * NT;
+      if (q < PLANE) s_u[buf][q] = pf[i];
     }
   };
 
   int cur = 0;
-  load_plane(cur, k0 - 1);
+  fetch_plane(k0 - 1);
+  store_plane(cur);
+  fetch_plane(k0);
   T acc[3] = {T(0), T(0), T(0)};
   T xown[3] = {T(0), T(0), T(0)};
   double dotp = 0.0;
   const int kend = k0 + a.tz;
   for (int k = k0 - 1; k < kend; ++k) {
     const int nxt = cur ^ 1;
-    load_plane(nxt, k + 1);
+    store_plane(nxt);  // plane k + 1
+    if (k + 1 < kend) fetch_plane(k + 2);
     __syncthreads();
     if (owner) {
 #pragma unroll
@@ -208,12 +225,13 @@ __global__ void __launch_bounds__(NT) ku_tiled(KuOp<T> a) {
   }
 }
 
-template <typename T, bool DENSE>
-void dispatch(const KuOp<T>& op, hipStream_t s) {
+template <typename T, int TX, bool DENSE>
+void dispatch_tx(const KuOp<T>& op, hipStream_t s) {
+  constexpr int TY = NT / TX;
   const dim3 grid((op.g.nnx + TX - 2) / (TX - 1), (op.g.nny + TY - 2) / (TY - 1),
                   (op.g.nnz + op.tz - 1) / op.tz);
   const bool dot = op.partials != nullptr;
-#define TOPOHIP_L(E, D) hipLaunchKernelGGL((ku_tiled<T, DENSE, E, D>), grid, dim3(NT), 0, s, op)
+#define TOPOHIP_L(E, D) hipLaunchKernelGGL((ku_tiled<T, TX, DENSE, E, D>), grid, dim3(NT), 0, s, op)
   if (op.epi == EPI_AX) {
     if (dot) TOPOHIP_L(EPI_AX, true); else TOPOHIP_L(EPI_AX, false);
   } else if (op.epi == EPI_RES) {
@@ -225,10 +243,19 @@ void dispatch(const KuOp<T>& op, hipStream_t s) {
   TOPOHIP_CHECK(hipGetLastError());
 }
 
+template <typename T, bool DENSE>
+void dispatch(const KuOp<T>& op, hipStream_t s) {
+  if (op.tx == 32) dispatch_tx<T, 32, DENSE>(op, s);
+  else if (op.tx == 64) dispatch_tx<T, 64, DENSE>(op, s);
+  else dispatch_tx<T, 16, DENSE>(op, s);
+}
+
 }  // namespace
 
-int ku_blocks(const Geom& g, int tz) {
-  return ((g.nnx + TX - 2) / (TX - 1)) * ((g.nny + TY - 2) / (TY - 1)) * ((g.nnz + tz - 1) / tz);
+int ku_blocks(const Geom& g, int tz, int tx) {
+  if (tx != 32 && tx != 64) tx = 16;
+  const int ty = NT / tx;
+  return ((g.nnx + tx - 2) / (tx - 1)) * ((g.nny + ty - 2) / (ty - 1)) * ((g.nnz + tz - 1) / tz);
 }
 
 template <typename T>

@@ -1,0 +1,121 @@
+// stencil.hip — assembled coarse-level operators as 27-point node stencils.
+//
+// Below the fine level the Galerkin operators (A_l = P^T A_{l-1} P,
+// core/src/fea/multigrid.cpp:557-716, 1543-1868) are stored the way the
+// structured grid sees them: each node couples to its 27 neighbours through a
+// 3x3 block. The blocks are assembled once per hierarchy build from the
+// per-coarse-element Galerkin matrices (fixed element order, no atomics) and
+// applied by a node-parallel gather whose coefficient loads are coalesced
+// (entry-major layout).
+#include "kernels.hpp"
+
+namespace topohip {
+
+namespace {
+
+template <typename T>
+__global__ void assemble_stencil_k(Geom g, const double* __restrict__ K, T* __restrict__ S) {
+  const long long nodes = g.nodes();
+  const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (t >= nodes * 27) return;
+  const long long ni = t % nodes;
+  const int o = static_cast<int>(t / nodes);
+  const int ox = o % 3 - 1, oy = (o / 3) % 3 - 1, oz = o / 9 - 1;
+  const int a = static_cast<int>(ni % g.nnx);
+  const int b = static_cast<int>((ni / g.nnx) % g.nny);
+  const int c = static_cast<int>(ni / (static_cast<long long>(g.nnx) * g.nny));
+  const int aj = a + ox, bj = b + oy, cj = c + oz;
+  double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (aj >= 0 && bj >= 0 && cj >= 0 && aj < g.nnx && bj < g.nny && cj < g.nnz) {
+    for (int dz = 0; dz < 2; ++dz)
+      for (int dy = 0; dy < 2; ++dy)
+        for (int dx = 0; dx < 2; ++dx) {
+          const int i = a - dx, j = b - dy, k = c - dz;
+          if (i < 0 || j < 0 || k < 0 || i >= g.ex || j >= g.ey || k >= g.ez) continue;
+          const int bx = aj - i, by = bj - j, bz = cj - k;
+          if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+          const long long e = (static_cast<long long>(k) * g.ey + j) * g.ex + i;
+          const int mi = dx + 2 * dy + 4 * dz, mj = bx + 2 * by + 4 * bz;
+          for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q) blk[p * 3 + q] += K[e * 576 + (3 * mi + p) * 24 + 3 * mj + q];
+        }
+  }
+  for (int pq = 0; pq < 9; ++pq) S[(static_cast<long long>(o) * 9 + pq) * nodes + ni] = static_cast<T>(blk[pq]);
+}
+
+template <typename T, int EPI>
+__global__ void __launch_bounds__(256)
+stencil_apply_k(Geom g, const T* __restrict__ S, const uint8_t* __restrict__ mask,
+                const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ b,
+                const T* __restrict__ dinv, T omega, const int* done) {
+  if (done && *done) return;
+  const long long nodes = g.nodes();
+  const long long n = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (n >= nodes) return;
+  const int a = static_cast<int>(n % g.nnx);
+  const int bb = static_cast<int>((n / g.nnx) % g.nny);
+  const int c = static_cast<int>(n / (static_cast<long long>(g.nnx) * g.nny));
+  T acc[3] = {T(0), T(0), T(0)};
+#pragma unroll
+  for (int o = 0; o < 27; ++o) {
+    const int ox = o % 3 - 1, oy = (o / 3) % 3 - 1, oz = o / 9 - 1;
+    const int aj = a + ox, bj = bb + oy, cj = c + oz;
+    if (aj < 0 || bj < 0 || cj < 0 || aj >= g.nnx || bj >= g.nny || cj >= g.nnz) continue;
+    const long long m = (static_cast<long long>(cj) * g.nny + bj) * g.nnx + aj;
+    const int mm = mask ? mask[m] : 7;
+    T xv[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) xv[q] = ((mm >> q) & 1) ? x[m * 3 + q] : T(0);
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) acc[p] += S[(static_cast<long long>(o) * 9 + p * 3 + q) * nodes + n] * xv[q];
+  }
+  const int mk = mask ? mask[n] : 7;
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    T out = T(0);
+    if ((mk >> p) & 1) {
+      if constexpr (EPI == EPI_AX) out = acc[p];
+      else if constexpr (EPI == EPI_RES) out = b[n * 3 + p] - acc[p];
+      else out = x[n * 3 + p] + omega * dinv[n * 3 + p] * (b[n * 3 + p] - acc[p]);
+    }
+    y[n * 3 + p] = out;
+  }
+}
+
+}  // namespace
+
+template <typename T>
+void launch_assemble_stencil(const Geom& g, const double* Kem, T* S, hipStream_t s) {
+  const long long t = g.nodes() * 27;
+  hipLaunchKernelGGL((assemble_stencil_k<T>), dim3(static_cast<unsigned>((t + 255) / 256)), dim3(256),
+                     0, s, g, Kem, S);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+template void launch_assemble_stencil<float>(const Geom&, const double*, float*, hipStream_t);
+template void launch_assemble_stencil<double>(const Geom&, const double*, double*, hipStream_t);
+
+template <typename T>
+void launch_stencil_apply(const Geom& g, const T* S, const uint8_t* mask, const T* x, T* y,
+                          const T* b, const T* dinv, double omega, int epi, const int* done,
+                          hipStream_t s) {
+  const long long n = g.nodes();
+  const dim3 grid(static_cast<unsigned>((n + 255) / 256));
+  const T om = static_cast<T>(omega);
+  if (epi == EPI_AX)
+    hipLaunchKernelGGL((stencil_apply_k<T, EPI_AX>), grid, dim3(256), 0, s, g, S, mask, x, y, b, dinv, om, done);
+  else if (epi == EPI_RES)
+    hipLaunchKernelGGL((stencil_apply_k<T, EPI_RES>), grid, dim3(256), 0, s, g, S, mask, x, y, b, dinv, om, done);
+  else
+    hipLaunchKernelGGL((stencil_apply_k<T, EPI_JAC>), grid, dim3(256), 0, s, g, S, mask, x, y, b, dinv, om, done);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+template void launch_stencil_apply<float>(const Geom&, const float*, const uint8_t*, const float*,
+                                          float*, const float*, const float*, double, int,
+                                          const int*, hipStream_t);
+template void launch_stencil_apply<double>(const Geom&, const double*, const uint8_t*, const double*,
+                                           double*, const double*, const double*, double, int,
+                                           const int*, hipStream_t);
+
+}  // namespace topohip
