@@ -125,9 +125,9 @@ int topohip_problem_create(int nx, int ny, int nz, double h, double poisson, con
   return guarded([&] {
     need(out, "out");
     if (n_bc > 0) { need(bc_node, "bc_node"); need(bc_comp, "bc_comp"); need(bc_val, "bc_val"); }
-    TOPOHIP_CHECK(hipSetDevice(g_device));
     auto h_ = std::make_unique<topohip_problem>();
-    h_->P = create_problem(nx, ny, nz, h, poisson, solid, n_bc, bc_node, bc_comp, bc_val, precision);
+    h_->P = create_problem(nx, ny, nz, h, poisson, solid, n_bc, bc_node, bc_comp, bc_val,
+                           precision, g_device);
     *out = h_.release();
   });
 }

@@ -570,7 +570,7 @@ void autotune_ku(Problem& P) {
 std::unique_ptr<Problem> create_problem(int nx, int ny, int nz, double h, double nu,
                                         const uint8_t* solid, int n_bc, const int32_t* bc_node,
                                         const int32_t* bc_comp, const double* bc_val,
-                                        int precision) {
+                                        int precision, int device) {
   if (nx < 1 || ny < 1 || nz < 1) throw Error(TOPOHIP_EINVAL, "grid dimensions must be >= 1");
   if (precision != TOPOHIP_PREC_MIXED && precision != TOPOHIP_PREC_F64)
     throw Error(TOPOHIP_EINVAL, "unknown precision");
@@ -580,15 +580,13 @@ std::unique_ptr<Problem> create_problem(int nx, int ny, int nz, double h, double
   P->nu = nu;
   P->precision = precision;
   element_constants(*P);  // validates h and nu
-  TOPOHIP_CHECK(hipGetDevice(&P->device));
-  TOPOHIP_CHECK(hipStreamCreateWithFlags(&P->s, hipStreamNonBlocking));
+  // Host-side validation and masks first: no HIP call before the inputs are
+  // known to be good.
   const long long nv = P->g.elems(), nn = P->g.nodes();
   P->all_solid = solid == nullptr;
   if (solid) {
     P->solid_h.assign(solid, solid + nv);
     for (auto& v : P->solid_h) v = v ? 1 : 0;
-    P->solid.ensure(static_cast<size_t>(nv));
-    h2d(P->solid.p, P->solid_h.data(), static_cast<size_t>(nv), P->s);
   }
   P->n_solid = 0;
   P->touched_h.assign(static_cast<size_t>(nn), 0);
@@ -610,6 +608,14 @@ std::unique_ptr<Problem> create_problem(int nx, int ny, int nz, double h, double
     P->fixed_h[bc_node[i]] |= static_cast<uint8_t>(1 << bc_comp[i]);
     P->up_h[3 * static_cast<long long>(bc_node[i]) + bc_comp[i]] = bc_val[i];
     if (bc_val[i] != 0.0) P->nonzero_bc = true;
+  }
+
+  TOPOHIP_CHECK(hipSetDevice(device));
+  P->device = device;
+  TOPOHIP_CHECK(hipStreamCreateWithFlags(&P->s, hipStreamNonBlocking));
+  if (solid) {
+    P->solid.ensure(static_cast<size_t>(nv));
+    h2d(P->solid.p, P->solid_h.data(), static_cast<size_t>(nv), P->s);
   }
   P->up.ensure(static_cast<size_t>(3 * nn));
   h2d(P->up.p, P->up_h.data(), P->up_h.size(), P->s);

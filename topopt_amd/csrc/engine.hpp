@@ -106,7 +106,7 @@ struct Problem {
 std::unique_ptr<Problem> create_problem(int nx, int ny, int nz, double h, double nu,
                                         const uint8_t* solid, int n_bc, const int32_t* bc_node,
                                         const int32_t* bc_comp, const double* bc_val,
-                                        int precision);
+                                        int precision, int device);
 void set_loads(Problem& P, int n_rhs, const int32_t* counts, const int32_t* node,
                const int32_t* comp, const double* val);
 // E64 (device) -> E32, diagonals, Galerkin levels and the coarsest inverse.

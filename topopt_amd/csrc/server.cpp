@@ -86,7 +86,7 @@ struct topohip_server {
       std::vector<double> bv, lv;
       job_bcs(j, bn, bc, bv, ln, lc, lv);
       auto P = create_problem(j.nx, j.ny, j.nz, 1.0, 0.3, nullptr, static_cast<int>(bn.size()),
-                              bn.data(), bc.data(), bv.data(), j.precision);
+                              bn.data(), bc.data(), bv.data(), j.precision, device);
       const int32_t cnt = static_cast<int32_t>(ln.size());
       set_loads(*P, 1, &cnt, ln.data(), lc.data(), lv.data());
       topohip_options o{};
