@@ -6,9 +6,29 @@
 // the bench's cpu_baseline leg do.
 //
 // Every function names the reference file:line it restates. Paths are relative
-// to the reference checkout (core/...). Parity status of each part (which
-// golden vector / closed form / invariant pins it) is listed in the test files
-// under tests/test_oracle_*.py.
+// to the reference checkout (core/...). What pins each part (tests in
+// tests/test_oracle.py unless named otherwise):
+//   hex8_stiffness      reference fixture hex_stiffness_reference.json (nu 0.30
+//                       and 0.40, all 576 entries); symmetry, 6 rigid modes
+//   apply_full          explicitly assembled K on small grids; K.t = 0,
+//                       symmetry, positivity, linearity in E; void voxels
+//   build_reduced       void-gate cases of core/tests/test_matfree.cpp:351-441
+//   mf_mgpcg / solve_*  dense direct solve of the reduced system (MG and
+//                       Jacobi-CG routes); uniform-strain patch test;
+//                       mesh-independent MG iteration counts
+//   build_hierarchy     level count and DOF sizes of the reference rule
+//   simp_compliance     c = f.u; central differences of c in x
+//   DensityFilter       O(N^2) brute force of the cone weights; stencil sizes
+//   oc_update           brute-force bisection re-run, bounds, move limit,
+//                       volume target
+//   penalty_at          continuation-ramp values; heaviside: the reference's
+//                       projection_unit_reference spot values (1e-12)
+//   simp_optimize       structural gates (monotone compliance, volume,
+//                       xPhys = filter(design))
+// Parity unpinned (no reference-held vector exists and the reference cannot
+// be built here without Eigen): the reference's exact MG-PCG iteration counts
+// and CgInfo traces, the SimplicialLDLT rounding of its coarsest solve, and its
+// per-iteration compliance trajectory at the benchmark sizes.
 #pragma once
 
 #include <array>

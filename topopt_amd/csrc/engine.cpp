@@ -283,6 +283,29 @@ void alloc_levels(Problem& P) {
     if (mixed) lv.dinv32.ensure(static_cast<size_t>(nd));
     if (l > 0) {
       lv.Kem.ensure(static_cast<size_t>(lv.g.elems()) * 576);
+      if (l == 1) {
+        // Masks are fixed per problem (BCs and solid tags), so the coarse
+        // elements that need the masked Galerkin path are listed once.
+        const Level& f = *P.L[0];
+        std::vector<int> sp;
+        for (int K = 0; K < lv.g.ez; ++K)
+          for (int J = 0; J < lv.g.ey; ++J)
+            for (int I = 0; I < lv.g.ex; ++I) {
+              bool any = false;
+              for (int c = 0; c < 3 && !any; ++c)
+                for (int b = 0; b < 3 && !any; ++b)
+                  for (int a = 0; a < 3 && !any; ++a)
+                    any = f.mask_h[(static_cast<long long>(2 * K + c) * f.g.nny + 2 * J + b) *
+                                       f.g.nnx + 2 * I + a] != 7;
+              if (any) sp.push_back((K * lv.g.ey + J) * lv.g.ex + I);
+            }
+        lv.n_special = static_cast<int>(sp.size());
+        if (!sp.empty()) {
+          lv.special.ensure(sp.size());
+          h2d(lv.special.p, sp.data(), sp.size(), P.s);
+          sync(P);
+        }
+      }
       const bool coarsest = l + 1 == P.L.size();
       if (!coarsest) {
         if (mixed) lv.K32.ensure(static_cast<size_t>(lv.g.nodes()) * 243);
@@ -720,8 +743,8 @@ void build_hierarchy(Problem& P, Timers& t) {
       Level& c = *P.L[l];
       Level& pf = *P.L[l - 1];
       if (l == 1)
-        launch_galerkin_fine(c.g, pf.g, P.E64.p, pf.mask.p, c.mask.p, P.dS8.p, P.dKb.p, c.Kem.p,
-                             P.s);
+        launch_galerkin_fine(c.g, pf.g, P.E64.p, pf.mask.p, c.mask.p, P.dS8.p, P.dKb.p,
+                             c.special.p, c.n_special, c.Kem.p, P.s);
       else
         launch_galerkin_coarse(c.g, pf.g, pf.Kem.p, c.mask.p, c.Kem.p, P.s);
       const bool coarsest = l + 1 == P.L.size();

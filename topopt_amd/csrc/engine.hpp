@@ -19,6 +19,8 @@ struct Level {
   DBuf<uint8_t> mask;
   long long nact = 0;           // active DOFs
   DBuf<double> Kem;             // element-major FP64 matrices (levels >= 1)
+  DBuf<int> special;            // level 1: coarse elements with an inactive fine DOF
+  int n_special = 0;
   DBuf<float> K32;              // entry-major apply copies
   DBuf<double> K64;
   DBuf<float> dinv32;

@@ -64,14 +64,54 @@ __global__ void diag_fine_k(Geom g, const double* __restrict__ E, const double* 
     dinv[n * 3 + q] = ((mk >> q) & 1) && d[q] > 0.0 ? static_cast<T>(1.0 / d[q]) : T(0);
 }
 
-// One block (576 threads) per coarse element.
+// Coarse elements none of whose 27 fine nodes has an inactive DOF: every
+// child uses its precomputed S_c, so K1_E[q] = sum_c E_c S_c[q] (c ascending,
+// void children skipped) needs no LDS and no barrier. One thread per entry q,
+// GF_EPB coarse elements per block; the S_c column stays in registers.
+constexpr int GF_EPB = 8;
+__global__ void __launch_bounds__(576)
+galerkin_fine_fast_k(Geom gc, Geom gf, const double* __restrict__ E,
+                     const uint8_t* __restrict__ mc, const double* __restrict__ S8,
+                     double* __restrict__ Kout) {
+  const int q = threadIdx.x;
+  double sc[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) sc[c] = S8[c * 576 + q];
+  const int r = q / 24, s = q % 24;
+  const int mr = r / 3, ms = s / 3;
+  const long long ne = gc.elems();
+  for (int i = 0; i < GF_EPB; ++i) {
+    const long long Eid = static_cast<long long>(blockIdx.x) * GF_EPB + i;
+    if (Eid >= ne) return;
+    const int I = static_cast<int>(Eid % gc.ex);
+    const int J = static_cast<int>((Eid / gc.ex) % gc.ey);
+    const int K = static_cast<int>(Eid / (static_cast<long long>(gc.ex) * gc.ey));
+    double acc = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int fi = 2 * I + (c & 1), fj = 2 * J + ((c >> 1) & 1), fk = 2 * K + (c >> 2);
+      const double e = E[(static_cast<long long>(fk) * gf.ey + fj) * gf.ex + fi];
+      if (e != 0.0) acc += e * sc[c];
+    }
+    const long long nr = node_of(gc, I + (mr & 1), J + ((mr >> 1) & 1), K + (mr >> 2));
+    const long long ns = node_of(gc, I + (ms & 1), J + ((ms >> 1) & 1), K + (ms >> 2));
+    const int kr = mc ? mc[nr] : 7, ks = mc ? mc[ns] : 7;
+    if (!((kr >> (r % 3)) & 1) || !((ks >> (s % 3)) & 1)) acc = 0.0;
+    Kout[Eid * 576 + q] = acc;
+  }
+}
+
+// One block (576 threads) per coarse element of `list` (or per coarse element
+// when list is null): children touching an inactive fine DOF form
+// W_c^T (Mf Ke Mf) W_c explicitly.
 __global__ void __launch_bounds__(576)
 galerkin_fine_k(Geom gc, Geom gf, const double* __restrict__ E, const uint8_t* __restrict__ mf,
                 const uint8_t* __restrict__ mc, const double* __restrict__ S8,
-                const double* __restrict__ Ke, double* __restrict__ Kout) {
+                const double* __restrict__ Ke, const int* __restrict__ list,
+                double* __restrict__ Kout) {
   __shared__ double T[576];
   __shared__ int s_mask[8];
-  const long long Eid = blockIdx.x;
+  const long long Eid = list ? list[blockIdx.x] : blockIdx.x;
   const int I = static_cast<int>(Eid % gc.ex);
   const int J = static_cast<int>((Eid / gc.ex) % gc.ey);
   const int K = static_cast<int>(Eid / (static_cast<long long>(gc.ex) * gc.ey));
@@ -121,39 +161,48 @@ galerkin_fine_k(Geom gc, Geom gf, const double* __restrict__ E, const uint8_t* _
   Kout[Eid * 576 + q] = acc;
 }
 
+// One block (576 threads) per coarse element; the 8 child blocks are staged
+// in LDS together so a coarse element costs three barriers.
 __global__ void __launch_bounds__(576)
 galerkin_coarse_k(Geom gc, Geom gf, const double* __restrict__ Kf,
                   const uint8_t* __restrict__ mc, double* __restrict__ Kout) {
-  __shared__ double Ks[576];
-  __shared__ double T[576];
+  __shared__ double Ks[8][576];
   const long long Eid = blockIdx.x;
   const int I = static_cast<int>(Eid % gc.ex);
   const int J = static_cast<int>((Eid / gc.ex) % gc.ey);
   const int K = static_cast<int>(Eid / (static_cast<long long>(gc.ex) * gc.ey));
   const int q = threadIdx.x;
   const int r = q / 24, s = q % 24;
-  double acc = 0.0;
+#pragma unroll
   for (int c = 0; c < 8; ++c) {
     const int fi = 2 * I + (c & 1), fj = 2 * J + ((c >> 1) & 1), fk = 2 * K + (c >> 2);
     const long long fe = (static_cast<long long>(fk) * gf.ey + fj) * gf.ex + fi;
-    Ks[q] = Kf[fe * 576 + q];
-    __syncthreads();
-    {
-      double t = 0.0;  // T[a][s] = sum_b K[a][b] W(b, s), a = r
-      for (int mb = 0; mb < 8; ++mb) {
-        const int bb = 3 * mb + (s % 3);
-        t += Ks[r * 24 + bb] * wc(c, bb, s);
-      }
-      T[r * 24 + s] = t;
+    Ks[c][q] = Kf[fe * 576 + q];
+  }
+  __syncthreads();
+  double t[8];  // T_c[r][s] = sum_b K_c[r][b] W_c(b, s)
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    double v = 0.0;
+    for (int mb = 0; mb < 8; ++mb) {
+      const int bb = 3 * mb + (s % 3);
+      v += Ks[c][r * 24 + bb] * wc(c, bb, s);
     }
-    __syncthreads();
+    t[c] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 8; ++c) Ks[c][q] = t[c];
+  __syncthreads();
+  double acc = 0.0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
     double sv = 0.0;
     for (int ma = 0; ma < 8; ++ma) {
       const int aa = 3 * ma + (r % 3);
-      sv += wc(c, aa, r) * T[aa * 24 + s];
+      sv += wc(c, aa, r) * Ks[c][aa * 24 + s];
     }
     acc += sv;
-    __syncthreads();
   }
   const int mr = r / 3, ms = s / 3;
   const long long nr = node_of(gc, I + (mr & 1), J + ((mr >> 1) & 1), K + (mr >> 2));
@@ -304,11 +353,19 @@ template void launch_diag_fine<double>(const Geom&, const double*, const double*
                                        double*, hipStream_t);
 
 void launch_galerkin_fine(const Geom& gc, const Geom& gf, const double* E, const uint8_t* mask_f,
-                          const uint8_t* mask_c, const double* S8, const double* Ke, double* Kout,
-                          hipStream_t s) {
-  hipLaunchKernelGGL(galerkin_fine_k, dim3(static_cast<unsigned>(gc.elems())), dim3(576), 0, s, gc,
-                     gf, E, mask_f, mask_c, S8, Ke, Kout);
+                          const uint8_t* mask_c, const double* S8, const double* Ke,
+                          const int* special, int n_special, double* Kout, hipStream_t s) {
+  const long long ne = gc.elems();
+  hipLaunchKernelGGL(galerkin_fine_fast_k, dim3(static_cast<unsigned>((ne + GF_EPB - 1) / GF_EPB)),
+                     dim3(576), 0, s, gc, gf, E, mask_c, S8, Kout);
   TOPOHIP_CHECK(hipGetLastError());
+  // Elements with an inactive fine DOF are recomputed (after the fast pass on
+  // the same stream) by the masked path.
+  if (n_special > 0) {
+    hipLaunchKernelGGL(galerkin_fine_k, dim3(static_cast<unsigned>(n_special)), dim3(576), 0, s,
+                       gc, gf, E, mask_f, mask_c, S8, Ke, special, Kout);
+    TOPOHIP_CHECK(hipGetLastError());
+  }
 }
 
 void launch_galerkin_coarse(const Geom& gc, const Geom& gf, const double* Kf, const uint8_t* mask_c,

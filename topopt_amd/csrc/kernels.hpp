@@ -93,9 +93,11 @@ template <typename T>
 void launch_diag_fine(const Geom& g, const double* E, const double* kediag, const uint8_t* mask,
                       T* dinv, hipStream_t s);
 // Level-1 element matrices: K1_E = Mc (sum_c E_c W_c^T (Mf Ke Mf) W_c) Mc.
+// Level-1 Galerkin blocks; `special` lists the n_special coarse elements with
+// an inactive DOF among their 27 fine nodes (they take the masked path).
 void launch_galerkin_fine(const Geom& gc, const Geom& gf, const double* E, const uint8_t* mask_f,
                           const uint8_t* mask_c, const double* S8, const double* Ke,
-                          double* Kout, hipStream_t s);
+                          const int* special, int n_special, double* Kout, hipStream_t s);
 // Deeper element matrices: K_E = Mc (sum_c W_c^T K_c W_c) Mc.
 void launch_galerkin_coarse(const Geom& gc, const Geom& gf, const double* Kf,
                             const uint8_t* mask_c, double* Kout, hipStream_t s);
