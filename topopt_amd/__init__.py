@@ -294,6 +294,70 @@ class Problem:
         return ms.value
 
 
+class Server:
+    """Persistent continuous-batching job loop (topohip_server_*).
+
+    Jobs are dicts (or Job structs) with nx, ny, nz, bc_kind (0 cantilever,
+    1 MBB), volfrac, rmin, penalty, move, cg_tol, iterations, precision.
+    """
+
+    def __init__(self, budget_bytes=64 << 30, max_slots=8):
+        self._s = C.c_void_p()
+        code = lib().topohip_server_create(C.c_size_t(budget_bytes), int(max_slots),
+                                           C.byref(self._s))
+        if code != OK:
+            raise ValueError(f"topohip_server_create failed ({code})")
+
+    @staticmethod
+    def make_job(nx, ny, nz, bc_kind=0, volfrac=0.3, rmin=1.5, penalty=3.0, move=0.2,
+                 cg_tol=1e-6, iterations=60, precision=PREC_MIXED):
+        return Job(int(nx), int(ny), int(nz), int(bc_kind), volfrac, rmin, penalty, move, cg_tol,
+                   int(iterations), int(precision))
+
+    @staticmethod
+    def footprint(job):
+        return lib().topohip_job_footprint(C.byref(job))
+
+    def submit(self, job):
+        if isinstance(job, dict):
+            job = self.make_job(**job)
+        jid = C.c_int64()
+        code = lib().topohip_server_submit(self._s, C.byref(job), C.byref(jid))
+        if code != OK:
+            raise ValueError(f"topohip_server_submit rejected the job ({code})")
+        return jid.value
+
+    def poll(self, jid):
+        r = JobResult()
+        _check(lib().topohip_server_poll(self._s, C.c_int64(jid), C.byref(r)))
+        return r.as_dict()
+
+    def drain(self):
+        _check(lib().topohip_server_drain(self._s))
+
+    def density(self, jid, nvox):
+        out = np.zeros(int(nvox))
+        _check(lib().topohip_server_fetch_density(self._s, C.c_int64(jid), _p(out)))
+        return out
+
+    def close(self):
+        if getattr(self, "_s", None) is not None and self._s.value:
+            lib().topohip_server_destroy(self._s)
+            self._s = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # -- reference-named free functions ------------------------------------------
 
 def fea_matfree_apply(nx, ny, nz, youngs_per_voxel, u, poisson=0.3, h=1.0, solid=None,

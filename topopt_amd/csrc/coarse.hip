@@ -26,126 +26,106 @@ namespace topohip {
 
 namespace {
 
-constexpr int GJ_T = 16;           // 16 x 16 lanes per inversion (one wave per SIMD, so a
-constexpr int GJ_MAXR = 16;        // lane may use the whole register file); m <= 256
+constexpr int IV_MP = 256;  // blocks are padded to 256 x 256 (m <= 256)
+constexpr int IV_NB = 32;   // pivots per step = output tile edge
+static_assert(IV_NB % 2 == 0, "the ping-pong pivot sweep must end in Sw");
 
-// Batched inversion of SPD m x m blocks with the symmetric sweep operator.
-// Sweeping an index block B maps
-//   A_BB -> Sw = -A_BB^-1,  A_iB -> U_i = A_iB A_BB^-1,
-//   A_ij -> A_ij - U_i A_jB^T                      (i, j outside B),
-// which keeps the matrix symmetric; sweeping every block leaves -A^-1.
-// Pivots go in blocks of 16 (one 16 x 16 sweep in LDS, then a rank-16 update),
-// so an inversion takes m/16 barrier-separated steps. Each lane holds only
-// lower-triangle entries of its 16-strided row/column slots in registers:
-// lane (ti, tj) owns rows ti+16r and columns tj+16c for r > c, and r == c when
-// ti >= tj. offs holds (source, destination) offset pairs per block; the
-// destination receives the full symmetric inverse.
-constexpr int SW_NB = GJ_T;
-__global__ void __launch_bounds__(256) gj_inverse_k(double* W, const long long* offs, int m) {
-  __shared__ double colB[GJ_T * GJ_MAXR][SW_NB + 1];
-  __shared__ double U[GJ_T * GJ_MAXR][SW_NB + 1];
-  __shared__ double Sw[SW_NB][SW_NB + 1];
-  const double* A = W + offs[2 * blockIdx.x];
-  double* Ainv = W + offs[2 * blockIdx.x + 1];
+// Batched inversion of SPD m x m blocks with the symmetric sweep operator,
+// one launch per 32-pivot block B and one workgroup per 32 x 32 output tile:
+// sweeping B maps
+//   A_BB -> Sw = -A_BB^-1,   A_iB -> U_i = A_iB A_BB^-1 = -A_iB Sw,
+//   A_ij -> A_ij - U_i A_jB^T                         (i, j outside B),
+// so after every block is swept the matrix holds -A^-1. Each step reads the
+// previous state and writes the next one (ping-pong between two padded
+// 256 x 256 buffers per block), so tiles of one step never race. Pad entries
+// (index >= m) enter as the identity, which leaves the inverse of the m x m
+// block in its leading corner. Every workgroup sweeps the 32 x 32 pivot block
+// itself in LDS (cheap) instead of waiting for another launch.
+// offs holds (source, destination) offset pairs; step 0 reads the source,
+// the last step writes the negated result to the destination.
+__global__ void __launch_bounds__(256)
+inv_step_k(double* W, const long long* offs, int m, int step, int nsteps, double* scratch) {
+  __shared__ double Sw[IV_NB][IV_NB + 1];
+  __shared__ double Ai[IV_NB][IV_NB + 1];
+  __shared__ double Aj[IV_NB][IV_NB + 1];
+  __shared__ double Ui[IV_NB][IV_NB + 1];
+  __shared__ double Uj[IV_NB][IV_NB + 1];
+  const int tiles = (m + IV_NB - 1) / IV_NB;
+  const int I = blockIdx.x / tiles, J = blockIdx.x % tiles;
+  const int b = blockIdx.y;
+  const double* src = W + offs[2 * b];
+  double* dst = W + offs[2 * b + 1];
+  const long long pb = static_cast<long long>(IV_MP) * IV_MP;
+  const double* in = scratch + (2LL * b + (step & 1)) * pb;
+  double* out = scratch + (2LL * b + ((step + 1) & 1)) * pb;
+  const bool first = step == 0, last = step + 1 == nsteps;
+  auto rd = [&](int i, int j) -> double {
+    if (first) return (i < m && j < m) ? src[static_cast<long long>(i) * m + j] : (i == j ? 1.0 : 0.0);
+    return in[i * IV_MP + j];
+  };
   const int t = threadIdx.x;
-  const int ti = t / GJ_T, tj = t % GJ_T;
-  const bool diag_ok = ti >= tj;
-  double a[GJ_MAXR][GJ_MAXR];
+  const int tr = t / 8, tc = (t % 8) * 4;  // 4 consecutive columns per lane
+  const int k0 = step * IV_NB;
 #pragma unroll
-  for (int r = 0; r < GJ_MAXR; ++r)
+  for (int q = 0; q < 4; ++q) {
+    Sw[tr][tc + q] = rd(k0 + tr, k0 + tc + q);
+    Ai[tr][tc + q] = rd(I * IV_NB + tr, k0 + tc + q);
+    Aj[tr][tc + q] = rd(J * IV_NB + tr, k0 + tc + q);
+  }
+  __syncthreads();
+  // 32 single-pivot sweeps, ping-ponging between Sw and Ui (free until the
+  // U products below), one barrier per pivot; IV_NB is even, so the swept
+  // block ends in Sw.
+  for (int p = 0; p < IV_NB; ++p) {
+    double (*S)[IV_NB + 1] = (p & 1) ? Ui : Sw;
+    double (*D)[IV_NB + 1] = (p & 1) ? Sw : Ui;
+    const double inv = 1.0 / S[p][p];
 #pragma unroll
-    for (int c = 0; c <= r; ++c) {
-      const int i = ti + GJ_T * r, j = tj + GJ_T * c;
-      a[r][c] = (i < m && j < m && (r > c || diag_ok)) ? A[static_cast<long long>(i) * m + j] : 0.0;
-    }
-  for (int k0 = 0; k0 < m; k0 += SW_NB) {
-    const int kc = k0 / SW_NB;
-    const int nb = min(SW_NB, m - k0);
-    // (a) column block colB[i][b] = A[i][k0+b] for every row i (symmetric storage);
-    // columns past a short last block are zero so they drop out of (d).
-    if (nb < SW_NB)
-      for (int q = t; q < m * SW_NB; q += 256)
-        if (q % SW_NB >= nb) colB[q / SW_NB][q % SW_NB] = 0.0;
-#pragma unroll
-    for (int r = 0; r < GJ_MAXR; ++r)
-#pragma unroll
-      for (int cc = 0; cc <= r; ++cc) {
-        if (!(r > cc || diag_ok)) continue;
-        const int i = ti + GJ_T * r, j = tj + GJ_T * cc;
-        if (i >= m || j >= m) continue;
-        if (cc == kc && tj < nb) colB[i][tj] = a[r][cc];   // (i, k0+tj), i >= k0+tj
-        if (r == kc && ti < nb) colB[j][ti] = a[r][cc];    // (k0+ti, j) = (j, k0+ti)
-      }
-    __syncthreads();
-    // (b) Sw = sweep of the pivot block (16 single-pivot sweeps in LDS).
-    const int sa = t / SW_NB, sb = t % SW_NB;
-    if (sa < nb && sb < nb) Sw[sa][sb] = colB[k0 + sa][sb];
-    __syncthreads();
-    for (int p = 0; p < nb; ++p) {
-      double nv = 0.0;
-      if (sa < nb && sb < nb) {
-        const double inv = 1.0 / Sw[p][p];
-        const double v = Sw[sa][sb];
-        if (sa == p && sb == p) nv = -inv;
-        else if (sa == p || sb == p) nv = v * inv;
-        else nv = v - Sw[sa][p] * (Sw[p][sb] * inv);
-      }
-      __syncthreads();
-      if (sa < nb && sb < nb) Sw[sa][sb] = nv;
-      __syncthreads();
-    }
-    // (c) U_i = A_iB A_BB^-1 = -colB[i] Sw
-    for (int q = t; q < m * SW_NB; q += 256) {
-      const int i = q / SW_NB, b = q % SW_NB;
-      double s = 0.0;
-      if (b < nb)
-        for (int e = 0; e < nb; ++e) s -= colB[i][e] * Sw[e][b];
-      U[i][b] = s;
-    }
-    __syncthreads();
-    // (d) rank-16 update a_ij -= sum_b U[i][b] colB[j][b] as register-blocked
-    // outer products, then the entries in row/column block B are replaced.
-#pragma unroll 1
-    for (int b = 0; b < SW_NB; ++b) {
-      double ur[GJ_MAXR], cb[GJ_MAXR];
-#pragma unroll
-      for (int r = 0; r < GJ_MAXR; ++r) {
-        const int i = min(ti + GJ_T * r, GJ_T * GJ_MAXR - 1);
-        ur[r] = U[i][b];
-        cb[r] = colB[min(tj + GJ_T * r, GJ_T * GJ_MAXR - 1)][b];
-      }
-#pragma unroll
-      for (int r = 0; r < GJ_MAXR; ++r)
-#pragma unroll
-        for (int cc = 0; cc <= r; ++cc) a[r][cc] -= ur[r] * cb[cc];
-    }
-#pragma unroll
-    for (int r = 0; r < GJ_MAXR; ++r) {
-      const int i = ti + GJ_T * r;
-      const bool ib = r == kc;  // i in B (rows k0..k0+15 are row slot kc)
-#pragma unroll
-      for (int cc = 0; cc <= r; ++cc) {
-        const bool jb = cc == kc;
-        if (!(ib || jb)) continue;
-        const int j = tj + GJ_T * cc;
-        if (i >= m || j >= m || !(r > cc || diag_ok)) continue;
-        if (ib && jb) a[r][cc] = Sw[ti][tj];
-        else if (jb) a[r][cc] = U[i][tj];
-        else a[r][cc] = U[j][ti];
-      }
+    for (int q = 0; q < 4; ++q) {
+      const int sa = tr, sb = tc + q;
+      const double v = S[sa][sb];
+      double nv;
+      if (sa == p && sb == p) nv = -inv;
+      else if (sa == p || sb == p) nv = v * inv;
+      else nv = v - S[sa][p] * (S[p][sb] * inv);
+      D[sa][sb] = nv;
     }
     __syncthreads();
   }
+  // U_I = -A_IB Sw, U_J = -A_JB Sw
 #pragma unroll
-  for (int r = 0; r < GJ_MAXR; ++r)
-#pragma unroll
-    for (int c = 0; c <= r; ++c) {
-      const int i = ti + GJ_T * r, j = tj + GJ_T * c;
-      if (i < m && j < m && (r > c || diag_ok)) {
-        Ainv[static_cast<long long>(i) * m + j] = -a[r][c];
-        Ainv[static_cast<long long>(j) * m + i] = -a[r][c];
-      }
+  for (int q = 0; q < 4; ++q) {
+    double si = 0.0, sj = 0.0;
+    for (int e = 0; e < IV_NB; ++e) {
+      si -= Ai[tr][e] * Sw[e][tc + q];
+      sj -= Aj[tr][e] * Sw[e][tc + q];
     }
+    Ui[tr][tc + q] = si;
+    Uj[tr][tc + q] = sj;
+  }
+  __syncthreads();
+  const int i = I * IV_NB + tr;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int j = J * IV_NB + tc + q;
+    double v;
+    if (I == step && J == step) {
+      v = Sw[tr][tc + q];
+    } else if (J == step) {
+      v = Ui[tr][tc + q];
+    } else if (I == step) {
+      v = Uj[tc + q][tr];
+    } else {
+      double s = 0.0;
+      for (int e = 0; e < IV_NB; ++e) s += Ui[tr][e] * Aj[tc + q][e];
+      v = rd(i, j) - s;
+    }
+    if (last) {
+      if (i < m && j < m) dst[static_cast<long long>(i) * m + j] = -v;
+    } else {
+      out[i * IV_MP + j] = v;
+    }
+  }
 }
 
 // Batched C = beta*Din + sum_t alpha_t op(A_t) op(B_t), all m x m row-major.
@@ -206,7 +186,9 @@ __global__ void __launch_bounds__(256) gemm_jobs_k(double* W, const CrGemm* jobs
 // Batched y = beta*y0 + sum_t alpha_t M_t^T v_t (length-m vectors, FP64; every
 // matrix is stored so that the product reads it row by row, coalesced). Each
 // job's k range is split over GV_KS workgroups (partials to tmp), then a
-// second kernel sums the partials in a fixed order.
+// second kernel sums the partials in a fixed order. (A single kernel whose
+// last-arriving workgroup reduces needs a device-scope fence, which costs more
+// than the second launch on a multi-XCD part.)
 constexpr int GV_KS = 32;
 __global__ void __launch_bounds__(256) gemv_partial_k(const double* W, const CrGemv* jobs, int m,
                                                       double* tmp, const int* done) {
@@ -313,7 +295,7 @@ bool CoarseCR::plan(const Geom& g) {
   planes = dims[axis];
   const int u = axis == 0 ? 1 : 0, w = axis == 2 ? 1 : 2;
   m = 3 * dims[u] * dims[w];
-  if (m > GJ_T * GJ_MAXR) return false;
+  if (m > IV_MP) return false;
   const long long nodes = g.nodes();
   std::vector<int> plane_of(static_cast<size_t>(nodes)), local_of(static_cast<size_t>(nodes));
   std::vector<int> dst(static_cast<size_t>(3 * nodes));
@@ -461,6 +443,10 @@ bool CoarseCR::plan(const Geom& g) {
     maxjobs = std::max(maxjobs, static_cast<size_t>(lv_bw[r + 1] - lv_bw[r]));
   }
   tmp.ensure(maxjobs * GV_KS * 256);
+  size_t maxinv = 1;
+  for (size_t r = 0; r + 1 < lv_inv.size(); ++r)
+    maxinv = std::max(maxinv, static_cast<size_t>(lv_inv[r + 1] - lv_inv[r]));
+  inv_scratch.ensure(maxinv * 2 * IV_MP * IV_MP);
   up(d_inv, inv_offs);
   up(d_gp, gemm_p);
   up(d_gd, gemm_d);
@@ -484,8 +470,12 @@ void CoarseCR::build(const double* Kem, const uint8_t* mask, hipStream_t s) {
   for (size_t r = 0; r < lv.size(); ++r) {
     const int ni = lv_inv[r + 1] - lv_inv[r];
     if (ni > 0) {
-      hipLaunchKernelGGL(gj_inverse_k, dim3(ni), dim3(GJ_T * GJ_T), 0, s, W.p, d_inv.p + 2 * lv_inv[r], m);
-      TOPOHIP_CHECK(hipGetLastError());
+      const int nt = (m + IV_NB - 1) / IV_NB;
+      for (int st = 0; st < nt; ++st) {
+        hipLaunchKernelGGL(inv_step_k, dim3(nt * nt, ni), dim3(256), 0, s, W.p,
+                           d_inv.p + 2 * lv_inv[r], m, st, nt, inv_scratch.p);
+        TOPOHIP_CHECK(hipGetLastError());
+      }
     }
     const int np = lv_gp[r + 1] - lv_gp[r];
     if (np > 0) {

@@ -34,8 +34,8 @@ struct CrLevel {
 
 class CoarseCR {
  public:
-  // Plans the reduction for a level of geometry g (false if a plane is wider
-  // than the register-resident inversion supports).
+  // Plans the reduction for a level of geometry g (false if a plane holds
+  // more than 256 DOFs).
   bool plan(const Geom& g);
   // Factor from the level's element-major FP64 element matrices.
   void build(const double* Kem, const uint8_t* mask, hipStream_t s);
@@ -53,7 +53,7 @@ class CoarseCR {
   std::vector<CrLevel> lv;
   std::vector<int> lv_inv, lv_gp, lv_gd, lv_fw, lv_bw;
   void gemv(const CrGemv* jobs, int n, const int* done, hipStream_t s);
-  DBuf<double> W, tmp;
+  DBuf<double> W, tmp, inv_scratch;
   DBuf<long long> d_inv;
   DBuf<CrGemm> d_gp, d_gd;
   DBuf<CrGemv> d_fw, d_bw;

@@ -7,10 +7,8 @@ namespace topohip {
 
 enum Epilogue { EPI_AX = 0, EPI_RES = 1, EPI_JAC = 2 };
 
-// One tiled operator apply over a structured level. The fine level uses the
-// modal element operator scaled by per-element moduli E; coarse levels use a
-// dense 24x24 matrix per element (Kel, entry-major [576][elems], bit-ordered
-// local DOFs). Epilogues: y = A x, y = b - A x, y = x + omega*dinv*(b - A x).
+// One tiled operator apply over the fine structured level: the modal element
+// operator scaled by per-element moduli E. Epilogues: y = A x, y = b - A x, y = x + omega*dinv*(b - A x).
 // With `partials` set, each block also writes sum(x . A x) over its owned kept
 // DOFs (FP64) to partials[block].
 template <typename T>
@@ -23,7 +21,6 @@ struct KuOp {
   T omega = T(0);
   const uint8_t* mask = nullptr;  // per node: bit c = DOF c kept; null = all kept
   const T* E = nullptr;           // fine level
-  const T* Kel = nullptr;         // coarse levels
   Modal<T> mc;
   int epi = EPI_AX;
   double* partials = nullptr;
@@ -34,7 +31,6 @@ struct KuOp {
 
 int ku_blocks(const Geom& g, int tz, int tx);
 template <typename T> void launch_ku_fine(const KuOp<T>& op, hipStream_t s);
-template <typename T> void launch_ku_dense(const KuOp<T>& op, hipStream_t s);
 
 // ---- transfers between a fine level f and the coincident coarse level c ----
 // bc = mask_c .* P^T rf; optionally xc = omega * dinv_c .* bc (pre-smooth from 0).

@@ -1,21 +1,24 @@
-// ku.hip — matrix-free hex8 operator apply on a structured level.
+// ku.hip — matrix-free hex8 operator apply on the fine level.
 //
 // Replaces the colour-ordered element loop of core/src/fea/matfree.cpp:75-123,
 // 535-553 (gather 24 / 24x24 column AXPY / scatter 24). Design for gfx950:
 //
 //  * A workgroup owns a (TX-1) x (TY-1) tile of nodes in x-y and marches a
-//    z-chunk of node planes. Each lane computes one element column, so the
-//    block computes a TX x TY element tile (one halo element on the low x and
-//    y side) and TZ+1 element layers; node planes are staged in LDS and
-//    double-buffered along z.
-//  * Node-owned deterministic accumulation: every element writes its 8 nodal
-//    results to LDS and the owning lane sums its (at most) 8 element
-//    contributions in a fixed order. No atomics, bit-identical run to run.
-//  * Fine level: the 24x24 element stiffness is applied in the Hadamard
-//    (parity-mode) basis, where it has 45 non-zeros instead of 576: forward
-//    butterfly (72 add/sub), 45 FMA, scale by E_e, inverse butterfly.
-//  * Coarse levels: a dense per-element 24x24 (Galerkin product), read
-//    entry-major so each load instruction is coalesced across the wave.
+//    z-chunk of node planes. Each lane handles one element column (one halo
+//    element on the low x and y side); node planes are staged in LDS, double
+//    buffered, with the next plane (and the next layer's moduli, masks and
+//    epilogue operands) prefetched into registers one layer ahead.
+//  * The element matrix is applied in the Hadamard (parity-mode) basis of the
+//    8 corners, where E*Ke has 45 non-zeros: butterfly, 21 scalings by E,
+//    45 FMA, inverse butterfly. The butterfly factorises by axis, and its x-y
+//    stages act on one node plane at a time, so each lane transforms the 4-node
+//    face of its column once per plane and shares it between the element below
+//    and the element above; on the way back the z stage splits each element's
+//    result into a bottom and a top face, the two faces that meet on a plane
+//    are added first, and the inverse x-y stages run once per plane.
+//  * Node-owned deterministic accumulation: every lane writes its face's 4
+//    corner results to LDS and the owning lane sums its 4 face contributions
+//    in a fixed order. No atomics, bit-identical run to run.
 #include "kernels.hpp"
 
 namespace topohip {
@@ -24,64 +27,34 @@ namespace {
 
 constexpr int NT = 256;
 
-template <typename T>
-__device__ inline void hadamard3(T (&a)[24]) {
-#pragma unroll
-  for (int bit = 1; bit < 8; bit <<= 1) {
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      if (m & bit) continue;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const T p = a[3 * m + c];
-        const T q = a[3 * (m | bit) + c];
-        a[3 * m + c] = p + q;
-        a[3 * (m | bit) + c] = p - q;
-      }
-    }
-  }
-}
-
-// y = E * Ke * u in bit-ordered local DOFs via the modal basis.
-template <typename T>
-__device__ inline void modal_apply(T (&u)[24], T (&v)[24], T e, const Modal<T>& mc) {
-  hadamard3(u);
-#pragma unroll
-  for (int s = 3; s < 24; ++s) u[s] *= e;
-#pragma unroll
-  for (int r = 0; r < 24; ++r) v[r] = T(0);
-#define TOPOHIP_X(k, r, s) v[r] += mc.c[k] * u[s];
-  TOPOHIP_MODAL_PATTERN(TOPOHIP_X)
-#undef TOPOHIP_X
-  hadamard3(v);
-}
-
-template <typename T>
-__device__ inline void dense_apply(const T (&u)[24], T (&v)[24], const T* __restrict__ K,
-                                   long long ne, long long e) {
-#pragma unroll
-  for (int r = 0; r < 24; ++r) {
-    T s = T(0);
-#pragma unroll
-    for (int c = 0; c < 24; ++c) s += K[(r * 24 + c) * ne + e] * u[c];
-    v[r] = s;
-  }
-}
-
 __device__ inline double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   return v;
 }
 
-template <typename T, int TX, bool DENSE, int EPI, bool DOT>
-__global__ void __launch_bounds__(NT) ku_tiled(KuOp<T> a) {
+// Forward x-y butterfly of a 4-node face (corner q = bx + 2 by, 3 components).
+template <typename T>
+__device__ inline void face_fwd(const T (&u)[12], T (&F)[12]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const T a0 = u[c] + u[3 + c], a1 = u[c] - u[3 + c];
+    const T a2 = u[6 + c] + u[9 + c], a3 = u[6 + c] - u[9 + c];
+    F[c] = a0 + a2;
+    F[6 + c] = a0 - a2;
+    F[3 + c] = a1 + a3;
+    F[9 + c] = a1 - a3;
+  }
+}
+
+template <typename T, int TX, int EPI, bool DOT>
+__global__ void __launch_bounds__(NT) ku_face(KuOp<T> a) {
   constexpr int TY = NT / TX;
   constexpr int PLANE = (TX + 1) * (TY + 1) * 3;
   constexpr int NPF = (PLANE + NT - 1) / NT;  // prefetch registers per lane
   if (a.done && *a.done) return;
   __shared__ T s_u[2][PLANE];
-  __shared__ T s_c[8][NT * 3];
+  __shared__ T s_c[4][NT * 3];
   __shared__ double s_red[NT / 64];
 
   const Geom g = a.g;
@@ -94,9 +67,8 @@ __global__ void __launch_bounds__(NT) ku_tiled(KuOp<T> a) {
   const bool exy = ei >= 0 && ei < g.ex && ej >= 0 && ej < g.ey;
   const uint8_t* __restrict__ mask = a.mask;
   const T* __restrict__ x = a.x;
+  const T* __restrict__ E = a.E;
 
-  // Node plane c is fetched into registers one layer ahead (software
-  // pipelining) and written to LDS at the top of the next layer.
   T pf[NPF];
   auto fetch_plane = [&](int c) {
 #pragma unroll
@@ -124,92 +96,130 @@ __global__ void __launch_bounds__(NT) ku_tiled(KuOp<T> a) {
       if (q < PLANE) s_u[buf][q] = pf[i];
     }
   };
+  auto load_face = [&](int buf, T (&F)[12]) {
+    T u[12];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int base = ((ty + (q >> 1)) * (TX + 1) + tx + (q & 1)) * 3;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) u[3 * q + c] = s_u[buf][base + c];
+    }
+    face_fwd(u, F);
+  };
+  auto load_own = [&](int buf, T (&xo)[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xo[c] = s_u[buf][((ty + 1) * (TX + 1) + (tx + 1)) * 3 + c];
+  };
+  auto modulus = [&](int k) -> T {
+    if (!exy || k < 0 || k >= g.ez) return T(0);
+    return E[(static_cast<long long>(k) * g.ey + ej) * g.ex + ei];
+  };
+  // Owner-node operands of plane c, fetched one layer ahead.
+  int mk_n = 0;
+  T b_n[3] = {T(0), T(0), T(0)}, d_n[3] = {T(0), T(0), T(0)};
+  auto fetch_owner = [&](int c) {
+    if (!owner || c < 0 || c >= g.nnz) return;
+    const long long node = (static_cast<long long>(c) * g.nny + nb) * g.nnx + na;
+    mk_n = mask ? mask[node] : 7;
+    if constexpr (EPI != EPI_AX) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) b_n[q] = a.b[node * 3 + q];
+    }
+    if constexpr (EPI == EPI_JAC) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) d_n[q] = a.dinv[node * 3 + q];
+    }
+  };
 
   int cur = 0;
   fetch_plane(k0 - 1);
   store_plane(cur);
   fetch_plane(k0);
-  T acc[3] = {T(0), T(0), T(0)};
-  T xown[3] = {T(0), T(0), T(0)};
+  T e_n = modulus(k0 - 1);
+  fetch_owner(k0);
+  T Fp[12], Vt[12], xo[3] = {T(0), T(0), T(0)};
+#pragma unroll
+  for (int r = 0; r < 12; ++r) Vt[r] = T(0);
   double dotp = 0.0;
   const int kend = k0 + a.tz;
   for (int k = k0 - 1; k < kend; ++k) {
     const int nxt = cur ^ 1;
     store_plane(nxt);  // plane k + 1
     if (k + 1 < kend) fetch_plane(k + 2);
+    const T e = e_n;
+    e_n = modulus(k + 1);
+    const int mk = mk_n;
+    T bo[3], dv[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) { bo[q] = b_n[q]; dv[q] = d_n[q]; }
+    if (k >= k0) fetch_owner(k + 1);
     __syncthreads();
-    if (owner) {
+    if (k == k0 - 1) load_face(cur, Fp);
+    T Fn[12], xn[3];
+    load_face(nxt, Fn);
+    load_own(nxt, xn);
+    // Element layer k between planes k (Fp) and k + 1 (Fn): z stage, modal
+    // core, inverse z stage split into the bottom (Vb) and top faces.
+    T A[24], v[24];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) xown[c] = s_u[cur][((ty + 1) * (TX + 1) + (tx + 1)) * 3 + c];
-    }
-    T yl[24];
-    bool has = exy && k >= 0 && k < g.ez;
-    const long long eidx = (static_cast<long long>(k) * g.ey + ej) * g.ex + ei;
-    T e = T(0);
-    if (!DENSE && has) {
-      e = a.E[eidx];
-      has = e != T(0);
-    }
-    if (has) {
-      T ul[24];
-#pragma unroll
-      for (int m = 0; m < 8; ++m) {
-        const int bx = m & 1, by = (m >> 1) & 1, bz = m >> 2;
-        const T* pl = bz ? s_u[nxt] : s_u[cur];
-        const int base = ((ty + by) * (TX + 1) + (tx + bx)) * 3;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) ul[3 * m + c] = pl[base + c];
-      }
-      if constexpr (DENSE) {
-        dense_apply(ul, yl, a.Kel, g.elems(), eidx);
-      } else {
-        modal_apply(ul, yl, e, a.mc);
-      }
-    } else {
-#pragma unroll
-      for (int r = 0; r < 24; ++r) yl[r] = T(0);
+    for (int q = 0; q < 12; ++q) {
+      A[q] = Fp[q] + Fn[q];
+      A[12 + q] = Fp[q] - Fn[q];
     }
 #pragma unroll
-    for (int m = 0; m < 8; ++m)
+    for (int s = 3; s < 24; ++s) A[s] *= e;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) s_c[m][t * 3 + c] = yl[3 * m + c];
-    __syncthreads();
-    if (owner) {
-      // Node (na, nb) is corner m of element (na-1+bx', nb-1+by') with
-      // bit x = 1 - (element offset along x), etc. Fixed summation order.
-      const int t00 = t, t10 = t + 1, t01 = t + TX, t11 = t + TX + 1;
-      T bot[3], top[3];
+    for (int r = 0; r < 24; ++r) v[r] = T(0);
+#define TOPOHIP_X(kk, r, s) v[r] += a.mc.c[kk] * A[s];
+    TOPOHIP_MODAL_PATTERN(TOPOHIP_X)
+#undef TOPOHIP_X
+    T G[12];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+      G[q] = Vt[q] + (v[q] + v[12 + q]);
+      Vt[q] = v[q] - v[12 + q];
+    }
+    if (k >= k0) {
+      // Inverse x-y butterfly of plane k's face, corner results to LDS.
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        bot[c] = ((s_c[3][t00 * 3 + c] + s_c[2][t10 * 3 + c]) + s_c[1][t01 * 3 + c]) +
-                 s_c[0][t11 * 3 + c];
-        top[c] = ((s_c[7][t00 * 3 + c] + s_c[6][t10 * 3 + c]) + s_c[5][t01 * 3 + c]) +
-                 s_c[4][t11 * 3 + c];
+        const T b0 = G[c] + G[3 + c], b1 = G[c] - G[3 + c];
+        const T b2 = G[6 + c] + G[9 + c], b3 = G[6 + c] - G[9 + c];
+        s_c[0][t * 3 + c] = b0 + b2;
+        s_c[2][t * 3 + c] = b0 - b2;
+        s_c[1][t * 3 + c] = b1 + b3;
+        s_c[3][t * 3 + c] = b1 - b3;
       }
-      if (k >= k0 && k < g.nnz) {
-        const long long node = (static_cast<long long>(k) * g.nny + nb) * g.nnx + na;
-        const int mk = mask ? mask[node] : 7;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const T ax = acc[c] + bot[c];
-          const bool kept = (mk >> c) & 1;
-          T out = T(0);
-          if (kept) {
-            if constexpr (EPI == EPI_AX) {
-              out = ax;
-            } else if constexpr (EPI == EPI_RES) {
-              out = a.b[node * 3 + c] - ax;
-            } else {
-              out = xown[c] + a.omega * a.dinv[node * 3 + c] * (a.b[node * 3 + c] - ax);
-            }
-            if constexpr (DOT) dotp += static_cast<double>(xown[c]) * static_cast<double>(ax);
-          }
-          a.y[node * 3 + c] = out;
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) acc[c] = top[c];
     }
+    __syncthreads();
+    if (k >= k0 && owner && k < g.nnz) {
+      // Node (na, nb) is corner 3 of column (tx, ty), 2 of (tx+1, ty),
+      // 1 of (tx, ty+1) and 0 of (tx+1, ty+1). Fixed summation order.
+      const int t00 = t, t10 = t + 1, t01 = t + TX, t11 = t + TX + 1;
+      const long long node = (static_cast<long long>(k) * g.nny + nb) * g.nnx + na;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T ax = ((s_c[3][t00 * 3 + c] + s_c[2][t10 * 3 + c]) + s_c[1][t01 * 3 + c]) +
+                     s_c[0][t11 * 3 + c];
+        const bool kept = (mk >> c) & 1;
+        T out = T(0);
+        if (kept) {
+          if constexpr (EPI == EPI_AX) {
+            out = ax;
+          } else if constexpr (EPI == EPI_RES) {
+            out = bo[c] - ax;
+          } else {
+            out = xo[c] + a.omega * dv[c] * (bo[c] - ax);
+          }
+          if constexpr (DOT) dotp += static_cast<double>(xo[c]) * static_cast<double>(ax);
+        }
+        a.y[node * 3 + c] = out;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 12; ++q) Fp[q] = Fn[q];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xo[c] = xn[c];
     cur = nxt;
   }
   if constexpr (DOT) {
@@ -225,13 +235,13 @@ __global__ void __launch_bounds__(NT) ku_tiled(KuOp<T> a) {
   }
 }
 
-template <typename T, int TX, bool DENSE>
+template <typename T, int TX>
 void dispatch_tx(const KuOp<T>& op, hipStream_t s) {
   constexpr int TY = NT / TX;
   const dim3 grid((op.g.nnx + TX - 2) / (TX - 1), (op.g.nny + TY - 2) / (TY - 1),
                   (op.g.nnz + op.tz - 1) / op.tz);
   const bool dot = op.partials != nullptr;
-#define TOPOHIP_L(E, D) hipLaunchKernelGGL((ku_tiled<T, TX, DENSE, E, D>), grid, dim3(NT), 0, s, op)
+#define TOPOHIP_L(E, D) hipLaunchKernelGGL((ku_face<T, TX, E, D>), grid, dim3(NT), 0, s, op)
   if (op.epi == EPI_AX) {
     if (dot) TOPOHIP_L(EPI_AX, true); else TOPOHIP_L(EPI_AX, false);
   } else if (op.epi == EPI_RES) {
@@ -241,13 +251,6 @@ void dispatch_tx(const KuOp<T>& op, hipStream_t s) {
   }
 #undef TOPOHIP_L
   TOPOHIP_CHECK(hipGetLastError());
-}
-
-template <typename T, bool DENSE>
-void dispatch(const KuOp<T>& op, hipStream_t s) {
-  if (op.tx == 32) dispatch_tx<T, 32, DENSE>(op, s);
-  else if (op.tx == 64) dispatch_tx<T, 64, DENSE>(op, s);
-  else dispatch_tx<T, 16, DENSE>(op, s);
 }
 
 }  // namespace
@@ -260,16 +263,12 @@ int ku_blocks(const Geom& g, int tz, int tx) {
 
 template <typename T>
 void launch_ku_fine(const KuOp<T>& op, hipStream_t s) {
-  dispatch<T, false>(op, s);
-}
-template <typename T>
-void launch_ku_dense(const KuOp<T>& op, hipStream_t s) {
-  dispatch<T, true>(op, s);
+  if (op.tx == 32) dispatch_tx<T, 32>(op, s);
+  else if (op.tx == 64) dispatch_tx<T, 64>(op, s);
+  else dispatch_tx<T, 16>(op, s);
 }
 
 template void launch_ku_fine<float>(const KuOp<float>&, hipStream_t);
 template void launch_ku_fine<double>(const KuOp<double>&, hipStream_t);
-template void launch_ku_dense<float>(const KuOp<float>&, hipStream_t);
-template void launch_ku_dense<double>(const KuOp<double>&, hipStream_t);
 
 }  // namespace topohip
