@@ -19,6 +19,11 @@ Also reported on the same JSON line:
                8-colour threaded apply) timed on this host for one design
                iteration of the same workload
 
+Other workloads (--workload): cantilever_60x20x20, bridge_256x64x96 (four load
+cases solved as one block PCG, summed compliance), cantilever_512x256x256, and
+serving (a Poisson stream of optimisation jobs through the continuous-batching
+server: p50/p99 job latency and aggregate design-iters/s; --jobs, --rate).
+
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
 import argparse
@@ -43,12 +48,12 @@ def ku_bytes(nx, ny, nz, word):
 def run_design(ta, wl, precision, steps, warmup):
     nx, ny, nz = wl["grid"]
     with ta.Problem(nx, ny, nz, wl["bcs"], wl["loads"], precision=precision) as p:
+        kw = dict(penalty=wl["penalty"], move=wl["move"], cg_tol=wl["tol"],
+                  continuation=wl.get("continuation"))
         if warmup > 0:
-            p.optimize(wl["volfrac"], wl["rmin"], warmup, penalty=wl["penalty"], move=wl["move"],
-                       cg_tol=wl["tol"])
+            p.optimize(wl["volfrac"], wl["rmin"], warmup, **kw)
         t0 = time.perf_counter()
-        out = p.optimize(wl["volfrac"], wl["rmin"], steps, penalty=wl["penalty"], move=wl["move"],
-                         cg_tol=wl["tol"])
+        out = p.optimize(wl["volfrac"], wl["rmin"], steps, **kw)
         dt = time.perf_counter() - t0
     return out, dt
 
@@ -67,6 +72,39 @@ def ku_measure(ta, nx, ny, nz, reps=100):
                      "hbm_frac": gbs / HBM_PEAK_GBS, "bytes": b,
                      "gflops": 1152.0 * nx * ny * nz / (ms * 1e-3) / 1e9}
     return res
+
+
+def run_serving(ta, njobs, rate, seed, iterations):
+    """Poisson arrivals of jobs with uneven grids through the serving loop."""
+    rng = np.random.default_rng(seed)
+    grids = [(48, 16, 16), (96, 32, 32), (128, 64, 32), (192, 64, 64)]
+    jobs = []
+    for _ in range(njobs):
+        g = grids[rng.integers(len(grids))]
+        jobs.append(dict(nx=g[0], ny=g[1], nz=g[2], bc_kind=int(rng.integers(2)),
+                         volfrac=float(rng.uniform(0.1, 0.4)), rmin=float(rng.uniform(1.2, 2.5)),
+                         iterations=iterations, cg_tol=1e-6))
+    gaps = rng.exponential(1.0 / rate, njobs)
+    with ta.Server(budget_bytes=200 << 30, max_slots=16) as srv:
+        ids = []
+        t0 = time.perf_counter()
+        t_next = t0
+        for j, gap in zip(jobs, gaps):
+            t_next += gap
+            delay = t_next - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(srv.submit(j))
+        srv.drain()
+        makespan = time.perf_counter() - t0
+        res = [srv.poll(i) for i in ids]
+    failed = sum(r["state"] != 2 for r in res)
+    lat = np.array([r["t_end_ms"] - r["t_submit_ms"] for r in res])
+    its = sum(r["iterations"] for r in res)
+    return {"jobs": njobs, "failed": failed, "rate_per_s": rate, "seed": seed,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "design_iters": its, "makespan_s": makespan,
+            "design_iters_per_s": its / makespan}
 
 
 def cpu_baseline(wl, threads):
@@ -96,6 +134,9 @@ def main():
     ap.add_argument("--workload", default="mbb_192x64x64")
     ap.add_argument("--no-fp64-form", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--jobs", type=int, default=128)
+    ap.add_argument("--rate", type=float, default=4.0)
+    ap.add_argument("--seed", type=int, default=0)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this engine has no multi-GPU path (each job and each "
@@ -103,6 +144,21 @@ def main():
         sys.exit(2)
     import topopt_amd as ta
     from topopt_amd import problems
+
+    if a.workload == "serving":
+        print(f"# serving: {a.jobs} jobs, Poisson rate {a.rate}/s, seed {a.seed}", file=sys.stderr,
+              flush=True)
+        sv = run_serving(ta, a.jobs, a.rate, a.seed, iterations=60)
+        print(json.dumps({
+            "metric": "serving: aggregate design iterations/s (job p50/p99 latency in config)",
+            "value": sv["design_iters_per_s"], "unit": "design iters/s", "n_gpus": 1,
+            "steps": sv["design_iters"], "warmup": 0,
+            "ms_per_step": 1e3 * sv["makespan_s"] / max(1, sv["design_iters"]),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 operator/V-cycle, f64 outer PCG",
+            "data": "synthetic (Poisson job stream, cantilever/MBB boxes)",
+            "config": {"workload": "serving", **sv}}), flush=True)
+        return
 
     wl = problems.workload(a.workload)
     nx, ny, nz = wl["grid"]
@@ -130,7 +186,7 @@ def main():
         print(f"# cpu baseline {cpu['value']:.4f} design iters/s on {threads} threads",
               file=sys.stderr, flush=True)
     line = {
-        "metric": "design iterations/s (MBB 192x64x64, FP32 K.u + FP32 MG in FP64-outer PCG)",
+        "metric": f"design iterations/s ({a.workload}, FP32 K.u + FP32 MG in FP64-outer PCG)",
         "value": value,
         "unit": "design iters/s",
         "n_gpus": 1,
@@ -141,10 +197,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 operator/V-cycle, f64 outer PCG",
-        "data": "synthetic (analytic MBB box, uniform initial density)",
+        "data": "synthetic (analytic box workload, uniform initial density)",
         "config": {"workload": a.workload, "grid": [nx, ny, nz], "dof": ku["f32"]["bytes"] and
                    3 * (nx + 1) * (ny + 1) * (nz + 1), "volfrac": wl["volfrac"], "rmin": wl["rmin"],
-                   "penalty": wl["penalty"], "cg_tol": wl["tol"], "mg_levels": 4,
+                   "penalty": wl["penalty"], "cg_tol": wl["tol"], "load_cases": len(wl["loads"]),
                    "compliance": out["compliance"], "cg_iterations_total": out["cg_iterations_total"],
                    "t_solve_ms": out["t_solve_ms"], "t_build_ms": out["t_build_ms"],
                    "t_oc_ms": out["t_oc_ms"], "t_filter_ms": out["t_filter_ms"]},

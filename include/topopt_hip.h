@@ -193,6 +193,8 @@ int topohip_server_poll(topohip_server* s, int64_t job_id, topohip_job_result* o
 int topohip_server_drain(topohip_server* s);
 /* Copies a finished job's physical density (nx*ny*nz doubles). */
 int topohip_server_fetch_density(topohip_server* s, int64_t job_id, double* xphys);
+/* Message of a failed job (empty when it succeeded). */
+int topohip_server_job_error(topohip_server* s, int64_t job_id, char* buf, size_t n);
 int topohip_server_destroy(topohip_server* s);
 /* Device bytes a job of this shape keeps resident (admission estimate). */
 size_t topohip_job_footprint(const topohip_job* job);

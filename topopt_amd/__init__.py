@@ -330,7 +330,12 @@ class Server:
     def poll(self, jid):
         r = JobResult()
         _check(lib().topohip_server_poll(self._s, C.c_int64(jid), C.byref(r)))
-        return r.as_dict()
+        out = r.as_dict()
+        if out["state"] == 3:
+            buf = C.create_string_buffer(1024)
+            lib().topohip_server_job_error(self._s, C.c_int64(jid), buf, 1024)
+            out["error"] = buf.value.decode(errors="replace")
+        return out
 
     def drain(self):
         _check(lib().topohip_server_drain(self._s))

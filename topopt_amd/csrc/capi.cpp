@@ -82,7 +82,7 @@ void apply_host(Problem& P, const double* youngs, const double* u, double* y) {
     launch_convert<double, float>(P.E64.p, P.E32.p, nv, P.s);
     launch_convert<double, float>(u64.p, u32.p, nd, P.s);
     op.E = P.E32.p;
-    op.mc = P.mc32;
+    op.mg = P.mg32;
     op.x = u32.p;
     op.y = y32.p;
     launch_ku_fine<float>(op, P.s);
@@ -90,7 +90,7 @@ void apply_host(Problem& P, const double* youngs, const double* u, double* y) {
     TOPOHIP_CHECK(hipStreamSynchronize(P.s));
   } else {
     op.E = P.E64.p;
-    op.mc = P.mc64;
+    op.mg = P.mg64;
     op.x = u64.p;
     op.y = y64.p;
     launch_ku_fine<double>(op, P.s);
@@ -175,7 +175,7 @@ int topohip_solve(topohip_problem* p, const double* youngs, double tol, int max_
     }
     topohip_cginfo ci{};
     try {
-      solve_rhs(P, 0, tol, max_it, guess ? dg.p : nullptr, &ci);
+      solve_cases(P, 1, tol, max_it, guess ? dg.p : nullptr, &ci);
     } catch (...) {
       ci.t_mg_build_ms = t.mg_build;
       if (info) *info = ci;
@@ -200,7 +200,7 @@ int topohip_simp_compliance(topohip_problem* p, const double* xphys, double E0, 
     *c = simp_compliance_dev(P, P.xphys.p, E0, penalty, rho_min, tol, max_it, info);
     if (dc)
       TOPOHIP_CHECK(hipMemcpyAsync(dc, P.dc.p, nv * sizeof(double), hipMemcpyDeviceToHost, P.s));
-    if (u_out && P.nrhs == 1)
+    if (u_out)  // load case 0
       TOPOHIP_CHECK(hipMemcpyAsync(u_out, P.u.p, nd * sizeof(double), hipMemcpyDeviceToHost, P.s));
     TOPOHIP_CHECK(hipStreamSynchronize(P.s));
   });
@@ -284,12 +284,12 @@ int topohip_bench_ku(topohip_problem* p, int precision, int reps, double* ms_per
     auto run = [&]() {
       if (precision == TOPOHIP_PREC_F64) {
         KuOp<double> op;
-        op.g = P.g; op.tz = P.tz; op.tx = P.tx; op.epi = EPI_AX; op.E = P.E64.p; op.mc = P.mc64;
+        op.g = P.g; op.tz = P.tz; op.tx = P.tx; op.epi = EPI_AX; op.E = P.E64.p; op.mg = P.mg64;
         op.x = p->bu64.p; op.y = p->by64.p;
         launch_ku_fine<double>(op, P.s);
       } else {
         KuOp<float> op;
-        op.g = P.g; op.tz = P.tz; op.tx = P.tx; op.epi = EPI_AX; op.E = P.E32.p; op.mc = P.mc32;
+        op.g = P.g; op.tz = P.tz; op.tx = P.tx; op.epi = EPI_AX; op.E = P.E32.p; op.mg = P.mg32;
         op.x = p->bu32.p; op.y = p->by32.p;
         launch_ku_fine<float>(op, P.s);
       }

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "coarse.hpp"
+#include "../../include/topopt_hip.h"
 
 namespace topohip {
 
@@ -190,9 +191,14 @@ __global__ void __launch_bounds__(256) gemm_jobs_k(double* W, const CrGemm* jobs
 // last-arriving workgroup reduces needs a device-scope fence, which costs more
 // than the second launch on a multi-XCD part.)
 constexpr int GV_KS = 32;
-__global__ void __launch_bounds__(256) gemv_partial_k(const double* W, const CrGemv* jobs, int m,
-                                                      double* tmp, const int* done) {
-  if (done && *done) return;
+__global__ void __launch_bounds__(256) gemv_partial_k(const double* W, const double* V,
+                                                      long long vsz, const CrGemv* jobs, int m,
+                                                      double* tmp, long long tsz,
+                                                      const int* done) {
+  const int rc = blockIdx.z;
+  if (case_done(done, rc)) return;
+  V += rc * vsz;
+  tmp += rc * tsz;
   const CrGemv& jb = jobs[blockIdx.x];
   const int ks = blockIdx.y;
   const int kc = (m + GV_KS - 1) / GV_KS;
@@ -202,7 +208,7 @@ __global__ void __launch_bounds__(256) gemv_partial_k(const double* W, const CrG
   if (o < m) {
     for (int t = 0; t < jb.nterms; ++t) {
       const double* M = W + jb.m[t];
-      const double* v = W + jb.v[t];
+      const double* v = V + jb.v[t];
       double st = 0.0;
 #pragma unroll 8
       for (int k = k0; k < k1; ++k) st += M[static_cast<long long>(k) * m + o] * v[k];
@@ -212,16 +218,20 @@ __global__ void __launch_bounds__(256) gemv_partial_k(const double* W, const CrG
   tmp[(static_cast<long long>(blockIdx.x) * GV_KS + ks) * 256 + o] = s;
 }
 
-__global__ void __launch_bounds__(256) gemv_reduce_k(double* W, const CrGemv* jobs, int m,
-                                                     const double* tmp, const int* done) {
-  if (done && *done) return;
+__global__ void __launch_bounds__(256) gemv_reduce_k(double* V, long long vsz, const CrGemv* jobs,
+                                                     int m, const double* tmp, long long tsz,
+                                                     const int* done) {
+  const int rc = blockIdx.y;
+  if (case_done(done, rc)) return;
+  V += rc * vsz;
+  tmp += rc * tsz;
   const CrGemv& jb = jobs[blockIdx.x];
   const int o = threadIdx.x;
   if (o >= m) return;
   double y = 0.0;
   for (int ks = 0; ks < GV_KS; ++ks) y += tmp[(static_cast<long long>(blockIdx.x) * GV_KS + ks) * 256 + o];
-  if (jb.beta != 0.0) y += jb.beta * W[jb.y0 + o];
-  W[jb.out + o] = y;
+  if (jb.beta != 0.0) y += jb.beta * V[jb.y0 + o];
+  V[jb.out + o] = y;
 }
 
 // Blocks of the coarsest operator in plane order, from the element matrices.
@@ -271,17 +281,20 @@ __global__ void cr_assemble_k(Geom g, const double* __restrict__ K, const uint8_
 // Node-ordered level vector (T) <-> plane-ordered FP64 vector.
 template <typename T>
 __global__ void cr_gather_k(const T* __restrict__ v, const int* __restrict__ dst, long long n,
-                            double* W, long long off, const int* done) {
-  if (done && *done) return;
+                            double* V, long long vsz, long long off, const int* done) {
+  const int rc = blockIdx.y;
+  if (case_done(done, rc)) return;
   const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
-  if (i < n) W[off + dst[i]] = static_cast<double>(v[i]);
+  if (i < n) V[rc * vsz + off + dst[i]] = static_cast<double>(v[rc * n + i]);
 }
 template <typename T>
-__global__ void cr_scatter_k(const double* W, long long off, const int* __restrict__ dst,
-                             long long n, T* __restrict__ v, const int* done) {
-  if (done && *done) return;
+__global__ void cr_scatter_k(const double* V, long long vsz, long long off,
+                             const int* __restrict__ dst, long long n, T* __restrict__ v,
+                             const int* done) {
+  const int rc = blockIdx.y;
+  if (case_done(done, rc)) return;
   const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
-  if (i < n) v[i] = static_cast<T>(W[off + dst[i]]);
+  if (i < n) v[rc * n + i] = static_cast<T>(V[rc * vsz + off + dst[i]]);
 }
 
 }  // namespace
@@ -311,7 +324,7 @@ bool CoarseCR::plan(const Geom& g) {
       }
   // Level sizes and workspace layout.
   lv.clear();
-  long long off = 0;
+  long long off = 0, voff = 0;  // matrices in W, vectors (per load case) in V
   const long long mm = static_cast<long long>(m) * m;
   int n = planes;
   while (true) {
@@ -324,13 +337,14 @@ bool CoarseCR::plan(const Geom& g) {
     L.Q = off; off += n * mm;
     L.Pt = off; off += n * mm;
     L.Qt = off; off += n * mm;
-    L.b = off; off += static_cast<long long>(n) * m;
-    L.x = off; off += static_cast<long long>(n) * m;
+    L.b = voff; voff += static_cast<long long>(n) * m;
+    L.x = voff; voff += static_cast<long long>(n) * m;
     lv.push_back(L);
     if (n == 1) break;
     n = (n + 1) / 2;
   }
   wsize = off;
+  vsize = voff;
   // Job tables.
   std::vector<long long> inv_offs;
   std::vector<CrGemm> gemm_p, gemm_d;
@@ -442,7 +456,9 @@ bool CoarseCR::plan(const Geom& g) {
     maxjobs = std::max(maxjobs, static_cast<size_t>(lv_fw[r + 1] - lv_fw[r]));
     maxjobs = std::max(maxjobs, static_cast<size_t>(lv_bw[r + 1] - lv_bw[r]));
   }
-  tmp.ensure(maxjobs * GV_KS * 256);
+  tsize = static_cast<long long>(maxjobs) * GV_KS * 256;
+  ncases = 0;
+  reserve_cases(1);
   size_t maxinv = 1;
   for (size_t r = 0; r + 1 < lv_inv.size(); ++r)
     maxinv = std::max(maxinv, static_cast<size_t>(lv_inv[r + 1] - lv_inv[r]));
@@ -492,29 +508,41 @@ void CoarseCR::build(const double* Kem, const uint8_t* mask, hipStream_t s) {
   }
 }
 
-void CoarseCR::gemv(const CrGemv* jobs, int n, const int* done, hipStream_t s) {
+void CoarseCR::reserve_cases(int nr) {
+  if (nr <= ncases) return;
+  V.ensure(static_cast<size_t>(vsize) * nr);
+  tmp.ensure(static_cast<size_t>(tsize) * nr);
+  ncases = nr;
+}
+
+void CoarseCR::gemv(const CrGemv* jobs, int n, const int* done, hipStream_t s, int nr) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(gemv_partial_k, dim3(n, GV_KS), dim3(256), 0, s, W.p, jobs, m, tmp.p, done);
-  hipLaunchKernelGGL(gemv_reduce_k, dim3(n), dim3(256), 0, s, W.p, jobs, m, tmp.p, done);
+  hipLaunchKernelGGL(gemv_partial_k, dim3(n, GV_KS, nr), dim3(256), 0, s, W.p, V.p, vsize, jobs,
+                     m, tmp.p, tsize, done);
+  hipLaunchKernelGGL(gemv_reduce_k, dim3(n, nr), dim3(256), 0, s, V.p, vsize, jobs, m, tmp.p,
+                     tsize, done);
 }
 
 template <typename T>
-void CoarseCR::solve(const T* b, T* x, const int* done, hipStream_t s) {
+void CoarseCR::solve(const T* b, T* x, const int* done, hipStream_t s, int nr) {
+  if (nr > ncases) throw Error(TOPOHIP_EINTERNAL, "coarse solve: load-case space not reserved");
   const long long n = 3 * geom.nodes();
-  const unsigned nb = static_cast<unsigned>((n + 255) / 256);
-  hipLaunchKernelGGL((cr_gather_k<T>), dim3(nb), dim3(256), 0, s, b, d_dst.p, n, W.p, lv[0].b, done);
+  const dim3 grid(static_cast<unsigned>((n + 255) / 256), nr);
+  hipLaunchKernelGGL((cr_gather_k<T>), grid, dim3(256), 0, s, b, d_dst.p, n, V.p, vsize, lv[0].b,
+                     done);
   for (size_t r = 0; r + 1 < lv.size(); ++r) {
     const int nf = lv_fw[r + 1] - lv_fw[r];
-    gemv(d_fw.p + lv_fw[r], nf, done, s);
+    gemv(d_fw.p + lv_fw[r], nf, done, s, nr);
   }
   for (int r = static_cast<int>(lv.size()) - 1; r >= 0; --r) {
     const int nbw = lv_bw[r + 1] - lv_bw[r];
-    gemv(d_bw.p + lv_bw[r], nbw, done, s);
+    gemv(d_bw.p + lv_bw[r], nbw, done, s, nr);
   }
-  hipLaunchKernelGGL((cr_scatter_k<T>), dim3(nb), dim3(256), 0, s, W.p, lv[0].x, d_dst.p, n, x, done);
+  hipLaunchKernelGGL((cr_scatter_k<T>), grid, dim3(256), 0, s, V.p, vsize, lv[0].x, d_dst.p, n, x,
+                     done);
   TOPOHIP_CHECK(hipGetLastError());
 }
-template void CoarseCR::solve<float>(const float*, float*, const int*, hipStream_t);
-template void CoarseCR::solve<double>(const double*, double*, const int*, hipStream_t);
+template void CoarseCR::solve<float>(const float*, float*, const int*, hipStream_t, int);
+template void CoarseCR::solve<double>(const double*, double*, const int*, hipStream_t, int);
 
 }  // namespace topohip

@@ -18,7 +18,8 @@ struct CrGemm {
   double beta;
   int nterms;
 };
-// out = beta*y0 + sum_t alpha_t op(M_t) v_t on length-m vectors.
+// out = beta*y0 + sum_t alpha_t op(M_t) v_t on length-m vectors (matrix
+// offsets index W, vector offsets a load case's vector space V).
 struct CrGemv {
   long long out, y0;
   long long m[3], v[3];
@@ -29,7 +30,8 @@ struct CrGemv {
 };
 struct CrLevel {
   int n = 0;  // planes
-  long long D = 0, L = 0, Dinv = 0, P = 0, Q = 0, Pt = 0, Qt = 0, b = 0, x = 0;
+  long long D = 0, L = 0, Dinv = 0, P = 0, Q = 0, Pt = 0, Qt = 0;  // in W
+  long long b = 0, x = 0;                                           // in V
 };
 
 class CoarseCR {
@@ -39,21 +41,27 @@ class CoarseCR {
   bool plan(const Geom& g);
   // Factor from the level's element-major FP64 element matrices.
   void build(const double* Kem, const uint8_t* mask, hipStream_t s);
-  // x = A^-1 b (node-ordered level vectors, FP64 arithmetic inside).
+  // Vector space for nr load cases solved together.
+  void reserve_cases(int nr);
+  // x = A^-1 b for nr stacked load cases (node-ordered level vectors, stride
+  // 3*nodes; FP64 arithmetic inside).
   template <typename T>
-  void solve(const T* b, T* x, const int* done, hipStream_t s);
+  void solve(const T* b, T* x, const int* done, hipStream_t s, int nr = 1);
   int block() const { return m; }
   int nplanes() const { return planes; }
-  size_t bytes() const { return static_cast<size_t>(wsize) * sizeof(double); }
+  size_t bytes() const {
+    return static_cast<size_t>(wsize + (vsize + tsize) * ncases) * sizeof(double);
+  }
 
  private:
   Geom geom;
   int axis = 0, planes = 0, m = 0;
-  long long wsize = 0;
+  long long wsize = 0, vsize = 0, tsize = 0;
+  int ncases = 0;
   std::vector<CrLevel> lv;
   std::vector<int> lv_inv, lv_gp, lv_gd, lv_fw, lv_bw;
-  void gemv(const CrGemv* jobs, int n, const int* done, hipStream_t s);
-  DBuf<double> W, tmp, inv_scratch;
+  void gemv(const CrGemv* jobs, int n, const int* done, hipStream_t s, int nr);
+  DBuf<double> W, V, tmp, inv_scratch;
   DBuf<long long> d_inv;
   DBuf<CrGemm> d_gp, d_gd;
   DBuf<CrGemv> d_fw, d_bw;

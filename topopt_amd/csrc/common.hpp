@@ -57,6 +57,19 @@ inline Geom make_geom(int ex, int ey, int ez) {
   X(44, 23, 23)
 constexpr int kModalNnz = 45;
 
+// The 45 modal entries take 8 values, by the cubic symmetry of the isotropic
+// element (index r = 3*mode + component, mode bits x + 2y + 4z):
+//   normal modes {3, 7, 14}:            [[a,b,b],[b,a,b],[b,b,a]]
+//   shear pairs {4,6} {5,12} {8,13}:    d * [[1,1],[1,1]]
+//   pairs {9,20} {10,17} {15,19}:       [[e,f],[f,e]]
+//   triple {11,16,18}:                  [[g,h,h],[h,g,h],[h,h,g]]
+//   modes {21,22,23}:                   k * I
+// stored as (a - b, b, d, e, f, g - h, h, k), already divided by 64.
+template <typename T>
+struct ModalGroups {
+  T amb, b, d, e, f, gmh, h, k;
+};
+
 template <typename T>
 struct Modal {
   T c[kModalNnz];  // already divided by 64 (the Hadamard normalisation)
@@ -77,5 +90,15 @@ struct CgState {
   int iters;      // completed iterations
   int cap;        // iteration cap (fail = 2 when reached unconverged)
 };
+
+// Load cases solved together (multi-RHS PCG): case r's vectors start at
+// base + r * stride, its state is CgState[r], and kernels that take a `done`
+// pointer (&st[0].done) skip case r once done[r * kDoneStride] is set. The
+// case index is a grid dimension, so one launch covers every case.
+constexpr int kDoneStride = static_cast<int>(sizeof(CgState) / sizeof(int));
+static_assert(sizeof(CgState) % sizeof(int) == 0, "CgState must be int-strided");
+__host__ __device__ inline bool case_done(const int* done, int r) {
+  return done && done[r * kDoneStride];
+}
 
 }  // namespace topohip

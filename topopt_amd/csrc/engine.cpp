@@ -10,11 +10,15 @@
 
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
-#include <cstring>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <initializer_list>
+#include <map>
+#include <mutex>
 #include <random>
 
 namespace topohip {
@@ -161,6 +165,36 @@ void element_constants(Problem& P) {
   for (int i = 0; i < 576; ++i)
     if (!in_pat[i] && std::fabs(M[i]) > 1e-12 * mx)
       throw Error(TOPOHIP_EINTERNAL, "element stiffness outside the modal pattern");
+  // Grouped form used by the K.u kernel; every entry of M is checked against it.
+  {
+    auto m = [&](int r, int c) { return M[r * 24 + c] / 64.0; };
+    const double a = m(3, 3), b = m(3, 7), d = m(4, 4), e = m(9, 9), f = m(9, 20);
+    const double g = m(11, 11), h = m(11, 16), kk = m(21, 21);
+    double R[576] = {};
+    auto set_block = [&](std::initializer_list<int> idx, double diag, double off) {
+      for (int r : idx)
+        for (int c : idx) R[r * 24 + c] = r == c ? diag : off;
+    };
+    set_block({3, 7, 14}, a, b);
+    set_block({4, 6}, d, d);
+    set_block({5, 12}, d, d);
+    set_block({8, 13}, d, d);
+    set_block({9, 20}, e, f);
+    set_block({10, 17}, e, f);
+    set_block({15, 19}, e, f);
+    set_block({11, 16, 18}, g, h);
+    set_block({21}, kk, 0.0);
+    set_block({22}, kk, 0.0);
+    set_block({23}, kk, 0.0);
+    for (int i = 0; i < 576; ++i)
+      if (std::fabs(M[i] / 64.0 - R[i]) > 1e-12 * mx / 64.0)
+        throw Error(TOPOHIP_EINTERNAL, "element stiffness does not have the grouped modal form");
+    P.mg64 = ModalGroups<double>{a - b, b, d, e, f, g - h, h, kk};
+    P.mg32 = ModalGroups<float>{static_cast<float>(a - b), static_cast<float>(b),
+                                static_cast<float>(d), static_cast<float>(e),
+                                static_cast<float>(f), static_cast<float>(g - h),
+                                static_cast<float>(h), static_cast<float>(kk)};
+  }
   for (int c = 0; c < 8; ++c) {
     double W[576];
     for (int a = 0; a < 24; ++a)
@@ -267,9 +301,9 @@ template <> double* Tv<double>(Level& l) { return l.t64.p; }
 template <typename T> const T* Ev(Problem& P);
 template <> const float* Ev<float>(Problem& P) { return P.E32.p; }
 template <> const double* Ev<double>(Problem& P) { return P.E64.p; }
-template <typename T> const Modal<T>& Mc(Problem& P);
-template <> const Modal<float>& Mc<float>(Problem& P) { return P.mc32; }
-template <> const Modal<double>& Mc<double>(Problem& P) { return P.mc64; }
+template <typename T> const ModalGroups<T>& Mg(Problem& P);
+template <> const ModalGroups<float>& Mg<float>(Problem& P) { return P.mg32; }
+template <> const ModalGroups<double>& Mg<double>(Problem& P) { return P.mg64; }
 
 void alloc_levels(Problem& P) {
   const bool mixed = P.precision == TOPOHIP_PREC_MIXED;
@@ -308,70 +342,81 @@ void alloc_levels(Problem& P) {
       }
       const bool coarsest = l + 1 == P.L.size();
       if (!coarsest) {
-        if (mixed) lv.K32.ensure(static_cast<size_t>(lv.g.nodes()) * 243);
-        else lv.K64.ensure(static_cast<size_t>(lv.g.nodes()) * 243);
+        // symmetric half stencil: centre + 13 neighbour blocks of 3x3
+        if (mixed) lv.K32.ensure(static_cast<size_t>(lv.g.nodes()) * 126);
+        else lv.K64.ensure(static_cast<size_t>(lv.g.nodes()) * 126);
       }
-      if (mixed) { lv.b32.ensure(nd); lv.x32.ensure(nd); lv.t32.ensure(nd); }
-      else { lv.b64.ensure(nd); lv.x64.ensure(nd); lv.t64.ensure(nd); }
-    } else {
-      if (mixed) { lv.x32.ensure(nd); lv.t32.ensure(nd); }
-      else { lv.x64.ensure(nd); lv.t64.ensure(nd); }
     }
   }
   if (P.mg_plan_ok && !P.cr.plan(P.L.back()->g)) P.mg_plan_ok = false;
 }
 
+int max_partials(const Problem& P) {
+  return std::max(ku_blocks(P.g, P.tz, P.tx), std::max(dot_blocks(P.g.ndof()), sens_blocks(P.g)));
+}
+
 // ---- V-cycle ----------------------------------------------------------------
+// All level vectors hold nc stacked load cases (stride: the level's ndof);
+// every launch covers the nc cases, and a case whose PCG is done is skipped.
 // Coarse levels l >= 1: restrict the level-(l-1) residual rprev, smooth,
 // recurse, prolong, smooth. Result lives in t_l (x_l at the coarsest).
 template <typename T>
-void coarse_cycle(Problem& P, size_t l, const T* rprev, const int* done) {
+void coarse_cycle(Problem& P, size_t l, const T* rprev, const int* done, int nc) {
   Level& f = *P.L[l - 1];
   Level& c = *P.L[l];
   const bool coarsest = l + 1 == P.L.size();
   if (coarsest) {
-    launch_restrict<T, T>(f.g, c.g, rprev, c.mask.p, Bv<T>(c), nullptr, nullptr, 0.0, done, P.s);
-    P.cr.solve<T>(Bv<T>(c), Xv<T>(c), done, P.s);
+    launch_restrict<T, T>(f.g, c.g, rprev, c.mask.p, Bv<T>(c), nullptr, nullptr, 0.0, done, P.s,
+                          nc);
+    P.cr.solve<T>(Bv<T>(c), Xv<T>(c), done, P.s, nc);
     return;
   }
   launch_restrict<T, T>(f.g, c.g, rprev, c.mask.p, Bv<T>(c), Xv<T>(c), Dinv<T>(c), kOmega, done,
-                        P.s);
+                        P.s, nc);
   launch_stencil_apply<T>(c.g, Kapp<T>(c), c.mask.p, Xv<T>(c), Tv<T>(c), Bv<T>(c), nullptr, 0.0,
-                          EPI_RES, done, P.s);
-  coarse_cycle<T>(P, l + 1, Tv<T>(c), done);
+                          EPI_RES, done, P.s, nc);
+  coarse_cycle<T>(P, l + 1, Tv<T>(c), done, nc);
   Level& n = *P.L[l + 1];
   const bool ncoarsest = l + 2 == P.L.size();
   launch_prolong_add<T, T>(n.g, c.g, ncoarsest ? Xv<T>(n) : Tv<T>(n), c.mask.p, Xv<T>(c), done,
-                           P.s);
+                           P.s, nc);
   launch_stencil_apply<T>(c.g, Kapp<T>(c), c.mask.p, Xv<T>(c), Tv<T>(c), Bv<T>(c), Dinv<T>(c),
-                          kOmega, EPI_JAC, done, P.s);
+                          kOmega, EPI_JAC, done, P.s, nc);
+}
+
+template <typename T>
+KuOp<T> fine_op(Problem& P, const int* done, int nc) {
+  KuOp<T> op;
+  op.g = P.g;
+  op.mask = P.L[0]->mask.p;
+  op.E = Ev<T>(P);
+  op.mg = Mg<T>(P);
+  op.tz = P.tz;
+  op.tx = P.tx;
+  op.done = done;
+  op.nrhs = nc;
+  op.pstride = P.pstride;
+  return op;
 }
 
 // z = M^{-1} b0 on the fine level (mf_v_cycle with 1+1 damped Jacobi sweeps;
 // the first sweep starts from zero so it is the scaling omega D^-1 b0).
 template <typename T>
-void vcycle(Problem& P, const T* b0, T* z, const int* done) {
+void vcycle(Problem& P, const T* b0, T* z, const int* done, int nc) {
   Level& f = *P.L[0];
   const long long nd = f.g.ndof();
-  launch_jacobi0<T>(Xv<T>(f), Dinv<T>(f), b0, kOmega, nd, done, P.s);
-  KuOp<T> op;
-  op.g = f.g;
-  op.mask = f.mask.p;
-  op.E = Ev<T>(P);
-  op.mc = Mc<T>(P);
-  op.tz = P.tz;
-  op.tx = P.tx;
-  op.done = done;
+  launch_jacobi0<T>(Xv<T>(f), Dinv<T>(f), b0, kOmega, nd, done, P.s, nc);
+  KuOp<T> op = fine_op<T>(P, done, nc);
   op.x = Xv<T>(f);
   op.b = b0;
   op.y = Tv<T>(f);
   op.epi = EPI_RES;
   launch_ku_fine<T>(op, P.s);
-  coarse_cycle<T>(P, 1, Tv<T>(f), done);
+  coarse_cycle<T>(P, 1, Tv<T>(f), done, nc);
   Level& c = *P.L[1];
   const bool ccoarsest = P.L.size() == 2;
   launch_prolong_add<T, T>(c.g, f.g, ccoarsest ? Xv<T>(c) : Tv<T>(c), f.mask.p, Xv<T>(f), done,
-                           P.s);
+                           P.s, nc);
   op.epi = EPI_JAC;
   op.dinv = Dinv<T>(f);
   op.omega = static_cast<T>(kOmega);
@@ -381,15 +426,8 @@ void vcycle(Problem& P, const T* b0, T* z, const int* done) {
 
 template <typename T>
 void fine_apply(Problem& P, const T* x, T* y, const T* b, int epi, double* partials,
-                const int* done) {
-  KuOp<T> op;
-  op.g = P.g;
-  op.mask = P.L[0]->mask.p;
-  op.E = Ev<T>(P);
-  op.mc = Mc<T>(P);
-  op.tz = P.tz;
-  op.tx = P.tx;
-  op.done = done;
+                const int* done, int nc) {
+  KuOp<T> op = fine_op<T>(P, done, nc);
   op.x = x;
   op.y = y;
   op.b = b;
@@ -398,143 +436,204 @@ void fine_apply(Problem& P, const T* x, T* y, const T* b, int epi, double* parti
   launch_ku_fine<T>(op, P.s);
 }
 
-int max_partials(const Problem& P) {
-  return std::max(ku_blocks(P.g, P.tz, P.tx), std::max(dot_blocks(P.g.ndof()), sens_blocks(P.g)));
-}
-
 enum class Prec { MG32, MG64, JACOBI };
 
+const int* done_flags(Problem& P) { return &P.st.p->done; }
+
 // Preconditioner application z = M^{-1} r. Mixed: r32 must hold float(r).
-void precondition(Problem& P, Prec pc, const int* done) {
+void precondition(Problem& P, Prec pc, const int* done, int nc) {
   const long long nd = P.g.ndof();
-  if (pc == Prec::MG32) vcycle<float>(P, P.r32.p, P.z32.p, done);
-  else if (pc == Prec::MG64) vcycle<double>(P, P.r.p, P.z64.p, done);
-  else launch_jacobi0<double>(P.z64.p, P.L[0]->dinv64.p, P.r.p, 1.0, nd, done, P.s);
+  if (pc == Prec::MG32) vcycle<float>(P, P.r32.p, P.z32.p, done, nc);
+  else if (pc == Prec::MG64) vcycle<double>(P, P.r.p, P.z64.p, done, nc);
+  else launch_jacobi0<double>(P.z64.p, P.L[0]->dinv64.p, P.r.p, 1.0, nd, done, P.s, nc);
 }
 
-void dot_rz(Problem& P, Prec pc, const int* done) {
+void dot_rz(Problem& P, Prec pc, const int* done, int nc) {
   const long long nd = P.g.ndof();
   const int nb = dot_blocks(nd);
-  if (pc == Prec::MG32) launch_dot<double, float>(P.r.p, P.z32.p, nd, P.partials.p, nb, done, P.s);
-  else launch_dot<double, double>(P.r.p, P.z64.p, nd, P.partials.p, nb, done, P.s);
+  if (pc == Prec::MG32)
+    launch_dot<double, float>(P.r.p, P.z32.p, nd, P.partials.p, nb, done, P.s, nc, P.pstride);
+  else
+    launch_dot<double, double>(P.r.p, P.z64.p, nd, P.partials.p, nb, done, P.s, nc, P.pstride);
 }
 
-// Restart the recurrence from the current r: z = M^-1 r, rz = r.z, p = z.
-void restart(Problem& P, Prec pc) {
+// Restart the recurrence of every case that is not done from its current r:
+// z = M^-1 r, rz = r.z, p = z.
+void restart(Problem& P, Prec pc, int nc) {
   const long long nd = P.g.ndof();
-  if (pc != Prec::JACOBI) P.matvecs += 2;
-  const int* done = &P.st.p->done;
-  if (pc == Prec::MG32) launch_convert<double, float>(P.r.p, P.r32.p, nd, P.s);
-  precondition(P, pc, done);
-  dot_rz(P, pc, done);
-  launch_cg_set_rz(P.st.p, P.partials.p, dot_blocks(nd), P.s);
-  if (pc == Prec::MG32) launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, P.p32.p, nd, 1, P.s);
-  else launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nullptr, nd, 1, P.s);
+  const int* done = done_flags(P);
+  if (pc == Prec::MG32) launch_convert<double, float>(P.r.p, P.r32.p, nd * nc, P.s);
+  precondition(P, pc, done, nc);
+  dot_rz(P, pc, done, nc);
+  launch_cg_set_rz(P.st.p, P.partials.p, dot_blocks(nd), P.s, nc, P.pstride);
+  if (pc == Prec::MG32) launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, P.p32.p, nd, 1, P.s, nc);
+  else launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nullptr, nd, 1, P.s, nc);
 }
 
-void iteration(Problem& P, Prec pc) {
+void iteration(Problem& P, Prec pc, int nc) {
   const long long nd = P.g.ndof();
-  const int* done = &P.st.p->done;
+  const int* done = done_flags(P);
   const int kb = ku_blocks(P.g, P.tz, P.tx);
   const int db = dot_blocks(nd);
   if (pc == Prec::MG32) {
-    fine_apply<float>(P, P.p32.p, P.Ap32.p, nullptr, EPI_AX, P.partials.p, done);
-    launch_cg_alpha(P.st.p, P.partials.p, kb, P.s);
+    fine_apply<float>(P, P.p32.p, P.Ap32.p, nullptr, EPI_AX, P.partials.p, done, nc);
+    launch_cg_alpha(P.st.p, P.partials.p, kb, P.s, nc, P.pstride);
     launch_cg_xr<float, float>(P.st.p, P.x.p, P.r.p, P.p32.p, P.Ap32.p, P.r32.p, nd, P.partials.p,
-                               db, P.s);
+                               db, P.s, nc, P.pstride);
   } else {
-    fine_apply<double>(P, P.p.p, P.Ap64.p, nullptr, EPI_AX, P.partials.p, done);
-    launch_cg_alpha(P.st.p, P.partials.p, kb, P.s);
+    fine_apply<double>(P, P.p.p, P.Ap64.p, nullptr, EPI_AX, P.partials.p, done, nc);
+    launch_cg_alpha(P.st.p, P.partials.p, kb, P.s, nc, P.pstride);
     launch_cg_xr<double, double>(P.st.p, P.x.p, P.r.p, P.p.p, P.Ap64.p, nullptr, nd, P.partials.p,
-                                 db, P.s);
+                                 db, P.s, nc, P.pstride);
   }
-  launch_cg_check(P.st.p, P.partials.p, db, P.s);
-  precondition(P, pc, done);
-  dot_rz(P, pc, done);
-  launch_cg_beta(P.st.p, P.partials.p, db, P.s);
-  if (pc == Prec::MG32) launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, P.p32.p, nd, 0, P.s);
-  else launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nullptr, nd, 0, P.s);
+  launch_cg_check(P.st.p, P.partials.p, db, P.s, nc, P.pstride);
+  precondition(P, pc, done, nc);
+  dot_rz(P, pc, done, nc);
+  launch_cg_beta(P.st.p, P.partials.p, db, P.s, nc, P.pstride);
+  if (pc == Prec::MG32) launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, P.p32.p, nd, 0, P.s, nc);
+  else launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nullptr, nd, 0, P.s, nc);
 }
 
-void read_state(Problem& P) { d2h(P.st_h, P.st.p, 1, P.s); }
+void read_state(Problem& P, int nc) { d2h(P.st_h, P.st.p, static_cast<size_t>(nc), P.s); }
 
-// rt = b - A x in FP64; returns ||rt||^2.
-double true_residual(Problem& P) {
+// rt = b - A x in FP64 for every case; scal_h[r] = ||rt_r||^2.
+void true_residual(Problem& P, int nc) {
   const long long nd = P.g.ndof();
-  P.matvecs += 1;
-  fine_apply<double>(P, P.x.p, P.rt.p, P.b.p, EPI_RES, nullptr, nullptr);
-  launch_dot<double, double>(P.rt.p, P.rt.p, nd, P.partials.p, dot_blocks(nd), nullptr, P.s);
-  launch_sum_partials(P.partials.p, dot_blocks(nd), P.scal.p, P.s);
-  d2h(P.scal_h, P.scal.p, 1, P.s);
-  return P.scal_h[0];
+  P.matvecs += nc;
+  fine_apply<double>(P, P.x.p, P.rt.p, P.b.p, EPI_RES, nullptr, nullptr, nc);
+  launch_dot<double, double>(P.rt.p, P.rt.p, nd, P.partials.p, dot_blocks(nd), nullptr, P.s, nc,
+                             P.pstride);
+  launch_sum_partials(P.partials.p, dot_blocks(nd), P.scal.p, P.s, nc, P.pstride);
+  d2h(P.scal_h, P.scal.p, static_cast<size_t>(nc), P.s);
 }
 
-// PCG on the device. Returns true when ||b - A x|| <= tol ||b|| in FP64.
-// x must hold the start vector. iters/resid report the attempt.
-bool pcg(Problem& P, Prec pc, double tol, int cap, int& iters, double& resid, bool warm) {
+void set_done(Problem& P, int r, int v) {
+  TOPOHIP_CHECK(hipMemcpyAsync(&P.st.p[r].done, &v, sizeof(int), hipMemcpyHostToDevice, P.s));
+  sync(P);
+}
+
+// Block PCG over nc stacked load cases. Each case keeps its own recurrence
+// (alpha, beta, stopping test, breakdown checks, as mf_mgpcg,
+// core/src/fea/multigrid.cpp:1211-1318), but every operator apply, V-cycle
+// and vector kernel is one launch over all cases, and a converged case drops
+// out of the launches. x must hold the start vectors. Cases with skip[r] are
+// left untouched and count as solved. A case is solved when its FP64 true
+// residual meets ||b - A x|| <= tol ||b||; the mixed form restarts a case
+// whose true residual misses it from r = b - A x (residual replacement).
+void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool* skip,
+         int* iters, double* resid, bool* ok) {
   const long long nd = P.g.ndof();
   const int db = dot_blocks(nd);
-  // r = b - A x (warm) or b.
   if (warm) {
-    fine_apply<double>(P, P.x.p, P.r.p, P.b.p, EPI_RES, nullptr, nullptr);
+    fine_apply<double>(P, P.x.p, P.r.p, P.b.p, EPI_RES, nullptr, nullptr, nc);
   } else {
-    TOPOHIP_CHECK(hipMemcpyAsync(P.r.p, P.b.p, nd * sizeof(double), hipMemcpyDeviceToDevice, P.s));
+    TOPOHIP_CHECK(hipMemcpyAsync(P.r.p, P.b.p, nd * nc * sizeof(double),
+                                 hipMemcpyDeviceToDevice, P.s));
   }
-  launch_dot<double, double>(P.b.p, P.b.p, nd, P.partials.p, db, nullptr, P.s);
-  launch_cg_init(P.st.p, P.partials.p, db, tol, cap, P.s);
+  launch_dot<double, double>(P.b.p, P.b.p, nd, P.partials.p, db, nullptr, P.s, nc, P.pstride);
+  launch_cg_init(P.st.p, P.partials.p, db, tol, cap, P.s, nc, P.pstride);
   if (warm) {
-    launch_dot<double, double>(P.r.p, P.r.p, nd, P.partials.p, db, nullptr, P.s);
-    launch_cg_resume(P.st.p, P.partials.p, db, P.s);
+    launch_dot<double, double>(P.r.p, P.r.p, nd, P.partials.p, db, nullptr, P.s, nc, P.pstride);
+    launch_cg_resume(P.st.p, P.partials.p, db, P.s, nc, P.pstride);
   }
-  restart(P, pc);
-  read_state(P);
-  iters = 0;
-  resid = 0.0;
-  if (P.st_h->bnorm2 == 0.0) {
-    launch_fill<double>(P.x.p, 0.0, nd, P.s);
+  for (int r = 0; r < nc; ++r)
+    if (skip && skip[r]) set_done(P, r, 1);
+  restart(P, pc, nc);
+  read_state(P, nc);
+  for (int r = 0; r < nc; ++r) {
+    iters[r] = 0;
+    resid[r] = 0.0;
+    ok[r] = skip && skip[r];
+    if (!ok[r] && P.st_h[r].bnorm2 == 0.0) {
+      launch_fill<double>(P.x.p + r * nd, 0.0, nd, P.s);
+      ok[r] = true;  // done was set by the init
+    }
+  }
+  auto all_done = [&] {
+    for (int r = 0; r < nc; ++r)
+      if (!P.st_h[r].done) return false;
     return true;
-  }
+  };
   const int batch = 4;
-  // The iteration body is captured once per preconditioner kind into a HIP
-  // graph (all pointers are fixed for the problem's lifetime) and replayed;
-  // every kernel in it returns at once when the device-side done flag is set.
-  hipGraphExec_t& gx = P.cg_graph[static_cast<int>(pc)];
+  // The iteration body is captured once per (preconditioner kind, case
+  // count) into a HIP graph (all pointers are fixed while the capacity
+  // stands) and replayed; every kernel in it skips cases whose done flag is set.
+  // Relaxed capture: the body makes no synchronising calls, and other threads
+  // (other problems of the serving loop) keep allocating and copying meanwhile.
+  const int gi = static_cast<int>(pc);
+  hipGraphExec_t& gx = P.cg_graph[gi];
+  if (gx && P.cg_graph_nc[gi] != nc) {
+    (void)hipGraphExecDestroy(gx);
+    gx = nullptr;
+  }
   if (!gx) {
     hipGraph_t gr;
-    TOPOHIP_CHECK(hipStreamBeginCapture(P.s, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < batch; ++i) iteration(P, pc);
+    TOPOHIP_CHECK(hipStreamBeginCapture(P.s, hipStreamCaptureModeRelaxed));
+    for (int i = 0; i < batch; ++i) iteration(P, pc, nc);
     TOPOHIP_CHECK(hipStreamEndCapture(P.s, &gr));
     TOPOHIP_CHECK(hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0));
     TOPOHIP_CHECK(hipGraphDestroy(gr));
+    P.cg_graph_nc[gi] = nc;
   }
+  const long long per_it = pc == Prec::JACOBI ? 1 : 3;
   for (long long guard = 0;; ++guard) {
     if (guard > 4LL * cap + 64) throw Error(TOPOHIP_EINTERNAL, "pcg: iteration state stalled");
-    if (!P.st_h->done) {
-      const int before = P.st_h->iters;
+    if (!all_done()) {
+      long long before = 0;
+      for (int r = 0; r < nc; ++r) before += P.st_h[r].iters;
       TOPOHIP_CHECK(hipGraphLaunch(gx, P.s));
-      read_state(P);
-      P.matvecs += static_cast<long long>(P.st_h->iters - before) * (pc == Prec::JACOBI ? 1 : 3);
-      if (!P.st_h->done) continue;
+      read_state(P, nc);
+      long long after = 0;
+      for (int r = 0; r < nc; ++r) after += P.st_h[r].iters;
+      P.matvecs += (after - before) * per_it;
+      if (!all_done()) continue;
     }
-    iters = P.st_h->iters;
-    if (P.st_h->fail) {
-      resid = std::sqrt(P.st_h->rr / P.st_h->bnorm2);
-      return false;
+    // Every recurrence has stopped: confirm with the FP64 true residuals.
+    bool any_pending = false;
+    for (int r = 0; r < nc; ++r)
+      if (!ok[r] && !P.st_h[r].fail) any_pending = true;
+    if (!any_pending) break;
+    true_residual(P, nc);
+    bool restarted = false;
+    for (int r = 0; r < nc; ++r) {
+      if (ok[r]) continue;
+      const CgState& st = P.st_h[r];
+      iters[r] = st.iters;
+      if (st.fail) {
+        resid[r] = std::sqrt(st.rr / st.bnorm2);
+        continue;
+      }
+      const double rr = P.scal_h[r];
+      resid[r] = std::sqrt(rr / st.bnorm2);
+      if (!std::isfinite(rr)) {
+        set_done(P, r, 1);
+        P.st_h[r].fail = 1;
+        continue;
+      }
+      if (rr <= st.thresh2) {
+        ok[r] = true;
+        continue;
+      }
+      if (st.iters >= cap) {
+        P.st_h[r].fail = 2;
+        continue;
+      }
+      // Residual replacement for this case: continue from r = b - A x.
+      ++P.replacements;
+      TOPOHIP_CHECK(hipMemcpyAsync(P.r.p + r * nd, P.rt.p + r * nd, nd * sizeof(double),
+                                   hipMemcpyDeviceToDevice, P.s));
+      launch_dot<double, double>(P.r.p + r * nd, P.r.p + r * nd, nd,
+                                 P.partials.p + static_cast<long long>(r) * P.pstride, db,
+                                 nullptr, P.s);
+      launch_cg_resume(P.st.p + r, P.partials.p + static_cast<long long>(r) * P.pstride, db, P.s);
+      restarted = true;
     }
-    // Recurrence converged: confirm with the FP64 true residual.
-    const double rr = true_residual(P);
-    resid = std::sqrt(rr / P.st_h->bnorm2);
-    if (!std::isfinite(rr)) return false;
-    if (rr <= P.st_h->thresh2) return true;
-    if (iters >= cap) return false;
-    // Residual replacement: continue from r = b - A x.
-    ++P.replacements;
-    TOPOHIP_CHECK(hipMemcpyAsync(P.r.p, P.rt.p, nd * sizeof(double), hipMemcpyDeviceToDevice, P.s));
-    launch_dot<double, double>(P.r.p, P.r.p, nd, P.partials.p, db, nullptr, P.s);
-    launch_cg_resume(P.st.p, P.partials.p, db, P.s);
-    restart(P, pc);
-    read_state(P);
+    if (!restarted) break;
+    restart(P, pc, nc);
+    read_state(P, nc);
   }
+  for (int r = 0; r < nc; ++r)
+    if (!(skip && skip[r])) iters[r] = P.st_h[r].iters;
 }
 
 }  // namespace
@@ -550,6 +649,21 @@ void autotune_ku(Problem& P) {
     return;
   }
   if (P.g.elems() < 4096) return;
+  // The choice is cached per (device, grid, precision) for the life of the
+  // process: the tile decides the order of the PCG's partial dot products, so
+  // problems of one shape then give identical results whatever else runs.
+  static std::mutex mu;
+  static std::map<std::array<int, 5>, std::pair<int, int>> cache;
+  const std::array<int, 5> key{P.device, P.g.ex, P.g.ey, P.g.ez, P.precision};
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      P.tx = it->second.first;
+      P.tz = it->second.second;
+      return;
+    }
+  }
   const bool mixed = P.precision == TOPOHIP_PREC_MIXED;
   const long long nd = P.g.ndof();
   launch_fill<double>(P.E64.p, 0.5, P.g.elems(), P.s);
@@ -567,12 +681,12 @@ void autotune_ku(Problem& P) {
       auto run = [&]() {
         if (mixed) {
           KuOp<float> op;
-          op.g = P.g; op.E = P.E32.p; op.mc = P.mc32; op.tz = tz; op.tx = tx;
+          op.g = P.g; op.E = P.E32.p; op.mg = P.mg32; op.tz = tz; op.tx = tx;
           op.x = P.p32.p; op.y = P.Ap32.p; op.epi = EPI_AX;
           launch_ku_fine<float>(op, P.s);
         } else {
           KuOp<double> op;
-          op.g = P.g; op.E = P.E64.p; op.mc = P.mc64; op.tz = tz; op.tx = tx;
+          op.g = P.g; op.E = P.E64.p; op.mg = P.mg64; op.tz = tz; op.tx = tx;
           op.x = P.x.p; op.y = P.Ap64.p; op.epi = EPI_AX;
           launch_ku_fine<double>(op, P.s);
         }
@@ -588,6 +702,59 @@ void autotune_ku(Problem& P) {
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  std::lock_guard<std::mutex> lk(mu);
+  auto ins = cache.emplace(key, std::make_pair(P.tx, P.tz));
+  P.tx = ins.first->second.first;  // a concurrent tuner of the same shape may have won
+  P.tz = ins.first->second.second;
+}
+
+void drop_graphs(Problem& P) {
+  for (auto& g : P.cg_graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+// Capacity for nb stacked load cases: solver vectors, level work vectors,
+// PCG states, partial sums and the coarse solver's vector space. Growing it
+// moves buffers, so the captured PCG graphs are dropped.
+void ensure_cases(Problem& P, int nb) {
+  nb = std::max(nb, 1);
+  const int pstride = max_partials(P) + 8;
+  if (nb <= P.nb && pstride <= P.pstride) return;
+  sync(P);
+  drop_graphs(P);
+  nb = std::max(nb, P.nb);
+  P.pstride = std::max(pstride, P.pstride);
+  const bool mixed = P.precision == TOPOHIP_PREC_MIXED;
+  const size_t nd = static_cast<size_t>(P.g.ndof()) * nb;
+  P.b.ensure(nd); P.x.ensure(nd); P.r.ensure(nd); P.p.ensure(nd); P.rt.ensure(nd);
+  P.u.ensure(nd); P.Ap64.ensure(nd); P.z64.ensure(nd);
+  if (mixed) { P.p32.ensure(nd); P.Ap32.ensure(nd); P.r32.ensure(nd); P.z32.ensure(nd); }
+  for (size_t l = 0; l < P.L.size(); ++l) {
+    Level& lv = *P.L[l];
+    const size_t ndl = static_cast<size_t>(lv.g.ndof()) * nb;
+    if (mixed) { lv.x32.ensure(ndl); lv.t32.ensure(ndl); }
+    else { lv.x64.ensure(ndl); lv.t64.ensure(ndl); }
+    if (l > 0) {
+      if (mixed) lv.b32.ensure(ndl);
+      else lv.b64.ensure(ndl);
+    }
+  }
+  P.st.ensure(static_cast<size_t>(nb));
+  P.scal.ensure(static_cast<size_t>(std::max(8, nb)));
+  if (nb > P.nb || !P.st_h) {
+    if (P.st_h) (void)hipHostFree(P.st_h);
+    if (P.scal_h) (void)hipHostFree(P.scal_h);
+    P.st_h = nullptr;
+    P.scal_h = nullptr;
+    TOPOHIP_CHECK(hipHostMalloc(&P.st_h, sizeof(CgState) * nb));
+    TOPOHIP_CHECK(hipHostMalloc(&P.scal_h, sizeof(double) * std::max(8, nb)));
+  }
+  P.partials.ensure(static_cast<size_t>(nb) * P.pstride);
+  if (P.mg_plan_ok) P.cr.reserve_cases(nb);
+  P.nb = nb;
 }
 
 std::unique_ptr<Problem> create_problem(int nx, int ny, int nz, double h, double nu,
@@ -650,23 +817,13 @@ std::unique_ptr<Problem> create_problem(int nx, int ny, int nz, double h, double
   h2d(P->dkdiag.p, P->kdiag, 24, P->s);
   P->E64.ensure(static_cast<size_t>(nv));
   P->E32.ensure(static_cast<size_t>(nv));
-  const size_t nd = static_cast<size_t>(3 * nn);
-  P->b.ensure(nd); P->x.ensure(nd); P->r.ensure(nd); P->p.ensure(nd); P->rt.ensure(nd);
-  P->u.ensure(nd);
-  if (precision == TOPOHIP_PREC_MIXED) {
-    P->p32.ensure(nd); P->Ap32.ensure(nd); P->r32.ensure(nd); P->z32.ensure(nd);
-  }
-  P->Ap64.ensure(nd); P->z64.ensure(nd);
-  P->st.ensure(1);
-  TOPOHIP_CHECK(hipHostMalloc(&P->st_h, sizeof(CgState)));
   TOPOHIP_CHECK(hipHostMalloc(&P->oc_h, sizeof(OcState)));
-  TOPOHIP_CHECK(hipHostMalloc(&P->scal_h, 8 * sizeof(double)));
-  P->scal.ensure(8);
   P->oc.ensure(1);
   plan_levels(*P);
   alloc_levels(*P);
+  ensure_cases(*P, 1);
   autotune_ku(*P);
-  P->partials.ensure(static_cast<size_t>(max_partials(*P)) + 8);
+  ensure_cases(*P, 1);  // partial sums sized for the tuned tile
   sync(*P);
   return P;
 }
@@ -717,6 +874,7 @@ void set_loads(Problem& P, int n_rhs, const int32_t* counts, const int32_t* node
   if (!P.nonzero_bc)
     for (int r = 0; r < n_rhs; ++r) void_gate(P, P.f_h[r]);
   P.nrhs = n_rhs;
+  ensure_cases(P, n_rhs);
   P.f.ensure(static_cast<size_t>(n_rhs * nd));
   const Level& f0 = *P.L[0];
   for (int r = 0; r < n_rhs; ++r) {
@@ -765,20 +923,21 @@ void build_hierarchy(Problem& P, Timers& t) {
   t.mg_build += now_ms() - t0;
 }
 
-void solve_rhs(Problem& P, int rhs, double tol, int max_it, const double* d_guess,
-               topohip_cginfo* info) {
+void solve_cases(Problem& P, int nc, double tol, int max_it, const double* d_guess,
+                 topohip_cginfo* info) {
   const double t0 = now_ms();
   const long long nn = P.g.nodes(), nd = 3 * nn;
-  if (rhs < 0 || rhs >= P.nrhs) throw Error(TOPOHIP_EINVAL, "no load case set");
+  if (nc < 1 || nc > P.nrhs) throw Error(TOPOHIP_EINVAL, "no load case set");
+  if (d_guess && nc != 1) throw Error(TOPOHIP_EINTERNAL, "warm start needs a single load case");
+  ensure_cases(P, nc);
   Level& f = *P.L[0];
   if (P.nonzero_bc) {
     // rhs = f - K(E) up depends on the moduli: form it on the device, check
     // the void gate, and mask it (core/src/fea/matfree.cpp:745-834).
-    std::vector<double> rhs_full = P.f_h[rhs];
     KuOp<double> op;
     op.g = P.g;
     op.E = P.E64.p;
-    op.mc = P.mc64;
+    op.mg = P.mg64;
     op.tz = P.tz;
     op.tx = P.tx;
     op.x = P.up.p;
@@ -787,21 +946,24 @@ void solve_rhs(Problem& P, int rhs, double tol, int max_it, const double* d_gues
     launch_ku_fine<double>(op, P.s);
     std::vector<double> kup(static_cast<size_t>(nd));
     d2h(kup.data(), P.rt.p, static_cast<size_t>(nd), P.s);
-    for (long long d = 0; d < nd; ++d) rhs_full[d] -= kup[d];
-    void_gate(P, rhs_full);
-    for (long long n = 0; n < nn; ++n)
-      for (int c = 0; c < 3; ++c)
-        if (!((f.mask_h[n] >> c) & 1)) rhs_full[3 * n + c] = 0.0;
-    h2d(P.b.p, rhs_full.data(), static_cast<size_t>(nd), P.s);
+    for (int r = 0; r < nc; ++r) {
+      std::vector<double> rhs_full = P.f_h[r];
+      for (long long d = 0; d < nd; ++d) rhs_full[d] -= kup[d];
+      void_gate(P, rhs_full);
+      for (long long n = 0; n < nn; ++n)
+        for (int c = 0; c < 3; ++c)
+          if (!((f.mask_h[n] >> c) & 1)) rhs_full[3 * n + c] = 0.0;
+      h2d(P.b.p + r * nd, rhs_full.data(), static_cast<size_t>(nd), P.s);
+    }
   } else {
-    // Zero prescribed values: the masked RHS was formed and gated once by set_loads.
-    TOPOHIP_CHECK(hipMemcpyAsync(P.b.p, P.f.p + rhs * nd, nd * sizeof(double),
-                                 hipMemcpyDeviceToDevice, P.s));
+    // Zero prescribed values: the masked RHS were formed and gated by set_loads.
+    TOPOHIP_CHECK(hipMemcpyAsync(P.b.p, P.f.p, nc * nd * sizeof(double), hipMemcpyDeviceToDevice,
+                                 P.s));
   }
-  long long ng = f.nact;
+  const long long ng = f.nact;
   const int cap = max_it > 0 ? max_it : static_cast<int>(std::max<long long>(1000, 2 * ng));
-  topohip_cginfo d{};
-  d.converged = 1;
+  std::vector<topohip_cginfo> d(static_cast<size_t>(nc));
+  for (auto& di : d) di.converged = 1;
   P.replacements = 0;
   const long long mv0 = P.matvecs;
   if (ng > 0) {
@@ -820,56 +982,75 @@ void solve_rhs(Problem& P, int rhs, double tol, int max_it, const double* d_gues
         }
       if (warm) h2d(P.x.p, gh.data(), static_cast<size_t>(nd), P.s);
     }
-    if (!warm) launch_fill<double>(P.x.p, 0.0, nd, P.s);
-    bool solved = false;
-    int it = 0;
-    double res = 0.0;
-    const double tm = now_ms();
-    d.hier_built = P.mg_plan_ok;
+    if (!warm) launch_fill<double>(P.x.p, 0.0, nd * nc, P.s);
+    std::vector<int> it(static_cast<size_t>(nc), 0);
+    std::vector<double> res(static_cast<size_t>(nc), 0.0);
+    std::unique_ptr<bool[]> ok(new bool[nc]());
     if (P.mg_plan_ok) {
+      const double tm = now_ms();
       const Prec pc = P.precision == TOPOHIP_PREC_MIXED ? Prec::MG32 : Prec::MG64;
-      solved = pcg(P, pc, tol, std::min(cap, kMgIterBudget), it, res, warm);
-      d.mg_cycles_attempted = it;
-      if (solved) {
-        d.iterations = it;
-        d.residual = res;
-        d.converged = 1;
-        d.used_multigrid = 1;
-        d.mg_levels = static_cast<int>(P.L.size());
-        d.coarse_dofs = static_cast<int>(P.L.back()->nact);
+      pcg(P, pc, nc, tol, std::min(cap, kMgIterBudget), warm, nullptr, it.data(), res.data(),
+          ok.get());
+      const double tmg = now_ms() - tm;
+      for (int r = 0; r < nc; ++r) {
+        d[r].hier_built = 1;
+        d[r].mg_cycles_attempted = it[r];
+        d[r].t_mg_ms = tmg;
+        if (ok[r]) {
+          d[r].iterations = it[r];
+          d[r].residual = res[r];
+          d[r].used_multigrid = 1;
+          d[r].mg_levels = static_cast<int>(P.L.size());
+          d[r].coarse_dofs = static_cast<int>(P.L.back()->nact);
+        }
       }
-      d.t_mg_ms = now_ms() - tm;
     }
-    if (!solved) {
-      // Exact Jacobi-PCG fallback (core/src/fea/matfree.cpp:841-1076), FP64.
+    bool all_ok = true;
+    for (int r = 0; r < nc; ++r) all_ok = all_ok && ok[r];
+    if (!all_ok) {
+      // Exact Jacobi-PCG fallback (core/src/fea/matfree.cpp:841-1076), FP64, for
+      // the cases the multigrid solve did not finish; the others keep their x.
       const double tc = now_ms();
-      if (warm) h2d(P.x.p, gh.data(), static_cast<size_t>(nd), P.s);
-      else launch_fill<double>(P.x.p, 0.0, nd, P.s);
-      solved = pcg(P, Prec::JACOBI, tol, cap, it, res, warm);
-      d.iterations = it;
-      d.residual = res;
-      d.converged = solved;
-      d.used_multigrid = 0;
-      d.mg_levels = 0;
-      d.t_cg_ms = now_ms() - tc;
-      d.matvecs = P.matvecs - mv0;
-      d.residual_replacements = P.replacements;
-      d.t_total_ms = now_ms() - t0;
-      if (info) *info = d;
-      if (!solved)
-        throw Error(TOPOHIP_ENONCONVERGENCE,
-                    "fea_solve_mgcg_matfree: CG did not reach the requested tolerance within "
-                    "max_iterations");
+      std::unique_ptr<bool[]> skip(new bool[nc]);
+      for (int r = 0; r < nc; ++r) {
+        skip[r] = ok[r];
+        if (ok[r]) continue;
+        if (warm) h2d(P.x.p, gh.data(), static_cast<size_t>(nd), P.s);
+        else launch_fill<double>(P.x.p + r * nd, 0.0, nd, P.s);
+      }
+      pcg(P, Prec::JACOBI, nc, tol, cap, warm, skip.get(), it.data(), res.data(), ok.get());
+      const double tcg = now_ms() - tc;
+      for (int r = 0; r < nc; ++r) {
+        if (skip[r]) continue;
+        d[r].iterations = it[r];
+        d[r].residual = res[r];
+        d[r].converged = ok[r];
+        d[r].used_multigrid = 0;
+        d[r].mg_levels = 0;
+        d[r].t_cg_ms = tcg;
+      }
+      for (int r = 0; r < nc; ++r)
+        if (!ok[r]) {
+          d[r].matvecs = P.matvecs - mv0;
+          d[r].residual_replacements = P.replacements;
+          d[r].t_total_ms = now_ms() - t0;
+          if (info) std::copy(d.begin(), d.end(), info);
+          throw Error(TOPOHIP_ENONCONVERGENCE,
+                      "fea_solve_mgcg_matfree: CG did not reach the requested tolerance within "
+                      "max_iterations");
+        }
     }
   } else {
-    launch_fill<double>(P.x.p, 0.0, nd, P.s);
+    launch_fill<double>(P.x.p, 0.0, nd * nc, P.s);
   }
-  launch_add(P.x.p, P.up.p, P.u.p, nd, P.s);
+  for (int r = 0; r < nc; ++r) launch_add(P.x.p + r * nd, P.up.p, P.u.p + r * nd, nd, P.s);
   sync(P);
-  d.matvecs = P.matvecs - mv0;
-  d.residual_replacements = P.replacements;
-  d.t_total_ms = now_ms() - t0;
-  if (info) *info = d;
+  for (auto& di : d) {
+    di.matvecs = P.matvecs - mv0;
+    di.residual_replacements = P.replacements;
+    di.t_total_ms = now_ms() - t0;
+  }
+  if (info) std::copy(d.begin(), d.end(), info);
 }
 
 FilterCache& filter_for(Problem& P, double radius) {
@@ -930,21 +1111,22 @@ double simp_compliance_dev(Problem& P, const double* d_xphys, double E0, double 
   build_hierarchy(P, t);
   P.dc.ensure(static_cast<size_t>(nv));
   const int sb = sens_blocks(P.g);
+  const long long nd = P.g.ndof();
+  std::vector<topohip_cginfo> ci(static_cast<size_t>(P.nrhs));
+  solve_cases(P, P.nrhs, tol, max_it, nullptr, ci.data());
+  // Multi-load objective: c = sum_r c_r, dc = sum_r dc_r.
   double c = 0.0;
-  topohip_cginfo agg{};
+  topohip_cginfo agg = ci[0];
   for (int r = 0; r < P.nrhs; ++r) {
-    topohip_cginfo ci{};
-    solve_rhs(P, r, tol, max_it, nullptr, &ci);
-    if (r == 0) agg = ci;
-    else {
-      agg.iterations += ci.iterations;
-      agg.matvecs += ci.matvecs;
-      agg.t_total_ms += ci.t_total_ms;
-      agg.residual = std::max(agg.residual, ci.residual);
-      agg.converged = agg.converged && ci.converged;
+    if (r > 0) {
+      agg.iterations = std::max(agg.iterations, ci[r].iterations);
+      agg.mg_cycles_attempted = std::max(agg.mg_cycles_attempted, ci[r].mg_cycles_attempted);
+      agg.residual = std::max(agg.residual, ci[r].residual);
+      agg.converged = agg.converged && ci[r].converged;
+      agg.used_multigrid = agg.used_multigrid && ci[r].used_multigrid;
     }
-    launch_sensitivity(P.g, P.u.p, d_xphys, sol, P.mc64, E0, penalty, rho_min, P.dc.p, r > 0,
-                       P.partials.p, sb, P.s);
+    launch_sensitivity(P.g, P.u.p + r * nd, d_xphys, sol, P.mc64, E0, penalty, rho_min, P.dc.p,
+                       r > 0, P.partials.p, sb, P.s);
     launch_sum_partials(P.partials.p, sb, P.scal.p, P.s);
     d2h(P.scal_h, P.scal.p, 1, P.s);
     c += P.scal_h[0];

@@ -25,8 +25,8 @@ struct Level {
   DBuf<double> K64;
   DBuf<float> dinv32;
   DBuf<double> dinv64;
-  DBuf<float> b32, x32, t32;    // V-cycle work (mixed form)
-  DBuf<double> b64, x64, t64;   // V-cycle work (FP64 form)
+  DBuf<float> b32, x32, t32;    // V-cycle work (mixed form), stacked load cases
+  DBuf<double> b64, x64, t64;   // V-cycle work (FP64 form), stacked load cases
 };
 
 struct FilterCache {
@@ -66,6 +66,8 @@ struct Problem {
   double kdiag[24];
   Modal<float> mc32;
   Modal<double> mc64;
+  ModalGroups<float> mg32;   // K.u form of the modal coefficients
+  ModalGroups<double> mg64;
   DBuf<double> dKb, dS8, dkdiag;
 
   int nrhs = 0;
@@ -80,6 +82,7 @@ struct Problem {
   CoarseCR cr;  // exact coarsest-level solver
 
   // PCG vectors
+  // PCG vectors, nb stacked load cases each (stride ndof)
   DBuf<double> b, x, r, p, Ap64, z64, rt, u;
   DBuf<float> p32, Ap32, r32, z32;
   DBuf<double> partials;
@@ -95,9 +98,12 @@ struct Problem {
   DBuf<double> xd, xphys, dc, xnew, dcf;
   DBuf<uint8_t> active;
 
+  int nb = 0;        // stacked load-case capacity of the solver buffers
+  int pstride = 0;   // partial sums per case
   long long matvecs = 0;
   int replacements = 0;
   hipGraphExec_t cg_graph[3] = {nullptr, nullptr, nullptr};  // per preconditioner kind
+  int cg_graph_nc[3] = {0, 0, 0};                             // case count captured
 
   Problem() = default;
   Problem(const Problem&) = delete;
@@ -113,9 +119,11 @@ void set_loads(Problem& P, int n_rhs, const int32_t* counts, const int32_t* node
                const int32_t* comp, const double* val);
 // E64 (device) -> E32, diagonals, Galerkin levels and the coarsest inverse.
 void build_hierarchy(Problem& P, Timers& t);
-// Solve load case `rhs` with the current moduli; leaves the full displacement in P.u.
-void solve_rhs(Problem& P, int rhs, double tol, int max_it, const double* d_guess,
-               topohip_cginfo* info);
+// Solve load cases 0..nc-1 together (block PCG) with the current moduli;
+// leaves the full displacements in P.u (stride ndof). d_guess (nc == 1 only)
+// is a warm start. info[r] per case.
+void solve_cases(Problem& P, int nc, double tol, int max_it, const double* d_guess,
+                 topohip_cginfo* info);
 FilterCache& filter_for(Problem& P, double radius);
 double simp_compliance_dev(Problem& P, const double* d_xphys, double E0, double penalty,
                            double rho_min, double tol, int max_it, topohip_cginfo* info);

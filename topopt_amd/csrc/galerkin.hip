@@ -12,9 +12,9 @@
 // with W_c the trilinear weights of child c and Mf/Mc the active-DOF masks
 // (rows/columns the reference's P drops). Children whose nodes are all active
 // use the 8 precomputed S_c = W_c^T Ke W_c blocks (the reference's colour
-// cache, :1760-1800). Element matrices are kept element-major in FP64 for the
-// build and transposed to entry-major (FP32 or FP64) for the apply kernel.
-// The coarsest level is assembled dense (FP64) for the explicit-inverse solve.
+// cache, :1760-1800). Element matrices are kept element-major in FP64; the
+// levels the V-cycle applies are then assembled into node stencils
+// (stencil.hip) and the coarsest into plane blocks (coarse.hip).
 #include "kernels.hpp"
 
 namespace topohip {
@@ -235,108 +235,6 @@ __global__ void diag_dense_k(Geom g, const double* __restrict__ K, const uint8_t
     dinv[n * 3 + q] = ((mk >> q) & 1) && d[q] > 0.0 ? static_cast<T>(1.0 / d[q]) : T(0);
 }
 
-template <typename T>
-__global__ void transpose_k(const double* __restrict__ Kem, long long ne, T* __restrict__ Kapp) {
-  // Kapp[q*ne + e] = Kem[e*576 + q]; one block handles 64 elements x 576 entries via LDS.
-  __shared__ double tile[64][65];
-  const long long e0 = blockIdx.x * 64LL;
-  for (int q0 = 0; q0 < 576; q0 += 64) {
-    for (int t = threadIdx.x; t < 64 * 64; t += blockDim.x) {
-      const int le = t / 64, lq = t % 64;
-      const long long e = e0 + le;
-      tile[le][lq] = e < ne ? Kem[e * 576 + q0 + lq] : 0.0;
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < 64 * 64; t += blockDim.x) {
-      const int lq = t / 64, le = t % 64;
-      const long long e = e0 + le;
-      if (e < ne) Kapp[static_cast<long long>(q0 + lq) * ne + e] = static_cast<T>(tile[le][lq]);
-    }
-    __syncthreads();
-  }
-}
-
-// Dense assembly: one thread per (node i, neighbour offset o in 27).
-__global__ void assemble_dense_k(Geom g, const double* __restrict__ K, double* __restrict__ A) {
-  const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
-  if (t >= g.nodes() * 27) return;
-  const long long ni = t / 27;
-  const int o = static_cast<int>(t % 27);
-  const int ox = o % 3 - 1, oy = (o / 3) % 3 - 1, oz = o / 9 - 1;
-  const int a = static_cast<int>(ni % g.nnx);
-  const int b = static_cast<int>((ni / g.nnx) % g.nny);
-  const int c = static_cast<int>(ni / (static_cast<long long>(g.nnx) * g.nny));
-  const int aj = a + ox, bj = b + oy, cj = c + oz;
-  if (aj < 0 || bj < 0 || cj < 0 || aj >= g.nnx || bj >= g.nny || cj >= g.nnz) return;
-  const long long nj = node_of(g, aj, bj, cj);
-  const long long N = 3 * g.nodes();
-  double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int dz = 0; dz < 2; ++dz)
-    for (int dy = 0; dy < 2; ++dy)
-      for (int dx = 0; dx < 2; ++dx) {
-        const int i = a - dx, j = b - dy, k = c - dz;  // element containing node i at bits d
-        if (i < 0 || j < 0 || k < 0 || i >= g.ex || j >= g.ey || k >= g.ez) continue;
-        const int bx = aj - i, by = bj - j, bz = cj - k;
-        if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
-        const long long e = (static_cast<long long>(k) * g.ey + j) * g.ex + i;
-        const int mi = dx + 2 * dy + 4 * dz, mj = bx + 2 * by + 4 * bz;
-        for (int p = 0; p < 3; ++p)
-          for (int qq = 0; qq < 3; ++qq) blk[p * 3 + qq] += K[e * 576 + (3 * mi + p) * 24 + 3 * mj + qq];
-      }
-  for (int p = 0; p < 3; ++p)
-    for (int qq = 0; qq < 3; ++qq) A[(ni * 3 + p) * N + nj * 3 + qq] = blk[p * 3 + qq];
-}
-
-__global__ void dense_identity_inactive_k(Geom g, const uint8_t* __restrict__ mask, double* A) {
-  const long long n = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
-  if (n >= g.nodes()) return;
-  const long long N = 3 * g.nodes();
-  const int mk = mask ? mask[n] : 7;
-  for (int q = 0; q < 3; ++q)
-    if (!((mk >> q) & 1)) A[(n * 3 + q) * N + n * 3 + q] = 1.0;
-}
-
-__global__ void symmetrize_lower_k(double* A, int n) {
-  const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
-  if (t >= static_cast<long long>(n) * n) return;
-  const int i = static_cast<int>(t / n), j = static_cast<int>(t % n);
-  // column-major lower triangle (rocSOLVER) -> full symmetric; A[i + j*n] valid for i >= j
-  if (i < j) A[i + static_cast<long long>(j) * n] = A[j + static_cast<long long>(i) * n];
-}
-
-constexpr int GV_BS = 256;   // rows (i) per block
-constexpr int GV_JC = 64;    // columns (j) per block
-
-template <typename TB>
-__global__ void sym_gemv_partial_k(const double* __restrict__ A, int n, const TB* __restrict__ b,
-                                   double* __restrict__ tmp, const int* done) {
-  if (done && *done) return;
-  __shared__ double bs[GV_JC];
-  const int i = blockIdx.x * GV_BS + threadIdx.x;
-  const int j0 = blockIdx.y * GV_JC;
-  if (threadIdx.x < GV_JC) {
-    const int j = j0 + threadIdx.x;
-    bs[threadIdx.x] = j < n ? static_cast<double>(b[j]) : 0.0;
-  }
-  __syncthreads();
-  if (i >= n) return;
-  double s = 0.0;
-  const int jn = min(GV_JC, n - j0);
-  for (int jj = 0; jj < jn; ++jj) s += A[static_cast<long long>(j0 + jj) * n + i] * bs[jj];
-  tmp[static_cast<long long>(blockIdx.y) * n + i] = s;
-}
-
-template <typename TX>
-__global__ void sym_gemv_reduce_k(const double* __restrict__ tmp, int n, int nj, TX* __restrict__ x,
-                                  const int* done) {
-  if (done && *done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double s = 0.0;
-  for (int jb = 0; jb < nj; ++jb) s += tmp[static_cast<long long>(jb) * n + i];
-  x[i] = static_cast<TX>(s);
-}
-
 }  // namespace
 
 template <typename T>
@@ -386,54 +284,5 @@ template void launch_diag_dense<float>(const Geom&, const double*, const uint8_t
                                        hipStream_t);
 template void launch_diag_dense<double>(const Geom&, const double*, const uint8_t*, double*,
                                         hipStream_t);
-
-template <typename T>
-void launch_transpose_K(const double* Kem, long long ne, T* Kapp, hipStream_t s) {
-  hipLaunchKernelGGL((transpose_k<T>), dim3(static_cast<unsigned>((ne + 63) / 64)), dim3(256), 0,
-                     s, Kem, ne, Kapp);
-  TOPOHIP_CHECK(hipGetLastError());
-}
-template void launch_transpose_K<float>(const double*, long long, float*, hipStream_t);
-template void launch_transpose_K<double>(const double*, long long, double*, hipStream_t);
-
-void launch_assemble_dense(const Geom& g, const double* K, const uint8_t* mask, double* A,
-                           hipStream_t s) {
-  const long long N = 3 * g.nodes();
-  launch_fill<double>(A, 0.0, N * N, s);
-  const long long t = g.nodes() * 27;
-  hipLaunchKernelGGL(assemble_dense_k, dim3(static_cast<unsigned>((t + 255) / 256)), dim3(256), 0,
-                     s, g, K, A);
-  TOPOHIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(dense_identity_inactive_k, dim3(static_cast<unsigned>((g.nodes() + 255) / 256)),
-                     dim3(256), 0, s, g, mask, A);
-  TOPOHIP_CHECK(hipGetLastError());
-}
-
-void launch_symmetrize_lower(double* A, int n, hipStream_t s) {
-  const long long t = static_cast<long long>(n) * n;
-  hipLaunchKernelGGL(symmetrize_lower_k, dim3(static_cast<unsigned>((t + 255) / 256)), dim3(256),
-                     0, s, A, n);
-  TOPOHIP_CHECK(hipGetLastError());
-}
-
-long long gemv_tmp_size(int n) {
-  return static_cast<long long>((n + GV_JC - 1) / GV_JC) * n;
-}
-
-template <typename TB, typename TX>
-void launch_sym_gemv(const double* A, int n, const TB* b, TX* x, double* tmp, const int* done,
-                     hipStream_t s) {
-  const int nj = (n + GV_JC - 1) / GV_JC;
-  hipLaunchKernelGGL((sym_gemv_partial_k<TB>), dim3((n + GV_BS - 1) / GV_BS, nj), dim3(GV_BS), 0, s,
-                     A, n, b, tmp, done);
-  TOPOHIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL((sym_gemv_reduce_k<TX>), dim3((n + 255) / 256), dim3(256), 0, s, tmp, n, nj, x,
-                     done);
-  TOPOHIP_CHECK(hipGetLastError());
-}
-template void launch_sym_gemv<double, double>(const double*, int, const double*, double*, double*,
-                                              const int*, hipStream_t);
-template void launch_sym_gemv<float, float>(const double*, int, const float*, float*, double*,
-                                            const int*, hipStream_t);
 
 }  // namespace topohip

@@ -21,12 +21,14 @@ struct KuOp {
   T omega = T(0);
   const uint8_t* mask = nullptr;  // per node: bit c = DOF c kept; null = all kept
   const T* E = nullptr;           // fine level
-  Modal<T> mc;
+  ModalGroups<T> mg;              // grouped modal coefficients of Ke(E=1)
   int epi = EPI_AX;
   double* partials = nullptr;
   int tz = 8;                     // node planes per workgroup (z-march length)
   int tx = 16;                    // workgroup tile: tx x (256/tx) element columns
-  const int* done = nullptr;      // early exit when *done != 0
+  const int* done = nullptr;      // case r exits early when done[r * kDoneStride] != 0
+  int nrhs = 1;                   // stacked load cases (x, y, b stride g.ndof())
+  int pstride = 0;                // partials of case r at partials + r * pstride
 };
 
 int ku_blocks(const Geom& g, int tz, int tx);
@@ -34,54 +36,65 @@ template <typename T> void launch_ku_fine(const KuOp<T>& op, hipStream_t s);
 
 // ---- transfers between a fine level f and the coincident coarse level c ----
 // bc = mask_c .* P^T rf; optionally xc = omega * dinv_c .* bc (pre-smooth from 0).
+// With nr load cases the vectors are stacked with the level's ndof as stride.
 template <typename Tf, typename Tc>
 void launch_restrict(const Geom& gf, const Geom& gc, const Tf* rf, const uint8_t* mask_c,
                      Tc* bc, Tc* xc, const Tc* dinv_c, double omega, const int* done,
-                     hipStream_t s);
+                     hipStream_t s, int nr = 1);
 // xf += mask_f .* P xc
 template <typename Tc, typename Tf>
 void launch_prolong_add(const Geom& gc, const Geom& gf, const Tc* xc, const uint8_t* mask_f,
-                        Tf* xf, const int* done, hipStream_t s);
+                        Tf* xf, const int* done, hipStream_t s, int nr = 1);
 
 // ---- reductions / vector kernels -------------------------------------------
-// out[0] = sum(partials[0..n)) in a fixed order (one block).
-void launch_sum_partials(const double* partials, int n, double* out, hipStream_t s);
-// partial dot products over n entries: partials[block] = sum a[i]*b[i]
+// out[r] = sum(partials[r*pstride + 0..n)) in a fixed order (one block per case).
+void launch_sum_partials(const double* partials, int n, double* out, hipStream_t s, int nr = 1,
+                         int pstride = 0);
+// partial dot products over n entries: partials[r*pstride + block] = sum a[i]*b[i]
+// for each of nr stacked case vectors (stride n).
 template <typename TA, typename TB>
 void launch_dot(const TA* a, const TB* b, long long n, double* partials, int nblocks,
-                const int* done, hipStream_t s);
+                const int* done, hipStream_t s, int nr = 1, int pstride = 0);
 int dot_blocks(long long n);
 // x = omega * dinv .* b (one damped-Jacobi sweep from a zero start).
 template <typename T>
 void launch_jacobi0(T* x, const T* dinv, const T* b, double omega, long long n, const int* done,
-                    hipStream_t s);
+                    hipStream_t s, int nr = 1);
 // y = x + up (scatter prescribed values into the solution).
 void launch_add(const double* x, const double* up, double* y, long long n, hipStream_t s);
 template <typename T> void launch_fill(T* p, T v, long long n, hipStream_t s);
 template <typename TI, typename TO> void launch_convert(const TI* in, TO* out, long long n,
                                                         hipStream_t s);
 
-// PCG pieces (device-resident scalars in CgState).
+// PCG pieces (device-resident scalars in CgState). Every launcher covers nr
+// load cases: states st[0..nr), partials of case r at partials + r*pstride,
+// vectors stacked with stride n.
 // alpha = rz / pAp (pAp = sum of partials); breakdown sets fail/done.
-void launch_cg_alpha(CgState* st, const double* partials, int n, hipStream_t s);
+void launch_cg_alpha(CgState* st, const double* partials, int n, hipStream_t s, int nr = 1,
+                     int pstride = 0);
 // x += alpha*p; r -= alpha*Ap; rr partials; r32 = float(r) (nullable)
 template <typename TP, typename TA>
 void launch_cg_xr(CgState* st, double* x, double* r, const TP* p, const TA* Ap, float* r32,
-                  long long n, double* partials, int nblocks, hipStream_t s);
+                  long long n, double* partials, int nblocks, hipStream_t s, int nr = 1,
+                  int pstride = 0);
 // rr = sum(partials); iters++; done if rr <= thresh2
-void launch_cg_check(CgState* st, const double* partials, int n, hipStream_t s);
+void launch_cg_check(CgState* st, const double* partials, int n, hipStream_t s, int nr = 1,
+                     int pstride = 0);
 // rz_new = sum(partials); beta = rz_new / rz; rz = rz_new
-void launch_cg_beta(CgState* st, const double* partials, int n, hipStream_t s);
+void launch_cg_beta(CgState* st, const double* partials, int n, hipStream_t s, int nr = 1,
+                    int pstride = 0);
 // p = z + beta*p (beta = 0 on restart); optional p32 shadow
 template <typename TZ>
 void launch_cg_p(CgState* st, const TZ* z, double* p, float* p32, long long n, int restart,
-                 hipStream_t s);
+                 hipStream_t s, int nr = 1);
 // start: bnorm2, thresh2, rz from partials etc.
 void launch_cg_init(CgState* st, const double* bb_partials, int nb, double tol, int cap,
-                    hipStream_t s);
+                    hipStream_t s, int nr = 1, int pstride = 0);
 // rr = sum(partials) of a recomputed residual; done = rr <= thresh2 (resume).
-void launch_cg_resume(CgState* st, const double* partials, int n, hipStream_t s);
-void launch_cg_set_rz(CgState* st, const double* partials, int n, hipStream_t s);
+void launch_cg_resume(CgState* st, const double* partials, int n, hipStream_t s, int nr = 1,
+                      int pstride = 0);
+void launch_cg_set_rz(CgState* st, const double* partials, int n, hipStream_t s, int nr = 1,
+                      int pstride = 0);
 
 // ---- hierarchy build ---------------------------------------------------------
 // Fine Jacobi inverse diagonal: dinv = 1 / sum_e E_e Ke(l,l) on kept DOFs, else 0.
@@ -101,21 +114,6 @@ void launch_galerkin_coarse(const Geom& gc, const Geom& gf, const double* Kf,
 template <typename T>
 void launch_diag_dense(const Geom& g, const double* K, const uint8_t* mask, T* dinv,
                        hipStream_t s);
-// Element-major FP64 [elems][576] -> entry-major [576][elems] (apply layout).
-template <typename T>
-void launch_transpose_K(const double* Kem, long long ne, T* Kapp, hipStream_t s);
-// Dense FP64 assembly of an element-form level (3*nodes square, identity on
-// inactive DOFs).
-void launch_assemble_dense(const Geom& g, const double* K, const uint8_t* mask, double* A,
-                           hipStream_t s);
-void launch_symmetrize_lower(double* A, int n, hipStream_t s);
-// x = A b for a dense symmetric n x n FP64 matrix (column-coalesced), with
-// 2-D partials then a fixed-order reduction. tmp: gemv_tmp_size(n) doubles.
-long long gemv_tmp_size(int n);
-template <typename TB, typename TX>
-void launch_sym_gemv(const double* A, int n, const TB* b, TX* x, double* tmp, const int* done,
-                     hipStream_t s);
-
 // ---- SIMP kernels --------------------------------------------------------------
 // E = clamp(xphys, rho_min, 1)^p * E0 on solid voxels (0 elsewhere).
 void launch_youngs(const double* xphys, const uint8_t* solid, long long n, double E0,
@@ -163,13 +161,16 @@ void launch_sum_vec(const T* v, long long n, double* partials, int nblocks, hipS
 
 namespace topohip {
 // ---- assembled coarse levels: 27-point node stencil of 3x3 blocks ----------
-// Coefficient (o, p, q) of node n (neighbour o = (dz+1)*9 + (dy+1)*3 + (dx+1),
-// row component p, column component q) at S[(o*9 + p*3 + q) * nodes + n].
+// Symmetric half storage: block (o, p, q) of node n for neighbour
+// o = (dz+1)*9 + (dy+1)*3 + (dx+1) >= 13 (centre and lexicographically larger
+// neighbours; row component p, column component q) at
+// S[((o-13)*9 + p*3 + q) * nodes + n], 126 coefficients per node. Blocks
+// towards smaller neighbours are the transposes stored at those neighbours.
 template <typename T>
 void launch_assemble_stencil(const Geom& g, const double* Kem, T* S, hipStream_t s);
 // y = A x / b - A x / x + omega*dinv*(b - A x) on a stencil level (masked).
 template <typename T>
 void launch_stencil_apply(const Geom& g, const T* S, const uint8_t* mask, const T* x, T* y,
                           const T* b, const T* dinv, double omega, int epi, const int* done,
-                          hipStream_t s);
+                          hipStream_t s, int nr = 1);
 }  // namespace topohip

@@ -47,46 +47,91 @@ __device__ inline void face_fwd(const T (&u)[12], T (&F)[12]) {
   }
 }
 
+// v = E * C A for the 24 modal amplitudes A, with C in its grouped form
+// (8 distinct values): 45 multiply-adds, no per-entry coefficient registers.
+template <typename T>
+__device__ inline void modal_core(const T (&A)[24], T (&v)[24], T e, const ModalGroups<T>& mg) {
+  const T ea = e * mg.amb, eb = e * mg.b, ed = e * mg.d, ee = e * mg.e, ef = e * mg.f;
+  const T eg = e * mg.gmh, eh = e * mg.h, ek = e * mg.k;
+  v[0] = v[1] = v[2] = T(0);  // rigid translations
+  const T sn = (A[3] + A[7]) + A[14];
+  v[3] = ea * A[3] + eb * sn;
+  v[7] = ea * A[7] + eb * sn;
+  v[14] = ea * A[14] + eb * sn;
+  v[4] = v[6] = ed * (A[4] + A[6]);
+  v[5] = v[12] = ed * (A[5] + A[12]);
+  v[8] = v[13] = ed * (A[8] + A[13]);
+  v[9] = ee * A[9] + ef * A[20];
+  v[20] = ef * A[9] + ee * A[20];
+  v[10] = ee * A[10] + ef * A[17];
+  v[17] = ef * A[10] + ee * A[17];
+  v[15] = ee * A[15] + ef * A[19];
+  v[19] = ef * A[15] + ee * A[19];
+  const T su = (A[11] + A[16]) + A[18];
+  v[11] = eg * A[11] + eh * su;
+  v[16] = eg * A[16] + eh * su;
+  v[18] = eg * A[18] + eh * su;
+  v[21] = ek * A[21];
+  v[22] = ek * A[22];
+  v[23] = ek * A[23];
+}
+
+// Inputs x must be zero on inactive DOFs (every solver vector is); the mask
+// only decides which outputs are kept.
 template <typename T, int TX, int EPI, bool DOT>
 __global__ void __launch_bounds__(NT) ku_face(KuOp<T> a) {
   constexpr int TY = NT / TX;
-  constexpr int PLANE = (TX + 1) * (TY + 1) * 3;
+  constexpr int PW = TX + 1;                  // staged plane: PW x (TY + 1) nodes
+  constexpr int PLANE = PW * (TY + 1) * 3;
   constexpr int NPF = (PLANE + NT - 1) / NT;  // prefetch registers per lane
-  if (a.done && *a.done) return;
+  const Geom g = a.g;
+  const int nchunk = (g.nnz + a.tz - 1) / a.tz;
+  const int rc = blockIdx.z / nchunk;  // load case
+  if (case_done(a.done, rc)) return;
   __shared__ T s_u[2][PLANE];
   __shared__ T s_c[4][NT * 3];
   __shared__ double s_red[NT / 64];
 
-  const Geom g = a.g;
   const int t = threadIdx.x;
   const int tx = t % TX, ty = t / TX;
-  const int i0 = blockIdx.x * (TX - 1), j0 = blockIdx.y * (TY - 1), k0 = blockIdx.z * a.tz;
+  const int i0 = blockIdx.x * (TX - 1), j0 = blockIdx.y * (TY - 1);
+  const int k0 = (blockIdx.z - rc * nchunk) * a.tz;
   const int ei = i0 - 1 + tx, ej = j0 - 1 + ty;
   const int na = i0 + tx, nb = j0 + ty;
   const bool owner = tx < TX - 1 && ty < TY - 1 && na < g.nnx && nb < g.nny;
   const bool exy = ei >= 0 && ei < g.ex && ej >= 0 && ej < g.ey;
+  const long long vo = rc * g.ndof();       // case offset of x, y, b
+  const long long pn = static_cast<long long>(g.nnx) * g.nny;  // nodes per plane
+  const long long pe = static_cast<long long>(g.ex) * g.ey;    // elements per layer
+  const T* __restrict__ xs = a.x + vo;
+  T* __restrict__ ys = a.y + vo;
+  const T* __restrict__ bs = a.b ? a.b + vo : nullptr;
   const uint8_t* __restrict__ mask = a.mask;
-  const T* __restrict__ x = a.x;
   const T* __restrict__ E = a.E;
+  // Loop-invariant in-plane offsets (-1: outside the grid).
+  int foff[NPF];
+#pragma unroll
+  for (int i = 0; i < NPF; ++i) {
+    const int q = t + i * NT;
+    foff[i] = -1;
+    if (q < PLANE) {
+      const int n = q / 3, comp = q - 3 * n;
+      const int ga = i0 - 1 + n % PW, gb = j0 - 1 + n / PW;
+      if (ga >= 0 && ga < g.nnx && gb >= 0 && gb < g.nny) foff[i] = (gb * g.nnx + ga) * 3 + comp;
+    }
+  }
+  const int eoff = exy ? ej * g.ex + ei : 0;
+  const int noff = owner ? nb * g.nnx + na : 0;
 
   T pf[NPF];
   auto fetch_plane = [&](int c) {
+    if (c >= 0 && c < g.nnz) {
+      const T* xp = xs + c * pn * 3;
 #pragma unroll
-    for (int i = 0; i < NPF; ++i) {
-      const int q = t + i * NT;
-      T v = T(0);
-      if (q < PLANE) {
-        const int comp = q % 3;
-        const int n = q / 3;
-        const int lx = n % (TX + 1), ly = n / (TX + 1);
-        const int ga = i0 - 1 + lx, gb = j0 - 1 + ly;
-        if (c >= 0 && c < g.nnz && ga >= 0 && ga < g.nnx && gb >= 0 && gb < g.nny) {
-          const long long node = (static_cast<long long>(c) * g.nny + gb) * g.nnx + ga;
-          const bool kept = !mask || ((mask[node] >> comp) & 1);
-          if (kept) v = x[node * 3 + comp];
-        }
-      }
-      pf[i] = v;
+      for (int i = 0; i < NPF; ++i) pf[i] = foff[i] >= 0 ? xp[foff[i]] : T(0);
+    } else {
+#pragma unroll
+      for (int i = 0; i < NPF; ++i) pf[i] = T(0);
     }
   };
   auto store_plane = [&](int buf) {
@@ -100,7 +145,7 @@ __global__ void __launch_bounds__(NT) ku_face(KuOp<T> a) {
     T u[12];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int base = ((ty + (q >> 1)) * (TX + 1) + tx + (q & 1)) * 3;
+      const int base = ((ty + (q >> 1)) * PW + tx + (q & 1)) * 3;
 #pragma unroll
       for (int c = 0; c < 3; ++c) u[3 * q + c] = s_u[buf][base + c];
     }
@@ -108,22 +153,22 @@ __global__ void __launch_bounds__(NT) ku_face(KuOp<T> a) {
   };
   auto load_own = [&](int buf, T (&xo)[3]) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) xo[c] = s_u[buf][((ty + 1) * (TX + 1) + (tx + 1)) * 3 + c];
+    for (int c = 0; c < 3; ++c) xo[c] = s_u[buf][((ty + 1) * PW + (tx + 1)) * 3 + c];
   };
   auto modulus = [&](int k) -> T {
     if (!exy || k < 0 || k >= g.ez) return T(0);
-    return E[(static_cast<long long>(k) * g.ey + ej) * g.ex + ei];
+    return E[k * pe + eoff];
   };
   // Owner-node operands of plane c, fetched one layer ahead.
   int mk_n = 0;
   T b_n[3] = {T(0), T(0), T(0)}, d_n[3] = {T(0), T(0), T(0)};
   auto fetch_owner = [&](int c) {
     if (!owner || c < 0 || c >= g.nnz) return;
-    const long long node = (static_cast<long long>(c) * g.nny + nb) * g.nnx + na;
+    const long long node = c * pn + noff;
     mk_n = mask ? mask[node] : 7;
     if constexpr (EPI != EPI_AX) {
 #pragma unroll
-      for (int q = 0; q < 3; ++q) b_n[q] = a.b[node * 3 + q];
+      for (int q = 0; q < 3; ++q) b_n[q] = bs[node * 3 + q];
     }
     if constexpr (EPI == EPI_JAC) {
 #pragma unroll
@@ -159,20 +204,14 @@ __global__ void __launch_bounds__(NT) ku_face(KuOp<T> a) {
     load_face(nxt, Fn);
     load_own(nxt, xn);
     // Element layer k between planes k (Fp) and k + 1 (Fn): z stage, modal
-    // core, inverse z stage split into the bottom (Vb) and top faces.
+    // core, inverse z stage split into the bottom and top faces.
     T A[24], v[24];
 #pragma unroll
     for (int q = 0; q < 12; ++q) {
       A[q] = Fp[q] + Fn[q];
       A[12 + q] = Fp[q] - Fn[q];
     }
-#pragma unroll
-    for (int s = 3; s < 24; ++s) A[s] *= e;
-#pragma unroll
-    for (int r = 0; r < 24; ++r) v[r] = T(0);
-#define TOPOHIP_X(kk, r, s) v[r] += a.mc.c[kk] * A[s];
-    TOPOHIP_MODAL_PATTERN(TOPOHIP_X)
-#undef TOPOHIP_X
+    modal_core(A, v, e, a.mg);
     T G[12];
 #pragma unroll
     for (int q = 0; q < 12; ++q) {
@@ -196,7 +235,7 @@ __global__ void __launch_bounds__(NT) ku_face(KuOp<T> a) {
       // Node (na, nb) is corner 3 of column (tx, ty), 2 of (tx+1, ty),
       // 1 of (tx, ty+1) and 0 of (tx+1, ty+1). Fixed summation order.
       const int t00 = t, t10 = t + 1, t01 = t + TX, t11 = t + TX + 1;
-      const long long node = (static_cast<long long>(k) * g.nny + nb) * g.nnx + na;
+      T* yp = ys + (k * pn + noff) * 3;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const T ax = ((s_c[3][t00 * 3 + c] + s_c[2][t10 * 3 + c]) + s_c[1][t01 * 3 + c]) +
@@ -213,7 +252,7 @@ __global__ void __launch_bounds__(NT) ku_face(KuOp<T> a) {
           }
           if constexpr (DOT) dotp += static_cast<double>(xo[c]) * static_cast<double>(ax);
         }
-        a.y[node * 3 + c] = out;
+        yp[c] = out;
       }
     }
 #pragma unroll
@@ -229,8 +268,8 @@ __global__ void __launch_bounds__(NT) ku_face(KuOp<T> a) {
     if (t == 0) {
       double s = 0.0;
       for (int w = 0; w < NT / 64; ++w) s += s_red[w];
-      const int blin = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-      a.partials[blin] = s;
+      const int blin = ((blockIdx.z - rc * nchunk) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+      a.partials[static_cast<long long>(rc) * a.pstride + blin] = s;
     }
   }
 }
@@ -239,7 +278,7 @@ template <typename T, int TX>
 void dispatch_tx(const KuOp<T>& op, hipStream_t s) {
   constexpr int TY = NT / TX;
   const dim3 grid((op.g.nnx + TX - 2) / (TX - 1), (op.g.nny + TY - 2) / (TY - 1),
-                  (op.g.nnz + op.tz - 1) / op.tz);
+                  ((op.g.nnz + op.tz - 1) / op.tz) * op.nrhs);
   const bool dot = op.partials != nullptr;
 #define TOPOHIP_L(E, D) hipLaunchKernelGGL((ku_face<T, TX, E, D>), grid, dim3(NT), 0, s, op)
   if (op.epi == EPI_AX) {

@@ -11,9 +11,11 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <mutex>
+#include <string>
 #include <thread>
 
 #include "engine.hpp"
@@ -51,6 +53,7 @@ struct JobRec {
   topohip_job_result res{};
   size_t footprint = 0;
   std::vector<double> xphys;
+  std::string error;  // message of a failed job
 };
 }  // namespace
 
@@ -78,6 +81,7 @@ struct topohip_server {
     }
     topohip_job j = rec->job;
     int status = TOPOHIP_OK;
+    std::string err;
     topohip_result r{};
     std::vector<double> xp(static_cast<size_t>(j.nx) * j.ny * j.nz);
     try {
@@ -96,8 +100,10 @@ struct topohip_server {
       optimize(*P, o, nullptr, xp.data(), nullptr, &r);
     } catch (const Error& e) {
       status = e.code;
-    } catch (const std::exception&) {
+      err = e.what();
+    } catch (const std::exception& e) {
       status = TOPOHIP_EDEVICE;
+      err = e.what();
     }
     std::lock_guard<std::mutex> lk(mu);
     rec->res.iterations = r.iterations;
@@ -107,6 +113,7 @@ struct topohip_server {
     rec->res.t_end_ms = wall_ms() - t0;
     rec->res.state = status == TOPOHIP_OK ? 2 : 3;
     rec->xphys = std::move(xp);
+    rec->error = std::move(err);
     in_use -= rec->footprint;
     --running;
     cv.notify_all();
@@ -205,6 +212,16 @@ int topohip_server_fetch_density(topohip_server* s, int64_t job_id, double* xphy
   auto it = s->jobs.find(job_id);
   if (it == s->jobs.end() || it->second.res.state != 2) return TOPOHIP_EINVAL;
   std::copy(it->second.xphys.begin(), it->second.xphys.end(), xphys);
+  return TOPOHIP_OK;
+}
+
+int topohip_server_job_error(topohip_server* s, int64_t job_id, char* buf, size_t n) {
+  if (!s || !buf || n == 0) return TOPOHIP_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(job_id);
+  if (it == s->jobs.end()) return TOPOHIP_EINVAL;
+  std::strncpy(buf, it->second.error.c_str(), n - 1);
+  buf[n - 1] = '\0';
   return TOPOHIP_OK;
 }
 

@@ -6,20 +6,28 @@
 // 3x3 block. The blocks are assembled once per hierarchy build from the
 // per-coarse-element Galerkin matrices (fixed element order, no atomics) and
 // applied by a node-parallel gather whose coefficient loads are coalesced
-// (entry-major layout).
+// (entry-major layout). The operator is symmetric, so only the centre block
+// and the 13 blocks towards lexicographically larger neighbours are stored
+// (offset index o = 13 + h, h = 0..13); the block towards a smaller neighbour
+// m is the transpose of the block m stores towards this node. That halves the
+// stored coefficients, and the transposed reads of a neighbour's blocks mostly
+// hit the caches.
 #include "kernels.hpp"
 
 namespace topohip {
 
 namespace {
 
+constexpr int SH = 14;  // stored blocks per node
+
 template <typename T>
 __global__ void assemble_stencil_k(Geom g, const double* __restrict__ K, T* __restrict__ S) {
   const long long nodes = g.nodes();
   const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
-  if (t >= nodes * 27) return;
+  if (t >= nodes * SH) return;
   const long long ni = t % nodes;
-  const int o = static_cast<int>(t / nodes);
+  const int h = static_cast<int>(t / nodes);
+  const int o = 13 + h;
   const int ox = o % 3 - 1, oy = (o / 3) % 3 - 1, oz = o / 9 - 1;
   const int a = static_cast<int>(ni % g.nnx);
   const int b = static_cast<int>((ni / g.nnx) % g.nny);
@@ -40,7 +48,7 @@ __global__ void assemble_stencil_k(Geom g, const double* __restrict__ K, T* __re
             for (int q = 0; q < 3; ++q) blk[p * 3 + q] += K[e * 576 + (3 * mi + p) * 24 + 3 * mj + q];
         }
   }
-  for (int pq = 0; pq < 9; ++pq) S[(static_cast<long long>(o) * 9 + pq) * nodes + ni] = static_cast<T>(blk[pq]);
+  for (int pq = 0; pq < 9; ++pq) S[(static_cast<long long>(h) * 9 + pq) * nodes + ni] = static_cast<T>(blk[pq]);
 }
 
 template <typename T, int EPI>
@@ -48,8 +56,12 @@ __global__ void __launch_bounds__(256)
 stencil_apply_k(Geom g, const T* __restrict__ S, const uint8_t* __restrict__ mask,
                 const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ b,
                 const T* __restrict__ dinv, T omega, const int* done) {
-  if (done && *done) return;
+  const int rc = blockIdx.y;
+  if (case_done(done, rc)) return;
   const long long nodes = g.nodes();
+  x += rc * 3 * nodes;
+  y += rc * 3 * nodes;
+  if (b) b += rc * 3 * nodes;
   const long long n = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
   if (n >= nodes) return;
   const int a = static_cast<int>(n % g.nnx);
@@ -66,10 +78,19 @@ stencil_apply_k(Geom g, const T* __restrict__ S, const uint8_t* __restrict__ mas
     T xv[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) xv[q] = ((mm >> q) & 1) ? x[m * 3 + q] : T(0);
+    if (o >= 13) {  // own block towards m
+      const T* Sb = S + static_cast<long long>(o - 13) * 9 * nodes + n;
 #pragma unroll
-    for (int p = 0; p < 3; ++p)
+      for (int p = 0; p < 3; ++p)
 #pragma unroll
-      for (int q = 0; q < 3; ++q) acc[p] += S[(static_cast<long long>(o) * 9 + p * 3 + q) * nodes + n] * xv[q];
+        for (int q = 0; q < 3; ++q) acc[p] += Sb[(p * 3 + q) * nodes] * xv[q];
+    } else {  // transpose of m's block towards n (offset 26 - o)
+      const T* Sb = S + static_cast<long long>(13 - o) * 9 * nodes + m;
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) acc[p] += Sb[(q * 3 + p) * nodes] * xv[q];
+    }
   }
   const int mk = mask ? mask[n] : 7;
 #pragma unroll
@@ -88,7 +109,7 @@ stencil_apply_k(Geom g, const T* __restrict__ S, const uint8_t* __restrict__ mas
 
 template <typename T>
 void launch_assemble_stencil(const Geom& g, const double* Kem, T* S, hipStream_t s) {
-  const long long t = g.nodes() * 27;
+  const long long t = g.nodes() * SH;
   hipLaunchKernelGGL((assemble_stencil_k<T>), dim3(static_cast<unsigned>((t + 255) / 256)), dim3(256),
                      0, s, g, Kem, S);
   TOPOHIP_CHECK(hipGetLastError());
@@ -99,9 +120,9 @@ template void launch_assemble_stencil<double>(const Geom&, const double*, double
 template <typename T>
 void launch_stencil_apply(const Geom& g, const T* S, const uint8_t* mask, const T* x, T* y,
                           const T* b, const T* dinv, double omega, int epi, const int* done,
-                          hipStream_t s) {
+                          hipStream_t s, int nr) {
   const long long n = g.nodes();
-  const dim3 grid(static_cast<unsigned>((n + 255) / 256));
+  const dim3 grid(static_cast<unsigned>((n + 255) / 256), nr);
   const T om = static_cast<T>(omega);
   if (epi == EPI_AX)
     hipLaunchKernelGGL((stencil_apply_k<T, EPI_AX>), grid, dim3(256), 0, s, g, S, mask, x, y, b, dinv, om, done);
@@ -113,9 +134,9 @@ void launch_stencil_apply(const Geom& g, const T* S, const uint8_t* mask, const 
 }
 template void launch_stencil_apply<float>(const Geom&, const float*, const uint8_t*, const float*,
                                           float*, const float*, const float*, double, int,
-                                          const int*, hipStream_t);
+                                          const int*, hipStream_t, int);
 template void launch_stencil_apply<double>(const Geom&, const double*, const uint8_t*, const double*,
                                            double*, const double*, const double*, double, int,
-                                           const int*, hipStream_t);
+                                           const int*, hipStream_t, int);
 
 }  // namespace topohip

@@ -43,8 +43,12 @@ template <typename Tf, typename Tc>
 __global__ void restrict_k(Geom gf, Geom gc, const Tf* __restrict__ rf,
                            const uint8_t* __restrict__ mask_c, Tc* __restrict__ bc,
                            Tc* __restrict__ xc, const Tc* __restrict__ dinv_c, double omega,
-                           const int* done) {
-  if (done && *done) return;
+                           long long sf, long long sc, const int* done) {
+  const int rc = blockIdx.y;
+  if (case_done(done, rc)) return;
+  rf += rc * sf;
+  bc += rc * sc;
+  if (xc) xc += rc * sc;
   const long long nodes = gc.nodes();
   const long long n = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
   if (n >= nodes) return;
@@ -85,8 +89,11 @@ __global__ void restrict_k(Geom gf, Geom gc, const Tf* __restrict__ rf,
 template <typename Tc, typename Tf>
 __global__ void prolong_add_k(Geom gc, Geom gf, const Tc* __restrict__ xc,
                               const uint8_t* __restrict__ mask_f, Tf* __restrict__ xf,
-                              const int* done) {
-  if (done && *done) return;
+                              long long sc, long long sf, const int* done) {
+  const int rc = blockIdx.y;
+  if (case_done(done, rc)) return;
+  xc += rc * sc;
+  xf += rc * sf;
   const long long nodes = gf.nodes();
   const long long n = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
   if (n >= nodes) return;
@@ -118,7 +125,9 @@ __global__ void prolong_add_k(Geom gc, Geom gf, const Tc* __restrict__ xc,
     if ((mk >> q) & 1) xf[n * 3 + q] = static_cast<Tf>(static_cast<double>(xf[n * 3 + q]) + s[q]);
 }
 
-__global__ void sum_partials_k(const double* __restrict__ p, int n, double* out) {
+__global__ void sum_partials_k(const double* __restrict__ p, int n, double* out, int pstride) {
+  p += static_cast<long long>(blockIdx.x) * pstride;
+  out += blockIdx.x;
   double s = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) s += p[i];
   s = block_sum(s);
@@ -141,8 +150,12 @@ __global__ void max_partials_k(const double* __restrict__ p, int n, double* out)
 
 template <typename TA, typename TB>
 __global__ void dot_k(const TA* __restrict__ a, const TB* __restrict__ b, long long n,
-                      double* partials, const int* done) {
-  if (done && *done) return;
+                      double* partials, int pstride, const int* done) {
+  const int rc = blockIdx.y;
+  if (case_done(done, rc)) return;
+  a += rc * n;
+  b += rc * n;
+  partials += static_cast<long long>(rc) * pstride;
   double s = 0.0;
   for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<long long>(gridDim.x) * blockDim.x)
@@ -178,7 +191,10 @@ __global__ void convert_k(const TI* __restrict__ in, TO* __restrict__ out, long 
 template <typename T>
 __global__ void jacobi0_k(T* __restrict__ x, const T* __restrict__ dinv, const T* __restrict__ b,
                           double omega, long long n, const int* done) {
-  if (done && *done) return;
+  const int rc = blockIdx.y;
+  if (case_done(done, rc)) return;
+  x += rc * n;
+  b += rc * n;
   for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<long long>(gridDim.x) * blockDim.x)
     x[i] = static_cast<T>(omega * static_cast<double>(dinv[i]) * static_cast<double>(b[i]));
@@ -198,7 +214,9 @@ __device__ inline double fixed_sum(const double* p, int n) {
   return block_sum(s);
 }
 
-__global__ void cg_init_k(CgState* st, const double* p, int n, double tol, int cap) {
+__global__ void cg_init_k(CgState* st, const double* p, int n, int pstride, double tol, int cap) {
+  st += blockIdx.x;
+  p += static_cast<long long>(blockIdx.x) * pstride;
   const double bb = fixed_sum(p, n);
   if (threadIdx.x == 0) {
     st->bnorm2 = bb;
@@ -212,7 +230,9 @@ __global__ void cg_init_k(CgState* st, const double* p, int n, double tol, int c
   }
 }
 
-__global__ void cg_set_rz_k(CgState* st, const double* p, int n) {
+__global__ void cg_set_rz_k(CgState* st, const double* p, int n, int pstride) {
+  st += blockIdx.x;
+  p += static_cast<long long>(blockIdx.x) * pstride;
   if (st->done) return;
   const double rz = fixed_sum(p, n);
   if (threadIdx.x == 0) {
@@ -221,7 +241,9 @@ __global__ void cg_set_rz_k(CgState* st, const double* p, int n) {
   }
 }
 
-__global__ void cg_alpha_k(CgState* st, const double* p, int n) {
+__global__ void cg_alpha_k(CgState* st, const double* p, int n, int pstride) {
+  st += blockIdx.x;
+  p += static_cast<long long>(blockIdx.x) * pstride;
   if (st->done) return;
   const double pAp = fixed_sum(p, n);
   if (threadIdx.x == 0) {
@@ -236,7 +258,9 @@ __global__ void cg_alpha_k(CgState* st, const double* p, int n) {
   }
 }
 
-__global__ void cg_check_k(CgState* st, const double* p, int n) {
+__global__ void cg_check_k(CgState* st, const double* p, int n, int pstride) {
+  st += blockIdx.x;
+  p += static_cast<long long>(blockIdx.x) * pstride;
   if (st->done) return;
   const double rr = fixed_sum(p, n);
   if (threadIdx.x == 0) {
@@ -248,7 +272,9 @@ __global__ void cg_check_k(CgState* st, const double* p, int n) {
   }
 }
 
-__global__ void cg_beta_k(CgState* st, const double* p, int n) {
+__global__ void cg_beta_k(CgState* st, const double* p, int n, int pstride) {
+  st += blockIdx.x;
+  p += static_cast<long long>(blockIdx.x) * pstride;
   if (st->done) return;
   const double rzn = fixed_sum(p, n);
   if (threadIdx.x == 0) {
@@ -259,7 +285,9 @@ __global__ void cg_beta_k(CgState* st, const double* p, int n) {
   }
 }
 
-__global__ void cg_resume_k(CgState* st, const double* p, int n) {
+__global__ void cg_resume_k(CgState* st, const double* p, int n, int pstride) {
+  st += blockIdx.x;
+  p += static_cast<long long>(blockIdx.x) * pstride;
   const double rr = fixed_sum(p, n);
   if (threadIdx.x == 0) {
     st->rr = rr;
@@ -271,8 +299,16 @@ __global__ void cg_resume_k(CgState* st, const double* p, int n) {
 template <typename TP, typename TA>
 __global__ void cg_xr_k(CgState* st, double* __restrict__ x, double* __restrict__ r,
                         const TP* __restrict__ p, const TA* __restrict__ Ap, float* r32,
-                        long long n, double* partials) {
+                        long long n, double* partials, int pstride) {
+  const int rc = blockIdx.y;
+  st += rc;
   if (st->done) return;
+  x += rc * n;
+  r += rc * n;
+  p += rc * n;
+  Ap += rc * n;
+  if (r32) r32 += rc * n;
+  partials += static_cast<long long>(rc) * pstride;
   const double alpha = st->alpha;
   double s = 0.0;
   for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
@@ -290,7 +326,12 @@ __global__ void cg_xr_k(CgState* st, double* __restrict__ x, double* __restrict_
 template <typename TZ>
 __global__ void cg_p_k(CgState* st, const TZ* __restrict__ z, double* __restrict__ p,
                        float* p32, long long n, int restart) {
+  const int rc = blockIdx.y;
+  st += rc;
   if (st->done) return;
+  z += rc * n;
+  p += rc * n;
+  if (p32) p32 += rc * n;
   const double beta = restart ? 0.0 : st->beta;
   for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<long long>(gridDim.x) * blockDim.x) {
@@ -311,40 +352,45 @@ int grid_for(long long n, int bs, int cap) {
 
 template <typename Tf, typename Tc>
 void launch_restrict(const Geom& gf, const Geom& gc, const Tf* rf, const uint8_t* mask_c, Tc* bc,
-                     Tc* xc, const Tc* dinv_c, double omega, const int* done, hipStream_t s) {
+                     Tc* xc, const Tc* dinv_c, double omega, const int* done, hipStream_t s,
+                     int nr) {
   const long long n = gc.nodes();
-  hipLaunchKernelGGL((restrict_k<Tf, Tc>), dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256),
-                     0, s, gf, gc, rf, mask_c, bc, xc, dinv_c, omega, done);
+  hipLaunchKernelGGL((restrict_k<Tf, Tc>), dim3(static_cast<unsigned>((n + 255) / 256), nr),
+                     dim3(256), 0, s, gf, gc, rf, mask_c, bc, xc, dinv_c, omega, gf.ndof(),
+                     gc.ndof(), done);
   TOPOHIP_CHECK(hipGetLastError());
 }
 
 template <typename Tc, typename Tf>
 void launch_prolong_add(const Geom& gc, const Geom& gf, const Tc* xc, const uint8_t* mask_f,
-                        Tf* xf, const int* done, hipStream_t s) {
+                        Tf* xf, const int* done, hipStream_t s, int nr) {
   const long long n = gf.nodes();
-  hipLaunchKernelGGL((prolong_add_k<Tc, Tf>), dim3(static_cast<unsigned>((n + 255) / 256)),
-                     dim3(256), 0, s, gc, gf, xc, mask_f, xf, done);
+  hipLaunchKernelGGL((prolong_add_k<Tc, Tf>), dim3(static_cast<unsigned>((n + 255) / 256), nr),
+                     dim3(256), 0, s, gc, gf, xc, mask_f, xf, gc.ndof(), gf.ndof(), done);
   TOPOHIP_CHECK(hipGetLastError());
 }
 
 template void launch_restrict<float, float>(const Geom&, const Geom&, const float*, const uint8_t*,
                                             float*, float*, const float*, double, const int*,
-                                            hipStream_t);
+                                            hipStream_t, int);
 template void launch_restrict<double, double>(const Geom&, const Geom&, const double*,
                                               const uint8_t*, double*, double*, const double*,
-                                              double, const int*, hipStream_t);
+                                              double, const int*, hipStream_t, int);
 template void launch_restrict<float, double>(const Geom&, const Geom&, const float*,
                                              const uint8_t*, double*, double*, const double*,
-                                             double, const int*, hipStream_t);
+                                             double, const int*, hipStream_t, int);
 template void launch_prolong_add<float, float>(const Geom&, const Geom&, const float*,
-                                               const uint8_t*, float*, const int*, hipStream_t);
+                                               const uint8_t*, float*, const int*, hipStream_t, int);
 template void launch_prolong_add<double, double>(const Geom&, const Geom&, const double*,
-                                                 const uint8_t*, double*, const int*, hipStream_t);
+                                                 const uint8_t*, double*, const int*, hipStream_t,
+                                                 int);
 template void launch_prolong_add<double, float>(const Geom&, const Geom&, const double*,
-                                                const uint8_t*, float*, const int*, hipStream_t);
+                                                const uint8_t*, float*, const int*, hipStream_t,
+                                                int);
 
-void launch_sum_partials(const double* partials, int n, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(sum_partials_k, dim3(1), dim3(RB), 0, s, partials, n, out);
+void launch_sum_partials(const double* partials, int n, double* out, hipStream_t s, int nr,
+                         int pstride) {
+  hipLaunchKernelGGL(sum_partials_k, dim3(nr), dim3(RB), 0, s, partials, n, out, pstride);
   TOPOHIP_CHECK(hipGetLastError());
 }
 void launch_max_partials(const double* partials, int n, double* out, hipStream_t s) {
@@ -356,16 +402,17 @@ int dot_blocks(long long n) { return grid_for(n, RB * 4, 1024); }
 
 template <typename TA, typename TB>
 void launch_dot(const TA* a, const TB* b, long long n, double* partials, int nblocks,
-                const int* done, hipStream_t s) {
-  hipLaunchKernelGGL((dot_k<TA, TB>), dim3(nblocks), dim3(RB), 0, s, a, b, n, partials, done);
+                const int* done, hipStream_t s, int nr, int pstride) {
+  hipLaunchKernelGGL((dot_k<TA, TB>), dim3(nblocks, nr), dim3(RB), 0, s, a, b, n, partials,
+                     pstride, done);
   TOPOHIP_CHECK(hipGetLastError());
 }
 template void launch_dot<double, double>(const double*, const double*, long long, double*, int,
-                                         const int*, hipStream_t);
+                                         const int*, hipStream_t, int, int);
 template void launch_dot<double, float>(const double*, const float*, long long, double*, int,
-                                        const int*, hipStream_t);
+                                        const int*, hipStream_t, int, int);
 template void launch_dot<float, float>(const float*, const float*, long long, double*, int,
-                                       const int*, hipStream_t);
+                                       const int*, hipStream_t, int, int);
 
 template <typename T>
 void launch_sum_vec(const T* v, long long n, double* partials, int nblocks, hipStream_t s) {
@@ -396,69 +443,62 @@ template void launch_convert<float, double>(const float*, double*, long long, hi
 
 template <typename T>
 void launch_jacobi0(T* x, const T* dinv, const T* b, double omega, long long n, const int* done,
-                    hipStream_t s) {
-  hipLaunchKernelGGL((jacobi0_k<T>), dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, x, dinv, b,
-                     omega, n, done);
+                    hipStream_t s, int nr) {
+  hipLaunchKernelGGL((jacobi0_k<T>), dim3(grid_for(n, 256, 4096), nr), dim3(256), 0, s, x, dinv,
+                     b, omega, n, done);
   TOPOHIP_CHECK(hipGetLastError());
 }
 template void launch_jacobi0<float>(float*, const float*, const float*, double, long long,
-                                    const int*, hipStream_t);
+                                    const int*, hipStream_t, int);
 template void launch_jacobi0<double>(double*, const double*, const double*, double, long long,
-                                     const int*, hipStream_t);
+                                     const int*, hipStream_t, int);
 
 void launch_add(const double* x, const double* up, double* y, long long n, hipStream_t s) {
   hipLaunchKernelGGL(add_k, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, x, up, y, n);
   TOPOHIP_CHECK(hipGetLastError());
 }
 
-void launch_cg_init(CgState* st, const double* p, int n, double tol, int cap, hipStream_t s) {
-  hipLaunchKernelGGL(cg_init_k, dim3(1), dim3(RB), 0, s, st, p, n, tol, cap);
+void launch_cg_init(CgState* st, const double* p, int n, double tol, int cap, hipStream_t s,
+                    int nr, int pstride) {
+  hipLaunchKernelGGL(cg_init_k, dim3(nr), dim3(RB), 0, s, st, p, n, pstride, tol, cap);
   TOPOHIP_CHECK(hipGetLastError());
 }
-void launch_cg_set_rz(CgState* st, const double* p, int n, hipStream_t s) {
-  hipLaunchKernelGGL(cg_set_rz_k, dim3(1), dim3(RB), 0, s, st, p, n);
-  TOPOHIP_CHECK(hipGetLastError());
-}
-void launch_cg_resume(CgState* st, const double* p, int n, hipStream_t s) {
-  hipLaunchKernelGGL(cg_resume_k, dim3(1), dim3(RB), 0, s, st, p, n);
-  TOPOHIP_CHECK(hipGetLastError());
-}
-void launch_cg_alpha(CgState* st, const double* p, int n, hipStream_t s) {
-  hipLaunchKernelGGL(cg_alpha_k, dim3(1), dim3(RB), 0, s, st, p, n);
-  TOPOHIP_CHECK(hipGetLastError());
-}
-void launch_cg_check(CgState* st, const double* p, int n, hipStream_t s) {
-  hipLaunchKernelGGL(cg_check_k, dim3(1), dim3(RB), 0, s, st, p, n);
-  TOPOHIP_CHECK(hipGetLastError());
-}
-void launch_cg_beta(CgState* st, const double* p, int n, hipStream_t s) {
-  hipLaunchKernelGGL(cg_beta_k, dim3(1), dim3(RB), 0, s, st, p, n);
-  TOPOHIP_CHECK(hipGetLastError());
-}
+#define TOPOHIP_CG_SCALAR(NAME, KERNEL)                                                    \
+  void NAME(CgState* st, const double* p, int n, hipStream_t s, int nr, int pstride) {     \
+    hipLaunchKernelGGL(KERNEL, dim3(nr), dim3(RB), 0, s, st, p, n, pstride);               \
+    TOPOHIP_CHECK(hipGetLastError());                                                      \
+  }
+TOPOHIP_CG_SCALAR(launch_cg_set_rz, cg_set_rz_k)
+TOPOHIP_CG_SCALAR(launch_cg_resume, cg_resume_k)
+TOPOHIP_CG_SCALAR(launch_cg_alpha, cg_alpha_k)
+TOPOHIP_CG_SCALAR(launch_cg_check, cg_check_k)
+TOPOHIP_CG_SCALAR(launch_cg_beta, cg_beta_k)
+#undef TOPOHIP_CG_SCALAR
 
 template <typename TP, typename TA>
 void launch_cg_xr(CgState* st, double* x, double* r, const TP* p, const TA* Ap, float* r32,
-                  long long n, double* partials, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL((cg_xr_k<TP, TA>), dim3(nblocks), dim3(RB), 0, s, st, x, r, p, Ap, r32, n,
-                     partials);
+                  long long n, double* partials, int nblocks, hipStream_t s, int nr,
+                  int pstride) {
+  hipLaunchKernelGGL((cg_xr_k<TP, TA>), dim3(nblocks, nr), dim3(RB), 0, s, st, x, r, p, Ap, r32,
+                     n, partials, pstride);
   TOPOHIP_CHECK(hipGetLastError());
 }
 template void launch_cg_xr<double, double>(CgState*, double*, double*, const double*,
                                            const double*, float*, long long, double*, int,
-                                           hipStream_t);
+                                           hipStream_t, int, int);
 template void launch_cg_xr<float, float>(CgState*, double*, double*, const float*, const float*,
-                                         float*, long long, double*, int, hipStream_t);
+                                         float*, long long, double*, int, hipStream_t, int, int);
 
 template <typename TZ>
 void launch_cg_p(CgState* st, const TZ* z, double* p, float* p32, long long n, int restart,
-                 hipStream_t s) {
-  hipLaunchKernelGGL((cg_p_k<TZ>), dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, st, z, p, p32,
-                     n, restart);
+                 hipStream_t s, int nr) {
+  hipLaunchKernelGGL((cg_p_k<TZ>), dim3(grid_for(n, 256, 4096), nr), dim3(256), 0, s, st, z, p,
+                     p32, n, restart);
   TOPOHIP_CHECK(hipGetLastError());
 }
 template void launch_cg_p<float>(CgState*, const float*, double*, float*, long long, int,
-                                 hipStream_t);
+                                 hipStream_t, int);
 template void launch_cg_p<double>(CgState*, const double*, double*, float*, long long, int,
-                                  hipStream_t);
+                                  hipStream_t, int);
 
 }  // namespace topohip
