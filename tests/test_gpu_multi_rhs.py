@@ -22,6 +22,15 @@ def setup_module():
     oracle.set_threads(0)
 
 
+def smooth_density(nx, ny, nz, bcs, loads, seed=7):
+    """A filtered random field in [0.2, 1]: spatially smooth like a SIMP design.
+    (Uncorrelated per-voxel moduli with a 125:1 contrast need ~200 MG-PCG
+    iterations in the reference algorithm and exhaust its 300-cycle budget
+    for the self-weight case, which the oracle reproduces.)"""
+    x = np.random.default_rng(seed).uniform(0.2, 1.0, nx * ny * nz)
+    return oracle.filter(oracle.Setup(nx, ny, nz, bcs=bcs, loads=loads), 1.5, x)
+
+
 def bridge_cases(nx, ny, nz):
     bcs, cases = problems.bridge(nx, ny, nz)
     cases.append(problems.self_weight(nx, ny, nz))
@@ -33,12 +42,13 @@ def test_block_compliance_matches_sum_of_oracle_solves(precision):
     nx, ny, nz = DIMS
     print("multi-rhs compliance", DIMS, "precision", precision, flush=True)
     bcs, cases = bridge_cases(nx, ny, nz)
-    rng = np.random.default_rng(7)
-    x = rng.uniform(0.2, 1.0, nx * ny * nz)
-    c_ref, dc_ref, u0_ref = 0.0, np.zeros(nx * ny * nz), None
+    x = smooth_density(nx, ny, nz, bcs, cases[0])
+    c_ref, dc_ref, u0_ref, its_ref = 0.0, np.zeros(nx * ny * nz), None, []
     for r, case in enumerate(cases):
         s = oracle.Setup(nx, ny, nz, bcs=bcs, loads=case)
-        c, dc, u, _ = oracle.simp_compliance(s, x, tol=1e-10)
+        c, dc, u, info = oracle.simp_compliance(s, x, tol=1e-10)
+        assert info["used_multigrid"]
+        its_ref.append(info["iterations"])
         c_ref += c
         dc_ref += dc
         if r == 0:
@@ -50,21 +60,30 @@ def test_block_compliance_matches_sum_of_oracle_solves(precision):
     assert np.abs(dc - dc_ref).max() / np.abs(dc_ref).max() < 1e-7
     assert np.abs(u0 - u0_ref).max() / np.abs(u0_ref).max() < 1e-6
     assert info["converged"] == 1 and info["used_multigrid"] == 1
+    # block iterations = the slowest case's. The FP64 form runs the reference's
+    # MG-PCG and matches its count; the FP32 V-cycle needs more at this tight
+    # 1e-10 tolerance (the reference gates iters32 <= iters64 + 8 at 1e-8).
+    if precision == ta.PREC_F64:
+        assert max(its_ref) - 2 <= info["iterations"] <= max(its_ref) + 2
 
 
 def test_block_equals_one_case_at_a_time():
     nx, ny, nz = DIMS
     bcs, cases = bridge_cases(nx, ny, nz)
-    x = np.random.default_rng(3).uniform(0.1, 1.0, nx * ny * nz)
+    x = smooth_density(nx, ny, nz, bcs, cases[0], seed=3)
     with ta.Problem(nx, ny, nz, bcs, cases) as p:
-        c_blk, dc_blk, _, _ = p.compliance(x, tol=1e-10)
+        c_blk, dc_blk, u_blk, _ = p.compliance(x, tol=1e-10, want_u=True)
     c_one, dc_one = 0.0, np.zeros_like(dc_blk)
-    for case in cases:
+    for r, case in enumerate(cases):
         with ta.Problem(nx, ny, nz, bcs, case) as p:
-            c, dc, _, _ = p.compliance(x, tol=1e-10)
+            c, dc, u, _ = p.compliance(x, tol=1e-10, want_u=True)
         c_one += c
         dc_one += dc
-    # each case's recurrence is independent of the others in the block
+        if r == 0:
+            u_one = u
+    # each case's recurrence is independent of the others in the block, and
+    # its kernels see the same data, so the displacement is bit-identical
+    assert np.array_equal(u_blk, u_one)
     assert abs(c_blk - c_one) <= 1e-12 * c_one
     assert np.abs(dc_blk - dc_one).max() <= 1e-12 * np.abs(dc_one).max()
 

@@ -299,7 +299,7 @@ __global__ void cr_scatter_k(const double* V, long long vsz, long long off,
 
 }  // namespace
 
-bool CoarseCR::plan(const Geom& g) {
+bool CoarseCR::plan(const Geom& g, hipStream_t s) {
   geom = g;
   const int dims[3] = {g.nnx, g.nny, g.nnz};
   axis = 0;
@@ -445,10 +445,14 @@ bool CoarseCR::plan(const Geom& g) {
   lv_gd.push_back(static_cast<int>(gemm_d.size()));
   lv_fw.push_back(static_cast<int>(fwd.size()));
   lv_bw.push_back(static_cast<int>(bwd.size()));
-  auto up = [](auto& buf, const auto& host) {
+  // Uploads go through the problem's own stream: a legacy-stream copy would
+  // have to order itself against other problems' streams, which may be
+  // capturing a graph at that moment.
+  auto up = [&](auto& buf, const auto& host) {
     buf.ensure(host.size());
     if (!host.empty())
-      TOPOHIP_CHECK(hipMemcpy(buf.p, host.data(), host.size() * sizeof(host[0]), hipMemcpyHostToDevice));
+      TOPOHIP_CHECK(hipMemcpyAsync(buf.p, host.data(), host.size() * sizeof(host[0]),
+                                   hipMemcpyHostToDevice, s));
   };
   W.ensure(static_cast<size_t>(wsize));
   size_t maxjobs = 1;
@@ -471,6 +475,7 @@ bool CoarseCR::plan(const Geom& g) {
   up(d_plane, plane_of);
   up(d_local, local_of);
   up(d_dst, dst);
+  TOPOHIP_CHECK(hipStreamSynchronize(s));  // host tables go out of scope
   return true;
 }
 

@@ -38,7 +38,7 @@ class CoarseCR {
  public:
   // Plans the reduction for a level of geometry g (false if a plane holds
   // more than 256 DOFs).
-  bool plan(const Geom& g);
+  bool plan(const Geom& g, hipStream_t s);
   // Factor from the level's element-major FP64 element matrices.
   void build(const double* Kem, const uint8_t* mask, hipStream_t s);
   // Vector space for nr load cases solved together.

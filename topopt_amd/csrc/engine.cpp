@@ -115,6 +115,7 @@ void hex8_stiffness_host(double E, double nu, double h, double* K) {
 }
 
 Problem::~Problem() {
+  std::lock_guard<std::recursive_mutex> lk(device_api_mutex());
   for (auto& g : cg_graph)
     if (g) (void)hipGraphExecDestroy(g);
   if (st_h) (void)hipHostFree(st_h);
@@ -348,7 +349,7 @@ void alloc_levels(Problem& P) {
       }
     }
   }
-  if (P.mg_plan_ok && !P.cr.plan(P.L.back()->g)) P.mg_plan_ok = false;
+  if (P.mg_plan_ok && !P.cr.plan(P.L.back()->g, P.s)) P.mg_plan_ok = false;
 }
 
 int max_partials(const Problem& P) {
@@ -563,10 +564,12 @@ void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool
   const int gi = static_cast<int>(pc);
   hipGraphExec_t& gx = P.cg_graph[gi];
   if (gx && P.cg_graph_nc[gi] != nc) {
+    std::lock_guard<std::recursive_mutex> lk(device_api_mutex());
     (void)hipGraphExecDestroy(gx);
     gx = nullptr;
   }
   if (!gx) {
+    std::lock_guard<std::recursive_mutex> lk(device_api_mutex());
     hipGraph_t gr;
     TOPOHIP_CHECK(hipStreamBeginCapture(P.s, hipStreamCaptureModeRelaxed));
     for (int i = 0; i < batch; ++i) iteration(P, pc, nc);
@@ -709,6 +712,7 @@ void autotune_ku(Problem& P) {
 }
 
 void drop_graphs(Problem& P) {
+  std::lock_guard<std::recursive_mutex> lk(device_api_mutex());
   for (auto& g : P.cg_graph)
     if (g) {
       (void)hipGraphExecDestroy(g);
@@ -735,16 +739,21 @@ void ensure_cases(Problem& P, int nb) {
   for (size_t l = 0; l < P.L.size(); ++l) {
     Level& lv = *P.L[l];
     const size_t ndl = static_cast<size_t>(lv.g.ndof()) * nb;
-    if (mixed) { lv.x32.ensure(ndl); lv.t32.ensure(ndl); }
-    else { lv.x64.ensure(ndl); lv.t64.ensure(ndl); }
-    if (l > 0) {
-      if (mixed) lv.b32.ensure(ndl);
-      else lv.b64.ensure(ndl);
+    if (mixed) {
+      lv.x32.ensure(ndl);
+      lv.t32.ensure(ndl);
+      if (l > 0) lv.b32.ensure(ndl);
+    }
+    if (!mixed || P.mg64_alloc) {
+      lv.x64.ensure(ndl);
+      lv.t64.ensure(ndl);
+      if (l > 0) lv.b64.ensure(ndl);
     }
   }
   P.st.ensure(static_cast<size_t>(nb));
   P.scal.ensure(static_cast<size_t>(std::max(8, nb)));
   if (nb > P.nb || !P.st_h) {
+    std::lock_guard<std::recursive_mutex> lk(device_api_mutex());
     if (P.st_h) (void)hipHostFree(P.st_h);
     if (P.scal_h) (void)hipHostFree(P.scal_h);
     P.st_h = nullptr;
@@ -755,6 +764,29 @@ void ensure_cases(Problem& P, int nb) {
   P.partials.ensure(static_cast<size_t>(nb) * P.pstride);
   if (P.mg_plan_ok) P.cr.reserve_cases(nb);
   P.nb = nb;
+}
+
+// FP64 V-cycle data of the mixed form, built on demand for the FP64 retry of a
+// mixed MG-PCG that did not converge (the reference retries its FP32 V-cycle
+// in FP64 before the Jacobi-CG fallback, core/src/fea/multigrid.cpp:2098-2319).
+void ensure_mg64(Problem& P) {
+  if (P.precision != TOPOHIP_PREC_MIXED || !P.mg_plan_ok) return;
+  if (!P.mg64_alloc) {
+    for (size_t l = 1; l + 1 < P.L.size(); ++l)
+      P.L[l]->K64.ensure(static_cast<size_t>(P.L[l]->g.nodes()) * 126);
+    P.mg64_alloc = true;
+    const int nb = P.nb;
+    P.nb = 0;  // re-run the capacity pass so the FP64 level vectors exist
+    ensure_cases(P, nb);
+  }
+  if (!P.mg64_ready) {
+    for (size_t l = 1; l + 1 < P.L.size(); ++l) {
+      Level& c = *P.L[l];
+      launch_diag_dense<double>(c.g, c.Kem.p, c.mask.p, c.dinv64.p, P.s);
+      launch_assemble_stencil<double>(c.g, c.Kem.p, c.K64.p, P.s);
+    }
+    P.mg64_ready = true;
+  }
 }
 
 std::unique_ptr<Problem> create_problem(int nx, int ny, int nz, double h, double nu,
@@ -802,7 +834,10 @@ std::unique_ptr<Problem> create_problem(int nx, int ny, int nz, double h, double
 
   TOPOHIP_CHECK(hipSetDevice(device));
   P->device = device;
-  TOPOHIP_CHECK(hipStreamCreateWithFlags(&P->s, hipStreamNonBlocking));
+  {
+    std::lock_guard<std::recursive_mutex> lk(device_api_mutex());
+    TOPOHIP_CHECK(hipStreamCreateWithFlags(&P->s, hipStreamNonBlocking));
+  }
   if (solid) {
     P->solid.ensure(static_cast<size_t>(nv));
     h2d(P->solid.p, P->solid_h.data(), static_cast<size_t>(nv), P->s);
@@ -817,7 +852,10 @@ std::unique_ptr<Problem> create_problem(int nx, int ny, int nz, double h, double
   h2d(P->dkdiag.p, P->kdiag, 24, P->s);
   P->E64.ensure(static_cast<size_t>(nv));
   P->E32.ensure(static_cast<size_t>(nv));
-  TOPOHIP_CHECK(hipHostMalloc(&P->oc_h, sizeof(OcState)));
+  {
+    std::lock_guard<std::recursive_mutex> lk(device_api_mutex());
+    TOPOHIP_CHECK(hipHostMalloc(&P->oc_h, sizeof(OcState)));
+  }
   P->oc.ensure(1);
   plan_levels(*P);
   alloc_levels(*P);
@@ -890,6 +928,7 @@ void set_loads(Problem& P, int n_rhs, const int32_t* counts, const int32_t* node
 
 void build_hierarchy(Problem& P, Timers& t) {
   const double t0 = now_ms();
+  P.mg64_ready = false;
   const long long nv = P.g.elems();
   const bool mixed = P.precision == TOPOHIP_PREC_MIXED;
   launch_convert<double, float>(P.E64.p, P.E32.p, nv, P.s);
@@ -1007,6 +1046,37 @@ void solve_cases(Problem& P, int nc, double tol, int max_it, const double* d_gue
     }
     bool all_ok = true;
     for (int r = 0; r < nc; ++r) all_ok = all_ok && ok[r];
+    if (!all_ok && P.mg_plan_ok && P.precision == TOPOHIP_PREC_MIXED) {
+      // FP64 V-cycle retry of the cases the mixed solve did not finish.
+      ensure_mg64(P);
+      const double tm = now_ms();
+      std::unique_ptr<bool[]> skip(new bool[nc]);
+      for (int r = 0; r < nc; ++r) {
+        skip[r] = ok[r];
+        if (ok[r]) continue;
+        if (warm) h2d(P.x.p, gh.data(), static_cast<size_t>(nd), P.s);
+        else launch_fill<double>(P.x.p + r * nd, 0.0, nd, P.s);
+      }
+      std::vector<int> it64(static_cast<size_t>(nc), 0);
+      pcg(P, Prec::MG64, nc, tol, std::min(cap, kMgIterBudget), warm, skip.get(), it64.data(),
+          res.data(), ok.get());
+      const double tmg = now_ms() - tm;
+      all_ok = true;
+      for (int r = 0; r < nc; ++r) {
+        all_ok = all_ok && ok[r];
+        if (skip[r]) continue;
+        d[r].mg_cycles_attempted += it64[r];
+        d[r].t_mg_ms += tmg;
+        it[r] = it64[r];
+        if (ok[r]) {
+          d[r].iterations = it64[r];
+          d[r].residual = res[r];
+          d[r].used_multigrid = 1;
+          d[r].mg_levels = static_cast<int>(P.L.size());
+          d[r].coarse_dofs = static_cast<int>(P.L.back()->nact);
+        }
+      }
+    }
     if (!all_ok) {
       // Exact Jacobi-PCG fallback (core/src/fea/matfree.cpp:841-1076), FP64, for
       // the cases the multigrid solve did not finish; the others keep their x.

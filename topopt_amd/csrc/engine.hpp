@@ -79,6 +79,8 @@ struct Problem {
 
   std::vector<std::unique_ptr<Level>> L;
   bool mg_plan_ok = false;
+  bool mg64_alloc = false;   // mixed form: FP64 V-cycle buffers exist (retry path)
+  bool mg64_ready = false;   // ... and hold the current hierarchy
   CoarseCR cr;  // exact coarsest-level solver
 
   // PCG vectors
