@@ -40,6 +40,19 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(workload):
+    """HBM bytes per fine K.u launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads for a
+    plain copy of known size), or None when no summary covers the workload."""
+    path = os.path.join(ROOT, "profiles", f"ku_fine_f32_{workload.split('_', 1)[-1]}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def ku_bytes(nx, ny, nz, word):
     ndof = 3 * (nx + 1) * (ny + 1) * (nz + 1)
     return 2 * word * ndof + word * nx * ny * nz, ndof
@@ -207,8 +220,9 @@ def main():
         "fp64_form": fp64,
         "ku": ku,
         "roofline": {"bound": "hbm", "achieved": ku["f32"]["GBs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": ku["f32"]["hbm_frac"], "traffic": None,
-                     "kernel": "ku_tiled<float> fine modal apply",
+                     "unit": "GB/s", "frac": ku["f32"]["hbm_frac"],
+                     "traffic": measured_traffic(a.workload),
+                     "kernel": "ku_face<float> fine modal apply (autotuned tile)",
                      "bytes_per_launch": ku["f32"]["bytes"], "us_per_launch": ku["f32"]["ms"] * 1e3},
         "cpu_baseline": cpu,
     }

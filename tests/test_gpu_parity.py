@@ -44,6 +44,29 @@ def test_ku_apply_matches_oracle(dims):
     assert np.array_equal(y32, y32b)  # bitwise repeatable
 
 
+def test_ku_apply_every_tile_shape_is_bit_identical(monkeypatch):
+    """Every launch shape the autotuner may pick (x-y tile, z-chunk) gives the
+    same bits: each node sums its 4 face contributions in a fixed order."""
+    nx, ny, nz = 70, 20, 19
+    rng = np.random.default_rng(11)
+    u = rng.uniform(-1, 1, 3 * (nx + 1) * (ny + 1) * (nz + 1))
+    E = rng.uniform(1e-3, 1, nx * ny * nz) ** 3
+    y_ref = oracle.apply(oracle.Setup(nx, ny, nz), E, u)
+    first = None
+    for tile in ["16,16,8", "32,8,2", "64,4,16", "32,16,4", "64,8,8", "32,16,16"]:
+        monkeypatch.setenv("TOPOHIP_KU_TILE", tile)
+        with ta.Problem(nx, ny, nz) as p:
+            y32 = p.apply(E, u, "f32")
+            y64 = p.apply(E, u, "f64")
+        print("tile", tile, f"f32 relL2={rel_l2(y32, y_ref):.3e}", flush=True)
+        assert rel_l2(y64, y_ref) < 1e-12
+        assert rel_l2(y32, y_ref) < 1e-6
+        if first is None:
+            first = (y32, y64)
+        else:
+            assert np.array_equal(y32, first[0]) and np.array_equal(y64, first[1])
+
+
 def test_ku_apply_with_voids_and_tags():
     nx, ny, nz = 10, 8, 6
     rng = np.random.default_rng(1)

@@ -73,7 +73,7 @@ void apply_host(Problem& P, const double* youngs, const double* u, double* y) {
   TOPOHIP_CHECK(hipMemcpyAsync(u64.p, u, nd * sizeof(double), hipMemcpyHostToDevice, P.s));
   KuOp<T> op;
   op.g = P.g;
-  op.tz = P.tz; op.tx = P.tx;
+  op.tz = P.tz; op.tx = P.tx; op.ty = P.ty;
   op.epi = EPI_AX;
   if constexpr (sizeof(T) == 4) {
     DBuf<float> u32, y32;
@@ -284,12 +284,12 @@ int topohip_bench_ku(topohip_problem* p, int precision, int reps, double* ms_per
     auto run = [&]() {
       if (precision == TOPOHIP_PREC_F64) {
         KuOp<double> op;
-        op.g = P.g; op.tz = P.tz; op.tx = P.tx; op.epi = EPI_AX; op.E = P.E64.p; op.mg = P.mg64;
+        op.g = P.g; op.tz = P.tz; op.tx = P.tx; op.ty = P.ty; op.epi = EPI_AX; op.E = P.E64.p; op.mg = P.mg64;
         op.x = p->bu64.p; op.y = p->by64.p;
         launch_ku_fine<double>(op, P.s);
       } else {
         KuOp<float> op;
-        op.g = P.g; op.tz = P.tz; op.tx = P.tx; op.epi = EPI_AX; op.E = P.E32.p; op.mg = P.mg32;
+        op.g = P.g; op.tz = P.tz; op.tx = P.tx; op.ty = P.ty; op.epi = EPI_AX; op.E = P.E32.p; op.mg = P.mg32;
         op.x = p->bu32.p; op.y = p->by32.p;
         launch_ku_fine<float>(op, P.s);
       }

@@ -353,7 +353,7 @@ void alloc_levels(Problem& P) {
 }
 
 int max_partials(const Problem& P) {
-  return std::max(ku_blocks(P.g, P.tz, P.tx), std::max(dot_blocks(P.g.ndof()), sens_blocks(P.g)));
+  return std::max(ku_blocks(P.g, P.tz, P.tx, P.ty), std::max(dot_blocks(P.g.ndof()), sens_blocks(P.g)));
 }
 
 // ---- V-cycle ----------------------------------------------------------------
@@ -394,6 +394,7 @@ KuOp<T> fine_op(Problem& P, const int* done, int nc) {
   op.mg = Mg<T>(P);
   op.tz = P.tz;
   op.tx = P.tx;
+  op.ty = P.ty;
   op.done = done;
   op.nrhs = nc;
   op.pstride = P.pstride;
@@ -474,7 +475,7 @@ void restart(Problem& P, Prec pc, int nc) {
 void iteration(Problem& P, Prec pc, int nc) {
   const long long nd = P.g.ndof();
   const int* done = done_flags(P);
-  const int kb = ku_blocks(P.g, P.tz, P.tx);
+  const int kb = ku_blocks(P.g, P.tz, P.tx, P.ty);
   const int db = dot_blocks(nd);
   if (pc == Prec::MG32) {
     fine_apply<float>(P, P.p32.p, P.Ap32.p, nullptr, EPI_AX, P.partials.p, done, nc);
@@ -641,14 +642,19 @@ void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool
 
 }  // namespace
 
-// Tile autotuner: time the fine-level apply for each (tile width, z-chunk)
-// candidate on the problem's own buffers and keep the fastest. The choice only
-// changes the launch shape; every candidate computes bit-identical results
-// (the per-node summation order does not depend on the tile).
+// Tile autotuner: time the fine-level apply for each (tile, z-chunk)
+// candidate on the problem's own buffers and keep the fastest (override:
+// TOPOHIP_KU_TILE=tx,ty,tz). The choice only changes the launch shape; every
+// candidate computes bit-identical applies (the per-node summation order does
+// not depend on the tile).
 void autotune_ku(Problem& P) {
   if (const char* env = std::getenv("TOPOHIP_KU_TILE")) {
-    int tx = 16, tz = 8;
-    if (std::sscanf(env, "%d,%d", &tx, &tz) == 2 && tz > 0) { P.tx = tx; P.tz = tz; }
+    int tx = 16, ty = 16, tz = 8;
+    if (std::sscanf(env, "%d,%d,%d", &tx, &ty, &tz) == 3 && tz > 0 && ku_tile_supported(tx, ty)) {
+      P.tx = tx;
+      P.ty = ty;
+      P.tz = tz;
+    }
     return;
   }
   if (P.g.elems() < 4096) return;
@@ -656,14 +662,15 @@ void autotune_ku(Problem& P) {
   // process: the tile decides the order of the PCG's partial dot products, so
   // problems of one shape then give identical results whatever else runs.
   static std::mutex mu;
-  static std::map<std::array<int, 5>, std::pair<int, int>> cache;
+  static std::map<std::array<int, 5>, std::array<int, 3>> cache;
   const std::array<int, 5> key{P.device, P.g.ex, P.g.ey, P.g.ez, P.precision};
   {
     std::lock_guard<std::mutex> lk(mu);
     auto it = cache.find(key);
     if (it != cache.end()) {
-      P.tx = it->second.first;
-      P.tz = it->second.second;
+      P.tx = it->second[0];
+      P.ty = it->second[1];
+      P.tz = it->second[2];
       return;
     }
   }
@@ -677,19 +684,19 @@ void autotune_ku(Problem& P) {
   TOPOHIP_CHECK(hipEventCreate(&e0));
   TOPOHIP_CHECK(hipEventCreate(&e1));
   float best = 1e30f;
-  const int txs[3] = {16, 32, 64};
+  const int tiles[5][2] = {{16, 16}, {32, 8}, {64, 4}, {32, 16}, {64, 8}};
   const int tzs[4] = {2, 4, 8, 16};
-  for (int tx : txs)
+  for (const auto& tl : tiles)
     for (int tz : tzs) {
       auto run = [&]() {
         if (mixed) {
           KuOp<float> op;
-          op.g = P.g; op.E = P.E32.p; op.mg = P.mg32; op.tz = tz; op.tx = tx;
+          op.g = P.g; op.E = P.E32.p; op.mg = P.mg32; op.tz = tz; op.tx = tl[0]; op.ty = tl[1];
           op.x = P.p32.p; op.y = P.Ap32.p; op.epi = EPI_AX;
           launch_ku_fine<float>(op, P.s);
         } else {
           KuOp<double> op;
-          op.g = P.g; op.E = P.E64.p; op.mg = P.mg64; op.tz = tz; op.tx = tx;
+          op.g = P.g; op.E = P.E64.p; op.mg = P.mg64; op.tz = tz; op.tx = tl[0]; op.ty = tl[1];
           op.x = P.x.p; op.y = P.Ap64.p; op.epi = EPI_AX;
           launch_ku_fine<double>(op, P.s);
         }
@@ -701,14 +708,15 @@ void autotune_ku(Problem& P) {
       TOPOHIP_CHECK(hipEventSynchronize(e1));
       float ms = 0.f;
       TOPOHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) { best = ms; P.tx = tx; P.tz = tz; }
+      if (ms < best) { best = ms; P.tx = tl[0]; P.ty = tl[1]; P.tz = tz; }
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   std::lock_guard<std::mutex> lk(mu);
-  auto ins = cache.emplace(key, std::make_pair(P.tx, P.tz));
-  P.tx = ins.first->second.first;  // a concurrent tuner of the same shape may have won
-  P.tz = ins.first->second.second;
+  auto ins = cache.emplace(key, std::array<int, 3>{P.tx, P.ty, P.tz});
+  P.tx = ins.first->second[0];  // a concurrent tuner of the same shape may have won
+  P.ty = ins.first->second[1];
+  P.tz = ins.first->second[2];
 }
 
 void drop_graphs(Problem& P) {
@@ -979,6 +987,7 @@ void solve_cases(Problem& P, int nc, double tol, int max_it, const double* d_gue
     op.mg = P.mg64;
     op.tz = P.tz;
     op.tx = P.tx;
+    op.ty = P.ty;
     op.x = P.up.p;
     op.y = P.rt.p;
     op.epi = EPI_AX;

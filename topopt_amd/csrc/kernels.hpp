@@ -25,13 +25,15 @@ struct KuOp {
   int epi = EPI_AX;
   double* partials = nullptr;
   int tz = 8;                     // node planes per workgroup (z-march length)
-  int tx = 16;                    // workgroup tile: tx x (256/tx) element columns
+  int tx = 16;                    // workgroup tile: tx x ty element columns (lanes)
+  int ty = 16;
   const int* done = nullptr;      // case r exits early when done[r * kDoneStride] != 0
   int nrhs = 1;                   // stacked load cases (x, y, b stride g.ndof())
   int pstride = 0;                // partials of case r at partials + r * pstride
 };
 
-int ku_blocks(const Geom& g, int tz, int tx);
+bool ku_tile_supported(int tx, int ty);
+int ku_blocks(const Geom& g, int tz, int tx, int ty);
 template <typename T> void launch_ku_fine(const KuOp<T>& op, hipStream_t s);
 
 // ---- transfers between a fine level f and the coincident coarse level c ----

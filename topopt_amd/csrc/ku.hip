@@ -3,19 +3,21 @@
 // Replaces the colour-ordered element loop of core/src/fea/matfree.cpp:75-123,
 // 535-553 (gather 24 / 24x24 column AXPY / scatter 24). Design for gfx950:
 //
-//  * A workgroup owns a (TX-1) x (TY-1) tile of nodes in x-y and marches a
-//    z-chunk of node planes. Each lane handles one element column (one halo
-//    element on the low x and y side); node planes are staged in LDS, double
-//    buffered, with the next plane (and the next layer's moduli, masks and
-//    epilogue operands) prefetched into registers one layer ahead.
+//  * A workgroup of TX x TY lanes (256 or 512) owns a (TX-1) x (TY-1) tile of
+//    nodes in x-y and marches a z-chunk of node planes. Each lane handles one
+//    element column (one halo element on the low x and y side); node planes
+//    are staged in LDS, double buffered, with the next plane (and the next
+//    layer's moduli, masks and epilogue operands) prefetched into registers
+//    one layer ahead.
 //  * The element matrix is applied in the Hadamard (parity-mode) basis of the
-//    8 corners, where E*Ke has 45 non-zeros: butterfly, 21 scalings by E,
-//    45 FMA, inverse butterfly. The butterfly factorises by axis, and its x-y
-//    stages act on one node plane at a time, so each lane transforms the 4-node
-//    face of its column once per plane and shares it between the element below
-//    and the element above; on the way back the z stage splits each element's
-//    result into a bottom and a top face, the two faces that meet on a plane
-//    are added first, and the inverse x-y stages run once per plane.
+//    8 corners, where E*Ke has 45 non-zeros taking 8 distinct values (the
+//    grouped core below: 45 multiply-adds including the scaling by E). The
+//    butterfly factorises by axis, and its x-y stages act on one node plane at
+//    a time, so each lane transforms the 4-node face of its column once per
+//    plane and shares it between the element below and the element above; on
+//    the way back the z stage splits each element's result into a bottom and a
+//    top face, the two faces that meet on a plane are added first, and the
+//    inverse x-y stages run once per plane.
 //  * Node-owned deterministic accumulation: every lane writes its face's 4
 //    corner results to LDS and the owning lane sums its 4 face contributions
 //    in a fixed order. No atomics, bit-identical run to run.
@@ -24,8 +26,6 @@
 namespace topohip {
 
 namespace {
-
-constexpr int NT = 256;
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -47,6 +47,11 @@ __device__ inline void face_fwd(const T (&u)[12], T (&F)[12]) {
   }
 }
 
+// The K.u kernel is compiled without automatic FP contraction and spells out
+// its fused multiply-adds, so every tile shape (whose code the compiler
+// schedules differently) produces the same bits.
+#pragma clang fp contract(off)
+
 // v = E * C A for the 24 modal amplitudes A, with C in its grouped form
 // (8 distinct values): 45 multiply-adds, no per-entry coefficient registers.
 template <typename T>
@@ -55,22 +60,24 @@ __device__ inline void modal_core(const T (&A)[24], T (&v)[24], T e, const Modal
   const T eg = e * mg.gmh, eh = e * mg.h, ek = e * mg.k;
   v[0] = v[1] = v[2] = T(0);  // rigid translations
   const T sn = (A[3] + A[7]) + A[14];
-  v[3] = ea * A[3] + eb * sn;
-  v[7] = ea * A[7] + eb * sn;
-  v[14] = ea * A[14] + eb * sn;
+  const T bn = eb * sn;
+  v[3] = fma(ea, A[3], bn);
+  v[7] = fma(ea, A[7], bn);
+  v[14] = fma(ea, A[14], bn);
   v[4] = v[6] = ed * (A[4] + A[6]);
   v[5] = v[12] = ed * (A[5] + A[12]);
   v[8] = v[13] = ed * (A[8] + A[13]);
-  v[9] = ee * A[9] + ef * A[20];
-  v[20] = ef * A[9] + ee * A[20];
-  v[10] = ee * A[10] + ef * A[17];
-  v[17] = ef * A[10] + ee * A[17];
-  v[15] = ee * A[15] + ef * A[19];
-  v[19] = ef * A[15] + ee * A[19];
+  v[9] = fma(ee, A[9], ef * A[20]);
+  v[20] = fma(ee, A[20], ef * A[9]);
+  v[10] = fma(ee, A[10], ef * A[17]);
+  v[17] = fma(ee, A[17], ef * A[10]);
+  v[15] = fma(ee, A[15], ef * A[19]);
+  v[19] = fma(ee, A[19], ef * A[15]);
   const T su = (A[11] + A[16]) + A[18];
-  v[11] = eg * A[11] + eh * su;
-  v[16] = eg * A[16] + eh * su;
-  v[18] = eg * A[18] + eh * su;
+  const T hu = eh * su;
+  v[11] = fma(eg, A[11], hu);
+  v[16] = fma(eg, A[16], hu);
+  v[18] = fma(eg, A[18], hu);
   v[21] = ek * A[21];
   v[22] = ek * A[22];
   v[23] = ek * A[23];
@@ -78,9 +85,9 @@ __device__ inline void modal_core(const T (&A)[24], T (&v)[24], T e, const Modal
 
 // Inputs x must be zero on inactive DOFs (every solver vector is); the mask
 // only decides which outputs are kept.
-template <typename T, int TX, int EPI, bool DOT>
-__global__ void __launch_bounds__(NT) ku_face(KuOp<T> a) {
-  constexpr int TY = NT / TX;
+template <typename T, int TX, int TY, int EPI, bool DOT>
+__global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
+  constexpr int NT = TX * TY;
   constexpr int PW = TX + 1;                  // staged plane: PW x (TY + 1) nodes
   constexpr int PLANE = PW * (TY + 1) * 3;
   constexpr int NPF = (PLANE + NT - 1) / NT;  // prefetch registers per lane
@@ -248,9 +255,9 @@ __global__ void __launch_bounds__(NT) ku_face(KuOp<T> a) {
           } else if constexpr (EPI == EPI_RES) {
             out = bo[c] - ax;
           } else {
-            out = xo[c] + a.omega * dv[c] * (bo[c] - ax);
+            out = fma(a.omega * dv[c], bo[c] - ax, xo[c]);
           }
-          if constexpr (DOT) dotp += static_cast<double>(xo[c]) * static_cast<double>(ax);
+          if constexpr (DOT) dotp = fma(static_cast<double>(xo[c]), static_cast<double>(ax), dotp);
         }
         yp[c] = out;
       }
@@ -274,13 +281,12 @@ __global__ void __launch_bounds__(NT) ku_face(KuOp<T> a) {
   }
 }
 
-template <typename T, int TX>
-void dispatch_tx(const KuOp<T>& op, hipStream_t s) {
-  constexpr int TY = NT / TX;
+template <typename T, int TX, int TY>
+void dispatch_tile(const KuOp<T>& op, hipStream_t s) {
   const dim3 grid((op.g.nnx + TX - 2) / (TX - 1), (op.g.nny + TY - 2) / (TY - 1),
                   ((op.g.nnz + op.tz - 1) / op.tz) * op.nrhs);
   const bool dot = op.partials != nullptr;
-#define TOPOHIP_L(E, D) hipLaunchKernelGGL((ku_face<T, TX, E, D>), grid, dim3(NT), 0, s, op)
+#define TOPOHIP_L(E, D) hipLaunchKernelGGL((ku_face<T, TX, TY, E, D>), grid, dim3(TX * TY), 0, s, op)
   if (op.epi == EPI_AX) {
     if (dot) TOPOHIP_L(EPI_AX, true); else TOPOHIP_L(EPI_AX, false);
   } else if (op.epi == EPI_RES) {
@@ -294,17 +300,23 @@ void dispatch_tx(const KuOp<T>& op, hipStream_t s) {
 
 }  // namespace
 
-int ku_blocks(const Geom& g, int tz, int tx) {
-  if (tx != 32 && tx != 64) tx = 16;
-  const int ty = NT / tx;
+bool ku_tile_supported(int tx, int ty) {
+  return (tx == 16 && ty == 16) || (tx == 32 && ty == 8) || (tx == 64 && ty == 4) ||
+         (tx == 32 && ty == 16) || (tx == 64 && ty == 8);
+}
+
+int ku_blocks(const Geom& g, int tz, int tx, int ty) {
+  if (!ku_tile_supported(tx, ty)) { tx = 16; ty = 16; }
   return ((g.nnx + tx - 2) / (tx - 1)) * ((g.nny + ty - 2) / (ty - 1)) * ((g.nnz + tz - 1) / tz);
 }
 
 template <typename T>
 void launch_ku_fine(const KuOp<T>& op, hipStream_t s) {
-  if (op.tx == 32) dispatch_tx<T, 32>(op, s);
-  else if (op.tx == 64) dispatch_tx<T, 64>(op, s);
-  else dispatch_tx<T, 16>(op, s);
+  if (op.tx == 32 && op.ty == 8) dispatch_tile<T, 32, 8>(op, s);
+  else if (op.tx == 64 && op.ty == 4) dispatch_tile<T, 64, 4>(op, s);
+  else if (op.tx == 32 && op.ty == 16) dispatch_tile<T, 32, 16>(op, s);
+  else if (op.tx == 64 && op.ty == 8) dispatch_tile<T, 64, 8>(op, s);
+  else dispatch_tile<T, 16, 16>(op, s);
 }
 
 template void launch_ku_fine<float>(const KuOp<float>&, hipStream_t);
