@@ -150,6 +150,9 @@ def main():
     ap.add_argument("--jobs", type=int, default=128)
     ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--ku-only", action="store_true",
+                    help="only the K.u microbenchmark, after a 256 MiB device copy that "
+                         "calibrates the PMC byte counters (for rocprofv3 --pmc passes)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this engine has no multi-GPU path (each job and each "
@@ -175,6 +178,17 @@ def main():
 
     wl = problems.workload(a.workload)
     nx, ny, nz = wl["grid"]
+    if a.ku_only:
+        import torch  # device buffers for the calibration copy only
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        ku = ku_measure(ta, nx, ny, nz, reps=10)
+        print(json.dumps({"workload": a.workload, "ku": ku, "calibration_copy_bytes": 256 << 20}),
+              flush=True)
+        return
     print(f"# workload {a.workload}: warmup {a.warmup}, timed {a.steps} design iterations",
           file=sys.stderr, flush=True)
     out, dt = run_design(ta, wl, ta.PREC_MIXED, a.steps, a.warmup)

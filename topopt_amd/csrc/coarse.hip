@@ -528,6 +528,33 @@ void CoarseCR::gemv(const CrGemv* jobs, int n, const int* done, hipStream_t s, i
                      tsize, done);
 }
 
+CrMap CoarseCR::rhs_map() {
+  CrMap cm;
+  cm.dst = d_dst.p;
+  cm.V = V.p;
+  cm.vstride = vsize;
+  cm.off = lv[0].b;
+  return cm;
+}
+
+CrMap CoarseCR::sol_map() {
+  CrMap cm = rhs_map();
+  cm.off = lv[0].x;
+  return cm;
+}
+
+void CoarseCR::solve_in_place(const int* done, hipStream_t s, int nr) {
+  if (nr > ncases) throw Error(TOPOHIP_EINTERNAL, "coarse solve: load-case space not reserved");
+  for (size_t r = 0; r + 1 < lv.size(); ++r) {
+    const int nf = lv_fw[r + 1] - lv_fw[r];
+    gemv(d_fw.p + lv_fw[r], nf, done, s, nr);
+  }
+  for (int r = static_cast<int>(lv.size()) - 1; r >= 0; --r) {
+    const int nbw = lv_bw[r + 1] - lv_bw[r];
+    gemv(d_bw.p + lv_bw[r], nbw, done, s, nr);
+  }
+}
+
 template <typename T>
 void CoarseCR::solve(const T* b, T* x, const int* done, hipStream_t s, int nr) {
   if (nr > ncases) throw Error(TOPOHIP_EINTERNAL, "coarse solve: load-case space not reserved");

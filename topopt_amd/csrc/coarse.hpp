@@ -47,6 +47,12 @@ class CoarseCR {
   // 3*nodes; FP64 arithmetic inside).
   template <typename T>
   void solve(const T* b, T* x, const int* done, hipStream_t s, int nr = 1);
+  // The plane-ordered right-hand side / solution space of load case r
+  // (restriction writes rhs_map() directly, prolongation reads sol_map()),
+  // and the solve between them.
+  CrMap rhs_map();
+  CrMap sol_map();
+  void solve_in_place(const int* done, hipStream_t s, int nr = 1);
   int block() const { return m; }
   int nplanes() const { return planes; }
   size_t bytes() const {

@@ -367,9 +367,11 @@ void coarse_cycle(Problem& P, size_t l, const T* rprev, const int* done, int nc)
   Level& c = *P.L[l];
   const bool coarsest = l + 1 == P.L.size();
   if (coarsest) {
-    launch_restrict<T, T>(f.g, c.g, rprev, c.mask.p, Bv<T>(c), nullptr, nullptr, 0.0, done, P.s,
-                          nc);
-    P.cr.solve<T>(Bv<T>(c), Xv<T>(c), done, P.s, nc);
+    // restrict into the coarse solver's plane-ordered FP64 space, solve there;
+    // the prolongation reads the solution from the same space
+    launch_restrict<T, T>(f.g, c.g, rprev, c.mask.p, static_cast<T*>(nullptr), nullptr, nullptr,
+                          0.0, done, P.s, nc, P.cr.rhs_map());
+    P.cr.solve_in_place(done, P.s, nc);
     return;
   }
   launch_restrict<T, T>(f.g, c.g, rprev, c.mask.p, Bv<T>(c), Xv<T>(c), Dinv<T>(c), kOmega, done,
@@ -379,8 +381,8 @@ void coarse_cycle(Problem& P, size_t l, const T* rprev, const int* done, int nc)
   coarse_cycle<T>(P, l + 1, Tv<T>(c), done, nc);
   Level& n = *P.L[l + 1];
   const bool ncoarsest = l + 2 == P.L.size();
-  launch_prolong_add<T, T>(n.g, c.g, ncoarsest ? Xv<T>(n) : Tv<T>(n), c.mask.p, Xv<T>(c), done,
-                           P.s, nc);
+  launch_prolong_add<T, T>(n.g, c.g, ncoarsest ? nullptr : Tv<T>(n), c.mask.p, Xv<T>(c), done,
+                           P.s, nc, ncoarsest ? P.cr.sol_map() : CrMap());
   launch_stencil_apply<T>(c.g, Kapp<T>(c), c.mask.p, Xv<T>(c), Tv<T>(c), Bv<T>(c), Dinv<T>(c),
                           kOmega, EPI_JAC, done, P.s, nc);
 }
@@ -417,8 +419,8 @@ void vcycle(Problem& P, const T* b0, T* z, const int* done, int nc) {
   coarse_cycle<T>(P, 1, Tv<T>(f), done, nc);
   Level& c = *P.L[1];
   const bool ccoarsest = P.L.size() == 2;
-  launch_prolong_add<T, T>(c.g, f.g, ccoarsest ? Xv<T>(c) : Tv<T>(c), f.mask.p, Xv<T>(f), done,
-                           P.s, nc);
+  launch_prolong_add<T, T>(c.g, f.g, ccoarsest ? nullptr : Tv<T>(c), f.mask.p, Xv<T>(f), done,
+                           P.s, nc, ccoarsest ? P.cr.sol_map() : CrMap());
   op.epi = EPI_JAC;
   op.dinv = Dinv<T>(f);
   op.omega = static_cast<T>(kOmega);
@@ -468,8 +470,10 @@ void restart(Problem& P, Prec pc, int nc) {
   precondition(P, pc, done, nc);
   dot_rz(P, pc, done, nc);
   launch_cg_set_rz(P.st.p, P.partials.p, dot_blocks(nd), P.s, nc, P.pstride);
-  if (pc == Prec::MG32) launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, P.p32.p, nd, 1, P.s, nc);
-  else launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nullptr, nd, 1, P.s, nc);
+  if (pc == Prec::MG32)
+    launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, P.p32.p, nd, 1, nullptr, 0, P.s, nc, P.pstride);
+  else
+    launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nullptr, nd, 1, nullptr, 0, P.s, nc, P.pstride);
 }
 
 void iteration(Problem& P, Prec pc, int nc) {
@@ -477,23 +481,26 @@ void iteration(Problem& P, Prec pc, int nc) {
   const int* done = done_flags(P);
   const int kb = ku_blocks(P.g, P.tz, P.tx, P.ty);
   const int db = dot_blocks(nd);
+  // A.p partials go to `partials`, the rr partials of cg_xr to `partials2`
+  // (cg_xr re-sums the former in every block while it writes the latter).
   if (pc == Prec::MG32) {
     fine_apply<float>(P, P.p32.p, P.Ap32.p, nullptr, EPI_AX, P.partials.p, done, nc);
-    launch_cg_alpha(P.st.p, P.partials.p, kb, P.s, nc, P.pstride);
     launch_cg_xr<float, float>(P.st.p, P.x.p, P.r.p, P.p32.p, P.Ap32.p, P.r32.p, nd, P.partials.p,
-                               db, P.s, nc, P.pstride);
+                               kb, P.partials2.p, db, P.s, nc, P.pstride);
   } else {
     fine_apply<double>(P, P.p.p, P.Ap64.p, nullptr, EPI_AX, P.partials.p, done, nc);
-    launch_cg_alpha(P.st.p, P.partials.p, kb, P.s, nc, P.pstride);
     launch_cg_xr<double, double>(P.st.p, P.x.p, P.r.p, P.p.p, P.Ap64.p, nullptr, nd, P.partials.p,
-                                 db, P.s, nc, P.pstride);
+                                 kb, P.partials2.p, db, P.s, nc, P.pstride);
   }
-  launch_cg_check(P.st.p, P.partials.p, db, P.s, nc, P.pstride);
+  launch_cg_check(P.st.p, P.partials2.p, db, P.s, nc, P.pstride);
   precondition(P, pc, done, nc);
   dot_rz(P, pc, done, nc);
-  launch_cg_beta(P.st.p, P.partials.p, db, P.s, nc, P.pstride);
-  if (pc == Prec::MG32) launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, P.p32.p, nd, 0, P.s, nc);
-  else launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nullptr, nd, 0, P.s, nc);
+  if (pc == Prec::MG32)
+    launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, P.p32.p, nd, 0, P.partials.p, db, P.s, nc,
+                       P.pstride);
+  else
+    launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nullptr, nd, 0, P.partials.p, db, P.s, nc,
+                        P.pstride);
 }
 
 void read_state(Problem& P, int nc) { d2h(P.st_h, P.st.p, static_cast<size_t>(nc), P.s); }
@@ -770,6 +777,7 @@ void ensure_cases(Problem& P, int nb) {
     TOPOHIP_CHECK(hipHostMalloc(&P.scal_h, sizeof(double) * std::max(8, nb)));
   }
   P.partials.ensure(static_cast<size_t>(nb) * P.pstride);
+  P.partials2.ensure(static_cast<size_t>(nb) * P.pstride);
   if (P.mg_plan_ok) P.cr.reserve_cases(nb);
   P.nb = nb;
 }

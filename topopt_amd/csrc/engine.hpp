@@ -88,7 +88,7 @@ struct Problem {
   // PCG vectors, nb stacked load cases each (stride ndof)
   DBuf<double> b, x, r, p, Ap64, z64, rt, u;
   DBuf<float> p32, Ap32, r32, z32;
-  DBuf<double> partials;
+  DBuf<double> partials, partials2;  // nb cases x pstride block partial sums
   DBuf<CgState> st;
   CgState* st_h = nullptr;
   DBuf<double> scal;

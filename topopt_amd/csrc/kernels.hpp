@@ -37,16 +37,25 @@ int ku_blocks(const Geom& g, int tz, int tx, int ty);
 template <typename T> void launch_ku_fine(const KuOp<T>& op, hipStream_t s);
 
 // ---- transfers between a fine level f and the coincident coarse level c ----
+// Coarse solver's plane-ordered FP64 vector space: when dst is set, the
+// restriction writes (and the prolongation reads) V[case * vstride + off +
+// dst[3 * node + comp]] instead of a node-ordered coarse vector.
+struct CrMap {
+  const int* dst = nullptr;
+  double* V = nullptr;
+  long long vstride = 0;
+  long long off = 0;
+};
 // bc = mask_c .* P^T rf; optionally xc = omega * dinv_c .* bc (pre-smooth from 0).
 // With nr load cases the vectors are stacked with the level's ndof as stride.
 template <typename Tf, typename Tc>
 void launch_restrict(const Geom& gf, const Geom& gc, const Tf* rf, const uint8_t* mask_c,
                      Tc* bc, Tc* xc, const Tc* dinv_c, double omega, const int* done,
-                     hipStream_t s, int nr = 1);
+                     hipStream_t s, int nr = 1, CrMap cm = CrMap());
 // xf += mask_f .* P xc
 template <typename Tc, typename Tf>
 void launch_prolong_add(const Geom& gc, const Geom& gf, const Tc* xc, const uint8_t* mask_f,
-                        Tf* xf, const int* done, hipStream_t s, int nr = 1);
+                        Tf* xf, const int* done, hipStream_t s, int nr = 1, CrMap cm = CrMap());
 
 // ---- reductions / vector kernels -------------------------------------------
 // out[r] = sum(partials[r*pstride + 0..n)) in a fixed order (one block per case).
@@ -71,24 +80,21 @@ template <typename TI, typename TO> void launch_convert(const TI* in, TO* out, l
 // PCG pieces (device-resident scalars in CgState). Every launcher covers nr
 // load cases: states st[0..nr), partials of case r at partials + r*pstride,
 // vectors stacked with stride n.
-// alpha = rz / pAp (pAp = sum of partials); breakdown sets fail/done.
-void launch_cg_alpha(CgState* st, const double* partials, int n, hipStream_t s, int nr = 1,
-                     int pstride = 0);
-// x += alpha*p; r -= alpha*Ap; rr partials; r32 = float(r) (nullable)
+// alpha = rz_new / pAp (pAp = fixed-order sum of the npap A.p partials at
+// pap, breakdown sets fail/done); x += alpha*p; r -= alpha*Ap; rr partials
+// to `partials` (a different buffer); r32 = float(r) (nullable).
 template <typename TP, typename TA>
 void launch_cg_xr(CgState* st, double* x, double* r, const TP* p, const TA* Ap, float* r32,
-                  long long n, double* partials, int nblocks, hipStream_t s, int nr = 1,
-                  int pstride = 0);
-// rr = sum(partials); iters++; done if rr <= thresh2
+                  long long n, const double* pap, int npap, double* partials, int nblocks,
+                  hipStream_t s, int nr = 1, int pstride = 0);
+// rr = sum(partials); iters++; done if rr <= thresh2; rz = rz_new
 void launch_cg_check(CgState* st, const double* partials, int n, hipStream_t s, int nr = 1,
                      int pstride = 0);
-// rz_new = sum(partials); beta = rz_new / rz; rz = rz_new
-void launch_cg_beta(CgState* st, const double* partials, int n, hipStream_t s, int nr = 1,
-                    int pstride = 0);
-// p = z + beta*p (beta = 0 on restart); optional p32 shadow
+// rz_new = fixed-order sum of the nrz <r,z> partials at rzp; beta = rz_new / rz;
+// p = z + beta*p (restart: p = z, partials unused); optional p32 shadow
 template <typename TZ>
 void launch_cg_p(CgState* st, const TZ* z, double* p, float* p32, long long n, int restart,
-                 hipStream_t s, int nr = 1);
+                 const double* rzp, int nrz, hipStream_t s, int nr = 1, int pstride = 0);
 // start: bnorm2, thresh2, rz from partials etc.
 void launch_cg_init(CgState* st, const double* bb_partials, int nb, double tol, int cap,
                     hipStream_t s, int nr = 1, int pstride = 0);

@@ -43,11 +43,11 @@ template <typename Tf, typename Tc>
 __global__ void restrict_k(Geom gf, Geom gc, const Tf* __restrict__ rf,
                            const uint8_t* __restrict__ mask_c, Tc* __restrict__ bc,
                            Tc* __restrict__ xc, const Tc* __restrict__ dinv_c, double omega,
-                           long long sf, long long sc, const int* done) {
+                           long long sf, long long sc, CrMap cm, const int* done) {
   const int rc = blockIdx.y;
   if (case_done(done, rc)) return;
   rf += rc * sf;
-  bc += rc * sc;
+  if (bc) bc += rc * sc;
   if (xc) xc += rc * sc;
   const long long nodes = gc.nodes();
   const long long n = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
@@ -75,6 +75,12 @@ __global__ void restrict_k(Geom gf, Geom gc, const Tf* __restrict__ rf,
       }
     }
   }
+  if (cm.dst) {  // straight into the coarse solver's plane-ordered FP64 right-hand side
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      cm.V[rc * cm.vstride + cm.off + cm.dst[n * 3 + q]] = ((mk >> q) & 1) ? s[q] : 0.0;
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const bool kept = (mk >> q) & 1;
@@ -89,10 +95,10 @@ __global__ void restrict_k(Geom gf, Geom gc, const Tf* __restrict__ rf,
 template <typename Tc, typename Tf>
 __global__ void prolong_add_k(Geom gc, Geom gf, const Tc* __restrict__ xc,
                               const uint8_t* __restrict__ mask_f, Tf* __restrict__ xf,
-                              long long sc, long long sf, const int* done) {
+                              long long sc, long long sf, CrMap cm, const int* done) {
   const int rc = blockIdx.y;
   if (case_done(done, rc)) return;
-  xc += rc * sc;
+  if (xc) xc += rc * sc;
   xf += rc * sf;
   const long long nodes = gf.nodes();
   const long long n = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
@@ -117,8 +123,14 @@ __global__ void prolong_add_k(Geom gc, Geom gf, const Tc* __restrict__ xc,
       for (int z = 0; z < nc; ++z) {
         const double w = wa[x] * wb[y] * wc[z];
         const long long cn = (static_cast<long long>(ic[z]) * gc.nny + ib[y]) * gc.nnx + ia[x];
+        if (cm.dst) {  // coarse values straight from the coarse solver's FP64 solution
 #pragma unroll
-        for (int q = 0; q < 3; ++q) s[q] += w * static_cast<double>(xc[cn * 3 + q]);
+          for (int q = 0; q < 3; ++q)
+            s[q] += w * cm.V[rc * cm.vstride + cm.off + cm.dst[cn * 3 + q]];
+        } else {
+#pragma unroll
+          for (int q = 0; q < 3; ++q) s[q] += w * static_cast<double>(xc[cn * 3 + q]);
+        }
       }
 #pragma unroll
   for (int q = 0; q < 3; ++q)
@@ -214,6 +226,15 @@ __device__ inline double fixed_sum(const double* p, int n) {
   return block_sum(s);
 }
 
+// fixed_sum with the result in every thread of the block.
+__device__ inline double fixed_sum_all(const double* p, int n) {
+  __shared__ double r;
+  const double s = fixed_sum(p, n);
+  if (threadIdx.x == 0) r = s;
+  __syncthreads();
+  return r;
+}
+
 __global__ void cg_init_k(CgState* st, const double* p, int n, int pstride, double tol, int cap) {
   st += blockIdx.x;
   p += static_cast<long long>(blockIdx.x) * pstride;
@@ -230,6 +251,8 @@ __global__ void cg_init_k(CgState* st, const double* p, int n, int pstride, doub
   }
 }
 
+// Restart: rz = rz_new = <r, z>. (In the iteration, cg_xr uses rz_new for
+// alpha, cg_check moves it to rz, and cg_p forms beta = <r,z>_new / rz.)
 __global__ void cg_set_rz_k(CgState* st, const double* p, int n, int pstride) {
   st += blockIdx.x;
   p += static_cast<long long>(blockIdx.x) * pstride;
@@ -237,24 +260,8 @@ __global__ void cg_set_rz_k(CgState* st, const double* p, int n, int pstride) {
   const double rz = fixed_sum(p, n);
   if (threadIdx.x == 0) {
     st->rz = rz;
+    st->rz_new = rz;
     if (!isfinite(rz)) { st->fail = 1; st->done = 1; }
-  }
-}
-
-__global__ void cg_alpha_k(CgState* st, const double* p, int n, int pstride) {
-  st += blockIdx.x;
-  p += static_cast<long long>(blockIdx.x) * pstride;
-  if (st->done) return;
-  const double pAp = fixed_sum(p, n);
-  if (threadIdx.x == 0) {
-    st->pAp = pAp;
-    if (!(pAp > 0.0) || !isfinite(pAp)) {
-      st->fail = 1;
-      st->done = 1;
-      st->alpha = 0.0;
-    } else {
-      st->alpha = st->rz / pAp;
-    }
   }
 }
 
@@ -269,19 +276,7 @@ __global__ void cg_check_k(CgState* st, const double* p, int n, int pstride) {
     if (!isfinite(rr)) { st->fail = 1; st->done = 1; }
     else if (rr <= st->thresh2) st->done = 1;
     else if (st->cap > 0 && st->iters >= st->cap) { st->fail = 2; st->done = 1; }
-  }
-}
-
-__global__ void cg_beta_k(CgState* st, const double* p, int n, int pstride) {
-  st += blockIdx.x;
-  p += static_cast<long long>(blockIdx.x) * pstride;
-  if (st->done) return;
-  const double rzn = fixed_sum(p, n);
-  if (threadIdx.x == 0) {
-    st->rz_new = rzn;
-    if (!isfinite(rzn)) { st->fail = 1; st->done = 1; st->beta = 0.0; return; }
-    st->beta = rzn / st->rz;
-    st->rz = rzn;
+    st->rz = st->rz_new;
   }
 }
 
@@ -296,10 +291,14 @@ __global__ void cg_resume_k(CgState* st, const double* p, int n, int pstride) {
   }
 }
 
+// alpha = rz_new / pAp with pAp the fixed-order sum of the case's A.p partials
+// (every block forms the same value; block 0 records it and the breakdown
+// test, pAp <= 0 or non-finite), then x += alpha p, r -= alpha Ap, rr partials.
 template <typename TP, typename TA>
 __global__ void cg_xr_k(CgState* st, double* __restrict__ x, double* __restrict__ r,
                         const TP* __restrict__ p, const TA* __restrict__ Ap, float* r32,
-                        long long n, double* partials, int pstride) {
+                        long long n, const double* pap, int npap, double* partials,
+                        int pstride) {
   const int rc = blockIdx.y;
   st += rc;
   if (st->done) return;
@@ -308,8 +307,23 @@ __global__ void cg_xr_k(CgState* st, double* __restrict__ x, double* __restrict_
   p += rc * n;
   Ap += rc * n;
   if (r32) r32 += rc * n;
+  const double pAp = fixed_sum_all(pap + static_cast<long long>(rc) * pstride, npap);
   partials += static_cast<long long>(rc) * pstride;
-  const double alpha = st->alpha;
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!(pAp > 0.0) || !isfinite(pAp)) {
+    if (first) {
+      st->pAp = pAp;
+      st->alpha = 0.0;
+      st->fail = 1;
+      st->done = 1;
+    }
+    return;
+  }
+  const double alpha = st->rz_new / pAp;
+  if (first) {
+    st->pAp = pAp;
+    st->alpha = alpha;
+  }
   double s = 0.0;
   for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<long long>(gridDim.x) * blockDim.x) {
@@ -323,16 +337,38 @@ __global__ void cg_xr_k(CgState* st, double* __restrict__ x, double* __restrict_
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// p = z + beta p with beta = <r,z>_new / rz, <r,z>_new the fixed-order sum of
+// the case's partials (every block forms the same value; block 0 records it
+// and the breakdown test). restart: p = z.
 template <typename TZ>
 __global__ void cg_p_k(CgState* st, const TZ* __restrict__ z, double* __restrict__ p,
-                       float* p32, long long n, int restart) {
+                       float* p32, long long n, int restart, const double* rzp, int nrz,
+                       int pstride) {
   const int rc = blockIdx.y;
   st += rc;
   if (st->done) return;
   z += rc * n;
   p += rc * n;
   if (p32) p32 += rc * n;
-  const double beta = restart ? 0.0 : st->beta;
+  double beta = 0.0;
+  if (!restart) {
+    const double rzn = fixed_sum_all(rzp + static_cast<long long>(rc) * pstride, nrz);
+    const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+    if (!isfinite(rzn)) {
+      if (first) {
+        st->rz_new = rzn;
+        st->beta = 0.0;
+        st->fail = 1;
+        st->done = 1;
+      }
+      return;
+    }
+    beta = rzn / st->rz;
+    if (first) {
+      st->rz_new = rzn;
+      st->beta = beta;
+    }
+  }
   for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<long long>(gridDim.x) * blockDim.x) {
     const double v = static_cast<double>(z[i]) + (restart ? 0.0 : beta * p[i]);
@@ -353,40 +389,41 @@ int grid_for(long long n, int bs, int cap) {
 template <typename Tf, typename Tc>
 void launch_restrict(const Geom& gf, const Geom& gc, const Tf* rf, const uint8_t* mask_c, Tc* bc,
                      Tc* xc, const Tc* dinv_c, double omega, const int* done, hipStream_t s,
-                     int nr) {
+                     int nr, CrMap cm) {
   const long long n = gc.nodes();
   hipLaunchKernelGGL((restrict_k<Tf, Tc>), dim3(static_cast<unsigned>((n + 255) / 256), nr),
                      dim3(256), 0, s, gf, gc, rf, mask_c, bc, xc, dinv_c, omega, gf.ndof(),
-                     gc.ndof(), done);
+                     gc.ndof(), cm, done);
   TOPOHIP_CHECK(hipGetLastError());
 }
 
 template <typename Tc, typename Tf>
 void launch_prolong_add(const Geom& gc, const Geom& gf, const Tc* xc, const uint8_t* mask_f,
-                        Tf* xf, const int* done, hipStream_t s, int nr) {
+                        Tf* xf, const int* done, hipStream_t s, int nr, CrMap cm) {
   const long long n = gf.nodes();
   hipLaunchKernelGGL((prolong_add_k<Tc, Tf>), dim3(static_cast<unsigned>((n + 255) / 256), nr),
-                     dim3(256), 0, s, gc, gf, xc, mask_f, xf, gc.ndof(), gf.ndof(), done);
+                     dim3(256), 0, s, gc, gf, xc, mask_f, xf, gc.ndof(), gf.ndof(), cm, done);
   TOPOHIP_CHECK(hipGetLastError());
 }
 
 template void launch_restrict<float, float>(const Geom&, const Geom&, const float*, const uint8_t*,
                                             float*, float*, const float*, double, const int*,
-                                            hipStream_t, int);
+                                            hipStream_t, int, CrMap);
 template void launch_restrict<double, double>(const Geom&, const Geom&, const double*,
                                               const uint8_t*, double*, double*, const double*,
-                                              double, const int*, hipStream_t, int);
+                                              double, const int*, hipStream_t, int, CrMap);
 template void launch_restrict<float, double>(const Geom&, const Geom&, const float*,
                                              const uint8_t*, double*, double*, const double*,
-                                             double, const int*, hipStream_t, int);
+                                             double, const int*, hipStream_t, int, CrMap);
 template void launch_prolong_add<float, float>(const Geom&, const Geom&, const float*,
-                                               const uint8_t*, float*, const int*, hipStream_t, int);
+                                               const uint8_t*, float*, const int*, hipStream_t,
+                                               int, CrMap);
 template void launch_prolong_add<double, double>(const Geom&, const Geom&, const double*,
                                                  const uint8_t*, double*, const int*, hipStream_t,
-                                                 int);
+                                                 int, CrMap);
 template void launch_prolong_add<double, float>(const Geom&, const Geom&, const double*,
                                                 const uint8_t*, float*, const int*, hipStream_t,
-                                                int);
+                                                int, CrMap);
 
 void launch_sum_partials(const double* partials, int n, double* out, hipStream_t s, int nr,
                          int pstride) {
@@ -470,35 +507,35 @@ void launch_cg_init(CgState* st, const double* p, int n, double tol, int cap, hi
   }
 TOPOHIP_CG_SCALAR(launch_cg_set_rz, cg_set_rz_k)
 TOPOHIP_CG_SCALAR(launch_cg_resume, cg_resume_k)
-TOPOHIP_CG_SCALAR(launch_cg_alpha, cg_alpha_k)
 TOPOHIP_CG_SCALAR(launch_cg_check, cg_check_k)
-TOPOHIP_CG_SCALAR(launch_cg_beta, cg_beta_k)
 #undef TOPOHIP_CG_SCALAR
 
 template <typename TP, typename TA>
 void launch_cg_xr(CgState* st, double* x, double* r, const TP* p, const TA* Ap, float* r32,
-                  long long n, double* partials, int nblocks, hipStream_t s, int nr,
-                  int pstride) {
+                  long long n, const double* pap, int npap, double* partials, int nblocks,
+                  hipStream_t s, int nr, int pstride) {
   hipLaunchKernelGGL((cg_xr_k<TP, TA>), dim3(nblocks, nr), dim3(RB), 0, s, st, x, r, p, Ap, r32,
-                     n, partials, pstride);
+                     n, pap, npap, partials, pstride);
   TOPOHIP_CHECK(hipGetLastError());
 }
 template void launch_cg_xr<double, double>(CgState*, double*, double*, const double*,
-                                           const double*, float*, long long, double*, int,
-                                           hipStream_t, int, int);
+                                           const double*, float*, long long, const double*, int,
+                                           double*, int, hipStream_t, int, int);
 template void launch_cg_xr<float, float>(CgState*, double*, double*, const float*, const float*,
-                                         float*, long long, double*, int, hipStream_t, int, int);
+                                         float*, long long, const double*, int, double*, int,
+                                         hipStream_t, int, int);
 
 template <typename TZ>
 void launch_cg_p(CgState* st, const TZ* z, double* p, float* p32, long long n, int restart,
-                 hipStream_t s, int nr) {
-  hipLaunchKernelGGL((cg_p_k<TZ>), dim3(grid_for(n, 256, 4096), nr), dim3(256), 0, s, st, z, p,
-                     p32, n, restart);
+                 const double* rzp, int nrz, hipStream_t s, int nr, int pstride) {
+  // every block re-sums the <r,z> partials, so the grid stays at 1024 blocks
+  hipLaunchKernelGGL((cg_p_k<TZ>), dim3(grid_for(n, 256, 1024), nr), dim3(256), 0, s, st, z, p,
+                     p32, n, restart, rzp, nrz, pstride);
   TOPOHIP_CHECK(hipGetLastError());
 }
 template void launch_cg_p<float>(CgState*, const float*, double*, float*, long long, int,
-                                 hipStream_t, int);
+                                 const double*, int, hipStream_t, int, int);
 template void launch_cg_p<double>(CgState*, const double*, double*, float*, long long, int,
-                                  hipStream_t, int);
+                                  const double*, int, hipStream_t, int, int);
 
 }  // namespace topohip
