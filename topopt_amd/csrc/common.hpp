@@ -91,6 +91,20 @@ struct CgState {
   int cap;        // iteration cap (fail = 2 when reached unconverged)
 };
 
+// MI355X places consecutive workgroup ids on its 8 XCDs round-robin, and each
+// XCD has its own L2. Kernels whose neighbouring tiles share input (halos,
+// transposed stencil blocks) launch a 1-D grid padded to a multiple of 8 and
+// map the hardware id to a logical tile so that each XCD walks one contiguous
+// eighth of the tiles: neighbours then meet in the same L2.
+constexpr int kXcds = 8;
+__host__ __device__ inline unsigned xcd_grid(long long tiles) {
+  return static_cast<unsigned>((tiles + kXcds - 1) / kXcds * kXcds);
+}
+__device__ inline long long xcd_tile(unsigned hw_block, unsigned padded_grid) {
+  const unsigned per = padded_grid / kXcds;
+  return static_cast<long long>(hw_block % kXcds) * per + hw_block / kXcds;
+}
+
 // Load cases solved together (multi-RHS PCG): case r's vectors start at
 // base + r * stride, its state is CgState[r], and kernels that take a `done`
 // pointer (&st[0].done) skip case r once done[r * kDoneStride] is set. The

@@ -92,8 +92,17 @@ __global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
   constexpr int PLANE = PW * (TY + 1) * 3;
   constexpr int NPF = (PLANE + NT - 1) / NT;  // prefetch registers per lane
   const Geom g = a.g;
+  // Logical tile (x fastest, then y, z-chunk, load case) of this workgroup,
+  // XCD-contiguous so that tiles sharing halo planes share an L2.
+  const int gx = (g.nnx + TX - 2) / (TX - 1), gy = (g.nny + TY - 2) / (TY - 1);
   const int nchunk = (g.nnz + a.tz - 1) / a.tz;
-  const int rc = blockIdx.z / nchunk;  // load case
+  const long long tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (tile >= static_cast<long long>(gx) * gy * nchunk * a.nrhs) return;
+  const int bx = static_cast<int>(tile % gx);
+  const int by = static_cast<int>((tile / gx) % gy);
+  const int bz = static_cast<int>(tile / (static_cast<long long>(gx) * gy));
+  const int rc = bz / nchunk;  // load case
+  const int chunk = bz - rc * nchunk;
   if (case_done(a.done, rc)) return;
   __shared__ T s_u[2][PLANE];
   __shared__ T s_c[4][NT * 3];
@@ -101,8 +110,8 @@ __global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
 
   const int t = threadIdx.x;
   const int tx = t % TX, ty = t / TX;
-  const int i0 = blockIdx.x * (TX - 1), j0 = blockIdx.y * (TY - 1);
-  const int k0 = (blockIdx.z - rc * nchunk) * a.tz;
+  const int i0 = bx * (TX - 1), j0 = by * (TY - 1);
+  const int k0 = chunk * a.tz;
   const int ei = i0 - 1 + tx, ej = j0 - 1 + ty;
   const int na = i0 + tx, nb = j0 + ty;
   const bool owner = tx < TX - 1 && ty < TY - 1 && na < g.nnx && nb < g.nny;
@@ -275,7 +284,7 @@ __global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
     if (t == 0) {
       double s = 0.0;
       for (int w = 0; w < NT / 64; ++w) s += s_red[w];
-      const int blin = ((blockIdx.z - rc * nchunk) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+      const int blin = (chunk * gy + by) * gx + bx;
       a.partials[static_cast<long long>(rc) * a.pstride + blin] = s;
     }
   }
@@ -283,8 +292,10 @@ __global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
 
 template <typename T, int TX, int TY>
 void dispatch_tile(const KuOp<T>& op, hipStream_t s) {
-  const dim3 grid((op.g.nnx + TX - 2) / (TX - 1), (op.g.nny + TY - 2) / (TY - 1),
-                  ((op.g.nnz + op.tz - 1) / op.tz) * op.nrhs);
+  const long long tiles = static_cast<long long>((op.g.nnx + TX - 2) / (TX - 1)) *
+                          ((op.g.nny + TY - 2) / (TY - 1)) * ((op.g.nnz + op.tz - 1) / op.tz) *
+                          op.nrhs;
+  const dim3 grid(xcd_grid(tiles));
   const bool dot = op.partials != nullptr;
 #define TOPOHIP_L(E, D) hipLaunchKernelGGL((ku_face<T, TX, TY, E, D>), grid, dim3(TX * TY), 0, s, op)
   if (op.epi == EPI_AX) {

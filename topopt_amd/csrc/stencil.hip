@@ -62,7 +62,7 @@ stencil_apply_k(Geom g, const T* __restrict__ S, const uint8_t* __restrict__ mas
   x += rc * 3 * nodes;
   y += rc * 3 * nodes;
   if (b) b += rc * 3 * nodes;
-  const long long n = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  const long long n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (n >= nodes) return;
   const int a = static_cast<int>(n % g.nnx);
   const int bb = static_cast<int>((n / g.nnx) % g.nny);
@@ -122,7 +122,7 @@ void launch_stencil_apply(const Geom& g, const T* S, const uint8_t* mask, const 
                           const T* b, const T* dinv, double omega, int epi, const int* done,
                           hipStream_t s, int nr) {
   const long long n = g.nodes();
-  const dim3 grid(static_cast<unsigned>((n + 255) / 256), nr);
+  const dim3 grid(xcd_grid((n + 255) / 256), nr);
   const T om = static_cast<T>(omega);
   if (epi == EPI_AX)
     hipLaunchKernelGGL((stencil_apply_k<T, EPI_AX>), grid, dim3(256), 0, s, g, S, mask, x, y, b, dinv, om, done);

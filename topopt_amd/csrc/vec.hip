@@ -50,7 +50,7 @@ __global__ void restrict_k(Geom gf, Geom gc, const Tf* __restrict__ rf,
   if (bc) bc += rc * sc;
   if (xc) xc += rc * sc;
   const long long nodes = gc.nodes();
-  const long long n = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  const long long n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (n >= nodes) return;
   const int A = static_cast<int>(n % gc.nnx);
   const int B = static_cast<int>((n / gc.nnx) % gc.nny);
@@ -101,7 +101,7 @@ __global__ void prolong_add_k(Geom gc, Geom gf, const Tc* __restrict__ xc,
   if (xc) xc += rc * sc;
   xf += rc * sf;
   const long long nodes = gf.nodes();
-  const long long n = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  const long long n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (n >= nodes) return;
   const int a = static_cast<int>(n % gf.nnx);
   const int b = static_cast<int>((n / gf.nnx) % gf.nny);
@@ -391,7 +391,7 @@ void launch_restrict(const Geom& gf, const Geom& gc, const Tf* rf, const uint8_t
                      Tc* xc, const Tc* dinv_c, double omega, const int* done, hipStream_t s,
                      int nr, CrMap cm) {
   const long long n = gc.nodes();
-  hipLaunchKernelGGL((restrict_k<Tf, Tc>), dim3(static_cast<unsigned>((n + 255) / 256), nr),
+  hipLaunchKernelGGL((restrict_k<Tf, Tc>), dim3(xcd_grid((n + 255) / 256), nr),
                      dim3(256), 0, s, gf, gc, rf, mask_c, bc, xc, dinv_c, omega, gf.ndof(),
                      gc.ndof(), cm, done);
   TOPOHIP_CHECK(hipGetLastError());
@@ -401,7 +401,7 @@ template <typename Tc, typename Tf>
 void launch_prolong_add(const Geom& gc, const Geom& gf, const Tc* xc, const uint8_t* mask_f,
                         Tf* xf, const int* done, hipStream_t s, int nr, CrMap cm) {
   const long long n = gf.nodes();
-  hipLaunchKernelGGL((prolong_add_k<Tc, Tf>), dim3(static_cast<unsigned>((n + 255) / 256), nr),
+  hipLaunchKernelGGL((prolong_add_k<Tc, Tf>), dim3(xcd_grid((n + 255) / 256), nr),
                      dim3(256), 0, s, gc, gf, xc, mask_f, xf, gc.ndof(), gf.ndof(), cm, done);
   TOPOHIP_CHECK(hipGetLastError());
 }
