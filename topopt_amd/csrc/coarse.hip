@@ -184,54 +184,39 @@ __global__ void __launch_bounds__(256) gemm_jobs_k(double* W, const CrGemm* jobs
     }
 }
 
-// Batched y = beta*y0 + sum_t alpha_t M_t^T v_t (length-m vectors, FP64; every
-// matrix is stored so that the product reads it row by row, coalesced). Each
-// job's k range is split over GV_KS workgroups (partials to tmp), then a
-// second kernel sums the partials in a fixed order. (A single kernel whose
-// last-arriving workgroup reduces needs a device-scope fence, which costs more
-// than the second launch on a multi-XCD part.)
-constexpr int GV_KS = 32;
-__global__ void __launch_bounds__(256) gemv_partial_k(const double* W, const double* V,
-                                                      long long vsz, const CrGemv* jobs, int m,
-                                                      double* tmp, long long tsz,
-                                                      const int* done) {
+// Batched y = beta*y0 + sum_t alpha_t M_t v_t (length-m vectors, FP64; M_t
+// row-major, one row per output). One wave per output row: the lanes stride
+// the row (coalesced), then a fixed shuffle tree adds the 64 partial sums, so
+// each reduction level is one launch with ~m/4 * jobs workgroups and the result
+// is deterministic. (The plan stores every plane block in both orientations,
+// so each product reads its matrix along the rows.)
+__device__ inline double wave_sum64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__global__ void __launch_bounds__(256) gemv_rows_k(const double* W, double* V, long long vsz,
+                                                   const CrGemv* jobs, int m, const int* done) {
   const int rc = blockIdx.z;
   if (case_done(done, rc)) return;
   V += rc * vsz;
-  tmp += rc * tsz;
   const CrGemv& jb = jobs[blockIdx.x];
-  const int ks = blockIdx.y;
-  const int kc = (m + GV_KS - 1) / GV_KS;
-  const int k0 = ks * kc, k1 = min(m, k0 + kc);
-  const int o = threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int o = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (o >= m) return;  // whole wave
   double s = 0.0;
-  if (o < m) {
-    for (int t = 0; t < jb.nterms; ++t) {
-      const double* M = W + jb.m[t];
-      const double* v = V + jb.v[t];
-      double st = 0.0;
-#pragma unroll 8
-      for (int k = k0; k < k1; ++k) st += M[static_cast<long long>(k) * m + o] * v[k];
-      s += jb.alpha[t] * st;
-    }
+  for (int t = 0; t < jb.nterms; ++t) {
+    const double* M = W + jb.m[t] + static_cast<long long>(o) * m;
+    const double* v = V + jb.v[t];
+    double st = 0.0;
+    for (int k = lane; k < m; k += 64) st += M[k] * v[k];
+    s += jb.alpha[t] * st;
   }
-  tmp[(static_cast<long long>(blockIdx.x) * GV_KS + ks) * 256 + o] = s;
-}
-
-__global__ void __launch_bounds__(256) gemv_reduce_k(double* V, long long vsz, const CrGemv* jobs,
-                                                     int m, const double* tmp, long long tsz,
-                                                     const int* done) {
-  const int rc = blockIdx.y;
-  if (case_done(done, rc)) return;
-  V += rc * vsz;
-  tmp += rc * tsz;
-  const CrGemv& jb = jobs[blockIdx.x];
-  const int o = threadIdx.x;
-  if (o >= m) return;
-  double y = 0.0;
-  for (int ks = 0; ks < GV_KS; ++ks) y += tmp[(static_cast<long long>(blockIdx.x) * GV_KS + ks) * 256 + o];
-  if (jb.beta != 0.0) y += jb.beta * V[jb.y0 + o];
-  V[jb.out + o] = y;
+  s = wave_sum64(s);
+  if (lane == 0) {
+    if (jb.beta != 0.0) s += jb.beta * V[jb.y0 + o];
+    V[jb.out + o] = s;
+  }
 }
 
 // Blocks of the coarsest operator in plane order, from the element matrices.
@@ -362,7 +347,7 @@ bool CoarseCR::plan(const Geom& g, hipStream_t s) {
       inv_offs.push_back(L.D);  // final block: Dinv = D^-1
       inv_offs.push_back(L.Dinv);
       CrGemv j{};
-      j.out = L.x; j.nterms = 1; j.m[0] = L.Dinv; j.v[0] = L.b; j.tm[0] = 1; j.alpha[0] = 1.0;
+      j.out = L.x; j.nterms = 1; j.m[0] = L.Dinv; j.v[0] = L.b; j.alpha[0] = 1.0;
       bwd.push_back(j);
       continue;
     }
@@ -413,12 +398,12 @@ bool CoarseCR::plan(const Geom& g, hipStream_t s) {
       f.out = N.b + static_cast<long long>(j / 2) * m; f.y0 = L.b + static_cast<long long>(j) * m;
       f.beta = 1.0;
       if (j >= 1) {
-        f.m[f.nterms] = L.Q + (j - 1) * mm; f.v[f.nterms] = L.b + static_cast<long long>(j - 1) * m;
-        f.tm[f.nterms] = 1; f.alpha[f.nterms++] = -1.0;
+        f.m[f.nterms] = L.Qt + (j - 1) * mm; f.v[f.nterms] = L.b + static_cast<long long>(j - 1) * m;
+        f.alpha[f.nterms++] = -1.0;
       }
       if (j + 1 < nr) {
-        f.m[f.nterms] = L.P + (j + 1) * mm; f.v[f.nterms] = L.b + static_cast<long long>(j + 1) * m;
-        f.tm[f.nterms] = 1; f.alpha[f.nterms++] = -1.0;
+        f.m[f.nterms] = L.Pt + (j + 1) * mm; f.v[f.nterms] = L.b + static_cast<long long>(j + 1) * m;
+        f.alpha[f.nterms++] = -1.0;
       }
       fwd.push_back(f);
       CrGemv c{};  // x_j = x'_{j/2}
@@ -429,13 +414,13 @@ bool CoarseCR::plan(const Geom& g, hipStream_t s) {
     for (int i = 1; i < nr; i += 2) {
       CrGemv bi{};  // x_i = Dinv_i b_i - P_i x_{i-1} - Q_i x_{i+1}; x_{i+-1} read from the coarser level
       bi.out = L.x + static_cast<long long>(i) * m;
-      bi.m[0] = L.Dinv + i * mm; bi.v[0] = L.b + static_cast<long long>(i) * m; bi.tm[0] = 1;
+      bi.m[0] = L.Dinv + i * mm; bi.v[0] = L.b + static_cast<long long>(i) * m;
       bi.alpha[0] = 1.0; bi.nterms = 1;
-      bi.m[bi.nterms] = L.Pt + i * mm; bi.v[bi.nterms] = N.x + static_cast<long long>((i - 1) / 2) * m;
-      bi.tm[bi.nterms] = 1; bi.alpha[bi.nterms++] = -1.0;
+      bi.m[bi.nterms] = L.P + i * mm; bi.v[bi.nterms] = N.x + static_cast<long long>((i - 1) / 2) * m;
+      bi.alpha[bi.nterms++] = -1.0;
       if (i + 1 < nr) {
-        bi.m[bi.nterms] = L.Qt + i * mm; bi.v[bi.nterms] = N.x + static_cast<long long>((i + 1) / 2) * m;
-        bi.tm[bi.nterms] = 1; bi.alpha[bi.nterms++] = -1.0;
+        bi.m[bi.nterms] = L.Q + i * mm; bi.v[bi.nterms] = N.x + static_cast<long long>((i + 1) / 2) * m;
+        bi.alpha[bi.nterms++] = -1.0;
       }
       bwd.push_back(bi);
     }
@@ -455,12 +440,6 @@ bool CoarseCR::plan(const Geom& g, hipStream_t s) {
                                    hipMemcpyHostToDevice, s));
   };
   W.ensure(static_cast<size_t>(wsize));
-  size_t maxjobs = 1;
-  for (size_t r = 0; r + 1 < lv_fw.size(); ++r) {
-    maxjobs = std::max(maxjobs, static_cast<size_t>(lv_fw[r + 1] - lv_fw[r]));
-    maxjobs = std::max(maxjobs, static_cast<size_t>(lv_bw[r + 1] - lv_bw[r]));
-  }
-  tsize = static_cast<long long>(maxjobs) * GV_KS * 256;
   ncases = 0;
   reserve_cases(1);
   size_t maxinv = 1;
@@ -516,16 +495,13 @@ void CoarseCR::build(const double* Kem, const uint8_t* mask, hipStream_t s) {
 void CoarseCR::reserve_cases(int nr) {
   if (nr <= ncases) return;
   V.ensure(static_cast<size_t>(vsize) * nr);
-  tmp.ensure(static_cast<size_t>(tsize) * nr);
   ncases = nr;
 }
 
 void CoarseCR::gemv(const CrGemv* jobs, int n, const int* done, hipStream_t s, int nr) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(gemv_partial_k, dim3(n, GV_KS, nr), dim3(256), 0, s, W.p, V.p, vsize, jobs,
-                     m, tmp.p, tsize, done);
-  hipLaunchKernelGGL(gemv_reduce_k, dim3(n, nr), dim3(256), 0, s, V.p, vsize, jobs, m, tmp.p,
-                     tsize, done);
+  hipLaunchKernelGGL(gemv_rows_k, dim3(n, (m + 3) / 4, nr), dim3(256), 0, s, W.p, V.p, vsize, jobs,
+                     m, done);
 }
 
 CrMap CoarseCR::rhs_map() {

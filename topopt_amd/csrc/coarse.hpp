@@ -18,12 +18,11 @@ struct CrGemm {
   double beta;
   int nterms;
 };
-// out = beta*y0 + sum_t alpha_t op(M_t) v_t on length-m vectors (matrix
-// offsets index W, vector offsets a load case's vector space V).
+// out = beta*y0 + sum_t alpha_t M_t v_t on length-m vectors, M_t row-major
+// (matrix offsets index W, vector offsets a load case's vector space V).
 struct CrGemv {
   long long out, y0;
   long long m[3], v[3];
-  int tm[3];
   double alpha[3];
   double beta;
   int nterms;
@@ -56,18 +55,18 @@ class CoarseCR {
   int block() const { return m; }
   int nplanes() const { return planes; }
   size_t bytes() const {
-    return static_cast<size_t>(wsize + (vsize + tsize) * ncases) * sizeof(double);
+    return static_cast<size_t>(wsize + vsize * ncases) * sizeof(double);
   }
 
  private:
   Geom geom;
   int axis = 0, planes = 0, m = 0;
-  long long wsize = 0, vsize = 0, tsize = 0;
+  long long wsize = 0, vsize = 0;
   int ncases = 0;
   std::vector<CrLevel> lv;
   std::vector<int> lv_inv, lv_gp, lv_gd, lv_fw, lv_bw;
   void gemv(const CrGemv* jobs, int n, const int* done, hipStream_t s, int nr);
-  DBuf<double> W, V, tmp, inv_scratch;
+  DBuf<double> W, V, inv_scratch;
   DBuf<long long> d_inv;
   DBuf<CrGemm> d_gp, d_gd;
   DBuf<CrGemv> d_fw, d_bw;
