@@ -404,17 +404,20 @@ KuOp<T> fine_op(Problem& P, const int* done, int nc) {
 }
 
 // z = M^{-1} b0 on the fine level (mf_v_cycle with 1+1 damped Jacobi sweeps;
-// the first sweep starts from zero so it is the scaling omega D^-1 b0).
+// the first sweep starts from zero so it is the scaling omega D^-1 b0, which
+// the first operator apply forms on the fly). With rz_partials set, the last
+// apply also writes the per-block partials of <r64, z> there.
 template <typename T>
-void vcycle(Problem& P, const T* b0, T* z, const int* done, int nc) {
+void vcycle(Problem& P, const T* b0, T* z, const int* done, int nc, const double* r64,
+            double* rz_partials) {
   Level& f = *P.L[0];
-  const long long nd = f.g.ndof();
-  launch_jacobi0<T>(Xv<T>(f), Dinv<T>(f), b0, kOmega, nd, done, P.s, nc);
   KuOp<T> op = fine_op<T>(P, done, nc);
-  op.x = Xv<T>(f);
   op.b = b0;
+  op.dinv = Dinv<T>(f);
+  op.omega_d = kOmega;
+  op.xw = Xv<T>(f);
   op.y = Tv<T>(f);
-  op.epi = EPI_RES;
+  op.epi = EPI_RES0;
   launch_ku_fine<T>(op, P.s);
   coarse_cycle<T>(P, 1, Tv<T>(f), done, nc);
   Level& c = *P.L[1];
@@ -422,9 +425,12 @@ void vcycle(Problem& P, const T* b0, T* z, const int* done, int nc) {
   launch_prolong_add<T, T>(c.g, f.g, ccoarsest ? nullptr : Tv<T>(c), f.mask.p, Xv<T>(f), done,
                            P.s, nc, ccoarsest ? P.cr.sol_map() : CrMap());
   op.epi = EPI_JAC;
-  op.dinv = Dinv<T>(f);
+  op.x = Xv<T>(f);
+  op.xw = nullptr;
   op.omega = static_cast<T>(kOmega);
   op.y = z;
+  op.partials = rz_partials;
+  op.rdot = r64;
   launch_ku_fine<T>(op, P.s);
 }
 
@@ -444,21 +450,23 @@ enum class Prec { MG32, MG64, JACOBI };
 
 const int* done_flags(Problem& P) { return &P.st.p->done; }
 
-// Preconditioner application z = M^{-1} r. Mixed: r32 must hold float(r).
-void precondition(Problem& P, Prec pc, const int* done, int nc) {
+// Preconditioner application z = M^{-1} r (mixed: r32 must hold float(r)),
+// with the partials of <r, z> written to P.partials; returns their count per
+// case. The V-cycle's last operator apply forms them; Jacobi uses a dot pass.
+int precondition_rz(Problem& P, Prec pc, const int* done, int nc) {
   const long long nd = P.g.ndof();
-  if (pc == Prec::MG32) vcycle<float>(P, P.r32.p, P.z32.p, done, nc);
-  else if (pc == Prec::MG64) vcycle<double>(P, P.r.p, P.z64.p, done, nc);
-  else launch_jacobi0<double>(P.z64.p, P.L[0]->dinv64.p, P.r.p, 1.0, nd, done, P.s, nc);
-}
-
-void dot_rz(Problem& P, Prec pc, const int* done, int nc) {
-  const long long nd = P.g.ndof();
+  if (pc == Prec::MG32) {
+    vcycle<float>(P, P.r32.p, P.z32.p, done, nc, P.r.p, P.partials.p);
+    return ku_blocks(P.g, P.tz, P.tx, P.ty);
+  }
+  if (pc == Prec::MG64) {
+    vcycle<double>(P, P.r.p, P.z64.p, done, nc, P.r.p, P.partials.p);
+    return ku_blocks(P.g, P.tz, P.tx, P.ty);
+  }
+  launch_jacobi0<double>(P.z64.p, P.L[0]->dinv64.p, P.r.p, 1.0, nd, done, P.s, nc);
   const int nb = dot_blocks(nd);
-  if (pc == Prec::MG32)
-    launch_dot<double, float>(P.r.p, P.z32.p, nd, P.partials.p, nb, done, P.s, nc, P.pstride);
-  else
-    launch_dot<double, double>(P.r.p, P.z64.p, nd, P.partials.p, nb, done, P.s, nc, P.pstride);
+  launch_dot<double, double>(P.r.p, P.z64.p, nd, P.partials.p, nb, done, P.s, nc, P.pstride);
+  return nb;
 }
 
 // Restart the recurrence of every case that is not done from its current r:
@@ -467,9 +475,8 @@ void restart(Problem& P, Prec pc, int nc) {
   const long long nd = P.g.ndof();
   const int* done = done_flags(P);
   if (pc == Prec::MG32) launch_convert<double, float>(P.r.p, P.r32.p, nd * nc, P.s);
-  precondition(P, pc, done, nc);
-  dot_rz(P, pc, done, nc);
-  launch_cg_set_rz(P.st.p, P.partials.p, dot_blocks(nd), P.s, nc, P.pstride);
+  const int nrz = precondition_rz(P, pc, done, nc);
+  launch_cg_set_rz(P.st.p, P.partials.p, nrz, P.s, nc, P.pstride);
   if (pc == Prec::MG32)
     launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, P.p32.p, nd, 1, nullptr, 0, P.s, nc, P.pstride);
   else
@@ -493,13 +500,12 @@ void iteration(Problem& P, Prec pc, int nc) {
                                  kb, P.partials2.p, db, P.s, nc, P.pstride);
   }
   launch_cg_check(P.st.p, P.partials2.p, db, P.s, nc, P.pstride);
-  precondition(P, pc, done, nc);
-  dot_rz(P, pc, done, nc);
+  const int nrz = precondition_rz(P, pc, done, nc);
   if (pc == Prec::MG32)
-    launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, P.p32.p, nd, 0, P.partials.p, db, P.s, nc,
+    launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, P.p32.p, nd, 0, P.partials.p, nrz, P.s, nc,
                        P.pstride);
   else
-    launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nullptr, nd, 0, P.partials.p, db, P.s, nc,
+    launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nullptr, nd, 0, P.partials.p, nrz, P.s, nc,
                         P.pstride);
 }
 

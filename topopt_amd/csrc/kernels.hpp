@@ -5,12 +5,16 @@
 
 namespace topohip {
 
-enum Epilogue { EPI_AX = 0, EPI_RES = 1, EPI_JAC = 2 };
+// EPI_RES0 (fine operator only): x = omega*dinv .* b is formed on the fly from
+// b and dinv (one damped-Jacobi sweep from zero), written to xw, and y = b - A x.
+enum Epilogue { EPI_AX = 0, EPI_RES = 1, EPI_JAC = 2, EPI_RES0 = 3 };
 
 // One tiled operator apply over the fine structured level: the modal element
-// operator scaled by per-element moduli E. Epilogues: y = A x, y = b - A x, y = x + omega*dinv*(b - A x).
-// With `partials` set, each block also writes sum(x . A x) over its owned kept
-// DOFs (FP64) to partials[block].
+// operator scaled by per-element moduli E. Epilogues: y = A x, y = b - A x,
+// y = x + omega*dinv*(b - A x), and EPI_RES0 above.
+// With `partials` set, each block also writes an FP64 sum over its owned kept
+// DOFs to partials[block]: sum(x . A x) for EPI_AX / EPI_RES, sum(rdot . y)
+// for EPI_JAC (rdot required).
 template <typename T>
 struct KuOp {
   Geom g;
@@ -19,6 +23,9 @@ struct KuOp {
   const T* b = nullptr;
   const T* dinv = nullptr;
   T omega = T(0);
+  double omega_d = 0.0;           // EPI_RES0: omega in FP64 (x = T(omega_d * dinv * b))
+  T* xw = nullptr;                // EPI_RES0: x written here
+  const double* rdot = nullptr;   // EPI_JAC with partials: FP64 vector dotted with y
   const uint8_t* mask = nullptr;  // per node: bit c = DOF c kept; null = all kept
   const T* E = nullptr;           // fine level
   ModalGroups<T> mg;              // grouped modal coefficients of Ke(E=1)

@@ -85,7 +85,8 @@ __device__ inline void modal_core(const T (&A)[24], T (&v)[24], T e, const Modal
 
 // Inputs x must be zero on inactive DOFs (every solver vector is); the mask
 // only decides which outputs are kept.
-template <typename T, int TX, int TY, int EPI, bool DOT>
+// DOTK: 0 none, 1 <x, Ax> over kept DOFs, 2 <rdot, y> over kept DOFs (FP64 partials).
+template <typename T, int TX, int TY, int EPI, int DOTK>
 __global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
   constexpr int NT = TX * TY;
   constexpr int PW = TX + 1;                  // staged plane: PW x (TY + 1) nodes
@@ -119,9 +120,12 @@ __global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
   const long long vo = rc * g.ndof();       // case offset of x, y, b
   const long long pn = static_cast<long long>(g.nnx) * g.nny;  // nodes per plane
   const long long pe = static_cast<long long>(g.ex) * g.ey;    // elements per layer
-  const T* __restrict__ xs = a.x + vo;
+  const T* __restrict__ xs = a.x ? a.x + vo : nullptr;
   T* __restrict__ ys = a.y + vo;
   const T* __restrict__ bs = a.b ? a.b + vo : nullptr;
+  const T* __restrict__ ds = a.dinv;
+  T* __restrict__ xw = a.xw ? a.xw + vo : nullptr;
+  const double* __restrict__ rd = a.rdot ? a.rdot + vo : nullptr;
   const uint8_t* __restrict__ mask = a.mask;
   const T* __restrict__ E = a.E;
   // Loop-invariant in-plane offsets (-1: outside the grid).
@@ -142,9 +146,20 @@ __global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
   T pf[NPF];
   auto fetch_plane = [&](int c) {
     if (c >= 0 && c < g.nnz) {
-      const T* xp = xs + c * pn * 3;
+      if constexpr (EPI == EPI_RES0) {
+        // one damped-Jacobi sweep from zero, formed on the fly: x0 = omega D^-1 b
+        const T* bp = bs + c * pn * 3;
+        const T* dp = ds + c * pn * 3;
 #pragma unroll
-      for (int i = 0; i < NPF; ++i) pf[i] = foff[i] >= 0 ? xp[foff[i]] : T(0);
+        for (int i = 0; i < NPF; ++i)
+          pf[i] = foff[i] >= 0 ? static_cast<T>(a.omega_d * static_cast<double>(dp[foff[i]]) *
+                                                static_cast<double>(bp[foff[i]]))
+                               : T(0);
+      } else {
+        const T* xp = xs + c * pn * 3;
+#pragma unroll
+        for (int i = 0; i < NPF; ++i) pf[i] = foff[i] >= 0 ? xp[foff[i]] : T(0);
+      }
     } else {
 #pragma unroll
       for (int i = 0; i < NPF; ++i) pf[i] = T(0);
@@ -178,6 +193,7 @@ __global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
   // Owner-node operands of plane c, fetched one layer ahead.
   int mk_n = 0;
   T b_n[3] = {T(0), T(0), T(0)}, d_n[3] = {T(0), T(0), T(0)};
+  double r_n[3] = {0.0, 0.0, 0.0};
   auto fetch_owner = [&](int c) {
     if (!owner || c < 0 || c >= g.nnz) return;
     const long long node = c * pn + noff;
@@ -188,7 +204,11 @@ __global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
     }
     if constexpr (EPI == EPI_JAC) {
 #pragma unroll
-      for (int q = 0; q < 3; ++q) d_n[q] = a.dinv[node * 3 + q];
+      for (int q = 0; q < 3; ++q) d_n[q] = ds[node * 3 + q];
+    }
+    if constexpr (DOTK == 2) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) r_n[q] = rd[node * 3 + q];
     }
   };
 
@@ -211,8 +231,9 @@ __global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
     e_n = modulus(k + 1);
     const int mk = mk_n;
     T bo[3], dv[3];
+    double r_n_k[3];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) { bo[q] = b_n[q]; dv[q] = d_n[q]; }
+    for (int q = 0; q < 3; ++q) { bo[q] = b_n[q]; dv[q] = d_n[q]; r_n_k[q] = r_n[q]; }
     if (k >= k0) fetch_owner(k + 1);
     __syncthreads();
     if (k == k0 - 1) load_face(cur, Fp);
@@ -252,6 +273,8 @@ __global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
       // 1 of (tx, ty+1) and 0 of (tx+1, ty+1). Fixed summation order.
       const int t00 = t, t10 = t + 1, t01 = t + TX, t11 = t + TX + 1;
       T* yp = ys + (k * pn + noff) * 3;
+      T* xwp = nullptr;
+      if constexpr (EPI == EPI_RES0) xwp = xw + (k * pn + noff) * 3;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const T ax = ((s_c[3][t00 * 3 + c] + s_c[2][t10 * 3 + c]) + s_c[1][t01 * 3 + c]) +
@@ -261,14 +284,16 @@ __global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
         if (kept) {
           if constexpr (EPI == EPI_AX) {
             out = ax;
-          } else if constexpr (EPI == EPI_RES) {
+          } else if constexpr (EPI == EPI_RES || EPI == EPI_RES0) {
             out = bo[c] - ax;
           } else {
             out = fma(a.omega * dv[c], bo[c] - ax, xo[c]);
           }
-          if constexpr (DOT) dotp = fma(static_cast<double>(xo[c]), static_cast<double>(ax), dotp);
+          if constexpr (DOTK == 1) dotp = fma(static_cast<double>(xo[c]), static_cast<double>(ax), dotp);
+          if constexpr (DOTK == 2) dotp = fma(r_n_k[c], static_cast<double>(out), dotp);
         }
         yp[c] = out;
+        if constexpr (EPI == EPI_RES0) xwp[c] = xo[c];
       }
     }
 #pragma unroll
@@ -277,7 +302,7 @@ __global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
     for (int c = 0; c < 3; ++c) xo[c] = xn[c];
     cur = nxt;
   }
-  if constexpr (DOT) {
+  if constexpr (DOTK != 0) {
     dotp = wave_sum(dotp);
     if ((t & 63) == 0) s_red[t >> 6] = dotp;
     __syncthreads();
@@ -299,11 +324,16 @@ void dispatch_tile(const KuOp<T>& op, hipStream_t s) {
   const bool dot = op.partials != nullptr;
 #define TOPOHIP_L(E, D) hipLaunchKernelGGL((ku_face<T, TX, TY, E, D>), grid, dim3(TX * TY), 0, s, op)
   if (op.epi == EPI_AX) {
-    if (dot) TOPOHIP_L(EPI_AX, true); else TOPOHIP_L(EPI_AX, false);
+    if (dot) TOPOHIP_L(EPI_AX, 1); else TOPOHIP_L(EPI_AX, 0);
   } else if (op.epi == EPI_RES) {
-    if (dot) TOPOHIP_L(EPI_RES, true); else TOPOHIP_L(EPI_RES, false);
+    if (dot) TOPOHIP_L(EPI_RES, 1); else TOPOHIP_L(EPI_RES, 0);
+  } else if (op.epi == EPI_RES0) {
+    if (dot || !op.xw || !op.b || !op.dinv)
+      throw HipError("K.u EPI_RES0 needs b, dinv and xw and takes no partials");
+    TOPOHIP_L(EPI_RES0, 0);
   } else {
-    if (dot) TOPOHIP_L(EPI_JAC, true); else TOPOHIP_L(EPI_JAC, false);
+    if (dot && !op.rdot) throw HipError("K.u EPI_JAC partials need rdot");
+    if (dot) TOPOHIP_L(EPI_JAC, 2); else TOPOHIP_L(EPI_JAC, 0);
   }
 #undef TOPOHIP_L
   TOPOHIP_CHECK(hipGetLastError());
