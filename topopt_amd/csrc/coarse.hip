@@ -204,13 +204,27 @@ __global__ void __launch_bounds__(256) gemv_rows_k(const double* W, double* V, l
   const int lane = threadIdx.x & 63;
   const int o = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (o >= m) return;  // whole wave
+  // Fixed trip counts (3 terms, m <= IV_MP): absent terms and the tail of the
+  // row are clamped to valid addresses and weighted by zero, so all loads of
+  // a lane are used unconditionally and issued together.
+  static_assert(IV_MP == 4 * 64, "gemv_rows_k covers a row in 4 strides of 64 lanes");
   double s = 0.0;
-  for (int t = 0; t < jb.nterms; ++t) {
-    const double* M = W + jb.m[t] + static_cast<long long>(o) * m;
-    const double* v = V + jb.v[t];
+  if (jb.nterms > 0) {  // (jobs without terms are plain copies y = y0)
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    const bool tok = t < jb.nterms;
+    const double* M = W + (tok ? jb.m[t] : jb.m[0]) + static_cast<long long>(o) * m;
+    const double* v = V + (tok ? jb.v[t] : jb.v[0]);
     double st = 0.0;
-    for (int k = lane; k < m; k += 64) st += M[k] * v[k];
-    s += jb.alpha[t] * st;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int k = lane + 64 * i;
+      const int kc = k < m ? k : m - 1;
+      const double f = k < m ? 1.0 : 0.0;
+      st += (M[kc] * f) * v[kc];
+    }
+    s += (tok ? jb.alpha[t] : 0.0) * st;
+  }
   }
   s = wave_sum64(s);
   if (lane == 0) {

@@ -5,8 +5,8 @@
 // structured grid sees them: each node couples to its 27 neighbours through a
 // 3x3 block. The blocks are assembled once per hierarchy build from the
 // per-coarse-element Galerkin matrices (fixed element order, no atomics) and
-// applied by a node-parallel gather whose coefficient loads are coalesced
-// (entry-major layout). The operator is symmetric, so only the centre block
+// applied by a gather whose coefficient loads are coalesced (entry-major
+// layout). The operator is symmetric, so only the centre block
 // and the 13 blocks towards lexicographically larger neighbours are stored
 // (offset index o = 13 + h, h = 0..13); the block towards a smaller neighbour
 // m is the transpose of the block m stores towards this node. That halves the
@@ -51,58 +51,95 @@ __global__ void assemble_stencil_k(Geom g, const double* __restrict__ K, T* __re
   for (int pq = 0; pq < 9; ++pq) S[(static_cast<long long>(h) * 9 + pq) * nodes + ni] = static_cast<T>(blk[pq]);
 }
 
+// Apply: a workgroup of 9 waves covers 64 consecutive nodes; wave j gathers
+// the 3 neighbours (dx = -1, 0, 1) of row j = (dz + 1) * 3 + (dy + 1) for its
+// lane's node, the 9 row sums meet in LDS and 3 waves finish the 64 x 3 outputs
+// in a fixed order. Neighbour indices are clamped into the grid and invalid
+// terms weighted by zero, so every load is used unconditionally and a wave
+// issues them all up front instead of behind a branch each.
+constexpr int SA_NODES = 64;
+constexpr int SA_ROWS = 9;
+
 template <typename T, int EPI>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(SA_NODES * SA_ROWS)
 stencil_apply_k(Geom g, const T* __restrict__ S, const uint8_t* __restrict__ mask,
                 const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ b,
                 const T* __restrict__ dinv, T omega, const int* done) {
+  __shared__ T s_acc[SA_ROWS][3][SA_NODES];
   const int rc = blockIdx.y;
   if (case_done(done, rc)) return;
   const long long nodes = g.nodes();
   x += rc * 3 * nodes;
   y += rc * 3 * nodes;
   if (b) b += rc * 3 * nodes;
-  const long long n = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (n >= nodes) return;
-  const int a = static_cast<int>(n % g.nnx);
-  const int bb = static_cast<int>((n / g.nnx) % g.nny);
-  const int c = static_cast<int>(n / (static_cast<long long>(g.nnx) * g.nny));
+  const int lane = threadIdx.x % SA_NODES, row = threadIdx.x / SA_NODES;
+  const long long n0 = xcd_tile(blockIdx.x, gridDim.x) * SA_NODES;
+  const long long n = n0 + lane;
+  const bool live = n < nodes;
+  const long long nn = live ? n : nodes - 1;
+  const int a = static_cast<int>(nn % g.nnx);
+  const int bb = static_cast<int>((nn / g.nnx) % g.nny);
+  const int c = static_cast<int>(nn / (static_cast<long long>(g.nnx) * g.nny));
+  const int oy = row % 3 - 1, oz = row / 3 - 1;
+  const int bj = bb + oy, cj = c + oz;
+  const bool rowok = bj >= 0 && cj >= 0 && bj < g.nny && cj < g.nnz;
+  long long m[3];
+  int mm[3];
+  bool ok[3];
+#pragma unroll
+  for (int dx = 0; dx < 3; ++dx) {
+    const int aj = a + dx - 1;
+    ok[dx] = rowok && aj >= 0 && aj < g.nnx;
+    m[dx] = ok[dx] ? (static_cast<long long>(cj) * g.nny + bj) * g.nnx + aj : nn;
+    mm[dx] = 7;
+  }
+  if (mask) {
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx) mm[dx] = mask[m[dx]];
+  }
   T acc[3] = {T(0), T(0), T(0)};
 #pragma unroll
-  for (int o = 0; o < 27; ++o) {
-    const int ox = o % 3 - 1, oy = (o / 3) % 3 - 1, oz = o / 9 - 1;
-    const int aj = a + ox, bj = bb + oy, cj = c + oz;
-    if (aj < 0 || bj < 0 || cj < 0 || aj >= g.nnx || bj >= g.nny || cj >= g.nnz) continue;
-    const long long m = (static_cast<long long>(cj) * g.nny + bj) * g.nnx + aj;
-    const int mm = mask ? mask[m] : 7;
+  for (int dx = 0; dx < 3; ++dx) {
+    const int o = row * 3 + dx;  // (dz+1)*9 + (dy+1)*3 + (dx+1)
+    const int mk = ok[dx] ? mm[dx] : 0;
     T xv[3];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) xv[q] = ((mm >> q) & 1) ? x[m * 3 + q] : T(0);
-    if (o >= 13) {  // own block towards m
-      const T* Sb = S + static_cast<long long>(o - 13) * 9 * nodes + n;
+    for (int q = 0; q < 3; ++q) xv[q] = x[m[dx] * 3 + q] * static_cast<T>((mk >> q) & 1);
+    // own block towards m (o >= 13), else the transpose of m's block towards n
+    const T* Sb = o >= 13 ? S + static_cast<long long>(o - 13) * 9 * nodes + nn
+                          : S + static_cast<long long>(13 - o) * 9 * nodes + m[dx];
+    T blk[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) blk[e] = Sb[e * nodes];
+    if (o >= 13) {
 #pragma unroll
       for (int p = 0; p < 3; ++p)
 #pragma unroll
-        for (int q = 0; q < 3; ++q) acc[p] += Sb[(p * 3 + q) * nodes] * xv[q];
-    } else {  // transpose of m's block towards n (offset 26 - o)
-      const T* Sb = S + static_cast<long long>(13 - o) * 9 * nodes + m;
+        for (int q = 0; q < 3; ++q) acc[p] += blk[p * 3 + q] * xv[q];
+    } else {
 #pragma unroll
       for (int p = 0; p < 3; ++p)
 #pragma unroll
-        for (int q = 0; q < 3; ++q) acc[p] += Sb[(q * 3 + p) * nodes] * xv[q];
+        for (int q = 0; q < 3; ++q) acc[p] += blk[q * 3 + p] * xv[q];
     }
   }
+#pragma unroll
+  for (int p = 0; p < 3; ++p) s_acc[row][p][lane] = acc[p];
+  __syncthreads();
+  if (threadIdx.x >= 3 * SA_NODES) return;
+  const int p = threadIdx.x / SA_NODES;
+  if (!live) return;
+  T sum = s_acc[0][p][lane];
+#pragma unroll
+  for (int r = 1; r < SA_ROWS; ++r) sum += s_acc[r][p][lane];
   const int mk = mask ? mask[n] : 7;
-#pragma unroll
-  for (int p = 0; p < 3; ++p) {
-    T out = T(0);
-    if ((mk >> p) & 1) {
-      if constexpr (EPI == EPI_AX) out = acc[p];
-      else if constexpr (EPI == EPI_RES) out = b[n * 3 + p] - acc[p];
-      else out = x[n * 3 + p] + omega * dinv[n * 3 + p] * (b[n * 3 + p] - acc[p]);
-    }
-    y[n * 3 + p] = out;
+  T out = T(0);
+  if ((mk >> p) & 1) {
+    if constexpr (EPI == EPI_AX) out = sum;
+    else if constexpr (EPI == EPI_RES) out = b[n * 3 + p] - sum;
+    else out = x[n * 3 + p] + omega * dinv[n * 3 + p] * (b[n * 3 + p] - sum);
   }
+  y[n * 3 + p] = out;
 }
 
 }  // namespace
@@ -122,14 +159,14 @@ void launch_stencil_apply(const Geom& g, const T* S, const uint8_t* mask, const 
                           const T* b, const T* dinv, double omega, int epi, const int* done,
                           hipStream_t s, int nr) {
   const long long n = g.nodes();
-  const dim3 grid(xcd_grid((n + 255) / 256), nr);
+  const dim3 grid(xcd_grid((n + SA_NODES - 1) / SA_NODES), nr), blk(SA_NODES * SA_ROWS);
   const T om = static_cast<T>(omega);
   if (epi == EPI_AX)
-    hipLaunchKernelGGL((stencil_apply_k<T, EPI_AX>), grid, dim3(256), 0, s, g, S, mask, x, y, b, dinv, om, done);
+    hipLaunchKernelGGL((stencil_apply_k<T, EPI_AX>), grid, blk, 0, s, g, S, mask, x, y, b, dinv, om, done);
   else if (epi == EPI_RES)
-    hipLaunchKernelGGL((stencil_apply_k<T, EPI_RES>), grid, dim3(256), 0, s, g, S, mask, x, y, b, dinv, om, done);
+    hipLaunchKernelGGL((stencil_apply_k<T, EPI_RES>), grid, blk, 0, s, g, S, mask, x, y, b, dinv, om, done);
   else
-    hipLaunchKernelGGL((stencil_apply_k<T, EPI_JAC>), grid, dim3(256), 0, s, g, S, mask, x, y, b, dinv, om, done);
+    hipLaunchKernelGGL((stencil_apply_k<T, EPI_JAC>), grid, blk, 0, s, g, S, mask, x, y, b, dinv, om, done);
   TOPOHIP_CHECK(hipGetLastError());
 }
 template void launch_stencil_apply<float>(const Geom&, const float*, const uint8_t*, const float*,

@@ -56,20 +56,28 @@ __global__ void restrict_k(Geom gf, Geom gc, const Tf* __restrict__ rf,
   const int B = static_cast<int>((n / gc.nnx) % gc.nny);
   const int C = static_cast<int>(n / (static_cast<long long>(gc.nnx) * gc.nny));
   const int mk = mask_c ? mask_c[n] : 7;
+  // 27 fine nodes in a fixed order; out-of-grid ones are clamped to the
+  // coincident node 2*(A, B, C) and weighted by zero. (Every load is used
+  // unconditionally, so the compiler issues them up front rather than
+  // behind a branch each.)
   double s[3] = {0.0, 0.0, 0.0};
+#pragma unroll
   for (int dz = -1; dz <= 1; ++dz) {
     const int c = 2 * C + dz;
-    if (c < 0 || c >= gf.nnz) continue;
+    const bool okz = c >= 0 && c < gf.nnz;
     const double wz = dz == 0 ? 1.0 : 0.5;
+#pragma unroll
     for (int dy = -1; dy <= 1; ++dy) {
       const int b = 2 * B + dy;
-      if (b < 0 || b >= gf.nny) continue;
+      const bool oky = okz && b >= 0 && b < gf.nny;
       const double wy = wz * (dy == 0 ? 1.0 : 0.5);
+#pragma unroll
       for (int dx = -1; dx <= 1; ++dx) {
         const int a = 2 * A + dx;
-        if (a < 0 || a >= gf.nnx) continue;
-        const double w = wy * (dx == 0 ? 1.0 : 0.5);
-        const long long fn = (static_cast<long long>(c) * gf.nny + b) * gf.nnx + a;
+        const bool ok = oky && a >= 0 && a < gf.nnx;
+        const double w = ok ? wy * (dx == 0 ? 1.0 : 0.5) : 0.0;
+        const long long fn = ok ? (static_cast<long long>(c) * gf.nny + b) * gf.nnx + a
+                                : (static_cast<long long>(2 * C) * gf.nny + 2 * B) * gf.nnx + 2 * A;
 #pragma unroll
         for (int q = 0; q < 3; ++q) s[q] += w * static_cast<double>(rf[fn * 3 + q]);
       }
@@ -81,12 +89,17 @@ __global__ void restrict_k(Geom gf, Geom gc, const Tf* __restrict__ rf,
       cm.V[rc * cm.vstride + cm.off + cm.dst[n * 3 + q]] = ((mk >> q) & 1) ? s[q] : 0.0;
     return;
   }
+  Tc dv[3] = {Tc(0), Tc(0), Tc(0)};
+  if (xc) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) dv[q] = dinv_c[n * 3 + q];
+  }
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const bool kept = (mk >> q) & 1;
     const Tc v = kept ? static_cast<Tc>(s[q]) : Tc(0);
     bc[n * 3 + q] = v;
-    if (xc) xc[n * 3 + q] = kept ? static_cast<Tc>(omega * static_cast<double>(dinv_c[n * 3 + q]) *
+    if (xc) xc[n * 3 + q] = kept ? static_cast<Tc>(omega * static_cast<double>(dv[q]) *
                                                    static_cast<double>(v))
                                  : Tc(0);
   }
@@ -108,33 +121,49 @@ __global__ void prolong_add_k(Geom gc, Geom gf, const Tc* __restrict__ xc,
   const int c = static_cast<int>(n / (static_cast<long long>(gf.nnx) * gf.nny));
   const int mk = mask_f ? mask_f[n] : 7;
   if (mk == 0) return;
-  int ia[2], ib[2], ic[2];
-  double wa[2], wb[2], wc[2];
-  int na, nb, nc;
-  if (a & 1) { ia[0] = (a - 1) / 2; ia[1] = (a + 1) / 2; wa[0] = wa[1] = 0.5; na = 2; }
-  else { ia[0] = a / 2; wa[0] = 1.0; na = 1; }
-  if (b & 1) { ib[0] = (b - 1) / 2; ib[1] = (b + 1) / 2; wb[0] = wb[1] = 0.5; nb = 2; }
-  else { ib[0] = b / 2; wb[0] = 1.0; nb = 1; }
-  if (c & 1) { ic[0] = (c - 1) / 2; ic[1] = (c + 1) / 2; wc[0] = wc[1] = 0.5; nc = 2; }
-  else { ic[0] = c / 2; wc[0] = 1.0; nc = 1; }
+  // The 1, 2, 4 or 8 coarse nodes of fine node (a, b, c), in a fixed order;
+  // the second neighbour along an even axis is absent: its (in-grid) load is
+  // weighted by zero, so every load is used and issued up front.
+  const int ia[2] = {a >> 1, (a + 1) >> 1}, ib[2] = {b >> 1, (b + 1) >> 1};
+  const int ic[2] = {c >> 1, (c + 1) >> 1};
+  const double wa = (a & 1) ? 0.5 : 1.0, wb = (b & 1) ? 0.5 : 1.0, wc = (c & 1) ? 0.5 : 1.0;
+  double w[8], v[8][3];
+  long long cn[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const int x = t >> 2, y = (t >> 1) & 1, z = t & 1;
+    const bool ok = (x == 0 || (a & 1)) && (y == 0 || (b & 1)) && (z == 0 || (c & 1));
+    w[t] = ok ? wa * wb * wc : 0.0;
+    cn[t] = (static_cast<long long>(ic[z]) * gc.nny + ib[y]) * gc.nnx + ia[x];
+  }
+  if (cm.dst) {  // coarse values straight from the coarse solver's FP64 solution
+    int d[8][3];
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) d[t][q] = cm.dst[cn[t] * 3 + q];
+    const double* V = cm.V + rc * cm.vstride + cm.off;
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) v[t][q] = V[d[t][q]];
+  } else {
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) v[t][q] = static_cast<double>(xc[cn[t] * 3 + q]);
+  }
   double s[3] = {0.0, 0.0, 0.0};
-  for (int x = 0; x < na; ++x)
-    for (int y = 0; y < nb; ++y)
-      for (int z = 0; z < nc; ++z) {
-        const double w = wa[x] * wb[y] * wc[z];
-        const long long cn = (static_cast<long long>(ic[z]) * gc.nny + ib[y]) * gc.nnx + ia[x];
-        if (cm.dst) {  // coarse values straight from the coarse solver's FP64 solution
 #pragma unroll
-          for (int q = 0; q < 3; ++q)
-            s[q] += w * cm.V[rc * cm.vstride + cm.off + cm.dst[cn * 3 + q]];
-        } else {
+  for (int t = 0; t < 8; ++t)
 #pragma unroll
-          for (int q = 0; q < 3; ++q) s[q] += w * static_cast<double>(xc[cn * 3 + q]);
-        }
-      }
+    for (int q = 0; q < 3; ++q) s[q] += w[t] * v[t][q];
+  Tf cur[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) cur[q] = xf[n * 3 + q];
 #pragma unroll
   for (int q = 0; q < 3; ++q)
-    if ((mk >> q) & 1) xf[n * 3 + q] = static_cast<Tf>(static_cast<double>(xf[n * 3 + q]) + s[q]);
+    if ((mk >> q) & 1) xf[n * 3 + q] = static_cast<Tf>(static_cast<double>(cur[q]) + s[q]);
 }
 
 __global__ void sum_partials_k(const double* __restrict__ p, int n, double* out, int pstride) {
