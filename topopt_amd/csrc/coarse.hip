@@ -29,7 +29,6 @@ namespace {
 
 constexpr int IV_MP = 256;  // blocks are padded to 256 x 256 (m <= 256)
 constexpr int IV_NB = 32;   // pivots per step = output tile edge
-static_assert(IV_NB % 2 == 0, "the ping-pong pivot sweep must end in Sw");
 
 // Batched inversion of SPD m x m blocks with the symmetric sweep operator,
 // one launch per 32-pivot block B and one workgroup per 32 x 32 output tile:
@@ -41,16 +40,31 @@ static_assert(IV_NB % 2 == 0, "the ping-pong pivot sweep must end in Sw");
 // 256 x 256 buffers per block), so tiles of one step never race. Pad entries
 // (index >= m) enter as the identity, which leaves the inverse of the m x m
 // block in its leading corner. Every workgroup sweeps the 32 x 32 pivot block
-// itself in LDS (cheap) instead of waiting for another launch.
+// itself (cheap) instead of waiting for another launch: each lane keeps its 4
+// entries of the block in registers and only the next pivot's row and column
+// go through LDS, one barrier per pivot. All global reads of a step (the
+// pivot block, the two panels and the output tile) are issued up front.
 // offs holds (source, destination) offset pairs; step 0 reads the source,
 // the last step writes the negated result to the destination.
+constexpr int IV_LD = IV_NB + 2;  // LDS row stride (even: 16-byte aligned rows)
+
+// 1 / d from the hardware reciprocal estimate and two Newton steps (the
+// estimate's relative error is below 2^-22, so the result is within an ulp).
+__device__ inline double recip(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return r;
+}
+
 __global__ void __launch_bounds__(256)
 inv_step_k(double* W, const long long* offs, int m, int step, int nsteps, double* scratch) {
-  __shared__ double Sw[IV_NB][IV_NB + 1];
-  __shared__ double Ai[IV_NB][IV_NB + 1];
-  __shared__ double Aj[IV_NB][IV_NB + 1];
-  __shared__ double Ui[IV_NB][IV_NB + 1];
-  __shared__ double Uj[IV_NB][IV_NB + 1];
+  __shared__ double Sw[IV_NB][IV_LD];
+  __shared__ double Ai[IV_NB][IV_LD];
+  __shared__ double AjT[IV_NB][IV_LD];  // transposed: AjT[e][r] = A_JB[r][e]
+  __shared__ double Ui[IV_NB][IV_LD];
+  __shared__ double Uj[IV_NB][IV_LD];
+  __shared__ double rowb[2][IV_NB], colb[2][IV_NB];
   const int tiles = (m + IV_NB - 1) / IV_NB;
   const int I = blockIdx.x / tiles, J = blockIdx.x % tiles;
   const int b = blockIdx.y;
@@ -65,123 +79,196 @@ inv_step_k(double* W, const long long* offs, int m, int step, int nsteps, double
     return in[i * IV_MP + j];
   };
   const int t = threadIdx.x;
-  const int tr = t / 8, tc = (t % 8) * 4;  // 4 consecutive columns per lane
+  const int tr = t / 8, tc = (t % 8) * 4;  // lane entries (tr, tc .. tc + 3) of every tile
   const int k0 = step * IV_NB;
+  const int i = I * IV_NB + tr;
+  const bool gen = I != step && J != step;  // output tile outside the pivot row/column
+  double e[4], ai[4], aj[4], aij[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    Sw[tr][tc + q] = rd(k0 + tr, k0 + tc + q);
-    Ai[tr][tc + q] = rd(I * IV_NB + tr, k0 + tc + q);
-    Aj[tr][tc + q] = rd(J * IV_NB + tr, k0 + tc + q);
+    e[q] = rd(k0 + tr, k0 + tc + q);
+    ai[q] = rd(i, k0 + tc + q);
+    aj[q] = rd(J * IV_NB + tr, k0 + tc + q);
+    if (gen) aij[q] = rd(i, J * IV_NB + tc + q);
   }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    Ai[tr][tc + q] = ai[q];
+    AjT[tc + q][tr] = aj[q];
+  }
+  if (tr == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) rowb[0][tc + q] = e[q];
+  }
+  if (tc == 0) colb[0][tr] = e[0];
   __syncthreads();
-  // 32 single-pivot sweeps, ping-ponging between Sw and Ui (free until the
-  // U products below), one barrier per pivot; IV_NB is even, so the swept
-  // block ends in Sw.
+  // 32 single-pivot sweeps of the pivot block, in registers.
   for (int p = 0; p < IV_NB; ++p) {
-    double (*S)[IV_NB + 1] = (p & 1) ? Ui : Sw;
-    double (*D)[IV_NB + 1] = (p & 1) ? Sw : Ui;
-    const double inv = 1.0 / S[p][p];
+    const int cur = p & 1, nxt = cur ^ 1;
+    const double inv = recip(rowb[cur][p]);
+    const double cp = colb[cur][tr];  // S[tr][p]
+    double rp[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) rp[q] = rowb[cur][tc + q];  // S[p][tc + q]
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int sa = tr, sb = tc + q;
-      const double v = S[sa][sb];
-      double nv;
-      if (sa == p && sb == p) nv = -inv;
-      else if (sa == p || sb == p) nv = v * inv;
-      else nv = v - S[sa][p] * (S[p][sb] * inv);
-      D[sa][sb] = nv;
+      const int sb = tc + q;
+      if (tr == p && sb == p) e[q] = -inv;
+      else if (tr == p || sb == p) e[q] = e[q] * inv;
+      else e[q] = e[q] - cp * (rp[q] * inv);
+    }
+    if (p + 1 < IV_NB) {  // publish the next pivot's row and column
+      if (tr == p + 1) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) rowb[nxt][tc + q] = e[q];
+      }
+      const int d = p + 1 - tc;
+      if (d >= 0 && d < 4) colb[nxt][tr] = d == 0 ? e[0] : d == 1 ? e[1] : d == 2 ? e[2] : e[3];
     }
     __syncthreads();
   }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) Sw[tr][tc + q] = e[q];
+  __syncthreads();
   // U_I = -A_IB Sw, U_J = -A_JB Sw
+  double si[4] = {0.0, 0.0, 0.0, 0.0}, sj[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < IV_NB; ++k) {
+    const double a = Ai[tr][k], c = AjT[k][tr];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double w = Sw[k][tc + q];
+      si[q] -= a * w;
+      sj[q] -= c * w;
+    }
+  }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    double si = 0.0, sj = 0.0;
-    for (int e = 0; e < IV_NB; ++e) {
-      si -= Ai[tr][e] * Sw[e][tc + q];
-      sj -= Aj[tr][e] * Sw[e][tc + q];
-    }
-    Ui[tr][tc + q] = si;
-    Uj[tr][tc + q] = sj;
+    Ui[tr][tc + q] = si[q];
+    Uj[tr][tc + q] = sj[q];
   }
   __syncthreads();
-  const int i = I * IV_NB + tr;
+  double v[4];
+  if (I == step && J == step) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = Sw[tr][tc + q];
+  } else if (J == step) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = Ui[tr][tc + q];
+  } else if (I == step) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = Uj[tc + q][tr];
+  } else {
+    double sacc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < IV_NB; ++k) {
+      const double u = Ui[tr][k];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sacc[q] += u * AjT[k][tc + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = aij[q] - sacc[q];
+  }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int j = J * IV_NB + tc + q;
-    double v;
-    if (I == step && J == step) {
-      v = Sw[tr][tc + q];
-    } else if (J == step) {
-      v = Ui[tr][tc + q];
-    } else if (I == step) {
-      v = Uj[tc + q][tr];
-    } else {
-      double s = 0.0;
-      for (int e = 0; e < IV_NB; ++e) s += Ui[tr][e] * Aj[tc + q][e];
-      v = rd(i, j) - s;
-    }
     if (last) {
-      if (i < m && j < m) dst[static_cast<long long>(i) * m + j] = -v;
+      if (i < m && j < m) dst[static_cast<long long>(i) * m + j] = -v[q];
     } else {
-      out[i * IV_MP + j] = v;
+      out[i * IV_MP + j] = v[q];
     }
   }
 }
 
-// Batched C = beta*Din + sum_t alpha_t op(A_t) op(B_t), all m x m row-major.
+// Batched C = beta*D + sum_t alpha_t op(A_t) op(B_t), all m x m row-major
+// (m <= 256), and Ct = C^T as well when the job names one. One workgroup per
+// 32 x 32 output tile; wave w multiplies the k range [64w, 64w + 64) for the
+// whole tile on the FP64 matrix cores (2 x 2 blocks of 16 x 16, k = 4 per
+// instruction, two batches of 8 instructions per term with every operand of
+// a batch loaded first), and the four partial tiles are added in LDS in a
+// fixed order. Operands past row/column m are read in bounds (the workspace
+// has 256 x 256 doubles of tail padding) and zeroed by a bit mask, not a
+// branch. alpha_t scales the A operand (every job uses alpha = +-1, exact).
+typedef double dbl4 __attribute__((ext_vector_type(4)));
 constexpr int GT = 32;
+constexpr long long kWPad = static_cast<long long>(IV_MP) * IV_MP;
+
+__device__ inline double ld_keep(const double* p, long long i, bool keep) {
+  const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(p[i]));
+  return __longlong_as_double(static_cast<long long>(u & (keep ? ~0ULL : 0ULL)));
+}
+
 __global__ void __launch_bounds__(256) gemm_jobs_k(double* W, const CrGemm* jobs, int m) {
-  __shared__ double As[GT][GT + 1];
-  __shared__ double Bs[GT][GT + 1];
+  __shared__ double red[4][GT][GT + 1];
   const CrGemm& jb = jobs[blockIdx.y];  // read in place: indexing a local copy by term would spill
   const int tiles = (m + GT - 1) / GT;
   const int tr = blockIdx.x / tiles, tc = blockIdx.x % tiles;
-  const int lr = threadIdx.x / 16, lc = threadIdx.x % 16;  // 16 x 16 lanes, 2 x 2 outputs each
-  double acc[2][2] = {{0, 0}, {0, 0}};
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const int li = l % 16, lk = l / 16;
+  // v_mfma_f64_16x16x4f64 lane layout: A[i][k] from lane (i + 16k), B[k][j]
+  // from lane (j + 16k); result register r of lane l is D[l/16 + 4r][l%16].
+  // Lane (li, lk) supplies k = kb + s at instruction s of a batch, kb its
+  // batch base, so each lane walks 8 consecutive k.
+  dbl4 acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) acc[x][y] = dbl4{0.0, 0.0, 0.0, 0.0};
   for (int t = 0; t < jb.nterms; ++t) {
     const double* A = W + jb.a[t];
     const double* B = W + jb.b[t];
-    double tacc[2][2] = {{0, 0}, {0, 0}};
-    for (int k0 = 0; k0 < m; k0 += GT) {
-      for (int q = threadIdx.x; q < GT * GT; q += 256) {
-        const int r = q / GT, c = q % GT;
-        const int gr = tr * GT + r, gk = k0 + c;  // op(A)[gr][gk]
-        double va = 0.0;
-        if (gr < m && gk < m) va = jb.ta[t] ? A[static_cast<long long>(gk) * m + gr] : A[static_cast<long long>(gr) * m + gk];
-        As[r][c] = va;
-        const int bk = k0 + r, bc = tc * GT + c;  // op(B)[bk][bc]
-        double vb = 0.0;
-        if (bk < m && bc < m) vb = jb.tb[t] ? B[static_cast<long long>(bc) * m + bk] : B[static_cast<long long>(bk) * m + bc];
-        Bs[r][c] = vb;
+    const long long sa = jb.ta[t] ? m : 1, ra = jb.ta[t] ? 1 : m;  // A(r, k) = A[r*ra + k*sa]
+    const long long sb = jb.tb[t] ? 1 : m, cb = jb.tb[t] ? m : 1;  // B(k, c) = B[k*sb + c*cb]
+    const double al = jb.alpha[t];
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+      const int kb = 64 * w + 32 * h + 8 * lk;
+      double a[2][8], b[2][8];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        const int r = tr * GT + 16 * x + li, c = tc * GT + 16 * x + li;
+#pragma unroll
+        for (int s8 = 0; s8 < 8; ++s8) {
+          const int k = kb + s8;
+          a[x][s8] = al * ld_keep(A, r * ra + k * sa, r < m && k < m);
+          b[x][s8] = ld_keep(B, k * sb + c * cb, k < m && c < m);
+        }
       }
-      __syncthreads();
-#pragma unroll 8
-      for (int kk = 0; kk < GT; ++kk) {
-        const double a0 = As[lr * 2][kk], a1 = As[lr * 2 + 1][kk];
-        const double b0 = Bs[kk][lc * 2], b1 = Bs[kk][lc * 2 + 1];
-        tacc[0][0] += a0 * b0; tacc[0][1] += a0 * b1;
-        tacc[1][0] += a1 * b0; tacc[1][1] += a1 * b1;
-      }
-      __syncthreads();
+#pragma unroll
+      for (int s8 = 0; s8 < 8; ++s8)
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int y = 0; y < 2; ++y)
+            acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x][s8], b[y][s8], acc[x][y], 0, 0, 0);
     }
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-      for (int y = 0; y < 2; ++y) acc[x][y] += jb.alpha[t] * tacc[x][y];
   }
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int y = 0; y < 2; ++y) {
-      const int gr = tr * GT + lr * 2 + x, gc = tc * GT + lc * 2 + y;
-      if (gr < m && gc < m) {
-        const long long o = static_cast<long long>(gr) * m + gc;
-        double v = acc[x][y];
-        if (jb.beta != 0.0) v += jb.beta * W[jb.d + o];
-        W[jb.c + o] = v;
-      }
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[w][16 * x + lk + 4 * r][16 * y + li] = acc[x][y][r];
+  __syncthreads();
+  auto tile_sum = [&](int i, int j) {
+    return ((red[0][i][j] + red[1][i][j]) + red[2][i][j]) + red[3][i][j];
+  };
+  const int i = threadIdx.x / 8, j0 = (threadIdx.x % 8) * 4;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int gr = tr * GT + i, gc = tc * GT + j0 + q;
+    if (gr < m && gc < m) {
+      const long long o = static_cast<long long>(gr) * m + gc;
+      double v = tile_sum(i, j0 + q);
+      if (jb.beta != 0.0) v += jb.beta * W[jb.d + o];
+      W[jb.c + o] = v;
     }
+  }
+  if (jb.ct >= 0 && jb.beta == 0.0) {  // C^T, written along its rows
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int gr = tr * GT + j0 + q, gc = tc * GT + i;  // C[gr][gc] -> Ct[gc][gr]
+      if (gr < m && gc < m) W[jb.ct + static_cast<long long>(gc) * m + gr] = tile_sum(j0 + q, i);
+    }
+  }
 }
 
 // Batched y = beta*y0 + sum_t alpha_t M_t v_t (length-m vectors, FP64; M_t
@@ -371,23 +458,15 @@ bool CoarseCR::plan(const Geom& g, hipStream_t s) {
       inv_offs.push_back(L.Dinv + i * mm);
     }
     for (int i = 1; i < nr; i += 2) {
-      CrGemm g1{};  // P_i = Dinv_i L_i
-      g1.c = L.P + i * mm; g1.nterms = 1; g1.a[0] = L.Dinv + i * mm; g1.b[0] = L.L + i * mm;
-      g1.alpha[0] = 1.0;
+      CrGemm g1{};  // P_i = Dinv_i L_i, and P_i^T
+      g1.c = L.P + i * mm; g1.ct = L.Pt + i * mm; g1.nterms = 1; g1.a[0] = L.Dinv + i * mm;
+      g1.b[0] = L.L + i * mm; g1.alpha[0] = 1.0;
       gemm_p.push_back(g1);
-      CrGemm g1t{};  // P_i^T = L_i^T Dinv_i
-      g1t.c = L.Pt + i * mm; g1t.nterms = 1; g1t.a[0] = L.L + i * mm; g1t.ta[0] = 1;
-      g1t.b[0] = L.Dinv + i * mm; g1t.alpha[0] = 1.0;
-      gemm_p.push_back(g1t);
       if (i + 1 < nr) {
-        CrGemm g2{};  // Q_i = Dinv_i L_{i+1}^T
-        g2.c = L.Q + i * mm; g2.nterms = 1; g2.a[0] = L.Dinv + i * mm;
+        CrGemm g2{};  // Q_i = Dinv_i L_{i+1}^T, and Q_i^T
+        g2.c = L.Q + i * mm; g2.ct = L.Qt + i * mm; g2.nterms = 1; g2.a[0] = L.Dinv + i * mm;
         g2.b[0] = L.L + (i + 1) * mm; g2.tb[0] = 1; g2.alpha[0] = 1.0;
         gemm_p.push_back(g2);
-        CrGemm g2t{};  // Q_i^T = L_{i+1} Dinv_i
-        g2t.c = L.Qt + i * mm; g2t.nterms = 1; g2t.a[0] = L.L + (i + 1) * mm;
-        g2t.b[0] = L.Dinv + i * mm; g2t.alpha[0] = 1.0;
-        gemm_p.push_back(g2t);
       }
     }
     for (int j = 0; j < nr; j += 2) {
@@ -453,7 +532,7 @@ bool CoarseCR::plan(const Geom& g, hipStream_t s) {
       TOPOHIP_CHECK(hipMemcpyAsync(buf.p, host.data(), host.size() * sizeof(host[0]),
                                    hipMemcpyHostToDevice, s));
   };
-  W.ensure(static_cast<size_t>(wsize));
+  W.ensure(static_cast<size_t>(wsize + kWPad));  // tail pad: the GEMM reads up to 256 x 256 per block
   ncases = 0;
   reserve_cases(1);
   size_t maxinv = 1;

@@ -9,9 +9,10 @@
 namespace topohip {
 
 // C = beta*D + sum_t alpha_t op(A_t) op(B_t) on m x m blocks of a workspace
-// (offsets in doubles).
+// (offsets in doubles); with ct >= 0 (and beta = 0) also Ct = C^T.
 struct CrGemm {
   long long c, d;
+  long long ct = -1;
   long long a[2], b[2];
   int ta[2], tb[2];
   double alpha[2];
