@@ -178,29 +178,36 @@ inv_step_k(double* W, const long long* offs, int m, int step, int nsteps, double
   }
 }
 
-// Batched C = beta*D + sum_t alpha_t op(A_t) op(B_t), all m x m row-major
-// (m <= 256), and Ct = C^T as well when the job names one. One workgroup per
-// 32 x 32 output tile; wave w multiplies the k range [64w, 64w + 64) for the
-// whole tile on the FP64 matrix cores (2 x 2 blocks of 16 x 16, k = 4 per
-// instruction, two batches of 8 instructions per term with every operand of
-// a batch loaded first), and the four partial tiles are added in LDS in a
-// fixed order. Operands past row/column m are read in bounds (the workspace
-// has 256 x 256 doubles of tail padding) and zeroed by a bit mask, not a
-// branch. alpha_t scales the A operand (every job uses alpha = +-1, exact).
+// Batched C = beta*D + sum_t alpha_t op(A_t) op(B_t): op(A) m x m (m <= 256),
+// op(B) m x n, C and D m x n, B, C and D with row stride ld (square jobs:
+// n = ld = m), and Ct = C^T as well when a square job names one. One
+// workgroup per 32 x 32 output tile; wave w multiplies the k range
+// [64w, 64w + 64) for the whole tile on the FP64 matrix cores (2 x 2 blocks of
+// 16 x 16, k = 4 per instruction, two batches of 8 instructions per term with
+// every operand of a batch loaded first), and the four partial tiles are added
+// in LDS in a fixed order. Operands past row/column m (or n) are read at a
+// clamped in-bounds address and zeroed by a bit mask, not a branch. alpha_t
+// scales the A operand (every job uses alpha = +-1, exact).
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 constexpr int GT = 32;
-constexpr long long kWPad = static_cast<long long>(IV_MP) * IV_MP;
+
+__device__ inline double wave_sum64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
 
 __device__ inline double ld_keep(const double* p, long long i, bool keep) {
   const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(p[i]));
   return __longlong_as_double(static_cast<long long>(u & (keep ? ~0ULL : 0ULL)));
 }
 
-__global__ void __launch_bounds__(256) gemm_jobs_k(double* W, const CrGemm* jobs, int m) {
+__global__ void __launch_bounds__(256) gemm_jobs_k(double* W, const CrGemm* jobs, int m, int n,
+                                                   int ld) {
   __shared__ double red[4][GT][GT + 1];
   const CrGemm& jb = jobs[blockIdx.y];  // read in place: indexing a local copy by term would spill
-  const int tiles = (m + GT - 1) / GT;
-  const int tr = blockIdx.x / tiles, tc = blockIdx.x % tiles;
+  const int tcols = (n + GT - 1) / GT;
+  const int tr = blockIdx.x / tcols, tc = blockIdx.x % tcols;
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
   const int li = l % 16, lk = l / 16;
   // v_mfma_f64_16x16x4f64 lane layout: A[i][k] from lane (i + 16k), B[k][j]
@@ -215,8 +222,8 @@ __global__ void __launch_bounds__(256) gemm_jobs_k(double* W, const CrGemm* jobs
   for (int t = 0; t < jb.nterms; ++t) {
     const double* A = W + jb.a[t];
     const double* B = W + jb.b[t];
-    const long long sa = jb.ta[t] ? m : 1, ra = jb.ta[t] ? 1 : m;  // A(r, k) = A[r*ra + k*sa]
-    const long long sb = jb.tb[t] ? 1 : m, cb = jb.tb[t] ? m : 1;  // B(k, c) = B[k*sb + c*cb]
+    const long long sa = jb.ta[t] ? m : 1, ra = jb.ta[t] ? 1 : m;    // A(r, k) = A[r*ra + k*sa]
+    const long long sb = jb.tb[t] ? 1 : ld, cb = jb.tb[t] ? ld : 1;  // B(k, c) = B[k*sb + c*cb]
     const double al = jb.alpha[t];
 #pragma unroll 1
     for (int h = 0; h < 2; ++h) {
@@ -225,11 +232,13 @@ __global__ void __launch_bounds__(256) gemm_jobs_k(double* W, const CrGemm* jobs
 #pragma unroll
       for (int x = 0; x < 2; ++x) {
         const int r = tr * GT + 16 * x + li, c = tc * GT + 16 * x + li;
+        const int rc = r < m ? r : m - 1, cc = c < n ? c : n - 1;
 #pragma unroll
         for (int s8 = 0; s8 < 8; ++s8) {
           const int k = kb + s8;
-          a[x][s8] = al * ld_keep(A, r * ra + k * sa, r < m && k < m);
-          b[x][s8] = ld_keep(B, k * sb + c * cb, k < m && c < m);
+          const int kc = k < m ? k : m - 1;
+          a[x][s8] = al * ld_keep(A, rc * ra + kc * sa, r < m && k < m);
+          b[x][s8] = ld_keep(B, kc * sb + cc * cb, k < m && c < n);
         }
       }
 #pragma unroll
@@ -255,14 +264,14 @@ __global__ void __launch_bounds__(256) gemm_jobs_k(double* W, const CrGemm* jobs
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int gr = tr * GT + i, gc = tc * GT + j0 + q;
-    if (gr < m && gc < m) {
-      const long long o = static_cast<long long>(gr) * m + gc;
+    if (gr < m && gc < n) {
+      const long long o = static_cast<long long>(gr) * ld + gc;
       double v = tile_sum(i, j0 + q);
       if (jb.beta != 0.0) v += jb.beta * W[jb.d + o];
       W[jb.c + o] = v;
     }
   }
-  if (jb.ct >= 0 && jb.beta == 0.0) {  // C^T, written along its rows
+  if (jb.ct >= 0 && jb.beta == 0.0 && n == m && ld == m) {  // C^T, written along its rows
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int gr = tr * GT + j0 + q, gc = tc * GT + i;  // C[gr][gc] -> Ct[gc][gr]
@@ -271,17 +280,43 @@ __global__ void __launch_bounds__(256) gemm_jobs_k(double* W, const CrGemm* jobs
   }
 }
 
+// y = M v for one dense n x n FP64 matrix (the inverse of the coarse solver's
+// tail level): one wave per row, 8 strides of 64 lanes per batch with every
+// load of a batch issued first, then the fixed shuffle tree.
+__global__ void __launch_bounds__(256) gemv_dense_k(const double* W, double* V, long long vsz,
+                                                    long long moff, long long voff,
+                                                    long long ooff, int n, const int* done) {
+  const int rc = blockIdx.z;
+  if (case_done(done, rc)) return;
+  V += rc * vsz;
+  const int lane = threadIdx.x & 63;
+  const int o = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (o >= n) return;  // whole wave
+  const double* M = W + moff + static_cast<long long>(o) * n;
+  const double* v = V + voff;
+  double s = 0.0;
+  for (int k0 = 0; k0 < n; k0 += 512) {
+    double pm[8], pv[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int k = k0 + lane + 64 * i;
+      const int kc = k < n ? k : n - 1;
+      pm[i] = ld_keep(M, kc, k < n);
+      pv[i] = v[kc];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s += pm[i] * pv[i];
+  }
+  s = wave_sum64(s);
+  if (lane == 0) V[ooff + o] = s;
+}
+
 // Batched y = beta*y0 + sum_t alpha_t M_t v_t (length-m vectors, FP64; M_t
 // row-major, one row per output). One wave per output row: the lanes stride
 // the row (coalesced), then a fixed shuffle tree adds the 64 partial sums, so
 // each reduction level is one launch with ~m/4 * jobs workgroups and the result
 // is deterministic. (The plan stores every plane block in both orientations,
 // so each product reads its matrix along the rows.)
-__device__ inline double wave_sum64(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
-}
 __global__ void __launch_bounds__(256) gemv_rows_k(const double* W, double* V, long long vsz,
                                                    const CrGemv* jobs, int m, const int* done) {
   const int rc = blockIdx.z;
@@ -318,6 +353,12 @@ __global__ void __launch_bounds__(256) gemv_rows_k(const double* W, double* V, l
     if (jb.beta != 0.0) s += jb.beta * V[jb.y0 + o];
     V[jb.out + o] = s;
   }
+}
+
+// A = I (n x n).
+__global__ void eye_k(double* A, long long n) {
+  const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (t < n * n) A[t] = (t / n == t % n) ? 1.0 : 0.0;
 }
 
 // Blocks of the coarsest operator in plane order, from the element matrices.
@@ -523,6 +564,54 @@ bool CoarseCR::plan(const Geom& g, hipStream_t s) {
   lv_gd.push_back(static_cast<int>(gemm_d.size()));
   lv_fw.push_back(static_cast<int>(fwd.size()));
   lv_bw.push_back(static_cast<int>(bwd.size()));
+  // Dense tail: the first reduction level dl whose whole system has at most
+  // kDenseMax unknowns is solved by one GEMV with its explicit inverse, which
+  // the factorisation forms by running the remaining reduction on the
+  // identity: each length-m vector of a level r >= dl becomes an m x nden
+  // block (in W), and every GEMV job of those levels the same GEMM job.
+  dl = 0;
+  while (static_cast<long long>(lv[dl].n) * m > kDenseMax) ++dl;
+  nden = static_cast<long long>(lv[dl].n) * m;
+  std::vector<long long> bm(lv.size(), -1), xm(lv.size(), -1);
+  for (size_t r = dl; r < lv.size(); ++r) {
+    bm[r] = off; off += static_cast<long long>(lv[r].n) * m * nden;
+    xm[r] = off; off += static_cast<long long>(lv[r].n) * m * nden;
+  }
+  xm_dense = xm[dl];
+  eye_off = bm[dl];
+  wsize = off;
+  auto to_mat = [&](long long o) {
+    for (size_t r = dl; r < lv.size(); ++r) {
+      const long long span = static_cast<long long>(lv[r].n) * m;
+      if (o >= lv[r].b && o < lv[r].b + span) return bm[r] + (o - lv[r].b) * nden;
+      if (o >= lv[r].x && o < lv[r].x + span) return xm[r] + (o - lv[r].x) * nden;
+    }
+    throw Error(TOPOHIP_EINTERNAL, "coarse solver: vector outside the dense tail");
+  };
+  auto as_gemm = [&](const CrGemv& v) {
+    CrGemm g{};
+    g.c = to_mat(v.out);
+    g.beta = v.beta;
+    if (v.beta != 0.0) g.d = to_mat(v.y0);
+    g.nterms = v.nterms;
+    for (int t = 0; t < v.nterms; ++t) {
+      g.a[t] = v.m[t];
+      g.b[t] = to_mat(v.v[t]);
+      g.alpha[t] = v.alpha[t];
+    }
+    return g;
+  };
+  std::vector<CrGemm> mj;
+  lv_mj.clear();
+  for (size_t r = dl; r + 1 < lv.size(); ++r) {
+    lv_mj.push_back(static_cast<int>(mj.size()));
+    for (int j = lv_fw[r]; j < lv_fw[r + 1]; ++j) mj.push_back(as_gemm(fwd[j]));
+  }
+  for (int r = static_cast<int>(lv.size()) - 1; r >= dl; --r) {
+    lv_mj.push_back(static_cast<int>(mj.size()));
+    for (int j = lv_bw[r]; j < lv_bw[r + 1]; ++j) mj.push_back(as_gemm(bwd[j]));
+  }
+  lv_mj.push_back(static_cast<int>(mj.size()));
   // Uploads go through the problem's own stream: a legacy-stream copy would
   // have to order itself against other problems' streams, which may be
   // capturing a graph at that moment.
@@ -532,7 +621,10 @@ bool CoarseCR::plan(const Geom& g, hipStream_t s) {
       TOPOHIP_CHECK(hipMemcpyAsync(buf.p, host.data(), host.size() * sizeof(host[0]),
                                    hipMemcpyHostToDevice, s));
   };
-  W.ensure(static_cast<size_t>(wsize + kWPad));  // tail pad: the GEMM reads up to 256 x 256 per block
+  W.ensure(static_cast<size_t>(wsize));
+  hipLaunchKernelGGL(eye_k, dim3(static_cast<unsigned>((nden * nden + 255) / 256)), dim3(256), 0, s,
+                     W.p + eye_off, nden);
+  TOPOHIP_CHECK(hipGetLastError());
   ncases = 0;
   reserve_cases(1);
   size_t maxinv = 1;
@@ -542,6 +634,7 @@ bool CoarseCR::plan(const Geom& g, hipStream_t s) {
   up(d_inv, inv_offs);
   up(d_gp, gemm_p);
   up(d_gd, gemm_d);
+  up(d_mj, mj);
   up(d_fw, fwd);
   up(d_bw, bwd);
   up(d_plane, plane_of);
@@ -573,15 +666,26 @@ void CoarseCR::build(const double* Kem, const uint8_t* mask, hipStream_t s) {
     const int np = lv_gp[r + 1] - lv_gp[r];
     if (np > 0) {
       hipLaunchKernelGGL(gemm_jobs_k, dim3(tiles * tiles, np), dim3(256), 0, s, W.p,
-                         d_gp.p + lv_gp[r], m);
+                         d_gp.p + lv_gp[r], m, m, m);
       TOPOHIP_CHECK(hipGetLastError());
     }
     const int nd = lv_gd[r + 1] - lv_gd[r];
     if (nd > 0) {
       hipLaunchKernelGGL(gemm_jobs_k, dim3(tiles * tiles, nd), dim3(256), 0, s, W.p,
-                         d_gd.p + lv_gd[r], m);
+                         d_gd.p + lv_gd[r], m, m, m);
       TOPOHIP_CHECK(hipGetLastError());
     }
+  }
+  // Dense tail: A_dl^-1 = the reduction solve of levels >= dl applied to the
+  // identity (every vector of those levels is an m x nden block row).
+  const int nt = static_cast<int>(nden);
+  const unsigned mtiles = static_cast<unsigned>(tiles * ((nt + 31) / 32));
+  for (size_t i = 0; i + 1 < lv_mj.size(); ++i) {
+    const int nj = lv_mj[i + 1] - lv_mj[i];
+    if (nj == 0) continue;
+    hipLaunchKernelGGL(gemm_jobs_k, dim3(mtiles, nj), dim3(256), 0, s, W.p, d_mj.p + lv_mj[i], m,
+                       nt, nt);
+    TOPOHIP_CHECK(hipGetLastError());
   }
 }
 
@@ -612,16 +716,20 @@ CrMap CoarseCR::sol_map() {
   return cm;
 }
 
+// Forward reduction down to the dense level, its explicit inverse, backward
+// substitution back up.
+void CoarseCR::reduce_solve(const int* done, hipStream_t s, int nr) {
+  for (int r = 0; r < dl; ++r) gemv(d_fw.p + lv_fw[r], lv_fw[r + 1] - lv_fw[r], done, s, nr);
+  const int nt = static_cast<int>(nden);
+  hipLaunchKernelGGL(gemv_dense_k, dim3((nt + 3) / 4, 1, nr), dim3(256), 0, s, W.p, V.p, vsize,
+                     xm_dense, lv[dl].b, lv[dl].x, nt, done);
+  TOPOHIP_CHECK(hipGetLastError());
+  for (int r = dl - 1; r >= 0; --r) gemv(d_bw.p + lv_bw[r], lv_bw[r + 1] - lv_bw[r], done, s, nr);
+}
+
 void CoarseCR::solve_in_place(const int* done, hipStream_t s, int nr) {
   if (nr > ncases) throw Error(TOPOHIP_EINTERNAL, "coarse solve: load-case space not reserved");
-  for (size_t r = 0; r + 1 < lv.size(); ++r) {
-    const int nf = lv_fw[r + 1] - lv_fw[r];
-    gemv(d_fw.p + lv_fw[r], nf, done, s, nr);
-  }
-  for (int r = static_cast<int>(lv.size()) - 1; r >= 0; --r) {
-    const int nbw = lv_bw[r + 1] - lv_bw[r];
-    gemv(d_bw.p + lv_bw[r], nbw, done, s, nr);
-  }
+  reduce_solve(done, s, nr);
 }
 
 template <typename T>
@@ -631,14 +739,7 @@ void CoarseCR::solve(const T* b, T* x, const int* done, hipStream_t s, int nr) {
   const dim3 grid(static_cast<unsigned>((n + 255) / 256), nr);
   hipLaunchKernelGGL((cr_gather_k<T>), grid, dim3(256), 0, s, b, d_dst.p, n, V.p, vsize, lv[0].b,
                      done);
-  for (size_t r = 0; r + 1 < lv.size(); ++r) {
-    const int nf = lv_fw[r + 1] - lv_fw[r];
-    gemv(d_fw.p + lv_fw[r], nf, done, s, nr);
-  }
-  for (int r = static_cast<int>(lv.size()) - 1; r >= 0; --r) {
-    const int nbw = lv_bw[r + 1] - lv_bw[r];
-    gemv(d_bw.p + lv_bw[r], nbw, done, s, nr);
-  }
+  reduce_solve(done, s, nr);
   hipLaunchKernelGGL((cr_scatter_k<T>), grid, dim3(256), 0, s, V.p, vsize, lv[0].x, d_dst.p, n, x,
                      done);
   TOPOHIP_CHECK(hipGetLastError());

@@ -8,14 +8,14 @@
 
 namespace topohip {
 
-// C = beta*D + sum_t alpha_t op(A_t) op(B_t) on m x m blocks of a workspace
-// (offsets in doubles); with ct >= 0 (and beta = 0) also Ct = C^T.
+// C = beta*D + sum_t alpha_t op(A_t) op(B_t) on blocks of a workspace
+// (offsets in doubles); with ct >= 0 (square jobs, beta = 0) also Ct = C^T.
 struct CrGemm {
   long long c, d;
   long long ct = -1;
-  long long a[2], b[2];
-  int ta[2], tb[2];
-  double alpha[2];
+  long long a[3], b[3];
+  int ta[3], tb[3];
+  double alpha[3];
   double beta;
   int nterms;
 };
@@ -66,10 +66,18 @@ class CoarseCR {
   int ncases = 0;
   std::vector<CrLevel> lv;
   std::vector<int> lv_inv, lv_gp, lv_gd, lv_fw, lv_bw;
+  // Dense tail: levels >= dl are solved with the explicit inverse of level
+  // dl's system (nden x nden at W + xm_dense), formed by the GEMM jobs
+  // d_mj (one launch per lv_mj range) from the identity at W + eye_off.
+  static constexpr long long kDenseMax = 2048;
+  int dl = 0;
+  long long nden = 0, xm_dense = 0, eye_off = 0;
+  std::vector<int> lv_mj;
   void gemv(const CrGemv* jobs, int n, const int* done, hipStream_t s, int nr);
+  void reduce_solve(const int* done, hipStream_t s, int nr);
   DBuf<double> W, V, inv_scratch;
   DBuf<long long> d_inv;
-  DBuf<CrGemm> d_gp, d_gd;
+  DBuf<CrGemm> d_gp, d_gd, d_mj;
   DBuf<CrGemv> d_fw, d_bw;
   DBuf<int> d_plane, d_local, d_dst;
 };
