@@ -222,6 +222,38 @@ void element_constants(Problem& P) {
         P.S8[c * 576 + r * 24 + s] = v;
       }
   }
+  // Two coarsenings at once: T[c2 * 8 + c1] = W_{c2}^T S_{c1} W_{c2}, the
+  // level-2 contribution of fine element c1 of level-1 child c2 per unit modulus.
+  P.T64.assign(64 * 576, 0.0);
+  for (int c2 = 0; c2 < 8; ++c2) {
+    double W[576];
+    for (int a = 0; a < 24; ++a)
+      for (int s = 0; s < 24; ++s) {
+        double w = 0.0;
+        if (a % 3 == s % 3) {
+          const int ma = a / 3, ms = s / 3;
+          w = w1h((c2 & 1) + (ma & 1), ms & 1) *
+              w1h(((c2 >> 1) & 1) + ((ma >> 1) & 1), (ms >> 1) & 1) *
+              w1h(((c2 >> 2) & 1) + ((ma >> 2) & 1), (ms >> 2) & 1);
+        }
+        W[a * 24 + s] = w;
+      }
+    for (int c1 = 0; c1 < 8; ++c1) {
+      double SW[576];
+      for (int a = 0; a < 24; ++a)
+        for (int s = 0; s < 24; ++s) {
+          double v = 0.0;
+          for (int b = 0; b < 24; ++b) v += P.S8[c1 * 576 + a * 24 + b] * W[b * 24 + s];
+          SW[a * 24 + s] = v;
+        }
+      for (int r = 0; r < 24; ++r)
+        for (int s = 0; s < 24; ++s) {
+          double v = 0.0;
+          for (int a = 0; a < 24; ++a) v += W[a * 24 + r] * SW[a * 24 + s];
+          P.T64[(c2 * 8 + c1) * 576 + r * 24 + s] = v;
+        }
+    }
+  }
 }
 
 // Coarsening plan (build_mf_hierarchy + build_hierarchy acceptance rules,
@@ -241,6 +273,7 @@ void plan_levels(Problem& P) {
   lv0->nact = nact;
   P.L.push_back(std::move(lv0));
   P.mg_plan_ok = false;
+  P.l1_direct = false;
 
   auto coarsen = [](const Level& f, std::unique_ptr<Level>& out) -> bool {
     if ((f.g.ex & 1) || (f.g.ey & 1) || (f.g.ez & 1)) return false;
@@ -282,6 +315,7 @@ void plan_levels(Problem& P) {
   }
   for (auto& l : sub) P.L.push_back(std::move(l));
   P.mg_plan_ok = true;
+  P.l1_direct = P.L.size() >= 3;
 }
 
 template <typename T> T* Kapp(Level& l);
@@ -317,7 +351,7 @@ void alloc_levels(Problem& P) {
     lv.dinv64.ensure(static_cast<size_t>(nd));
     if (mixed) lv.dinv32.ensure(static_cast<size_t>(nd));
     if (l > 0) {
-      lv.Kem.ensure(static_cast<size_t>(lv.g.elems()) * 576);
+      if (!(l == 1 && P.l1_direct)) lv.Kem.ensure(static_cast<size_t>(lv.g.elems()) * 576);
       if (l == 1) {
         // Masks are fixed per problem (BCs and solid tags), so the coarse
         // elements that need the masked Galerkin path are listed once.
@@ -332,6 +366,37 @@ void alloc_levels(Problem& P) {
                   for (int a = 0; a < 3 && !any; ++a)
                     any = f.mask_h[(static_cast<long long>(2 * K + c) * f.g.nny + 2 * J + b) *
                                        f.g.nnx + 2 * I + a] != 7;
+              if (any) sp.push_back((K * lv.g.ey + J) * lv.g.ex + I);
+            }
+        lv.n_special = static_cast<int>(sp.size());
+        if (!sp.empty()) {
+          lv.special.ensure(sp.size());
+          h2d(lv.special.p, sp.data(), sp.size(), P.s);
+          sync(P);
+        }
+        if (P.l1_direct) {
+          lv.K1sp.ensure(std::max<size_t>(1, sp.size()) * 576);
+          std::vector<int> map(static_cast<size_t>(lv.g.elems()), -1);
+          for (size_t i = 0; i < sp.size(); ++i) map[static_cast<size_t>(sp[i])] = static_cast<int>(i);
+          lv.spmap.ensure(map.size());
+          h2d(lv.spmap.p, map.data(), map.size(), P.s);
+          sync(P);
+        }
+      }
+      if (l == 2 && P.l1_direct) {
+        // Level-2 elements whose 125 fine nodes are not all fully active take
+        // the Galerkin product over their level-1 children.
+        const Level& f = *P.L[0];
+        std::vector<int> sp;
+        for (int K = 0; K < lv.g.ez; ++K)
+          for (int J = 0; J < lv.g.ey; ++J)
+            for (int I = 0; I < lv.g.ex; ++I) {
+              bool any = false;
+              for (int c = 0; c < 5 && !any; ++c)
+                for (int b = 0; b < 5 && !any; ++b)
+                  for (int a = 0; a < 5 && !any; ++a)
+                    any = f.mask_h[(static_cast<long long>(4 * K + c) * f.g.nny + 4 * J + b) *
+                                       f.g.nnx + 4 * I + a] != 7;
               if (any) sp.push_back((K * lv.g.ey + J) * lv.g.ex + I);
             }
         lv.n_special = static_cast<int>(sp.size());
@@ -804,6 +869,11 @@ void ensure_mg64(Problem& P) {
   if (!P.mg64_ready) {
     for (size_t l = 1; l + 1 < P.L.size(); ++l) {
       Level& c = *P.L[l];
+      if (l == 1 && P.l1_direct) {  // (its special matrices are current: same hierarchy)
+        launch_stencil_l1<double>(c.g, P.L[0]->g, P.E64.p, P.dS8.p, c.K1sp.p, c.spmap.p,
+                                  c.mask.p, c.K64.p, c.dinv64.p, P.s);
+        continue;
+      }
       launch_diag_dense<double>(c.g, c.Kem.p, c.mask.p, c.dinv64.p, P.s);
       launch_assemble_stencil<double>(c.g, c.Kem.p, c.K64.p, P.s);
     }
@@ -870,6 +940,8 @@ std::unique_ptr<Problem> create_problem(int nx, int ny, int nz, double h, double
   h2d(P->dKb.p, P->Kb, 576, P->s);
   P->dS8.ensure(8 * 576);
   h2d(P->dS8.p, P->S8, 8 * 576, P->s);
+  P->dT64.ensure(64 * 576);
+  h2d(P->dT64.p, P->T64.data(), 64 * 576, P->s);
   P->dkdiag.ensure(24);
   h2d(P->dkdiag.p, P->kdiag, 24, P->s);
   P->E64.ensure(static_cast<size_t>(nv));
@@ -961,11 +1033,28 @@ void build_hierarchy(Problem& P, Timers& t) {
     for (size_t l = 1; l < P.L.size(); ++l) {
       Level& c = *P.L[l];
       Level& pf = *P.L[l - 1];
-      if (l == 1)
+      if (l == 1 && P.l1_direct) {
+        // stencil and diagonal straight from the fine moduli
+        launch_galerkin_fine_compact(c.g, pf.g, P.E64.p, pf.mask.p, c.mask.p, P.dS8.p, P.dKb.p,
+                                     c.special.p, c.n_special, c.K1sp.p, P.s);
+        if (mixed)
+          launch_stencil_l1<float>(c.g, pf.g, P.E64.p, P.dS8.p, c.K1sp.p, c.spmap.p, c.mask.p,
+                                   c.K32.p, c.dinv32.p, P.s);
+        else
+          launch_stencil_l1<double>(c.g, pf.g, P.E64.p, P.dS8.p, c.K1sp.p, c.spmap.p, c.mask.p,
+                                    c.K64.p, c.dinv64.p, P.s);
+        continue;
+      }
+      if (l == 1) {
         launch_galerkin_fine(c.g, pf.g, P.E64.p, pf.mask.p, c.mask.p, P.dS8.p, P.dKb.p,
                              c.special.p, c.n_special, c.Kem.p, P.s);
-      else
+      } else if (l == 2 && P.l1_direct) {
+        const Level& f0 = *P.L[0];
+        launch_galerkin_l2(c.g, pf.g, f0.g, P.E64.p, P.dT64.p, P.dS8.p, pf.K1sp.p, pf.spmap.p,
+                           c.mask.p, c.special.p, c.n_special, c.Kem.p, P.s);
+      } else {
         launch_galerkin_coarse(c.g, pf.g, pf.Kem.p, c.mask.p, c.Kem.p, P.s);
+      }
       const bool coarsest = l + 1 == P.L.size();
       if (!coarsest) {
         if (mixed) {

@@ -19,8 +19,10 @@ struct Level {
   DBuf<uint8_t> mask;
   long long nact = 0;           // active DOFs
   DBuf<double> Kem;             // element-major FP64 matrices (levels >= 1)
-  DBuf<int> special;            // level 1: coarse elements with an inactive fine DOF
+  DBuf<int> special;            // levels 1, 2: elements with an inactive fine DOF
   int n_special = 0;
+  DBuf<double> K1sp;            // level 1 (no stored operator): matrices of `special`
+  DBuf<int> spmap;              // level 1 (no stored operator): element -> K1sp slot or -1
   DBuf<float> K32;              // entry-major apply copies
   DBuf<double> K64;
   DBuf<float> dinv32;
@@ -64,12 +66,13 @@ struct Problem {
   double Ke[576];   // reference DOF order
   double Kb[576];   // parity-bit DOF order
   double S8[8 * 576];
+  std::vector<double> T64;   // W_{c2}^T S_{c1} W_{c2}, 64 blocks (level-2 elements from E)
   double kdiag[24];
   Modal<float> mc32;
   Modal<double> mc64;
   ModalGroups<float> mg32;   // K.u form of the modal coefficients
   ModalGroups<double> mg64;
-  DBuf<double> dKb, dS8, dkdiag;
+  DBuf<double> dKb, dS8, dkdiag, dT64;
 
   int nrhs = 0;
   std::vector<std::vector<double>> f_h;
@@ -80,6 +83,9 @@ struct Problem {
 
   std::vector<std::unique_ptr<Level>> L;
   bool mg_plan_ok = false;
+  // Level 1 is a stencil level (three or more levels): its operator is built
+  // straight from the fine moduli and no level-1 element matrices are stored.
+  bool l1_direct = false;
   bool mg64_alloc = false;   // mixed form: FP64 V-cycle buffers exist (retry path)
   bool mg64_ready = false;   // ... and hold the current hierarchy
   CoarseCR cr;  // exact coarsest-level solver

@@ -122,6 +122,28 @@ void launch_diag_fine(const Geom& g, const double* E, const double* kediag, cons
 void launch_galerkin_fine(const Geom& gc, const Geom& gf, const double* E, const uint8_t* mask_f,
                           const uint8_t* mask_c, const double* S8, const double* Ke,
                           const int* special, int n_special, double* Kout, hipStream_t s);
+// The masked path only, for the n_special listed level-1 elements, written
+// compactly: K1sp[i] is the matrix of special[i].
+void launch_galerkin_fine_compact(const Geom& gc, const Geom& gf, const double* E,
+                                  const uint8_t* mask_f, const uint8_t* mask_c, const double* S8,
+                                  const double* Ke, const int* special, int n_special,
+                                  double* K1sp, hipStream_t s);
+// Level-1 stencil (symmetric half storage, see launch_assemble_stencil) and
+// Jacobi inverse diagonal straight from the fine moduli, with no stored
+// level-1 element matrices: non-special elements from the S_c blocks, the
+// others from K1sp via spmap (level-1 element -> slot or -1).
+template <typename T>
+void launch_stencil_l1(const Geom& g1, const Geom& g0, const double* E, const double* S8,
+                       const double* K1sp, const int* spmap, const uint8_t* mask1, T* S, T* dinv,
+                       hipStream_t s);
+// Level-2 element matrices straight from the fine moduli (T64: the 64 blocks
+// W_{c2}^T S_{c1} W_{c2}); the n_special listed elements (an inactive DOF among
+// their 125 fine nodes) by the Galerkin product over level-1 children formed
+// as in launch_stencil_l1.
+void launch_galerkin_l2(const Geom& g2, const Geom& g1, const Geom& g0, const double* E,
+                        const double* T64, const double* S8, const double* K1sp, const int* spmap,
+                        const uint8_t* mask2, const int* special, int n_special, double* Kout,
+                        hipStream_t s);
 // Deeper element matrices: K_E = Mc (sum_c W_c^T K_c W_c) Mc.
 void launch_galerkin_coarse(const Geom& gc, const Geom& gf, const double* Kf,
                             const uint8_t* mask_c, double* Kout, hipStream_t s);
