@@ -156,6 +156,31 @@ typedef struct {
 int topohip_optimize(topohip_problem* p, const topohip_options* opt, double* design_out,
                      double* xphys_out, double* history, topohip_result* result);
 
+/* ---- passive regions: the masked design loop ------------------------------
+ * mask: one byte per voxel, the reference's MaskValue (core/include/topopt/
+ * voxel.hpp:42-46): 0 Active, 1 FrozenSolid, 2 FrozenVoid.
+ *
+ * topohip_optimize_masked mirrors simp_optimize(grid, params, bcs, loads,
+ * options, mask) (core/src/simp/simp.cpp:2989-3689) with the OC updater and
+ * one unprojected stage: the effective mask (effective_design_mask, :2610-2633:
+ * empty voxels -> FrozenVoid), the Active-only filter (make_density_filter
+ * with mask, :834-890, no bleed across the mask boundary), pins after every
+ * filter (apply_mask_pins, :2649-2654: FrozenSolid -> 1, FrozenVoid -> 0),
+ * oc_update_masked (:2662-2734: frozen voxels keep x, budget
+ * volfrac * n_active), volume fraction over the Active voxels. The handle must
+ * be created on the analysis grid (analysis_grid_for_mask, :2640-2645):
+ * FrozenVoid voxels empty; a FrozenVoid voxel that is solid in the handle's
+ * grid is TOPOHIP_EINVAL. Load/Fixture voxels are the caller's FrozenSolid.
+ * design_out / xphys_out / history / result as topohip_optimize (xphys_out
+ * pinned). */
+int topohip_optimize_masked(topohip_problem* p, const uint8_t* mask, const topohip_options* opt,
+                            double* design_out, double* xphys_out, double* history,
+                            topohip_result* result);
+/* make_density_filter(grid, radius, mask) applied to x (adjoint = 0:
+ * filter_density; 1: filter_sensitivity), core/src/simp/simp.cpp:834-944. */
+int topohip_filter_masked(topohip_problem* p, double radius, const uint8_t* mask,
+                          const double* x, double* out, int adjoint);
+
 /* ---- device-resident benchmarking entry (inputs already in HBM) -------------
  * Runs `reps` back-to-back applies of the fine operator on the handle's own
  * device buffers (random u, E = rho^3 set by topohip_bench_prepare) and

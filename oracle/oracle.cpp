@@ -1092,8 +1092,14 @@ SimpCompliance simp_compliance(const Grid& g, const SimpParams& p,
   return out;
 }
 
-DensityFilter make_density_filter(const Grid& g, double radius) {
+DensityFilter make_density_filter(const Grid& g, double radius,
+                                  const std::vector<std::uint8_t>& mask) {
   if (!(radius > 0.0)) throw std::invalid_argument("make_density_filter: radius must be > 0");
+  if (mask.size() != g.voxels())
+    throw std::invalid_argument("make_density_filter: mask size != voxel_count");
+  auto is_active = [&](int i, int j, int k) {
+    return g.is_solid(i, j, k) && mask[g.index(i, j, k)] == kActive;
+  };
   DensityFilter f;
   f.voxel_count = g.voxels();
   f.row_start.assign(f.voxel_count + 1, 0);
@@ -1104,14 +1110,14 @@ DensityFilter make_density_filter(const Grid& g, double radius) {
       for (int i = 0; i < g.nx; ++i) {
         const std::size_t e = g.index(i, j, k);
         f.row_start[e] = f.neighbor.size();
-        if (!g.is_solid(i, j, k)) continue;
+        if (!is_active(i, j, k)) continue;
         double hs = 0.0;
         for (int dk = -R; dk <= R; ++dk)
           for (int dj = -R; dj <= R; ++dj)
             for (int di = -R; di <= R; ++di) {
               const int ii = i + di, jj = j + dj, kk = k + dk;
               if (ii < 0 || ii >= g.nx || jj < 0 || jj >= g.ny || kk < 0 || kk >= g.nz) continue;
-              if (!g.is_solid(ii, jj, kk)) continue;
+              if (!is_active(ii, jj, kk)) continue;
               const double dist = std::sqrt(static_cast<double>(di * di + dj * dj + dk * dk));
               const double hw = radius - dist;
               if (hw <= 0.0) continue;
@@ -1123,6 +1129,11 @@ DensityFilter make_density_filter(const Grid& g, double radius) {
       }
   f.row_start[f.voxel_count] = f.neighbor.size();
   return f;
+}
+
+// The plain filter is the all-Active mask (simp.cpp:892-896).
+DensityFilter make_density_filter(const Grid& g, double radius) {
+  return make_density_filter(g, radius, std::vector<std::uint8_t>(g.voxels(), kActive));
 }
 
 std::vector<double> DensityFilter::filter_density(const std::vector<double>& x) const {
@@ -1286,6 +1297,157 @@ SimpResult simp_optimize(const Grid& g, const SimpParams& p,
     }
   }
   res.volume_fraction = volfrac(res.physical_density);
+  return res;
+}
+
+std::vector<std::uint8_t> effective_design_mask(const Grid& g,
+                                                const std::vector<std::uint8_t>& mask) {
+  if (mask.size() != g.voxels())
+    throw std::invalid_argument("effective_design_mask: mask size != grid.voxel_count()");
+  std::vector<std::uint8_t> eff = mask;
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i)
+        if (!g.is_solid(i, j, k)) eff[g.index(i, j, k)] = kFrozenVoid;
+  return eff;
+}
+
+Grid analysis_grid_for_mask(const Grid& g, const std::vector<std::uint8_t>& eff) {
+  Grid a = g;
+  if (a.solid.empty()) a.solid.assign(g.voxels(), 1);
+  for (std::size_t e = 0; e < a.solid.size(); ++e)
+    if (eff[e] == kFrozenVoid) a.solid[e] = 0;
+  return a;
+}
+
+void apply_mask_pins(const std::vector<std::uint8_t>& eff, std::vector<double>& xphys) {
+  for (std::size_t e = 0; e < xphys.size(); ++e) {
+    if (eff[e] == kFrozenSolid) xphys[e] = 1.0;
+    else if (eff[e] == kFrozenVoid) xphys[e] = 0.0;
+  }
+}
+
+std::vector<double> oc_update_masked(const Grid& g, const DensityFilter& f,
+                                     const std::vector<std::uint8_t>& eff,
+                                     const std::vector<double>& x,
+                                     const std::vector<double>& dcomp, double volfrac,
+                                     double move, double density_min, int* nbis,
+                                     double* lambda_out) {
+  const std::size_t N = g.voxels();
+  if (x.size() != N || dcomp.size() != N || eff.size() != N)
+    throw std::invalid_argument("oc_update_masked: vector size != voxel_count");
+  const std::vector<double> dc = f.filter_sensitivity(dcomp);
+  std::vector<double> ones(N, 0.0);
+  std::size_t n_active = 0;
+  for (std::size_t e = 0; e < N; ++e)
+    if (eff[e] == kActive) { ones[e] = 1.0; ++n_active; }
+  const std::vector<double> dv = f.filter_sensitivity(ones);
+  const double target = volfrac * static_cast<double>(n_active);
+  auto candidate = [&](double lambda) {
+    std::vector<double> xn = x;  // frozen voxels keep their x
+    for (std::size_t e = 0; e < N; ++e) {
+      if (eff[e] != kActive) continue;
+      const double xe = x[e];
+      double be = 0.0;
+      if (dv[e] > 0.0) be = -dc[e] / (lambda * dv[e]);
+      if (be < 0.0) be = 0.0;
+      double xt = xe * std::sqrt(be);
+      const double lo = std::max(density_min, xe - move);
+      const double hi = std::min(1.0, xe + move);
+      if (xt < lo) xt = lo;
+      if (xt > hi) xt = hi;
+      xn[e] = xt;
+    }
+    return xn;
+  };
+  auto phys_vol = [&](const std::vector<double>& xn) {
+    const std::vector<double> xp = f.filter_density(xn);
+    double v = 0.0;
+    for (double q : xp) v += q;
+    return v;
+  };
+  double l1 = 0.0, l2 = 1.0;
+  while (l2 < 1e30 && phys_vol(candidate(l2)) > target) l2 *= 2.0;
+  int it = 0;
+  for (; it < 100 && (l2 - l1) > 1e-6 * (l1 + l2); ++it) {
+    const double lmid = 0.5 * (l1 + l2);
+    if (phys_vol(candidate(lmid)) > target) l1 = lmid; else l2 = lmid;
+  }
+  if (nbis) *nbis = it;
+  if (lambda_out) *lambda_out = 0.5 * (l1 + l2);
+  return candidate(0.5 * (l1 + l2));
+}
+
+SimpResult simp_optimize_masked(const Grid& g, const SimpParams& p,
+                                const std::vector<DirichletBC>& bcs,
+                                const std::vector<NodalLoad>& loads, const SimpOptions& o,
+                                const std::vector<std::uint8_t>& mask) {
+  if (!(o.volume_fraction > 0.0 && o.volume_fraction <= 1.0))
+    throw std::invalid_argument("simp_optimize: volume_fraction must be in (0, 1]");
+  if (!(o.move > 0.0)) throw std::invalid_argument("simp_optimize: move must be > 0");
+  if (o.max_iterations < 1) throw std::invalid_argument("simp_optimize: max_iterations must be >= 1");
+  if (mask.size() != g.voxels()) throw std::invalid_argument("simp_optimize: mask size != voxel_count");
+  const std::vector<std::uint8_t> eff = effective_design_mask(g, mask);
+  const DensityFilter f = make_density_filter(g, o.filter_radius, eff);
+  const Grid analysis = analysis_grid_for_mask(g, eff);
+  double n_active = 0.0;
+  for (std::uint8_t m : eff)
+    if (m == kActive) n_active += 1.0;
+  const double vf_target = o.volume_fraction;
+  auto active_volfrac = [&](const std::vector<double>& xp) {
+    double v = 0.0;
+    for (double q : xp) v += q;
+    return n_active > 0.0 ? v / n_active : 0.0;
+  };
+  std::vector<double> x(g.voxels(), 0.0);
+  for (std::size_t e = 0; e < x.size(); ++e) {
+    if (eff[e] == kActive) x[e] = vf_target;
+    else if (eff[e] == kFrozenSolid) x[e] = 1.0;
+  }
+  SimpResult res;
+  for (int it = 0; it < o.max_iterations; ++it) {
+    std::vector<double> xphys = f.filter_density(x);
+    apply_mask_pins(eff, xphys);
+    SimpParams tp = p;
+    if (!o.penalty_continuation.empty())
+      tp.penalty = penalty_at_iteration(o.penalty_continuation, res.iterations + 1);
+    SimpCompliance c;
+    try {
+      c = simp_compliance(analysis, tp, xphys, bcs, loads, o.cg_tolerance, o.cg_max_iterations,
+                          o.use_mg);
+    } catch (const SolverNonConvergence&) {
+      res.non_convergent = true;
+      break;
+    }
+    const std::vector<double> xn =
+        oc_update_masked(g, f, eff, x, c.dc, vf_target, o.move, p.density_min);
+    double change = 0.0;
+    for (std::size_t e = 0; e < x.size(); ++e) change = std::max(change, std::fabs(xn[e] - x[e]));
+    x = xn;
+    if (res.iterations == 0) res.initial_compliance = c.compliance;
+    ++res.iterations;
+    const std::vector<double> xa = f.filter_density(x);
+    res.history.push_back({c.compliance, change, active_volfrac(xa), c.cg.iterations});
+    if (change < o.change_tol) { res.converged = true; break; }
+  }
+  res.design = x;
+  res.physical_density = f.filter_density(x);
+  const std::vector<double> unpinned = res.physical_density;
+  apply_mask_pins(eff, res.physical_density);
+  if (res.non_convergent) {
+    res.compliance = res.history.empty() ? 0.0 : res.history.back().compliance;
+  } else {
+    try {
+      const SimpCompliance fc = simp_compliance(analysis, p, res.physical_density, bcs, loads,
+                                                o.cg_tolerance, o.cg_max_iterations, o.use_mg);
+      res.compliance = fc.compliance;
+    } catch (const SolverNonConvergence&) {
+      res.non_convergent = true;
+      res.converged = false;
+      res.compliance = res.history.empty() ? 0.0 : res.history.back().compliance;
+    }
+  }
+  res.volume_fraction = active_volfrac(unpinned);
   return res;
 }
 

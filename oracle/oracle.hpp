@@ -279,6 +279,37 @@ SimpResult simp_optimize(const Grid& g, const SimpParams& p,
                          const std::vector<DirichletBC>& bcs,
                          const std::vector<NodalLoad>& loads, const SimpOptions& o);
 
+// ---- Passive regions: the masked OC loop -----------------------------------
+// MaskValue (core/include/topopt/voxel.hpp:42-46): one byte per voxel.
+enum : std::uint8_t { kActive = 0, kFrozenSolid = 1, kFrozenVoid = 2 };
+// effective_design_mask (core/src/simp/simp.cpp:2610-2633): Empty voxels become
+// FrozenVoid. (Load/Fixture tags -> FrozenSolid: the oracle grid carries only
+// solid/empty, so that part is the caller's.)
+std::vector<std::uint8_t> effective_design_mask(const Grid& g, const std::vector<std::uint8_t>& mask);
+// analysis_grid_for_mask (simp.cpp:2640-2645): FrozenVoid voxels become Empty.
+Grid analysis_grid_for_mask(const Grid& g, const std::vector<std::uint8_t>& eff);
+// apply_mask_pins (simp.cpp:2649-2654): FrozenSolid -> 1, FrozenVoid -> 0.
+void apply_mask_pins(const std::vector<std::uint8_t>& eff, std::vector<double>& xphys);
+// make_density_filter(grid, radius, mask) (simp.cpp:834-890): neighbours and
+// rows restricted to solid Active voxels (no bleed across the mask boundary).
+DensityFilter make_density_filter(const Grid& g, double radius,
+                                  const std::vector<std::uint8_t>& mask);
+// oc_update_masked (simp.cpp:2662-2734): only Active voxels move, frozen ones
+// keep their x; budget volfrac * n_active on the masked filter's volume.
+std::vector<double> oc_update_masked(const Grid& g, const DensityFilter& f,
+                                     const std::vector<std::uint8_t>& eff,
+                                     const std::vector<double>& x,
+                                     const std::vector<double>& dcompliance, double volfrac,
+                                     double move, double density_min, int* bisections = nullptr,
+                                     double* lambda = nullptr);
+// simp_optimize(grid, params, bcs, loads, options, mask) (simp.cpp:2989-3689)
+// restricted like simp_optimize above: OC updater, one unprojected stage, no
+// SEMDOT / lattice coupling / adaptive move, cold solves.
+SimpResult simp_optimize_masked(const Grid& g, const SimpParams& p,
+                                const std::vector<DirichletBC>& bcs,
+                                const std::vector<NodalLoad>& loads, const SimpOptions& o,
+                                const std::vector<std::uint8_t>& mask);
+
 // ---- Heaviside projection (core/src/simp/simp.cpp:352-379) ---------------
 double heaviside_project(double rt, double beta, double eta);
 double heaviside_project_derivative(double rt, double beta, double eta);

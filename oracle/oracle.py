@@ -146,34 +146,64 @@ def simp_compliance(s, density, E0=1.0, nu=0.3, penalty=3.0, rho_min=1e-3, tol=1
     return c.value, dc, u, info.as_dict()
 
 
-def filter(s, radius, x, adjoint=False):
+# Design masks: one MaskValue byte per voxel (0 Active, 1 FrozenSolid,
+# 2 FrozenVoid; core/include/topopt/voxel.hpp:42-46). mask=None is the plain path.
+ACTIVE, FROZEN_SOLID, FROZEN_VOID = 0, 1, 2
+
+
+def _mask(s, mask):
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    if m.size != s.nvox:
+        raise ValueError("mask size != voxel count")
+    return m, m.ctypes.data_as(C.c_void_p)
+
+
+def filter(s, radius, x, adjoint=False, mask=None):
     x = np.ascontiguousarray(x, dtype=np.float64)
     out = np.zeros(s.nvox)
-    _check(lib().oracle_filter(C.byref(s.p), C.c_double(radius), _ptr(x), _ptr(out), int(adjoint)))
+    if mask is None:
+        _check(lib().oracle_filter(C.byref(s.p), C.c_double(radius), _ptr(x), _ptr(out),
+                                   int(adjoint)))
+    else:
+        m, mp = _mask(s, mask)
+        _check(lib().oracle_filter_masked(C.byref(s.p), C.c_double(radius), mp, _ptr(x), _ptr(out),
+                                          int(adjoint)))
     return out
 
 
-def filter_rows(s, radius):
+def filter_rows(s, radius, mask=None):
     nnz = np.zeros(s.nvox, dtype=np.int32)
     hs = np.zeros(s.nvox)
-    _check(lib().oracle_filter_rows(C.byref(s.p), C.c_double(radius), _ptr(nnz), _ptr(hs)))
+    if mask is None:
+        _check(lib().oracle_filter_rows(C.byref(s.p), C.c_double(radius), _ptr(nnz), _ptr(hs)))
+    else:
+        m, mp = _mask(s, mask)
+        _check(lib().oracle_filter_rows_masked(C.byref(s.p), C.c_double(radius), mp, _ptr(nnz),
+                                               _ptr(hs)))
     return nnz, hs
 
 
-def oc_update(s, radius, x, dc, volfrac, move=0.2, rho_min=1e-3):
+def oc_update(s, radius, x, dc, volfrac, move=0.2, rho_min=1e-3, mask=None):
     x = np.ascontiguousarray(x, dtype=np.float64)
     dc = np.ascontiguousarray(dc, dtype=np.float64)
     out = np.zeros(s.nvox)
     nb = C.c_int(0)
     lam = C.c_double(0)
-    _check(lib().oracle_oc_update(C.byref(s.p), C.c_double(radius), _ptr(x), _ptr(dc),
-                                  C.c_double(volfrac), C.c_double(move), C.c_double(rho_min),
-                                  _ptr(out), C.byref(nb), C.byref(lam)))
+    if mask is None:
+        _check(lib().oracle_oc_update(C.byref(s.p), C.c_double(radius), _ptr(x), _ptr(dc),
+                                      C.c_double(volfrac), C.c_double(move), C.c_double(rho_min),
+                                      _ptr(out), C.byref(nb), C.byref(lam)))
+    else:
+        m, mp = _mask(s, mask)
+        _check(lib().oracle_oc_update_masked(C.byref(s.p), C.c_double(radius), mp, _ptr(x),
+                                             _ptr(dc), C.c_double(volfrac), C.c_double(move),
+                                             C.c_double(rho_min), _ptr(out), C.byref(nb),
+                                             C.byref(lam)))
     return out, nb.value, lam.value
 
 
 def optimize(s, volfrac, radius, iterations, E0=1.0, nu=0.3, penalty=3.0, rho_min=1e-3,
-             move=0.2, change_tol=0.0, cg_tol=1e-8, use_mg=True, continuation=None):
+             move=0.2, change_tol=0.0, cg_tol=1e-8, use_mg=True, continuation=None, mask=None):
     design = np.zeros(s.nvox)
     xphys = np.zeros(s.nvox)
     hist = np.zeros(4 * iterations)
@@ -181,12 +211,16 @@ def optimize(s, volfrac, radius, iterations, E0=1.0, nu=0.3, penalty=3.0, rho_mi
     comp = C.c_double(0)
     vf = C.c_double(0)
     p0, p1, step, per = continuation if continuation else (0.0, 0.0, 0.0, 0)
-    _check(lib().oracle_optimize(C.byref(s.p), C.c_double(E0), C.c_double(nu), C.c_double(penalty),
-                                 C.c_double(rho_min), C.c_double(volfrac), C.c_double(radius),
-                                 C.c_double(move), int(iterations), C.c_double(change_tol),
-                                 C.c_double(cg_tol), int(use_mg), C.c_double(p0), C.c_double(p1),
-                                 C.c_double(step), int(per), _ptr(design), _ptr(xphys), _ptr(hist),
-                                 C.byref(it), C.byref(comp), C.byref(vf)))
+    args = (C.c_double(E0), C.c_double(nu), C.c_double(penalty), C.c_double(rho_min),
+            C.c_double(volfrac), C.c_double(radius), C.c_double(move), int(iterations),
+            C.c_double(change_tol), C.c_double(cg_tol), int(use_mg), C.c_double(p0),
+            C.c_double(p1), C.c_double(step), int(per), _ptr(design), _ptr(xphys), _ptr(hist),
+            C.byref(it), C.byref(comp), C.byref(vf))
+    if mask is None:
+        _check(lib().oracle_optimize(C.byref(s.p), *args))
+    else:
+        m, mp = _mask(s, mask)
+        _check(lib().oracle_optimize_masked(C.byref(s.p), mp, *args))
     n = it.value
     return {"design": design, "xphys": xphys, "history": hist[: 4 * n].reshape(n, 4),
             "iterations": n, "compliance": comp.value, "volume_fraction": vf.value}

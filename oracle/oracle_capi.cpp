@@ -232,6 +232,87 @@ int oracle_optimize(const OracleProblem* p, double E0, double nu, double penalty
   });
 }
 
+// ---- masked (passive-region) variants: mask = one MaskValue byte per voxel ----
+int oracle_filter_masked(const OracleProblem* p, double radius, const uint8_t* mask,
+                         const double* x, double* out, int adjoint) {
+  return guarded([&] {
+    const oracle::Grid g = grid_of(p);
+    const std::vector<uint8_t> eff =
+        oracle::effective_design_mask(g, std::vector<uint8_t>(mask, mask + g.voxels()));
+    const oracle::DensityFilter f = oracle::make_density_filter(g, radius, eff);
+    std::vector<double> xv(x, x + g.voxels());
+    const std::vector<double> r = adjoint ? f.filter_sensitivity(xv) : f.filter_density(xv);
+    std::memcpy(out, r.data(), r.size() * sizeof(double));
+  });
+}
+
+int oracle_filter_rows_masked(const OracleProblem* p, double radius, const uint8_t* mask,
+                              int* nnz, double* hs) {
+  return guarded([&] {
+    const oracle::Grid g = grid_of(p);
+    const std::vector<uint8_t> eff =
+        oracle::effective_design_mask(g, std::vector<uint8_t>(mask, mask + g.voxels()));
+    const oracle::DensityFilter f = oracle::make_density_filter(g, radius, eff);
+    for (std::size_t e = 0; e < f.voxel_count; ++e) {
+      nnz[e] = static_cast<int>(f.row_start[e + 1] - f.row_start[e]);
+      hs[e] = f.weight_sum[e];
+    }
+  });
+}
+
+int oracle_oc_update_masked(const OracleProblem* p, double radius, const uint8_t* mask,
+                            const double* x, const double* dc, double volfrac, double move,
+                            double rho_min, double* x_new, int* bisections, double* lambda) {
+  return guarded([&] {
+    const oracle::Grid g = grid_of(p);
+    const std::vector<uint8_t> eff =
+        oracle::effective_design_mask(g, std::vector<uint8_t>(mask, mask + g.voxels()));
+    const oracle::DensityFilter f = oracle::make_density_filter(g, radius, eff);
+    std::vector<double> xv(x, x + g.voxels()), dv(dc, dc + g.voxels());
+    const std::vector<double> r = oracle::oc_update_masked(g, f, eff, xv, dv, volfrac, move,
+                                                           rho_min, bisections, lambda);
+    std::memcpy(x_new, r.data(), r.size() * sizeof(double));
+  });
+}
+
+// Masked OC loop; arguments as oracle_optimize plus the mask.
+int oracle_optimize_masked(const OracleProblem* p, const uint8_t* mask, double E0, double nu,
+                           double penalty, double rho_min, double volfrac, double radius,
+                           double move, int max_iterations, double change_tol, double cg_tol,
+                           int use_mg, double pc_p0, double pc_p1, double pc_step,
+                           int pc_per_stage, double* design, double* xphys, double* hist,
+                           int* iterations, double* compliance, double* final_volfrac) {
+  return guarded([&] {
+    const oracle::Grid g = grid_of(p);
+    oracle::SimpParams sp{E0, nu, penalty, rho_min};
+    oracle::SimpOptions o;
+    o.volume_fraction = volfrac;
+    o.filter_radius = radius;
+    o.move = move;
+    o.max_iterations = max_iterations;
+    o.change_tol = change_tol;
+    o.cg_tolerance = cg_tol;
+    o.use_mg = use_mg != 0;
+    if (pc_per_stage > 0)
+      o.penalty_continuation = oracle::penalty_continuation_ramp(pc_p0, pc_p1, pc_step, pc_per_stage);
+    const oracle::SimpResult r = oracle::simp_optimize_masked(
+        g, sp, bcs_of(p), loads_of(p), o, std::vector<uint8_t>(mask, mask + g.voxels()));
+    if (design) std::memcpy(design, r.design.data(), r.design.size() * sizeof(double));
+    if (xphys)
+      std::memcpy(xphys, r.physical_density.data(), r.physical_density.size() * sizeof(double));
+    for (std::size_t i = 0; i < r.history.size(); ++i) {
+      hist[4 * i + 0] = r.history[i].compliance;
+      hist[4 * i + 1] = r.history[i].change;
+      hist[4 * i + 2] = r.history[i].volume_fraction;
+      hist[4 * i + 3] = r.history[i].cg_iterations;
+    }
+    *iterations = r.iterations;
+    *compliance = r.compliance;
+    *final_volfrac = r.volume_fraction;
+    if (r.non_convergent) throw oracle::SolverNonConvergence("oracle_optimize: non-convergent", 0, 0);
+  });
+}
+
 // Self-weight load vector (full, 3*nodes) for the given grid.
 int oracle_self_weight(const OracleProblem* p, double density, double gravity, double dx,
                        double dy, double dz, double* f) {

@@ -1,0 +1,163 @@
+"""Oracle pins for the masked (passive-region) design loop.
+
+The oracle restates simp_optimize(grid, params, bcs, loads, options, mask)
+(core/src/simp/simp.cpp:2989-3689) with the OC updater: the effective mask
+(:2610-2633), the Active-only filter (make_density_filter with mask, :834-890),
+the pins (apply_mask_pins, :2649-2654), oc_update_masked (:2662-2734) and the
+analysis grid with FrozenVoid removed (:2640-2645). These checks pin it
+against what is fixed independently of its code:
+  * an all-Active mask reproduces the plain loop bit for bit (the reference
+    states this equivalence, simp.cpp:845 and :892-896, and the plain oracle is
+    pinned in test_oracle.py);
+  * the masked filter against its definition, evaluated by brute force;
+  * no bleed: frozen voxels' x never reaches an Active voxel's density;
+  * the OC step keeps frozen x, respects the box and move limits and meets the
+    Active budget;
+  * the loop's pins and its use of the analysis grid (the reference's own
+    test_simp.cpp:820-836 and :985-1009 check the pins).
+"""
+import numpy as np
+
+import oracle
+from topopt_amd import problems
+
+A, FS, FV = oracle.ACTIVE, oracle.FROZEN_SOLID, oracle.FROZEN_VOID
+
+
+def masked_filter_brute(nx, ny, nz, solid, mask, r, x):
+    active = (solid != 0) & (mask == A)
+    R = int(np.ceil(r))
+    out = np.zeros(nx * ny * nz)
+    hs = np.zeros(nx * ny * nz)
+    nnz = np.zeros(nx * ny * nz, dtype=np.int64)
+    idx = lambda i, j, k: (k * ny + j) * nx + i
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                e = idx(i, j, k)
+                if not active[e]:
+                    continue
+                acc = 0.0
+                for dk in range(-R, R + 1):
+                    for dj in range(-R, R + 1):
+                        for di in range(-R, R + 1):
+                            ii, jj, kk = i + di, j + dj, k + dk
+                            if not (0 <= ii < nx and 0 <= jj < ny and 0 <= kk < nz):
+                                continue
+                            f = idx(ii, jj, kk)
+                            if not active[f]:
+                                continue
+                            w = r - np.sqrt(di * di + dj * dj + dk * dk)
+                            if w <= 0:
+                                continue
+                            acc += w * x[f]
+                            hs[e] += w
+                            nnz[e] += 1
+                out[e] = acc / hs[e]
+    return out, nnz, hs
+
+
+def test_masked_filter_matches_definition_and_does_not_bleed():
+    nx, ny, nz = 7, 5, 4
+    rng = np.random.default_rng(2)
+    solid = (rng.uniform(size=nx * ny * nz) > 0.15).astype(np.uint8)
+    mask = rng.choice([A, A, A, FS, FV], size=nx * ny * nz).astype(np.uint8)
+    x = rng.uniform(0, 1, nx * ny * nz)
+    s = oracle.Setup(nx, ny, nz, solid=solid)
+    for r in (1.2, 1.5, 2.0):
+        ref, nnz_ref, hs_ref = masked_filter_brute(nx, ny, nz, solid, mask, r, x)
+        out = oracle.filter(s, r, x, mask=mask)
+        nnz, hs = oracle.filter_rows(s, r, mask=mask)
+        assert np.array_equal(nnz, nnz_ref)
+        assert np.allclose(hs, hs_ref, rtol=1e-15, atol=0)
+        assert np.allclose(out, ref, rtol=1e-14, atol=0)
+        # no bleed: the frozen voxels' x does not reach any output
+        x2 = x.copy()
+        frozen = mask != A
+        x2[frozen] = rng.uniform(0, 1, frozen.sum())
+        assert np.array_equal(oracle.filter(s, r, x2, mask=mask), out)
+
+
+def test_all_active_mask_is_the_plain_loop():
+    nx, ny, nz = 12, 4, 4
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    rng = np.random.default_rng(4)
+    solid = np.ones(nx * ny * nz, dtype=np.uint8)
+    solid[rng.choice(nx * ny * nz, 6, replace=False)] = 0
+    solid[:ny * nx] = 1  # keep the clamped face attached
+    s = oracle.Setup(nx, ny, nz, solid=solid, bcs=bcs, loads=loads)
+    allA = np.zeros(nx * ny * nz, dtype=np.uint8)
+    x = rng.uniform(0.1, 1, nx * ny * nz) * solid
+    dc = -rng.uniform(0, 3, nx * ny * nz)
+    assert np.array_equal(oracle.filter(s, 1.5, x), oracle.filter(s, 1.5, x, mask=allA))
+    assert np.array_equal(oracle.filter(s, 1.5, dc, adjoint=True),
+                          oracle.filter(s, 1.5, dc, adjoint=True, mask=allA))
+    xp, nbp, lp = oracle.oc_update(s, 1.5, x, dc, 0.3)
+    xm, nbm, lm = oracle.oc_update(s, 1.5, x, dc, 0.3, mask=allA)
+    assert (nbp, lp) == (nbm, lm) and np.array_equal(xp, xm)
+    plain = oracle.optimize(s, 0.3, 1.5, 4, cg_tol=1e-8)
+    masked = oracle.optimize(s, 0.3, 1.5, 4, cg_tol=1e-8, mask=allA)
+    assert np.array_equal(plain["history"], masked["history"])
+    assert np.array_equal(plain["design"], masked["design"])
+    assert np.array_equal(plain["xphys"], masked["xphys"])
+    assert plain["compliance"] == masked["compliance"]
+
+
+def test_masked_oc_step_keeps_frozen_and_meets_budget():
+    nx, ny, nz = 10, 6, 4
+    rng = np.random.default_rng(9)
+    n = nx * ny * nz
+    mask = rng.choice([A, A, A, FS, FV], size=n).astype(np.uint8)
+    s = oracle.Setup(nx, ny, nz)
+    # Active x near the budget so that it is reachable within the move limit
+    x = np.where(mask == A, rng.uniform(0.2, 0.5, n), np.where(mask == FS, 1.0, 0.0))
+    dc = -rng.uniform(0, 3, n)
+    vf, move = 0.35, 0.2
+    xn, nb, lam = oracle.oc_update(s, 1.5, x, dc, vf, move=move, mask=mask)
+    act = mask == A
+    assert np.array_equal(xn[~act], x[~act])
+    assert np.all(xn[act] >= np.maximum(1e-3, x[act] - move) - 1e-15)
+    assert np.all(xn[act] <= np.minimum(1.0, x[act] + move) + 1e-15)
+    vol = oracle.filter(s, 1.5, xn, mask=mask).sum()
+    assert abs(vol - vf * act.sum()) < 1e-3 * vf * act.sum()
+    assert 0 < nb <= 100 and lam > 0
+
+
+def frozen_box_problem(nx, ny, nz):
+    """Cantilever with a FrozenSolid pad under the load and a FrozenVoid
+    channel through the middle (the analysis grid drops the channel)."""
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    n = nx * ny * nz
+    mask = np.zeros(n, dtype=np.uint8)
+    idx = lambda i, j, k: (k * ny + j) * nx + i
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                if i >= nx - 2 and k <= 1:
+                    mask[idx(i, j, k)] = FS
+                elif nx // 3 <= i < nx // 3 + 3 and nz // 3 <= k < nz - nz // 3 and 1 <= j < ny - 1:
+                    mask[idx(i, j, k)] = FV
+    analysis = (mask != FV).astype(np.uint8)
+    return bcs, loads, mask, analysis
+
+
+def test_masked_loop_pins_and_analysis_grid():
+    nx, ny, nz = 16, 6, 6
+    bcs, loads, mask, analysis = frozen_box_problem(nx, ny, nz)
+    s = oracle.Setup(nx, ny, nz, bcs=bcs, loads=loads)
+    out = oracle.optimize(s, 0.3, 1.5, 5, cg_tol=1e-9, mask=mask)
+    assert out["iterations"] == 5
+    xp, d = out["xphys"], out["design"]
+    assert np.all(xp[mask == FS] == 1.0) and np.all(xp[mask == FV] == 0.0)
+    assert np.all(d[mask == FS] == 1.0) and np.all(d[mask == FV] == 0.0)
+    act = mask == A
+    vf_active = oracle.filter(s, 1.5, d, mask=mask)[act].sum() / act.sum()
+    assert abs(out["volume_fraction"] - vf_active) < 1e-14
+    assert abs(vf_active - 0.3) < 2e-3
+    # the final compliance is the solve on the analysis grid (FrozenVoid empty)
+    sa = oracle.Setup(nx, ny, nz, solid=analysis, bcs=bcs, loads=loads)
+    c, _, _, _ = oracle.simp_compliance(sa, xp, tol=1e-9)
+    assert abs(c - out["compliance"]) <= 1e-12 * c
+    # ... and not the solve with the channel filled at rho_min
+    c_full, _, _, _ = oracle.simp_compliance(s, xp, tol=1e-9)
+    assert abs(c_full - c) > 1e-9 * c

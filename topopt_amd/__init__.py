@@ -249,10 +249,25 @@ class Problem:
         _check(code, info)
         return c.value, dc, u, info.as_dict()
 
-    def filter(self, radius, x, adjoint=False):
+    def _mask(self, mask):
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        if m.size != self.nvox:
+            raise ValueError("mask size != voxel count")
+        return m
+
+    def filter(self, radius, x, adjoint=False, mask=None):
+        """filter_density (adjoint False) / filter_sensitivity; with a design
+        mask (MaskValue bytes: 0 Active, 1 FrozenSolid, 2 FrozenVoid) the
+        Active-only filter of make_density_filter(grid, radius, mask)."""
         x = _f64(x, self.nvox)
         out = np.zeros(self.nvox)
-        _check(lib().topohip_filter(self._h, C.c_double(radius), _p(x), _p(out), int(adjoint)))
+        if mask is None:
+            _check(lib().topohip_filter(self._h, C.c_double(radius), _p(x), _p(out),
+                                        int(adjoint)))
+        else:
+            m = self._mask(mask)
+            _check(lib().topohip_filter_masked(self._h, C.c_double(radius), _p(m), _p(x),
+                                               _p(out), int(adjoint)))
         return out
 
     def oc_update(self, radius, x, dc, volfrac, move=0.2, rho_min=1e-3):
@@ -268,7 +283,9 @@ class Problem:
         return out, act, nb.value, lam.value
 
     def optimize(self, volfrac, rmin, iterations, penalty=3.0, move=0.2, E0=1.0, rho_min=1e-3,
-                 change_tol=0.0, cg_tol=1e-8, cg_max_iterations=0, continuation=None):
+                 change_tol=0.0, cg_tol=1e-8, cg_max_iterations=0, continuation=None, mask=None):
+        """The OC design loop; with a design mask the masked loop (the handle
+        must be built on the analysis grid: FrozenVoid voxels empty)."""
         o = Options(volfrac, rmin, move, penalty, E0, rho_min, change_tol, cg_tol, int(iterations),
                     int(cg_max_iterations), 0.0, 0.0, 0.0, 0)
         if continuation:
@@ -277,8 +294,13 @@ class Problem:
         xphys = np.zeros(self.nvox)
         hist = np.zeros(4 * iterations)
         res = Result()
-        _check(lib().topohip_optimize(self._h, C.byref(o), _p(design), _p(xphys), _p(hist),
-                                      C.byref(res)))
+        if mask is None:
+            _check(lib().topohip_optimize(self._h, C.byref(o), _p(design), _p(xphys), _p(hist),
+                                          C.byref(res)))
+        else:
+            m = self._mask(mask)
+            _check(lib().topohip_optimize_masked(self._h, _p(m), C.byref(o), _p(design),
+                                                 _p(xphys), _p(hist), C.byref(res)))
         n = res.iterations
         out = res.as_dict()
         out.update(design=design, xphys=xphys, history=hist[: 4 * n].reshape(n, 4))

@@ -252,6 +252,38 @@ int topohip_optimize(topohip_problem* p, const topohip_options* opt, double* des
   });
 }
 
+int topohip_optimize_masked(topohip_problem* p, const uint8_t* mask, const topohip_options* opt,
+                            double* design_out, double* xphys_out, double* history,
+                            topohip_result* result) {
+  return guarded([&] {
+    need(p, "problem"); need(mask, "mask"); need(opt, "options");
+    optimize(*p->P, *opt, design_out, xphys_out, history, result, mask);
+  });
+}
+
+int topohip_filter_masked(topohip_problem* p, double radius, const uint8_t* mask,
+                          const double* x, double* out, int adjoint) {
+  return guarded([&] {
+    need(p, "problem"); need(mask, "mask"); need(x, "x"); need(out, "out");
+    Problem& P = *p->P;
+    std::vector<uint8_t> eff, dset;
+    long long n_design = 0;
+    design_mask_of(P, mask, eff, dset, n_design);
+    const long long nv = P.g.elems();
+    DBuf<uint8_t> d_dset;
+    d_dset.ensure(static_cast<size_t>(nv));
+    TOPOHIP_CHECK(hipMemcpyAsync(d_dset.p, dset.data(), nv, hipMemcpyHostToDevice, P.s));
+    std::unique_ptr<FilterCache> fc = make_filter(P, radius, d_dset.p);
+    DBuf<double> xi, xo;
+    xi.ensure(nv);
+    xo.ensure(nv);
+    TOPOHIP_CHECK(hipMemcpyAsync(xi.p, x, nv * sizeof(double), hipMemcpyHostToDevice, P.s));
+    launch_filter(P.g, fc->st, d_dset.p, fc->hs.p, xi.p, xo.p, adjoint ? 1 : 0, P.s);
+    TOPOHIP_CHECK(hipMemcpyAsync(out, xo.p, nv * sizeof(double), hipMemcpyDeviceToHost, P.s));
+    TOPOHIP_CHECK(hipStreamSynchronize(P.s));
+  });
+}
+
 int topohip_bench_prepare(topohip_problem* p, uint64_t seed) {
   return guarded([&] {
     need(p, "problem");

@@ -1235,10 +1235,10 @@ void solve_cases(Problem& P, int nc, double tol, int max_it, const double* d_gue
   if (info) std::copy(d.begin(), d.end(), info);
 }
 
-FilterCache& filter_for(Problem& P, double radius) {
+// Density filter of the given radius over the design set `dset` (device, one
+// byte per voxel, null = every voxel): stencil, Hs and dv = H^T(1/Hs).
+std::unique_ptr<FilterCache> make_filter(Problem& P, double radius, const uint8_t* dset) {
   if (!(radius > 0.0)) throw Error(TOPOHIP_EINVAL, "make_density_filter: radius must be > 0");
-  for (auto& fc : P.filters)
-    if (fc->radius == radius) return *fc;
   auto fc = std::make_unique<FilterCache>();
   fc->radius = radius;
   const int R = static_cast<int>(std::ceil(radius));
@@ -1266,7 +1266,7 @@ FilterCache& filter_for(Problem& P, double radius) {
   const long long nv = P.g.elems();
   fc->hs.ensure(static_cast<size_t>(nv));
   fc->dv.ensure(static_cast<size_t>(nv));
-  const uint8_t* sol = P.all_solid ? nullptr : P.solid.p;
+  const uint8_t* sol = dset;
   launch_filter_hs(P.g, fc->st, sol, fc->hs.p, P.s);
   // dv = H^T (ones / Hs) over design voxels (ones on non-design voxels are
   // ignored by the adjoint, which skips them).
@@ -1275,7 +1275,14 @@ FilterCache& filter_for(Problem& P, double radius) {
   launch_fill<double>(ones.p, 1.0, nv, P.s);
   launch_filter(P.g, fc->st, sol, fc->hs.p, ones.p, fc->dv.p, 1, P.s);
   sync(P);
-  P.filters.push_back(std::move(fc));
+  return fc;
+}
+
+FilterCache& filter_for(Problem& P, double radius) {
+  if (!(radius > 0.0)) throw Error(TOPOHIP_EINVAL, "make_density_filter: radius must be > 0");
+  for (auto& fc : P.filters)
+    if (fc->radius == radius) return *fc;
+  P.filters.push_back(make_filter(P, radius, P.all_solid ? nullptr : P.solid.p));
   return *P.filters.back();
 }
 
@@ -1318,20 +1325,24 @@ double simp_compliance_dev(Problem& P, const double* d_xphys, double E0, double 
   return c;
 }
 
-void oc_update_dev(Problem& P, double radius, const double* d_x, const double* d_dc,
-                   double volfrac, double move, double rho_min, double* d_xnew, uint8_t* d_active,
-                   int* bisections, double* lambda, double* change) {
+// One OC update over the design set dset (device bytes, null = every voxel)
+// with fc built on the same set: oc_update (keep = false: voxels off the set
+// get 0) or oc_update_masked (keep = true: they keep their x; budget
+// volfrac * n_design).
+void oc_update_core(Problem& P, FilterCache& fc, const uint8_t* dset, double n_design, bool keep,
+                    const double* d_x, const double* d_dc, double volfrac, double move,
+                    double rho_min, double* d_xnew, uint8_t* d_active, int* bisections,
+                    double* lambda, double* change) {
   if (!(volfrac > 0.0 && volfrac <= 1.0))
     throw Error(TOPOHIP_EINVAL, "oc_update: volume_fraction must be in (0, 1]");
   if (!(move > 0.0)) throw Error(TOPOHIP_EINVAL, "oc_update: move must be > 0");
   if (!(rho_min > 0.0 && rho_min <= 1.0))
     throw Error(TOPOHIP_EINVAL, "oc_update: density_min must be in (0, 1]");
-  FilterCache& fc = filter_for(P, radius);
   const long long nv = P.g.elems();
-  const uint8_t* sol = P.all_solid ? nullptr : P.solid.p;
+  const uint8_t* sol = dset;
   P.dcf.ensure(static_cast<size_t>(nv));
   launch_filter(P.g, fc.st, sol, fc.hs.p, d_dc, P.dcf.p, 1, P.s);
-  const double target = volfrac * static_cast<double>(P.n_solid);
+  const double target = volfrac * n_design;
   launch_oc_init(P.oc.p, target, P.s);
   const int ob = oc_blocks(nv);
   // The search takes at most ~100 bracket + 100 bisection trials.
@@ -1345,7 +1356,7 @@ void oc_update_dev(Problem& P, double radius, const double* d_x, const double* d
     if (P.oc_h->phase == 2) break;
   }
   launch_oc_final(d_x, P.dcf.p, fc.dv.p, sol, nv, move, rho_min, P.oc.p, d_xnew, d_active,
-                  P.partials.p, ob, P.s);
+                  P.partials.p, ob, P.s, keep ? 1 : 0);
   launch_max_partials(P.partials.p, ob, P.scal.p, P.s);
   d2h(P.scal_h, P.scal.p, 1, P.s);
   if (bisections) *bisections = P.oc_h->steps;
@@ -1353,8 +1364,38 @@ void oc_update_dev(Problem& P, double radius, const double* d_x, const double* d
   if (change) *change = P.scal_h[0];
 }
 
+void oc_update_dev(Problem& P, double radius, const double* d_x, const double* d_dc,
+                   double volfrac, double move, double rho_min, double* d_xnew, uint8_t* d_active,
+                   int* bisections, double* lambda, double* change) {
+  FilterCache& fc = filter_for(P, radius);
+  oc_update_core(P, fc, P.all_solid ? nullptr : P.solid.p, static_cast<double>(P.n_solid), false,
+                 d_x, d_dc, volfrac, move, rho_min, d_xnew, d_active, bisections, lambda, change);
+}
+
+// Host-side design mask of the masked loop: eff (effective_design_mask,
+// core/src/simp/simp.cpp:2610-2633: empty voxels -> FrozenVoid) and the design
+// set (solid Active voxels). The handle's grid is the analysis grid: a
+// FrozenVoid voxel must be empty in it (analysis_grid_for_mask, :2640-2645).
+void design_mask_of(const Problem& P, const uint8_t* mask, std::vector<uint8_t>& eff,
+                    std::vector<uint8_t>& dset, long long& n_design) {
+  const long long nv = P.g.elems();
+  eff.assign(static_cast<size_t>(nv), 0);
+  dset.assign(static_cast<size_t>(nv), 0);
+  n_design = 0;
+  for (long long e = 0; e < nv; ++e) {
+    const bool solid = P.all_solid || P.solid_h[e];
+    if (mask[e] > 2) throw Error(TOPOHIP_EINVAL, "simp_optimize: mask value must be 0, 1 or 2");
+    if (mask[e] == 2 && solid)
+      throw Error(TOPOHIP_EINVAL,
+                  "simp_optimize: a FrozenVoid voxel is solid in the problem grid (create the "
+                  "problem on the analysis grid: FrozenVoid voxels empty)");
+    eff[e] = solid ? mask[e] : 2;
+    if (eff[e] == 0) { dset[e] = 1; ++n_design; }
+  }
+}
+
 void optimize(Problem& P, const topohip_options& o, double* design_out, double* xphys_out,
-              double* history, topohip_result* res) {
+              double* history, topohip_result* res, const uint8_t* mask) {
   if (!(o.volfrac > 0.0 && o.volfrac <= 1.0))
     throw Error(TOPOHIP_EINVAL, "simp_optimize: volume_fraction must be in (0, 1]");
   if (!(o.move > 0.0)) throw Error(TOPOHIP_EINVAL, "simp_optimize: move must be > 0");
@@ -1379,25 +1420,48 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
   };
   const double t0 = now_ms();
   const long long nv = P.g.elems();
+  // Design set: the solid voxels (plain loop) or, with a design mask, the solid
+  // Active voxels (masked loop, core/src/simp/simp.cpp:2989-3689), whose
+  // filter admits only Active neighbours and whose OC update moves only them.
   const uint8_t* sol = P.all_solid ? nullptr : P.solid.p;
-  FilterCache& fc = filter_for(P, o.rmin);
+  FilterCache* fcp = nullptr;
+  std::unique_ptr<FilterCache> masked_fc;
+  DBuf<uint8_t> d_eff, d_dset;
+  double nd = static_cast<double>(P.n_solid);
+  std::vector<double> x0(static_cast<size_t>(nv), 0.0);
+  if (mask) {
+    std::vector<uint8_t> eff, dset;
+    long long n_design = 0;
+    design_mask_of(P, mask, eff, dset, n_design);
+    d_eff.ensure(eff.size());
+    d_dset.ensure(dset.size());
+    h2d(d_eff.p, eff.data(), eff.size(), P.s);
+    h2d(d_dset.p, dset.data(), dset.size(), P.s);
+    sol = d_dset.p;
+    nd = static_cast<double>(n_design);
+    masked_fc = make_filter(P, o.rmin, sol);
+    fcp = masked_fc.get();
+    // Active at the target fraction, FrozenSolid at 1, the rest 0 (:3073-3080)
+    for (long long e = 0; e < nv; ++e)
+      x0[e] = eff[e] == 0 ? o.volfrac : eff[e] == 1 ? 1.0 : 0.0;
+  } else {
+    fcp = &filter_for(P, o.rmin);
+    // x = volfrac on solid voxels (simp_uniform_density, simp.cpp:305-312)
+    for (long long e = 0; e < nv; ++e)
+      if (P.all_solid || P.solid_h[e]) x0[e] = o.volfrac;
+  }
+  FilterCache& fc = *fcp;
   P.xd.ensure(static_cast<size_t>(nv));
   P.xphys.ensure(static_cast<size_t>(nv));
   P.xnew.ensure(static_cast<size_t>(nv));
-  // x = volfrac on solid voxels (simp_uniform_density, simp.cpp:305-312)
-  {
-    std::vector<double> x0(static_cast<size_t>(nv), 0.0);
-    for (long long e = 0; e < nv; ++e)
-      if (P.all_solid || P.solid_h[e]) x0[e] = o.volfrac;
-    h2d(P.xd.p, x0.data(), static_cast<size_t>(nv), P.s);
-  }
+  h2d(P.xd.p, x0.data(), static_cast<size_t>(nv), P.s);
   topohip_result r{};
   double t_solve = 0, t_filter = 0, t_oc = 0, t_build = 0;
-  const double nd = static_cast<double>(P.n_solid);
   const int vb = dot_blocks(nv);
   for (int it = 0; it < o.max_iterations; ++it) {
     double ta = now_ms();
     launch_filter(P.g, fc.st, sol, fc.hs.p, P.xd.p, P.xphys.p, 0, P.s);
+    if (mask) launch_mask_pins(P.xphys.p, d_eff.p, nv, P.s);
     sync(P);
     t_filter += now_ms() - ta;
     ta = now_ms();
@@ -1415,8 +1479,8 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
     t_build += ci.t_mg_build_ms;
     ta = now_ms();
     double change = 0.0;
-    oc_update_dev(P, o.rmin, P.xd.p, P.dc.p, o.volfrac, o.move, o.rho_min, P.xnew.p, nullptr,
-                  nullptr, nullptr, &change);
+    oc_update_core(P, fc, sol, nd, mask != nullptr, P.xd.p, P.dc.p, o.volfrac, o.move, o.rho_min,
+                   P.xnew.p, nullptr, nullptr, nullptr, &change);
     std::swap(P.xd.p, P.xnew.p);
     t_oc += now_ms() - ta;
     if (r.iterations == 0) r.initial_compliance = c;
@@ -1441,6 +1505,12 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
     }
   }
   launch_filter(P.g, fc.st, sol, fc.hs.p, P.xd.p, P.xphys.p, 0, P.s);
+  // volume fraction of the unpinned physical density (active_volfrac)
+  launch_sum_vec<double>(P.xphys.p, nv, P.partials.p, vb, P.s);
+  launch_sum_partials(P.partials.p, vb, P.scal.p, P.s);
+  d2h(P.scal_h, P.scal.p, 1, P.s);
+  r.volume_fraction = nd > 0 ? P.scal_h[0] / nd : 0.0;
+  if (mask) launch_mask_pins(P.xphys.p, d_eff.p, nv, P.s);
   if (r.non_convergent) {
     r.compliance = history && r.iterations > 0 ? history[4 * (r.iterations - 1)] : 0.0;
   } else {
@@ -1456,10 +1526,6 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
       r.converged = 0;
     }
   }
-  launch_sum_vec<double>(P.xphys.p, nv, P.partials.p, vb, P.s);
-  launch_sum_partials(P.partials.p, vb, P.scal.p, P.s);
-  d2h(P.scal_h, P.scal.p, 1, P.s);
-  r.volume_fraction = nd > 0 ? P.scal_h[0] / nd : 0.0;
   if (design_out) d2h(design_out, P.xd.p, static_cast<size_t>(nv), P.s);
   if (xphys_out) d2h(xphys_out, P.xphys.p, static_cast<size_t>(nv), P.s);
   r.t_total_ms = now_ms() - t0;

@@ -139,8 +139,13 @@ double simp_compliance_dev(Problem& P, const double* d_xphys, double E0, double 
 void oc_update_dev(Problem& P, double radius, const double* d_x, const double* d_dc,
                    double volfrac, double move, double rho_min, double* d_xnew,
                    uint8_t* d_active, int* bisections, double* lambda, double* change);
+// mask (nullable): one MaskValue byte per voxel (0 Active, 1 FrozenSolid,
+// 2 FrozenVoid) for the masked loop; FrozenVoid voxels must be empty in P's grid.
 void optimize(Problem& P, const topohip_options& o, double* design_out, double* xphys_out,
-              double* history, topohip_result* res);
+              double* history, topohip_result* res, const uint8_t* mask = nullptr);
+std::unique_ptr<FilterCache> make_filter(Problem& P, double radius, const uint8_t* dset);
+void design_mask_of(const Problem& P, const uint8_t* mask, std::vector<uint8_t>& eff,
+                    std::vector<uint8_t>& dset, long long& n_design);
 void hex8_stiffness_host(double E, double nu, double h, double* K576);
 
 }  // namespace topohip

@@ -187,9 +187,14 @@ void launch_oc_trial(const double* x, const double* dcf, const double* dv, const
 void launch_oc_decide(OcState* st, const double* partials, int n, hipStream_t s);
 int oc_blocks(long long n);
 // Final candidate at lambda = (l1+l2)/2 + active set + change partials (max).
+// Voxels off the design set `solid` get 0 (keep = 0, plain oc_update) or keep
+// their x (keep = 1, oc_update_masked).
 void launch_oc_final(const double* x, const double* dcf, const double* dv, const uint8_t* solid,
                      long long n, double move, double rho_min, const OcState* st, double* xnew,
-                     uint8_t* active, double* change_partials, int nblocks, hipStream_t s);
+                     uint8_t* active, double* change_partials, int nblocks, hipStream_t s,
+                     int keep = 0);
+// Design-mask pins on a physical density: eff 1 (FrozenSolid) -> 1, 2 (FrozenVoid) -> 0.
+void launch_mask_pins(double* xphys, const uint8_t* eff, long long n, hipStream_t s);
 void launch_max_partials(const double* partials, int n, double* out, hipStream_t s);
 template <typename T>
 void launch_sum_vec(const T* v, long long n, double* partials, int nblocks, hipStream_t s);

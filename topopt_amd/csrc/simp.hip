@@ -249,15 +249,16 @@ __global__ void oc_final_k(const double* __restrict__ x, const double* __restric
                            const double* __restrict__ dv, const uint8_t* __restrict__ solid,
                            long long n, double move, double rho_min, const OcState* st,
                            double* __restrict__ xnew, uint8_t* __restrict__ active,
-                           double* __restrict__ partials) {
+                           double* __restrict__ partials, int keep) {
   const double lambda = 0.5 * (st->l1 + st->l2);
   double m = 0.0;
   for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
        e += static_cast<long long>(gridDim.x) * blockDim.x) {
-    if (solid && !solid[e]) {
-      xnew[e] = 0.0;
+    if (solid && !solid[e]) {  // plain: 0 off the design set; masked: x carried through
+      const double xo = keep ? x[e] : 0.0;
+      xnew[e] = xo;
       if (active) active[e] = 0;
-      m = fmax(m, fabs(0.0 - x[e]));
+      m = fmax(m, fabs(xo - x[e]));
       continue;
     }
     double xt;
@@ -332,9 +333,23 @@ void launch_oc_decide(OcState* st, const double* partials, int n, hipStream_t s)
 
 void launch_oc_final(const double* x, const double* dcf, const double* dv, const uint8_t* solid,
                      long long n, double move, double rho_min, const OcState* st, double* xnew,
-                     uint8_t* active, double* partials, int nblocks, hipStream_t s) {
+                     uint8_t* active, double* partials, int nblocks, hipStream_t s, int keep) {
   hipLaunchKernelGGL(oc_final_k, dim3(nblocks), dim3(RB), 0, s, x, dcf, dv, solid, n, move, rho_min,
-                     st, xnew, active, partials);
+                     st, xnew, active, partials, keep);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+
+// apply_mask_pins (core/src/simp/simp.cpp:2649-2654): FrozenSolid -> 1, FrozenVoid -> 0.
+__global__ void mask_pins_k(double* __restrict__ xphys, const uint8_t* __restrict__ eff,
+                            long long n) {
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    if (eff[e] == 1) xphys[e] = 1.0;
+    else if (eff[e] == 2) xphys[e] = 0.0;
+  }
+}
+void launch_mask_pins(double* xphys, const uint8_t* eff, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(mask_pins_k, dim3(grid_for(n, RB, 4096)), dim3(RB), 0, s, xphys, eff, n);
   TOPOHIP_CHECK(hipGetLastError());
 }
 
