@@ -156,6 +156,24 @@ typedef struct {
 int topohip_optimize(topohip_problem* p, const topohip_options* opt, double* design_out,
                      double* xphys_out, double* history, topohip_result* result);
 
+/* The design loop with Heaviside projection stages (SimpOptions::projection /
+ * projection_eta, core/include/topopt/simp.hpp:518-524; loop
+ * core/src/simp/simp.cpp:2018-2608 with build_stage_plan :1053-1068): stage s
+ * runs iterations[s] OC updates at sharpness beta[s] with move limit move[s]
+ * (opt->move and opt->max_iterations are not used). Each iteration projects
+ * the filtered design (project_solid, :1637-1645), solves, and takes the
+ * projected OC step (oc_update_projected, :1647-1716: sensitivities chained
+ * through the projection derivative before the filter transpose, unclamped dv,
+ * a non-finite ratio holds the voxel, volume on the projected candidate);
+ * change < change_tol ends a stage. The final physical density is projected
+ * at the last stage's beta. The reference schedule of its Gate V2 test is
+ * heaviside_continuation_schedule (:367-373). history needs
+ * 4 * sum(iterations) doubles. */
+int topohip_optimize_projected(topohip_problem* p, const topohip_options* opt, int n_stages,
+                               const double* beta, const double* move, const int* iterations,
+                               double eta, double* design_out, double* xphys_out, double* history,
+                               topohip_result* result);
+
 /* ---- passive regions: the masked design loop ------------------------------
  * mask: one byte per voxel, the reference's MaskValue (core/include/topopt/
  * voxel.hpp:42-46): 0 Active, 1 FrozenSolid, 2 FrozenVoid.

@@ -1239,6 +1239,86 @@ std::vector<PenaltyStage> penalty_continuation_ramp(double p0, double p1, double
   return out;
 }
 
+std::vector<ProjStage> heaviside_continuation_schedule() {
+  return {{1.0, 0.2, 50}, {2.0, 0.2, 50}, {4.0, 0.2, 50},
+          {8.0, 0.2, 50}, {16.0, 0.1, 50}, {32.0, 0.05, 50}};
+}
+
+namespace {
+// project_solid (simp.cpp:1637-1645): design voxels only.
+void project_solid(const Grid& g, std::vector<double>& x, double beta, double eta) {
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i)
+        if (g.is_solid(i, j, k)) {
+          const std::size_t e = g.index(i, j, k);
+          x[e] = heaviside_project(x[e], beta, eta);
+        }
+}
+}  // namespace
+
+std::vector<double> oc_update_projected(const Grid& g, const DensityFilter& f,
+                                        const std::vector<double>& x,
+                                        const std::vector<double>& xtilde,
+                                        const std::vector<double>& dcomp, double beta,
+                                        double eta, double volfrac, double move,
+                                        double density_min, int* nbis, double* lambda_out) {
+  const std::size_t N = g.voxels();
+  std::vector<double> dc_phys(N, 0.0), dproj(N, 0.0);
+  std::size_t n_design = 0;
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) {
+        if (!g.is_solid(i, j, k)) continue;
+        const std::size_t e = g.index(i, j, k);
+        dproj[e] = heaviside_project_derivative(xtilde[e], beta, eta);
+        dc_phys[e] = dcomp[e] * dproj[e];
+        ++n_design;
+      }
+  const std::vector<double> dc = f.filter_sensitivity(dc_phys);
+  const std::vector<double> dv = f.filter_sensitivity(dproj);
+  const double target = volfrac * static_cast<double>(n_design);
+  auto candidate = [&](double lambda) {
+    std::vector<double> xn(N, 0.0);
+    for (int k = 0; k < g.nz; ++k)
+      for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i < g.nx; ++i) {
+          if (!g.is_solid(i, j, k)) continue;
+          const std::size_t e = g.index(i, j, k);
+          const double xe = x[e];
+          double be = -dc[e] / (lambda * dv[e]);
+          if (!std::isfinite(be)) be = 1.0;
+          if (be < 0.0) be = 0.0;
+          double xt = xe * std::sqrt(be);
+          const double lo = std::max(density_min, xe - move);
+          const double hi = std::min(1.0, xe + move);
+          if (xt < lo) xt = lo;
+          if (xt > hi) xt = hi;
+          xn[e] = xt;
+        }
+    return xn;
+  };
+  auto proj_vol = [&](const std::vector<double>& xn) {
+    const std::vector<double> xp = f.filter_density(xn);
+    double v = 0.0;
+    for (int k = 0; k < g.nz; ++k)
+      for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i < g.nx; ++i)
+          if (g.is_solid(i, j, k)) v += heaviside_project(xp[g.index(i, j, k)], beta, eta);
+    return v;
+  };
+  double l1 = 0.0, l2 = 1.0;
+  while (l2 < 1e30 && proj_vol(candidate(l2)) > target) l2 *= 2.0;
+  int it = 0;
+  for (; it < 100 && (l2 - l1) > 1e-6 * (l1 + l2); ++it) {
+    const double lmid = 0.5 * (l1 + l2);
+    if (proj_vol(candidate(lmid)) > target) l1 = lmid; else l2 = lmid;
+  }
+  if (nbis) *nbis = it;
+  if (lambda_out) *lambda_out = 0.5 * (l1 + l2);
+  return candidate(0.5 * (l1 + l2));
+}
+
 SimpResult simp_optimize(const Grid& g, const SimpParams& p,
                          const std::vector<DirichletBC>& bcs,
                          const std::vector<NodalLoad>& loads, const SimpOptions& o) {
@@ -1253,37 +1333,61 @@ SimpResult simp_optimize(const Grid& g, const SimpParams& p,
     for (double q : xp) v += q;
     return n_design > 0.0 ? v / n_design : 0.0;
   };
+  // Stage plan (build_stage_plan, simp.cpp:1053-1068).
+  struct Stage { bool project; double beta, move; int iterations; };
+  std::vector<Stage> plan;
+  if (o.projection.empty()) plan.push_back({false, 0.0, o.move, o.max_iterations});
+  else
+    for (const ProjStage& ps : o.projection) plan.push_back({true, ps.beta, ps.move, ps.iterations});
+  const double eta = o.projection_eta;
   std::vector<double> x(g.voxels(), 0.0);
   for (int k = 0; k < g.nz; ++k)
     for (int j = 0; j < g.ny; ++j)
       for (int i = 0; i < g.nx; ++i)
         if (g.is_solid(i, j, k)) x[g.index(i, j, k)] = o.volume_fraction;
   SimpResult res;
-  for (int it = 0; it < o.max_iterations; ++it) {
-    const std::vector<double> xphys = f.filter_density(x);
-    SimpParams tp = p;
-    if (!o.penalty_continuation.empty())
-      tp.penalty = penalty_at_iteration(o.penalty_continuation, res.iterations + 1);
-    SimpCompliance c;
-    try {
-      c = simp_compliance(g, tp, xphys, bcs, loads, o.cg_tolerance, o.cg_max_iterations, o.use_mg);
-    } catch (const SolverNonConvergence&) {
-      res.non_convergent = true;
-      break;
+  for (const Stage& st : plan) {
+    bool stage_converged = false;
+    for (int it = 0; it < st.iterations; ++it) {
+      std::vector<double> xphys = f.filter_density(x);
+      if (st.project) project_solid(g, xphys, st.beta, eta);
+      SimpParams tp = p;
+      if (!o.penalty_continuation.empty())
+        tp.penalty = penalty_at_iteration(o.penalty_continuation, res.iterations + 1);
+      SimpCompliance c;
+      try {
+        c = simp_compliance(g, tp, xphys, bcs, loads, o.cg_tolerance, o.cg_max_iterations,
+                            o.use_mg);
+      } catch (const SolverNonConvergence&) {
+        res.non_convergent = true;
+        break;
+      }
+      std::vector<double> xn;
+      if (st.project) {
+        const std::vector<double> xtilde = f.filter_density(x);
+        xn = oc_update_projected(g, f, x, xtilde, c.dc, st.beta, eta, o.volume_fraction, st.move,
+                                 p.density_min);
+      } else {
+        xn = oc_update(g, f, x, c.dc, o.volume_fraction, st.move, p.density_min);
+      }
+      double change = 0.0;
+      for (std::size_t e = 0; e < x.size(); ++e) change = std::max(change, std::fabs(xn[e] - x[e]));
+      x = xn;
+      if (res.iterations == 0) res.initial_compliance = c.compliance;
+      ++res.iterations;
+      std::vector<double> xa = f.filter_density(x);
+      if (st.project) project_solid(g, xa, st.beta, eta);
+      res.history.push_back({c.compliance, change, volfrac(xa), c.cg.iterations});
+      if (change < o.change_tol) { stage_converged = true; break; }
     }
-    const std::vector<double> xn = oc_update(g, f, x, c.dc, o.volume_fraction, o.move, p.density_min);
-    double change = 0.0;
-    for (std::size_t e = 0; e < x.size(); ++e) change = std::max(change, std::fabs(xn[e] - x[e]));
-    x = xn;
-    if (res.iterations == 0) res.initial_compliance = c.compliance;
-    ++res.iterations;
-    const std::vector<double> xa = f.filter_density(x);
-    res.history.push_back({c.compliance, change, volfrac(xa), c.cg.iterations});
-    if (change < o.change_tol) { res.converged = true; break; }
+    res.converged = stage_converged;
+    if (res.non_convergent) break;
   }
   res.design = x;
   res.physical_density = f.filter_density(x);
+  if (plan.back().project) project_solid(g, res.physical_density, plan.back().beta, eta);
   if (res.non_convergent) {
+    res.converged = false;
     res.compliance = res.history.empty() ? 0.0 : res.history.back().compliance;
   } else {
     try {

@@ -25,6 +25,12 @@
 //                       projection_unit_reference spot values (1e-12)
 //   simp_optimize       structural gates (monotone compliance, volume,
 //                       xPhys = filter(design))
+//   masked loop         (tests/test_oracle_masked.py) all-Active mask equals
+//                       the plain loop bit for bit; masked filter against a
+//                       brute-force evaluation of its definition; no bleed of
+//                       frozen x; OC keeps frozen x, box/move limits, Active
+//                       budget; pins (FrozenSolid 1 / FrozenVoid 0) and the
+//                       final solve on the analysis grid
 // Parity unpinned (no reference-held vector exists and the reference cannot
 // be built here without Eigen): the reference's exact MG-PCG iteration counts
 // and CgInfo traces, the SimplicialLDLT rounding of its coarsest solve, and its
@@ -254,6 +260,11 @@ double penalty_at_iteration(const std::vector<PenaltyStage>& s, int iteration);
 std::vector<PenaltyStage> penalty_continuation_ramp(double p0, double p1, double step,
                                                     int per_stage);
 
+// One beta-continuation stage (ProjectionStage, core/include/topopt/simp.hpp:518-524).
+struct ProjStage { double beta; double move; int iterations; };
+// heaviside_continuation_schedule (core/src/simp/simp.cpp:367-373).
+std::vector<ProjStage> heaviside_continuation_schedule();
+
 struct SimpOptions {
   double volume_fraction = 0.5;
   double filter_radius = 1.5;
@@ -264,6 +275,11 @@ struct SimpOptions {
   int cg_max_iterations = 0;
   bool use_mg = true;
   std::vector<PenaltyStage> penalty_continuation;
+  // Projected stages (SimpOptions::projection / projection_eta): when set, the
+  // loop runs them in order (build_stage_plan, simp.cpp:1053-1068) instead of
+  // one unprojected stage of max_iterations at `move`.
+  std::vector<ProjStage> projection;
+  double projection_eta = 0.5;
 };
 struct SimpIteration { double compliance, change, volume_fraction; int cg_iterations; };
 struct SimpResult {
@@ -313,5 +329,16 @@ SimpResult simp_optimize_masked(const Grid& g, const SimpParams& p,
 // ---- Heaviside projection (core/src/simp/simp.cpp:352-379) ---------------
 double heaviside_project(double rt, double beta, double eta);
 double heaviside_project_derivative(double rt, double beta, double eta);
+// oc_update_projected (simp.cpp:1647-1716): sensitivities chained through the
+// projection derivative before the filter transpose, UNCLAMPED dv in the OC
+// ratio (a non-finite ratio holds the voxel), volume on the projected
+// candidate. xtilde = filter_density(x).
+std::vector<double> oc_update_projected(const Grid& g, const DensityFilter& f,
+                                        const std::vector<double>& x,
+                                        const std::vector<double>& xtilde,
+                                        const std::vector<double>& dcompliance, double beta,
+                                        double eta, double volfrac, double move,
+                                        double density_min, int* bisections = nullptr,
+                                        double* lambda = nullptr);
 
 }  // namespace oracle

@@ -226,6 +226,35 @@ def optimize(s, volfrac, radius, iterations, E0=1.0, nu=0.3, penalty=3.0, rho_mi
             "iterations": n, "compliance": comp.value, "volume_fraction": vf.value}
 
 
+# heaviside_continuation_schedule (core/src/simp/simp.cpp:367-373): (beta, move, iterations)
+GATE_V2_SCHEDULE = [(1.0, 0.2, 50), (2.0, 0.2, 50), (4.0, 0.2, 50), (8.0, 0.2, 50),
+                    (16.0, 0.1, 50), (32.0, 0.05, 50)]
+
+
+def optimize_projected(s, volfrac, radius, stages, eta=0.5, E0=1.0, nu=0.3, penalty=3.0,
+                       rho_min=1e-3, change_tol=0.0, cg_tol=1e-8, use_mg=True):
+    """The OC loop with projected stages [(beta, move, iterations), ...]."""
+    n_it = sum(int(st[2]) for st in stages)
+    betas = np.array([st[0] for st in stages], dtype=np.float64)
+    moves = np.array([st[1] for st in stages], dtype=np.float64)
+    iters = np.array([st[2] for st in stages], dtype=np.int32)
+    design = np.zeros(s.nvox)
+    xphys = np.zeros(s.nvox)
+    hist = np.zeros(4 * n_it)
+    it = C.c_int(0)
+    comp = C.c_double(0)
+    vf = C.c_double(0)
+    _check(lib().oracle_optimize_projected(
+        C.byref(s.p), C.c_double(E0), C.c_double(nu), C.c_double(penalty), C.c_double(rho_min),
+        C.c_double(volfrac), C.c_double(radius), len(stages), _ptr(betas), _ptr(moves),
+        iters.ctypes.data_as(C.c_void_p), C.c_double(eta), C.c_double(change_tol),
+        C.c_double(cg_tol), int(use_mg), _ptr(design), _ptr(xphys), _ptr(hist), C.byref(it),
+        C.byref(comp), C.byref(vf)))
+    n = it.value
+    return {"design": design, "xphys": xphys, "history": hist[: 4 * n].reshape(n, 4),
+            "iterations": n, "compliance": comp.value, "volume_fraction": vf.value}
+
+
 def self_weight(s, density=1.0, gravity=1.0, direction=(0.0, 0.0, -1.0)):
     f = np.zeros(s.ndof)
     _check(lib().oracle_self_weight(C.byref(s.p), C.c_double(density), C.c_double(gravity),

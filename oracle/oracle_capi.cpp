@@ -232,6 +232,42 @@ int oracle_optimize(const OracleProblem* p, double E0, double nu, double penalty
   });
 }
 
+// Projected (Heaviside beta-continuation) OC loop: n_stages stages of
+// (beta, move, iterations), threshold eta; other arguments as oracle_optimize.
+int oracle_optimize_projected(const OracleProblem* p, double E0, double nu, double penalty,
+                              double rho_min, double volfrac, double radius, int n_stages,
+                              const double* betas, const double* moves, const int* iters,
+                              double eta, double change_tol, double cg_tol, int use_mg,
+                              double* design, double* xphys, double* hist, int* iterations,
+                              double* compliance, double* final_volfrac) {
+  return guarded([&] {
+    const oracle::Grid g = grid_of(p);
+    oracle::SimpParams sp{E0, nu, penalty, rho_min};
+    oracle::SimpOptions o;
+    o.volume_fraction = volfrac;
+    o.filter_radius = radius;
+    o.change_tol = change_tol;
+    o.cg_tolerance = cg_tol;
+    o.use_mg = use_mg != 0;
+    o.projection_eta = eta;
+    for (int s = 0; s < n_stages; ++s) o.projection.push_back({betas[s], moves[s], iters[s]});
+    const oracle::SimpResult r = oracle::simp_optimize(g, sp, bcs_of(p), loads_of(p), o);
+    if (design) std::memcpy(design, r.design.data(), r.design.size() * sizeof(double));
+    if (xphys)
+      std::memcpy(xphys, r.physical_density.data(), r.physical_density.size() * sizeof(double));
+    for (std::size_t i = 0; i < r.history.size(); ++i) {
+      hist[4 * i + 0] = r.history[i].compliance;
+      hist[4 * i + 1] = r.history[i].change;
+      hist[4 * i + 2] = r.history[i].volume_fraction;
+      hist[4 * i + 3] = r.history[i].cg_iterations;
+    }
+    *iterations = r.iterations;
+    *compliance = r.compliance;
+    *final_volfrac = r.volume_fraction;
+    if (r.non_convergent) throw oracle::SolverNonConvergence("oracle_optimize: non-convergent", 0, 0);
+  });
+}
+
 // ---- masked (passive-region) variants: mask = one MaskValue byte per voxel ----
 int oracle_filter_masked(const OracleProblem* p, double radius, const uint8_t* mask,
                          const double* x, double* out, int adjoint) {

@@ -341,3 +341,20 @@ def test_simp_loop_structural_gates():
     assert h[-1, 0] < 0.6 * h[0, 0]
     assert abs(out["volume_fraction"] - 0.5) < 5e-3
     assert np.allclose(out["xphys"], oracle.filter(s, 1.5, out["design"]), atol=1e-14)
+
+
+def test_gate_v2_first_stage_oracle():
+    """The oracle's projected loop against the Gate V2 reference (an
+    independent exact-solve computation, benchmarks.json): the beta = 1 stage
+    of the cantilever (the full 300-iteration schedule with every gate runs on
+    the GPU in tests/test_gpu_gate_v2.py)."""
+    import gate_v2
+    dims, bcs, loads = gate_v2.problem("cantilever_24x8x8")
+    s = oracle.Setup(*dims, bcs=bcs, loads=loads)
+    out = oracle.optimize_projected(s, gate_v2.VOLFRAC, gate_v2.RMIN, gate_v2.SCHEDULE[:1],
+                                    cg_tol=gate_v2.CG_TOL)
+    h = out["history"]
+    assert out["iterations"] == 50 and np.all(h[:, 1] <= 0.2 + 1e-9)
+    ref = gate_v2.fixture()["cantilever_24x8x8"]["reference"]["stage_end_compliance"]["1"]
+    assert abs(h[-1, 0] - ref) <= 1e-3 * ref  # the fixture gate is 1e-2
+    assert abs(out["volume_fraction"] - gate_v2.VOLFRAC) <= 1e-4

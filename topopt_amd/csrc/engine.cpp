@@ -1394,12 +1394,53 @@ void design_mask_of(const Problem& P, const uint8_t* mask, std::vector<uint8_t>&
   }
 }
 
+// oc_update_projected (core/src/simp/simp.cpp:1647-1716) over the design set
+// dset with its filter fc; d_xt = filter_density(x).
+void oc_update_projected_core(Problem& P, FilterCache& fc, const uint8_t* dset, double n_design,
+                              const double* d_x, const double* d_xt, const double* d_dc,
+                              double beta, double eta, double volfrac, double move,
+                              double rho_min, double* d_xnew, double* change) {
+  const long long nv = P.g.elems();
+  for (auto* b : {&P.dpr, &P.dcp, &P.dcf, &P.dvf, &P.xcand, &P.xcf})
+    b->ensure(static_cast<size_t>(nv));
+  launch_proj_chain(d_xt, d_dc, dset, nv, beta, eta, P.dcp.p, P.dpr.p, P.s);
+  launch_filter(P.g, fc.st, dset, fc.hs.p, P.dcp.p, P.dcf.p, 1, P.s);
+  launch_filter(P.g, fc.st, dset, fc.hs.p, P.dpr.p, P.dvf.p, 1, P.s);
+  launch_oc_init(P.oc.p, volfrac * n_design, P.s);
+  const int ob = oc_blocks(nv);
+  for (int round = 0;; ++round) {
+    if (round > 40) throw Error(TOPOHIP_EINTERNAL, "oc_update: lambda search did not finish");
+    for (int i = 0; i < 8; ++i) {
+      launch_oc_cand_proj(d_x, P.dcf.p, P.dvf.p, dset, nv, move, rho_min, P.oc.p, P.xcand.p, P.s);
+      launch_filter(P.g, fc.st, dset, fc.hs.p, P.xcand.p, P.xcf.p, 0, P.s);
+      launch_proj_sum(P.xcf.p, dset, nv, beta, eta, P.oc.p, P.partials.p, ob, P.s);
+      launch_oc_decide(P.oc.p, P.partials.p, ob, P.s);
+    }
+    d2h(P.oc_h, P.oc.p, 1, P.s);
+    if (P.oc_h->phase == 2) break;
+  }
+  launch_oc_final_proj(d_x, P.dcf.p, P.dvf.p, dset, nv, move, rho_min, P.oc.p, d_xnew,
+                       P.partials.p, ob, P.s);
+  launch_max_partials(P.partials.p, ob, P.scal.p, P.s);
+  d2h(P.scal_h, P.scal.p, 1, P.s);
+  if (change) *change = P.scal_h[0];
+}
+
 void optimize(Problem& P, const topohip_options& o, double* design_out, double* xphys_out,
-              double* history, topohip_result* res, const uint8_t* mask) {
+              double* history, topohip_result* res, const uint8_t* mask,
+              const std::vector<ProjStage>* proj, double eta) {
   if (!(o.volfrac > 0.0 && o.volfrac <= 1.0))
     throw Error(TOPOHIP_EINVAL, "simp_optimize: volume_fraction must be in (0, 1]");
   if (!(o.move > 0.0)) throw Error(TOPOHIP_EINVAL, "simp_optimize: move must be > 0");
   if (o.max_iterations < 1) throw Error(TOPOHIP_EINVAL, "simp_optimize: max_iterations must be >= 1");
+  if (proj && !proj->empty()) {
+    if (mask) throw Error(TOPOHIP_EINVAL, "simp_optimize: projected stages take no design mask here");
+    if (!(eta > 0.0 && eta < 1.0)) throw Error(TOPOHIP_EINVAL, "heaviside_project: eta must be in (0, 1)");
+    for (const ProjStage& ps : *proj)
+      if (!(std::isfinite(ps.beta) && ps.beta > 0.0) || !(ps.move > 0.0) || ps.iterations < 1)
+        throw Error(TOPOHIP_EINVAL, "simp_optimize: projection stage needs beta > 0, move > 0, "
+                                    "iterations >= 1");
+  }
   if (o.change_tol < 0.0) throw Error(TOPOHIP_EINVAL, "simp_optimize: change_tol must be >= 0");
   std::vector<std::pair<double, int>> stages;
   if (o.pc_per_stage > 0) {
@@ -1458,53 +1499,87 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
   topohip_result r{};
   double t_solve = 0, t_filter = 0, t_oc = 0, t_build = 0;
   const int vb = dot_blocks(nv);
-  for (int it = 0; it < o.max_iterations; ++it) {
-    double ta = now_ms();
-    launch_filter(P.g, fc.st, sol, fc.hs.p, P.xd.p, P.xphys.p, 0, P.s);
-    if (mask) launch_mask_pins(P.xphys.p, d_eff.p, nv, P.s);
-    sync(P);
-    t_filter += now_ms() - ta;
-    ta = now_ms();
-    topohip_cginfo ci{};
-    double c = 0.0;
-    try {
-      c = simp_compliance_dev(P, P.xphys.p, o.E0, penalty_at(r.iterations + 1), o.rho_min, o.cg_tol,
-                              o.cg_max_iterations, &ci);
-    } catch (const Error& e) {
-      if (e.code != TOPOHIP_ENONCONVERGENCE) throw;
-      r.non_convergent = 1;
-      break;
-    }
-    t_solve += now_ms() - ta;
-    t_build += ci.t_mg_build_ms;
-    ta = now_ms();
-    double change = 0.0;
-    oc_update_core(P, fc, sol, nd, mask != nullptr, P.xd.p, P.dc.p, o.volfrac, o.move, o.rho_min,
-                   P.xnew.p, nullptr, nullptr, nullptr, &change);
-    std::swap(P.xd.p, P.xnew.p);
-    t_oc += now_ms() - ta;
-    if (r.iterations == 0) r.initial_compliance = c;
-    ++r.iterations;
-    r.cg_iterations_total += ci.iterations;
-    ta = now_ms();
-    launch_filter(P.g, fc.st, sol, fc.hs.p, P.xd.p, P.xphys.p, 0, P.s);
-    launch_sum_vec<double>(P.xphys.p, nv, P.partials.p, vb, P.s);
-    launch_sum_partials(P.partials.p, vb, P.scal.p, P.s);
-    d2h(P.scal_h, P.scal.p, 1, P.s);
-    t_filter += now_ms() - ta;
-    const double vf = nd > 0 ? P.scal_h[0] / nd : 0.0;
-    if (history) {
-      history[4 * it + 0] = c;
-      history[4 * it + 1] = change;
-      history[4 * it + 2] = vf;
-      history[4 * it + 3] = ci.iterations;
-    }
-    if (change < o.change_tol) {
-      r.converged = 1;
-      break;
-    }
+  // Stage plan (build_stage_plan, core/src/simp/simp.cpp:1053-1068): one
+  // unprojected stage, or the projected stages in order.
+  struct Stage {
+    bool project;
+    double beta, move;
+    int iterations;
+  };
+  std::vector<Stage> plan;
+  if (proj && !proj->empty()) {
+    for (const ProjStage& ps : *proj) plan.push_back({true, ps.beta, ps.move, ps.iterations});
+    P.xtil.ensure(static_cast<size_t>(nv));
+  } else {
+    plan.push_back({false, 0.0, o.move, o.max_iterations});
   }
-  launch_filter(P.g, fc.st, sol, fc.hs.p, P.xd.p, P.xphys.p, 0, P.s);
+  // filtered (and, in a projected stage, projected) density of the design
+  // into P.xphys; the filtered one stays in P.xtil for the OC chain rule
+  auto physical = [&](const Stage& st) {
+    if (st.project) {
+      launch_filter(P.g, fc.st, sol, fc.hs.p, P.xd.p, P.xtil.p, 0, P.s);
+      launch_project(P.xtil.p, P.xphys.p, sol, nv, st.beta, eta, P.s);
+    } else {
+      launch_filter(P.g, fc.st, sol, fc.hs.p, P.xd.p, P.xphys.p, 0, P.s);
+    }
+  };
+  int it = 0;
+  for (const Stage& st : plan) {
+    bool stage_converged = false;
+    for (int si = 0; si < st.iterations; ++si, ++it) {
+      double ta = now_ms();
+      physical(st);
+      if (mask) launch_mask_pins(P.xphys.p, d_eff.p, nv, P.s);
+      sync(P);
+      t_filter += now_ms() - ta;
+      ta = now_ms();
+      topohip_cginfo ci{};
+      double c = 0.0;
+      try {
+        c = simp_compliance_dev(P, P.xphys.p, o.E0, penalty_at(r.iterations + 1), o.rho_min,
+                                o.cg_tol, o.cg_max_iterations, &ci);
+      } catch (const Error& e) {
+        if (e.code != TOPOHIP_ENONCONVERGENCE) throw;
+        r.non_convergent = 1;
+        break;
+      }
+      t_solve += now_ms() - ta;
+      t_build += ci.t_mg_build_ms;
+      ta = now_ms();
+      double change = 0.0;
+      if (st.project)
+        oc_update_projected_core(P, fc, sol, nd, P.xd.p, P.xtil.p, P.dc.p, st.beta, eta, o.volfrac,
+                                 st.move, o.rho_min, P.xnew.p, &change);
+      else
+        oc_update_core(P, fc, sol, nd, mask != nullptr, P.xd.p, P.dc.p, o.volfrac, st.move,
+                       o.rho_min, P.xnew.p, nullptr, nullptr, nullptr, &change);
+      std::swap(P.xd.p, P.xnew.p);
+      t_oc += now_ms() - ta;
+      if (r.iterations == 0) r.initial_compliance = c;
+      ++r.iterations;
+      r.cg_iterations_total += ci.iterations;
+      ta = now_ms();
+      physical(st);
+      launch_sum_vec<double>(P.xphys.p, nv, P.partials.p, vb, P.s);
+      launch_sum_partials(P.partials.p, vb, P.scal.p, P.s);
+      d2h(P.scal_h, P.scal.p, 1, P.s);
+      t_filter += now_ms() - ta;
+      const double vf = nd > 0 ? P.scal_h[0] / nd : 0.0;
+      if (history) {
+        history[4 * it + 0] = c;
+        history[4 * it + 1] = change;
+        history[4 * it + 2] = vf;
+        history[4 * it + 3] = ci.iterations;
+      }
+      if (change < o.change_tol) {
+        stage_converged = true;
+        break;
+      }
+    }
+    r.converged = stage_converged ? 1 : 0;
+    if (r.non_convergent) break;
+  }
+  physical(plan.back());
   // volume fraction of the unpinned physical density (active_volfrac)
   launch_sum_vec<double>(P.xphys.p, nv, P.partials.p, vb, P.s);
   launch_sum_partials(P.partials.p, vb, P.scal.p, P.s);

@@ -105,6 +105,7 @@ struct Problem {
   DBuf<OcState> oc;
   OcState* oc_h = nullptr;
   DBuf<double> xd, xphys, dc, xnew, dcf;
+  DBuf<double> xtil, dpr, dcp, dvf, xcand, xcf;  // projected OC work
   DBuf<uint8_t> active;
 
   int nb = 0;        // stacked load-case capacity of the solver buffers
@@ -139,10 +140,19 @@ double simp_compliance_dev(Problem& P, const double* d_xphys, double E0, double 
 void oc_update_dev(Problem& P, double radius, const double* d_x, const double* d_dc,
                    double volfrac, double move, double rho_min, double* d_xnew,
                    uint8_t* d_active, int* bisections, double* lambda, double* change);
+// One projected (Heaviside beta-continuation) stage: ProjectionStage,
+// core/include/topopt/simp.hpp:518-524.
+struct ProjStage {
+  double beta, move;
+  int iterations;
+};
 // mask (nullable): one MaskValue byte per voxel (0 Active, 1 FrozenSolid,
 // 2 FrozenVoid) for the masked loop; FrozenVoid voxels must be empty in P's grid.
+// proj (nullable, with eta): projected stages run in order instead of one
+// unprojected stage of o.max_iterations at o.move (plain loop only).
 void optimize(Problem& P, const topohip_options& o, double* design_out, double* xphys_out,
-              double* history, topohip_result* res, const uint8_t* mask = nullptr);
+              double* history, topohip_result* res, const uint8_t* mask = nullptr,
+              const std::vector<ProjStage>* proj = nullptr, double eta = 0.5);
 std::unique_ptr<FilterCache> make_filter(Problem& P, double radius, const uint8_t* dset);
 void design_mask_of(const Problem& P, const uint8_t* mask, std::vector<uint8_t>& eff,
                     std::vector<uint8_t>& dset, long long& n_design);

@@ -193,6 +193,23 @@ void launch_oc_final(const double* x, const double* dcf, const double* dv, const
                      long long n, double move, double rho_min, const OcState* st, double* xnew,
                      uint8_t* active, double* change_partials, int nblocks, hipStream_t s,
                      int keep = 0);
+// Heaviside projection (threshold eta, sharpness beta) of design voxels.
+void launch_project(const double* xt, double* xp, const uint8_t* solid, long long n, double beta,
+                    double eta, hipStream_t s);
+// dpr = projection derivative at xt, dcp = dc * dpr (design voxels; 0 elsewhere).
+void launch_proj_chain(const double* xt, const double* dc, const uint8_t* solid, long long n,
+                       double beta, double eta, double* dcp, double* dpr, hipStream_t s);
+// Projected OC: the candidate of the current lambda trial, the projected
+// volume partials of its filtered field, and the final candidate + change.
+void launch_oc_cand_proj(const double* x, const double* dcf, const double* dvf,
+                         const uint8_t* solid, long long n, double move, double rho_min,
+                         const OcState* st, double* xc, hipStream_t s);
+void launch_proj_sum(const double* xtc, const uint8_t* solid, long long n, double beta, double eta,
+                     const OcState* st, double* partials, int nblocks, hipStream_t s);
+void launch_oc_final_proj(const double* x, const double* dcf, const double* dvf,
+                          const uint8_t* solid, long long n, double move, double rho_min,
+                          const OcState* st, double* xnew, double* partials, int nblocks,
+                          hipStream_t s);
 // Design-mask pins on a physical density: eff 1 (FrozenSolid) -> 1, 2 (FrozenVoid) -> 0.
 void launch_mask_pins(double* xphys, const uint8_t* eff, long long n, hipStream_t s);
 void launch_max_partials(const double* partials, int n, double* out, hipStream_t s);

@@ -339,6 +339,129 @@ void launch_oc_final(const double* x, const double* dcf, const double* dv, const
   TOPOHIP_CHECK(hipGetLastError());
 }
 
+// ---- Heaviside projection (core/src/simp/simp.cpp:352-379) and the projected
+// OC update (oc_update_projected, :1647-1716) -----------------------------------
+namespace {
+__device__ inline double hproj(double rt, double beta, double eta) {
+  const double den = tanh(beta * eta) + tanh(beta * (1.0 - eta));
+  return (tanh(beta * eta) + tanh(beta * (rt - eta))) / den;
+}
+__device__ inline double hproj_d(double rt, double beta, double eta) {
+  const double den = tanh(beta * eta) + tanh(beta * (1.0 - eta));
+  const double ch = cosh(beta * (rt - eta));
+  return beta / (ch * ch * den);
+}
+// Projected OC candidate: UNCLAMPED dv in the ratio; a non-finite ratio (0/0)
+// holds the voxel (Be = 1).
+__device__ inline double oc_candidate_proj(double xe, double dce, double dve, double lambda,
+                                           double move, double rho_min) {
+  double be = -dce / (lambda * dve);
+  if (!isfinite(be)) be = 1.0;
+  if (be < 0.0) be = 0.0;
+  double xt = xe * sqrt(be);
+  const double lo = fmax(rho_min, xe - move);
+  const double hi = fmin(1.0, xe + move);
+  if (xt < lo) xt = lo;
+  if (xt > hi) xt = hi;
+  return xt;
+}
+
+// project_solid: design voxels projected, the rest copied.
+__global__ void project_k(const double* __restrict__ xt, double* __restrict__ xp,
+                          const uint8_t* __restrict__ solid, long long n, double beta, double eta) {
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * blockDim.x)
+    xp[e] = (!solid || solid[e]) ? hproj(xt[e], beta, eta) : xt[e];
+}
+// dproj = rho_bar'(xtilde), dc_phys = dc * dproj on design voxels (0 elsewhere).
+__global__ void proj_chain_k(const double* __restrict__ xt, const double* __restrict__ dc,
+                             const uint8_t* __restrict__ solid, long long n, double beta,
+                             double eta, double* __restrict__ dcp, double* __restrict__ dpr) {
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const bool d = !solid || solid[e];
+    const double g = d ? hproj_d(xt[e], beta, eta) : 0.0;
+    dpr[e] = g;
+    dcp[e] = d ? dc[e] * g : 0.0;
+  }
+}
+// Candidate of the current lambda trial (0 off the design set).
+__global__ void oc_cand_proj_k(const double* __restrict__ x, const double* __restrict__ dcf,
+                               const double* __restrict__ dvf, const uint8_t* __restrict__ solid,
+                               long long n, double move, double rho_min, const OcState* st,
+                               double* __restrict__ xc) {
+  if (st->phase == 2) return;
+  const double lambda = oc_lambda(st);
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * blockDim.x)
+    xc[e] = (!solid || solid[e]) ? oc_candidate_proj(x[e], dcf[e], dvf[e], lambda, move, rho_min)
+                                 : 0.0;
+}
+// Projected volume partials of a filtered candidate, over design voxels.
+__global__ void proj_sum_k(const double* __restrict__ xtc, const uint8_t* __restrict__ solid,
+                           long long n, double beta, double eta, const OcState* st,
+                           double* __restrict__ partials) {
+  if (st->phase == 2) return;
+  double s = 0.0;
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * blockDim.x)
+    if (!solid || solid[e]) s += hproj(xtc[e], beta, eta);
+  s = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+__global__ void oc_final_proj_k(const double* __restrict__ x, const double* __restrict__ dcf,
+                                const double* __restrict__ dvf, const uint8_t* __restrict__ solid,
+                                long long n, double move, double rho_min, const OcState* st,
+                                double* __restrict__ xnew, double* __restrict__ partials) {
+  const double lambda = 0.5 * (st->l1 + st->l2);
+  double m = 0.0;
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const double xt = (!solid || solid[e])
+                          ? oc_candidate_proj(x[e], dcf[e], dvf[e], lambda, move, rho_min)
+                          : 0.0;
+    xnew[e] = xt;
+    m = fmax(m, fabs(xt - x[e]));
+  }
+  m = block_max(m);
+  if (threadIdx.x == 0) partials[blockIdx.x] = m;
+}
+}  // namespace
+
+void launch_project(const double* xt, double* xp, const uint8_t* solid, long long n, double beta,
+                    double eta, hipStream_t s) {
+  hipLaunchKernelGGL(project_k, dim3(grid_for(n, RB, 4096)), dim3(RB), 0, s, xt, xp, solid, n,
+                     beta, eta);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+void launch_proj_chain(const double* xt, const double* dc, const uint8_t* solid, long long n,
+                       double beta, double eta, double* dcp, double* dpr, hipStream_t s) {
+  hipLaunchKernelGGL(proj_chain_k, dim3(grid_for(n, RB, 4096)), dim3(RB), 0, s, xt, dc, solid, n,
+                     beta, eta, dcp, dpr);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+void launch_oc_cand_proj(const double* x, const double* dcf, const double* dvf,
+                         const uint8_t* solid, long long n, double move, double rho_min,
+                         const OcState* st, double* xc, hipStream_t s) {
+  hipLaunchKernelGGL(oc_cand_proj_k, dim3(grid_for(n, RB, 4096)), dim3(RB), 0, s, x, dcf, dvf,
+                     solid, n, move, rho_min, st, xc);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+void launch_proj_sum(const double* xtc, const uint8_t* solid, long long n, double beta, double eta,
+                     const OcState* st, double* partials, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(proj_sum_k, dim3(nblocks), dim3(RB), 0, s, xtc, solid, n, beta, eta, st,
+                     partials);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+void launch_oc_final_proj(const double* x, const double* dcf, const double* dvf,
+                          const uint8_t* solid, long long n, double move, double rho_min,
+                          const OcState* st, double* xnew, double* partials, int nblocks,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(oc_final_proj_k, dim3(nblocks), dim3(RB), 0, s, x, dcf, dvf, solid, n, move,
+                     rho_min, st, xnew, partials);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+
 // apply_mask_pins (core/src/simp/simp.cpp:2649-2654): FrozenSolid -> 1, FrozenVoid -> 0.
 __global__ void mask_pins_k(double* __restrict__ xphys, const uint8_t* __restrict__ eff,
                             long long n) {
