@@ -87,6 +87,23 @@ def ku_measure(ta, nx, ny, nz, reps=100):
     return res
 
 
+def fields_measure(ta, nx, ny, nz, reps=50):
+    """Result-field kernels (von Mises -> f32, displacement -> f32, density ->
+    f16) on resident inputs; algorithmic bytes: u once (8 B per DOF) + 4 B per
+    voxel out; 8 + 4 B per DOF; 8 + 2 B per voxel."""
+    with ta.Problem(nx, ny, nz) as p:
+        p.bench_prepare(0x5EED)
+        ms = p.bench_fields(reps)
+    nv = nx * ny * nz
+    nd = 3 * (nx + 1) * (ny + 1) * (nz + 1)
+    out = {}
+    for name, t, b in (("von_mises_f32", ms[0], 8 * nd + 4 * nv), ("narrow_f32", ms[1], 12 * nd),
+                       ("density_f16", ms[2], 10 * nv)):
+        gbs = b / (t * 1e-3) / 1e9
+        out[name] = {"us": t * 1e3, "bytes": b, "GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+    return out
+
+
 def run_serving(ta, njobs, rate, seed, iterations):
     """Poisson arrivals of jobs with uneven grids through the serving loop."""
     rng = np.random.default_rng(seed)
@@ -204,6 +221,9 @@ def main():
                 "compliance_rel_diff_vs_mixed": rel, "cg_iterations_total": out64["cg_iterations_total"]}
         print(f"# fp64 form: {v64:.2f} design iters/s", file=sys.stderr, flush=True)
     ku = ku_measure(ta, nx, ny, nz)
+    fields = fields_measure(ta, nx, ny, nz)
+    print("# fields " + ", ".join(f"{k} {v['us']:.1f} us {v['GBs']:.0f} GB/s"
+                                  for k, v in fields.items()), file=sys.stderr, flush=True)
     print(f"# K.u f32 {ku['f32']['ms']*1e3:.1f} us {ku['f32']['GBs']:.0f} GB/s; "
           f"f64 {ku['f64']['ms']*1e3:.1f} us", file=sys.stderr, flush=True)
     cpu = None
@@ -233,6 +253,7 @@ def main():
                    "t_oc_ms": out["t_oc_ms"], "t_filter_ms": out["t_filter_ms"]},
         "fp64_form": fp64,
         "ku": ku,
+        "result_fields": fields,
         "roofline": {"bound": "hbm", "achieved": ku["f32"]["GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ku["f32"]["hbm_frac"],
                      "traffic": measured_traffic(a.workload),

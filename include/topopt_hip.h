@@ -116,6 +116,18 @@ int topohip_simp_compliance(topohip_problem* p, const double* xphys, double E0,
                             double penalty, double rho_min, double tol, int max_it,
                             double* c, double* dc, double* u_out, topohip_cginfo* info);
 
+/* simp_compliance with a warm start (its initial_guess argument,
+ * core/src/simp/simp.cpp:567-828, as the PLSM state solve calls it at a loose
+ * tolerance, core/src/simp/plsm.cpp:671-689): guess (nullable) is a full
+ * displacement, 3 * nodes, used as the MG-PCG start on the free DOFs (a
+ * non-finite entry makes the solve cold, as the reference's handling). One load
+ * case only (TOPOHIP_EINVAL otherwise). Other arguments as
+ * topohip_simp_compliance. */
+int topohip_simp_compliance_warm(topohip_problem* p, const double* xphys, double E0,
+                                 double penalty, double rho_min, double tol, int max_it,
+                                 const double* guess, double* c, double* dc, double* u_out,
+                                 topohip_cginfo* info);
+
 /* Density filter with cone weights max(0, r - dist) over solid voxels
  * (make_density_filter / filter_density / filter_sensitivity,
  * core/src/simp/simp.cpp:834-944). adjoint == 0: out = H x / Hs;
@@ -206,6 +218,11 @@ int topohip_filter_masked(topohip_problem* p, double radius, const uint8_t* mask
  * stream. precision: TOPOHIP_PREC_MIXED = FP32 apply, TOPOHIP_PREC_F64. */
 int topohip_bench_prepare(topohip_problem* p, uint64_t seed);
 int topohip_bench_ku(topohip_problem* p, int precision, int reps, double* ms_per_apply);
+/* Average device time (ms) of the result-field kernels over reps launches on
+ * the bench inputs (topohip_bench_prepare): ms3[0] von Mises to float32,
+ * ms3[1] displacement narrowing, ms3[2] density halves. No reference
+ * counterpart. */
+int topohip_bench_fields(topohip_problem* p, int reps, double* ms3);
 
 /* ---- serving: persistent continuous-batching loop ---------------------------
  * Jobs of uneven grid sizes are admitted as device memory (budget_bytes) and
@@ -241,6 +258,145 @@ int topohip_server_job_error(topohip_server* s, int64_t job_id, char* buf, size_
 int topohip_server_destroy(topohip_server* s);
 /* Device bytes a job of this shape keeps resident (admission estimate). */
 size_t topohip_job_footprint(const topohip_job* job);
+
+/* ---- result fields ----------------------------------------------------------
+ * Formed on the device from the handle's grid. u (3 * nodes doubles) is
+ * nullable everywhere below: NULL takes the handle's resident displacement of
+ * its last solve (load case 0). */
+
+/* fea_von_mises_field (core/include/topopt/fea.hpp:1324-1331,
+ * core/src/fea/assembly.cpp:976-996): von Mises stress at each solid voxel's
+ * centroid for Young's modulus `youngs` and the handle's Poisson ratio and
+ * spacing; 0 on empty voxels. vm_out: nx*ny*nz doubles. */
+int topohip_von_mises_field(topohip_problem* p, double youngs, const double* u, double* vm_out);
+
+/* The two float32 arrays of a fields.bin variant block (write_fields_file,
+ * core/src/io/fields.cpp:54-119): the von Mises field narrowed to float
+ * (vm_out, nx*ny*nz) and the displacement narrowed to float (disp_out,
+ * 3 * nodes). Either output may be NULL. */
+int topohip_fields_f32(topohip_problem* p, double youngs, const double* u, float* vm_out,
+                       float* disp_out);
+
+/* The halves of a density snapshot (write_density_snapshot,
+ * core/src/simp/observability.cpp:362-384): float_to_half(float(d[e])) per
+ * voxel, IEEE binary16 round-to-nearest-even. density NULL = the handle's
+ * resident physical density (after topohip_optimize*). */
+int topohip_density_f16(topohip_problem* p, const double* density, uint16_t* out);
+
+/* ---- per-iteration observation ----------------------------------------------
+ * SimpIterationObservation + IterationPhaseTimes
+ * (core/include/topopt/simp.hpp:870-1000). Fields for subsystems this library
+ * does not have (plateau detector, infeasibility verdict, Krylov recycling,
+ * GenEO) are always 0, which is what the reference writes when they are off.
+ * The memory columns follow process_memory (core/src/simp/observability.cpp:
+ * 116-205): negative = not answered. */
+typedef struct {
+  int iteration;              /* 1-based, monotone across stages */
+  double compliance;          /* of xPhys at the start of the step */
+  double change;
+  double volume_fraction;     /* achieved after the step */
+  int cg_iterations, cg_used_multigrid, cg_mg_levels, cg_hier_built, cg_mg_cycles_attempted;
+  double beta;                /* stage beta while projecting, else 0 */
+  double active_fraction;     /* 1 (the full analysis grid is solved) */
+  int plateau, infeasible, recycle_dim, geneo_dim, geneo_action, geneo_burn, geneo_threshold;
+  double total_ms, tail_prev_ms, filter_ms, project_ms, solve_ms, fea_ms, sens_ms, update_ms,
+      analysis_ms, observe_ms, residual_ms;
+  double solver_build_ms, solver_mg_build_ms, solver_mg_ms, solver_cg_ms,
+      solver_geneo_setup_ms, solver_geneo_apply_ms, solver_recycle_ms;
+  long long fea_solves, matvecs;
+  double rss_mb, peak_rss_mb, compressed_mb, available_mb;
+  long long major_faults, swapins;
+} topohip_iteration_obs;
+
+/* SimpOptions::observe (core/include/topopt/simp.hpp:1534-1542): called on
+ * the optimising thread after every design iteration of topohip_optimize,
+ * topohip_optimize_masked and topohip_optimize_projected on this handle. NULL
+ * clears it. The callback must not call back into the same handle. */
+typedef void (*topohip_iteration_cb)(const topohip_iteration_obs* obs, void* user);
+int topohip_set_iteration_observer(topohip_problem* p, topohip_iteration_cb cb, void* user);
+
+/* ---- run records (host formats, byte-compatible with the reference) -------- */
+
+/* iterations.csv: IterationCsvWriter (core/include/topopt/observability.hpp:
+ * 60-214, core/src/simp/observability.cpp:252-330). The header is
+ * kIterationCsvHeader; a row is formatted exactly as append_at formats it. */
+const char* topohip_iterations_csv_header(void);
+/* Formats one row (with its newline) into buf; *len (nullable) receives its
+ * length. EINVAL if n is too small. */
+int topohip_iterations_csv_row(size_t rung, const topohip_iteration_obs* obs, long long wall_ms,
+                               char* buf, size_t n, size_t* len);
+typedef struct topohip_csv_writer topohip_csv_writer;
+/* Opens (truncating) path and writes the header; every append flushes. */
+int topohip_csv_open(const char* path, topohip_csv_writer** out);
+/* append stamps the epoch wall clock in ms (wall_clock_ms); append_at takes it. */
+int topohip_csv_append(topohip_csv_writer* w, size_t rung, const topohip_iteration_obs* obs);
+int topohip_csv_append_at(topohip_csv_writer* w, size_t rung, const topohip_iteration_obs* obs,
+                          long long wall_ms);
+size_t topohip_csv_rows(const topohip_csv_writer* w);
+int topohip_csv_close(topohip_csv_writer* w);
+
+/* Density snapshot: <dir>/<basename>.f16 (raw little-endian halves, x-fastest)
+ * and the <dir>/<basename>.json sidecar, as write_density_snapshot
+ * (core/src/simp/observability.cpp:362-408); dir is created if absent. The
+ * halves are formed on the device (topohip_density_f16); dims and spacing are
+ * the handle's, origin (3 doubles, nullable = 0) is the grid's minimum corner. */
+int topohip_write_density_snapshot(topohip_problem* p, const char* dir, const char* basename,
+                                   const double* origin, const double* density, size_t rung,
+                                   int iteration, int boundary);
+/* read_density_f16 (core/src/simp/observability.cpp:410-425): EINVAL unless
+ * the file holds exactly count halves. */
+int topohip_read_density_f16(const char* path, size_t count, float* out);
+
+/* The solved grid a design.bin / fields.bin container indexes to. */
+typedef struct {
+  int nx, ny, nz;
+  double origin[3];
+  double spacing;
+} topohip_grid_frame;
+
+/* design.bin v1 (core/include/topopt/design_store.hpp:40-100,
+ * core/src/io/design_store.cpp:75-243). */
+typedef struct {
+  double requested_volume_fraction, achieved_volume_fraction;
+  double margin_worst_case, margin_effective, max_von_mises_mpa;
+  int accepted, iterations;
+  double applied_build_dir[3];
+  int build_direction_auto_applied, export_baked;
+  uint64_t fingerprint;       /* read: as stored (verified); write: ignored */
+  const double* density;      /* write: nx*ny*nz doubles */
+} topohip_design_variant;
+/* design_fingerprint: FNV-1a 64 over the little-endian bytes of the doubles. */
+uint64_t topohip_design_fingerprint(const double* density, size_t n);
+/* write_design_file: atomic (written beside, then renamed). */
+int topohip_write_design_file(const char* path, const topohip_grid_frame* grid, int n_variants,
+                              const topohip_design_variant* variants);
+/* read_design_file in two steps: the header, then variant `index` (its
+ * density into density_out, nx*ny*nz doubles, nullable). Version, header
+ * sanity, counts and each fingerprint are checked (EINVAL on a mismatch). */
+int topohip_read_design_header(const char* path, topohip_grid_frame* grid, int* n_variants);
+int topohip_read_design_variant(const char* path, int index, topohip_design_variant* meta,
+                                double* density_out);
+
+/* fields.bin v1 (core/include/topopt/fields.hpp:28-70, core/src/io/fields.cpp:54-140):
+ * one block per variant passed (the caller passes the accepted ones). Arrays
+ * are float32 (topohip_fields_f32) and both are required: a variant without
+ * them is EINVAL, as the reference refuses fields that do not match the grid.
+ * Atomic (written beside, then renamed). */
+typedef struct {
+  double requested_volume_fraction, mass_grams;
+  int support_volume_voxels;
+  const float* von_mises;     /* nx*ny*nz */
+  const float* displacement;  /* 3*(nx+1)(ny+1)(nz+1) */
+} topohip_fields_variant;
+int topohip_write_fields_file(const char* path, const topohip_grid_frame* grid, int n_variants,
+                              const topohip_fields_variant* variants);
+
+/* The CLI's stdout protocol lines the LAN worker parses
+ * (core/src/cli/run_job.cpp:10223-10259, tools/topopt-worker/topopt_worker.py:231-266),
+ * newline included. */
+int topohip_format_progress(size_t rung, size_t rungs, int iteration, char* buf, size_t n);
+int topohip_format_variant(double vf, double achieved, double printed, double margin,
+                           const char* mesh, char* buf, size_t n);
 
 /* ---- misc ------------------------------------------------------------------ */
 int topohip_set_device(int device);

@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstring>
 #include <functional>
 #include <limits>
 #include <mutex>
@@ -176,6 +177,57 @@ void hex8_stiffness(double E, double nu, double h, double* K) {
             K[r * 24 + col] += acc * detJ;
           }
       }
+}
+
+// core/src/fea/hex_element.cpp:242-316. The full 6x24 B and 6x6 D products, in
+// the reference's order, at the given natural coordinates.
+Hex8Stress hex8_stress(double E, double nu, double h, const double* ue, double xi, double et,
+                       double ze) {
+  if (!(E > 0.0)) throw std::invalid_argument("hex8_stress: youngs_modulus must be > 0");
+  if (!(h > 0.0)) throw std::invalid_argument("hex8_stress: element_size must be > 0");
+  if (!(nu > -1.0 && nu < 0.5))
+    throw std::invalid_argument("hex8_stress: poisson must be in (-1, 0.5)");
+  static const double xs[8] = {-1, +1, +1, -1, -1, +1, +1, -1};
+  static const double ys[8] = {-1, -1, +1, +1, -1, -1, +1, +1};
+  static const double zs[8] = {-1, -1, -1, -1, +1, +1, +1, +1};
+  const double c = E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  double D[6][6] = {};
+  D[0][0] = D[1][1] = D[2][2] = c * (1.0 - nu);
+  D[0][1] = D[0][2] = D[1][0] = D[1][2] = D[2][0] = D[2][1] = c * nu;
+  D[3][3] = D[4][4] = D[5][5] = c * (1.0 - 2.0 * nu) / 2.0;
+  const double s = 2.0 / h;
+  double B[6][24] = {};
+  for (int a = 0; a < 8; ++a) {
+    const double dx = s * 0.125 * xs[a] * (1.0 + et * ys[a]) * (1.0 + ze * zs[a]);
+    const double dy = s * 0.125 * ys[a] * (1.0 + xi * xs[a]) * (1.0 + ze * zs[a]);
+    const double dz = s * 0.125 * zs[a] * (1.0 + xi * xs[a]) * (1.0 + et * ys[a]);
+    B[0][3 * a] = dx;
+    B[1][3 * a + 1] = dy;
+    B[2][3 * a + 2] = dz;
+    B[3][3 * a] = dy;
+    B[3][3 * a + 1] = dx;
+    B[4][3 * a + 1] = dz;
+    B[4][3 * a + 2] = dy;
+    B[5][3 * a] = dz;
+    B[5][3 * a + 2] = dx;
+  }
+  double eps[6];
+  for (int r = 0; r < 6; ++r) {
+    double acc = 0.0;
+    for (int col = 0; col < 24; ++col) acc += B[r][col] * ue[col];
+    eps[r] = acc;
+  }
+  Hex8Stress out{};
+  for (int r = 0; r < 6; ++r) {
+    double acc = 0.0;
+    for (int m = 0; m < 6; ++m) acc += D[r][m] * eps[m];
+    out.sigma[r] = acc;
+  }
+  const double* q = out.sigma;
+  out.von_mises = std::sqrt(0.5 * ((q[0] - q[1]) * (q[0] - q[1]) + (q[1] - q[2]) * (q[1] - q[2]) +
+                                   (q[2] - q[0]) * (q[2] - q[0])) +
+                            3.0 * (q[3] * q[3] + q[4] * q[4] + q[5] * q[5]));
+  return out;
 }
 
 std::array<int, 8> element_nodes(const Grid& g, int i, int j, int k) {
@@ -1049,7 +1101,7 @@ SimpCompliance simp_compliance(const Grid& g, const SimpParams& p,
                                const std::vector<double>& density,
                                const std::vector<DirichletBC>& bcs,
                                const std::vector<NodalLoad>& loads, double tol, int max_it,
-                               bool use_mg) {
+                               bool use_mg, const std::vector<double>* initial_guess) {
   if (density.size() != g.voxels())
     throw std::invalid_argument("simp_compliance: density vector size != voxel_count");
   std::vector<double> E(g.voxels(), 0.0);
@@ -1062,7 +1114,7 @@ SimpCompliance simp_compliance(const Grid& g, const SimpParams& p,
         E[e] = std::pow(rho, p.penalty) * p.E0;
       }
   SimpCompliance out;
-  out.u = use_mg ? solve_mgcg_matfree(g, E, p.nu, bcs, loads, tol, max_it, &out.cg)
+  out.u = use_mg ? solve_mgcg_matfree(g, E, p.nu, bcs, loads, tol, max_it, &out.cg, initial_guess)
                  : solve_cg_matfree(g, E, p.nu, bcs, loads, tol, max_it, &out.cg);
   double Ku[kDof * kDof];
   hex8_stiffness(1.0, p.nu, g.h, Ku);
@@ -1563,6 +1615,67 @@ double heaviside_project_derivative(double rt, double beta, double eta) {
   const double den = std::tanh(beta * eta) + std::tanh(beta * (1.0 - eta));
   const double ch = std::cosh(beta * (rt - eta));
   return beta / (ch * ch * den);
+}
+
+}  // namespace oracle
+
+namespace oracle {
+
+// core/src/fea/assembly.cpp:976-996.
+std::vector<double> von_mises_field(const Grid& g, double E, double nu,
+                                    const std::vector<double>& u) {
+  if (u.size() != 3 * static_cast<std::size_t>(g.nodes()))
+    throw std::invalid_argument("von_mises_field: displacement size != 3 * nodes");
+  std::vector<double> field(g.voxels(), 0.0);
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) {
+        if (!g.is_solid(i, j, k)) continue;
+        const auto en = element_nodes(g, i, j, k);
+        double ue[24];
+        for (int a = 0; a < 8; ++a)
+          for (int c = 0; c < 3; ++c) ue[3 * a + c] = u[3 * static_cast<std::size_t>(en[a]) + c];
+        field[g.index(i, j, k)] = hex8_stress(E, nu, g.h, ue).von_mises;
+      }
+  return field;
+}
+
+// core/src/simp/observability.cpp:35-75: sign, exponent re-bias, mantissa with
+// round-to-nearest-even on the dropped bits; subnormals by shifting in the
+// implicit bit; overflow to Inf; NaN keeps only the quiet bit.
+std::uint16_t float_to_half(float v) {
+  std::uint32_t x;
+  std::memcpy(&x, &v, 4);
+  const std::uint32_t sign = (x >> 16) & 0x8000u;
+  const int biased = static_cast<int>((x >> 23) & 0xFFu);
+  const std::uint32_t mant = x & 0x7FFFFFu;
+  if (biased == 255) return static_cast<std::uint16_t>(sign | 0x7C00u | (mant ? 0x200u : 0u));
+  const int e = biased - 112;  // half's biased exponent
+  if (e >= 31) return static_cast<std::uint16_t>(sign | 0x7C00u);
+  if (e <= 0) {
+    if (e < -10) return static_cast<std::uint16_t>(sign);
+    const std::uint32_t m = mant | 0x800000u;
+    const int sh = 14 - e;
+    std::uint32_t hm = m >> sh;
+    const std::uint32_t rem = m & ((1u << sh) - 1u), mid = 1u << (sh - 1);
+    if (rem > mid || (rem == mid && (hm & 1u))) ++hm;
+    return static_cast<std::uint16_t>(sign | hm);
+  }
+  std::uint32_t hv = sign | (static_cast<std::uint32_t>(e) << 10) | (mant >> 13);
+  const std::uint32_t rem = mant & 0x1FFFu;
+  if (rem > 0x1000u || (rem == 0x1000u && (hv & 1u))) ++hv;
+  return static_cast<std::uint16_t>(hv);
+}
+
+// core/src/io/design_store.cpp:75-88 (FNV-1a, 64-bit offset basis and prime).
+std::uint64_t fnv1a64(const void* data, std::size_t n) {
+  const auto* b = static_cast<const std::uint8_t*>(data);
+  std::uint64_t h = 14695981039346656037ULL;
+  for (std::size_t i = 0; i < n; ++i) {
+    h ^= b[i];
+    h *= 1099511628211ULL;
+  }
+  return h;
 }
 
 }  // namespace oracle

@@ -25,6 +25,12 @@
 //                       projection_unit_reference spot values (1e-12)
 //   simp_optimize       structural gates (monotone compliance, volume,
 //                       xPhys = filter(design))
+//   hex8_stress         the reference's closed forms (uniaxial strain, pure
+//                       shear, rigid translation; core/tests/unit/test_fea.cpp:303-390)
+//   float_to_half       numpy's IEEE float16 conversion (every finite input),
+//                       the reference's anchors and round-trip bound
+//                       (core/tests/unit/test_observability.cpp:55-74)
+//   fnv1a64             the published FNV-1a 64-bit test vectors
 //   masked loop         (tests/test_oracle_masked.py) all-Active mask equals
 //                       the plain loop bit for bit; masked filter against a
 //                       brute-force evaluation of its definition; no bleed of
@@ -234,7 +240,8 @@ SimpCompliance simp_compliance(const Grid& g, const SimpParams& p,
                                const std::vector<double>& density,
                                const std::vector<DirichletBC>& bcs,
                                const std::vector<NodalLoad>& loads, double tol,
-                               int max_it, bool use_mg = true);
+                               int max_it, bool use_mg = true,
+                               const std::vector<double>* initial_guess = nullptr);
 
 // DensityFilter (core/include/topopt/simp.hpp:387-406; core/src/simp/simp.cpp:834-944).
 struct DensityFilter {
@@ -327,6 +334,19 @@ SimpResult simp_optimize_masked(const Grid& g, const SimpParams& p,
                                 const std::vector<std::uint8_t>& mask);
 
 // ---- Heaviside projection (core/src/simp/simp.cpp:352-379) ---------------
+// Element stress at natural coordinates (xi, eta, zeta): core/src/fea/hex_element.cpp:242-316
+// (sigma = D B u, Voigt [xx, yy, zz, xy, yz, zx], engineering shear).
+struct Hex8Stress { double sigma[6]; double von_mises; };
+Hex8Stress hex8_stress(double E, double nu, double h, const double* u_elem, double xi = 0.0,
+                       double eta = 0.0, double zeta = 0.0);
+// core/src/fea/assembly.cpp:976-996: centroid von Mises per solid voxel, 0 elsewhere.
+std::vector<double> von_mises_field(const Grid& g, double E, double nu,
+                                    const std::vector<double>& u);
+// core/src/simp/observability.cpp:35-75: float -> IEEE half, round to nearest even.
+std::uint16_t float_to_half(float v);
+// core/src/simp/design_store.cpp (design_fingerprint): 64-bit FNV-1a over bytes.
+std::uint64_t fnv1a64(const void* data, std::size_t n);
+
 double heaviside_project(double rt, double beta, double eta);
 double heaviside_project_derivative(double rt, double beta, double eta);
 // oc_update_projected (simp.cpp:1647-1716): sensitivities chained through the

@@ -56,6 +56,8 @@ def lib():
             build()
         _lib = C.CDLL(_LIB)
         _lib.oracle_last_error.restype = C.c_char_p
+        _lib.oracle_fnv1a64.restype = C.c_uint64
+        _lib.oracle_fnv1a64.argtypes = [C.c_void_p, C.c_size_t]
     return _lib
 
 
@@ -133,16 +135,24 @@ def hierarchy(s, E, nu=0.3):
 
 
 def simp_compliance(s, density, E0=1.0, nu=0.3, penalty=3.0, rho_min=1e-3, tol=1e-8,
-                    max_it=0, use_mg=True):
+                    max_it=0, use_mg=True, guess=None):
     d = np.ascontiguousarray(density, dtype=np.float64)
     c = C.c_double(0)
     dc = np.zeros(s.nvox)
     u = np.zeros(s.ndof)
     info = Info()
-    _check(lib().oracle_simp_compliance(C.byref(s.p), _ptr(d), C.c_double(E0), C.c_double(nu),
-                                        C.c_double(penalty), C.c_double(rho_min), C.c_double(tol),
-                                        int(max_it), int(use_mg), C.byref(c), _ptr(dc), _ptr(u),
-                                        C.byref(info)))
+    if guess is None:
+        _check(lib().oracle_simp_compliance(C.byref(s.p), _ptr(d), C.c_double(E0), C.c_double(nu),
+                                            C.c_double(penalty), C.c_double(rho_min),
+                                            C.c_double(tol), int(max_it), int(use_mg), C.byref(c),
+                                            _ptr(dc), _ptr(u), C.byref(info)))
+    else:
+        g = np.ascontiguousarray(guess, dtype=np.float64)
+        _check(lib().oracle_simp_compliance_warm(C.byref(s.p), _ptr(d), C.c_double(E0),
+                                                 C.c_double(nu), C.c_double(penalty),
+                                                 C.c_double(rho_min), C.c_double(tol), int(max_it),
+                                                 _ptr(g), C.byref(c), _ptr(dc), _ptr(u),
+                                                 C.byref(info)))
     return c.value, dc, u, info.as_dict()
 
 
@@ -276,3 +286,37 @@ def heaviside(rt, beta, eta=0.5):
     _check(lib().oracle_heaviside(C.c_double(rt), C.c_double(beta), C.c_double(eta), C.byref(v),
                                   C.byref(d)))
     return v.value, d.value
+
+
+def von_mises_field(s, u, E=1.0, nu=0.3):
+    """fea_von_mises_field: centroid von Mises per solid voxel (0 elsewhere)."""
+    uu = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros(s.nvox)
+    _check(lib().oracle_von_mises(C.byref(s.p), C.c_double(E), C.c_double(nu), _ptr(uu),
+                                  _ptr(out)))
+    return out
+
+
+def hex8_stress(E, nu, h, ue, xi=0.0, eta=0.0, zeta=0.0):
+    ue = np.ascontiguousarray(ue, dtype=np.float64)
+    sig = np.zeros(6)
+    vm = C.c_double(0)
+    _check(lib().oracle_hex8_stress(C.c_double(E), C.c_double(nu), C.c_double(h), _ptr(ue),
+                                    C.c_double(xi), C.c_double(eta), C.c_double(zeta), _ptr(sig),
+                                    C.byref(vm)))
+    return sig, vm.value
+
+
+def float_to_half(x):
+    """IEEE half bits (uint16) of float32(x), round to nearest even."""
+    f = np.ascontiguousarray(x, dtype=np.float32).ravel()
+    out = np.zeros(f.size, dtype=np.uint16)
+    _check(lib().oracle_float_to_half(f.ctypes.data_as(C.c_void_p), C.c_size_t(f.size),
+                                      out.ctypes.data_as(C.c_void_p)))
+    return out
+
+
+def fnv1a64(data):
+    b = bytes(data)
+    buf = C.create_string_buffer(b, len(b))
+    return int(lib().oracle_fnv1a64(buf, C.c_size_t(len(b))))

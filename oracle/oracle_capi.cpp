@@ -157,6 +157,25 @@ int oracle_simp_compliance(const OracleProblem* p, const double* density, double
   });
 }
 
+// As oracle_simp_compliance with a warm start (guess: 3 * nodes).
+int oracle_simp_compliance_warm(const OracleProblem* p, const double* density, double E0,
+                                double nu, double penalty, double rho_min, double tol, int max_it,
+                                const double* guess, double* c, double* dc, double* u,
+                                OracleInfo* info) {
+  return guarded([&] {
+    const oracle::Grid g = grid_of(p);
+    oracle::SimpParams sp{E0, nu, penalty, rho_min};
+    std::vector<double> d(density, density + g.voxels());
+    const std::vector<double> gv(guess, guess + 3 * static_cast<std::size_t>(g.nodes()));
+    const oracle::SimpCompliance r =
+        oracle::simp_compliance(g, sp, d, bcs_of(p), loads_of(p), tol, max_it, true, &gv);
+    fill_info(r.cg, info);
+    *c = r.compliance;
+    if (dc) std::memcpy(dc, r.dc.data(), r.dc.size() * sizeof(double));
+    if (u) std::memcpy(u, r.u.data(), r.u.size() * sizeof(double));
+  });
+}
+
 // adjoint == 0: xPhys = filter_density(x); adjoint != 0: filter_sensitivity(x).
 int oracle_filter(const OracleProblem* p, double radius, const double* x, double* out,
                   int adjoint) {
@@ -368,6 +387,32 @@ int oracle_penalty_at(double p0, double p1, double step, int per_stage, int iter
                                         iteration);
   });
 }
+
+int oracle_von_mises(const OracleProblem* p, double E, double nu, const double* u, double* vm) {
+  return guarded([&] {
+    const oracle::Grid g = grid_of(p);
+    const std::vector<double> uv(u, u + 3 * static_cast<std::size_t>(g.nodes()));
+    const auto f = oracle::von_mises_field(g, E, nu, uv);
+    std::memcpy(vm, f.data(), f.size() * sizeof(double));
+  });
+}
+
+int oracle_hex8_stress(double E, double nu, double h, const double* ue, double xi, double eta,
+                       double zeta, double* sigma6, double* vm) {
+  return guarded([&] {
+    const oracle::Hex8Stress st = oracle::hex8_stress(E, nu, h, ue, xi, eta, zeta);
+    std::memcpy(sigma6, st.sigma, sizeof(st.sigma));
+    *vm = st.von_mises;
+  });
+}
+
+int oracle_float_to_half(const float* in, std::size_t n, std::uint16_t* out) {
+  return guarded([&] {
+    for (std::size_t i = 0; i < n; ++i) out[i] = oracle::float_to_half(in[i]);
+  });
+}
+
+std::uint64_t oracle_fnv1a64(const void* data, std::size_t n) { return oracle::fnv1a64(data, n); }
 
 int oracle_heaviside(double rt, double beta, double eta, double* value, double* deriv) {
   return guarded([&] {

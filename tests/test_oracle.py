@@ -358,3 +358,24 @@ def test_gate_v2_first_stage_oracle():
     ref = gate_v2.fixture()["cantilever_24x8x8"]["reference"]["stage_end_compliance"]["1"]
     assert abs(h[-1, 0] - ref) <= 1e-3 * ref  # the fixture gate is 1e-2
     assert abs(out["volume_fraction"] - gate_v2.VOLFRAC) <= 1e-4
+
+
+def test_oracle_warm_start_converges_to_the_cold_answer():
+    """simp_compliance's initial_guess (core/src/simp/simp.cpp:696-721): a
+    guess at the solution stops after one iteration with the same compliance;
+    a neighbouring design's field cuts the iteration count and the loose
+    answer stays within the tolerance of the tight one."""
+    nx, ny, nz = 12, 4, 4
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    rng = np.random.default_rng(71)
+    x1 = rng.uniform(0.1, 1.0, nx * ny * nz)
+    x2 = np.clip(x1 + rng.uniform(-0.05, 0.05, x1.size), 0.1, 1.0)
+    s = oracle.Setup(nx, ny, nz, bcs=bcs, loads=loads)
+    c1, _, u1, _ = oracle.simp_compliance(s, x1, tol=1e-10)
+    c0, _, _, i0 = oracle.simp_compliance(s, x1, tol=1e-4, guess=u1)
+    assert i0["iterations"] == 1 and abs(c0 - c1) / c1 < 1e-9
+    cw, _, _, iw = oracle.simp_compliance(s, x2, tol=1e-4, guess=u1)
+    _, _, _, ic = oracle.simp_compliance(s, x2, tol=1e-4)
+    ct, _, _, _ = oracle.simp_compliance(s, x2, tol=1e-10)
+    assert iw["iterations"] < ic["iterations"]
+    assert abs(cw - ct) / ct < 1e-3

@@ -236,18 +236,71 @@ class Problem:
         return u, info.as_dict()
 
     def compliance(self, xphys, E0=1.0, penalty=3.0, rho_min=1e-3, tol=1e-8, max_iterations=0,
-                   want_u=False):
+                   want_u=False, guess=None):
+        """simp_compliance; guess (full displacement) warm-starts the solve."""
         x = _f64(xphys, self.nvox)
         c = C.c_double(0)
         dc = np.zeros(self.nvox)
         u = np.zeros(self.ndof) if want_u else None
         info = CgInfo()
-        code = lib().topohip_simp_compliance(self._h, _p(x), C.c_double(E0), C.c_double(penalty),
-                                             C.c_double(rho_min), C.c_double(tol),
-                                             int(max_iterations), C.byref(c), _p(dc), _p(u),
-                                             C.byref(info))
+        if guess is None:
+            code = lib().topohip_simp_compliance(self._h, _p(x), C.c_double(E0),
+                                                 C.c_double(penalty), C.c_double(rho_min),
+                                                 C.c_double(tol), int(max_iterations), C.byref(c),
+                                                 _p(dc), _p(u), C.byref(info))
+        else:
+            g = _f64(guess, self.ndof)
+            code = lib().topohip_simp_compliance_warm(self._h, _p(x), C.c_double(E0),
+                                                      C.c_double(penalty), C.c_double(rho_min),
+                                                      C.c_double(tol), int(max_iterations),
+                                                      _p(g), C.byref(c), _p(dc), _p(u),
+                                                      C.byref(info))
         _check(code, info)
         return c.value, dc, u, info.as_dict()
+
+    def von_mises(self, youngs=1.0, u=None):
+        """fea_von_mises_field: centroid von Mises per solid voxel (0 on empty
+        voxels) of u (default: the displacement of the last solve)."""
+        uu = None if u is None else _f64(u, self.ndof)
+        out = np.zeros(self.nvox)
+        _check(lib().topohip_von_mises_field(self._h, C.c_double(youngs), _p(uu), _p(out)))
+        return out
+
+    def fields_f32(self, youngs=1.0, u=None):
+        """fields.bin's float32 arrays: (von Mises, displacement)."""
+        uu = None if u is None else _f64(u, self.ndof)
+        vm = np.zeros(self.nvox, dtype=np.float32)
+        du = np.zeros(self.ndof, dtype=np.float32)
+        _check(lib().topohip_fields_f32(self._h, C.c_double(youngs), _p(uu), _p(vm), _p(du)))
+        return vm, du
+
+    def density_f16(self, density=None):
+        """Binary16 bits (uint16) of float(density) per voxel (default: the
+        physical density of the last optimisation)."""
+        d = None if density is None else _f64(density, self.nvox)
+        out = np.zeros(self.nvox, dtype=np.uint16)
+        _check(lib().topohip_density_f16(self._h, _p(d), _p(out)))
+        return out
+
+    def write_density_snapshot(self, directory, basename, rung, iteration, boundary=False,
+                               density=None, origin=(0.0, 0.0, 0.0)):
+        d = None if density is None else _f64(density, self.nvox)
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        _check(lib().topohip_write_density_snapshot(self._h, str(directory).encode(),
+                                                    str(basename).encode(), _p(o), _p(d),
+                                                    C.c_size_t(rung), int(iteration),
+                                                    int(bool(boundary))))
+
+    def set_observer(self, fn):
+        """fn(IterationObs) after every design iteration (None clears)."""
+        from .records import ITERATION_CB
+        if fn is None:
+            self._observer = None
+            _check(lib().topohip_set_iteration_observer(self._h, None, None))
+            return
+        cb = ITERATION_CB(lambda o, _u: fn(o.contents))
+        self._observer = cb  # keep the trampoline alive
+        _check(lib().topohip_set_iteration_observer(self._h, cb, None))
 
     def _mask(self, mask):
         m = np.ascontiguousarray(mask, dtype=np.uint8)
@@ -335,6 +388,12 @@ class Problem:
         ms = C.c_double(0)
         _check(lib().topohip_bench_ku(self._h, int(precision), int(reps), C.byref(ms)))
         return ms.value
+
+    def bench_fields(self, reps=50):
+        """ms per launch: (von Mises -> f32, displacement -> f32, density -> f16)."""
+        ms = (C.c_double * 3)()
+        _check(lib().topohip_bench_fields(self._h, int(reps), ms))
+        return tuple(ms)
 
 
 class Server:
@@ -426,3 +485,9 @@ def fea_solve_mgcg_matfree(nx, ny, nz, youngs_per_voxel, poisson, bcs, loads, to
 
 def version():
     return lib().topohip_version().decode()
+
+
+from .records import (CsvWriter, DesignVariant, FieldsVariant, GridFrame,  # noqa: E402
+                      IterationObs, design_fingerprint, format_progress, format_variant,
+                      iterations_csv_header, iterations_csv_row, read_density_f16,
+                      read_design_file, write_design_file, write_fields_file)

@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "libtopopt_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["ku.hip", "vec.hip", "galerkin.hip", "simp.hip", "stencil.hip", "coarse.hip", "engine.cpp", "capi.cpp", "server.cpp"]
+SOURCES = ["ku.hip", "vec.hip", "galerkin.hip", "simp.hip", "stencil.hip", "coarse.hip", "fields.hip", "engine.cpp", "capi.cpp", "server.cpp", "formats.cpp"]
 FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
          # packed FP32 arithmetic stays off: plain scalar FP32 only
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

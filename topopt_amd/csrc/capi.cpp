@@ -6,6 +6,7 @@
 #include <string>
 
 #include "engine.hpp"
+#include "formats.hpp"
 
 using namespace topohip;
 
@@ -206,6 +207,31 @@ int topohip_simp_compliance(topohip_problem* p, const double* xphys, double E0, 
   });
 }
 
+int topohip_simp_compliance_warm(topohip_problem* p, const double* xphys, double E0,
+                                 double penalty, double rho_min, double tol, int max_it,
+                                 const double* guess, double* c, double* dc, double* u_out,
+                                 topohip_cginfo* info) {
+  return guarded([&] {
+    need(p, "problem"); need(xphys, "xphys"); need(c, "c");
+    Problem& P = *p->P;
+    const long long nv = P.g.elems(), nd = P.g.ndof();
+    P.xphys.ensure(nv);
+    TOPOHIP_CHECK(hipMemcpyAsync(P.xphys.p, xphys, nv * sizeof(double), hipMemcpyHostToDevice, P.s));
+    DBuf<double> dg;
+    if (guess) {
+      dg.ensure(nd);
+      TOPOHIP_CHECK(hipMemcpyAsync(dg.p, guess, nd * sizeof(double), hipMemcpyHostToDevice, P.s));
+    }
+    *c = simp_compliance_dev(P, P.xphys.p, E0, penalty, rho_min, tol, max_it, info,
+                             guess ? dg.p : nullptr);
+    if (dc)
+      TOPOHIP_CHECK(hipMemcpyAsync(dc, P.dc.p, nv * sizeof(double), hipMemcpyDeviceToHost, P.s));
+    if (u_out)
+      TOPOHIP_CHECK(hipMemcpyAsync(u_out, P.u.p, nd * sizeof(double), hipMemcpyDeviceToHost, P.s));
+    TOPOHIP_CHECK(hipStreamSynchronize(P.s));
+  });
+}
+
 int topohip_filter(topohip_problem* p, double radius, const double* x, double* out, int adjoint) {
   return guarded([&] {
     need(p, "problem"); need(x, "x"); need(out, "out");
@@ -351,6 +377,216 @@ int topohip_bench_ku(topohip_problem* p, int precision, int reps, double* ms_per
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   });
+}
+
+// ---- result fields ------------------------------------------------------------
+
+int topohip_bench_fields(topohip_problem* p, int reps, double* ms3) {
+  return guarded([&] {
+    need(p, "problem"); need(ms3, "ms3");
+    if (!p->bench_ready) throw Error(TOPOHIP_EINVAL, "call topohip_bench_prepare first");
+    if (reps < 1) throw Error(TOPOHIP_EINVAL, "reps must be >= 1");
+    Problem& P = *p->P;
+    const long long nv = P.g.elems(), nd = P.g.ndof();
+    P.fvm32.ensure(static_cast<size_t>(nv));
+    P.fu32.ensure(static_cast<size_t>(nd));
+    P.fh16.ensure(static_cast<size_t>(nv));
+    const double c = 1.0 / ((1.0 + P.nu) * (1.0 - 2.0 * P.nu));
+    const VmCoef k{(2.0 / P.h) * 0.125, c * (1.0 - P.nu), c * P.nu, c * (1.0 - 2.0 * P.nu) / 2.0};
+    const uint8_t* sol = P.all_solid ? nullptr : P.solid.p;
+    hipEvent_t e0, e1;
+    TOPOHIP_CHECK(hipEventCreate(&e0));
+    TOPOHIP_CHECK(hipEventCreate(&e1));
+    for (int which = 0; which < 3; ++which) {
+      auto run = [&]() {
+        if (which == 0) launch_von_mises(P.g, k, sol, p->bu64.p, nullptr, P.fvm32.p, P.s);
+        else if (which == 1) launch_narrow_f32(p->bu64.p, P.fu32.p, nd, P.s);
+        else launch_density_f16(p->bu64.p, P.fh16.p, nv, P.s);
+      };
+      run();
+      TOPOHIP_CHECK(hipEventRecord(e0, P.s));
+      for (int i = 0; i < reps; ++i) run();
+      TOPOHIP_CHECK(hipEventRecord(e1, P.s));
+      TOPOHIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      TOPOHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      ms3[which] = static_cast<double>(ms) / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  });
+}
+
+int topohip_von_mises_field(topohip_problem* p, double youngs, const double* u, double* vm_out) {
+  return guarded([&] {
+    need(p, "problem"); need(vm_out, "vm_out");
+    result_fields(*p->P, youngs, u, vm_out, nullptr, nullptr);
+  });
+}
+
+int topohip_fields_f32(topohip_problem* p, double youngs, const double* u, float* vm_out,
+                       float* disp_out) {
+  return guarded([&] {
+    need(p, "problem");
+    result_fields(*p->P, youngs, u, nullptr, vm_out, disp_out);
+  });
+}
+
+int topohip_density_f16(topohip_problem* p, const double* density, uint16_t* out) {
+  return guarded([&] {
+    need(p, "problem"); need(out, "out");
+    density_halves(*p->P, density, out);
+  });
+}
+
+int topohip_set_iteration_observer(topohip_problem* p, topohip_iteration_cb cb, void* user) {
+  return guarded([&] {
+    need(p, "problem");
+    p->P->observer = cb;
+    p->P->observer_user = cb ? user : nullptr;
+  });
+}
+
+// ---- run records ------------------------------------------------------------------
+
+const char* topohip_iterations_csv_header(void) { return kIterationCsvHeader; }
+
+int topohip_iterations_csv_row(size_t rung, const topohip_iteration_obs* obs, long long wall_ms,
+                               char* buf, size_t n, size_t* len) {
+  return guarded([&] {
+    need(obs, "obs"); need(buf, "buf");
+    const std::string row = iterations_csv_row(rung, *obs, wall_ms);
+    if (row.size() + 1 > n) throw Error(TOPOHIP_EINVAL, "buffer too small for the row");
+    std::memcpy(buf, row.c_str(), row.size() + 1);
+    if (len) *len = row.size();
+  });
+}
+
+}  // extern "C"
+
+struct topohip_csv_writer {
+  std::unique_ptr<CsvWriter> w;
+};
+
+extern "C" {
+
+int topohip_csv_open(const char* path, topohip_csv_writer** out) {
+  return guarded([&] {
+    need(path, "path"); need(out, "out");
+    auto h = std::make_unique<topohip_csv_writer>();
+    h->w = std::make_unique<CsvWriter>(path);
+    *out = h.release();
+  });
+}
+
+int topohip_csv_append(topohip_csv_writer* w, size_t rung, const topohip_iteration_obs* obs) {
+  return guarded([&] {
+    need(w, "writer"); need(obs, "obs");
+    w->w->append_at(rung, *obs, wall_clock_ms());
+  });
+}
+
+int topohip_csv_append_at(topohip_csv_writer* w, size_t rung, const topohip_iteration_obs* obs,
+                          long long wall_ms) {
+  return guarded([&] {
+    need(w, "writer"); need(obs, "obs");
+    w->w->append_at(rung, *obs, wall_ms);
+  });
+}
+
+size_t topohip_csv_rows(const topohip_csv_writer* w) { return w ? w->w->rows() : 0; }
+
+int topohip_csv_close(topohip_csv_writer* w) {
+  return guarded([&] { delete w; });
+}
+
+int topohip_write_density_snapshot(topohip_problem* p, const char* dir, const char* basename,
+                                   const double* origin, const double* density, size_t rung,
+                                   int iteration, int boundary) {
+  return guarded([&] {
+    need(p, "problem"); need(dir, "dir"); need(basename, "basename");
+    Problem& P = *p->P;
+    std::vector<uint16_t> halves(static_cast<size_t>(P.g.elems()));
+    density_halves(P, density, halves.data());
+    const double zero[3] = {0.0, 0.0, 0.0};
+    write_density_snapshot(dir, basename, P.g.ex, P.g.ey, P.g.ez, P.h, origin ? origin : zero,
+                           halves, rung, iteration, boundary != 0);
+  });
+}
+
+int topohip_read_density_f16(const char* path, size_t count, float* out) {
+  return guarded([&] {
+    need(path, "path"); need(out, "out");
+    const std::vector<float> v = read_density_f16(path, count);
+    std::copy(v.begin(), v.end(), out);
+  });
+}
+
+uint64_t topohip_design_fingerprint(const double* density, size_t n) {
+  return density ? design_fingerprint(density, n) : design_fingerprint(nullptr, 0);
+}
+
+int topohip_write_design_file(const char* path, const topohip_grid_frame* grid, int n_variants,
+                              const topohip_design_variant* variants) {
+  return guarded([&] {
+    need(path, "path"); need(grid, "grid");
+    write_design_file(path, *grid, n_variants, variants);
+  });
+}
+
+int topohip_read_design_header(const char* path, topohip_grid_frame* grid, int* n_variants) {
+  return guarded([&] {
+    need(path, "path");
+    const DesignFile f = read_design_file(path);
+    if (grid) *grid = f.grid;
+    if (n_variants) *n_variants = static_cast<int>(f.meta.size());
+  });
+}
+
+int topohip_read_design_variant(const char* path, int index, topohip_design_variant* meta,
+                                double* density_out) {
+  return guarded([&] {
+    need(path, "path");
+    const DesignFile f = read_design_file(path);
+    if (index < 0 || index >= static_cast<int>(f.meta.size()))
+      throw Error(TOPOHIP_EINVAL, "design.bin: no variant " + std::to_string(index));
+    if (meta) {
+      *meta = f.meta[static_cast<size_t>(index)];
+      meta->density = nullptr;
+    }
+    if (density_out) {
+      const auto& d = f.density[static_cast<size_t>(index)];
+      std::copy(d.begin(), d.end(), density_out);
+    }
+  });
+}
+
+int topohip_write_fields_file(const char* path, const topohip_grid_frame* grid, int n_variants,
+                              const topohip_fields_variant* variants) {
+  return guarded([&] {
+    need(path, "path"); need(grid, "grid");
+    write_fields_file(path, *grid, n_variants, variants);
+  });
+}
+
+namespace {
+int copy_line(const std::string& line, char* buf, size_t n) {
+  if (!buf || line.size() + 1 > n) {
+    g_err = "buffer too small for the line";
+    return TOPOHIP_EINVAL;
+  }
+  std::memcpy(buf, line.c_str(), line.size() + 1);
+  return TOPOHIP_OK;
+}
+}  // namespace
+
+int topohip_format_progress(size_t rung, size_t rungs, int iteration, char* buf, size_t n) {
+  return copy_line(progress_line(rung, rungs, iteration), buf, n);
+}
+
+int topohip_format_variant(double vf, double achieved, double printed, double margin,
+                           const char* mesh, char* buf, size_t n) {
+  return copy_line(variant_line(vf, achieved, printed, margin, mesh ? mesh : ""), buf, n);
 }
 
 }  // extern "C"

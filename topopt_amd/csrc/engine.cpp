@@ -7,6 +7,7 @@
 // (core/src/fea/matfree.cpp:670-834). Vectors stay full-grid (3 per node) with
 // inactive DOFs held at zero, which is the reduced system embedded in the grid.
 #include "engine.hpp"
+#include "formats.hpp"
 
 
 #include <algorithm>
@@ -611,8 +612,12 @@ void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool
                                  hipMemcpyDeviceToDevice, P.s));
   }
   launch_dot<double, double>(P.b.p, P.b.p, nd, P.partials.p, db, nullptr, P.s, nc, P.pstride);
+  // A warm-started MG-PCG is not tested before its first iteration (the
+  // stopping test follows each update, core/src/fea/multigrid.cpp:1250-1300);
+  // a warm-started Jacobi-CG returns at once when its initial residual already
+  // meets the tolerance (core/src/fea/matfree.cpp:956-967).
   launch_cg_init(P.st.p, P.partials.p, db, tol, cap, P.s, nc, P.pstride);
-  if (warm) {
+  if (warm && pc == Prec::JACOBI) {
     launch_dot<double, double>(P.r.p, P.r.p, nd, P.partials.p, db, nullptr, P.s, nc, P.pstride);
     launch_cg_resume(P.st.p, P.partials.p, db, P.s, nc, P.pstride);
   }
@@ -1080,6 +1085,7 @@ void solve_cases(Problem& P, int nc, double tol, int max_it, const double* d_gue
   if (nc < 1 || nc > P.nrhs) throw Error(TOPOHIP_EINVAL, "no load case set");
   if (d_guess && nc != 1) throw Error(TOPOHIP_EINTERNAL, "warm start needs a single load case");
   ensure_cases(P, nc);
+  P.u_valid = false;
   Level& f = *P.L[0];
   if (P.nonzero_bc) {
     // rhs = f - K(E) up depends on the moduli: form it on the device, check
@@ -1227,6 +1233,7 @@ void solve_cases(Problem& P, int nc, double tol, int max_it, const double* d_gue
   }
   for (int r = 0; r < nc; ++r) launch_add(P.x.p + r * nd, P.up.p, P.u.p + r * nd, nd, P.s);
   sync(P);
+  P.u_valid = true;
   for (auto& di : d) {
     di.matvecs = P.matvecs - mv0;
     di.residual_replacements = P.replacements;
@@ -1287,7 +1294,8 @@ FilterCache& filter_for(Problem& P, double radius) {
 }
 
 double simp_compliance_dev(Problem& P, const double* d_xphys, double E0, double penalty,
-                           double rho_min, double tol, int max_it, topohip_cginfo* info) {
+                           double rho_min, double tol, int max_it, topohip_cginfo* info,
+                           const double* d_guess) {
   if (!(E0 > 0.0)) throw Error(TOPOHIP_EINVAL, "simp: youngs_modulus (E0) must be > 0");
   if (!(penalty > 0.0)) throw Error(TOPOHIP_EINVAL, "simp: penalty (p) must be > 0");
   if (!(rho_min > 0.0 && rho_min <= 1.0))
@@ -1302,7 +1310,9 @@ double simp_compliance_dev(Problem& P, const double* d_xphys, double E0, double 
   const int sb = sens_blocks(P.g);
   const long long nd = P.g.ndof();
   std::vector<topohip_cginfo> ci(static_cast<size_t>(P.nrhs));
-  solve_cases(P, P.nrhs, tol, max_it, nullptr, ci.data());
+  if (d_guess && P.nrhs != 1)
+    throw Error(TOPOHIP_EINVAL, "simp_compliance: an initial guess needs a single load case");
+  solve_cases(P, P.nrhs, tol, max_it, d_guess, ci.data());
   // Multi-load objective: c = sum_r c_r, dc = sum_r dc_r.
   double c = 0.0;
   topohip_cginfo agg = ci[0];
@@ -1524,14 +1534,17 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
     }
   };
   int it = 0;
+  double t_prev_obs = -1.0;  // end of the previous iteration's observation
   for (const Stage& st : plan) {
     bool stage_converged = false;
     for (int si = 0; si < st.iterations; ++si, ++it) {
-      double ta = now_ms();
+      const double t_top = now_ms();
+      double ta = t_top;
       physical(st);
       if (mask) launch_mask_pins(P.xphys.p, d_eff.p, nv, P.s);
       sync(P);
-      t_filter += now_ms() - ta;
+      const double ms_filter = now_ms() - ta;
+      t_filter += ms_filter;
       ta = now_ms();
       topohip_cginfo ci{};
       double c = 0.0;
@@ -1543,7 +1556,8 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
         r.non_convergent = 1;
         break;
       }
-      t_solve += now_ms() - ta;
+      const double ms_solve = now_ms() - ta;
+      t_solve += ms_solve;
       t_build += ci.t_mg_build_ms;
       ta = now_ms();
       double change = 0.0;
@@ -1554,7 +1568,8 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
         oc_update_core(P, fc, sol, nd, mask != nullptr, P.xd.p, P.dc.p, o.volfrac, st.move,
                        o.rho_min, P.xnew.p, nullptr, nullptr, nullptr, &change);
       std::swap(P.xd.p, P.xnew.p);
-      t_oc += now_ms() - ta;
+      const double ms_update = now_ms() - ta;
+      t_oc += ms_update;
       if (r.iterations == 0) r.initial_compliance = c;
       ++r.iterations;
       r.cg_iterations_total += ci.iterations;
@@ -1563,13 +1578,54 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
       launch_sum_vec<double>(P.xphys.p, nv, P.partials.p, vb, P.s);
       launch_sum_partials(P.partials.p, vb, P.scal.p, P.s);
       d2h(P.scal_h, P.scal.p, 1, P.s);
-      t_filter += now_ms() - ta;
+      const double ms_analysis = now_ms() - ta;
+      t_filter += ms_analysis;
       const double vf = nd > 0 ? P.scal_h[0] / nd : 0.0;
       if (history) {
         history[4 * it + 0] = c;
         history[4 * it + 1] = change;
         history[4 * it + 2] = vf;
         history[4 * it + 3] = ci.iterations;
+      }
+      if (P.observer) {
+        // SimpIterationObservation + IterationPhaseTimes
+        // (core/include/topopt/simp.hpp:870-1000). The projection and mask
+        // pins run inside the filter's launch sequence: their time is in
+        // filter_ms. The sensitivity sweep is solve_ms minus the linear solve.
+        ta = now_ms();
+        topohip_iteration_obs ob{};
+        ob.iteration = it + 1;
+        ob.compliance = c;
+        ob.change = change;
+        ob.volume_fraction = vf;
+        ob.cg_iterations = ci.iterations;
+        ob.cg_used_multigrid = ci.used_multigrid;
+        ob.cg_mg_levels = ci.mg_levels;
+        ob.cg_hier_built = ci.hier_built;
+        ob.cg_mg_cycles_attempted = ci.mg_cycles_attempted;
+        ob.beta = st.project ? st.beta : 0.0;
+        ob.active_fraction = 1.0;
+        ob.tail_prev_ms = t_prev_obs < 0.0 ? 0.0 : t_top - t_prev_obs;
+        ob.filter_ms = ms_filter;
+        ob.solve_ms = ms_solve;
+        ob.fea_ms = ci.t_total_ms;
+        ob.sens_ms = ms_solve > ci.t_total_ms ? ms_solve - ci.t_total_ms : 0.0;
+        ob.update_ms = ms_update;
+        ob.analysis_ms = ms_analysis;
+        ob.solver_build_ms = ci.t_build_ms;
+        ob.solver_mg_build_ms = ci.t_mg_build_ms;
+        ob.solver_mg_ms = ci.t_mg_ms;
+        ob.solver_cg_ms = ci.t_cg_ms;
+        ob.fea_solves = 1;
+        ob.matvecs = ci.matvecs;
+        sample_process_memory(ob);
+        const double t_obs = now_ms();
+        ob.observe_ms = t_obs - ta;
+        ob.total_ms = t_obs - t_top;
+        ob.residual_ms = ob.total_ms - (ob.filter_ms + ob.project_ms + ob.solve_ms +
+                                        ob.update_ms + ob.analysis_ms + ob.observe_ms);
+        P.observer(&ob, P.observer_user);
+        t_prev_obs = now_ms();
       }
       if (change < o.change_tol) {
         stage_converged = true;
@@ -1609,6 +1665,58 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
   r.t_filter_ms = t_filter;
   r.t_oc_ms = t_oc;
   if (res) *res = r;
+}
+
+// fea_von_mises_field (core/src/fea/assembly.cpp:976-996) and the float
+// arrays of fields.bin (core/src/io/fields.cpp:107-118), formed on the device.
+void result_fields(Problem& P, double E, const double* u_host, double* vm64, float* vm32,
+                   float* disp32) {
+  const double nu = P.nu, h = P.h;
+  if (!(E > 0.0)) throw Error(TOPOHIP_EINVAL, "hex8_stress: youngs_modulus must be > 0");
+  if (!(h > 0.0)) throw Error(TOPOHIP_EINVAL, "hex8_stress: element_size must be > 0");
+  if (!(nu > -1.0 && nu < 0.5))
+    throw Error(TOPOHIP_EINVAL, "hex8_stress: poisson must be in (-1, 0.5)");
+  const long long nv = P.g.elems(), nd = P.g.ndof();
+  const double* du = P.u.p;
+  if (u_host) {
+    P.fu.ensure(static_cast<size_t>(nd));
+    h2d(P.fu.p, u_host, static_cast<size_t>(nd), P.s);
+    du = P.fu.p;
+  } else if (!P.u_valid) {
+    throw Error(TOPOHIP_EINVAL, "no displacement: pass u or solve first");
+  }
+  // D and the centroid derivative as hex8_stress forms them (:258-266, :270-281)
+  const double c = E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  VmCoef k{(2.0 / h) * 0.125, c * (1.0 - nu), c * nu, c * (1.0 - 2.0 * nu) / 2.0};
+  if (vm64) P.fvm.ensure(static_cast<size_t>(nv));
+  if (vm32) P.fvm32.ensure(static_cast<size_t>(nv));
+  if (vm64 || vm32)
+    launch_von_mises(P.g, k, P.all_solid ? nullptr : P.solid.p, du, vm64 ? P.fvm.p : nullptr,
+                     vm32 ? P.fvm32.p : nullptr, P.s);
+  if (disp32) {
+    P.fu32.ensure(static_cast<size_t>(nd));
+    launch_narrow_f32(du, P.fu32.p, nd, P.s);
+  }
+  if (vm64) d2h(vm64, P.fvm.p, static_cast<size_t>(nv), P.s);
+  if (vm32) d2h(vm32, P.fvm32.p, static_cast<size_t>(nv), P.s);
+  if (disp32) d2h(disp32, P.fu32.p, static_cast<size_t>(nd), P.s);
+  sync(P);
+}
+
+// The halves of write_density_snapshot (core/src/simp/observability.cpp:374-377).
+void density_halves(Problem& P, const double* d_host, uint16_t* out) {
+  const long long nv = P.g.elems();
+  const double* d = P.xphys.p;
+  if (d_host) {
+    P.fu.ensure(static_cast<size_t>(nv));
+    h2d(P.fu.p, d_host, static_cast<size_t>(nv), P.s);
+    d = P.fu.p;
+  } else if (!d || P.xphys.n < static_cast<size_t>(nv)) {
+    throw Error(TOPOHIP_EINVAL, "no physical density: pass one or optimise first");
+  }
+  P.fh16.ensure(static_cast<size_t>(nv));
+  launch_density_f16(d, P.fh16.p, nv, P.s);
+  d2h(out, P.fh16.p, static_cast<size_t>(nv), P.s);
 }
 
 }  // namespace topohip

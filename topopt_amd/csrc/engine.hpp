@@ -93,6 +93,7 @@ struct Problem {
   // PCG vectors
   // PCG vectors, nb stacked load cases each (stride ndof)
   DBuf<double> b, x, r, p, Ap64, z64, rt, u;
+  bool u_valid = false;  // P.u holds the displacements of a completed solve
   DBuf<float> p32, Ap32, r32, z32;
   DBuf<double> partials, partials2;  // nb cases x pstride block partial sums
   DBuf<CgState> st;
@@ -107,6 +108,13 @@ struct Problem {
   DBuf<double> xd, xphys, dc, xnew, dcf;
   DBuf<double> xtil, dpr, dcp, dvf, xcand, xcf;  // projected OC work
   DBuf<uint8_t> active;
+
+  // result fields and run records
+  DBuf<double> fu, fvm;
+  DBuf<float> fvm32, fu32;
+  DBuf<uint16_t> fh16;
+  topohip_iteration_cb observer = nullptr;  // called after every design iteration
+  void* observer_user = nullptr;
 
   int nb = 0;        // stacked load-case capacity of the solver buffers
   int pstride = 0;   // partial sums per case
@@ -135,8 +143,10 @@ void build_hierarchy(Problem& P, Timers& t);
 void solve_cases(Problem& P, int nc, double tol, int max_it, const double* d_guess,
                  topohip_cginfo* info);
 FilterCache& filter_for(Problem& P, double radius);
+// d_guess (nullable, single load case): warm-start displacement (3 * nodes).
 double simp_compliance_dev(Problem& P, const double* d_xphys, double E0, double penalty,
-                           double rho_min, double tol, int max_it, topohip_cginfo* info);
+                           double rho_min, double tol, int max_it, topohip_cginfo* info,
+                           const double* d_guess = nullptr);
 void oc_update_dev(Problem& P, double radius, const double* d_x, const double* d_dc,
                    double volfrac, double move, double rho_min, double* d_xnew,
                    uint8_t* d_active, int* bisections, double* lambda, double* change);
@@ -157,5 +167,12 @@ std::unique_ptr<FilterCache> make_filter(Problem& P, double radius, const uint8_
 void design_mask_of(const Problem& P, const uint8_t* mask, std::vector<uint8_t>& eff,
                     std::vector<uint8_t>& dset, long long& n_design);
 void hex8_stiffness_host(double E, double nu, double h, double* K576);
+// fea_von_mises_field of u_host (3 * nodes; null = P.u of the last solve) for
+// Young's modulus E: FP64 field into vm64 and/or its float narrowing into
+// vm32, and the displacement narrowed to float into disp32 (outputs nullable).
+void result_fields(Problem& P, double E, const double* u_host, double* vm64, float* vm32,
+                   float* disp32);
+// Binary16 halves of float(d) per voxel (d_host null = P.xphys).
+void density_halves(Problem& P, const double* d_host, uint16_t* out);
 
 }  // namespace topohip

@@ -1,0 +1,71 @@
+// formats.hpp — the run records a design loop leaves behind, byte-compatible
+// with the reference: iterations.csv, density snapshots (.f16 + .json),
+// design.bin v1, fields.bin v1 and the CLI's PROGRESS / VARIANT stdout lines.
+// Host code only; the arrays they carry are formed on the device (fields.hip).
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/topopt_hip.h"
+
+namespace topohip {
+
+// kIterationCsvHeader, core/src/simp/observability.cpp:252-262.
+extern const char kIterationCsvHeader[];
+
+// IterationCsvWriter::append_at's row, core/src/simp/observability.cpp:282-330.
+std::string iterations_csv_row(std::size_t rung, const topohip_iteration_obs& o,
+                               long long wall_ms);
+
+// Epoch milliseconds from the system clock (the CSV's wall_ms column).
+long long wall_clock_ms();
+
+// process_memory's Linux branch (core/src/simp/observability.cpp:116-205) into
+// the memory columns of o.
+void sample_process_memory(topohip_iteration_obs& o);
+
+class CsvWriter {
+ public:
+  explicit CsvWriter(const std::string& path);
+  ~CsvWriter();
+  CsvWriter(const CsvWriter&) = delete;
+  CsvWriter& operator=(const CsvWriter&) = delete;
+  void append_at(std::size_t rung, const topohip_iteration_obs& o, long long wall_ms);
+  std::size_t rows() const { return rows_; }
+
+ private:
+  std::FILE* f_ = nullptr;
+  std::string path_;
+  std::size_t rows_ = 0;
+};
+
+void write_density_snapshot(const std::string& dir, const std::string& basename, int nx, int ny,
+                            int nz, double spacing, const double origin[3],
+                            const std::vector<std::uint16_t>& halves, std::size_t rung,
+                            int iteration, bool boundary);
+std::vector<float> read_density_f16(const std::string& path, std::size_t count);
+float half_to_float(std::uint16_t h);
+
+std::uint64_t design_fingerprint(const double* d, std::size_t n);
+
+void write_design_file(const std::string& path, const topohip_grid_frame& g, int n,
+                       const topohip_design_variant* v);
+
+struct DesignFile {
+  topohip_grid_frame grid{};
+  std::vector<topohip_design_variant> meta;
+  std::vector<std::vector<double>> density;
+};
+DesignFile read_design_file(const std::string& path);
+
+void write_fields_file(const std::string& path, const topohip_grid_frame& g, int n,
+                       const topohip_fields_variant* v);
+
+std::string progress_line(std::size_t rung, std::size_t rungs, int iteration);
+std::string variant_line(double vf, double achieved, double printed, double margin,
+                         const std::string& mesh);
+
+}  // namespace topohip

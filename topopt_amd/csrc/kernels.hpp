@@ -233,3 +233,21 @@ void launch_stencil_apply(const Geom& g, const T* S, const uint8_t* mask, const 
                           const T* b, const T* dinv, double omega, int epi, const int* done,
                           hipStream_t s, int nr = 1);
 }  // namespace topohip
+
+namespace topohip {
+// ---- result fields (fields.hip) -----------------------------------------------
+// Centroid stress constants of the isotropic hex8: q = (2/h)/8 is |dN/dx| at
+// the centroid; D has d0 on the normal diagonal, d1 off it, gs on the shear
+// diagonal (core/src/fea/hex_element.cpp:242-316).
+struct VmCoef {
+  double q, d0, d1, gs;
+};
+// Von Mises at every solid voxel's centroid from the full displacement u
+// (3 * nodes, reference DOF order); 0 on empty voxels. Either output nullable.
+void launch_von_mises(const Geom& g, const VmCoef& c, const uint8_t* solid, const double* u,
+                      double* vm64, float* vm32, hipStream_t s);
+// out[i] = float(in[i]) (round to nearest even).
+void launch_narrow_f32(const double* in, float* out, long long n, hipStream_t s);
+// out[i] = IEEE half of float(in[i]), round to nearest even.
+void launch_density_f16(const double* in, uint16_t* out, long long n, hipStream_t s);
+}  // namespace topohip
