@@ -58,15 +58,19 @@ def ku_bytes(nx, ny, nz, word):
     return 2 * word * ndof + word * nx * ny * nz, ndof
 
 
-def run_design(ta, wl, precision, steps, warmup):
+def run_design(ta, wl, precision, steps, warmup, mma=False):
+    """One timed optimisation run (OC, or MMA with the plateau stop off so the
+    run has exactly `steps` iterations)."""
     nx, ny, nz = wl["grid"]
     with ta.Problem(nx, ny, nz, wl["bcs"], wl["loads"], precision=precision) as p:
         kw = dict(penalty=wl["penalty"], move=wl["move"], cg_tol=wl["tol"],
                   continuation=wl.get("continuation"))
+        run = (lambda n: p.optimize_mma(wl["volfrac"], wl["rmin"], n, plateau_window=0, **kw)) \
+            if mma else (lambda n: p.optimize(wl["volfrac"], wl["rmin"], n, **kw))
         if warmup > 0:
-            p.optimize(wl["volfrac"], wl["rmin"], warmup, **kw)
+            run(warmup)
         t0 = time.perf_counter()
-        out = p.optimize(wl["volfrac"], wl["rmin"], steps, **kw)
+        out = run(steps)
         dt = time.perf_counter() - t0
     return out, dt
 
@@ -163,6 +167,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="mbb_192x64x64")
     ap.add_argument("--no-fp64-form", action="store_true")
+    ap.add_argument("--no-mma", action="store_true", help="skip the MMA-updater leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--jobs", type=int, default=128)
     ap.add_argument("--rate", type=float, default=4.0)
@@ -220,6 +225,13 @@ def main():
         fp64 = {"value": v64, "ratio": value / v64, "compliance": out64["compliance"],
                 "compliance_rel_diff_vs_mixed": rel, "cg_iterations_total": out64["cg_iterations_total"]}
         print(f"# fp64 form: {v64:.2f} design iters/s", file=sys.stderr, flush=True)
+    mma = None
+    if not a.no_mma:
+        outm, dtm = run_design(ta, wl, ta.PREC_MIXED, a.steps, min(a.warmup, 1), mma=True)
+        mma = {"value": outm["iterations"] / dtm, "compliance": outm["compliance"],
+               "cg_iterations_total": outm["cg_iterations_total"], "t_oc_ms": outm["t_oc_ms"]}
+        print(f"# mma updater: {mma['value']:.2f} design iters/s, compliance "
+              f"{outm['compliance']:.6g}", file=sys.stderr, flush=True)
     ku = ku_measure(ta, nx, ny, nz)
     fields = fields_measure(ta, nx, ny, nz)
     print("# fields " + ", ".join(f"{k} {v['us']:.1f} us {v['GBs']:.0f} GB/s"
@@ -252,6 +264,7 @@ def main():
                    "t_solve_ms": out["t_solve_ms"], "t_build_ms": out["t_build_ms"],
                    "t_oc_ms": out["t_oc_ms"], "t_filter_ms": out["t_filter_ms"]},
         "fp64_form": fp64,
+        "mma_updater": mma,
         "ku": ku,
         "result_fields": fields,
         "roofline": {"bound": "hbm", "achieved": ku["f32"]["GBs"], "peak": HBM_PEAK_GBS,

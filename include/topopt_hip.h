@@ -168,6 +168,39 @@ typedef struct {
 int topohip_optimize(topohip_problem* p, const topohip_options* opt, double* design_out,
                      double* xphys_out, double* history, topohip_result* result);
 
+/* ---- MMA updater (SimpUpdater::MMA, core/include/topopt/simp.hpp:622-639) ---- */
+
+/* The plateau stop of an MMA run (SimpOptions::mma_plateau_*,
+ * core/include/topopt/simp.hpp:1256-1276); plateau_window <= 0 keeps the
+ * change_tol test. Defaults: 10, 1e-3, 0.05, 3. */
+typedef struct {
+  int plateau_window;
+  double plateau_tol, plateau_min_drop;
+  int plateau_flat_windows;
+} topohip_mma_options;
+
+/* simp_optimize with options.updater = MMA (core/src/simp/simp.cpp:2305-2311):
+ * the plain (unprojected, unmasked) loop with mma_update as the design update
+ * and stage_should_stop (:1219-1232) as its stop rule. mma NULL = the
+ * reference's defaults. Outputs as topohip_optimize. */
+int topohip_optimize_mma(topohip_problem* p, const topohip_options* opt,
+                         const topohip_mma_options* mma, double* design_out, double* xphys_out,
+                         double* history, topohip_result* result);
+
+/* One mma_update (core/src/simp/simp.cpp:1433-1488, core/src/simp/mma_joint.cpp:
+ * 18-118) on the handle's solid voxels with the density filter of `radius`:
+ * mma_iter is 1-based and the handle keeps the state (asymptotes, last two
+ * designs) between calls, as MmaState does; restart a run at mma_iter 1.
+ * lambda (nullable): the volume multiplier; change (nullable): max |x_new - x|. */
+int topohip_mma_update(topohip_problem* p, double radius, int mma_iter, const double* x,
+                       const double* dc, double volfrac, double move, double rho_min,
+                       double* x_new, double* lambda, double* change);
+
+/* mma_objective_plateau (core/src/simp/simp.cpp:1081-1134) on a compliance
+ * history c[0..n); *out = 1 when it fires. */
+int topohip_mma_plateau(const double* c, int n, int window, double rel_tol, double min_drop,
+                        int flat_windows, int* out);
+
 /* The design loop with Heaviside projection stages (SimpOptions::projection /
  * projection_eta, core/include/topopt/simp.hpp:518-524; loop
  * core/src/simp/simp.cpp:2018-2608 with build_stage_plan :1053-1068): stage s

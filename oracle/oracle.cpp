@@ -1398,6 +1398,11 @@ SimpResult simp_optimize(const Grid& g, const SimpParams& p,
       for (int i = 0; i < g.nx; ++i)
         if (g.is_solid(i, j, k)) x[g.index(i, j, k)] = o.volume_fraction;
   SimpResult res;
+  if (o.mma && !o.projection.empty())
+    throw std::invalid_argument("simp_optimize: projection requires the OC updater");
+  MmaState mma_state;
+  int mma_iter = 0;
+  std::vector<double> curve;  // compliance history for the MMA plateau stop
   for (const Stage& st : plan) {
     bool stage_converged = false;
     for (int it = 0; it < st.iterations; ++it) {
@@ -1419,6 +1424,9 @@ SimpResult simp_optimize(const Grid& g, const SimpParams& p,
         const std::vector<double> xtilde = f.filter_density(x);
         xn = oc_update_projected(g, f, x, xtilde, c.dc, st.beta, eta, o.volume_fraction, st.move,
                                  p.density_min);
+      } else if (o.mma) {
+        xn = mma_update(mma_state, ++mma_iter, g, f, x, c.dc, o.volume_fraction, st.move,
+                        p.density_min);
       } else {
         xn = oc_update(g, f, x, c.dc, o.volume_fraction, st.move, p.density_min);
       }
@@ -1430,7 +1438,14 @@ SimpResult simp_optimize(const Grid& g, const SimpParams& p,
       std::vector<double> xa = f.filter_density(x);
       if (st.project) project_solid(g, xa, st.beta, eta);
       res.history.push_back({c.compliance, change, volfrac(xa), c.cg.iterations});
-      if (change < o.change_tol) { stage_converged = true; break; }
+      curve.push_back(c.compliance);
+      // stage_should_stop, core/src/simp/simp.cpp:1219-1232
+      const bool stop = (o.mma && o.mma_plateau_window > 0)
+                            ? mma_objective_plateau(curve, o.mma_plateau_window, o.mma_plateau_tol,
+                                                    o.mma_plateau_min_drop,
+                                                    o.mma_plateau_flat_windows)
+                            : change < o.change_tol;
+      if (stop) { stage_converged = true; break; }
     }
     res.converged = stage_converged;
     if (res.non_convergent) break;
@@ -1676,6 +1691,144 @@ std::uint64_t fnv1a64(const void* data, std::size_t n) {
     h *= 1099511628211ULL;
   }
   return h;
+}
+
+}  // namespace oracle
+
+namespace oracle {
+
+// core/src/simp/simp.cpp:1433-1488 and core/src/simp/mma_joint.cpp:18-118
+// (asyinit 0.5, asyincr 1.2, asydecr 0.7, albefa 0.1, raa0 1e-5, move as a
+// fraction of xrange, the 1-D dual bisection).
+std::vector<double> mma_update(MmaState& st, int mma_iter, const Grid& g, const DensityFilter& f,
+                               const std::vector<double>& x, const std::vector<double>& dcomp,
+                               double volfrac, double move, double density_min,
+                               double* lambda_out) {
+  const std::size_t N = g.voxels();
+  if (x.size() != N || dcomp.size() != N)
+    throw std::invalid_argument("mma_update: vector size != voxel_count");
+  std::vector<double> dc = f.filter_sensitivity(dcomp);
+  std::vector<double> ones(N, 0.0);
+  std::vector<std::size_t> dof;
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i)
+        if (g.is_solid(i, j, k)) {
+          const std::size_t e = g.index(i, j, k);
+          ones[e] = 1.0;
+          dof.push_back(e);
+        }
+  const std::vector<double> dv = f.filter_sensitivity(ones);
+  const double target = volfrac * static_cast<double>(dof.size());
+  double scale = 0.0;
+  for (std::size_t e : dof) scale = std::max(scale, std::fabs(dc[e]));
+  if (scale > 0.0)
+    for (std::size_t e : dof) dc[e] /= scale;
+  const std::vector<double> xphys = f.filter_density(x);
+  double vol = 0.0;
+  for (double v : xphys) vol += v;
+  const double g0 = vol - target;
+
+  const double asyinit = 0.5, asyincr = 1.2, asydecr = 0.7, albefa = 0.1, raa0 = 1e-5;
+  const double xmin = density_min, xmax = 1.0, xrange = xmax - xmin;
+  if (st.low.size() != N) {
+    st.low.assign(N, 0.0);
+    st.upp.assign(N, 0.0);
+  }
+  for (std::size_t e : dof) {
+    const double xe = x[e];
+    if (mma_iter <= 2) {
+      st.low[e] = xe - asyinit * xrange;
+      st.upp[e] = xe + asyinit * xrange;
+    } else {
+      const double s = (xe - st.xold1[e]) * (st.xold1[e] - st.xold2[e]);
+      const double gam = s < 0.0 ? asydecr : (s > 0.0 ? asyincr : 1.0);
+      double L = xe - gam * (st.xold1[e] - st.low[e]);
+      double U = xe + gam * (st.upp[e] - st.xold1[e]);
+      L = std::min(L, xe - 0.01 * xrange);
+      L = std::max(L, xe - 10.0 * xrange);
+      U = std::max(U, xe + 0.01 * xrange);
+      U = std::min(U, xe + 10.0 * xrange);
+      st.low[e] = L;
+      st.upp[e] = U;
+    }
+  }
+  std::vector<double> p0(N, 0.0), q0(N, 0.0), p1(N, 0.0), q1(N, 0.0), al(N, 0.0), be(N, 0.0);
+  double b = -g0;
+  for (std::size_t e : dof) {
+    const double xe = x[e], L = st.low[e], U = st.upp[e];
+    const double ux1 = U - xe, xl1 = xe - L;
+    const double dcp = std::max(dc[e], 0.0), dcm = std::max(-dc[e], 0.0);
+    const double dvp = std::max(dv[e], 0.0), dvm = std::max(-dv[e], 0.0);
+    p0[e] = ux1 * ux1 * (1.001 * dcp + 0.001 * dcm + raa0 / xrange);
+    q0[e] = xl1 * xl1 * (0.001 * dcp + 1.001 * dcm + raa0 / xrange);
+    p1[e] = ux1 * ux1 * (1.001 * dvp + 0.001 * dvm + raa0 / xrange);
+    q1[e] = xl1 * xl1 * (0.001 * dvp + 1.001 * dvm + raa0 / xrange);
+    b += p1[e] / ux1 + q1[e] / xl1;
+    al[e] = std::max({xmin, L + albefa * (xe - L), xe - move * xrange});
+    be[e] = std::min({xmax, U - albefa * (U - xe), xe + move * xrange});
+  }
+  auto candidate = [&](double lam) {
+    std::vector<double> xn(N, 0.0);
+    for (std::size_t e : dof) {
+      const double pl = std::sqrt(p0[e] + lam * p1[e]);
+      const double ql = std::sqrt(q0[e] + lam * q1[e]);
+      double xt = (pl * st.low[e] + ql * st.upp[e]) / (pl + ql);
+      if (xt < al[e]) xt = al[e];
+      if (xt > be[e]) xt = be[e];
+      xn[e] = xt;
+    }
+    return xn;
+  };
+  auto gval = [&](const std::vector<double>& xc) {
+    double s = 0.0;
+    for (std::size_t e : dof) s += p1[e] / (st.upp[e] - xc[e]) + q1[e] / (xc[e] - st.low[e]);
+    return s - b;
+  };
+  double lambda = 0.0;
+  if (gval(candidate(0.0)) > 0.0) {
+    double l1 = 0.0, l2 = 1.0;
+    while (l2 < 1e30 && gval(candidate(l2)) > 0.0) l2 *= 2.0;
+    for (int it = 0; it < 100 && (l2 - l1) > 1e-9 * (1.0 + l1 + l2); ++it) {
+      const double lm = 0.5 * (l1 + l2);
+      if (gval(candidate(lm)) > 0.0) l1 = lm;
+      else l2 = lm;
+    }
+    lambda = 0.5 * (l1 + l2);
+  }
+  std::vector<double> xn = candidate(lambda);
+  st.xold2 = st.xold1;
+  st.xold1 = x;
+  if (lambda_out) *lambda_out = lambda;
+  return xn;
+}
+
+// core/src/simp/simp.cpp:1081-1134: running-minimum improvement over the
+// trailing window below rel_tol, behind the progress gate (or its flatness
+// escape).
+bool mma_objective_plateau(const std::vector<double>& c, int window, double rel_tol,
+                           double min_drop, int min_flat_windows) {
+  if (window <= 0) return false;
+  const int n = static_cast<int>(c.size());
+  if (n < window + 1) return false;
+  double best_now = c[0];
+  for (int i = 1; i < n; ++i) best_now = std::min(best_now, c[i]);
+  if (min_drop > 0.0 && !(best_now <= c[0] * (1.0 - min_drop))) {
+    bool escape = false;
+    if (min_flat_windows > 0) {
+      const int span = min_flat_windows * window;
+      if (n >= span + 1) {
+        double best_before = c[0];
+        for (int i = 1; i < n - span; ++i) best_before = std::min(best_before, c[i]);
+        if (best_before > 0.0) escape = (best_before - best_now) / best_before < rel_tol;
+      }
+    }
+    if (!escape) return false;
+  }
+  double best_prev = c[0];
+  for (int i = 1; i < n - window; ++i) best_prev = std::min(best_prev, c[i]);
+  if (!(best_prev > 0.0)) return false;
+  return (best_prev - best_now) / best_prev < rel_tol;
 }
 
 }  // namespace oracle

@@ -287,6 +287,13 @@ struct SimpOptions {
   // one unprojected stage of max_iterations at `move`.
   std::vector<ProjStage> projection;
   double projection_eta = 0.5;
+  // SimpOptions::updater == MMA and its plateau stop (core/include/topopt/simp.hpp:
+  // 1119, 1256-1276): plain (unprojected) stage only.
+  bool mma = false;
+  int mma_plateau_window = 10;
+  double mma_plateau_tol = 1e-3;
+  double mma_plateau_min_drop = 0.05;
+  int mma_plateau_flat_windows = 3;
 };
 struct SimpIteration { double compliance, change, volume_fraction; int cg_iterations; };
 struct SimpResult {
@@ -346,6 +353,19 @@ std::vector<double> von_mises_field(const Grid& g, double E, double nu,
 std::uint16_t float_to_half(float v);
 // core/src/simp/design_store.cpp (design_fingerprint): 64-bit FNV-1a over bytes.
 std::uint64_t fnv1a64(const void* data, std::size_t n);
+
+// MMA (Svanberg 1987) with one volume constraint: mma_update,
+// core/src/simp/simp.cpp:1433-1488 over mma_update_joint,
+// core/src/simp/mma_joint.cpp:18-118. The state carries the asymptotes and the
+// last two iterates across the calls of one run; mma_iter is 1-based.
+struct MmaState { std::vector<double> low, upp, xold1, xold2; };
+std::vector<double> mma_update(MmaState& st, int mma_iter, const Grid& g, const DensityFilter& f,
+                               const std::vector<double>& x, const std::vector<double>& dcomp,
+                               double volfrac, double move, double density_min,
+                               double* lambda_out = nullptr);
+// core/src/simp/simp.cpp:1081-1134.
+bool mma_objective_plateau(const std::vector<double>& c, int window, double rel_tol,
+                           double min_drop, int min_flat_windows);
 
 double heaviside_project(double rt, double beta, double eta);
 double heaviside_project_derivative(double rt, double beta, double eta);

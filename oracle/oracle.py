@@ -320,3 +320,46 @@ def fnv1a64(data):
     b = bytes(data)
     buf = C.create_string_buffer(b, len(b))
     return int(lib().oracle_fnv1a64(buf, C.c_size_t(len(b))))
+
+
+def optimize_mma(s, volfrac, radius, iterations, E0=1.0, nu=0.3, penalty=3.0, rho_min=1e-3,
+                 move=0.2, change_tol=0.0, cg_tol=1e-8, plateau_window=10, plateau_tol=1e-3,
+                 plateau_min_drop=0.05, plateau_flat_windows=3):
+    """simp_optimize with SimpUpdater::MMA (plain stage, plateau stop)."""
+    design = np.zeros(s.nvox)
+    xphys = np.zeros(s.nvox)
+    hist = np.zeros(4 * iterations)
+    it = C.c_int(0)
+    comp = C.c_double(0)
+    vf = C.c_double(0)
+    _check(lib().oracle_optimize_mma(
+        C.byref(s.p), C.c_double(E0), C.c_double(nu), C.c_double(penalty), C.c_double(rho_min),
+        C.c_double(volfrac), C.c_double(radius), C.c_double(move), int(iterations),
+        C.c_double(change_tol), C.c_double(cg_tol), int(plateau_window), C.c_double(plateau_tol),
+        C.c_double(plateau_min_drop), int(plateau_flat_windows), _ptr(design), _ptr(xphys),
+        _ptr(hist), C.byref(it), C.byref(comp), C.byref(vf)))
+    n = it.value
+    return {"design": design, "xphys": xphys, "history": hist[: 4 * n].reshape(n, 4),
+            "iterations": n, "compliance": comp.value, "volume_fraction": vf.value}
+
+
+def mma_steps(s, radius, xs, dcs, volfrac, move=0.2, rho_min=1e-3):
+    """Successive mma_update calls (one state, mma_iter 1..n) on the given
+    designs and compliance sensitivities; returns (new designs, lambdas)."""
+    xs = np.ascontiguousarray(xs, dtype=np.float64).reshape(-1, s.nvox)
+    dcs = np.ascontiguousarray(dcs, dtype=np.float64).reshape(-1, s.nvox)
+    n = xs.shape[0]
+    out = np.zeros_like(xs)
+    lam = np.zeros(n)
+    _check(lib().oracle_mma_steps(C.byref(s.p), C.c_double(radius), n, _ptr(xs), _ptr(dcs),
+                                  C.c_double(volfrac), C.c_double(move), C.c_double(rho_min),
+                                  _ptr(out), _ptr(lam)))
+    return out, lam
+
+
+def mma_plateau(c, window=10, rel_tol=1e-3, min_drop=0.05, flat_windows=3):
+    cc = np.ascontiguousarray(c, dtype=np.float64)
+    out = C.c_int(0)
+    _check(lib().oracle_mma_plateau(_ptr(cc), cc.size, int(window), C.c_double(rel_tol),
+                                    C.c_double(min_drop), int(flat_windows), C.byref(out)))
+    return bool(out.value)

@@ -251,6 +251,75 @@ int oracle_optimize(const OracleProblem* p, double E0, double nu, double penalty
   });
 }
 
+// The plain loop with the MMA updater and its plateau stop (window <= 0: the
+// change_tol test); other arguments as oracle_optimize.
+int oracle_optimize_mma(const OracleProblem* p, double E0, double nu, double penalty,
+                        double rho_min, double volfrac, double radius, double move,
+                        int max_iterations, double change_tol, double cg_tol, int window,
+                        double plateau_tol, double min_drop, int flat_windows, double* design,
+                        double* xphys, double* hist, int* iterations, double* compliance,
+                        double* final_volfrac) {
+  return guarded([&] {
+    const oracle::Grid g = grid_of(p);
+    oracle::SimpParams sp{E0, nu, penalty, rho_min};
+    oracle::SimpOptions o;
+    o.volume_fraction = volfrac;
+    o.filter_radius = radius;
+    o.move = move;
+    o.max_iterations = max_iterations;
+    o.change_tol = change_tol;
+    o.cg_tolerance = cg_tol;
+    o.mma = true;
+    o.mma_plateau_window = window;
+    o.mma_plateau_tol = plateau_tol;
+    o.mma_plateau_min_drop = min_drop;
+    o.mma_plateau_flat_windows = flat_windows;
+    const oracle::SimpResult r = oracle::simp_optimize(g, sp, bcs_of(p), loads_of(p), o);
+    if (design) std::memcpy(design, r.design.data(), r.design.size() * sizeof(double));
+    if (xphys)
+      std::memcpy(xphys, r.physical_density.data(), r.physical_density.size() * sizeof(double));
+    for (std::size_t i = 0; i < r.history.size(); ++i) {
+      hist[4 * i + 0] = r.history[i].compliance;
+      hist[4 * i + 1] = r.history[i].change;
+      hist[4 * i + 2] = r.history[i].volume_fraction;
+      hist[4 * i + 3] = r.history[i].cg_iterations;
+    }
+    *iterations = r.iterations;
+    *compliance = r.compliance;
+    *final_volfrac = r.volume_fraction;
+    if (r.non_convergent) throw oracle::SolverNonConvergence("oracle_optimize_mma: non-convergent", 0, 0);
+  });
+}
+
+// n_steps successive mma_update calls (mma_iter 1..n_steps, one state) on the
+// given designs x[s] and compliance sensitivities dc[s] (each N long).
+int oracle_mma_steps(const OracleProblem* p, double radius, int n_steps, const double* x,
+                     const double* dc, double volfrac, double move, double rho_min, double* out,
+                     double* lambdas) {
+  return guarded([&] {
+    const oracle::Grid g = grid_of(p);
+    const oracle::DensityFilter f = oracle::make_density_filter(g, radius);
+    const std::size_t N = g.voxels();
+    oracle::MmaState st;
+    for (int s = 0; s < n_steps; ++s) {
+      const std::vector<double> xs(x + s * N, x + (s + 1) * N), ds(dc + s * N, dc + (s + 1) * N);
+      double lam = 0.0;
+      const std::vector<double> xn =
+          oracle::mma_update(st, s + 1, g, f, xs, ds, volfrac, move, rho_min, &lam);
+      std::memcpy(out + s * N, xn.data(), N * sizeof(double));
+      if (lambdas) lambdas[s] = lam;
+    }
+  });
+}
+
+int oracle_mma_plateau(const double* c, int n, int window, double rel_tol, double min_drop,
+                       int flat_windows, int* out) {
+  return guarded([&] {
+    *out = oracle::mma_objective_plateau(std::vector<double>(c, c + n), window, rel_tol, min_drop,
+                                         flat_windows) ? 1 : 0;
+  });
+}
+
 // Projected (Heaviside beta-continuation) OC loop: n_stages stages of
 // (beta, move, iterations), threshold eta; other arguments as oracle_optimize.
 int oracle_optimize_projected(const OracleProblem* p, double E0, double nu, double penalty,

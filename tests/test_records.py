@@ -201,3 +201,22 @@ def test_progress_and_variant_lines():
                     "accepted=1 mesh=/out/variant_30.stl\n")
     kv = _worker_kv(line, "VARIANT ")
     assert kv["accepted"] == "1" and float(kv["achieved"]) == 0.299877
+
+
+def test_library_plateau_stop_matches_reference_curves():
+    """The device loop's plateau stop (host code in the library) on the
+    reference's curves (tests/golden/mma_plateau_curves.json)."""
+    fix = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                      "mma_plateau_curves.json")))
+    for name, pre, w, tol, drop, fw, want in fix["cases"]:
+        cv = fix["curves"][name][: pre] if pre is not None else fix["curves"][name]
+        assert ta.mma_plateau(cv, w, tol, drop, fw) == want, (name, pre, w, drop, fw)
+    for name, spec in fix["first_fire"].items():
+        cv = fix["curves"][name]
+        fire = next(n for n in range(1, len(cv) + 1)
+                    if ta.mma_plateau(cv[:n], spec["window"], 1e-3, spec["min_drop"],
+                                      spec["flat_windows"]))
+        got = oracle.mma_plateau(cv[:fire], spec["window"], 1e-3, spec["min_drop"],
+                                 spec["flat_windows"])
+        assert got and not oracle.mma_plateau(cv[:fire - 1], spec["window"], 1e-3,
+                                               spec["min_drop"], spec["flat_windows"])

@@ -71,6 +71,20 @@ class Options(C.Structure):
     ]
 
 
+class MmaOptions(C.Structure):
+    _fields_ = [("plateau_window", C.c_int), ("plateau_tol", C.c_double),
+                ("plateau_min_drop", C.c_double), ("plateau_flat_windows", C.c_int)]
+
+
+def mma_plateau(c, window=10, rel_tol=1e-3, min_drop=0.05, flat_windows=3):
+    """mma_objective_plateau on a compliance history."""
+    cc = np.ascontiguousarray(c, dtype=np.float64)
+    out = C.c_int(0)
+    _check(lib().topohip_mma_plateau(_p(cc), cc.size, int(window), C.c_double(rel_tol),
+                                     C.c_double(min_drop), int(flat_windows), C.byref(out)))
+    return bool(out.value)
+
+
 class Result(C.Structure):
     _fields_ = [
         ("iterations", C.c_int), ("converged", C.c_int), ("non_convergent", C.c_int),
@@ -334,6 +348,40 @@ class Problem:
                                        C.c_double(volfrac), C.c_double(move), C.c_double(rho_min),
                                        _p(out), _p(act), C.byref(nb), C.byref(lam)))
         return out, act, nb.value, lam.value
+
+    def mma_update(self, radius, mma_iter, x, dc, volfrac, move=0.2, rho_min=1e-3):
+        """mma_update: one MMA step; the handle keeps MmaState between calls
+        (restart a run at mma_iter 1). Returns (x_new, lambda, change)."""
+        x = _f64(x, self.nvox)
+        dc = _f64(dc, self.nvox)
+        out = np.zeros(self.nvox)
+        lam = C.c_double(0)
+        ch = C.c_double(0)
+        _check(lib().topohip_mma_update(self._h, C.c_double(radius), int(mma_iter), _p(x), _p(dc),
+                                        C.c_double(volfrac), C.c_double(move),
+                                        C.c_double(rho_min), _p(out), C.byref(lam), C.byref(ch)))
+        return out, lam.value, ch.value
+
+    def optimize_mma(self, volfrac, rmin, iterations, penalty=3.0, move=0.2, E0=1.0, rho_min=1e-3,
+                     change_tol=0.0, cg_tol=1e-8, cg_max_iterations=0, continuation=None,
+                     plateau_window=10, plateau_tol=1e-3, plateau_min_drop=0.05,
+                     plateau_flat_windows=3):
+        """simp_optimize with SimpUpdater::MMA (plain loop, plateau stop)."""
+        o = Options(volfrac, rmin, move, penalty, E0, rho_min, change_tol, cg_tol, int(iterations),
+                    int(cg_max_iterations), 0.0, 0.0, 0.0, 0)
+        if continuation:
+            o.pc_p0, o.pc_p1, o.pc_step, o.pc_per_stage = continuation
+        m = MmaOptions(int(plateau_window), plateau_tol, plateau_min_drop, int(plateau_flat_windows))
+        design = np.zeros(self.nvox)
+        xphys = np.zeros(self.nvox)
+        hist = np.zeros(4 * iterations)
+        res = Result()
+        _check(lib().topohip_optimize_mma(self._h, C.byref(o), C.byref(m), _p(design), _p(xphys),
+                                          _p(hist), C.byref(res)))
+        n = res.iterations
+        out = res.as_dict()
+        out.update(design=design, xphys=xphys, history=hist[: 4 * n].reshape(n, 4))
+        return out
 
     def optimize(self, volfrac, rmin, iterations, penalty=3.0, move=0.2, E0=1.0, rho_min=1e-3,
                  change_tol=0.0, cg_tol=1e-8, cg_max_iterations=0, continuation=None, mask=None):

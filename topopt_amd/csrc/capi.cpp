@@ -278,6 +278,50 @@ int topohip_optimize(topohip_problem* p, const topohip_options* opt, double* des
   });
 }
 
+int topohip_mma_update(topohip_problem* p, double radius, int mma_iter, const double* x,
+                       const double* dc, double volfrac, double move, double rho_min,
+                       double* x_new, double* lambda, double* change) {
+  return guarded([&] {
+    need(p, "problem"); need(x, "x"); need(dc, "dc"); need(x_new, "x_new");
+    Problem& P = *p->P;
+    const long long nv = P.g.elems();
+    DBuf<double> xi, di, xo;
+    xi.ensure(nv); di.ensure(nv); xo.ensure(nv);
+    TOPOHIP_CHECK(hipMemcpyAsync(xi.p, x, nv * sizeof(double), hipMemcpyHostToDevice, P.s));
+    TOPOHIP_CHECK(hipMemcpyAsync(di.p, dc, nv * sizeof(double), hipMemcpyHostToDevice, P.s));
+    FilterCache& fc = filter_for(P, radius);
+    mma_update_core(P, fc, P.all_solid ? nullptr : P.solid.p, static_cast<double>(P.n_solid),
+                    mma_iter, xi.p, di.p, volfrac, move, rho_min, xo.p, lambda, change);
+    TOPOHIP_CHECK(hipMemcpyAsync(x_new, xo.p, nv * sizeof(double), hipMemcpyDeviceToHost, P.s));
+    TOPOHIP_CHECK(hipStreamSynchronize(P.s));
+  });
+}
+
+int topohip_optimize_mma(topohip_problem* p, const topohip_options* opt,
+                         const topohip_mma_options* mma, double* design_out, double* xphys_out,
+                         double* history, topohip_result* result) {
+  return guarded([&] {
+    need(p, "problem"); need(opt, "options");
+    MmaOpts m;
+    if (mma) {
+      m.plateau_window = mma->plateau_window;
+      m.plateau_tol = mma->plateau_tol;
+      m.plateau_min_drop = mma->plateau_min_drop;
+      m.plateau_flat_windows = mma->plateau_flat_windows;
+    }
+    optimize(*p->P, *opt, design_out, xphys_out, history, result, nullptr, nullptr, 0.5, &m);
+  });
+}
+
+int topohip_mma_plateau(const double* c, int n, int window, double rel_tol, double min_drop,
+                        int flat_windows, int* out) {
+  return guarded([&] {
+    need(out, "out");
+    if (n > 0) need(c, "c");
+    *out = mma_objective_plateau(c, n, window, rel_tol, min_drop, flat_windows) ? 1 : 0;
+  });
+}
+
 int topohip_optimize_masked(topohip_problem* p, const uint8_t* mask, const topohip_options* opt,
                             double* design_out, double* xphys_out, double* history,
                             topohip_result* result) {

@@ -121,6 +121,7 @@ Problem::~Problem() {
     if (g) (void)hipGraphExecDestroy(g);
   if (st_h) (void)hipHostFree(st_h);
   if (oc_h) (void)hipHostFree(oc_h);
+  if (mma_h) (void)hipHostFree(mma_h);
   if (scal_h) (void)hipHostFree(scal_h);
   filters.clear();
   L.clear();
@@ -954,8 +955,10 @@ std::unique_ptr<Problem> create_problem(int nx, int ny, int nz, double h, double
   {
     std::lock_guard<std::recursive_mutex> lk(device_api_mutex());
     TOPOHIP_CHECK(hipHostMalloc(&P->oc_h, sizeof(OcState)));
+    TOPOHIP_CHECK(hipHostMalloc(&P->mma_h, sizeof(MmaSearch)));
   }
   P->oc.ensure(1);
+  P->mma_st.ensure(1);
   plan_levels(*P);
   alloc_levels(*P);
   ensure_cases(*P, 1);
@@ -1382,6 +1385,81 @@ void oc_update_dev(Problem& P, double radius, const double* d_x, const double* d
                  d_x, d_dc, volfrac, move, rho_min, d_xnew, d_active, bisections, lambda, change);
 }
 
+// mma_update over mma_update_joint (core/src/simp/simp.cpp:1433-1488,
+// core/src/simp/mma_joint.cpp:18-118). The volume is the sum of the filtered
+// design; the search decisions run on the device, read back once per 8 trials.
+void mma_update_core(Problem& P, FilterCache& fc, const uint8_t* dset, double n_design,
+                     int mma_iter, const double* d_x, const double* d_dc, double volfrac,
+                     double move, double rho_min, double* d_xnew, double* lambda,
+                     double* change) {
+  if (!(volfrac > 0.0 && volfrac <= 1.0))
+    throw Error(TOPOHIP_EINVAL, "mma_update: volume_fraction must be in (0, 1]");
+  if (!(move > 0.0)) throw Error(TOPOHIP_EINVAL, "mma_update: move must be > 0");
+  if (!(rho_min > 0.0 && rho_min < 1.0))
+    throw Error(TOPOHIP_EINVAL, "mma_update: density_min must be in (0, 1)");
+  if (mma_iter < 1) throw Error(TOPOHIP_EINVAL, "mma_update: mma_iter must be >= 1");
+  const long long nv = P.g.elems();
+  const size_t n = static_cast<size_t>(nv);
+  for (auto* b : {&P.dcf, &P.mma_low, &P.mma_upp, &P.mma_x1, &P.mma_x2, &P.xcand}) b->ensure(n);
+  if (mma_iter > 1 && mma_iter != P.mma_last + 1)
+    throw Error(TOPOHIP_EINVAL, "mma_update: mma_iter must continue the run's state (1, 2, 3, ...)");
+  P.mma_c.ensure(6 * n);
+  MmaArgs a{P.mma_low.p, P.mma_upp.p, P.mma_c.p, P.mma_c.p + n, P.mma_c.p + 2 * n,
+            P.mma_c.p + 3 * n, P.mma_c.p + 4 * n, P.mma_c.p + 5 * n};
+  const int ob = oc_blocks(nv), vb = dot_blocks(nv);
+  launch_filter(P.g, fc.st, dset, fc.hs.p, d_dc, P.dcf.p, 1, P.s);
+  launch_mma_scale(P.dcf.p, dset, nv, P.partials.p, ob, P.s);
+  launch_max_partials(P.partials.p, ob, P.scal.p, P.s);
+  launch_filter(P.g, fc.st, dset, fc.hs.p, d_x, P.xcand.p, 0, P.s);
+  launch_sum_vec<double>(P.xcand.p, nv, P.partials.p, vb, P.s);
+  launch_sum_partials(P.partials.p, vb, P.scal.p + 1, P.s);
+  launch_mma_coef(a, d_x, P.mma_x1.p, P.mma_x2.p, P.dcf.p, fc.dv.p, dset, nv, mma_iter, move,
+                  rho_min, P.scal.p, P.partials2.p, ob, P.s);
+  launch_mma_init(P.mma_st.p, P.partials2.p, ob, P.scal.p + 1, volfrac * n_design, P.s);
+  // at most 1 + ~100 doublings + 100 bisections
+  for (int round = 0;; ++round) {
+    if (round > 40) throw Error(TOPOHIP_EINTERNAL, "mma_update: dual search did not finish");
+    for (int i = 0; i < 8; ++i) {
+      launch_mma_trial(a, dset, nv, P.mma_st.p, P.partials.p, ob, P.s);
+      launch_mma_decide(P.mma_st.p, P.partials.p, ob, P.s);
+    }
+    d2h(P.mma_h, P.mma_st.p, 1, P.s);
+    if (P.mma_h->phase == 3) break;
+  }
+  launch_mma_final(a, d_x, dset, nv, P.mma_st.p, d_xnew, P.partials.p, ob, P.s);
+  launch_max_partials(P.partials.p, ob, P.scal.p + 2, P.s);
+  // roll the history: xold2 <- xold1, xold1 <- x
+  std::swap(P.mma_x1.p, P.mma_x2.p);
+  std::swap(P.mma_x1.n, P.mma_x2.n);
+  TOPOHIP_CHECK(hipMemcpyAsync(P.mma_x1.p, d_x, n * sizeof(double), hipMemcpyDeviceToDevice, P.s));
+  d2h(P.scal_h, P.scal.p, 3, P.s);
+  P.mma_last = mma_iter;
+  if (lambda) *lambda = P.mma_h->lambda;
+  if (change) *change = P.scal_h[2];
+}
+
+bool mma_objective_plateau(const double* c, int n, int window, double rel_tol, double min_drop,
+                           int min_flat_windows) {
+  if (window <= 0 || n < window + 1) return false;
+  double best_now = c[0];
+  for (int i = 1; i < n; ++i) best_now = std::min(best_now, c[i]);
+  if (min_drop > 0.0 && !(best_now <= c[0] * (1.0 - min_drop))) {
+    // the progress gate is closed: only the flatness escape may fire
+    bool escape = false;
+    const int span = min_flat_windows * window;
+    if (min_flat_windows > 0 && n >= span + 1) {
+      double best_before = c[0];
+      for (int i = 1; i < n - span; ++i) best_before = std::min(best_before, c[i]);
+      if (best_before > 0.0) escape = (best_before - best_now) / best_before < rel_tol;
+    }
+    if (!escape) return false;
+  }
+  double best_prev = c[0];
+  for (int i = 1; i < n - window; ++i) best_prev = std::min(best_prev, c[i]);
+  if (!(best_prev > 0.0)) return false;
+  return (best_prev - best_now) / best_prev < rel_tol;
+}
+
 // Host-side design mask of the masked loop: eff (effective_design_mask,
 // core/src/simp/simp.cpp:2610-2633: empty voxels -> FrozenVoid) and the design
 // set (solid Active voxels). The handle's grid is the analysis grid: a
@@ -1438,7 +1516,7 @@ void oc_update_projected_core(Problem& P, FilterCache& fc, const uint8_t* dset, 
 
 void optimize(Problem& P, const topohip_options& o, double* design_out, double* xphys_out,
               double* history, topohip_result* res, const uint8_t* mask,
-              const std::vector<ProjStage>* proj, double eta) {
+              const std::vector<ProjStage>* proj, double eta, const MmaOpts* mma) {
   if (!(o.volfrac > 0.0 && o.volfrac <= 1.0))
     throw Error(TOPOHIP_EINVAL, "simp_optimize: volume_fraction must be in (0, 1]");
   if (!(o.move > 0.0)) throw Error(TOPOHIP_EINVAL, "simp_optimize: move must be > 0");
@@ -1452,6 +1530,13 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
                                     "iterations >= 1");
   }
   if (o.change_tol < 0.0) throw Error(TOPOHIP_EINVAL, "simp_optimize: change_tol must be >= 0");
+  // validate_updater_options: projection stages run with the OC updater only,
+  // and the masked MMA update is not built here
+  if (mma && ((proj && !proj->empty()) || mask))
+    throw Error(TOPOHIP_EINVAL, "simp_optimize: the MMA updater runs the plain (unprojected, "
+                                "unmasked) loop");
+  int mma_iter = 0;
+  std::vector<double> curve;  // compliance history for the MMA plateau stop
   std::vector<std::pair<double, int>> stages;
   if (o.pc_per_stage > 0) {
     if (!(o.pc_p0 > 0.0) || !(o.pc_p1 >= o.pc_p0) || !(o.pc_step > 0.0))
@@ -1564,6 +1649,9 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
       if (st.project)
         oc_update_projected_core(P, fc, sol, nd, P.xd.p, P.xtil.p, P.dc.p, st.beta, eta, o.volfrac,
                                  st.move, o.rho_min, P.xnew.p, &change);
+      else if (mma)
+        mma_update_core(P, fc, sol, nd, ++mma_iter, P.xd.p, P.dc.p, o.volfrac, st.move, o.rho_min,
+                        P.xnew.p, nullptr, &change);
       else
         oc_update_core(P, fc, sol, nd, mask != nullptr, P.xd.p, P.dc.p, o.volfrac, st.move,
                        o.rho_min, P.xnew.p, nullptr, nullptr, nullptr, &change);
@@ -1587,6 +1675,15 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
         history[4 * it + 2] = vf;
         history[4 * it + 3] = ci.iterations;
       }
+      // stage_should_stop (core/src/simp/simp.cpp:1219-1232): the MMA plateau
+      // detector on the compliance history, else the change test
+      curve.push_back(c);
+      const bool use_plateau = mma && mma->plateau_window > 0;
+      const bool plateau =
+          use_plateau && mma_objective_plateau(curve.data(), static_cast<int>(curve.size()),
+                                               mma->plateau_window, mma->plateau_tol,
+                                               mma->plateau_min_drop, mma->plateau_flat_windows);
+      const bool stop = use_plateau ? plateau : change < o.change_tol;
       if (P.observer) {
         // SimpIterationObservation + IterationPhaseTimes
         // (core/include/topopt/simp.hpp:870-1000). The projection and mask
@@ -1604,6 +1701,7 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
         ob.cg_hier_built = ci.hier_built;
         ob.cg_mg_cycles_attempted = ci.mg_cycles_attempted;
         ob.beta = st.project ? st.beta : 0.0;
+        ob.plateau = plateau ? 1 : 0;
         ob.active_fraction = 1.0;
         ob.tail_prev_ms = t_prev_obs < 0.0 ? 0.0 : t_top - t_prev_obs;
         ob.filter_ms = ms_filter;
@@ -1627,7 +1725,7 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
         P.observer(&ob, P.observer_user);
         t_prev_obs = now_ms();
       }
-      if (change < o.change_tol) {
+      if (stop) {
         stage_converged = true;
         break;
       }

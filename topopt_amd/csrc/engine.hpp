@@ -109,6 +109,13 @@ struct Problem {
   DBuf<double> xtil, dpr, dcp, dvf, xcand, xcf;  // projected OC work
   DBuf<uint8_t> active;
 
+  // MMA updater state of the current run: asymptotes, the last two designs,
+  // the coefficient arrays and the dual search
+  DBuf<double> mma_low, mma_upp, mma_x1, mma_x2, mma_c;
+  DBuf<MmaSearch> mma_st;
+  MmaSearch* mma_h = nullptr;
+  int mma_last = 0;  // mma_iter of the last completed step
+
   // result fields and run records
   DBuf<double> fu, fvm;
   DBuf<float> fvm32, fu32;
@@ -156,13 +163,32 @@ struct ProjStage {
   double beta, move;
   int iterations;
 };
+// SimpUpdater::MMA and its plateau stop (core/include/topopt/simp.hpp:1119,
+// 1256-1276); window <= 0 keeps the change_tol test.
+struct MmaOpts {
+  int plateau_window = 10;
+  double plateau_tol = 1e-3;
+  double plateau_min_drop = 0.05;
+  int plateau_flat_windows = 3;
+};
+// One MMA step (mma_update, core/src/simp/simp.cpp:1433-1488) on the design
+// set dset with its filter: mma_iter is 1-based (the asymptotes restart at 1
+// and 2); the state in P carries over between calls of one run.
+void mma_update_core(Problem& P, FilterCache& fc, const uint8_t* dset, double n_design,
+                     int mma_iter, const double* d_x, const double* d_dc, double volfrac,
+                     double move, double rho_min, double* d_xnew, double* lambda,
+                     double* change);
+// mma_objective_plateau, core/src/simp/simp.cpp:1081-1134.
+bool mma_objective_plateau(const double* c, int n, int window, double rel_tol, double min_drop,
+                           int min_flat_windows);
 // mask (nullable): one MaskValue byte per voxel (0 Active, 1 FrozenSolid,
 // 2 FrozenVoid) for the masked loop; FrozenVoid voxels must be empty in P's grid.
 // proj (nullable, with eta): projected stages run in order instead of one
 // unprojected stage of o.max_iterations at o.move (plain loop only).
 void optimize(Problem& P, const topohip_options& o, double* design_out, double* xphys_out,
               double* history, topohip_result* res, const uint8_t* mask = nullptr,
-              const std::vector<ProjStage>* proj = nullptr, double eta = 0.5);
+              const std::vector<ProjStage>* proj = nullptr, double eta = 0.5,
+              const MmaOpts* mma = nullptr);
 std::unique_ptr<FilterCache> make_filter(Problem& P, double radius, const uint8_t* dset);
 void design_mask_of(const Problem& P, const uint8_t* mask, std::vector<uint8_t>& eff,
                     std::vector<uint8_t>& dset, long long& n_design);

@@ -210,6 +210,31 @@ void launch_oc_final_proj(const double* x, const double* dcf, const double* dvf,
                           const uint8_t* solid, long long n, double move, double rho_min,
                           const OcState* st, double* xnew, double* partials, int nblocks,
                           hipStream_t s);
+// MMA: per-voxel state and coefficients (each n long) and the dual search.
+struct MmaArgs {
+  double *low, *upp, *p0, *q0, *p1, *q1, *al, *be;
+};
+struct MmaSearch {
+  double l1, l2, b, lambda;
+  int phase;  // 0 test lambda = 0, 1 bracket doubling, 2 bisection, 3 done
+  int it, steps, pad;
+};
+void launch_mma_scale(const double* dcf, const uint8_t* dset, long long n, double* partials,
+                      int nblocks, hipStream_t s);
+// Asymptotes (iter <= 2: initial half-span), coefficients and move box; the
+// partials of sum p1/(U-x) + q1/(x-L). scale: device max |dcf|.
+void launch_mma_coef(const MmaArgs& a, const double* x, const double* x1, const double* x2,
+                     const double* dcf, const double* dv, const uint8_t* dset, long long n,
+                     int iter, double move, double xmin, const double* scale, double* partials,
+                     int nblocks, hipStream_t s);
+void launch_mma_init(MmaSearch* st, const double* partials, int n, const double* vol,
+                     double target, hipStream_t s);
+void launch_mma_trial(const MmaArgs& a, const uint8_t* dset, long long n, const MmaSearch* st,
+                      double* partials, int nblocks, hipStream_t s);
+void launch_mma_decide(MmaSearch* st, const double* partials, int n, hipStream_t s);
+void launch_mma_final(const MmaArgs& a, const double* x, const uint8_t* dset, long long n,
+                      const MmaSearch* st, double* xnew, double* partials, int nblocks,
+                      hipStream_t s);
 // Design-mask pins on a physical density: eff 1 (FrozenSolid) -> 1, 2 (FrozenVoid) -> 0.
 void launch_mask_pins(double* xphys, const uint8_t* eff, long long n, hipStream_t s);
 void launch_max_partials(const double* partials, int n, double* out, hipStream_t s);

@@ -476,4 +476,215 @@ void launch_mask_pins(double* xphys, const uint8_t* eff, long long n, hipStream_
   TOPOHIP_CHECK(hipGetLastError());
 }
 
+// ---- MMA (Svanberg 1987), one volume constraint ---------------------------------
+// mma_update (core/src/simp/simp.cpp:1433-1488) over mma_update_joint
+// (core/src/simp/mma_joint.cpp:18-118): asymptotes, the separable convex
+// coefficients and the move box per design voxel, then the 1-D dual search
+// for the volume multiplier with its decisions taken on the device.
+namespace {
+
+// per-block max |dcf| over the design set (the scale-invariance normalisation)
+__global__ void mma_scale_k(const double* __restrict__ dcf, const uint8_t* __restrict__ dset,
+                            long long n, double* __restrict__ partials) {
+  double m = 0.0;
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * blockDim.x)
+    if (!dset || dset[e]) m = fmax(m, fabs(dcf[e]));
+  m = block_max(m);
+  if (threadIdx.x == 0) partials[blockIdx.x] = m;
+}
+
+__global__ void mma_coef_k(MmaArgs a, const double* __restrict__ x, const double* __restrict__ x1,
+                           const double* __restrict__ x2, const double* __restrict__ dcf,
+                           const double* __restrict__ dv, const uint8_t* __restrict__ dset,
+                           long long n, int iter, double move, double xmin,
+                           const double* __restrict__ scale, double* __restrict__ partials) {
+  const double asyinit = 0.5, asyincr = 1.2, asydecr = 0.7, albefa = 0.1, raa0 = 1e-5;
+  const double xmax = 1.0, xrange = xmax - xmin;
+  const double sc = *scale;
+  double bsum = 0.0;
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    if (dset && !dset[e]) continue;
+    const double xe = x[e];
+    double L, U;
+    if (iter <= 2) {
+      L = xe - asyinit * xrange;
+      U = xe + asyinit * xrange;
+    } else {
+      const double o1 = x1[e], o2 = x2[e];
+      const double sgn = (xe - o1) * (o1 - o2);
+      const double gam = sgn < 0.0 ? asydecr : (sgn > 0.0 ? asyincr : 1.0);
+      L = xe - gam * (o1 - a.low[e]);
+      U = xe + gam * (a.upp[e] - o1);
+      L = fmin(L, xe - 0.01 * xrange);
+      L = fmax(L, xe - 10.0 * xrange);
+      U = fmax(U, xe + 0.01 * xrange);
+      U = fmin(U, xe + 10.0 * xrange);
+    }
+    a.low[e] = L;
+    a.upp[e] = U;
+    const double d = sc > 0.0 ? dcf[e] / sc : dcf[e];
+    const double ux1 = U - xe, xl1 = xe - L;
+    const double dcp = fmax(d, 0.0), dcm = fmax(-d, 0.0);
+    const double dvp = fmax(dv[e], 0.0), dvm = fmax(-dv[e], 0.0);
+    const double r = raa0 / xrange;
+    const double p0 = ux1 * ux1 * (1.001 * dcp + 0.001 * dcm + r);
+    const double q0 = xl1 * xl1 * (0.001 * dcp + 1.001 * dcm + r);
+    const double p1 = ux1 * ux1 * (1.001 * dvp + 0.001 * dvm + r);
+    const double q1 = xl1 * xl1 * (0.001 * dvp + 1.001 * dvm + r);
+    a.p0[e] = p0;
+    a.q0[e] = q0;
+    a.p1[e] = p1;
+    a.q1[e] = q1;
+    bsum += p1 / ux1 + q1 / xl1;
+    a.al[e] = fmax(fmax(xmin, L + albefa * (xe - L)), xe - move * xrange);
+    a.be[e] = fmin(fmin(xmax, U - albefa * (U - xe)), xe + move * xrange);
+  }
+  bsum = block_sum(bsum);
+  if (threadIdx.x == 0) partials[blockIdx.x] = bsum;
+}
+
+// b = -(vol - target) + sum of the coefficient partials; the search starts at
+// lambda = 0.
+__global__ void mma_init_k(MmaSearch* st, const double* __restrict__ partials, int n,
+                           const double* __restrict__ vol, double target) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
+  s = block_sum(s);
+  if (threadIdx.x != 0) return;
+  st->b = -(*vol - target) + s;
+  st->l1 = 0.0;
+  st->l2 = 1.0;
+  st->lambda = 0.0;
+  st->phase = 0;
+  st->it = 0;
+  st->steps = 0;
+}
+
+__device__ inline double mma_trial_lambda(const MmaSearch* st) {
+  return st->phase == 0 ? 0.0 : (st->phase == 1 ? st->l2 : 0.5 * (st->l1 + st->l2));
+}
+
+__device__ inline double mma_candidate(const MmaArgs& a, long long e, double lam) {
+  const double pl = sqrt(a.p0[e] + lam * a.p1[e]);
+  const double ql = sqrt(a.q0[e] + lam * a.q1[e]);
+  double xt = (pl * a.low[e] + ql * a.upp[e]) / (pl + ql);
+  if (xt < a.al[e]) xt = a.al[e];
+  if (xt > a.be[e]) xt = a.be[e];
+  return xt;
+}
+
+__global__ void mma_trial_k(MmaArgs a, const uint8_t* __restrict__ dset, long long n,
+                            const MmaSearch* st, double* __restrict__ partials) {
+  if (st->phase == 3) return;
+  const double lam = mma_trial_lambda(st);
+  double s = 0.0;
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    if (dset && !dset[e]) continue;
+    const double xt = mma_candidate(a, e, lam);
+    s += a.p1[e] / (a.upp[e] - xt) + a.q1[e] / (xt - a.low[e]);
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// One decision of the search: g(0) > 0 opens the bracket [0, 1], doubled while
+// l2 < 1e30 and g(l2) > 0; then at most 100 bisections while
+// (l2 - l1) > 1e-9 (1 + l1 + l2); lambda = (l1 + l2) / 2 (0 when g(0) <= 0).
+__global__ void mma_decide_k(MmaSearch* st, const double* __restrict__ partials, int n) {
+  if (st->phase == 3) return;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
+  s = block_sum(s);
+  if (threadIdx.x != 0) return;
+  const double gv = s - st->b;
+  st->steps += 1;
+  auto bisect_or_finish = [&]() {
+    if (st->it < 100 && (st->l2 - st->l1) > 1e-9 * (1.0 + st->l1 + st->l2)) {
+      st->phase = 2;
+    } else {
+      st->lambda = 0.5 * (st->l1 + st->l2);
+      st->phase = 3;
+    }
+  };
+  if (st->phase == 0) {
+    if (gv > 0.0) {
+      st->l1 = 0.0;
+      st->l2 = 1.0;
+      st->phase = 1;
+    } else {
+      st->lambda = 0.0;
+      st->phase = 3;
+    }
+  } else if (st->phase == 1) {
+    if (gv > 0.0) {
+      st->l2 *= 2.0;
+      if (!(st->l2 < 1e30)) bisect_or_finish();
+    } else {
+      bisect_or_finish();
+    }
+  } else {
+    const double lm = 0.5 * (st->l1 + st->l2);
+    if (gv > 0.0) st->l1 = lm;
+    else st->l2 = lm;
+    st->it += 1;
+    bisect_or_finish();
+  }
+}
+
+// xnew = candidate(lambda) on the design set, 0 elsewhere; change partials.
+__global__ void mma_final_k(MmaArgs a, const double* __restrict__ x,
+                            const uint8_t* __restrict__ dset, long long n, const MmaSearch* st,
+                            double* __restrict__ xnew, double* __restrict__ partials) {
+  const double lam = st->lambda;
+  double m = 0.0;
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const double xt = (!dset || dset[e]) ? mma_candidate(a, e, lam) : 0.0;
+    xnew[e] = xt;
+    m = fmax(m, fabs(xt - x[e]));
+  }
+  m = block_max(m);
+  if (threadIdx.x == 0) partials[blockIdx.x] = m;
+}
+
+}  // namespace
+
+void launch_mma_scale(const double* dcf, const uint8_t* dset, long long n, double* partials,
+                      int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(mma_scale_k, dim3(nblocks), dim3(RB), 0, s, dcf, dset, n, partials);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+void launch_mma_coef(const MmaArgs& a, const double* x, const double* x1, const double* x2,
+                     const double* dcf, const double* dv, const uint8_t* dset, long long n,
+                     int iter, double move, double xmin, const double* scale, double* partials,
+                     int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(mma_coef_k, dim3(nblocks), dim3(RB), 0, s, a, x, x1, x2, dcf, dv, dset, n,
+                     iter, move, xmin, scale, partials);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+void launch_mma_init(MmaSearch* st, const double* partials, int n, const double* vol,
+                     double target, hipStream_t s) {
+  hipLaunchKernelGGL(mma_init_k, dim3(1), dim3(RB), 0, s, st, partials, n, vol, target);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+void launch_mma_trial(const MmaArgs& a, const uint8_t* dset, long long n, const MmaSearch* st,
+                      double* partials, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(mma_trial_k, dim3(nblocks), dim3(RB), 0, s, a, dset, n, st, partials);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+void launch_mma_decide(MmaSearch* st, const double* partials, int n, hipStream_t s) {
+  hipLaunchKernelGGL(mma_decide_k, dim3(1), dim3(RB), 0, s, st, partials, n);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+void launch_mma_final(const MmaArgs& a, const double* x, const uint8_t* dset, long long n,
+                      const MmaSearch* st, double* xnew, double* partials, int nblocks,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(mma_final_k, dim3(nblocks), dim3(RB), 0, s, a, x, dset, n, st, xnew,
+                     partials);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+
 }  // namespace topohip
