@@ -289,6 +289,13 @@ int topohip_server_fetch_density(topohip_server* s, int64_t job_id, double* xphy
 /* Message of a failed job (empty when it succeeded). */
 int topohip_server_job_error(topohip_server* s, int64_t job_id, char* buf, size_t n);
 int topohip_server_destroy(topohip_server* s);
+/* Run records per job (dir NULL or "" turns them off; affects jobs started
+ * afterwards): <dir>/job_<id>/iterations.csv (one row per design iteration),
+ * stdout.txt (the CLI's PROGRESS lines as iterations complete, then the
+ * VARIANT line), design.bin and fields.bin (one variant; von Mises of the
+ * final solve formed on the device) — the files the LAN worker and the app
+ * read from a run directory (tools/topopt-worker/topopt_worker.py:231-266). */
+int topohip_server_set_output(topohip_server* s, const char* dir);
 /* Device bytes a job of this shape keeps resident (admission estimate). */
 size_t topohip_job_footprint(const topohip_job* job);
 

@@ -86,3 +86,38 @@ def test_admission_respects_slots_and_budget():
         live += d
         peak = max(peak, live)
     assert peak <= 2
+
+
+def test_server_run_records(tmp_path):
+    """Per-job run directories: iterations.csv rows per iteration, the CLI's
+    PROGRESS / VARIANT lines as the worker parses them
+    (tools/topopt-worker/topopt_worker.py:231-266), design.bin holding the
+    job's density, fields.bin with the device's von Mises and displacement."""
+    jobs = [dict(nx=16, ny=8, nz=8, bc_kind=0, volfrac=0.3, rmin=1.5, iterations=4),
+            dict(nx=24, ny=8, nz=8, bc_kind=1, volfrac=0.4, rmin=1.5, iterations=3)]
+    with ta.Server(max_slots=2) as srv:
+        srv.set_output(tmp_path)
+        ids = [srv.submit(j) for j in jobs]
+        srv.drain()
+        res = [srv.poll(i) for i in ids]
+        dens = [srv.density(i, j["nx"] * j["ny"] * j["nz"]) for i, j in zip(ids, jobs)]
+    for jid, j, r, x in zip(ids, jobs, res, dens):
+        d = tmp_path / f"job_{jid}"
+        assert r["state"] == 2
+        lines = (d / "iterations.csv").read_text().splitlines()
+        assert lines[0] == ta.iterations_csv_header() and len(lines) == 1 + j["iterations"]
+        out = (d / "stdout.txt").read_text().splitlines()
+        assert out[:-1] == [f"PROGRESS rung=0 rungs=1 iter={k}" for k in range(1, j["iterations"] + 1)]
+        kv = dict(t.split("=", 1) for t in out[-1][len("VARIANT "):].split())
+        assert out[-1].startswith("VARIANT ") and kv["accepted"] == "1"
+        assert float(kv["vf"]) == pytest.approx(j["volfrac"], abs=1e-6)
+        assert float(kv["achieved"]) == pytest.approx(r["volume_fraction"], abs=1e-6)
+        back = ta.read_design_file(d / "design.bin")
+        v = back["variants"][0]
+        assert np.array_equal(v["density"], x) and v["iterations"] == j["iterations"]
+        b = (d / "fields.bin").read_bytes()
+        nv = j["nx"] * j["ny"] * j["nz"]
+        nd = 3 * (j["nx"] + 1) * (j["ny"] + 1) * (j["nz"] + 1)
+        assert len(b) == 64 + 48 + 4 * nv + 4 * nd
+        vm = np.frombuffer(b, "<f4", nv, 64 + 48)
+        assert np.all(vm >= 0) and vm.max() == np.float32(v["max_von_mises_mpa"])

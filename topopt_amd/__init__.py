@@ -468,6 +468,11 @@ class Server:
     def footprint(job):
         return lib().topohip_job_footprint(C.byref(job))
 
+    def set_output(self, directory):
+        """Per-job run records under directory/job_<id>/ (None turns them off)."""
+        d = None if directory is None else str(directory).encode()
+        _check(lib().topohip_server_set_output(self._s, d))
+
     def submit(self, job):
         if isinstance(job, dict):
             job = self.make_job(**job)

@@ -100,6 +100,12 @@ std::vector<std::uint8_t> slurp(const std::string& path) {
   return out;
 }
 
+std::string join(const std::string& dir, const std::string& name) {
+  return dir.empty() ? name : dir + "/" + name;
+}
+
+}  // namespace
+
 void make_dirs(const std::string& dir) {
   if (dir.empty()) return;
   std::string cur;
@@ -112,12 +118,6 @@ void make_dirs(const std::string& dir) {
       io("cannot create directory " + cur + ": " + std::strerror(errno));
   }
 }
-
-std::string join(const std::string& dir, const std::string& name) {
-  return dir.empty() ? name : dir + "/" + name;
-}
-
-}  // namespace
 
 // core/src/simp/observability.cpp:282-330: the column order and the printf
 // conversions of append_at (%.10g objective, %.6f fractions, %.6g beta, %.3f

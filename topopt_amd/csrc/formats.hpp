@@ -20,6 +20,9 @@ extern const char kIterationCsvHeader[];
 std::string iterations_csv_row(std::size_t rung, const topohip_iteration_obs& o,
                                long long wall_ms);
 
+// mkdir -p.
+void make_dirs(const std::string& dir);
+
 // Epoch milliseconds from the system clock (the CSV's wall_ms column).
 long long wall_clock_ms();
 

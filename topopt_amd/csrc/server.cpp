@@ -8,6 +8,7 @@
 // host thread, so the kernels of concurrently running jobs interleave on the
 // GPU. A job's kernels, launch shapes and reductions do not depend on what else
 // runs, so its result is the same alone and in the mix.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -19,6 +20,7 @@
 #include <thread>
 
 #include "engine.hpp"
+#include "formats.hpp"
 
 using namespace topohip;
 
@@ -47,6 +49,57 @@ void job_bcs(const topohip_job& j, std::vector<int32_t>& bn, std::vector<int32_t
   }
 }
 
+// Per-job run records (optional): iterations.csv fed by the design loop's
+// observer, the CLI's PROGRESS lines in stdout.txt as the iterations complete,
+// then design.bin, fields.bin and the VARIANT line once the job is done.
+struct JobOut {
+  std::unique_ptr<CsvWriter> csv;
+  std::FILE* log = nullptr;
+  ~JobOut() {
+    if (log) std::fclose(log);
+  }
+};
+
+void observe_job(const topohip_iteration_obs* o, void* user) {
+  auto* out = static_cast<JobOut*>(user);
+  out->csv->append_at(0, *o, wall_clock_ms());
+  const std::string line = progress_line(0, 1, o->iteration);
+  std::fputs(line.c_str(), out->log);
+  std::fflush(out->log);
+}
+
+// design.bin and fields.bin of the finished job (one variant, the job's volume
+// fraction; the device forms the von Mises field of the final solve) and the
+// VARIANT line. No margin gate runs here: the margin fields are 0 and the
+// variant counts as accepted; no mesh is exported.
+void write_job_records(Problem& P, const topohip_job& j, const topohip_result& r,
+                       const std::vector<double>& xphys, const std::string& dir, JobOut& jo) {
+  const size_t nv = xphys.size(), nd = static_cast<size_t>(P.g.ndof());
+  std::vector<float> vm(nv), du(nd);
+  result_fields(P, 1.0, nullptr, nullptr, vm.data(), du.data());
+  float vmax = 0.f;
+  for (float v : vm) vmax = std::max(vmax, v);
+  const topohip_grid_frame g{j.nx, j.ny, j.nz, {0.0, 0.0, 0.0}, 1.0};
+  topohip_design_variant d{};
+  d.requested_volume_fraction = j.volfrac;
+  d.achieved_volume_fraction = r.volume_fraction;
+  d.max_von_mises_mpa = vmax;
+  d.accepted = 1;
+  d.iterations = r.iterations;
+  d.applied_build_dir[2] = 1.0;
+  d.density = xphys.data();
+  write_design_file(dir + "/design.bin", g, 1, &d);
+  topohip_fields_variant f{};
+  f.requested_volume_fraction = j.volfrac;
+  f.von_mises = vm.data();
+  f.displacement = du.data();
+  write_fields_file(dir + "/fields.bin", g, 1, &f);
+  const std::string line =
+      variant_line(j.volfrac, r.volume_fraction, r.volume_fraction, 0.0, "");
+  std::fputs(line.c_str(), jo.log);
+  std::fflush(jo.log);
+}
+
 struct JobRec {
   int64_t id = 0;
   topohip_job job{};
@@ -58,6 +111,7 @@ struct JobRec {
 }  // namespace
 
 struct topohip_server {
+  std::string out_dir;  // empty: no run records
   size_t budget = 0;
   int max_slots = 1;
   int device = 0;
@@ -97,7 +151,23 @@ struct topohip_server {
       o.volfrac = j.volfrac; o.rmin = j.rmin; o.move = j.move; o.penalty = j.penalty;
       o.E0 = 1.0; o.rho_min = 1e-3; o.change_tol = 0.0; o.cg_tol = j.cg_tol;
       o.max_iterations = j.iterations; o.cg_max_iterations = 0;
+      std::unique_ptr<JobOut> jo;
+      std::string dir;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!out_dir.empty()) dir = out_dir + "/job_" + std::to_string(id);
+      }
+      if (!dir.empty()) {
+        jo = std::make_unique<JobOut>();
+        make_dirs(dir);
+        jo->csv = std::make_unique<CsvWriter>(dir + "/iterations.csv");
+        jo->log = std::fopen((dir + "/stdout.txt").c_str(), "wb");
+        if (!jo->log) throw Error(TOPOHIP_EINTERNAL, "cannot open " + dir + "/stdout.txt");
+        P->observer = observe_job;
+        P->observer_user = jo.get();
+      }
       optimize(*P, o, nullptr, xp.data(), nullptr, &r);
+      if (jo) write_job_records(*P, j, r, xp, dir, *jo);
     } catch (const Error& e) {
       status = e.code;
       err = e.what();
@@ -236,6 +306,13 @@ int topohip_server_destroy(topohip_server* s) {
   s->sched.join();
   for (auto& w : s->workers) w.join();
   delete s;
+  return TOPOHIP_OK;
+}
+
+int topohip_server_set_output(topohip_server* s, const char* dir) {
+  if (!s) return TOPOHIP_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->out_dir = dir ? dir : "";
   return TOPOHIP_OK;
 }
 
