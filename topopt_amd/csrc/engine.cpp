@@ -664,12 +664,18 @@ void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool
     P.cg_graph_nc[gi] = nc;
   }
   const long long per_it = pc == Prec::JACOBI ? 1 : 3;
+  // The first read-back waits for as many batches as the previous solve of
+  // this kind needed, less a margin (consecutive design iterations take
+  // similar counts); later ones follow every batch. Batches past convergence
+  // are no-ops (every kernel checks the done flag), so this only saves syncs.
+  int ahead = std::min(8, std::max(1, (P.pcg_hint[gi] - 2 * batch) / batch));
   for (long long guard = 0;; ++guard) {
     if (guard > 4LL * cap + 64) throw Error(TOPOHIP_EINTERNAL, "pcg: iteration state stalled");
     if (!all_done()) {
       long long before = 0;
       for (int r = 0; r < nc; ++r) before += P.st_h[r].iters;
-      TOPOHIP_CHECK(hipGraphLaunch(gx, P.s));
+      for (int k = 0; k < ahead; ++k) TOPOHIP_CHECK(hipGraphLaunch(gx, P.s));
+      ahead = 1;
       read_state(P, nc);
       long long after = 0;
       for (int r = 0; r < nc; ++r) after += P.st_h[r].iters;
@@ -720,8 +726,12 @@ void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool
     restart(P, pc, nc);
     read_state(P, nc);
   }
-  for (int r = 0; r < nc; ++r)
+  int most = 0;
+  for (int r = 0; r < nc; ++r) {
     if (!(skip && skip[r])) iters[r] = P.st_h[r].iters;
+    most = std::max(most, P.st_h[r].iters);
+  }
+  P.pcg_hint[gi] = most;
 }
 
 }  // namespace
@@ -1356,14 +1366,16 @@ void oc_update_core(Problem& P, FilterCache& fc, const uint8_t* dset, double n_d
   P.dcf.ensure(static_cast<size_t>(nv));
   launch_filter(P.g, fc.st, sol, fc.hs.p, d_dc, P.dcf.p, 1, P.s);
   const double target = volfrac * n_design;
-  launch_oc_init(P.oc.p, target, P.s);
+  launch_oc_init_multi(P.oc.p, target, P.s);
   const int ob = oc_blocks(nv);
-  // The search takes at most ~100 bracket + 100 bisection trials.
+  P.lam_partials.ensure(static_cast<size_t>(kLamPass) * ob);
+  // Each pass tries up to 7 lambdas (~100 bracket + 100 bisection trials at most).
   for (int round = 0;; ++round) {
     if (round > 40) throw Error(TOPOHIP_EINTERNAL, "oc_update: lambda search did not finish");
-    for (int i = 0; i < 8; ++i) {
-      launch_oc_trial(d_x, P.dcf.p, fc.dv.p, sol, nv, move, rho_min, P.oc.p, P.partials.p, ob, P.s);
-      launch_oc_decide(P.oc.p, P.partials.p, ob, P.s);
+    for (int i = 0; i < 6; ++i) {
+      launch_oc_trial_multi(d_x, P.dcf.p, fc.dv.p, sol, nv, move, rho_min, P.oc.p,
+                            P.lam_partials.p, ob, P.s);
+      launch_oc_decide_multi(P.oc.p, P.lam_partials.p, ob, P.s);
     }
     d2h(P.oc_h, P.oc.p, 1, P.s);
     if (P.oc_h->phase == 2) break;
@@ -1416,12 +1428,13 @@ void mma_update_core(Problem& P, FilterCache& fc, const uint8_t* dset, double n_
   launch_mma_coef(a, d_x, P.mma_x1.p, P.mma_x2.p, P.dcf.p, fc.dv.p, dset, nv, mma_iter, move,
                   rho_min, P.scal.p, P.partials2.p, ob, P.s);
   launch_mma_init(P.mma_st.p, P.partials2.p, ob, P.scal.p + 1, volfrac * n_design, P.s);
-  // at most 1 + ~100 doublings + 100 bisections
+  P.lam_partials.ensure(static_cast<size_t>(kLamPass) * ob);
+  // each pass tries up to 7 lambdas: at most 1 + ~100 doublings + 100 bisections
   for (int round = 0;; ++round) {
     if (round > 40) throw Error(TOPOHIP_EINTERNAL, "mma_update: dual search did not finish");
-    for (int i = 0; i < 8; ++i) {
-      launch_mma_trial(a, dset, nv, P.mma_st.p, P.partials.p, ob, P.s);
-      launch_mma_decide(P.mma_st.p, P.partials.p, ob, P.s);
+    for (int i = 0; i < 6; ++i) {
+      launch_mma_trial(a, dset, nv, P.mma_st.p, P.lam_partials.p, ob, P.s);
+      launch_mma_decide(P.mma_st.p, P.lam_partials.p, ob, P.s);
     }
     d2h(P.mma_h, P.mma_st.p, 1, P.s);
     if (P.mma_h->phase == 3) break;

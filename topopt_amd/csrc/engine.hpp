@@ -113,6 +113,7 @@ struct Problem {
   // the coefficient arrays and the dual search
   DBuf<double> mma_low, mma_upp, mma_x1, mma_x2, mma_c;
   DBuf<MmaSearch> mma_st;
+  DBuf<double> lam_partials;  // kLamPass x blocks: the multi-lambda search passes
   MmaSearch* mma_h = nullptr;
   int mma_last = 0;  // mma_iter of the last completed step
 
@@ -129,6 +130,7 @@ struct Problem {
   int replacements = 0;
   hipGraphExec_t cg_graph[3] = {nullptr, nullptr, nullptr};  // per preconditioner kind
   int cg_graph_nc[3] = {0, 0, 0};                             // case count captured
+  int pcg_hint[3] = {0, 0, 0};  // iterations of the last solve per preconditioner kind
 
   Problem() = default;
   Problem(const Problem&) = delete;

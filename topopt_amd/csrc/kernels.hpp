@@ -173,11 +173,16 @@ void launch_filter_hs(const Geom& g, const Stencil& st, const uint8_t* solid, do
                       hipStream_t s);
 
 // OC state: the lambda search runs on the device.
+// Up to kLamPass lambdas are tried in one pass over the voxels: the next
+// bracket doublings, or the three bisection levels ahead (7 midpoints in heap
+// order). The decisions then replay the sequential search exactly.
+constexpr int kLamPass = 7;
 struct OcState {
   double l1, l2, lmid, target, vol;
   int phase;      // 0 = bracketing, 1 = bisecting, 2 = done
   int steps;      // bisection steps taken
-  int pad[2];
+  int nlam, pad;
+  double lam[kLamPass];
 };
 void launch_oc_init(OcState* st, double target, hipStream_t s);
 // One lambda trial: volume partials of candidate(lambda_trial(st)).
@@ -185,6 +190,14 @@ void launch_oc_trial(const double* x, const double* dcf, const double* dv, const
                      long long n, double move, double rho_min, const OcState* st,
                      double* partials, int nblocks, hipStream_t s);
 void launch_oc_decide(OcState* st, const double* partials, int n, hipStream_t s);
+// The multi-lambda search of the plain update: init plans the first pass; a
+// trial writes the volume partials of every planned lambda (kLamPass x nblocks);
+// decide replays the sequential decisions over them and plans the next pass.
+void launch_oc_init_multi(OcState* st, double target, hipStream_t s);
+void launch_oc_trial_multi(const double* x, const double* dcf, const double* dv,
+                           const uint8_t* solid, long long n, double move, double rho_min,
+                           const OcState* st, double* partials, int nblocks, hipStream_t s);
+void launch_oc_decide_multi(OcState* st, const double* partials, int n, hipStream_t s);
 int oc_blocks(long long n);
 // Final candidate at lambda = (l1+l2)/2 + active set + change partials (max).
 // Voxels off the design set `solid` get 0 (keep = 0, plain oc_update) or keep
@@ -217,7 +230,8 @@ struct MmaArgs {
 struct MmaSearch {
   double l1, l2, b, lambda;
   int phase;  // 0 test lambda = 0, 1 bracket doubling, 2 bisection, 3 done
-  int it, steps, pad;
+  int it, steps, nlam;
+  double lam[kLamPass];  // lambdas of the current pass (as OcState)
 };
 void launch_mma_scale(const double* dcf, const uint8_t* dset, long long n, double* partials,
                       int nblocks, hipStream_t s);
@@ -229,6 +243,7 @@ void launch_mma_coef(const MmaArgs& a, const double* x, const double* x1, const 
                      int nblocks, hipStream_t s);
 void launch_mma_init(MmaSearch* st, const double* partials, int n, const double* vol,
                      double target, hipStream_t s);
+// partials: kLamPass x nblocks (one set per planned lambda)
 void launch_mma_trial(const MmaArgs& a, const uint8_t* dset, long long n, const MmaSearch* st,
                       double* partials, int nblocks, hipStream_t s);
 void launch_mma_decide(MmaSearch* st, const double* partials, int n, hipStream_t s);

@@ -47,6 +47,25 @@ __device__ inline double block_max(double v) {
   return s;
 }
 
+// N block sums at once (one barrier); the results are valid in thread 0.
+template <int N>
+__device__ inline void block_sum_n(double (&v)[N]) {
+  __shared__ double sh[N][RB / 64];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const double w = wave_sum(v[j]);
+    if ((threadIdx.x & 63) == 0) sh[j][threadIdx.x >> 6] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double s = 0.0;
+      for (int w = 0; w < RB / 64; ++w) s += sh[j][w];
+      v[j] = s;
+    }
+}
+
 __global__ void youngs_k(const double* __restrict__ xphys, const uint8_t* __restrict__ solid,
                          long long n, double E0, double p, double rho_min, double* __restrict__ E64,
                          float* __restrict__ E32) {
@@ -245,6 +264,126 @@ __global__ void oc_decide_k(OcState* st, const double* __restrict__ partials, in
   if (!(st->steps < 100 && (st->l2 - st->l1) > 1e-6 * (st->l1 + st->l2))) st->phase = 2;
 }
 
+// Lambdas of the next pass: the bracket doublings l2, 2 l2, ... below 1e30,
+// or the midpoints of the next three bisection levels in heap order (node j's
+// children 2j+1: lower half, 2j+2: upper half), each formed as the sequential
+// search forms it, 0.5 (l1 + l2) of the interval it would hold.
+__device__ inline void oc_plan(OcState* st) {
+  if (st->phase == 0) {
+    double l = st->l2;
+    int k = 0;
+    while (k < kLamPass && l < 1e30) {
+      st->lam[k++] = l;
+      l *= 2.0;
+    }
+    st->nlam = k;
+  } else if (st->phase == 1) {
+    const double l1 = st->l1, l2 = st->l2;
+    const double m0 = 0.5 * (l1 + l2);
+    const double mL = 0.5 * (l1 + m0), mR = 0.5 * (m0 + l2);
+    st->lam[0] = m0;
+    st->lam[1] = mL;
+    st->lam[2] = mR;
+    st->lam[3] = 0.5 * (l1 + mL);
+    st->lam[4] = 0.5 * (mL + m0);
+    st->lam[5] = 0.5 * (m0 + mR);
+    st->lam[6] = 0.5 * (mR + l2);
+    st->nlam = kLamPass;
+  } else {
+    st->nlam = 0;
+  }
+}
+
+__global__ void oc_init_multi_k(OcState* st, double target) {
+  st->l1 = 0.0;
+  st->l2 = 1.0;
+  st->lmid = 0.0;
+  st->target = target;
+  st->vol = 0.0;
+  st->phase = 0;
+  st->steps = 0;
+  oc_plan(st);
+}
+
+__global__ void oc_trial_multi_k(const double* __restrict__ x, const double* __restrict__ dcf,
+                                 const double* __restrict__ dv, const uint8_t* __restrict__ solid,
+                                 long long n, double move, double rho_min, const OcState* st,
+                                 double* __restrict__ partials) {
+  if (st->phase == 2) return;
+  const int nl = st->nlam;
+  double lam[kLamPass], s[kLamPass];
+#pragma unroll
+  for (int j = 0; j < kLamPass; ++j) {
+    lam[j] = j < nl ? st->lam[j] : 1.0;
+    s[j] = 0.0;
+  }
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    if (solid && !solid[e]) continue;
+    const double xe = x[e], dce = dcf[e], dve = dv[e];
+#pragma unroll
+    for (int j = 0; j < kLamPass; ++j) {
+      if (j >= nl) break;
+      double xt;
+      oc_candidate(xe, dce, dve, lam[j], move, rho_min, xt);
+      s[j] += xt * dve;
+    }
+  }
+  block_sum_n<kLamPass>(s);
+  if (threadIdx.x == 0)
+    for (int j = 0; j < nl; ++j) partials[j * gridDim.x + blockIdx.x] = s[j];
+}
+
+// The sequential search's decisions over the pass's volumes: bracketing
+// doubles l2 while vol(l2) > target and l2 < 1e30; bisection moves l1 or l2 to
+// the midpoint; both stop as oc_decide_k does.
+__global__ void oc_decide_multi_k(OcState* st, const double* __restrict__ partials, int n) {
+  if (st->phase == 2) return;
+  const int nl = st->nlam;
+  double S[kLamPass];
+#pragma unroll
+  for (int j = 0; j < kLamPass; ++j) {
+    double a = 0.0;
+    if (j < nl)
+      for (int i = threadIdx.x; i < n; i += blockDim.x) a += partials[j * n + i];
+    S[j] = a;
+  }
+  block_sum_n<kLamPass>(S);
+  if (threadIdx.x != 0) return;
+  auto stop = [&] { return !(st->steps < 100 && (st->l2 - st->l1) > 1e-6 * (st->l1 + st->l2)); };
+  if (st->phase == 0) {
+    for (int j = 0; j < nl; ++j) {
+      st->vol = S[j];
+      if (S[j] > st->target) {
+        st->l2 *= 2.0;
+        if (st->l2 < 1e30) continue;
+      }
+      st->phase = 1;
+      if (stop()) st->phase = 2;
+      break;
+    }
+  } else {
+    int node = 0;
+    for (int lev = 0; lev < 3; ++lev) {
+      const double lmid = st->lam[node];
+      st->vol = S[node];
+      if (S[node] > st->target) {
+        st->l1 = lmid;
+        node = 2 * node + 2;
+      } else {
+        st->l2 = lmid;
+        node = 2 * node + 1;
+      }
+      st->steps += 1;
+      if (stop()) {
+        st->phase = 2;
+        break;
+      }
+    }
+  }
+  oc_plan(st);
+}
+
 __global__ void oc_final_k(const double* __restrict__ x, const double* __restrict__ dcf,
                            const double* __restrict__ dv, const uint8_t* __restrict__ solid,
                            long long n, double move, double rho_min, const OcState* st,
@@ -323,6 +462,22 @@ void launch_oc_trial(const double* x, const double* dcf, const double* dv, const
                      double* partials, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(oc_trial_k, dim3(nblocks), dim3(RB), 0, s, x, dcf, dv, solid, n, move,
                      rho_min, st, partials);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+
+void launch_oc_init_multi(OcState* st, double target, hipStream_t s) {
+  hipLaunchKernelGGL(oc_init_multi_k, dim3(1), dim3(1), 0, s, st, target);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+void launch_oc_trial_multi(const double* x, const double* dcf, const double* dv,
+                           const uint8_t* solid, long long n, double move, double rho_min,
+                           const OcState* st, double* partials, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(oc_trial_multi_k, dim3(nblocks), dim3(RB), 0, s, x, dcf, dv, solid, n, move,
+                     rho_min, st, partials);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+void launch_oc_decide_multi(OcState* st, const double* partials, int n, hipStream_t s) {
+  hipLaunchKernelGGL(oc_decide_multi_k, dim3(1), dim3(RB), 0, s, st, partials, n);
   TOPOHIP_CHECK(hipGetLastError());
 }
 
@@ -545,6 +700,43 @@ __global__ void mma_coef_k(MmaArgs a, const double* __restrict__ x, const double
   if (threadIdx.x == 0) partials[blockIdx.x] = bsum;
 }
 
+// Lambdas of the next pass (as oc_plan): first 0 and the doublings 1, 2, 4,
+// ... that follow a positive g(0); then further doublings of l2 while below
+// 1e30; then three bisection levels of midpoints in heap order.
+__device__ inline void mma_plan(MmaSearch* st) {
+  if (st->phase == 0) {
+    st->lam[0] = 0.0;
+    double l = 1.0;
+    for (int k = 1; k < kLamPass; ++k) {
+      st->lam[k] = l;
+      l *= 2.0;
+    }
+    st->nlam = kLamPass;
+  } else if (st->phase == 1) {
+    double l = st->l2;
+    int k = 0;
+    while (k < kLamPass && l < 1e30) {
+      st->lam[k++] = l;
+      l *= 2.0;
+    }
+    st->nlam = k;
+  } else if (st->phase == 2) {
+    const double l1 = st->l1, l2 = st->l2;
+    const double m0 = 0.5 * (l1 + l2);
+    const double mL = 0.5 * (l1 + m0), mR = 0.5 * (m0 + l2);
+    st->lam[0] = m0;
+    st->lam[1] = mL;
+    st->lam[2] = mR;
+    st->lam[3] = 0.5 * (l1 + mL);
+    st->lam[4] = 0.5 * (mL + m0);
+    st->lam[5] = 0.5 * (m0 + mR);
+    st->lam[6] = 0.5 * (mR + l2);
+    st->nlam = kLamPass;
+  } else {
+    st->nlam = 0;
+  }
+}
+
 // b = -(vol - target) + sum of the coefficient partials; the search starts at
 // lambda = 0.
 __global__ void mma_init_k(MmaSearch* st, const double* __restrict__ partials, int n,
@@ -560,10 +752,7 @@ __global__ void mma_init_k(MmaSearch* st, const double* __restrict__ partials, i
   st->phase = 0;
   st->it = 0;
   st->steps = 0;
-}
-
-__device__ inline double mma_trial_lambda(const MmaSearch* st) {
-  return st->phase == 0 ? 0.0 : (st->phase == 1 ? st->l2 : 0.5 * (st->l1 + st->l2));
+  mma_plan(st);
 }
 
 __device__ inline double mma_candidate(const MmaArgs& a, long long e, double lam) {
@@ -578,29 +767,46 @@ __device__ inline double mma_candidate(const MmaArgs& a, long long e, double lam
 __global__ void mma_trial_k(MmaArgs a, const uint8_t* __restrict__ dset, long long n,
                             const MmaSearch* st, double* __restrict__ partials) {
   if (st->phase == 3) return;
-  const double lam = mma_trial_lambda(st);
-  double s = 0.0;
+  const int nl = st->nlam;
+  double lam[kLamPass], s[kLamPass];
+#pragma unroll
+  for (int j = 0; j < kLamPass; ++j) {
+    lam[j] = j < nl ? st->lam[j] : 0.0;
+    s[j] = 0.0;
+  }
   for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
        e += static_cast<long long>(gridDim.x) * blockDim.x) {
     if (dset && !dset[e]) continue;
-    const double xt = mma_candidate(a, e, lam);
-    s += a.p1[e] / (a.upp[e] - xt) + a.q1[e] / (xt - a.low[e]);
+    const double L = a.low[e], U = a.upp[e], p1 = a.p1[e], q1 = a.q1[e];
+#pragma unroll
+    for (int j = 0; j < kLamPass; ++j) {
+      if (j >= nl) break;
+      const double xt = mma_candidate(a, e, lam[j]);
+      s[j] += p1 / (U - xt) + q1 / (xt - L);
+    }
   }
-  s = block_sum(s);
-  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+  block_sum_n<kLamPass>(s);
+  if (threadIdx.x == 0)
+    for (int j = 0; j < nl; ++j) partials[j * gridDim.x + blockIdx.x] = s[j];
 }
 
-// One decision of the search: g(0) > 0 opens the bracket [0, 1], doubled while
-// l2 < 1e30 and g(l2) > 0; then at most 100 bisections while
-// (l2 - l1) > 1e-9 (1 + l1 + l2); lambda = (l1 + l2) / 2 (0 when g(0) <= 0).
+// The sequential search's decisions over the pass's values of g: g(0) > 0
+// opens the bracket [0, 1], doubled while l2 < 1e30 and g(l2) > 0; then at
+// most 100 bisections while (l2 - l1) > 1e-9 (1 + l1 + l2); lambda =
+// (l1 + l2) / 2 (0 when g(0) <= 0).
 __global__ void mma_decide_k(MmaSearch* st, const double* __restrict__ partials, int n) {
   if (st->phase == 3) return;
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
-  s = block_sum(s);
+  const int nl = st->nlam;
+  double S[kLamPass];
+#pragma unroll
+  for (int j = 0; j < kLamPass; ++j) {
+    double a = 0.0;
+    if (j < nl)
+      for (int i = threadIdx.x; i < n; i += blockDim.x) a += partials[j * n + i];
+    S[j] = a;
+  }
+  block_sum_n<kLamPass>(S);
   if (threadIdx.x != 0) return;
-  const double gv = s - st->b;
-  st->steps += 1;
   auto bisect_or_finish = [&]() {
     if (st->it < 100 && (st->l2 - st->l1) > 1e-9 * (1.0 + st->l1 + st->l2)) {
       st->phase = 2;
@@ -609,29 +815,46 @@ __global__ void mma_decide_k(MmaSearch* st, const double* __restrict__ partials,
       st->phase = 3;
     }
   };
+  int j = 0;
   if (st->phase == 0) {
-    if (gv > 0.0) {
+    st->steps += 1;
+    if (!(S[0] - st->b > 0.0)) {
+      st->lambda = 0.0;
+      st->phase = 3;
+    } else {
       st->l1 = 0.0;
       st->l2 = 1.0;
       st->phase = 1;
-    } else {
-      st->lambda = 0.0;
-      st->phase = 3;
     }
-  } else if (st->phase == 1) {
-    if (gv > 0.0) {
-      st->l2 *= 2.0;
-      if (!(st->l2 < 1e30)) bisect_or_finish();
-    } else {
+    j = 1;
+  }
+  if (st->phase == 1) {
+    for (; j < nl; ++j) {
+      st->steps += 1;
+      if (S[j] - st->b > 0.0) {
+        st->l2 *= 2.0;
+        if (st->l2 < 1e30) continue;
+      }
+      bisect_or_finish();
+      break;
+    }
+  } else if (st->phase == 2) {
+    int node = 0;
+    for (int lev = 0; lev < 3 && st->phase == 2; ++lev) {
+      st->steps += 1;
+      const double lm = st->lam[node];
+      if (S[node] - st->b > 0.0) {
+        st->l1 = lm;
+        node = 2 * node + 2;
+      } else {
+        st->l2 = lm;
+        node = 2 * node + 1;
+      }
+      st->it += 1;
       bisect_or_finish();
     }
-  } else {
-    const double lm = 0.5 * (st->l1 + st->l2);
-    if (gv > 0.0) st->l1 = lm;
-    else st->l2 = lm;
-    st->it += 1;
-    bisect_or_finish();
   }
+  mma_plan(st);
 }
 
 // xnew = candidate(lambda) on the design set, 0 elsewhere; change partials.
