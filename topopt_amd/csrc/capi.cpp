@@ -168,15 +168,14 @@ int topohip_solve(topohip_problem* p, const double* youngs, double tol, int max_
     upload_youngs(P, youngs);
     Timers t;
     build_hierarchy(P, t);
-    DBuf<double> dg;
     const long long nd = P.g.ndof();
     if (guess) {
-      dg.ensure(nd);
-      TOPOHIP_CHECK(hipMemcpyAsync(dg.p, guess, nd * sizeof(double), hipMemcpyHostToDevice, P.s));
+      P.guess.ensure(nd);
+      TOPOHIP_CHECK(hipMemcpyAsync(P.guess.p, guess, nd * sizeof(double), hipMemcpyHostToDevice, P.s));
     }
     topohip_cginfo ci{};
     try {
-      solve_cases(P, 1, tol, max_it, guess ? dg.p : nullptr, &ci);
+      solve_cases(P, 1, tol, max_it, guess ? P.guess.p : nullptr, &ci);
     } catch (...) {
       ci.t_mg_build_ms = t.mg_build;
       if (info) *info = ci;
@@ -217,13 +216,12 @@ int topohip_simp_compliance_warm(topohip_problem* p, const double* xphys, double
     const long long nv = P.g.elems(), nd = P.g.ndof();
     P.xphys.ensure(nv);
     TOPOHIP_CHECK(hipMemcpyAsync(P.xphys.p, xphys, nv * sizeof(double), hipMemcpyHostToDevice, P.s));
-    DBuf<double> dg;
     if (guess) {
-      dg.ensure(nd);
-      TOPOHIP_CHECK(hipMemcpyAsync(dg.p, guess, nd * sizeof(double), hipMemcpyHostToDevice, P.s));
+      P.guess.ensure(nd);
+      TOPOHIP_CHECK(hipMemcpyAsync(P.guess.p, guess, nd * sizeof(double), hipMemcpyHostToDevice, P.s));
     }
     *c = simp_compliance_dev(P, P.xphys.p, E0, penalty, rho_min, tol, max_it, info,
-                             guess ? dg.p : nullptr);
+                             guess ? P.guess.p : nullptr);
     if (dc)
       TOPOHIP_CHECK(hipMemcpyAsync(dc, P.dc.p, nv * sizeof(double), hipMemcpyDeviceToHost, P.s));
     if (u_out)

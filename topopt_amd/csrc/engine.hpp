@@ -119,6 +119,7 @@ struct Problem {
 
   // result fields and run records
   DBuf<double> fu, fvm;
+  DBuf<double> guess;  // warm-start displacement uploaded by the host entry points
   DBuf<float> fvm32, fu32;
   DBuf<uint16_t> fh16;
   topohip_iteration_cb observer = nullptr;  // called after every design iteration

@@ -75,20 +75,28 @@ void observe_job(const topohip_iteration_obs* o, void* user) {
 void write_job_records(Problem& P, const topohip_job& j, const topohip_result& r,
                        const std::vector<double>& xphys, const std::string& dir, JobOut& jo) {
   const size_t nv = xphys.size(), nd = static_cast<size_t>(P.g.ndof());
-  std::vector<float> vm(nv), du(nd);
-  result_fields(P, 1.0, nullptr, nullptr, vm.data(), du.data());
+  // A run whose final solve did not converge has no certified field: it gets
+  // design.bin only (the reference's run leaves such a rung out of fields.bin).
+  const bool have_u = P.u_valid && !r.non_convergent;
+  std::vector<float> vm, du;
   float vmax = 0.f;
-  for (float v : vm) vmax = std::max(vmax, v);
+  if (have_u) {
+    vm.resize(nv);
+    du.resize(nd);
+    result_fields(P, 1.0, nullptr, nullptr, vm.data(), du.data());
+    for (float v : vm) vmax = std::max(vmax, v);
+  }
   const topohip_grid_frame g{j.nx, j.ny, j.nz, {0.0, 0.0, 0.0}, 1.0};
   topohip_design_variant d{};
   d.requested_volume_fraction = j.volfrac;
   d.achieved_volume_fraction = r.volume_fraction;
   d.max_von_mises_mpa = vmax;
-  d.accepted = 1;
+  d.accepted = have_u ? 1 : 0;
   d.iterations = r.iterations;
   d.applied_build_dir[2] = 1.0;
   d.density = xphys.data();
   write_design_file(dir + "/design.bin", g, 1, &d);
+  if (!have_u) return;
   topohip_fields_variant f{};
   f.requested_volume_fraction = j.volfrac;
   f.von_mises = vm.data();
