@@ -31,6 +31,14 @@
 //                       the reference's anchors and round-trip bound
 //                       (core/tests/unit/test_observability.cpp:55-74)
 //   fnv1a64             the published FNV-1a 64-bit test vectors
+//   mma_update          (tests/test_oracle_mma.py) convergence to the closed-form
+//                       optimum of a separable problem (sum c/x at fixed volume),
+//                       bounds and move box on every step; the loop's descent
+//   mma_objective_plateau  the reference's hand-built curves and verdicts
+//                       (core/tests/unit/test_simp.cpp:1245-1435)
+//   simp_compliance     warm start: a guess at the solution stops after one
+//   (initial_guess)     iteration with the same compliance; a neighbouring field
+//                       cuts the iteration count and keeps the cold answer
 //   masked loop         (tests/test_oracle_masked.py) all-Active mask equals
 //                       the plain loop bit for bit; masked filter against a
 //                       brute-force evaluation of its definition; no bleed of
