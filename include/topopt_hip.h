@@ -180,12 +180,14 @@ typedef struct {
 } topohip_mma_options;
 
 /* simp_optimize with options.updater = MMA (core/src/simp/simp.cpp:2305-2311):
- * the plain (unprojected, unmasked) loop with mma_update as the design update
- * and stage_should_stop (:1219-1232) as its stop rule. mma NULL = the
- * reference's defaults. Outputs as topohip_optimize. */
+ * the unprojected loop with mma_update as the design update and
+ * stage_should_stop (:1219-1232) as its stop rule. mask (nullable) selects the
+ * masked loop as topohip_optimize_masked, with mma_update_masked
+ * (:2736-2783, :3351-3357): only Active voxels move, the others come back 0.
+ * mma NULL = the reference's defaults. Outputs as topohip_optimize. */
 int topohip_optimize_mma(topohip_problem* p, const topohip_options* opt,
-                         const topohip_mma_options* mma, double* design_out, double* xphys_out,
-                         double* history, topohip_result* result);
+                         const topohip_mma_options* mma, const uint8_t* mask, double* design_out,
+                         double* xphys_out, double* history, topohip_result* result);
 
 /* One mma_update (core/src/simp/simp.cpp:1433-1488, core/src/simp/mma_joint.cpp:
  * 18-118) on the handle's solid voxels with the density filter of `radius`:

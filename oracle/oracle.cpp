@@ -1576,6 +1576,11 @@ SimpResult simp_optimize_masked(const Grid& g, const SimpParams& p,
     else if (eff[e] == kFrozenSolid) x[e] = 1.0;
   }
   SimpResult res;
+  std::vector<std::uint8_t> active(g.voxels(), 0);
+  for (std::size_t e = 0; e < eff.size(); ++e) active[e] = eff[e] == kActive;
+  MmaState mma_state;
+  int mma_iter = 0;
+  std::vector<double> curve;
   for (int it = 0; it < o.max_iterations; ++it) {
     std::vector<double> xphys = f.filter_density(x);
     apply_mask_pins(eff, xphys);
@@ -1590,8 +1595,11 @@ SimpResult simp_optimize_masked(const Grid& g, const SimpParams& p,
       res.non_convergent = true;
       break;
     }
+    // core/src/simp/simp.cpp:3351-3357 (MMA) or the OC update
     const std::vector<double> xn =
-        oc_update_masked(g, f, eff, x, c.dc, vf_target, o.move, p.density_min);
+        o.mma ? mma_update(mma_state, ++mma_iter, g, f, x, c.dc, vf_target, o.move,
+                           p.density_min, nullptr, &active)
+              : oc_update_masked(g, f, eff, x, c.dc, vf_target, o.move, p.density_min);
     double change = 0.0;
     for (std::size_t e = 0; e < x.size(); ++e) change = std::max(change, std::fabs(xn[e] - x[e]));
     x = xn;
@@ -1599,7 +1607,13 @@ SimpResult simp_optimize_masked(const Grid& g, const SimpParams& p,
     ++res.iterations;
     const std::vector<double> xa = f.filter_density(x);
     res.history.push_back({c.compliance, change, active_volfrac(xa), c.cg.iterations});
-    if (change < o.change_tol) { res.converged = true; break; }
+    curve.push_back(c.compliance);
+    const bool stop = (o.mma && o.mma_plateau_window > 0)
+                          ? mma_objective_plateau(curve, o.mma_plateau_window, o.mma_plateau_tol,
+                                                  o.mma_plateau_min_drop,
+                                                  o.mma_plateau_flat_windows)
+                          : change < o.change_tol;
+    if (stop) { res.converged = true; break; }
   }
   res.design = x;
   res.physical_density = f.filter_density(x);
@@ -1703,7 +1717,7 @@ namespace oracle {
 std::vector<double> mma_update(MmaState& st, int mma_iter, const Grid& g, const DensityFilter& f,
                                const std::vector<double>& x, const std::vector<double>& dcomp,
                                double volfrac, double move, double density_min,
-                               double* lambda_out) {
+                               double* lambda_out, const std::vector<std::uint8_t>* dset) {
   const std::size_t N = g.voxels();
   if (x.size() != N || dcomp.size() != N)
     throw std::invalid_argument("mma_update: vector size != voxel_count");
@@ -1712,12 +1726,13 @@ std::vector<double> mma_update(MmaState& st, int mma_iter, const Grid& g, const 
   std::vector<std::size_t> dof;
   for (int k = 0; k < g.nz; ++k)
     for (int j = 0; j < g.ny; ++j)
-      for (int i = 0; i < g.nx; ++i)
-        if (g.is_solid(i, j, k)) {
-          const std::size_t e = g.index(i, j, k);
+      for (int i = 0; i < g.nx; ++i) {
+        const std::size_t e = g.index(i, j, k);
+        if (dset ? (*dset)[e] != 0 : g.is_solid(i, j, k)) {
           ones[e] = 1.0;
           dof.push_back(e);
         }
+      }
   const std::vector<double> dv = f.filter_sensitivity(ones);
   const double target = volfrac * static_cast<double>(dof.size());
   double scale = 0.0;

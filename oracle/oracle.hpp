@@ -367,10 +367,13 @@ std::uint64_t fnv1a64(const void* data, std::size_t n);
 // core/src/simp/mma_joint.cpp:18-118. The state carries the asymptotes and the
 // last two iterates across the calls of one run; mma_iter is 1-based.
 struct MmaState { std::vector<double> low, upp, xold1, xold2; };
+// dset (nullable): the design set of mma_update_masked (:2736-2783, the Active
+// voxels; f then the Active-only filter); null = the solid voxels.
 std::vector<double> mma_update(MmaState& st, int mma_iter, const Grid& g, const DensityFilter& f,
                                const std::vector<double>& x, const std::vector<double>& dcomp,
                                double volfrac, double move, double density_min,
-                               double* lambda_out = nullptr);
+                               double* lambda_out = nullptr,
+                               const std::vector<std::uint8_t>* dset = nullptr);
 // core/src/simp/simp.cpp:1081-1134.
 bool mma_objective_plateau(const std::vector<double>& c, int window, double rel_tol,
                            double min_drop, int min_flat_windows);

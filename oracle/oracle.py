@@ -324,19 +324,22 @@ def fnv1a64(data):
 
 def optimize_mma(s, volfrac, radius, iterations, E0=1.0, nu=0.3, penalty=3.0, rho_min=1e-3,
                  move=0.2, change_tol=0.0, cg_tol=1e-8, plateau_window=10, plateau_tol=1e-3,
-                 plateau_min_drop=0.05, plateau_flat_windows=3):
-    """simp_optimize with SimpUpdater::MMA (plain stage, plateau stop)."""
+                 plateau_min_drop=0.05, plateau_flat_windows=3, mask=None):
+    """simp_optimize with SimpUpdater::MMA (plateau stop); with a design mask
+    the masked loop with mma_update_masked."""
     design = np.zeros(s.nvox)
     xphys = np.zeros(s.nvox)
     hist = np.zeros(4 * iterations)
     it = C.c_int(0)
     comp = C.c_double(0)
     vf = C.c_double(0)
+    m, mp = (None, None) if mask is None else _mask(s, mask)  # m keeps the buffer alive
     _check(lib().oracle_optimize_mma(
         C.byref(s.p), C.c_double(E0), C.c_double(nu), C.c_double(penalty), C.c_double(rho_min),
         C.c_double(volfrac), C.c_double(radius), C.c_double(move), int(iterations),
         C.c_double(change_tol), C.c_double(cg_tol), int(plateau_window), C.c_double(plateau_tol),
-        C.c_double(plateau_min_drop), int(plateau_flat_windows), _ptr(design), _ptr(xphys),
+        C.c_double(plateau_min_drop), int(plateau_flat_windows),
+        mp, _ptr(design), _ptr(xphys),
         _ptr(hist), C.byref(it), C.byref(comp), C.byref(vf)))
     n = it.value
     return {"design": design, "xphys": xphys, "history": hist[: 4 * n].reshape(n, 4),

@@ -256,7 +256,8 @@ int oracle_optimize(const OracleProblem* p, double E0, double nu, double penalty
 int oracle_optimize_mma(const OracleProblem* p, double E0, double nu, double penalty,
                         double rho_min, double volfrac, double radius, double move,
                         int max_iterations, double change_tol, double cg_tol, int window,
-                        double plateau_tol, double min_drop, int flat_windows, double* design,
+                        double plateau_tol, double min_drop, int flat_windows,
+                        const std::uint8_t* mask, double* design,
                         double* xphys, double* hist, int* iterations, double* compliance,
                         double* final_volfrac) {
   return guarded([&] {
@@ -274,7 +275,10 @@ int oracle_optimize_mma(const OracleProblem* p, double E0, double nu, double pen
     o.mma_plateau_tol = plateau_tol;
     o.mma_plateau_min_drop = min_drop;
     o.mma_plateau_flat_windows = flat_windows;
-    const oracle::SimpResult r = oracle::simp_optimize(g, sp, bcs_of(p), loads_of(p), o);
+    const oracle::SimpResult r =
+        mask ? oracle::simp_optimize_masked(g, sp, bcs_of(p), loads_of(p), o,
+                                            std::vector<std::uint8_t>(mask, mask + g.voxels()))
+             : oracle::simp_optimize(g, sp, bcs_of(p), loads_of(p), o);
     if (design) std::memcpy(design, r.design.data(), r.design.size() * sizeof(double));
     if (xphys)
       std::memcpy(xphys, r.physical_density.data(), r.physical_density.size() * sizeof(double));

@@ -94,3 +94,36 @@ def test_mma_rejects_bad_inputs():
             p.mma_update(1.5, 3, x, -np.ones(128), 0.3)   # no state yet for mma_iter > 2
         with pytest.raises(ValueError):
             p.mma_update(1.5, 1, x, -np.ones(128), 1.5)   # volume fraction out of range
+
+
+def test_masked_mma_matches_oracle_f64():
+    nx, ny, nz = 16, 6, 6
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    mask = np.zeros((nz, ny, nx), np.uint8)
+    mask[:, :, -2:] = oracle.FROZEN_SOLID
+    mask[: nz // 2, : ny // 2, nx // 2] = oracle.FROZEN_VOID
+    m = mask.ravel()
+    solid = (m != oracle.FROZEN_VOID).astype(np.uint8)   # the analysis grid
+    s = oracle.Setup(nx, ny, nz, bcs=bcs, loads=loads)
+    ref = oracle.optimize_mma(s, 0.3, 1.5, 10, cg_tol=1e-10, plateau_window=0, mask=m)
+    with ta.Problem(nx, ny, nz, bcs, loads, solid=solid, precision=ta.PREC_F64) as p:
+        out = p.optimize_mma(0.3, 1.5, 10, cg_tol=1e-10, plateau_window=0, mask=m)
+    hc, hr = out["history"], ref["history"]
+    print("gpu", hc[:, 0], "\noracle", hr[:, 0], flush=True)
+    assert out["iterations"] == ref["iterations"] == 10
+    assert np.max(np.abs(hc[:, 0] - hr[:, 0]) / hr[:, 0]) < 1e-6
+    assert np.max(np.abs(hc[:, 1:3] - hr[:, 1:3])) < 1e-6
+    assert np.abs(out["design"] - ref["design"]).max() < 1e-5
+    assert np.all(out["xphys"][m == oracle.FROZEN_SOLID] == 1.0)
+    assert np.all(out["xphys"][m == oracle.FROZEN_VOID] == 0.0)
+
+
+def test_masked_mma_all_active_equals_plain_on_device():
+    nx, ny, nz = 16, 6, 6
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    with ta.Problem(nx, ny, nz, bcs, loads) as p:
+        a = p.optimize_mma(0.3, 1.5, 8, cg_tol=1e-8, plateau_window=0)
+        b = p.optimize_mma(0.3, 1.5, 8, cg_tol=1e-8, plateau_window=0,
+                           mask=np.zeros(nx * ny * nz, np.uint8))
+    assert np.array_equal(a["history"], b["history"])
+    assert np.array_equal(a["design"], b["design"])

@@ -91,3 +91,31 @@ def test_mma_loop_on_the_cantilever():
     ref = oracle.optimize(s, 0.3, 1.5, 30, cg_tol=1e-8)
     # MMA and OC reach designs of similar stiffness on this problem
     assert abs(out["compliance"] - ref["compliance"]) / ref["compliance"] < 0.1
+
+
+def test_masked_mma_with_an_all_active_mask_is_the_plain_loop():
+    """mma_update_masked with an all-Active mask on an all-solid grid
+    reproduces mma_update bit for bit (core/src/simp/simp.cpp:2733-2735)."""
+    nx, ny, nz = 12, 4, 4
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    s = oracle.Setup(nx, ny, nz, bcs=bcs, loads=loads)
+    a = oracle.optimize_mma(s, 0.3, 1.5, 8, cg_tol=1e-10, plateau_window=0)
+    b = oracle.optimize_mma(s, 0.3, 1.5, 8, cg_tol=1e-10, plateau_window=0,
+                            mask=np.zeros(nx * ny * nz, np.uint8))
+    assert np.array_equal(a["history"], b["history"])
+    assert np.array_equal(a["design"], b["design"])
+
+
+def test_masked_mma_moves_only_active_voxels():
+    nx, ny, nz = 12, 4, 4
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    s = oracle.Setup(nx, ny, nz, bcs=bcs, loads=loads)
+    mask = np.zeros((nz, ny, nx), np.uint8)
+    mask[:, :, -2:] = oracle.FROZEN_SOLID
+    out = oracle.optimize_mma(s, 0.3, 1.5, 6, cg_tol=1e-10, plateau_window=0, mask=mask.ravel())
+    m = mask.ravel()
+    assert np.all(out["design"][m == oracle.FROZEN_SOLID] == 0.0)   # mma_update_joint's zeros
+    assert np.all(out["xphys"][m == oracle.FROZEN_SOLID] == 1.0)    # pinned physical density
+    assert abs(out["volume_fraction"] - 0.3) < 5e-3
+    h = out["history"]
+    assert h[0, 1] == 1.0 and np.all(h[1:, 1] < 1.0)   # the first step zeroes the frozen x

@@ -365,8 +365,9 @@ class Problem:
     def optimize_mma(self, volfrac, rmin, iterations, penalty=3.0, move=0.2, E0=1.0, rho_min=1e-3,
                      change_tol=0.0, cg_tol=1e-8, cg_max_iterations=0, continuation=None,
                      plateau_window=10, plateau_tol=1e-3, plateau_min_drop=0.05,
-                     plateau_flat_windows=3):
-        """simp_optimize with SimpUpdater::MMA (plain loop, plateau stop)."""
+                     plateau_flat_windows=3, mask=None):
+        """simp_optimize with SimpUpdater::MMA (plateau stop); with a design
+        mask the masked loop (mma_update_masked)."""
         o = Options(volfrac, rmin, move, penalty, E0, rho_min, change_tol, cg_tol, int(iterations),
                     int(cg_max_iterations), 0.0, 0.0, 0.0, 0)
         if continuation:
@@ -376,8 +377,9 @@ class Problem:
         xphys = np.zeros(self.nvox)
         hist = np.zeros(4 * iterations)
         res = Result()
-        _check(lib().topohip_optimize_mma(self._h, C.byref(o), C.byref(m), _p(design), _p(xphys),
-                                          _p(hist), C.byref(res)))
+        mk = None if mask is None else self._mask(mask)
+        _check(lib().topohip_optimize_mma(self._h, C.byref(o), C.byref(m), _p(mk), _p(design),
+                                          _p(xphys), _p(hist), C.byref(res)))
         n = res.iterations
         out = res.as_dict()
         out.update(design=design, xphys=xphys, history=hist[: 4 * n].reshape(n, 4))

@@ -296,8 +296,8 @@ int topohip_mma_update(topohip_problem* p, double radius, int mma_iter, const do
 }
 
 int topohip_optimize_mma(topohip_problem* p, const topohip_options* opt,
-                         const topohip_mma_options* mma, double* design_out, double* xphys_out,
-                         double* history, topohip_result* result) {
+                         const topohip_mma_options* mma, const uint8_t* mask, double* design_out,
+                         double* xphys_out, double* history, topohip_result* result) {
   return guarded([&] {
     need(p, "problem"); need(opt, "options");
     MmaOpts m;
@@ -307,7 +307,7 @@ int topohip_optimize_mma(topohip_problem* p, const topohip_options* opt,
       m.plateau_min_drop = mma->plateau_min_drop;
       m.plateau_flat_windows = mma->plateau_flat_windows;
     }
-    optimize(*p->P, *opt, design_out, xphys_out, history, result, nullptr, nullptr, 0.5, &m);
+    optimize(*p->P, *opt, design_out, xphys_out, history, result, mask, nullptr, 0.5, &m);
   });
 }
 

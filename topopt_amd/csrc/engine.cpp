@@ -1543,11 +1543,11 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
                                     "iterations >= 1");
   }
   if (o.change_tol < 0.0) throw Error(TOPOHIP_EINVAL, "simp_optimize: change_tol must be >= 0");
-  // validate_updater_options: projection stages run with the OC updater only,
-  // and the masked MMA update is not built here
-  if (mma && ((proj && !proj->empty()) || mask))
-    throw Error(TOPOHIP_EINVAL, "simp_optimize: the MMA updater runs the plain (unprojected, "
-                                "unmasked) loop");
+  // validate_updater_options: projection stages run with the OC updater only.
+  // Masked, the MMA update moves the Active voxels and returns 0 elsewhere
+  // (mma_update_masked, core/src/simp/simp.cpp:2736-2783).
+  if (mma && proj && !proj->empty())
+    throw Error(TOPOHIP_EINVAL, "simp_optimize: projection stages need the OC updater");
   int mma_iter = 0;
   std::vector<double> curve;  // compliance history for the MMA plateau stop
   std::vector<std::pair<double, int>> stages;
