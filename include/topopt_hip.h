@@ -385,6 +385,21 @@ int topohip_csv_close(topohip_csv_writer* w);
 int topohip_write_density_snapshot(topohip_problem* p, const char* dir, const char* basename,
                                    const double* origin, const double* density, size_t rung,
                                    int iteration, int boundary);
+/* SnapshotCapture (core/include/topopt/observability.hpp:236-259,
+ * core/src/simp/observability.cpp:426-455): the density-snapshot feed's
+ * cadence and cap. Rung boundaries are always written and never evicted;
+ * other iterations every every_n (<= 0: 1); once more than cap (> 0)
+ * per-iteration snapshots exist the oldest (.f16 and .json) is deleted.
+ * Basenames are snap_rung<r>_iter<4-digit i>[_boundary]. */
+typedef struct topohip_snapshots topohip_snapshots;
+int topohip_snapshots_create(const char* dir, int every_n, int cap, topohip_snapshots** out);
+/* Consider one event; *written (nullable) = 1 when a snapshot was written.
+ * density NULL = the handle's resident physical density. */
+int topohip_snapshots_capture(topohip_snapshots* sc, topohip_problem* p, const double* origin,
+                              const double* density, size_t rung, int iteration, int boundary,
+                              int* written);
+int topohip_snapshots_counts(const topohip_snapshots* sc, size_t* written, size_t* evicted);
+int topohip_snapshots_destroy(topohip_snapshots* sc);
 /* read_density_f16 (core/src/simp/observability.cpp:410-425): EINVAL unless
  * the file holds exactly count halves. */
 int topohip_read_density_f16(const char* path, size_t count, float* out);

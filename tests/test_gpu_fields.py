@@ -169,3 +169,25 @@ def test_run_records_of_a_design(tmp_path):
     assert np.array_equal(np.frombuffer(b, "<f4", vm32.size, off), vm32)
     assert np.array_equal(np.frombuffer(b, "<f4", du32.size, off + 4 * vm32.size),
                           u.astype(np.float32))
+
+
+def test_snapshot_capture_cadence_and_eviction(tmp_path):
+    """The reference's cadence + cap case (core/tests/unit/test_observability.cpp:
+    221-244): every 10 iterations, cap 2, a boundary at 26 kept through
+    eviction, the oldest per-iteration snapshot removed at 30."""
+    bcs, loads = problems.cantilever(3, 3, 3)
+    d = tmp_path / "caps"
+    with ta.Problem(3, 3, 3, bcs, loads) as p, ta.SnapshotCapture(d, 10, 2) as cap:
+        dens = np.full(27, 0.4)
+        got = [cap.capture(p, 0, it, False, density=dens) for it in range(1, 26)]
+        assert [i + 1 for i, g in enumerate(got) if g] == [10, 20]
+        assert cap.capture(p, 0, 26, True, density=dens)
+        assert cap.counts() == (3, 0)
+        assert cap.capture(p, 0, 30, False, density=dens)
+        assert cap.counts() == (4, 1)
+    assert not (d / "snap_rung0_iter0010.f16").exists()
+    assert not (d / "snap_rung0_iter0010.json").exists()
+    for name in ("snap_rung0_iter0026_boundary", "snap_rung0_iter0020", "snap_rung0_iter0030"):
+        assert (d / f"{name}.f16").exists() and (d / f"{name}.json").exists()
+    raw = np.fromfile(d / "snap_rung0_iter0030.f16", dtype="<u2")
+    assert np.array_equal(raw, np.full(27, np.float16(0.4)).view(np.uint16))

@@ -545,4 +545,4 @@ def version():
 from .records import (CsvWriter, DesignVariant, FieldsVariant, GridFrame,  # noqa: E402
                       IterationObs, design_fingerprint, format_progress, format_variant,
                       iterations_csv_header, iterations_csv_row, read_density_f16,
-                      read_design_file, write_design_file, write_fields_file)
+                      read_design_file, write_design_file, write_fields_file, SnapshotCapture)

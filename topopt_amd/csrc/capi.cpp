@@ -556,6 +556,55 @@ int topohip_write_density_snapshot(topohip_problem* p, const char* dir, const ch
   });
 }
 
+}  // extern "C"
+
+struct topohip_snapshots {
+  std::unique_ptr<SnapshotCapture> cap;
+};
+
+extern "C" {
+
+int topohip_snapshots_create(const char* dir, int every_n, int cap, topohip_snapshots** out) {
+  return guarded([&] {
+    need(dir, "dir"); need(out, "out");
+    auto h = std::make_unique<topohip_snapshots>();
+    h->cap = std::make_unique<SnapshotCapture>(dir, every_n, cap);
+    *out = h.release();
+  });
+}
+
+int topohip_snapshots_capture(topohip_snapshots* sc, topohip_problem* p, const double* origin,
+                              const double* density, size_t rung, int iteration, int boundary,
+                              int* written) {
+  return guarded([&] {
+    need(sc, "snapshots"); need(p, "problem");
+    const bool b = boundary != 0;
+    const bool w = sc->cap->wants(iteration, b);
+    if (written) *written = w ? 1 : 0;
+    if (!w) return;
+    Problem& P = *p->P;
+    std::vector<uint16_t> halves(static_cast<size_t>(P.g.elems()));
+    density_halves(P, density, halves.data());
+    const double zero[3] = {0.0, 0.0, 0.0};
+    const std::string base = SnapshotCapture::basename(rung, iteration, b);
+    write_density_snapshot(sc->cap->dir(), base, P.g.ex, P.g.ey, P.g.ez, P.h,
+                           origin ? origin : zero, halves, rung, iteration, b);
+    sc->cap->written(base, b);
+  });
+}
+
+int topohip_snapshots_counts(const topohip_snapshots* sc, size_t* written, size_t* evicted) {
+  return guarded([&] {
+    need(sc, "snapshots");
+    if (written) *written = sc->cap->n_written();
+    if (evicted) *evicted = sc->cap->n_evicted();
+  });
+}
+
+int topohip_snapshots_destroy(topohip_snapshots* sc) {
+  return guarded([&] { delete sc; });
+}
+
 int topohip_read_density_f16(const char* path, size_t count, float* out) {
   return guarded([&] {
     need(path, "path"); need(out, "out");

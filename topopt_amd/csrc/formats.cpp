@@ -230,6 +230,26 @@ void write_density_snapshot(const std::string& dir, const std::string& basename,
   if (std::fclose(f) != 0 || !ok) io("write_density_snapshot: failed writing " + js);
 }
 
+std::string SnapshotCapture::basename(std::size_t rung, int iteration, bool boundary) {
+  char base[96];
+  std::snprintf(base, sizeof(base), "snap_rung%zu_iter%04d%s", rung, iteration,
+                boundary ? "_boundary" : "");
+  return base;
+}
+
+void SnapshotCapture::written(const std::string& base, bool boundary) {
+  ++written_;
+  if (boundary) return;
+  iter_snaps_.push_back(base);
+  while (cap_ > 0 && static_cast<int>(iter_snaps_.size()) > cap_) {
+    const std::string victim = iter_snaps_.front();
+    iter_snaps_.erase(iter_snaps_.begin());
+    std::remove(join(dir_, victim + ".f16").c_str());
+    std::remove(join(dir_, victim + ".json").c_str());
+    ++evicted_;
+  }
+}
+
 // IEEE binary16 -> float, exact (every half is a float).
 float half_to_float(std::uint16_t h) {
   const std::uint32_t sign = static_cast<std::uint32_t>(h & 0x8000u) << 16;

@@ -50,6 +50,32 @@ void write_density_snapshot(const std::string& dir, const std::string& basename,
                             const std::vector<std::uint16_t>& halves, std::size_t rung,
                             int iteration, bool boundary);
 std::vector<float> read_density_f16(const std::string& path, std::size_t count);
+
+// SnapshotCapture (core/src/simp/observability.cpp:426-455): rung boundaries
+// always written and never evicted; per-iteration snapshots every every_n
+// iterations, the oldest evicted once more than cap (> 0) are kept.
+class SnapshotCapture {
+ public:
+  SnapshotCapture(std::string dir, int every_n, int cap)
+      : dir_(std::move(dir)), every_n_(every_n > 0 ? every_n : 1), cap_(cap) {}
+  // Whether this event is written; the basename it would take.
+  bool wants(int iteration, bool boundary) const {
+    return boundary || iteration % every_n_ == 0;
+  }
+  static std::string basename(std::size_t rung, int iteration, bool boundary);
+  // Record a written snapshot and apply the cap.
+  void written(const std::string& base, bool boundary);
+  const std::string& dir() const { return dir_; }
+  std::size_t n_written() const { return written_; }
+  std::size_t n_evicted() const { return evicted_; }
+
+ private:
+  std::string dir_;
+  int every_n_;
+  int cap_;
+  std::size_t written_ = 0, evicted_ = 0;
+  std::vector<std::string> iter_snaps_;
+};
 float half_to_float(std::uint16_t h);
 
 std::uint64_t design_fingerprint(const double* d, std::size_t n);

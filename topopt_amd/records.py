@@ -106,6 +106,11 @@ def _setup():
                                         C.c_longlong]
     L.topohip_csv_close.argtypes = [C.c_void_p]
     L.topohip_read_density_f16.argtypes = [C.c_char_p, C.c_size_t, C.c_void_p]
+    L.topohip_snapshots_capture.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    L.topohip_snapshots_destroy.argtypes = [C.c_void_p]
+    L.topohip_snapshots_counts.argtypes = [C.c_void_p, C.POINTER(C.c_size_t),
+                                           C.POINTER(C.c_size_t)]
     L._records_ready = True
     return L
 
@@ -230,3 +235,40 @@ def format_variant(vf, achieved, printed, margin, mesh):
                                            C.c_double(printed), C.c_double(margin),
                                            str(mesh).encode(), buf, 4096))
     return buf.value.decode()
+
+
+class SnapshotCapture:
+    """SnapshotCapture: the density-snapshot cadence (every_n) and per-job cap
+    with oldest eviction; rung boundaries are always kept."""
+
+    def __init__(self, directory, every_n, cap):
+        self._h = C.c_void_p()
+        _check(_setup().topohip_snapshots_create(str(directory).encode(), int(every_n), int(cap),
+                                                 C.byref(self._h)))
+
+    def capture(self, problem, rung, iteration, boundary=False, density=None,
+                origin=(0.0, 0.0, 0.0)):
+        d = None if density is None else np.ascontiguousarray(density, dtype=np.float64).ravel()
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        w = C.c_int(0)
+        _check(_setup().topohip_snapshots_capture(self._h, problem._h, _p(o), _p(d),
+                                                  C.c_size_t(rung), int(iteration),
+                                                  int(bool(boundary)), C.byref(w)))
+        return bool(w.value)
+
+    def counts(self):
+        w = C.c_size_t(0)
+        e = C.c_size_t(0)
+        _check(_setup().topohip_snapshots_counts(self._h, C.byref(w), C.byref(e)))
+        return w.value, e.value
+
+    def close(self):
+        if self._h:
+            _check(_setup().topohip_snapshots_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
