@@ -1,8 +1,8 @@
 """Benchmark: voxel-SIMP design iterations/s on one MI355X.
 
 Workload (default): MBB half-beam 192x64x64 hex8 (2.45 M DOF), volfrac 0.2,
-rmin 2.0, p = 3, OC move 0.2, MG-PCG tol 1e-6, FP32 K.u + FP32 4-level V-cycle
-(FP64 coarsest) inside FP64-outer PCG. A step is one design iteration
+rmin 2.0, p = 3, OC move 0.2, MG-PCG tol 1e-6, FP32 4-level V-cycle (FP64
+coarsest) preconditioning an FP64 PCG (FP64 A.p). A step is one design iteration
 (filter -> hierarchy rebuild + solve -> sensitivity -> OC update); the timed
 region is one optimisation run of --steps iterations from the uniform design,
 final certification solve included. Inputs are generated on the host before
@@ -168,6 +168,7 @@ def main():
     ap.add_argument("--workload", default="mbb_192x64x64")
     ap.add_argument("--no-fp64-form", action="store_true")
     ap.add_argument("--no-mma", action="store_true", help="skip the MMA-updater leg")
+
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--jobs", type=int, default=128)
     ap.add_argument("--rate", type=float, default=4.0)
@@ -193,7 +194,7 @@ def main():
             "steps": sv["design_iters"], "warmup": 0,
             "ms_per_step": 1e3 * sv["makespan_s"] / max(1, sv["design_iters"]),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f32 operator/V-cycle, f64 outer PCG",
+            "dtype": "f32 V-cycle, f64 operator and PCG",
             "data": "synthetic (Poisson job stream, cantilever/MBB boxes)",
             "config": {"workload": "serving", **sv}}), flush=True)
         return
@@ -245,7 +246,7 @@ def main():
         print(f"# cpu baseline {cpu['value']:.4f} design iters/s on {threads} threads",
               file=sys.stderr, flush=True)
     line = {
-        "metric": f"design iterations/s ({a.workload}, FP32 K.u + FP32 MG in FP64-outer PCG)",
+        "metric": f"design iterations/s ({a.workload}, FP32 multigrid V-cycle in FP64 PCG)",
         "value": value,
         "unit": "design iters/s",
         "n_gpus": 1,
@@ -255,7 +256,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 operator/V-cycle, f64 outer PCG",
+        "dtype": "f32 V-cycle, f64 operator and PCG",
         "data": "synthetic (analytic box workload, uniform initial density)",
         "config": {"workload": a.workload, "grid": [nx, ny, nz], "dof": ku["f32"]["bytes"] and
                    3 * (nx + 1) * (ny + 1) * (nz + 1), "volfrac": wl["volfrac"], "rmin": wl["rmin"],

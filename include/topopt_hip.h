@@ -43,9 +43,11 @@ enum {
 };
 
 /* Solver precision of a problem handle.
- *   TOPOHIP_PREC_MIXED: FP32 K.u and FP32 multigrid V-cycle (FP64 coarsest
- *     solve) inside an FP64 outer PCG (x, r, p, dot products FP64) with an FP64
- *     true-residual check before convergence is declared.
+ *   TOPOHIP_PREC_MIXED: the multigrid V-cycle in FP32 (its K.u applies,
+ *     stencils, transfers; FP64 coarsest solve) preconditioning an FP64 PCG
+ *     (A.p, x, r, p and the dot products FP64), with an FP64 true-residual
+ *     check before convergence is declared — the reference's mixed-precision
+ *     MG-PCG (core/src/fea/multigrid.cpp:1211-1318).
  *   TOPOHIP_PREC_F64: every operator apply and the whole V-cycle in FP64 (the
  *     plain FP64 form). */
 enum { TOPOHIP_PREC_MIXED = 0, TOPOHIP_PREC_F64 = 1 };

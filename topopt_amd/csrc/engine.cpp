@@ -545,7 +545,7 @@ void restart(Problem& P, Prec pc, int nc) {
   const int nrz = precondition_rz(P, pc, done, nc);
   launch_cg_set_rz(P.st.p, P.partials.p, nrz, P.s, nc, P.pstride);
   if (pc == Prec::MG32)
-    launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, P.p32.p, nd, 1, nullptr, 0, P.s, nc, P.pstride);
+    launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, nullptr, nd, 1, nullptr, 0, P.s, nc, P.pstride);
   else
     launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nullptr, nd, 1, nullptr, 0, P.s, nc, P.pstride);
 }
@@ -557,19 +557,17 @@ void iteration(Problem& P, Prec pc, int nc) {
   const int db = dot_blocks(nd);
   // A.p partials go to `partials`, the rr partials of cg_xr to `partials2`
   // (cg_xr re-sums the former in every block while it writes the latter).
-  if (pc == Prec::MG32) {
-    fine_apply<float>(P, P.p32.p, P.Ap32.p, nullptr, EPI_AX, P.partials.p, done, nc);
-    launch_cg_xr<float, float>(P.st.p, P.x.p, P.r.p, P.p32.p, P.Ap32.p, P.r32.p, nd, P.partials.p,
-                               kb, P.partials2.p, db, P.s, nc, P.pstride);
-  } else {
-    fine_apply<double>(P, P.p.p, P.Ap64.p, nullptr, EPI_AX, P.partials.p, done, nc);
-    launch_cg_xr<double, double>(P.st.p, P.x.p, P.r.p, P.p.p, P.Ap64.p, nullptr, nd, P.partials.p,
-                                 kb, P.partials2.p, db, P.s, nc, P.pstride);
-  }
+  // A.p is FP64 in every form (it defines the residual, as mf_mgpcg's
+  // mf_fine_matvec, core/src/fea/multigrid.cpp:1286); the mixed form's FP32
+  // V-cycle reads float(r), which cg_xr writes alongside r.
+  fine_apply<double>(P, P.p.p, P.Ap64.p, nullptr, EPI_AX, P.partials.p, done, nc);
+  launch_cg_xr<double, double>(P.st.p, P.x.p, P.r.p, P.p.p, P.Ap64.p,
+                               pc == Prec::MG32 ? P.r32.p : nullptr, nd, P.partials.p, kb,
+                               P.partials2.p, db, P.s, nc, P.pstride);
   launch_cg_check(P.st.p, P.partials2.p, db, P.s, nc, P.pstride);
   const int nrz = precondition_rz(P, pc, done, nc);
   if (pc == Prec::MG32)
-    launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, P.p32.p, nd, 0, P.partials.p, nrz, P.s, nc,
+    launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, nullptr, nd, 0, P.partials.p, nrz, P.s, nc,
                        P.pstride);
   else
     launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nullptr, nd, 0, P.partials.p, nrz, P.s, nc,
@@ -773,7 +771,7 @@ void autotune_ku(Problem& P) {
   launch_fill<double>(P.E64.p, 0.5, P.g.elems(), P.s);
   launch_convert<double, float>(P.E64.p, P.E32.p, P.g.elems(), P.s);
   launch_fill<double>(P.x.p, 1.0, nd, P.s);
-  if (mixed) launch_fill<float>(P.p32.p, 1.0f, nd, P.s);
+  if (mixed) launch_fill<float>(P.r32.p, 1.0f, nd, P.s);
   hipEvent_t e0, e1;
   TOPOHIP_CHECK(hipEventCreate(&e0));
   TOPOHIP_CHECK(hipEventCreate(&e1));
@@ -786,7 +784,7 @@ void autotune_ku(Problem& P) {
         if (mixed) {
           KuOp<float> op;
           op.g = P.g; op.E = P.E32.p; op.mg = P.mg32; op.tz = tz; op.tx = tl[0]; op.ty = tl[1];
-          op.x = P.p32.p; op.y = P.Ap32.p; op.epi = EPI_AX;
+          op.x = P.r32.p; op.y = P.z32.p; op.epi = EPI_AX;  // (scratch at set-up)
           launch_ku_fine<float>(op, P.s);
         } else {
           KuOp<double> op;
@@ -837,7 +835,7 @@ void ensure_cases(Problem& P, int nb) {
   const size_t nd = static_cast<size_t>(P.g.ndof()) * nb;
   P.b.ensure(nd); P.x.ensure(nd); P.r.ensure(nd); P.p.ensure(nd); P.rt.ensure(nd);
   P.u.ensure(nd); P.Ap64.ensure(nd); P.z64.ensure(nd);
-  if (mixed) { P.p32.ensure(nd); P.Ap32.ensure(nd); P.r32.ensure(nd); P.z32.ensure(nd); }
+  if (mixed) { P.r32.ensure(nd); P.z32.ensure(nd); }
   for (size_t l = 0; l < P.L.size(); ++l) {
     Level& lv = *P.L[l];
     const size_t ndl = static_cast<size_t>(lv.g.ndof()) * nb;

@@ -94,7 +94,7 @@ struct Problem {
   // PCG vectors, nb stacked load cases each (stride ndof)
   DBuf<double> b, x, r, p, Ap64, z64, rt, u;
   bool u_valid = false;  // P.u holds the displacements of a completed solve
-  DBuf<float> p32, Ap32, r32, z32;
+  DBuf<float> r32, z32;  // mixed form: the V-cycle's FP32 input and output
   DBuf<double> partials, partials2;  // nb cases x pstride block partial sums
   DBuf<CgState> st;
   CgState* st_h = nullptr;
