@@ -545,9 +545,9 @@ void restart(Problem& P, Prec pc, int nc) {
   const int nrz = precondition_rz(P, pc, done, nc);
   launch_cg_set_rz(P.st.p, P.partials.p, nrz, P.s, nc, P.pstride);
   if (pc == Prec::MG32)
-    launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, nullptr, nd, 1, nullptr, 0, P.s, nc, P.pstride);
+    launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, nd, 1, nullptr, 0, P.s, nc, P.pstride);
   else
-    launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nullptr, nd, 1, nullptr, 0, P.s, nc, P.pstride);
+    launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nd, 1, nullptr, 0, P.s, nc, P.pstride);
 }
 
 void iteration(Problem& P, Prec pc, int nc) {
@@ -567,10 +567,10 @@ void iteration(Problem& P, Prec pc, int nc) {
   launch_cg_check(P.st.p, P.partials2.p, db, P.s, nc, P.pstride);
   const int nrz = precondition_rz(P, pc, done, nc);
   if (pc == Prec::MG32)
-    launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, nullptr, nd, 0, P.partials.p, nrz, P.s, nc,
+    launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, nd, 0, P.partials.p, nrz, P.s, nc,
                        P.pstride);
   else
-    launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nullptr, nd, 0, P.partials.p, nrz, P.s, nc,
+    launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nd, 0, P.partials.p, nrz, P.s, nc,
                         P.pstride);
 }
 

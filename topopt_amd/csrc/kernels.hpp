@@ -98,9 +98,9 @@ void launch_cg_xr(CgState* st, double* x, double* r, const TP* p, const TA* Ap, 
 void launch_cg_check(CgState* st, const double* partials, int n, hipStream_t s, int nr = 1,
                      int pstride = 0);
 // rz_new = fixed-order sum of the nrz <r,z> partials at rzp; beta = rz_new / rz;
-// p = z + beta*p (restart: p = z, partials unused); optional p32 shadow
+// p = z + beta*p (restart: p = z, partials unused)
 template <typename TZ>
-void launch_cg_p(CgState* st, const TZ* z, double* p, float* p32, long long n, int restart,
+void launch_cg_p(CgState* st, const TZ* z, double* p, long long n, int restart,
                  const double* rzp, int nrz, hipStream_t s, int nr = 1, int pstride = 0);
 // start: bnorm2, thresh2, rz from partials etc.
 void launch_cg_init(CgState* st, const double* bb_partials, int nb, double tol, int cap,

@@ -371,14 +371,12 @@ __global__ void cg_xr_k(CgState* st, double* __restrict__ x, double* __restrict_
 // and the breakdown test). restart: p = z.
 template <typename TZ>
 __global__ void cg_p_k(CgState* st, const TZ* __restrict__ z, double* __restrict__ p,
-                       float* p32, long long n, int restart, const double* rzp, int nrz,
-                       int pstride) {
+                       long long n, int restart, const double* rzp, int nrz, int pstride) {
   const int rc = blockIdx.y;
   st += rc;
   if (st->done) return;
   z += rc * n;
   p += rc * n;
-  if (p32) p32 += rc * n;
   double beta = 0.0;
   if (!restart) {
     const double rzn = fixed_sum_all(rzp + static_cast<long long>(rc) * pstride, nrz);
@@ -400,9 +398,7 @@ __global__ void cg_p_k(CgState* st, const TZ* __restrict__ z, double* __restrict
   }
   for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<long long>(gridDim.x) * blockDim.x) {
-    const double v = static_cast<double>(z[i]) + (restart ? 0.0 : beta * p[i]);
-    p[i] = v;
-    if (p32) p32[i] = static_cast<float>(v);
+    p[i] = static_cast<double>(z[i]) + (restart ? 0.0 : beta * p[i]);
   }
 }
 
@@ -555,16 +551,16 @@ template void launch_cg_xr<float, float>(CgState*, double*, double*, const float
                                          hipStream_t, int, int);
 
 template <typename TZ>
-void launch_cg_p(CgState* st, const TZ* z, double* p, float* p32, long long n, int restart,
+void launch_cg_p(CgState* st, const TZ* z, double* p, long long n, int restart,
                  const double* rzp, int nrz, hipStream_t s, int nr, int pstride) {
   // every block re-sums the <r,z> partials, so the grid stays at 1024 blocks
   hipLaunchKernelGGL((cg_p_k<TZ>), dim3(grid_for(n, 256, 1024), nr), dim3(256), 0, s, st, z, p,
-                     p32, n, restart, rzp, nrz, pstride);
+                     n, restart, rzp, nrz, pstride);
   TOPOHIP_CHECK(hipGetLastError());
 }
-template void launch_cg_p<float>(CgState*, const float*, double*, float*, long long, int,
+template void launch_cg_p<float>(CgState*, const float*, double*, long long, int,
                                  const double*, int, hipStream_t, int, int);
-template void launch_cg_p<double>(CgState*, const double*, double*, float*, long long, int,
+template void launch_cg_p<double>(CgState*, const double*, double*, long long, int,
                                   const double*, int, hipStream_t, int, int);
 
 }  // namespace topohip
