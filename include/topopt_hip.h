@@ -145,6 +145,17 @@ int topohip_oc_update(topohip_problem* p, double radius, const double* x, const 
                       double volfrac, double move, double rho_min, double* x_new,
                       uint8_t* active_set, int* bisections, double* lambda);
 
+/* Decision margin of the OC volume bisection (the test `vol > target` of
+ * oc_update, core/src/simp/simp.cpp:1002-1031): the smallest |vol - target|
+ * over the volumes the last search tested (last_gap) and over every OC step
+ * since the last topohip_optimize started (run_min_gap), both relative to the
+ * target, and the rounding bound (n - 1) * 2^-53 of a sum of the n
+ * non-negative per-voxel terms (bound). A gap above the bound decides the same
+ * way whatever the summation order, so the device's block-tree volume and the
+ * reference's sequential sum take the same bisection path. Outputs nullable.
+ * No reference counterpart (a diagnostic of the device's reduction order). */
+int topohip_oc_margin(topohip_problem* p, double* last_gap, double* run_min_gap, double* bound);
+
 /* The plain OC design loop (simp_optimize, core/src/simp/simp.cpp:2018-2608):
  * filter -> solve -> sensitivity -> OC, change = max|x_new - x|, stop when
  * change < change_tol, then a final solve on filter(design). Optional

@@ -134,6 +134,22 @@ def hierarchy(s, E, nu=0.3):
     return lv.value, dofs[: lv.value].tolist()
 
 
+def level_operator(s, E, level, nu=0.3):
+    """Hierarchy level `level` (>= 1) operator as (n, indptr, indices, data)
+    over the level's active DOFs (node-major, then component)."""
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    n = C.c_int(0)
+    nnz = C.c_int(0)
+    _check(lib().oracle_level_csr(C.byref(s.p), _ptr(E), C.c_double(nu), int(level), C.byref(n),
+                                  C.byref(nnz), None, None, None))
+    ptr = np.zeros(n.value + 1, dtype=np.int32)
+    col = np.zeros(max(1, nnz.value), dtype=np.int32)
+    val = np.zeros(max(1, nnz.value))
+    _check(lib().oracle_level_csr(C.byref(s.p), _ptr(E), C.c_double(nu), int(level), C.byref(n),
+                                  C.byref(nnz), _ptr(ptr), _ptr(col), _ptr(val)))
+    return n.value, ptr, col[: nnz.value], val[: nnz.value]
+
+
 def simp_compliance(s, density, E0=1.0, nu=0.3, penalty=3.0, rho_min=1e-3, tol=1e-8,
                     max_it=0, use_mg=True, guess=None):
     d = np.ascontiguousarray(density, dtype=np.float64)

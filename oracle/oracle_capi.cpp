@@ -141,6 +141,36 @@ int oracle_hierarchy(const OracleProblem* p, const double* E, double nu, int* le
   });
 }
 
+// Operator of hierarchy level `level` (1 = A1 from galerkin_a1, 2.. = the
+// sparse P^T A P levels) as CSR over that level's active DOFs, numbered
+// node-major then component. Two calls: with ptr == NULL only *n and *nnz are
+// set; then ptr (n + 1), col and val (nnz) are filled. Level 0 is the fine
+// reduced operator's row count only (*nnz = 0).
+int oracle_level_csr(const OracleProblem* p, const double* E, double nu, int level, int* n,
+                     int* nnz, int* ptr, int* col, double* val) {
+  return guarded([&] {
+    const oracle::Grid g = grid_of(p);
+    std::vector<double> Ev(E, E + g.voxels());
+    oracle::Reduced m = oracle::build_reduced(g, Ev, nu, bcs_of(p), loads_of(p), nullptr);
+    if (level == 0) {
+      *n = m.ng;
+      *nnz = 0;
+      return;
+    }
+    oracle::Hierarchy H;
+    if (!oracle::build_mf_hierarchy(m, g.nnx(), g.nny(), g.nnz(), H))
+      throw std::invalid_argument("no multigrid hierarchy for this grid");
+    if (level < 1 || level >= H.levels()) throw std::invalid_argument("level out of range");
+    const oracle::Csr& A = H.coarse[static_cast<std::size_t>(level - 1)].A;
+    *n = A.rows;
+    *nnz = static_cast<int>(A.col.size());
+    if (!ptr) return;
+    std::memcpy(ptr, A.ptr.data(), A.ptr.size() * sizeof(int));
+    std::memcpy(col, A.col.data(), A.col.size() * sizeof(int));
+    std::memcpy(val, A.val.data(), A.val.size() * sizeof(double));
+  });
+}
+
 int oracle_simp_compliance(const OracleProblem* p, const double* density, double E0, double nu,
                            double penalty, double rho_min, double tol, int max_it, int use_mg,
                            double* c, double* dc, double* u, OracleInfo* info) {

@@ -349,6 +349,14 @@ class Problem:
                                        _p(out), _p(act), C.byref(nb), C.byref(lam)))
         return out, act, nb.value, lam.value
 
+    def oc_margin(self):
+        """(last_gap, run_min_gap, bound): the OC search's smallest relative
+        |vol - target| (last search; smallest since the last optimize started)
+        and the rounding bound of the volume sum."""
+        a, b, c = C.c_double(0), C.c_double(0), C.c_double(0)
+        _check(lib().topohip_oc_margin(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
     def mma_update(self, radius, mma_iter, x, dc, volfrac, move=0.2, rho_min=1e-3):
         """mma_update: one MMA step; the handle keeps MmaState between calls
         (restart a run at mma_iter 1). Returns (x_new, lambda, change)."""

@@ -268,6 +268,17 @@ int topohip_oc_update(topohip_problem* p, double radius, const double* x, const 
   });
 }
 
+int topohip_oc_margin(topohip_problem* p, double* last_gap, double* run_min_gap,
+                      double* bound) {
+  return guarded([&] {
+    need(p, "problem");
+    const Problem& P = *p->P;
+    if (last_gap) *last_gap = P.oc_gap_rel;
+    if (run_min_gap) *run_min_gap = P.oc_gap_min;
+    if (bound) *bound = P.oc_gap_bound;
+  });
+}
+
 int topohip_optimize(topohip_problem* p, const topohip_options* opt, double* design_out,
                      double* xphys_out, double* history, topohip_result* result) {
   return guarded([&] {

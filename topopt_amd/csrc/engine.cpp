@@ -726,7 +726,13 @@ void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool
   }
   int most = 0;
   for (int r = 0; r < nc; ++r) {
-    if (!(skip && skip[r])) iters[r] = P.st_h[r].iters;
+    if (!(skip && skip[r])) {
+      iters[r] = P.st_h[r].iters;
+      // a recurrence that stopped on breakdown or at the cap reports the
+      // residual it reached (SolverNonConvergence's residual)
+      if (!ok[r] && P.st_h[r].fail && P.st_h[r].bnorm2 > 0.0)
+        resid[r] = std::sqrt(P.st_h[r].rr / P.st_h[r].bnorm2);
+    }
     most = std::max(most, P.st_h[r].iters);
   }
   P.pcg_hint[gi] = most;
@@ -1346,14 +1352,22 @@ double simp_compliance_dev(Problem& P, const double* d_xphys, double E0, double 
   return c;
 }
 
-// One OC update over the design set dset (device bytes, null = every voxel)
-// with fc built on the same set: oc_update (keep = false: voxels off the set
-// get 0) or oc_update_masked (keep = true: they keep their x; budget
-// volfrac * n_design).
-void oc_update_core(Problem& P, FilterCache& fc, const uint8_t* dset, double n_design, bool keep,
-                    const double* d_x, const double* d_dc, double volfrac, double move,
-                    double rho_min, double* d_xnew, uint8_t* d_active, int* bisections,
-                    double* lambda, double* change) {
+// Work memory of one OC search: device state and partial sums, their pinned
+// host read-back slots, and the filtered sensitivities.
+struct OcWork {
+  OcState* st;
+  OcState* st_h;
+  double* lam;   // kLamPass x oc_blocks
+  double* chg;   // oc_blocks
+  double* scal;  // >= 1
+  double* scal_h;
+  double* dcf;   // voxels
+};
+
+void oc_search(Problem& P, FilterCache& fc, const uint8_t* dset, double n_design, bool keep,
+               const double* d_x, const double* d_dc, double volfrac, double move,
+               double rho_min, double* d_xnew, uint8_t* d_active, int* bisections,
+               double* lambda, double* change, const OcWork& w) {
   if (!(volfrac > 0.0 && volfrac <= 1.0))
     throw Error(TOPOHIP_EINVAL, "oc_update: volume_fraction must be in (0, 1]");
   if (!(move > 0.0)) throw Error(TOPOHIP_EINVAL, "oc_update: move must be > 0");
@@ -1361,30 +1375,50 @@ void oc_update_core(Problem& P, FilterCache& fc, const uint8_t* dset, double n_d
     throw Error(TOPOHIP_EINVAL, "oc_update: density_min must be in (0, 1]");
   const long long nv = P.g.elems();
   const uint8_t* sol = dset;
-  P.dcf.ensure(static_cast<size_t>(nv));
-  launch_filter(P.g, fc.st, sol, fc.hs.p, d_dc, P.dcf.p, 1, P.s);
+  launch_filter(P.g, fc.st, sol, fc.hs.p, d_dc, w.dcf, 1, P.s);
   const double target = volfrac * n_design;
-  launch_oc_init_multi(P.oc.p, target, P.s);
+  launch_oc_init_multi(w.st, target, P.s);
   const int ob = oc_blocks(nv);
-  P.lam_partials.ensure(static_cast<size_t>(kLamPass) * ob);
   // Each pass tries up to 7 lambdas (~100 bracket + 100 bisection trials at most).
   for (int round = 0;; ++round) {
     if (round > 40) throw Error(TOPOHIP_EINTERNAL, "oc_update: lambda search did not finish");
     for (int i = 0; i < 6; ++i) {
-      launch_oc_trial_multi(d_x, P.dcf.p, fc.dv.p, sol, nv, move, rho_min, P.oc.p,
-                            P.lam_partials.p, ob, P.s);
-      launch_oc_decide_multi(P.oc.p, P.lam_partials.p, ob, P.s);
+      launch_oc_trial_multi(d_x, w.dcf, fc.dv.p, sol, nv, move, rho_min, w.st, w.lam, ob, P.s);
+      launch_oc_decide_multi(w.st, w.lam, ob, P.s);
     }
-    d2h(P.oc_h, P.oc.p, 1, P.s);
-    if (P.oc_h->phase == 2) break;
+    d2h(w.st_h, w.st, 1, P.s);
+    if (w.st_h->phase == 2) break;
   }
-  launch_oc_final(d_x, P.dcf.p, fc.dv.p, sol, nv, move, rho_min, P.oc.p, d_xnew, d_active,
-                  P.partials.p, ob, P.s, keep ? 1 : 0);
-  launch_max_partials(P.partials.p, ob, P.scal.p, P.s);
-  d2h(P.scal_h, P.scal.p, 1, P.s);
-  if (bisections) *bisections = P.oc_h->steps;
-  if (lambda) *lambda = 0.5 * (P.oc_h->l1 + P.oc_h->l2);
-  if (change) *change = P.scal_h[0];
+  launch_oc_final(d_x, w.dcf, fc.dv.p, sol, nv, move, rho_min, w.st, d_xnew, d_active, w.chg, ob,
+                  P.s, keep ? 1 : 0);
+  launch_max_partials(w.chg, ob, w.scal, P.s);
+  d2h(w.scal_h, w.scal, 1, P.s);
+  if (bisections) *bisections = w.st_h->steps;
+  if (lambda) *lambda = 0.5 * (w.st_h->l1 + w.st_h->l2);
+  if (change) *change = w.scal_h[0];
+  // Decision margin: the volume tested against the target closest to it, as a
+  // fraction of the target, beside the rounding bound (n - 1) u of a sum of n
+  // non-negative terms (the reference's sequential order; the device's block
+  // tree is tighter). A gap above the bound decides the same in either order.
+  P.oc_gap_rel = target > 0.0 ? w.st_h->gap / target : 0.0;
+  P.oc_gap_min = std::min(P.oc_gap_min, P.oc_gap_rel);
+  P.oc_gap_bound = std::max(0.0, n_design - 1.0) * std::ldexp(1.0, -53);
+}
+
+// One OC step (oc_update, core/src/simp/simp.cpp:948-1032) on the design set
+// dset with fc built on the same set: oc_update (keep = false: voxels off the
+// set get 0) or oc_update_masked (keep = true: they keep their x; budget
+// volfrac * n_design). Work memory: the handle's own buffers.
+void oc_update_core(Problem& P, FilterCache& fc, const uint8_t* dset, double n_design, bool keep,
+                    const double* d_x, const double* d_dc, double volfrac, double move,
+                    double rho_min, double* d_xnew, uint8_t* d_active, int* bisections,
+                    double* lambda, double* change) {
+  const long long nv = P.g.elems();
+  P.dcf.ensure(static_cast<size_t>(nv));
+  P.lam_partials.ensure(static_cast<size_t>(kLamPass) * oc_blocks(nv));
+  const OcWork w{P.oc.p, P.oc_h, P.lam_partials.p, P.partials.p, P.scal.p, P.scal_h, P.dcf.p};
+  oc_search(P, fc, dset, n_design, keep, d_x, d_dc, volfrac, move, rho_min, d_xnew, d_active,
+            bisections, lambda, change, w);
 }
 
 void oc_update_dev(Problem& P, double radius, const double* d_x, const double* d_dc,
@@ -1547,6 +1581,7 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
   if (mma && proj && !proj->empty())
     throw Error(TOPOHIP_EINVAL, "simp_optimize: projection stages need the OC updater");
   int mma_iter = 0;
+  P.oc_gap_min = 1e300;
   std::vector<double> curve;  // compliance history for the MMA plateau stop
   std::vector<std::pair<double, int>> stages;
   if (o.pc_per_stage > 0) {

@@ -105,6 +105,9 @@ struct Problem {
   std::vector<std::unique_ptr<FilterCache>> filters;
   DBuf<OcState> oc;
   OcState* oc_h = nullptr;
+  // OC decision margins: last search, smallest since the last reset (each
+  // optimisation run resets it) and the rounding bound, relative to the target
+  double oc_gap_rel = 0.0, oc_gap_min = 1e300, oc_gap_bound = 0.0;
   DBuf<double> xd, xphys, dc, xnew, dcf;
   DBuf<double> xtil, dpr, dcp, dvf, xcand, xcf;  // projected OC work
   DBuf<uint8_t> active;

@@ -183,6 +183,7 @@ struct OcState {
   int steps;      // bisection steps taken
   int nlam, pad;
   double lam[kLamPass];
+  double gap;     // smallest |vol - target| over the search's volume decisions
 };
 void launch_oc_init(OcState* st, double target, hipStream_t s);
 // One lambda trial: volume partials of candidate(lambda_trial(st)).

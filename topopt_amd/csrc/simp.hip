@@ -220,6 +220,7 @@ __global__ void oc_init_k(OcState* st, double target) {
   st->vol = 0.0;
   st->phase = 0;
   st->steps = 0;
+  st->gap = INFINITY;
 }
 
 __global__ void oc_trial_k(const double* __restrict__ x, const double* __restrict__ dcf,
@@ -250,6 +251,7 @@ __global__ void oc_decide_k(OcState* st, const double* __restrict__ partials, in
   s = block_sum(s);
   if (threadIdx.x != 0) return;
   st->vol = s;
+  st->gap = fmin(st->gap, fabs(s - st->target));
   if (st->phase == 0) {
     if (s > st->target) {
       st->l2 *= 2.0;
@@ -302,6 +304,7 @@ __global__ void oc_init_multi_k(OcState* st, double target) {
   st->vol = 0.0;
   st->phase = 0;
   st->steps = 0;
+  st->gap = INFINITY;
   oc_plan(st);
 }
 
@@ -354,6 +357,7 @@ __global__ void oc_decide_multi_k(OcState* st, const double* __restrict__ partia
   if (st->phase == 0) {
     for (int j = 0; j < nl; ++j) {
       st->vol = S[j];
+      st->gap = fmin(st->gap, fabs(S[j] - st->target));
       if (S[j] > st->target) {
         st->l2 *= 2.0;
         if (st->l2 < 1e30) continue;
@@ -367,6 +371,7 @@ __global__ void oc_decide_multi_k(OcState* st, const double* __restrict__ partia
     for (int lev = 0; lev < 3; ++lev) {
       const double lmid = st->lam[node];
       st->vol = S[node];
+      st->gap = fmin(st->gap, fabs(S[node] - st->target));
       if (S[node] > st->target) {
         st->l1 = lmid;
         node = 2 * node + 2;
