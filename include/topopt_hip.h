@@ -145,6 +145,23 @@ int topohip_oc_update(topohip_problem* p, double radius, const double* x, const 
                       double volfrac, double move, double rho_min, double* x_new,
                       uint8_t* active_set, int* bisections, double* lambda);
 
+/* Allocation-free variant of topohip_oc_update. No reference counterpart: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). x, dc, x_new
+ * and active_set (nullable) are DEVICE pointers on the handle's device, nx*ny*nz
+ * entries each, ready when the call starts; the call returns when x_new is
+ * written. The work memory is one static device array per GPU (about 16 MiB,
+ * held until the process ends) plus the handle's own buffers, so the call
+ * makes no hipMalloc/hipFree. Limits: at most 2^21 (2097152) voxels, and the
+ * density filter of `radius` must already be on the handle (built at set-up
+ * by topohip_filter or topohip_oc_update with that radius); otherwise
+ * TOPOHIP_EINVAL. Every call on one GPU shares the array: calls that overlap
+ * on one GPU race. Results are bit-identical to topohip_oc_update. */
+int topohip_oc_update_noalloc(topohip_problem* p, double radius, const double* x,
+                              const double* dc, double volfrac, double move, double rho_min,
+                              double* x_new, uint8_t* active_set, int* bisections,
+                              double* lambda);
+
 /* Decision margin of the OC volume bisection (the test `vol > target` of
  * oc_update, core/src/simp/simp.cpp:1002-1031): the smallest |vol - target|
  * over the volumes the last search tested (last_gap) and over every OC step

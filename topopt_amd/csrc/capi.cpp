@@ -268,6 +268,17 @@ int topohip_oc_update(topohip_problem* p, double radius, const double* x, const 
   });
 }
 
+int topohip_oc_update_noalloc(topohip_problem* p, double radius, const double* x,
+                              const double* dc, double volfrac, double move, double rho_min,
+                              double* x_new, uint8_t* active_set, int* bisections,
+                              double* lambda) {
+  return guarded([&] {
+    need(p, "problem"); need(x, "x"); need(dc, "dc"); need(x_new, "x_new");
+    oc_update_noalloc(*p->P, radius, x, dc, volfrac, move, rho_min, x_new, active_set,
+                      bisections, lambda);
+  });
+}
+
 int topohip_oc_margin(topohip_problem* p, double* last_gap, double* run_min_gap,
                       double* bound) {
   return guarded([&] {

@@ -946,6 +946,7 @@ std::unique_ptr<Problem> create_problem(int nx, int ny, int nz, double h, double
 
   TOPOHIP_CHECK(hipSetDevice(device));
   P->device = device;
+  P->scratch = noalloc_scratch(device);  // set-up of the allocation-free entry point
   {
     std::lock_guard<std::recursive_mutex> lk(device_api_mutex());
     TOPOHIP_CHECK(hipStreamCreateWithFlags(&P->s, hipStreamNonBlocking));
@@ -1427,6 +1428,38 @@ void oc_update_dev(Problem& P, double radius, const double* d_x, const double* d
   FilterCache& fc = filter_for(P, radius);
   oc_update_core(P, fc, P.all_solid ? nullptr : P.solid.p, static_cast<double>(P.n_solid), false,
                  d_x, d_dc, volfrac, move, rho_min, d_xnew, d_active, bisections, lambda, change);
+}
+
+// oc_update on device buffers with no allocation: the work memory is the
+// GPU's file-scope scratch array and the handle's pinned read-back slots, and
+// the density filter of `radius` must already be on the handle (built at
+// set-up by any filter or OC call with that radius).
+void oc_update_noalloc(Problem& P, double radius, const double* d_x, const double* d_dc,
+                       double volfrac, double move, double rho_min, double* d_xnew,
+                       uint8_t* d_active, int* bisections, double* lambda) {
+  const long long nv = P.g.elems();
+  if (nv > kNoAllocMaxVoxels)
+    throw Error(TOPOHIP_EINVAL, "oc_update_noalloc: " + std::to_string(nv) +
+                                    " voxels exceed the allocation-free limit of " +
+                                    std::to_string(kNoAllocMaxVoxels));
+  if (!(radius > 0.0)) throw Error(TOPOHIP_EINVAL, "make_density_filter: radius must be > 0");
+  FilterCache* fc = nullptr;
+  for (auto& f : P.filters)
+    if (f->radius == radius) fc = f.get();
+  if (!fc)
+    throw Error(TOPOHIP_EINVAL, "oc_update_noalloc: the density filter of this radius is not on "
+                                "the handle; build it at set-up with topohip_filter or "
+                                "topohip_oc_update at the same radius");
+  if (!P.scratch) throw Error(TOPOHIP_EINTERNAL, "oc_update_noalloc: no scratch address");
+  char* base = static_cast<char*>(P.scratch);
+  const OcWork w{reinterpret_cast<OcState*>(base + kScrState), P.oc_h,
+                 reinterpret_cast<double*>(base + kScrLam),
+                 reinterpret_cast<double*>(base + kScrChange),
+                 reinterpret_cast<double*>(base + kScrScal), P.scal_h,
+                 reinterpret_cast<double*>(base + kScrDcf)};
+  oc_search(P, *fc, P.all_solid ? nullptr : P.solid.p, static_cast<double>(P.n_solid), false, d_x,
+            d_dc, volfrac, move, rho_min, d_xnew, d_active, bisections, lambda, nullptr, w);
+  sync(P);
 }
 
 // mma_update over mma_update_joint (core/src/simp/simp.cpp:1433-1488,

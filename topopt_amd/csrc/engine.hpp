@@ -46,6 +46,7 @@ struct Timers {
 struct Problem {
   int device = 0;
   hipStream_t s = nullptr;
+  void* scratch = nullptr;  // this device's allocation-free work array (noalloc_scratch)
   Geom g;
   double h = 1.0, nu = 0.3;
   int precision = TOPOHIP_PREC_MIXED;
@@ -160,6 +161,9 @@ FilterCache& filter_for(Problem& P, double radius);
 double simp_compliance_dev(Problem& P, const double* d_xphys, double E0, double penalty,
                            double rho_min, double tol, int max_it, topohip_cginfo* info,
                            const double* d_guess = nullptr);
+void oc_update_noalloc(Problem& P, double radius, const double* d_x, const double* d_dc,
+                       double volfrac, double move, double rho_min, double* d_xnew,
+                       uint8_t* d_active, int* bisections, double* lambda);
 void oc_update_dev(Problem& P, double radius, const double* d_x, const double* d_dc,
                    double volfrac, double move, double rho_min, double* d_xnew,
                    uint8_t* d_active, int* bisections, double* lambda, double* change);

@@ -186,6 +186,23 @@ struct OcState {
   double gap;     // smallest |vol - target| over the search's volume decisions
 };
 void launch_oc_init(OcState* st, double target, hipStream_t s);
+
+// Allocation-free OC update: its work memory is one file-scope device array per
+// GPU (`scratch`, simp.hip), laid out as below. Voxel counts above
+// kNoAllocMaxVoxels are rejected. oc_blocks() never exceeds kOcMaxBlocks.
+constexpr long long kNoAllocMaxVoxels = 1LL << 21;
+constexpr int kOcMaxBlocks = 1024;
+constexpr size_t kScrState = 0;                                            // OcState
+constexpr size_t kScrLam = 256;                                            // kLamPass x blocks
+constexpr size_t kScrChange = kScrLam + sizeof(double) * kLamPass * kOcMaxBlocks;  // blocks
+constexpr size_t kScrScal = kScrChange + sizeof(double) * kOcMaxBlocks;    // 32 doubles
+constexpr size_t kScrDcf = kScrScal + 256;                                 // voxels
+constexpr size_t kNoAllocScratchBytes = kScrDcf + sizeof(double) * kNoAllocMaxVoxels;
+static_assert(sizeof(OcState) <= kScrLam, "OcState must fit its scratch slot");
+static_assert(kScrDcf % 256 == 0, "scratch slots stay 256-byte aligned");
+// Device address of that GPU's scratch array (hipGetSymbolAddress once per
+// device, with the device made current; cached for the process).
+void* noalloc_scratch(int device);
 // One lambda trial: volume partials of candidate(lambda_trial(st)).
 void launch_oc_trial(const double* x, const double* dcf, const double* dv, const uint8_t* solid,
                      long long n, double move, double rho_min, const OcState* st,

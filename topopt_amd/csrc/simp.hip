@@ -8,6 +8,8 @@
 // order, no fused multiply-add), so the filtered fields and the OC candidate
 // are bit-identical to an FP64 CPU evaluation on identical inputs.
 #pragma clang fp contract(off)
+#include <mutex>
+
 #include "kernels.hpp"
 
 namespace topohip {
@@ -460,7 +462,37 @@ void launch_oc_init(OcState* st, double target, hipStream_t s) {
   TOPOHIP_CHECK(hipGetLastError());
 }
 
-int oc_blocks(long long n) { return grid_for(n, RB * 4, 1024); }
+int oc_blocks(long long n) { return grid_for(n, RB * 4, kOcMaxBlocks); }
+
+// ---- work memory of the allocation-free OC update ------------------------------
+// One array per GPU for the life of the process (kNoAllocScratchBytes, about
+// 16 MiB). Host code never passes `scratch` itself to a kernel: its name is a
+// host-side stand-in, so every use goes through the address from
+// hipGetSymbolAddress below. Every call on one GPU shares the array; calls
+// that overlap on one GPU race.
+namespace {
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+void* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) throw HipError("noalloc_scratch: device id out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    TOPOHIP_CHECK(hipGetDevice(&cur));
+    if (cur != device) TOPOHIP_CHECK(hipSetDevice(device));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != device) TOPOHIP_CHECK(hipSetDevice(cur));
+    TOPOHIP_CHECK(e);
+    if (!ptr) throw HipError("noalloc_scratch: hipGetSymbolAddress returned no address");
+    g_scratch_ptr[device] = ptr;
+  }
+  return g_scratch_ptr[device];
+}
 
 void launch_oc_trial(const double* x, const double* dcf, const double* dv, const uint8_t* solid,
                      long long n, double move, double rho_min, const OcState* st,
