@@ -11,22 +11,37 @@ the timed region and live in HBM throughout.
 Also reported on the same JSON line:
   fp64_form    the same run in the plain FP64 GPU form (every apply and the
                whole V-cycle in FP64)
-  ku           the fine K.u apply (FP32 and FP64) timed with HIP events over
-               back-to-back launches on the problem's stream: GDOF/s and HBM
-               fraction of the algorithmic bytes
-  roofline     that FP32 K.u kernel against MI355X's 8 TB/s
+  vs_oracle    per-result errors of the first design iteration against the
+               FP64 oracle (compliance, ||u||, PCG count, the OC step's new
+               density and active set) and the OC step fed with the oracle's
+               own inputs (bit-exact or not), with its decision margin
+  ku           the fine K.u kernels timed with HIP events over launches that
+               rotate through operand sets larger than twice the last-level
+               cache: the plain FP32/FP64 apply and the three variants the
+               design loop launches (FP64 A.p with the fused <p,Ap>, the FP32
+               V-cycle pre-smoothing residual, the FP32 post-smoothing sweep
+               with the fused <r,z>), each with its algorithmic bytes
+  roofline     the in-loop FP64 A.p kernel (the largest per-launch K.u cost of
+               a PCG iteration) against MI355X's 8 TB/s
   cpu_baseline the FP64 oracle (a C++ restatement of the reference CPU path,
-               8-colour threaded apply) timed on this host for one design
-               iteration of the same workload
+               8-colour threaded apply) timed on this host for the first design
+               iteration of the same workload, at 1 thread and at the physical
+               cores available to this process
+  serving      512 optimisation jobs (grids 48x16x16 .. 192x64x64, cantilever /
+               MBB, volfrac U(0.1,0.4), rmin U(1.2,2.5), 60 iterations each)
+               arriving Poisson through the continuous-batching server: p50/p99
+               job latency and aggregate design-iters/s
+  large        cantilever 512x256x256 (101.6 M DOF): design iterations/s, device
+               memory held, K.u and smoother bandwidth
 
 Other workloads (--workload): cantilever_60x20x20, bridge_256x64x96 (four load
 cases solved as one block PCG, summed compliance), cantilever_512x256x256, and
-serving (a Poisson stream of optimisation jobs through the continuous-batching
-server: p50/p99 job latency and aggregate design-iters/s; --jobs, --rate).
+serving.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -40,22 +55,39 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def measured_traffic(workload):
-    """HBM bytes per fine K.u launch from the committed rocprofv3 PMC summary
-    (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads for a
-    plain copy of known size), or None when no summary covers the workload."""
-    path = os.path.join(ROOT, "profiles", f"ku_fine_f32_{workload.split('_', 1)[-1]}_traffic.md")
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def ku_bytes(nx, ny, nz, variant):
+    """Algorithmic HBM bytes of one fine K.u launch (BC mask bytes counted as 0)."""
+    ndof = 3 * (nx + 1) * (ny + 1) * (nz + 1)
+    nv = nx * ny * nz
+    w = 8 if variant in ("f64", "ap64_dot") else 4
+    b = 2 * w * ndof + w * nv                       # x read, y written, E read
+    if variant == "res0_f32":
+        b += 3 * w * ndof                           # b, D^-1 read; x0 written
+    if variant == "jac_dot_f32":
+        b += 3 * w * ndof + 8 * ndof                # x, b, D^-1 read; FP64 r read
+    return b, ndof
+
+
+def measured_traffic(workload, variant, tile):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary of this
+    kernel variant, workload and launch tile (FETCH_SIZE and WRITE_SIZE, each
+    scaled by what the counter reads for a plain copy of known size), or None
+    when no summary covers them."""
+    grid = workload.split("_", 1)[-1]
+    path = os.path.join(ROOT, "profiles", f"ku_{variant}_{grid}_traffic.md")
     if not os.path.exists(path):
         return None
+    val, t = None, None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
-
-
-def ku_bytes(nx, ny, nz, word):
-    ndof = 3 * (nx + 1) * (ny + 1) * (nz + 1)
-    return 2 * word * ndof + word * nx * ny * nz, ndof
+            val = float(line.split(":", 1)[1])
+        if line.startswith("tile:"):
+            t = tuple(int(v) for v in line.split(":", 1)[1].split(","))
+    return val if t == tuple(tile) else None
 
 
 def run_design(ta, wl, precision, steps, warmup, mma=False):
@@ -72,28 +104,30 @@ def run_design(ta, wl, precision, steps, warmup, mma=False):
         t0 = time.perf_counter()
         out = run(steps)
         dt = time.perf_counter() - t0
+        out["oc_gap"] = p.oc_margin()
     return out, dt
 
 
-def ku_measure(ta, nx, ny, nz, reps=100):
+KU_LOOP = ("ap64_dot", "res0_f32", "jac_dot_f32")
+
+
+def ku_measure(ta, nx, ny, nz, reps=100, variants=("f32", "f64") + KU_LOOP):
+    res = {}
     with ta.Problem(nx, ny, nz) as p:
         p.bench_prepare(0x5EED)
-        ms32 = p.bench_ku(ta.PREC_MIXED, reps)
-        ms64 = p.bench_ku(ta.PREC_F64, reps)
-    b32, ndof = ku_bytes(nx, ny, nz, 4)
-    b64, _ = ku_bytes(nx, ny, nz, 8)
-    res = {}
-    for name, ms, b in (("f32", ms32, b32), ("f64", ms64, b64)):
-        gbs = b / (ms * 1e-3) / 1e9
-        res[name] = {"ms": ms, "gdof_s": ndof / (ms * 1e-3) / 1e9, "GBs": gbs,
-                     "hbm_frac": gbs / HBM_PEAK_GBS, "bytes": b,
-                     "gflops": 1152.0 * nx * ny * nz / (ms * 1e-3) / 1e9}
-    return res
+        tile = p.ku_tile()
+        for v in variants:
+            ms, sets = p.bench_ku_variant(v, reps)
+            b, ndof = ku_bytes(nx, ny, nz, v)
+            gbs = b / (ms * 1e-3) / 1e9
+            res[v] = {"us": ms * 1e3, "GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS, "bytes": b,
+                      "gdof_s": ndof / (ms * 1e-3) / 1e9, "rotated_sets": sets}
+    return res, tile
 
 
 def fields_measure(ta, nx, ny, nz, reps=50):
     """Result-field kernels (von Mises -> f32, displacement -> f32, density ->
-    f16) on resident inputs; algorithmic bytes: u once (8 B per DOF) + 4 B per
+    f16) over rotated inputs; algorithmic bytes: u once (8 B per DOF) + 4 B per
     voxel out; 8 + 4 B per DOF; 8 + 2 B per voxel."""
     with ta.Problem(nx, ny, nz) as p:
         p.bench_prepare(0x5EED)
@@ -108,8 +142,7 @@ def fields_measure(ta, nx, ny, nz, reps=50):
     return out
 
 
-def run_serving(ta, njobs, rate, seed, iterations):
-    """Poisson arrivals of jobs with uneven grids through the serving loop."""
+def serving_jobs(njobs, seed, iterations):
     rng = np.random.default_rng(seed)
     grids = [(48, 16, 16), (96, 32, 32), (128, 64, 32), (192, 64, 64)]
     jobs = []
@@ -118,8 +151,14 @@ def run_serving(ta, njobs, rate, seed, iterations):
         jobs.append(dict(nx=g[0], ny=g[1], nz=g[2], bc_kind=int(rng.integers(2)),
                          volfrac=float(rng.uniform(0.1, 0.4)), rmin=float(rng.uniform(1.2, 2.5)),
                          iterations=iterations, cg_tol=1e-6))
-    gaps = rng.exponential(1.0 / rate, njobs)
-    with ta.Server(budget_bytes=200 << 30, max_slots=16) as srv:
+    return jobs, rng.exponential(1.0, njobs)
+
+
+def run_serving(ta, njobs, rate, seed, iterations, slots=16):
+    """Poisson arrivals of jobs with uneven grids through the serving loop."""
+    jobs, unit_gaps = serving_jobs(njobs, seed, iterations)
+    gaps = unit_gaps / rate
+    with ta.Server(budget_bytes=200 << 30, max_slots=slots) as srv:
         ids = []
         t0 = time.perf_counter()
         t_next = t0
@@ -129,35 +168,143 @@ def run_serving(ta, njobs, rate, seed, iterations):
             if delay > 0:
                 time.sleep(delay)
             ids.append(srv.submit(j))
+            if len(ids) % 64 == 0:
+                log(f"# serving: {len(ids)} of {njobs} jobs submitted")
         srv.drain()
         makespan = time.perf_counter() - t0
         res = [srv.poll(i) for i in ids]
     failed = sum(r["state"] != 2 for r in res)
     lat = np.array([r["t_end_ms"] - r["t_submit_ms"] for r in res])
     its = sum(r["iterations"] for r in res)
-    return {"jobs": njobs, "failed": failed, "rate_per_s": rate, "seed": seed,
+    return {"jobs": njobs, "failed": failed, "rate_per_s": rate, "seed": seed, "slots": slots,
+            "iterations_per_job": iterations,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean()), "design_iters": its, "makespan_s": makespan,
             "design_iters_per_s": its / makespan}
 
 
-def cpu_baseline(wl, threads):
-    """One design iteration of the workload on the FP64 oracle (bounded sample)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
+def hip_mem_used():
+    """Device bytes in use (hipMemGetInfo), or None without the runtime."""
+    try:
+        hip = C.CDLL("libamdhip64.so")
+        free, total = C.c_size_t(0), C.c_size_t(0)
+        if hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
+            return None
+        return total.value - free.value
+    except OSError:
+        return None
+
+
+def run_large(ta, problems, iterations=3):
+    """Cantilever 512x256x256 (101.6 M DOF): a few design iterations, the
+    device memory the problem holds, K.u / smoother bandwidth at that size."""
+    wl = problems.workload("cantilever_512x256x256")
+    nx, ny, nz = wl["grid"]
+    m0 = hip_mem_used()
+    with ta.Problem(nx, ny, nz, wl["bcs"], wl["loads"]) as p:
+        out = p.optimize(wl["volfrac"], wl["rmin"], 1, cg_tol=wl["tol"],
+                         continuation=wl["continuation"])
+        t0 = time.perf_counter()
+        out = p.optimize(wl["volfrac"], wl["rmin"], iterations, cg_tol=wl["tol"],
+                         continuation=wl["continuation"])
+        dt = time.perf_counter() - t0
+        m1 = hip_mem_used()
+    ku, tile = ku_measure(ta, nx, ny, nz, reps=10, variants=("f32", "f64") + KU_LOOP)
+    return {"workload": "cantilever_512x256x256", "dof": 3 * (nx + 1) * (ny + 1) * (nz + 1),
+            "iterations": out["iterations"], "design_iters_per_s": out["iterations"] / dt,
+            "cg_iterations_total": out["cg_iterations_total"], "compliance": out["compliance"],
+            "device_bytes_held": None if m0 is None or m1 is None else m1 - m0,
+            "ku_tile": list(tile), "ku": ku,
+            "note": f"{iterations} iterations from the uniform design (continuation stage p = 1) "
+                    f"after one warm-up iteration; the full config runs 150"}
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    allowed = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in allowed:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            cores.add((open(base + "physical_package_id").read().strip(),
+                       open(base + "core_id").read().strip()))
+        except OSError:
+            cores.add(("?", str(c)))
+    return {"model": model, "logical_cpus_host": os.cpu_count(), "cpus_allowed": len(allowed),
+            "physical_cores_allowed": len(cores)}
+
+
+def oracle_first_iteration(oracle, s, wl, threads):
     oracle.set_threads(threads)
     nx, ny, nz = wl["grid"]
-    s = oracle.Setup(nx, ny, nz, bcs=wl["bcs"], loads=wl["loads"][0])
     x = np.full(nx * ny * nz, wl["volfrac"])
     t0 = time.perf_counter()
     xp = oracle.filter(s, wl["rmin"], x)
-    c, dc, _, info = oracle.simp_compliance(s, xp, penalty=wl["penalty"], tol=wl["tol"])
-    oracle.oc_update(s, wl["rmin"], x, dc, wl["volfrac"], wl["move"])
+    c, dc, u, info = oracle.simp_compliance(s, xp, penalty=wl["penalty"], tol=wl["tol"])
+    x1, nb, lam = oracle.oc_update(s, wl["rmin"], x, dc, wl["volfrac"], wl["move"])
     dt = time.perf_counter() - t0
-    return {"value": 1.0 / dt, "unit": "design iters/s", "cores": threads, "kind": "port",
-            "sample": f"1 design iteration (filter + MG-PCG solve incl. hierarchy build + "
-                      f"sensitivity + OC) of {nx}x{ny}x{nz} from the uniform design, FP64 oracle, "
-                      f"{info['iterations']} PCG iterations, {dt:.1f} s"}
+    return dict(dt=dt, x0=x, xp=xp, c=c, dc=dc, u=u, info=info, x1=x1, nb=nb, lam=lam)
+
+
+def cpu_baseline_and_parity(ta, wl, workload, gpu_history):
+    """The oracle's first design iteration on this host's cores (1 thread and
+    the physical cores this process may use, at most 16), and the GPU's first
+    design iteration compared with it result by result."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    nx, ny, nz = wl["grid"]
+    s = oracle.Setup(nx, ny, nz, bcs=wl["bcs"], loads=wl["loads"][0])
+    ci = cpu_info()
+    nthreads = max(1, min(16, ci["physical_cores_allowed"]))
+    ref = oracle_first_iteration(oracle, s, wl, nthreads)
+    log(f"# cpu oracle first iteration on {nthreads} threads: {ref['dt']:.1f} s")
+    ref1 = oracle_first_iteration(oracle, s, wl, 1)
+    log(f"# cpu oracle first iteration on 1 thread: {ref1['dt']:.1f} s")
+    cpu = {"value": 1.0 / ref["dt"], "unit": "design iters/s", "cores": nthreads, "kind": "port",
+           "value_1_thread": 1.0 / ref1["dt"], "cpu": ci,
+           "sample": f"the first design iteration (filter + MG-PCG solve incl. hierarchy build + "
+                     f"sensitivity + OC) of {workload} from the uniform design on the FP64 oracle, "
+                     f"{ref['info']['iterations']} PCG iterations: {ref['dt']:.1f} s on "
+                     f"{nthreads} threads, {ref1['dt']:.1f} s on 1 thread (the run's later "
+                     f"iterations take about twice the PCG iterations)"}
+    # GPU first iteration, result by result
+    with ta.Problem(nx, ny, nz, wl["bcs"], wl["loads"]) as p:
+        c, dc, u, info = p.compliance(ref["xp"], penalty=wl["penalty"], tol=wl["tol"], want_u=True)
+        x1, act, nb, lam = p.oc_update(wl["rmin"], ref["x0"], dc, wl["volfrac"], wl["move"])
+        xo, acto, nbo, lamo = p.oc_update(wl["rmin"], ref["x0"], ref["dc"], wl["volfrac"],
+                                          wl["move"])
+        gap, _, bound = p.oc_margin()
+    lo = np.maximum(1e-3, ref["x0"] - wl["move"])
+    hi = np.minimum(1.0, ref["x0"] + wl["move"])
+    ref_act = np.where(ref["x1"] == lo, 1, np.where(ref["x1"] == hi, 2, 0))
+    par = {
+        "iteration": 1,
+        "compliance": c, "compliance_oracle": ref["c"],
+        "compliance_rel_err": abs(c - ref["c"]) / abs(ref["c"]),
+        "loop_history0_compliance": float(gpu_history[0, 0]),
+        "loop_history0_rel_err": abs(gpu_history[0, 0] - ref["c"]) / abs(ref["c"]),
+        "u_norm_rel_err": abs(np.linalg.norm(u) - np.linalg.norm(ref["u"])) / np.linalg.norm(ref["u"]),
+        "u_max_rel_err": float(np.abs(u - ref["u"]).max() / np.abs(ref["u"]).max()),
+        "dc_max_rel_err": float(np.abs(dc - ref["dc"]).max() / np.abs(ref["dc"]).max()),
+        "pcg_iterations": info["iterations"], "pcg_iterations_oracle": ref["info"]["iterations"],
+        "loop_history0_pcg_iterations": int(gpu_history[0, 3]),
+        "mg_levels": info["mg_levels"], "mg_levels_oracle": ref["info"]["mg_levels"],
+        "density_max_abs_err": float(np.abs(x1 - ref["x1"]).max()),
+        "active_set_mismatches": int(np.count_nonzero(act != ref_act)),
+        "oc_on_oracle_inputs_bit_exact": bool(np.array_equal(xo, ref["x1"]) and nbo == ref["nb"]
+                                              and lamo == ref["lam"]),
+        "oc_lambda": lamo, "oc_lambda_oracle": ref["lam"], "oc_bisections": nbo,
+        "oc_bisections_oracle": ref["nb"],
+        "oc_decision_gap_rel": gap, "oc_rounding_bound_rel": bound,
+    }
+    return cpu, par
 
 
 def main():
@@ -168,14 +315,18 @@ def main():
     ap.add_argument("--workload", default="mbb_192x64x64")
     ap.add_argument("--no-fp64-form", action="store_true")
     ap.add_argument("--no-mma", action="store_true", help="skip the MMA-updater leg")
-
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--jobs", type=int, default=128)
+    ap.add_argument("--no-serving", action="store_true", help="skip the 512-job serving leg")
+    ap.add_argument("--no-large", action="store_true", help="skip the 512x256x256 leg")
+    ap.add_argument("--jobs", type=int, default=512)
     ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--ku-only", action="store_true",
                     help="only the K.u microbenchmark, after a 256 MiB device copy that "
                          "calibrates the PMC byte counters (for rocprofv3 --pmc passes)")
+    ap.add_argument("--variant", default=None,
+                    help="with --ku-only: one K.u variant (f32, f64, ap64_dot, res0_f32, "
+                         "jac_dot_f32)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this engine has no multi-GPU path (each job and each "
@@ -185,8 +336,7 @@ def main():
     from topopt_amd import problems
 
     if a.workload == "serving":
-        print(f"# serving: {a.jobs} jobs, Poisson rate {a.rate}/s, seed {a.seed}", file=sys.stderr,
-              flush=True)
+        log(f"# serving: {a.jobs} jobs, Poisson rate {a.rate}/s, seed {a.seed}")
         sv = run_serving(ta, a.jobs, a.rate, a.seed, iterations=60)
         print(json.dumps({
             "metric": "serving: aggregate design iterations/s (job p50/p99 latency in config)",
@@ -208,43 +358,55 @@ def main():
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
-        ku = ku_measure(ta, nx, ny, nz, reps=10)
-        print(json.dumps({"workload": a.workload, "ku": ku, "calibration_copy_bytes": 256 << 20}),
-              flush=True)
+        variants = (a.variant,) if a.variant else ("f32", "f64") + KU_LOOP
+        ku, tile = ku_measure(ta, nx, ny, nz, reps=10, variants=variants)
+        print(json.dumps({"workload": a.workload, "tile": list(tile), "ku": ku,
+                          "calibration_copy_bytes": 256 << 20}), flush=True)
         return
-    print(f"# workload {a.workload}: warmup {a.warmup}, timed {a.steps} design iterations",
-          file=sys.stderr, flush=True)
+    log(f"# workload {a.workload}: warmup {a.warmup}, timed {a.steps} design iterations")
     out, dt = run_design(ta, wl, ta.PREC_MIXED, a.steps, a.warmup)
     value = out["iterations"] / dt
-    print(f"# mixed form: {value:.2f} design iters/s, compliance {out['compliance']:.6g}, "
-          f"cg iters {out['cg_iterations_total']}", file=sys.stderr, flush=True)
+    log(f"# mixed form: {value:.2f} design iters/s, compliance {out['compliance']:.6g}, "
+        f"cg iters {out['cg_iterations_total']}")
     fp64 = None
     if not a.no_fp64_form:
         out64, dt64 = run_design(ta, wl, ta.PREC_F64, a.steps, min(a.warmup, 1))
         v64 = out64["iterations"] / dt64
         rel = abs(out["compliance"] - out64["compliance"]) / abs(out64["compliance"])
         fp64 = {"value": v64, "ratio": value / v64, "compliance": out64["compliance"],
-                "compliance_rel_diff_vs_mixed": rel, "cg_iterations_total": out64["cg_iterations_total"]}
-        print(f"# fp64 form: {v64:.2f} design iters/s", file=sys.stderr, flush=True)
+                "compliance_rel_diff_vs_mixed": rel,
+                "cg_iterations_total": out64["cg_iterations_total"]}
+        log(f"# fp64 form: {v64:.2f} design iters/s")
     mma = None
     if not a.no_mma:
         outm, dtm = run_design(ta, wl, ta.PREC_MIXED, a.steps, min(a.warmup, 1), mma=True)
         mma = {"value": outm["iterations"] / dtm, "compliance": outm["compliance"],
                "cg_iterations_total": outm["cg_iterations_total"], "t_oc_ms": outm["t_oc_ms"]}
-        print(f"# mma updater: {mma['value']:.2f} design iters/s, compliance "
-              f"{outm['compliance']:.6g}", file=sys.stderr, flush=True)
-    ku = ku_measure(ta, nx, ny, nz)
+        log(f"# mma updater: {mma['value']:.2f} design iters/s, compliance "
+            f"{outm['compliance']:.6g}")
+    ku, tile = ku_measure(ta, nx, ny, nz)
+    log("# K.u " + ", ".join(f"{k} {v['us']:.1f} us {v['GBs']:.0f} GB/s" for k, v in ku.items())
+        + f" (tile {tile})")
     fields = fields_measure(ta, nx, ny, nz)
-    print("# fields " + ", ".join(f"{k} {v['us']:.1f} us {v['GBs']:.0f} GB/s"
-                                  for k, v in fields.items()), file=sys.stderr, flush=True)
-    print(f"# K.u f32 {ku['f32']['ms']*1e3:.1f} us {ku['f32']['GBs']:.0f} GB/s; "
-          f"f64 {ku['f64']['ms']*1e3:.1f} us", file=sys.stderr, flush=True)
-    cpu = None
+    log("# fields " + ", ".join(f"{k} {v['us']:.1f} us {v['GBs']:.0f} GB/s"
+                                for k, v in fields.items()))
+    cpu = par = None
     if not a.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(wl, threads)
-        print(f"# cpu baseline {cpu['value']:.4f} design iters/s on {threads} threads",
-              file=sys.stderr, flush=True)
+        cpu, par = cpu_baseline_and_parity(ta, wl, a.workload, out["history"])
+        log(f"# vs oracle: compliance rel {par['compliance_rel_err']:.2e}, ||u|| rel "
+            f"{par['u_norm_rel_err']:.2e}, OC on oracle inputs bit-exact "
+            f"{par['oc_on_oracle_inputs_bit_exact']}")
+    serving = large = None
+    if not a.no_serving and a.workload == "mbb_192x64x64":
+        log(f"# serving: {a.jobs} jobs, Poisson rate {a.rate}/s, seed {a.seed}")
+        serving = run_serving(ta, a.jobs, a.rate, a.seed, iterations=60)
+        log(f"# serving: {serving['design_iters_per_s']:.1f} design iters/s, p50 "
+            f"{serving['p50_ms']:.0f} ms, p99 {serving['p99_ms']:.0f} ms")
+    if not a.no_large and a.workload == "mbb_192x64x64":
+        large = run_large(ta, problems)
+        log(f"# 512x256x256: {large['design_iters_per_s']:.2f} design iters/s, "
+            f"K.u f32 {large['ku']['f32']['us']:.0f} us")
+    dom = ku["ap64_dot"]
     line = {
         "metric": f"design iterations/s ({a.workload}, FP32 multigrid V-cycle in FP64 PCG)",
         "value": value,
@@ -258,22 +420,33 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 V-cycle, f64 operator and PCG",
         "data": "synthetic (analytic box workload, uniform initial density)",
-        "config": {"workload": a.workload, "grid": [nx, ny, nz], "dof": ku["f32"]["bytes"] and
-                   3 * (nx + 1) * (ny + 1) * (nz + 1), "volfrac": wl["volfrac"], "rmin": wl["rmin"],
-                   "penalty": wl["penalty"], "cg_tol": wl["tol"], "load_cases": len(wl["loads"]),
-                   "compliance": out["compliance"], "cg_iterations_total": out["cg_iterations_total"],
+        "config": {"workload": a.workload, "grid": [nx, ny, nz],
+                   "dof": 3 * (nx + 1) * (ny + 1) * (nz + 1), "volfrac": wl["volfrac"],
+                   "rmin": wl["rmin"], "penalty": wl["penalty"], "cg_tol": wl["tol"],
+                   "load_cases": len(wl["loads"]), "compliance": out["compliance"],
+                   "cg_iterations_total": out["cg_iterations_total"],
                    "t_solve_ms": out["t_solve_ms"], "t_build_ms": out["t_build_ms"],
-                   "t_oc_ms": out["t_oc_ms"], "t_filter_ms": out["t_filter_ms"]},
+                   "t_oc_ms": out["t_oc_ms"], "t_filter_ms": out["t_filter_ms"],
+                   "ku_tile": list(tile), "oc_decision_gap_min_rel": out["oc_gap"][1],
+                   "oc_rounding_bound_rel": out["oc_gap"][2]},
         "fp64_form": fp64,
+        "vs_oracle": par,
         "mma_updater": mma,
         "ku": ku,
         "result_fields": fields,
-        "roofline": {"bound": "hbm", "achieved": ku["f32"]["GBs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": ku["f32"]["hbm_frac"],
-                     "traffic": measured_traffic(a.workload),
-                     "kernel": "ku_face<float> fine modal apply (autotuned tile)",
-                     "bytes_per_launch": ku["f32"]["bytes"], "us_per_launch": ku["f32"]["ms"] * 1e3},
+        "roofline": {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": dom["hbm_frac"],
+                     "traffic": measured_traffic(a.workload, "ap64_dot", tile),
+                     "kernel": "ku_face<double> EPI_AX + <p,Ap> partials (the PCG's A.p)",
+                     "tile": list(tile), "bytes_per_launch": dom["bytes"],
+                     "us_per_launch": dom["us"]},
+        "roofline_loop": {k: {"frac": ku[k]["hbm_frac"], "GBs": ku[k]["GBs"], "us": ku[k]["us"],
+                              "bytes": ku[k]["bytes"],
+                              "traffic": measured_traffic(a.workload, k, tile)}
+                          for k in KU_LOOP},
         "cpu_baseline": cpu,
+        "serving": serving,
+        "large": large,
     }
     print(json.dumps(line), flush=True)
 

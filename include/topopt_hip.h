@@ -277,14 +277,29 @@ int topohip_filter_masked(topohip_problem* p, double radius, const uint8_t* mask
                           const double* x, double* out, int adjoint);
 
 /* ---- device-resident benchmarking entry (inputs already in HBM) -------------
- * Runs `reps` back-to-back applies of the fine operator on the handle's own
- * device buffers (random u, E = rho^3 set by topohip_bench_prepare) and
+ * Runs `reps` back-to-back applies of the fine operator on copies of the
+ * handle's bench inputs (random u, E = rho^3 set by topohip_bench_prepare) and
  * returns the mean kernel time measured with HIP events on the handle's
  * stream. precision: TOPOHIP_PREC_MIXED = FP32 apply, TOPOHIP_PREC_F64. */
 int topohip_bench_prepare(topohip_problem* p, uint64_t seed);
 int topohip_bench_ku(topohip_problem* p, int precision, int reps, double* ms_per_apply);
+/* The fine-operator kernel variants the design loop launches, timed the same
+ * way: variant 0 = FP32 y = K u, 1 = FP64 y = K u, 2 = FP64 A.p with the
+ * fused <p, Ap> partials (the PCG's operator apply), 3 = FP32 zero-start
+ * damped-Jacobi residual (the V-cycle's pre-smoothing apply), 4 = FP32
+ * damped-Jacobi sweep with the fused <r, z> partials (post-smoothing). Every
+ * launch cycles to the next of *sets_used (nullable) operand sets, enough that
+ * they span more than twice the 256 MiB last-level cache, so inputs come from
+ * HBM. The launch shape is the handle's tile (topohip_ku_tile). No reference
+ * counterpart (topohip_bench_ku is variants 0 / 1). */
+int topohip_bench_ku_variant(topohip_problem* p, int variant, int reps, double* ms_per_apply,
+                             int* sets_used);
+/* The fine K.u launch shape the handle uses: tx x ty lanes per workgroup over a
+ * z-chunk of tz node planes (autotuned at creation, or TOPOHIP_KU_TILE). */
+int topohip_ku_tile(topohip_problem* p, int* tx, int* ty, int* tz);
 /* Average device time (ms) of the result-field kernels over reps launches on
- * the bench inputs (topohip_bench_prepare): ms3[0] von Mises to float32,
+ * the bench inputs (topohip_bench_prepare), rotated over operand sets that
+ * span more than twice the last-level cache: ms3[0] von Mises to float32,
  * ms3[1] displacement narrowing, ms3[2] density halves. No reference
  * counterpart. */
 int topohip_bench_fields(topohip_problem* p, int reps, double* ms3);

@@ -447,6 +447,21 @@ class Problem:
         _check(lib().topohip_bench_ku(self._h, int(precision), int(reps), C.byref(ms)))
         return ms.value
 
+    KU_VARIANTS = {"f32": 0, "f64": 1, "ap64_dot": 2, "res0_f32": 3, "jac_dot_f32": 4}
+
+    def bench_ku_variant(self, variant, reps=50):
+        """(ms per launch, rotated operand sets) of a fine K.u kernel variant."""
+        ms = C.c_double(0)
+        ns = C.c_int(0)
+        _check(lib().topohip_bench_ku_variant(self._h, int(self.KU_VARIANTS[variant]), int(reps),
+                                              C.byref(ms), C.byref(ns)))
+        return ms.value, ns.value
+
+    def ku_tile(self):
+        t = [C.c_int(0), C.c_int(0), C.c_int(0)]
+        _check(lib().topohip_ku_tile(self._h, *[C.byref(v) for v in t]))
+        return tuple(v.value for v in t)
+
     def bench_fields(self, reps=50):
         """ms per launch: (von Mises -> f32, displacement -> f32, density -> f16)."""
         ms = (C.c_double * 3)()

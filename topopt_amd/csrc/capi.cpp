@@ -2,6 +2,9 @@
 // to status codes and keeps the message per calling thread.
 #include <cmath>
 #include <cstring>
+#include <functional>
+#include <type_traits>
+#include <vector>
 #include <random>
 #include <string>
 
@@ -99,6 +102,40 @@ void apply_host(Problem& P, const double* youngs, const double* u, double* y) {
   TOPOHIP_CHECK(hipMemcpyAsync(y, y64.p, nd * sizeof(double), hipMemcpyDeviceToHost, P.s));
   TOPOHIP_CHECK(hipStreamSynchronize(P.s));
 }
+}  // namespace
+
+namespace {
+// Buffer sets cycled through by the microbenchmarks so that their working set
+// (sets x bytes per launch) exceeds twice the 256 MiB last-level cache: every
+// launch then reads its inputs from HBM, as the design loop's operands at
+// these sizes largely are.
+int rotation_sets(double bytes_per_launch) {
+  const double want = 2.0 * 256.0 * 1024.0 * 1024.0;
+  return static_cast<int>(std::min(64.0, std::max(1.0, std::ceil(want / bytes_per_launch))));
+}
+
+double time_launches(hipStream_t s, int reps, const std::function<void(int)>& run) {
+  hipEvent_t e0, e1;
+  TOPOHIP_CHECK(hipEventCreate(&e0));
+  TOPOHIP_CHECK(hipEventCreate(&e1));
+  run(0);
+  TOPOHIP_CHECK(hipEventRecord(e0, s));
+  for (int i = 0; i < reps; ++i) run(i + 1);
+  TOPOHIP_CHECK(hipEventRecord(e1, s));
+  TOPOHIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  TOPOHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return static_cast<double>(ms) / reps;
+}
+
+// One operand set of a fine K.u launch in precision T.
+template <typename T>
+struct KuSet {
+  T *x, *y, *b, *d, *xw, *E;
+  double* r;
+};
 }  // namespace
 
 extern "C" {
@@ -408,39 +445,93 @@ int topohip_bench_prepare(topohip_problem* p, uint64_t seed) {
   });
 }
 
-int topohip_bench_ku(topohip_problem* p, int precision, int reps, double* ms_per_apply) {
+int topohip_ku_tile(topohip_problem* p, int* tx, int* ty, int* tz) {
+  return guarded([&] {
+    need(p, "problem");
+    const Problem& P = *p->P;
+    if (tx) *tx = P.tx;
+    if (ty) *ty = P.ty;
+    if (tz) *tz = P.tz;
+  });
+}
+
+
+int topohip_bench_ku_variant(topohip_problem* p, int variant, int reps, double* ms_per_apply,
+                             int* sets_used) {
   return guarded([&] {
     need(p, "problem"); need(ms_per_apply, "ms_per_apply");
     if (!p->bench_ready) throw Error(TOPOHIP_EINVAL, "call topohip_bench_prepare first");
     if (reps < 1) throw Error(TOPOHIP_EINVAL, "reps must be >= 1");
+    if (variant < 0 || variant > 4) throw Error(TOPOHIP_EINVAL, "unknown K.u variant");
     Problem& P = *p->P;
-    hipEvent_t e0, e1;
-    TOPOHIP_CHECK(hipEventCreate(&e0));
-    TOPOHIP_CHECK(hipEventCreate(&e1));
-    auto run = [&]() {
-      if (precision == TOPOHIP_PREC_F64) {
-        KuOp<double> op;
-        op.g = P.g; op.tz = P.tz; op.tx = P.tx; op.ty = P.ty; op.epi = EPI_AX; op.E = P.E64.p; op.mg = P.mg64;
-        op.x = p->bu64.p; op.y = p->by64.p;
-        launch_ku_fine<double>(op, P.s);
-      } else {
-        KuOp<float> op;
-        op.g = P.g; op.tz = P.tz; op.tx = P.tx; op.ty = P.ty; op.epi = EPI_AX; op.E = P.E32.p; op.mg = P.mg32;
-        op.x = p->bu32.p; op.y = p->by32.p;
-        launch_ku_fine<float>(op, P.s);
+    const long long nd = P.g.ndof(), nv = P.g.elems();
+    const bool f64 = variant == 1 || variant == 2;
+    const int w = f64 ? 8 : 4;
+    // algorithmic bytes of the variant (the per-launch figures of the bench)
+    double bytes = 2.0 * w * nd + static_cast<double>(w) * nv;
+    if (variant == 3) bytes += 3.0 * w * nd;              // b, dinv read; x0 written
+    if (variant == 4) bytes += 3.0 * w * nd + 8.0 * nd;   // x, b, dinv + FP64 r read
+    const int ns = rotation_sets(bytes);
+    if (sets_used) *sets_used = ns;
+    // one arena: per set x, y, b, dinv, xw, E in T and r in FP64
+    const size_t per = static_cast<size_t>(5 * nd + nv) * w + static_cast<size_t>(nd) * 8;
+    DBuf<unsigned char> arena;
+    arena.ensure(per * ns);
+    auto fill_sets = [&](auto tag) {
+      using T = decltype(tag);
+      std::vector<KuSet<T>> S(static_cast<size_t>(ns));
+      const T* u = f64 ? reinterpret_cast<const T*>(p->bu64.p) : reinterpret_cast<const T*>(p->bu32.p);
+      const T* e = f64 ? reinterpret_cast<const T*>(P.E64.p) : reinterpret_cast<const T*>(P.E32.p);
+      for (int k = 0; k < ns; ++k) {
+        unsigned char* b0 = arena.p + per * k;
+        KuSet<T>& q = S[k];
+        q.x = reinterpret_cast<T*>(b0);
+        q.y = q.x + nd; q.b = q.y + nd; q.d = q.b + nd; q.xw = q.d + nd; q.E = q.xw + nd;
+        q.r = reinterpret_cast<double*>(q.E + nv);
+        TOPOHIP_CHECK(hipMemcpyAsync(q.x, u, nd * sizeof(T), hipMemcpyDeviceToDevice, P.s));
+        TOPOHIP_CHECK(hipMemcpyAsync(q.b, u, nd * sizeof(T), hipMemcpyDeviceToDevice, P.s));
+        TOPOHIP_CHECK(hipMemcpyAsync(q.E, e, nv * sizeof(T), hipMemcpyDeviceToDevice, P.s));
+        launch_fill<T>(q.d, T(0.5), nd, P.s);
+        TOPOHIP_CHECK(hipMemcpyAsync(q.r, p->bu64.p, nd * sizeof(double), hipMemcpyDeviceToDevice, P.s));
       }
+      return S;
     };
-    run();
-    TOPOHIP_CHECK(hipEventRecord(e0, P.s));
-    for (int i = 0; i < reps; ++i) run();
-    TOPOHIP_CHECK(hipEventRecord(e1, P.s));
-    TOPOHIP_CHECK(hipEventSynchronize(e1));
-    float ms = 0.f;
-    TOPOHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-    *ms_per_apply = ms / reps;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    auto op_of = [&](auto& q, auto mg) {
+      using T = std::remove_pointer_t<decltype(q.x)>;
+      KuOp<T> op;
+      op.g = P.g; op.tz = P.tz; op.tx = P.tx; op.ty = P.ty; op.mg = mg; op.E = q.E;
+      op.mask = P.L[0]->mask.p;
+      op.pstride = P.pstride;
+      op.y = q.y;
+      if (variant == 3) {
+        op.epi = EPI_RES0; op.b = q.b; op.dinv = q.d; op.omega_d = 0.6; op.xw = q.xw;
+      } else if (variant == 4) {
+        op.epi = EPI_JAC; op.x = q.x; op.b = q.b; op.dinv = q.d; op.omega = T(0.6);
+        op.partials = P.partials.p; op.rdot = q.r;
+      } else {
+        op.epi = EPI_AX; op.x = q.x;
+        if (variant == 2) op.partials = P.partials.p;
+      }
+      return op;
+    };
+    if (f64) {
+      auto S = fill_sets(double());
+      *ms_per_apply = time_launches(P.s, reps, [&](int i) {
+        launch_ku_fine<double>(op_of(S[i % ns], P.mg64), P.s);
+      });
+    } else {
+      auto S = fill_sets(float());
+      *ms_per_apply = time_launches(P.s, reps, [&](int i) {
+        launch_ku_fine<float>(op_of(S[i % ns], P.mg32), P.s);
+      });
+    }
+    TOPOHIP_CHECK(hipStreamSynchronize(P.s));
   });
+}
+
+int topohip_bench_ku(topohip_problem* p, int precision, int reps, double* ms_per_apply) {
+  return topohip_bench_ku_variant(p, precision == TOPOHIP_PREC_F64 ? 1 : 0, reps, ms_per_apply,
+                                  nullptr);
 }
 
 // ---- result fields ------------------------------------------------------------
@@ -452,32 +543,36 @@ int topohip_bench_fields(topohip_problem* p, int reps, double* ms3) {
     if (reps < 1) throw Error(TOPOHIP_EINVAL, "reps must be >= 1");
     Problem& P = *p->P;
     const long long nv = P.g.elems(), nd = P.g.ndof();
-    P.fvm32.ensure(static_cast<size_t>(nv));
-    P.fu32.ensure(static_cast<size_t>(nd));
-    P.fh16.ensure(static_cast<size_t>(nv));
     const double c = 1.0 / ((1.0 + P.nu) * (1.0 - 2.0 * P.nu));
     const VmCoef k{(2.0 / P.h) * 0.125, c * (1.0 - P.nu), c * P.nu, c * (1.0 - 2.0 * P.nu) / 2.0};
     const uint8_t* sol = P.all_solid ? nullptr : P.solid.p;
-    hipEvent_t e0, e1;
-    TOPOHIP_CHECK(hipEventCreate(&e0));
-    TOPOHIP_CHECK(hipEventCreate(&e1));
-    for (int which = 0; which < 3; ++which) {
-      auto run = [&]() {
-        if (which == 0) launch_von_mises(P.g, k, sol, p->bu64.p, nullptr, P.fvm32.p, P.s);
-        else if (which == 1) launch_narrow_f32(p->bu64.p, P.fu32.p, nd, P.s);
-        else launch_density_f16(p->bu64.p, P.fh16.p, nv, P.s);
-      };
-      run();
-      TOPOHIP_CHECK(hipEventRecord(e0, P.s));
-      for (int i = 0; i < reps; ++i) run();
-      TOPOHIP_CHECK(hipEventRecord(e1, P.s));
-      TOPOHIP_CHECK(hipEventSynchronize(e1));
-      float ms = 0.f;
-      TOPOHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-      ms3[which] = static_cast<double>(ms) / reps;
+    // per rotated set: the FP64 input (displacement, or density in its first
+    // nv entries) and the three outputs
+    const size_t per = static_cast<size_t>(nd) * 8 + static_cast<size_t>(nv) * 4 +
+                       static_cast<size_t>(nd) * 4 + static_cast<size_t>(nv) * 2 + 256;
+    const int ns = rotation_sets(static_cast<double>(12 * nd));
+    DBuf<unsigned char> arena;
+    arena.ensure(per * ns);
+    std::vector<double*> in(ns);
+    std::vector<float*> vm(ns), du(ns);
+    std::vector<uint16_t*> h16(ns);
+    for (int s = 0; s < ns; ++s) {
+      unsigned char* b0 = arena.p + per * s;
+      in[s] = reinterpret_cast<double*>(b0);
+      vm[s] = reinterpret_cast<float*>(in[s] + nd);
+      du[s] = vm[s] + nv;
+      h16[s] = reinterpret_cast<uint16_t*>(du[s] + nd);
+      TOPOHIP_CHECK(hipMemcpyAsync(in[s], p->bu64.p, nd * sizeof(double), hipMemcpyDeviceToDevice, P.s));
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    for (int which = 0; which < 3; ++which) {
+      ms3[which] = time_launches(P.s, reps, [&](int i) {
+        const int s = i % ns;
+        if (which == 0) launch_von_mises(P.g, k, sol, in[s], nullptr, vm[s], P.s);
+        else if (which == 1) launch_narrow_f32(in[s], du[s], nd, P.s);
+        else launch_density_f16(in[s], h16[s], nv, P.s);
+      });
+    }
+    TOPOHIP_CHECK(hipStreamSynchronize(P.s));
   });
 }
 
