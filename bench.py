@@ -81,13 +81,11 @@ def measured_traffic(workload, variant, tile):
     path = os.path.join(ROOT, "profiles", f"ku_{variant}_{grid}_traffic.md")
     if not os.path.exists(path):
         return None
-    val, t = None, None
+    key = "traffic_bytes_per_launch[" + ",".join(str(int(v)) for v in tile) + "]:"
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            val = float(line.split(":", 1)[1])
-        if line.startswith("tile:"):
-            t = tuple(int(v) for v in line.split(":", 1)[1].split(","))
-    return val if t == tuple(tile) else None
+        if line.startswith(key):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def run_design(ta, wl, precision, steps, warmup, mma=False):

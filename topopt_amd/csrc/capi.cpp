@@ -77,7 +77,7 @@ void apply_host(Problem& P, const double* youngs, const double* u, double* y) {
   TOPOHIP_CHECK(hipMemcpyAsync(u64.p, u, nd * sizeof(double), hipMemcpyHostToDevice, P.s));
   KuOp<T> op;
   op.g = P.g;
-  op.tz = P.tz; op.tx = P.tx; op.ty = P.ty;
+  op.tz = P.tz;
   op.epi = EPI_AX;
   if constexpr (sizeof(T) == 4) {
     DBuf<float> u32, y32;
@@ -449,8 +449,8 @@ int topohip_ku_tile(topohip_problem* p, int* tx, int* ty, int* tz) {
   return guarded([&] {
     need(p, "problem");
     const Problem& P = *p->P;
-    if (tx) *tx = P.tx;
-    if (ty) *ty = P.ty;
+    if (tx) *tx = kTX;
+    if (ty) *ty = kTY;
     if (tz) *tz = P.tz;
   });
 }
@@ -499,7 +499,7 @@ int topohip_bench_ku_variant(topohip_problem* p, int variant, int reps, double* 
     auto op_of = [&](auto& q, auto mg) {
       using T = std::remove_pointer_t<decltype(q.x)>;
       KuOp<T> op;
-      op.g = P.g; op.tz = P.tz; op.tx = P.tx; op.ty = P.ty; op.mg = mg; op.E = q.E;
+      op.g = P.g; op.tz = P.tz; op.mg = mg; op.E = q.E;
       op.mask = P.L[0]->mask.p;
       op.pstride = P.pstride;
       op.y = q.y;

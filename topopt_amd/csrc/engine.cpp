@@ -420,7 +420,7 @@ void alloc_levels(Problem& P) {
 }
 
 int max_partials(const Problem& P) {
-  return std::max(ku_blocks(P.g, P.tz, P.tx, P.ty), std::max(dot_blocks(P.g.ndof()), sens_blocks(P.g)));
+  return std::max(ku_partials(P.g), std::max(dot_blocks(P.g.ndof()), sens_blocks(P.g)));
 }
 
 // ---- V-cycle ----------------------------------------------------------------
@@ -462,8 +462,6 @@ KuOp<T> fine_op(Problem& P, const int* done, int nc) {
   op.E = Ev<T>(P);
   op.mg = Mg<T>(P);
   op.tz = P.tz;
-  op.tx = P.tx;
-  op.ty = P.ty;
   op.done = done;
   op.nrhs = nc;
   op.pstride = P.pstride;
@@ -524,11 +522,11 @@ int precondition_rz(Problem& P, Prec pc, const int* done, int nc) {
   const long long nd = P.g.ndof();
   if (pc == Prec::MG32) {
     vcycle<float>(P, P.r32.p, P.z32.p, done, nc, P.r.p, P.partials.p);
-    return ku_blocks(P.g, P.tz, P.tx, P.ty);
+    return ku_partials(P.g);
   }
   if (pc == Prec::MG64) {
     vcycle<double>(P, P.r.p, P.z64.p, done, nc, P.r.p, P.partials.p);
-    return ku_blocks(P.g, P.tz, P.tx, P.ty);
+    return ku_partials(P.g);
   }
   launch_jacobi0<double>(P.z64.p, P.L[0]->dinv64.p, P.r.p, 1.0, nd, done, P.s, nc);
   const int nb = dot_blocks(nd);
@@ -553,7 +551,7 @@ void restart(Problem& P, Prec pc, int nc) {
 void iteration(Problem& P, Prec pc, int nc) {
   const long long nd = P.g.ndof();
   const int* done = done_flags(P);
-  const int kb = ku_blocks(P.g, P.tz, P.tx, P.ty);
+  const int kb = ku_partials(P.g);
   const int db = dot_blocks(nd);
   // A.p partials go to `partials`, the rr partials of cg_xr to `partials2`
   // (cg_xr re-sums the former in every block while it writes the latter).
@@ -740,35 +738,33 @@ void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool
 
 }  // namespace
 
-// Tile autotuner: time the fine-level apply for each (tile, z-chunk)
-// candidate on the problem's own buffers and keep the fastest (override:
-// TOPOHIP_KU_TILE=tx,ty,tz). The choice only changes the launch shape; every
-// candidate computes bit-identical applies (the per-node summation order does
-// not depend on the tile).
+// Launch-shape autotuner: time the fine-level apply for each z-chunk length
+// on the problem's own buffers and keep the fastest (override:
+// TOPOHIP_KU_TILE=tz, or 32,8,tz). The choice changes the launch shape only:
+// every z-chunk computes bit-identical applies (each node sums its face
+// contributions in a fixed order) and identical dot partials (one per fixed
+// slab of planes), so results do not depend on what the tuner picks.
 void autotune_ku(Problem& P) {
   if (const char* env = std::getenv("TOPOHIP_KU_TILE")) {
-    int tx = 16, ty = 16, tz = 8;
-    if (std::sscanf(env, "%d,%d,%d", &tx, &ty, &tz) == 3 && tz > 0 && ku_tile_supported(tx, ty)) {
-      P.tx = tx;
-      P.ty = ty;
+    int a = 0, b = 0, c = 0;
+    const int n = std::sscanf(env, "%d,%d,%d", &a, &b, &c);
+    const int tz = n == 3 ? (a == kTX && b == kTY ? c : 0) : (n == 1 ? a : 0);
+    if (tz >= kSlab && tz % kSlab == 0) {
       P.tz = tz;
+      return;
     }
-    return;
   }
   if (P.g.elems() < 4096) return;
   // The choice is cached per (device, grid, precision) for the life of the
-  // process: the tile decides the order of the PCG's partial dot products, so
-  // problems of one shape then give identical results whatever else runs.
+  // process, so problems of one shape launch alike.
   static std::mutex mu;
-  static std::map<std::array<int, 5>, std::array<int, 3>> cache;
+  static std::map<std::array<int, 5>, int> cache;
   const std::array<int, 5> key{P.device, P.g.ex, P.g.ey, P.g.ez, P.precision};
   {
     std::lock_guard<std::mutex> lk(mu);
     auto it = cache.find(key);
     if (it != cache.end()) {
-      P.tx = it->second[0];
-      P.ty = it->second[1];
-      P.tz = it->second[2];
+      P.tz = it->second;
       return;
     }
   }
@@ -782,39 +778,34 @@ void autotune_ku(Problem& P) {
   TOPOHIP_CHECK(hipEventCreate(&e0));
   TOPOHIP_CHECK(hipEventCreate(&e1));
   float best = 1e30f;
-  const int tiles[5][2] = {{16, 16}, {32, 8}, {64, 4}, {32, 16}, {64, 8}};
-  const int tzs[4] = {2, 4, 8, 16};
-  for (const auto& tl : tiles)
-    for (int tz : tzs) {
-      auto run = [&]() {
-        if (mixed) {
-          KuOp<float> op;
-          op.g = P.g; op.E = P.E32.p; op.mg = P.mg32; op.tz = tz; op.tx = tl[0]; op.ty = tl[1];
-          op.x = P.r32.p; op.y = P.z32.p; op.epi = EPI_AX;  // (scratch at set-up)
-          launch_ku_fine<float>(op, P.s);
-        } else {
-          KuOp<double> op;
-          op.g = P.g; op.E = P.E64.p; op.mg = P.mg64; op.tz = tz; op.tx = tl[0]; op.ty = tl[1];
-          op.x = P.x.p; op.y = P.Ap64.p; op.epi = EPI_AX;
-          launch_ku_fine<double>(op, P.s);
-        }
-      };
-      run();
-      TOPOHIP_CHECK(hipEventRecord(e0, P.s));
-      for (int i = 0; i < 5; ++i) run();
-      TOPOHIP_CHECK(hipEventRecord(e1, P.s));
-      TOPOHIP_CHECK(hipEventSynchronize(e1));
-      float ms = 0.f;
-      TOPOHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) { best = ms; P.tx = tl[0]; P.ty = tl[1]; P.tz = tz; }
-    }
+  const int tzs[4] = {4, 8, 16, 32};
+  for (int tz : tzs) {
+    auto run = [&]() {
+      if (mixed) {
+        KuOp<float> op;
+        op.g = P.g; op.E = P.E32.p; op.mg = P.mg32; op.tz = tz;
+        op.x = P.r32.p; op.y = P.z32.p; op.epi = EPI_AX;  // (scratch at set-up)
+        launch_ku_fine<float>(op, P.s);
+      } else {
+        KuOp<double> op;
+        op.g = P.g; op.E = P.E64.p; op.mg = P.mg64; op.tz = tz;
+        op.x = P.x.p; op.y = P.Ap64.p; op.epi = EPI_AX;
+        launch_ku_fine<double>(op, P.s);
+      }
+    };
+    run();
+    TOPOHIP_CHECK(hipEventRecord(e0, P.s));
+    for (int i = 0; i < 5; ++i) run();
+    TOPOHIP_CHECK(hipEventRecord(e1, P.s));
+    TOPOHIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    TOPOHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) { best = ms; P.tz = tz; }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   std::lock_guard<std::mutex> lk(mu);
-  auto ins = cache.emplace(key, std::array<int, 3>{P.tx, P.ty, P.tz});
-  P.tx = ins.first->second[0];  // a concurrent tuner of the same shape may have won
-  P.ty = ins.first->second[1];
-  P.tz = ins.first->second[2];
+  P.tz = cache.emplace(key, P.tz).first->second;  // a concurrent tuner may have won
 }
 
 void drop_graphs(Problem& P) {
@@ -1113,8 +1104,6 @@ void solve_cases(Problem& P, int nc, double tol, int max_it, const double* d_gue
     op.E = P.E64.p;
     op.mg = P.mg64;
     op.tz = P.tz;
-    op.tx = P.tx;
-    op.ty = P.ty;
     op.x = P.up.p;
     op.y = P.rt.p;
     op.epi = EPI_AX;

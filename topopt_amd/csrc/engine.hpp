@@ -50,9 +50,7 @@ struct Problem {
   Geom g;
   double h = 1.0, nu = 0.3;
   int precision = TOPOHIP_PREC_MIXED;
-  int tz = 8;   // fine K.u z-chunk (autotuned)
-  int tx = 16;  // fine K.u tile (autotuned): tx x ty lanes
-  int ty = 16;
+  int tz = 8;   // fine K.u z-chunk (autotuned; a multiple of kSlab)
 
   bool all_solid = true;
   std::vector<uint8_t> solid_h;

@@ -32,15 +32,17 @@ struct KuOp {
   int epi = EPI_AX;
   double* partials = nullptr;
   int tz = 8;                     // node planes per workgroup (z-march length)
-  int tx = 16;                    // workgroup tile: tx x ty element columns (lanes)
-  int ty = 16;
   const int* done = nullptr;      // case r exits early when done[r * kDoneStride] != 0
   int nrhs = 1;                   // stacked load cases (x, y, b stride g.ndof())
   int pstride = 0;                // partials of case r at partials + r * pstride
 };
 
-bool ku_tile_supported(int tx, int ty);
-int ku_blocks(const Geom& g, int tz, int tx, int ty);
+// Fine K.u launch shape: kTX x kTY lanes per workgroup (one element column
+// each, owning (kTX-1) x (kTY-1) nodes) over a z-chunk of tz node planes, tz a
+// multiple of kSlab; the FP64 dot partials are one per (kSlab-plane slab,
+// x-y tile), ku_partials(g) per load case, whatever tz is.
+constexpr int kTX = 32, kTY = 8, kSlab = 4;
+int ku_partials(const Geom& g);
 template <typename T> void launch_ku_fine(const KuOp<T>& op, hipStream_t s);
 
 // ---- transfers between a fine level f and the coincident coarse level c ----

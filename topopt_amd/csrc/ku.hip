@@ -21,6 +21,8 @@
 //  * Node-owned deterministic accumulation: every lane writes its face's 4
 //    corner results to LDS and the owning lane sums its 4 face contributions
 //    in a fixed order. No atomics, bit-identical run to run.
+#include <string>
+
 #include "kernels.hpp"
 
 namespace topohip {
@@ -86,8 +88,16 @@ __device__ inline void modal_core(const T (&A)[24], T (&v)[24], T e, const Modal
 // Inputs x must be zero on inactive DOFs (every solver vector is); the mask
 // only decides which outputs are kept.
 // DOTK: 0 none, 1 <x, Ax> over kept DOFs, 2 <rdot, y> over kept DOFs (FP64 partials).
-template <typename T, int TX, int TY, int EPI, int DOTK>
-__global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
+//
+// Launch shape: kTX x kTY lanes (one element column each) over a z-chunk of
+// a.tz node planes, a.tz a multiple of kSlab. The FP64 dot partials are one
+// per (kSlab-plane slab, tile): each lane sums its node's terms over the slab's
+// planes in plane order and the workgroup adds its four waves in a fixed
+// order, so the partials (and every PCG scalar formed from them) do not depend
+// on the z-chunk the autotuner picks.
+template <typename T, int EPI, int DOTK>
+__global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
+  constexpr int TX = kTX, TY = kTY;
   constexpr int NT = TX * TY;
   constexpr int PW = TX + 1;                  // staged plane: PW x (TY + 1) nodes
   constexpr int PLANE = PW * (TY + 1) * 3;
@@ -113,6 +123,7 @@ __global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
   const int tx = t % TX, ty = t / TX;
   const int i0 = bx * (TX - 1), j0 = by * (TY - 1);
   const int k0 = chunk * a.tz;
+  const int kend = min(k0 + a.tz, g.nnz);
   const int ei = i0 - 1 + tx, ej = j0 - 1 + ty;
   const int na = i0 + tx, nb = j0 + ty;
   const bool owner = tx < TX - 1 && ty < TY - 1 && na < g.nnx && nb < g.nny;
@@ -128,48 +139,62 @@ __global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
   const double* __restrict__ rd = a.rdot ? a.rdot + vo : nullptr;
   const uint8_t* __restrict__ mask = a.mask;
   const T* __restrict__ E = a.E;
-  // Loop-invariant in-plane offsets (-1: outside the grid).
+  // Loop-invariant in-plane offsets (node 0 for a halo position outside the
+  // grid, whose value is zeroed through `valid`).
   int foff[NPF];
+  unsigned valid = 0;
 #pragma unroll
   for (int i = 0; i < NPF; ++i) {
     const int q = t + i * NT;
-    foff[i] = -1;
+    foff[i] = 0;
     if (q < PLANE) {
       const int n = q / 3, comp = q - 3 * n;
       const int ga = i0 - 1 + n % PW, gb = j0 - 1 + n / PW;
-      if (ga >= 0 && ga < g.nnx && gb >= 0 && gb < g.nny) foff[i] = (gb * g.nnx + ga) * 3 + comp;
+      if (ga >= 0 && ga < g.nnx && gb >= 0 && gb < g.nny) {
+        foff[i] = (gb * g.nnx + ga) * 3 + comp;
+        valid |= 1u << i;
+      }
     }
   }
   const int eoff = exy ? ej * g.ex + ei : 0;
   const int noff = owner ? nb * g.nnx + na : 0;
 
-  T pf[NPF];
-  auto fetch_plane = [&](int c) {
+  T pf[NPF], pd[NPF];  // plane values in flight (pd: D^-1 of EPI_RES0)
+  auto fetch_plane = [&](int c, T (&pf)[NPF]) {
     if (c >= 0 && c < g.nnz) {
       if constexpr (EPI == EPI_RES0) {
         // one damped-Jacobi sweep from zero, formed on the fly: x0 = omega D^-1 b
         const T* bp = bs + c * pn * 3;
         const T* dp = ds + c * pn * 3;
 #pragma unroll
-        for (int i = 0; i < NPF; ++i)
-          pf[i] = foff[i] >= 0 ? static_cast<T>(a.omega_d * static_cast<double>(dp[foff[i]]) *
-                                                static_cast<double>(bp[foff[i]]))
-                               : T(0);
+        for (int i = 0; i < NPF; ++i) {
+          pf[i] = bp[foff[i]];
+          pd[i] = dp[foff[i]];
+        }
       } else {
         const T* xp = xs + c * pn * 3;
 #pragma unroll
-        for (int i = 0; i < NPF; ++i) pf[i] = foff[i] >= 0 ? xp[foff[i]] : T(0);
+        for (int i = 0; i < NPF; ++i) pf[i] = (valid >> i) & 1u ? xp[foff[i]] : T(0);
       }
     } else {
 #pragma unroll
-      for (int i = 0; i < NPF; ++i) pf[i] = T(0);
+      for (int i = 0; i < NPF; ++i) pf[i] = pd[i] = T(0);
     }
   };
-  auto store_plane = [&](int buf) {
+  // The loaded values are first used here, a layer after their loads were
+  // issued. EPI_RES0 loads b and D^-1 unconditionally at in-grid addresses
+  // and zeroes halo positions outside the grid here (measured faster than
+  // predicated loads for that variant; the others load predicated).
+  auto store_plane = [&](int buf, const T (&pf)[NPF]) {
 #pragma unroll
     for (int i = 0; i < NPF; ++i) {
       const int q = t + i * NT;
-      if (q < PLANE) s_u[buf][q] = pf[i];
+      T v = pf[i];
+      if constexpr (EPI == EPI_RES0) {
+        v = static_cast<T>(a.omega_d * static_cast<double>(pd[i]) * static_cast<double>(v));
+        v = (valid >> i) & 1u ? v : T(0);
+      }
+      if (i + 1 < NPF || q < PLANE) s_u[buf][q] = v;
     }
   };
   auto load_face = [&](int buf, T (&F)[12]) {
@@ -186,69 +211,85 @@ __global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) xo[c] = s_u[buf][((ty + 1) * PW + (tx + 1)) * 3 + c];
   };
+  // Modulus of layer k as loaded; lanes off the grid's element columns and
+  // layers outside it get zero where it is used (use_modulus).
   auto modulus = [&](int k) -> T {
-    if (!exy || k < 0 || k >= g.ez) return T(0);
+    if (k < 0 || k >= g.ez) return T(0);
     return E[k * pe + eoff];
   };
-  // Owner-node operands of plane c, fetched one layer ahead.
-  int mk_n = 0;
-  T b_n[3] = {T(0), T(0), T(0)}, d_n[3] = {T(0), T(0), T(0)};
-  double r_n[3] = {0.0, 0.0, 0.0};
-  auto fetch_owner = [&](int c) {
-    if (!owner || c < 0 || c >= g.nnz) return;
+  // Owner-node operands of plane c (fetched one layer ahead).
+  struct Own {
+    int mk;
+    T b[3], d[3];
+    double r[3];
+  };
+  auto fetch_owner = [&](int c, Own& o) {
+    if (c < 0 || c >= g.nnz) return;
     const long long node = c * pn + noff;
-    mk_n = mask ? mask[node] : 7;
+    o.mk = mask ? mask[node] : 7;
     if constexpr (EPI != EPI_AX) {
 #pragma unroll
-      for (int q = 0; q < 3; ++q) b_n[q] = bs[node * 3 + q];
+      for (int q = 0; q < 3; ++q) o.b[q] = bs[node * 3 + q];
     }
     if constexpr (EPI == EPI_JAC) {
 #pragma unroll
-      for (int q = 0; q < 3; ++q) d_n[q] = ds[node * 3 + q];
+      for (int q = 0; q < 3; ++q) o.d[q] = ds[node * 3 + q];
     }
     if constexpr (DOTK == 2) {
 #pragma unroll
-      for (int q = 0; q < 3; ++q) r_n[q] = rd[node * 3 + q];
+      for (int q = 0; q < 3; ++q) o.r[q] = rd[node * 3 + q];
     }
   };
 
-  int cur = 0;
-  fetch_plane(k0 - 1);
-  store_plane(cur);
-  fetch_plane(k0);
-  T e_n = modulus(k0 - 1);
-  fetch_owner(k0);
-  T Fp[12], Vt[12], xo[3] = {T(0), T(0), T(0)};
+  T F0[12], F1[12], X0[3] = {T(0), T(0), T(0)}, X1[3], Vt[12];
+  Own O0{}, O1{};
+  T e0, e1;
 #pragma unroll
   for (int r = 0; r < 12; ++r) Vt[r] = T(0);
   double dotp = 0.0;
-  const int kend = k0 + a.tz;
-  for (int k = k0 - 1; k < kend; ++k) {
-    const int nxt = cur ^ 1;
-    store_plane(nxt);  // plane k + 1
-    if (k + 1 < kend) fetch_plane(k + 2);
-    const T e = e_n;
-    e_n = modulus(k + 1);
-    const int mk = mk_n;
-    T bo[3], dv[3];
-    double r_n_k[3];
+  int pending = -1;  // slab whose wave sums wait in s_red
+  double* const part = DOTK != 0 ? a.partials + static_cast<long long>(rc) * a.pstride : nullptr;
+  auto flush_pending = [&]() {
+    if (t == 0) {
+      double s = 0.0;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) { bo[q] = b_n[q]; dv[q] = d_n[q]; r_n_k[q] = r_n[q]; }
-    if (k >= k0) fetch_owner(k + 1);
+      for (int w = 0; w < NT / 64; ++w) s += s_red[w];
+      part[(static_cast<long long>(pending) * gy + by) * gx + bx] = s;
+    }
+    pending = -1;
+  };
+
+  fetch_plane(k0 - 1, pf);
+  store_plane(0, pf);
+  fetch_plane(k0, pf);
+  e0 = modulus(k0 - 1);
+
+  // One element layer k (between node planes k and k + 1): plane k + 1 goes to
+  // LDS buffer nxt, its face is transformed into Fn; layer k is applied and
+  // plane k's outputs (whose other half came from layer k - 1 through Vt) are
+  // finished. Fp / xo / e / oc hold plane k's face, own values, layer k's
+  // modulus and plane k's owner operands; the *n arguments receive plane k + 1's.
+  auto step = [&](int k, int cur, T (&Fp)[12], T (&Fn)[12], const T (&xo)[3], T (&xn)[3],
+                  const T& e, T& en, const Own& oc, Own& on) {
+    const int nxt = cur ^ 1;
+    store_plane(nxt, pf);  // plane k + 1
+    if (k + 1 < kend) fetch_plane(k + 2, pf);
+    en = modulus(k + 1);
+    fetch_owner(k + 1, on);
     __syncthreads();
+    if constexpr (DOTK != 0) {
+      if (pending >= 0) flush_pending();
+    }
     if (k == k0 - 1) load_face(cur, Fp);
-    T Fn[12], xn[3];
     load_face(nxt, Fn);
     load_own(nxt, xn);
-    // Element layer k between planes k (Fp) and k + 1 (Fn): z stage, modal
-    // core, inverse z stage split into the bottom and top faces.
     T A[24], v[24];
 #pragma unroll
     for (int q = 0; q < 12; ++q) {
       A[q] = Fp[q] + Fn[q];
       A[12 + q] = Fp[q] - Fn[q];
     }
-    modal_core(A, v, e, a.mg);
+    modal_core(A, v, exy ? e : T(0), a.mg);
     T G[12];
 #pragma unroll
     for (int q = 0; q < 12; ++q) {
@@ -268,61 +309,77 @@ __global__ void __launch_bounds__(TX * TY) ku_face(KuOp<T> a) {
       }
     }
     __syncthreads();
-    if (k >= k0 && owner && k < g.nnz) {
+    if (k >= k0 && owner) {
       // Node (na, nb) is corner 3 of column (tx, ty), 2 of (tx+1, ty),
       // 1 of (tx, ty+1) and 0 of (tx+1, ty+1). Fixed summation order.
       const int t00 = t, t10 = t + 1, t01 = t + TX, t11 = t + TX + 1;
       T* yp = ys + (k * pn + noff) * 3;
       T* xwp = nullptr;
       if constexpr (EPI == EPI_RES0) xwp = xw + (k * pn + noff) * 3;
+      T ax[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        ax[c] = ((s_c[3][t00 * 3 + c] + s_c[2][t10 * 3 + c]) + s_c[1][t01 * 3 + c]) +
+                s_c[0][t11 * 3 + c];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const T ax = ((s_c[3][t00 * 3 + c] + s_c[2][t10 * 3 + c]) + s_c[1][t01 * 3 + c]) +
-                     s_c[0][t11 * 3 + c];
-        const bool kept = (mk >> c) & 1;
-        T out = T(0);
-        if (kept) {
-          if constexpr (EPI == EPI_AX) {
-            out = ax;
-          } else if constexpr (EPI == EPI_RES || EPI == EPI_RES0) {
-            out = bo[c] - ax;
-          } else {
-            out = fma(a.omega * dv[c], bo[c] - ax, xo[c]);
-          }
-          if constexpr (DOTK == 1) dotp = fma(static_cast<double>(xo[c]), static_cast<double>(ax), dotp);
-          if constexpr (DOTK == 2) dotp = fma(r_n_k[c], static_cast<double>(out), dotp);
+        const bool kept = (oc.mk >> c) & 1;
+        T out;
+        if constexpr (EPI == EPI_AX) {
+          out = ax[c];
+        } else if constexpr (EPI == EPI_RES || EPI == EPI_RES0) {
+          out = oc.b[c] - ax[c];
+        } else {
+          out = fma(a.omega * oc.d[c], oc.b[c] - ax[c], xo[c]);
         }
+        out = kept ? out : T(0);
+        if constexpr (DOTK == 1)
+          dotp = fma(static_cast<double>(xo[c]), kept ? static_cast<double>(ax[c]) : 0.0, dotp);
+        if constexpr (DOTK == 2) dotp = fma(oc.r[c], static_cast<double>(out), dotp);
         yp[c] = out;
         if constexpr (EPI == EPI_RES0) xwp[c] = xo[c];
       }
     }
+    if constexpr (DOTK != 0) {
+      // end of a slab (or of the grid): wave sums to LDS, written out by
+      // thread 0 after the next barrier
+      if (k >= k0 && ((k + 1) % kSlab == 0 || k + 1 == g.nnz)) {
+        const double w = wave_sum(dotp);
+        if ((t & 63) == 0) s_red[t >> 6] = w;
+        dotp = 0.0;
+        pending = k / kSlab;
+      }
+    }
+  };
+
+  // (Unrolling this loop by two, so the hand-over below became a renaming,
+  // measured slower: the doubled body holds more registers, costing a wave
+  // per SIMD in FP64.)
+  for (int k = k0 - 1, cur = 0; k < kend; ++k, cur ^= 1) {
+    step(k, cur, F0, F1, X0, X1, e0, e1, O0, O1);
 #pragma unroll
-    for (int q = 0; q < 12; ++q) Fp[q] = Fn[q];
+    for (int q = 0; q < 12; ++q) F0[q] = F1[q];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) xo[c] = xn[c];
-    cur = nxt;
+    for (int q = 0; q < 3; ++q) X0[q] = X1[q];
+    e0 = e1;
+    O0 = O1;
   }
   if constexpr (DOTK != 0) {
-    dotp = wave_sum(dotp);
-    if ((t & 63) == 0) s_red[t >> 6] = dotp;
     __syncthreads();
-    if (t == 0) {
-      double s = 0.0;
-      for (int w = 0; w < NT / 64; ++w) s += s_red[w];
-      const int blin = (chunk * gy + by) * gx + bx;
-      a.partials[static_cast<long long>(rc) * a.pstride + blin] = s;
-    }
+    if (pending >= 0) flush_pending();
   }
 }
 
-template <typename T, int TX, int TY>
-void dispatch_tile(const KuOp<T>& op, hipStream_t s) {
-  const long long tiles = static_cast<long long>((op.g.nnx + TX - 2) / (TX - 1)) *
-                          ((op.g.nny + TY - 2) / (TY - 1)) * ((op.g.nnz + op.tz - 1) / op.tz) *
+template <typename T>
+void dispatch(const KuOp<T>& op, hipStream_t s) {
+  if (op.tz < kSlab || op.tz % kSlab != 0)
+    throw HipError("K.u z-chunk must be a positive multiple of " + std::to_string(kSlab));
+  const long long tiles = static_cast<long long>((op.g.nnx + kTX - 2) / (kTX - 1)) *
+                          ((op.g.nny + kTY - 2) / (kTY - 1)) * ((op.g.nnz + op.tz - 1) / op.tz) *
                           op.nrhs;
   const dim3 grid(xcd_grid(tiles));
   const bool dot = op.partials != nullptr;
-#define TOPOHIP_L(E, D) hipLaunchKernelGGL((ku_face<T, TX, TY, E, D>), grid, dim3(TX * TY), 0, s, op)
+#define TOPOHIP_L(E, D) hipLaunchKernelGGL((ku_face<T, E, D>), grid, dim3(kTX * kTY), 0, s, op)
   if (op.epi == EPI_AX) {
     if (dot) TOPOHIP_L(EPI_AX, 1); else TOPOHIP_L(EPI_AX, 0);
   } else if (op.epi == EPI_RES) {
@@ -341,23 +398,14 @@ void dispatch_tile(const KuOp<T>& op, hipStream_t s) {
 
 }  // namespace
 
-bool ku_tile_supported(int tx, int ty) {
-  return (tx == 16 && ty == 16) || (tx == 32 && ty == 8) || (tx == 64 && ty == 4) ||
-         (tx == 32 && ty == 16) || (tx == 64 && ty == 8);
-}
-
-int ku_blocks(const Geom& g, int tz, int tx, int ty) {
-  if (!ku_tile_supported(tx, ty)) { tx = 16; ty = 16; }
-  return ((g.nnx + tx - 2) / (tx - 1)) * ((g.nny + ty - 2) / (ty - 1)) * ((g.nnz + tz - 1) / tz);
+int ku_partials(const Geom& g) {
+  return ((g.nnx + kTX - 2) / (kTX - 1)) * ((g.nny + kTY - 2) / (kTY - 1)) *
+         ((g.nnz + kSlab - 1) / kSlab);
 }
 
 template <typename T>
 void launch_ku_fine(const KuOp<T>& op, hipStream_t s) {
-  if (op.tx == 32 && op.ty == 8) dispatch_tile<T, 32, 8>(op, s);
-  else if (op.tx == 64 && op.ty == 4) dispatch_tile<T, 64, 4>(op, s);
-  else if (op.tx == 32 && op.ty == 16) dispatch_tile<T, 32, 16>(op, s);
-  else if (op.tx == 64 && op.ty == 8) dispatch_tile<T, 64, 8>(op, s);
-  else dispatch_tile<T, 16, 16>(op, s);
+  dispatch<T>(op, s);
 }
 
 template void launch_ku_fine<float>(const KuOp<float>&, hipStream_t);
