@@ -191,6 +191,8 @@ typedef struct {
   double compliance, initial_compliance, volume_fraction;
   double t_total_ms, t_solve_ms, t_build_ms, t_filter_ms, t_sens_ms, t_oc_ms;
   long long cg_iterations_total;
+  int cancelled;          /* stopped by the cancel flag (topohip_set_cancel_flag) */
+  int deadline_exceeded;  /* a solve passed the thread's solve deadline */
 } topohip_result;
 
 /* history (nullable) receives 4 doubles per iteration: compliance, change,
@@ -499,6 +501,28 @@ int topohip_write_fields_file(const char* path, const topohip_grid_frame* grid, 
 int topohip_format_progress(size_t rung, size_t rungs, int iteration, char* buf, size_t n);
 int topohip_format_variant(double vf, double achieved, double printed, double margin,
                            const char* mesh, char* buf, size_t n);
+
+/* ---- solve deadline and cancellation -----------------------------------------
+ * fea_set_solve_deadline_ms (core/include/topopt/fea.hpp:440-469,
+ * core/src/fea/matfree.cpp:1267-1273): an absolute steady-clock time in ms
+ * (topohip_steady_ms' clock) for every solve of the CALLING THREAD; <= 0
+ * disarms it (the default). Thread-local and sticky; returns the previous
+ * value. Armed, it is polled before each batch of PCG iterations: a solve
+ * still running when it passes stops with TOPOHIP_EDEADLINE (the reference's
+ * SolverDeadlineExceeded, a SolverNonConvergence) and the iterations and
+ * residual it had reached in the info struct; a design loop ends as
+ * non-convergent with deadline_exceeded set. A run that does not reach it is
+ * bit-identical to a disarmed one. */
+double topohip_set_solve_deadline_ms(double absolute_steady_ms);
+double topohip_steady_ms(void);
+/* SimpOptions::cancel (core/include/topopt/simp.hpp:1505-1521): a caller-owned
+ * flag (NULL clears) polled once per design iteration, at its start, by
+ * topohip_optimize, topohip_optimize_masked, topohip_optimize_mma and
+ * topohip_optimize_projected on this handle. When nonzero the loop stops:
+ * result.cancelled = 1, converged = 0, and the outputs are still consistent
+ * (one final solve on the partially optimised density). The flag must outlive
+ * the call; another thread may set it at any time. */
+int topohip_set_cancel_flag(topohip_problem* p, const int* flag);
 
 /* ---- misc ------------------------------------------------------------------ */
 int topohip_set_device(int device);

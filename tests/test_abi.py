@@ -63,7 +63,8 @@ int main(void) {
   F(topohip_options, max_iterations) F(topohip_options, cg_max_iterations)
   F(topohip_options, pc_p0) F(topohip_options, pc_per_stage)
   S(topohip_result) F(topohip_result, compliance) F(topohip_result, t_oc_ms)
-  F(topohip_result, cg_iterations_total)
+  F(topohip_result, cg_iterations_total) F(topohip_result, cancelled)
+  F(topohip_result, deadline_exceeded)
   S(topohip_job) F(topohip_job, volfrac) F(topohip_job, iterations) F(topohip_job, precision)
   S(topohip_job_result) F(topohip_job_result, compliance) F(topohip_job_result, t_end_ms)
   F(topohip_job_result, status)
@@ -88,7 +89,7 @@ def test_struct_layout_matches_header(tmp_path):
         got = C.sizeof(cls) if field == "sizeof" else getattr(cls, field).offset
         assert got == int(value), (tname, field, got, value)
         n += 1
-    assert n == 27
+    assert n == 29
 
 
 def test_version_string():

@@ -42,6 +42,10 @@ class SolverNonConvergence(RuntimeError):
         self.residual = residual
 
 
+class SolverDeadlineExceeded(SolverNonConvergence):
+    """An armed solve deadline passed before CG reached its tolerance."""
+
+
 class DeviceError(RuntimeError):
     pass
 
@@ -93,6 +97,7 @@ class Result(C.Structure):
         ("t_total_ms", C.c_double), ("t_solve_ms", C.c_double), ("t_build_ms", C.c_double),
         ("t_filter_ms", C.c_double), ("t_sens_ms", C.c_double), ("t_oc_ms", C.c_double),
         ("cg_iterations_total", C.c_longlong),
+        ("cancelled", C.c_int), ("deadline_exceeded", C.c_int),
     ]
 
     def as_dict(self):
@@ -154,6 +159,10 @@ def _check(code, info=None):
         it = info.iterations if info is not None else 0
         res = info.residual if info is not None else 0.0
         raise SolverNonConvergence(msg, it, res)
+    if code == EDEADLINE:
+        it = info.iterations if info is not None else 0
+        res = info.residual if info is not None else 0.0
+        raise SolverDeadlineExceeded(msg, it, res)
     if code == EDEVICE:
         raise DeviceError(msg)
     raise RuntimeError(f"topohip status {code}: {msg}")
@@ -304,6 +313,12 @@ class Problem:
                                                     str(basename).encode(), _p(o), _p(d),
                                                     C.c_size_t(rung), int(iteration),
                                                     int(bool(boundary))))
+
+    def set_cancel_flag(self, flag):
+        """flag: a ctypes.c_int the caller keeps (nonzero cancels the design
+        loop at the start of its next iteration), or None to clear."""
+        self._cancel = flag
+        _check(lib().topohip_set_cancel_flag(self._h, None if flag is None else C.byref(flag)))
 
     def set_observer(self, fn):
         """fn(IterationObs) after every design iteration (None clears)."""
@@ -559,6 +574,20 @@ def fea_solve_mgcg_matfree(nx, ny, nz, youngs_per_voxel, poisson, bcs, loads, to
     with Problem(nx, ny, nz, bcs, loads, h=h, poisson=poisson, solid=solid,
                  precision=precision) as p:
         return p.solve(youngs_per_voxel, tolerance, max_iterations, initial_guess)
+
+
+def set_solve_deadline_ms(absolute_steady_ms):
+    """fea_set_solve_deadline_ms for the calling thread (<= 0 disarms);
+    returns the previous value. Times are on steady_ms()'s clock."""
+    f = lib().topohip_set_solve_deadline_ms
+    f.restype = C.c_double
+    return f(C.c_double(absolute_steady_ms))
+
+
+def steady_ms():
+    f = lib().topohip_steady_ms
+    f.restype = C.c_double
+    return f()
 
 
 def version():

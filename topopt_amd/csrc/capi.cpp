@@ -327,6 +327,19 @@ int topohip_oc_margin(topohip_problem* p, double* last_gap, double* run_min_gap,
   });
 }
 
+double topohip_set_solve_deadline_ms(double absolute_steady_ms) {
+  return set_solve_deadline_ms(absolute_steady_ms);
+}
+
+double topohip_steady_ms(void) { return steady_ms(); }
+
+int topohip_set_cancel_flag(topohip_problem* p, const int* flag) {
+  return guarded([&] {
+    need(p, "problem");
+    p->P->cancel = flag;
+  });
+}
+
 int topohip_optimize(topohip_problem* p, const topohip_options* opt, double* design_out,
                      double* xphys_out, double* history, topohip_result* result) {
   return guarded([&] {

@@ -38,6 +38,15 @@ double now_ms() {
 
 void sync(Problem& P) { TOPOHIP_CHECK(hipStreamSynchronize(P.s)); }
 
+// The solve deadline of the calling thread (absolute steady-clock ms; 0 =
+// disarmed), as fea_set_solve_deadline_ms (core/include/topopt/fea.hpp:
+// 440-455, core/src/fea/matfree.cpp:1267-1273): thread-local and sticky.
+thread_local double t_solve_deadline_ms = 0.0;
+
+bool deadline_passed() {
+  return t_solve_deadline_ms > 0.0 && now_ms() >= t_solve_deadline_ms;
+}
+
 template <typename T>
 void h2d(T* d, const T* h, size_t n, hipStream_t s) {
   TOPOHIP_CHECK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s));
@@ -667,6 +676,18 @@ void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool
   int ahead = std::min(8, std::max(1, (P.pcg_hint[gi] - 2 * batch) / batch));
   for (long long guard = 0;; ++guard) {
     if (guard > 4LL * cap + 64) throw Error(TOPOHIP_EINTERNAL, "pcg: iteration state stalled");
+    // An armed solve deadline is polled before every graph batch (the
+    // reference polls every 256 iterations, matfree.cpp:988-1001); when it has
+    // passed the unfinished cases stop with the state they reached.
+    if (deadline_passed()) {
+      for (int r = 0; r < nc; ++r) {
+        if (ok[r] || (skip && skip[r])) continue;
+        iters[r] = P.st_h[r].iters;
+        resid[r] = P.st_h[r].bnorm2 > 0.0 ? std::sqrt(P.st_h[r].rr / P.st_h[r].bnorm2) : 0.0;
+      }
+      P.deadline_hit = true;
+      return;
+    }
     if (!all_done()) {
       long long before = 0;
       for (int r = 0; r < nc; ++r) before += P.st_h[r].iters;
@@ -737,6 +758,14 @@ void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool
 }
 
 }  // namespace
+
+double set_solve_deadline_ms(double absolute_steady_ms) {
+  const double prev = t_solve_deadline_ms;
+  t_solve_deadline_ms = absolute_steady_ms > 0.0 ? absolute_steady_ms : 0.0;
+  return prev;
+}
+
+double steady_ms() { return now_ms(); }
 
 // Launch-shape autotuner: time the fine-level apply for each z-chunk length
 // on the problem's own buffers and keep the fastest (override:
@@ -1129,7 +1158,25 @@ void solve_cases(Problem& P, int nc, double tol, int max_it, const double* d_gue
   std::vector<topohip_cginfo> d(static_cast<size_t>(nc));
   for (auto& di : d) di.converged = 1;
   P.replacements = 0;
+  P.deadline_hit = false;
   const long long mv0 = P.matvecs;
+  // SolverDeadlineExceeded: the solve was abandoned at the deadline with the
+  // iterations and residual it had reached (core/include/topopt/fea.hpp:455-469).
+  auto check_deadline = [&](const std::vector<int>& it, const std::vector<double>& res,
+                            const bool* ok) {
+    if (!P.deadline_hit) return;
+    for (int r = 0; r < nc; ++r) {
+      if (ok[r]) continue;
+      d[r].converged = 0;
+      d[r].iterations = it[r];
+      d[r].residual = res[r];
+      d[r].matvecs = P.matvecs - mv0;
+      d[r].t_total_ms = now_ms() - t0;
+    }
+    if (info) std::copy(d.begin(), d.end(), info);
+    throw Error(TOPOHIP_EDEADLINE, "fea_solve_mgcg_matfree: the solve deadline passed before CG "
+                                   "reached the requested tolerance");
+  };
   if (ng > 0) {
     bool warm = false;
     std::vector<double> gh;
@@ -1155,6 +1202,7 @@ void solve_cases(Problem& P, int nc, double tol, int max_it, const double* d_gue
       const Prec pc = P.precision == TOPOHIP_PREC_MIXED ? Prec::MG32 : Prec::MG64;
       pcg(P, pc, nc, tol, std::min(cap, kMgIterBudget), warm, nullptr, it.data(), res.data(),
           ok.get());
+      check_deadline(it, res, ok.get());
       const double tmg = now_ms() - tm;
       for (int r = 0; r < nc; ++r) {
         d[r].hier_built = 1;
@@ -1185,6 +1233,7 @@ void solve_cases(Problem& P, int nc, double tol, int max_it, const double* d_gue
       std::vector<int> it64(static_cast<size_t>(nc), 0);
       pcg(P, Prec::MG64, nc, tol, std::min(cap, kMgIterBudget), warm, skip.get(), it64.data(),
           res.data(), ok.get());
+      check_deadline(it64, res, ok.get());
       const double tmg = now_ms() - tm;
       all_ok = true;
       for (int r = 0; r < nc; ++r) {
@@ -1214,6 +1263,7 @@ void solve_cases(Problem& P, int nc, double tol, int max_it, const double* d_gue
         else launch_fill<double>(P.x.p + r * nd, 0.0, nd, P.s);
       }
       pcg(P, Prec::JACOBI, nc, tol, cap, warm, skip.get(), it.data(), res.data(), ok.get());
+      check_deadline(it, res, ok.get());
       const double tcg = now_ms() - tc;
       for (int r = 0; r < nc; ++r) {
         if (skip[r]) continue;
@@ -1691,6 +1741,12 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
   for (const Stage& st : plan) {
     bool stage_converged = false;
     for (int si = 0; si < st.iterations; ++si, ++it) {
+      // SimpOptions::cancel, polled once per design iteration at its start
+      // (core/include/topopt/simp.hpp:1505-1521, core/src/simp/simp.cpp:2159-2163)
+      if (P.cancel && __atomic_load_n(P.cancel, __ATOMIC_ACQUIRE) != 0) {
+        r.cancelled = 1;
+        break;
+      }
       const double t_top = now_ms();
       double ta = t_top;
       physical(st);
@@ -1705,8 +1761,10 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
         c = simp_compliance_dev(P, P.xphys.p, o.E0, penalty_at(r.iterations + 1), o.rho_min,
                                 o.cg_tol, o.cg_max_iterations, &ci);
       } catch (const Error& e) {
-        if (e.code != TOPOHIP_ENONCONVERGENCE) throw;
+        // SolverDeadlineExceeded is-a SolverNonConvergence: either ends the run
+        if (e.code != TOPOHIP_ENONCONVERGENCE && e.code != TOPOHIP_EDEADLINE) throw;
         r.non_convergent = 1;
+        if (e.code == TOPOHIP_EDEADLINE) r.deadline_exceeded = 1;
         break;
       }
       const double ms_solve = now_ms() - ta;
@@ -1798,8 +1856,8 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
         break;
       }
     }
-    r.converged = stage_converged ? 1 : 0;
-    if (r.non_convergent) break;
+    r.converged = stage_converged && !r.cancelled ? 1 : 0;
+    if (r.non_convergent || r.cancelled) break;
   }
   physical(plan.back());
   // volume fraction of the unpinned physical density (active_volfrac)
@@ -1818,8 +1876,9 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
                                          o.cg_max_iterations, &ci);
       t_solve += now_ms() - ta;
     } catch (const Error& e) {
-      if (e.code != TOPOHIP_ENONCONVERGENCE) throw;
+      if (e.code != TOPOHIP_ENONCONVERGENCE && e.code != TOPOHIP_EDEADLINE) throw;
       r.non_convergent = 1;
+      if (e.code == TOPOHIP_EDEADLINE) r.deadline_exceeded = 1;
       r.converged = 0;
     }
   }

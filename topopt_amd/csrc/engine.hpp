@@ -131,6 +131,8 @@ struct Problem {
   int pstride = 0;   // partial sums per case
   long long matvecs = 0;
   int replacements = 0;
+  bool deadline_hit = false;      // the last solve stopped at the thread's solve deadline
+  const int* cancel = nullptr;    // caller-owned cancel flag of the design loops (nullable)
   hipGraphExec_t cg_graph[3] = {nullptr, nullptr, nullptr};  // per preconditioner kind
   int cg_graph_nc[3] = {0, 0, 0};                             // case count captured
   int pcg_hint[3] = {0, 0, 0};  // iterations of the last solve per preconditioner kind
@@ -201,6 +203,9 @@ std::unique_ptr<FilterCache> make_filter(Problem& P, double radius, const uint8_
 void design_mask_of(const Problem& P, const uint8_t* mask, std::vector<uint8_t>& eff,
                     std::vector<uint8_t>& dset, long long& n_design);
 void hex8_stiffness_host(double E, double nu, double h, double* K576);
+// fea_set_solve_deadline_ms for the calling thread; returns the previous value.
+double set_solve_deadline_ms(double absolute_steady_ms);
+double steady_ms();
 // fea_von_mises_field of u_host (3 * nodes; null = P.u of the last solve) for
 // Young's modulus E: FP64 field into vm64 and/or its float narrowing into
 // vm32, and the displacement narrowed to float into disp32 (outputs nullable).
