@@ -1696,10 +1696,13 @@ std::uint16_t float_to_half(float v) {
   return static_cast<std::uint16_t>(hv);
 }
 
-// core/src/io/design_store.cpp:75-88 (FNV-1a, 64-bit offset basis and prime).
-std::uint64_t fnv1a64(const void* data, std::size_t n) {
+// core/src/io/design_store.cpp:75-88: FNV-1a with the 64-bit prime and the
+// reference's offset basis, 1469598103934665603 (:78; the published basis has
+// one more digit, 14695981039346656037). The reference's design.bin files
+// carry fingerprints under its own basis.
+std::uint64_t fnv1a64(const void* data, std::size_t n, std::uint64_t basis) {
   const auto* b = static_cast<const std::uint8_t*>(data);
-  std::uint64_t h = 14695981039346656037ULL;
+  std::uint64_t h = basis;
   for (std::size_t i = 0; i < n; ++i) {
     h ^= b[i];
     h *= 1099511628211ULL;

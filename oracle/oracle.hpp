@@ -30,7 +30,10 @@
 //   float_to_half       numpy's IEEE float16 conversion (every finite input),
 //                       the reference's anchors and round-trip bound
 //                       (core/tests/unit/test_observability.cpp:55-74)
-//   fnv1a64             the published FNV-1a 64-bit test vectors
+//   fnv1a64             the fingerprints stored in design.bin files the reference
+//                       wrote (tests/golden/design_bin_reference_4x4x4.bin), and
+//                       the published FNV-1a 64-bit vectors once the reference's
+//                       offset basis is swapped for the published one
 //   mma_update          (tests/test_oracle_mma.py) convergence to the closed-form
 //                       optimum of a separable problem (sum c/x at fixed volume),
 //                       bounds and move box on every step; the loop's descent
@@ -360,7 +363,8 @@ std::vector<double> von_mises_field(const Grid& g, double E, double nu,
 // core/src/simp/observability.cpp:35-75: float -> IEEE half, round to nearest even.
 std::uint16_t float_to_half(float v);
 // core/src/simp/design_store.cpp (design_fingerprint): 64-bit FNV-1a over bytes.
-std::uint64_t fnv1a64(const void* data, std::size_t n);
+std::uint64_t fnv1a64(const void* data, std::size_t n,
+                      std::uint64_t basis = 1469598103934665603ULL);
 
 // MMA (Svanberg 1987) with one volume constraint: mma_update,
 // core/src/simp/simp.cpp:1433-1488 over mma_update_joint,

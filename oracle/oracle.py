@@ -58,6 +58,8 @@ def lib():
         _lib.oracle_last_error.restype = C.c_char_p
         _lib.oracle_fnv1a64.restype = C.c_uint64
         _lib.oracle_fnv1a64.argtypes = [C.c_void_p, C.c_size_t]
+        _lib.oracle_fnv1a64_basis.restype = C.c_uint64
+        _lib.oracle_fnv1a64_basis.argtypes = [C.c_void_p, C.c_size_t, C.c_uint64]
     return _lib
 
 
@@ -332,10 +334,14 @@ def float_to_half(x):
     return out
 
 
-def fnv1a64(data):
+def fnv1a64(data, basis=None):
+    """design_fingerprint's FNV-1a 64 (the reference's offset basis unless
+    another is given)."""
     b = bytes(data)
     buf = C.create_string_buffer(b, len(b))
-    return int(lib().oracle_fnv1a64(buf, C.c_size_t(len(b))))
+    if basis is None:
+        return int(lib().oracle_fnv1a64(buf, C.c_size_t(len(b))))
+    return int(lib().oracle_fnv1a64_basis(buf, C.c_size_t(len(b)), C.c_uint64(basis)))
 
 
 def optimize_mma(s, volfrac, radius, iterations, E0=1.0, nu=0.3, penalty=3.0, rho_min=1e-3,

@@ -516,6 +516,9 @@ int oracle_float_to_half(const float* in, std::size_t n, std::uint16_t* out) {
 }
 
 std::uint64_t oracle_fnv1a64(const void* data, std::size_t n) { return oracle::fnv1a64(data, n); }
+std::uint64_t oracle_fnv1a64_basis(const void* data, std::size_t n, std::uint64_t basis) {
+  return oracle::fnv1a64(data, n, basis);
+}
 
 int oracle_heaviside(double rt, double beta, double eta, double* value, double* deriv) {
   return guarded([&] {

@@ -4,9 +4,15 @@ hex8_stress: the reference's closed forms (core/tests/unit/test_fea.cpp:303-390)
 von_mises_field: uniform-strain and rigid-motion fields on a small grid with
 void voxels. float_to_half: numpy's IEEE float16 conversion for every finite
 float class, plus the reference's anchors and round-trip bound
-(core/tests/unit/test_observability.cpp:55-74). fnv1a64: the published FNV-1a
-64-bit test vectors.
+(core/tests/unit/test_observability.cpp:55-74). fnv1a64: the fingerprints a
+design.bin written by the reference stores (tests/golden/design_bin_reference_
+4x4x4.bin, from tools/topopt-worker/e2e/.../out/design.bin), and the published
+FNV-1a 64-bit test vectors under the published offset basis (the reference's
+basis, 1469598103934665603, is one digit shorter; core/src/io/design_store.cpp:78).
 """
+import os
+import struct
+
 import numpy as np
 import pytest
 
@@ -107,6 +113,24 @@ def test_float_to_half_reference_anchors():
 
 
 def test_fnv1a64_published_vectors():
-    assert oracle.fnv1a64(b"") == 0xCBF29CE484222325
-    assert oracle.fnv1a64(b"a") == 0xAF63DC4C8601EC8C
-    assert oracle.fnv1a64(b"foobar") == 0x85944171F73967E8
+    basis = 0xCBF29CE484222325
+    assert oracle.fnv1a64(b"", basis) == 0xCBF29CE484222325
+    assert oracle.fnv1a64(b"a", basis) == 0xAF63DC4C8601EC8C
+    assert oracle.fnv1a64(b"foobar", basis) == 0x85944171F73967E8
+    assert oracle.fnv1a64(b"") == 1469598103934665603
+
+
+def test_fnv1a64_matches_reference_written_design_bin():
+    b = open(os.path.join(os.path.dirname(__file__), "golden",
+                          "design_bin_reference_4x4x4.bin"), "rb").read()
+    nx, ny, nz = struct.unpack_from("<3i", b, 4)
+    count = struct.unpack_from("<i", b, 48)[0]
+    at = 56
+    for _ in range(count):
+        fp, n = struct.unpack_from("<Qq", b, at + 80)
+        assert n == nx * ny * nz
+        dens = b[at + 96:at + 96 + 8 * n]
+        assert oracle.fnv1a64(dens) == fp
+        assert oracle.fnv1a64(dens, 0xCBF29CE484222325) != fp
+        at += 96 + 8 * n
+    assert at == len(b) and count == 2

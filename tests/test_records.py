@@ -220,3 +220,62 @@ def test_library_plateau_stop_matches_reference_curves():
                                  spec["flat_windows"])
         assert got and not oracle.mma_plateau(cv[:fire - 1], spec["window"], 1e-3,
                                                spec["min_drop"], spec["flat_windows"])
+
+
+REF_FIELDS = ("/root/reference/docs/handoffs/evidence/2026-07-26-constrained-smooth-ui/smooth/"
+              "s000/fields.bin")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_FIELDS), reason="reference checkout not mounted")
+def test_reference_written_fields_bin_round_trips_byte_identically(tmp_path):
+    """A fields.bin the reference itself wrote (v1, 48x32x48, one variant,
+    docs/handoffs/evidence/2026-07-26-constrained-smooth-ui/smooth/s000/):
+    parsed with the independent reader above and re-emitted by this library's
+    writer, the bytes are identical."""
+    b = open(REF_FIELDS, "rb").read()
+    r = LE(b)
+    assert r.get("B") == 1
+    r.at += 3
+    nx, ny, nz = r.get("3i")
+    origin = r.get("3d")
+    spacing = r.get("d")
+    assert r.get("d") == spacing ** 3
+    nvar = r.get("i")
+    r.at += 4
+    assert (nx, ny, nz) == (48, 32, 48) and nvar == 1
+    nvox, nnode = nx * ny * nz, (nx + 1) * (ny + 1) * (nz + 1)
+    vs = []
+    for _ in range(nvar):
+        vf, mass = r.get("d"), r.get("d")
+        support = r.get("i")
+        r.at += 4
+        assert r.get("3q") == (nvox, 0, 3 * nnode)
+        vm = np.frombuffer(b, "<f4", nvox, r.at).copy()
+        r.at += 4 * nvox
+        du = np.frombuffer(b, "<f4", 3 * nnode, r.at).copy()
+        r.at += 12 * nnode
+        vs.append({"requested_volume_fraction": vf, "mass_grams": mass,
+                   "support_volume_voxels": support, "von_mises": vm, "displacement": du})
+    assert r.at == len(b)
+    grid = ta.GridFrame.make(nx, ny, nz, spacing, origin)
+    out = tmp_path / "fields.bin"
+    ta.write_fields_file(out, grid, vs)
+    assert out.read_bytes() == b
+
+
+def test_reference_written_design_bin_reads_and_round_trips(tmp_path):
+    """A design.bin the reference wrote (tools/topopt-worker/e2e/.../out/
+    design.bin, 4x4x4, two variants; committed as tests/golden/
+    design_bin_reference_4x4x4.bin): this library's reader accepts it (its
+    fingerprints are the reference's FNV-1a with the reference's offset basis)
+    and its writer re-emits the same bytes."""
+    src = os.path.join(os.path.dirname(__file__), "golden", "design_bin_reference_4x4x4.bin")
+    back = ta.read_design_file(src)
+    g = back["grid"]
+    assert (g.nx, g.ny, g.nz) == (4, 4, 4) and len(back["variants"]) == 2
+    for v in back["variants"]:
+        assert v["fingerprint"] == ta.design_fingerprint(v["density"])
+        assert v["iterations"] == 30 and v["accepted"] == 1
+    out = tmp_path / "design.bin"
+    ta.write_design_file(out, g, back["variants"])
+    assert out.read_bytes() == open(src, "rb").read()

@@ -286,9 +286,12 @@ std::vector<float> read_density_f16(const std::string& path, std::size_t count) 
   return out;
 }
 
-// core/src/io/design_store.cpp:75-88: FNV-1a 64 over the doubles' bytes.
+// core/src/io/design_store.cpp:75-88: FNV-1a 64 over the doubles' bytes,
+// started from the reference's offset basis 1469598103934665603 (one digit
+// shorter than the published FNV-1a basis): the fingerprints in the design.bin
+// files the reference writes are computed from it, so this one is too.
 std::uint64_t design_fingerprint(const double* d, std::size_t n) {
-  std::uint64_t h = 14695981039346656037ULL;
+  std::uint64_t h = 1469598103934665603ULL;
   const auto* b = reinterpret_cast<const std::uint8_t*>(d);
   for (std::size_t i = 0; i < n * sizeof(double); ++i) {
     h ^= b[i];
