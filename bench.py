@@ -11,6 +11,8 @@ the timed region and live in HBM throughout.
 Also reported on the same JSON line:
   fp64_form    the same run in the plain FP64 GPU form (every apply and the
                whole V-cycle in FP64)
+  chebyshev_smoother  the same run with the Chebyshev(2) smoother option
+               instead of the reference's damped Jacobi
   vs_oracle    per-result errors of the first design iteration against the
                FP64 oracle (compliance, ||u||, PCG count, the OC step's new
                density and active set) and the OC step fed with the oracle's
@@ -88,11 +90,14 @@ def measured_traffic(workload, variant, tile):
     return None
 
 
-def run_design(ta, wl, precision, steps, warmup, mma=False):
+def run_design(ta, wl, precision, steps, warmup, mma=False, smoother=None):
     """One timed optimisation run (OC, or MMA with the plateau stop off so the
-    run has exactly `steps` iterations)."""
+    run has exactly `steps` iterations); smoother = (kind, degree) or None for
+    the reference's damped Jacobi."""
     nx, ny, nz = wl["grid"]
     with ta.Problem(nx, ny, nz, wl["bcs"], wl["loads"], precision=precision) as p:
+        if smoother:
+            p.set_smoother(*smoother)
         kw = dict(penalty=wl["penalty"], move=wl["move"], cg_tol=wl["tol"],
                   continuation=wl.get("continuation"))
         run = (lambda n: p.optimize_mma(wl["volfrac"], wl["rmin"], n, plateau_window=0, **kw)) \
@@ -313,6 +318,8 @@ def main():
     ap.add_argument("--workload", default="mbb_192x64x64")
     ap.add_argument("--no-fp64-form", action="store_true")
     ap.add_argument("--no-mma", action="store_true", help="skip the MMA-updater leg")
+    ap.add_argument("--no-chebyshev", action="store_true",
+                    help="skip the Chebyshev-smoother leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the 512-job serving leg")
     ap.add_argument("--no-large", action="store_true", help="skip the 512x256x256 leg")
@@ -382,6 +389,17 @@ def main():
                "cg_iterations_total": outm["cg_iterations_total"], "t_oc_ms": outm["t_oc_ms"]}
         log(f"# mma updater: {mma['value']:.2f} design iters/s, compliance "
             f"{outm['compliance']:.6g}")
+    cheb = None
+    if not a.no_chebyshev:
+        outc, dtc = run_design(ta, wl, ta.PREC_MIXED, a.steps, min(a.warmup, 1),
+                               smoother=("chebyshev", 2))
+        cheb = {"degree": 2, "value": outc["iterations"] / dtc,
+                "compliance": outc["compliance"],
+                "compliance_rel_diff_vs_jacobi": abs(outc["compliance"] - out["compliance"]) /
+                abs(out["compliance"]),
+                "cg_iterations_total": outc["cg_iterations_total"]}
+        log(f"# chebyshev(2) smoother: {cheb['value']:.2f} design iters/s, "
+            f"cg iters {cheb['cg_iterations_total']} (Jacobi {out['cg_iterations_total']})")
     ku, tile = ku_measure(ta, nx, ny, nz)
     log("# K.u " + ", ".join(f"{k} {v['us']:.1f} us {v['GBs']:.0f} GB/s" for k, v in ku.items())
         + f" (tile {tile})")
@@ -430,6 +448,7 @@ def main():
         "fp64_form": fp64,
         "vs_oracle": par,
         "mma_updater": mma,
+        "chebyshev_smoother": cheb,
         "ku": ku,
         "result_fields": fields,
         "roofline": {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,

@@ -502,6 +502,20 @@ int topohip_format_progress(size_t rung, size_t rungs, int iteration, char* buf,
 int topohip_format_variant(double vf, double achieved, double printed, double margin,
                            const char* mesh, char* buf, size_t n);
 
+/* ---- V-cycle smoother ------------------------------------------------------------
+ * kind 0 (the default): damped Jacobi, omega 0.6, one pre- and one
+ * post-sweep per level, the reference's smoother (core/src/fea/multigrid.cpp:72,
+ * 1128-1160). kind 1: Chebyshev of `degree` (1..8) on the Jacobi-scaled
+ * operator of each smoothed level, an option of this engine with no reference
+ * counterpart: degree operator applies per smoothing, the interval
+ * [0.3, 1.1] * lambda_max(D^-1 A) with lambda_max from 10 power steps at every
+ * hierarchy build. Applies to later solves of this handle. */
+int topohip_set_smoother(topohip_problem* p, int kind, int degree);
+/* The Chebyshev smoother's lambda_max estimates of the last hierarchy build,
+ * one per smoothed level (level 0 = fine): up to n into lambdas, the count
+ * into *levels (0 with the Jacobi smoother). */
+int topohip_smoother_lambdas(topohip_problem* p, int n, double* lambdas, int* levels);
+
 /* ---- solve deadline and cancellation -----------------------------------------
  * fea_set_solve_deadline_ms (core/include/topopt/fea.hpp:440-469,
  * core/src/fea/matfree.cpp:1267-1273): an absolute steady-clock time in ms

@@ -859,6 +859,76 @@ Csr galerkin_a1(const Reduced& m, const std::vector<int>& active,
   return A;
 }
 
+int g_smoother = 0, g_cheb_degree = 1;
+
+// lambda_max(D^-1 A) by kChebPowerSteps power steps from cheb_start (the
+// level DOF of index i sits at node-major position gid[i]): y = A v, then
+// v = D^-1 y / ||D^-1 y|| for all but the last step, whose Rayleigh quotient
+// (v.Av) / (v.Dv) is the estimate.
+template <class Apply>
+double power_lambda(int n, const std::vector<long long>& gid, const std::vector<double>& dinv,
+                    Apply apply) {
+  std::vector<double> v(static_cast<std::size_t>(n)), y(static_cast<std::size_t>(n));
+  for (int i = 0; i < n; ++i) v[i] = cheb_start(gid[i]);
+  for (int k = 0;; ++k) {
+    apply(v.data(), y.data());
+    if (k + 1 == kChebPowerSteps) break;
+    for (int i = 0; i < n; ++i) v[i] = dinv[i] * y[i];
+    const double nv = norm2(v);
+    for (int i = 0; i < n; ++i) v[i] = v[i] / nv;
+  }
+  double vav = 0.0, vdv = 0.0;
+  for (int i = 0; i < n; ++i) {
+    vav += v[i] * y[i];
+    vdv += v[i] * v[i] / dinv[i];
+  }
+  return vav / vdv;
+}
+
+ChebLevel cheb_level(double lambda) {
+  ChebLevel c;
+  c.lambda = lambda;
+  const double hi = kChebHigh * lambda, lo = kChebLow * lambda;
+  c.theta = 0.5 * (hi + lo);
+  c.delta = 0.5 * (hi - lo);
+  return c;
+}
+
+// Chebyshev smoothing of A x = b on one level (degree operator applies when
+// starting from x, degree - 1 from zero): d = D^-1 r / theta, x += d, then
+// rho' = 1 / (2 sigma - rho), d = rho' rho d + (2 rho' / delta) D^-1 r,
+// x += d, with sigma = theta / delta, rho = 1 / sigma at the start.
+template <class Apply>
+void cheb_smooth(int n, const std::vector<double>& dinv, const ChebLevel& c, int degree,
+                 const std::vector<double>& b, std::vector<double>& x, bool zero_start,
+                 Apply apply) {
+  std::vector<double> r(static_cast<std::size_t>(n)), d(static_cast<std::size_t>(n));
+  if (zero_start) {
+    x.assign(static_cast<std::size_t>(n), 0.0);
+    r = b;
+  } else {
+    apply(x.data(), r.data());
+    for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
+  }
+  for (int i = 0; i < n; ++i) {
+    d[i] = dinv[i] * r[i] / c.theta;
+    x[i] += d[i];
+  }
+  const double sigma = c.theta / c.delta;
+  double rho = 1.0 / sigma;
+  for (int k = 1; k < degree; ++k) {
+    apply(x.data(), r.data());
+    for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
+    const double rho_new = 1.0 / (2.0 * sigma - rho);
+    const double c1 = rho_new * rho, c2 = 2.0 * rho_new / c.delta;
+    for (int i = 0; i < n; ++i) {
+      d[i] = c1 * d[i] + c2 * (dinv[i] * r[i]);
+      x[i] += d[i];
+    }
+    rho = rho_new;
+  }
+}
+
 void jacobi_sweep(const Level& L, const std::vector<double>& b, std::vector<double>& x) {
   std::vector<double> Ax(static_cast<std::size_t>(L.n));
   L.A.spmv(x.data(), Ax.data());
@@ -867,24 +937,31 @@ void jacobi_sweep(const Level& L, const std::vector<double>& b, std::vector<doub
 
 // v_cycle: core/src/fea/multigrid.cpp:798-819 (gamma 1, 1+1 sweeps).
 std::vector<double> v_cycle(const std::vector<Level>& levels, int l,
-                            const std::vector<double>& b) {
+                            const std::vector<double>& b, const std::vector<ChebLevel>& cheb) {
   const Level& L = levels[static_cast<std::size_t>(l)];
   std::vector<double> x(static_cast<std::size_t>(L.n), 0.0);
   if (L.coarsest) {
     skyline_solve(L.chol, b.data(), x.data());
     return x;
   }
-  jacobi_sweep(L, b, x);
+  auto spmv = [&](const double* in, double* out) { L.A.spmv(in, out); };
+  if (g_smoother == 1)
+    cheb_smooth(L.n, L.Dinv, cheb[static_cast<std::size_t>(l) + 1], g_cheb_degree, b, x, true, spmv);
+  else
+    jacobi_sweep(L, b, x);
   std::vector<double> r(static_cast<std::size_t>(L.n));
   L.A.spmv(x.data(), r.data());
   for (int i = 0; i < L.n; ++i) r[i] = b[i] - r[i];
   std::vector<double> bc(static_cast<std::size_t>(L.P.cols));
   L.P.spmv_t(r.data(), bc.data());
-  const std::vector<double> ec = v_cycle(levels, l + 1, bc);
+  const std::vector<double> ec = v_cycle(levels, l + 1, bc, cheb);
   std::vector<double> pe(static_cast<std::size_t>(L.n));
   L.P.spmv(ec.data(), pe.data());
   for (int i = 0; i < L.n; ++i) x[i] += pe[i];
-  jacobi_sweep(L, b, x);
+  if (g_smoother == 1)
+    cheb_smooth(L.n, L.Dinv, cheb[static_cast<std::size_t>(l) + 1], g_cheb_degree, b, x, false, spmv);
+  else
+    jacobi_sweep(L, b, x);
   return x;
 }
 }  // namespace
@@ -927,7 +1004,28 @@ bool build_mf_hierarchy(const Reduced& m, int nnx, int nny, int nnz, Hierarchy& 
   }
   H.m = &m;
   H.coarse = std::move(coarse);
+  H.cheb.clear();
+  if (g_smoother == 1) {
+    // level 0: the matrix-free fine operator over the kept DOFs
+    std::vector<long long> gid(m.kept_global.begin(), m.kept_global.end());
+    H.cheb.push_back(cheb_level(power_lambda(
+        m.ng, gid, m.invdiag, [&](const double* in, double* out) { m.apply_kgg(in, out); })));
+    for (const Level& L : H.coarse) {
+      if (L.coarsest) break;
+      std::vector<long long> lg(static_cast<std::size_t>(L.n));
+      for (std::size_t q = 0; q < L.active.size(); ++q)
+        if (L.active[q] >= 0) lg[static_cast<std::size_t>(L.active[q])] = static_cast<long long>(q);
+      H.cheb.push_back(cheb_level(power_lambda(
+          L.n, lg, L.Dinv, [&](const double* in, double* out) { L.A.spmv(in, out); })));
+    }
+  }
   return true;
+}
+
+void set_smoother(int kind, int degree) {
+  if (kind < 0 || kind > 1 || degree < 1) throw std::invalid_argument("set_smoother: bad arguments");
+  g_smoother = kind;
+  g_cheb_degree = degree;
 }
 
 void mf_v_cycle(const Hierarchy& H, const std::vector<double>& b, std::vector<double>& x) {
@@ -939,17 +1037,24 @@ void mf_v_cycle(const Hierarchy& H, const std::vector<double>& b, std::vector<do
     m.apply_kgg(x.data(), Ax.data());
     for (int i = 0; i < ng; ++i) x[i] += kJacobiOmega * (m.invdiag[i] * (b[i] - Ax[i]));
   };
-  sweep();
+  auto kgg = [&](const double* in, double* out) { m.apply_kgg(in, out); };
+  if (g_smoother == 1)
+    cheb_smooth(ng, m.invdiag, H.cheb[0], g_cheb_degree, b, x, true, kgg);
+  else
+    sweep();
   m.apply_kgg(x.data(), Ax.data());
   std::vector<double> vr(static_cast<std::size_t>(ng));
   for (int i = 0; i < ng; ++i) vr[i] = b[i] - Ax[i];
   std::vector<double> bc(static_cast<std::size_t>(H.P0.cols));
   H.P0.spmv_t(vr.data(), bc.data());
-  const std::vector<double> ec = v_cycle(H.coarse, 0, bc);
+  const std::vector<double> ec = v_cycle(H.coarse, 0, bc, H.cheb);
   std::vector<double> pe(static_cast<std::size_t>(ng));
   H.P0.spmv(ec.data(), pe.data());
   for (int i = 0; i < ng; ++i) x[i] += pe[i];
-  sweep();
+  if (g_smoother == 1)
+    cheb_smooth(ng, m.invdiag, H.cheb[0], g_cheb_degree, b, x, false, kgg);
+  else
+    sweep();
 }
 
 bool mf_mgpcg(const Hierarchy& H, const std::vector<double>& b, double tol, int max_it,

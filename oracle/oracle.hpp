@@ -202,12 +202,36 @@ struct Level {
   } chol;
 };
 
+// Chebyshev interval of one level's smoother: theta = (lU + lL) / 2,
+// delta = (lU - lL) / 2 for the bounds [lL, lU] on the spectrum of D^-1 A.
+struct ChebLevel {
+  double lambda = 0.0;  // power-iteration estimate of lambda_max(D^-1 A)
+  double theta = 0.0, delta = 0.0;
+};
+
 struct Hierarchy {
   const Reduced* m = nullptr;
   Csr P0;                     // fine kept rows x level-1 cols
   std::vector<Level> coarse;  // levels 1..L
+  std::vector<ChebLevel> cheb;  // Chebyshev smoother only: levels 0..L-2
   int levels() const { return 1 + static_cast<int>(coarse.size()); }
 };
+
+// V-cycle smoother: 0 = damped Jacobi, omega 0.6, 1 + 1 sweeps (the
+// reference's, core/src/fea/multigrid.cpp:72,1128-1160); 1 = Chebyshev of the
+// given degree (degree operator applies per smoothing) on the Jacobi-scaled
+// operator, an option of this engine with no reference counterpart. Process-wide.
+void set_smoother(int kind, int degree);
+// Chebyshev constants: power-iteration steps for lambda_max and the interval
+// [kChebLow, kChebHigh] * lambda_max it smooths.
+constexpr int kChebPowerSteps = 10;
+constexpr double kChebLow = 0.3, kChebHigh = 1.1;
+// The deterministic start vector of the power iteration at a level DOF whose
+// node-major index is g = 3 * node + component.
+inline double cheb_start(long long g) {
+  const unsigned h = static_cast<unsigned>(g + 1) * 2654435761u;
+  return static_cast<double>(h >> 8) * (1.0 / 16777216.0) - 0.5;
+}
 
 // build_mf_hierarchy: core/src/fea/multigrid.cpp:1344-1515 (geometric branch,
 // unpadded), galerkin_a1_from_prolong :1543-1868, build_hierarchy :557-716.

@@ -136,6 +136,21 @@ def hierarchy(s, E, nu=0.3):
     return lv.value, dofs[: lv.value].tolist()
 
 
+def set_smoother(kind="jacobi", degree=1):
+    """V-cycle smoother of every later solve: "jacobi" (the reference's) or
+    "chebyshev" of the given degree."""
+    _check(lib().oracle_set_smoother({"jacobi": 0, "chebyshev": 1}[kind], int(degree)))
+
+
+def cheb_lambdas(s, E, nu=0.3):
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    n = C.c_int(0)
+    lam = np.zeros(16)
+    _check(lib().oracle_cheb_lambdas(C.byref(s.p), _ptr(E), C.c_double(nu), C.byref(n),
+                                     _ptr(lam)))
+    return lam[: n.value].copy()
+
+
 def level_operator(s, E, level, nu=0.3):
     """Hierarchy level `level` (>= 1) operator as (n, indptr, indices, data)
     over the level's active DOFs (node-major, then component)."""

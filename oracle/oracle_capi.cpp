@@ -171,6 +171,26 @@ int oracle_level_csr(const OracleProblem* p, const double* E, double nu, int lev
   });
 }
 
+int oracle_set_smoother(int kind, int degree) {
+  return guarded([&] { oracle::set_smoother(kind, degree); });
+}
+
+// lambda_max(D^-1 A) estimates of the Chebyshev smoother per level (0 = fine)
+// for this system, from a hierarchy built with the current smoother setting.
+int oracle_cheb_lambdas(const OracleProblem* p, const double* E, double nu, int* n_out,
+                        double* lambdas) {
+  return guarded([&] {
+    const oracle::Grid g = grid_of(p);
+    std::vector<double> Ev(E, E + g.voxels());
+    oracle::Reduced m = oracle::build_reduced(g, Ev, nu, bcs_of(p), loads_of(p), nullptr);
+    oracle::Hierarchy H;
+    if (!oracle::build_mf_hierarchy(m, g.nnx(), g.nny(), g.nnz(), H))
+      throw std::invalid_argument("no multigrid hierarchy for this grid");
+    *n_out = static_cast<int>(H.cheb.size());
+    for (std::size_t i = 0; i < H.cheb.size() && i < 16; ++i) lambdas[i] = H.cheb[i].lambda;
+  });
+}
+
 int oracle_simp_compliance(const OracleProblem* p, const double* density, double E0, double nu,
                            double penalty, double rho_min, double tol, int max_it, int use_mg,
                            double* c, double* dc, double* u, OracleInfo* info) {

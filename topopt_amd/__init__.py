@@ -314,6 +314,18 @@ class Problem:
                                                     C.c_size_t(rung), int(iteration),
                                                     int(bool(boundary))))
 
+    def set_smoother(self, kind="jacobi", degree=1):
+        """V-cycle smoother of later solves: "jacobi" (damped, omega 0.6; the
+        reference's) or "chebyshev" of the given degree."""
+        _check(lib().topohip_set_smoother(self._h, {"jacobi": 0, "chebyshev": 1}[kind],
+                                          int(degree)))
+
+    def smoother_lambdas(self):
+        lam = np.zeros(16)
+        n = C.c_int(0)
+        _check(lib().topohip_smoother_lambdas(self._h, 16, _p(lam), C.byref(n)))
+        return lam[: n.value].copy()
+
     def set_cancel_flag(self, flag):
         """flag: a ctypes.c_int the caller keeps (nonzero cancels the design
         loop at the start of its next iteration), or None to clear."""

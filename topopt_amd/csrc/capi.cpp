@@ -333,6 +333,31 @@ double topohip_set_solve_deadline_ms(double absolute_steady_ms) {
 
 double topohip_steady_ms(void) { return steady_ms(); }
 
+int topohip_set_smoother(topohip_problem* p, int kind, int degree) {
+  return guarded([&] {
+    need(p, "problem");
+    set_smoother(*p->P, kind, degree);
+  });
+}
+
+int topohip_smoother_lambdas(topohip_problem* p, int n, double* lambdas, int* levels) {
+  return guarded([&] {
+    need(p, "problem");
+    Problem& P = *p->P;
+    int k = 0;
+    if (P.smoother == 1 && P.mg_plan_ok) {
+      for (size_t l = 0; l + 1 < P.L.size(); ++l, ++k) {
+        if (k >= n || !lambdas) continue;
+        double v = 0.0;
+        TOPOHIP_CHECK(hipMemcpyAsync(&v, P.L[l]->cheb.p, sizeof(double), hipMemcpyDeviceToHost, P.s));
+        TOPOHIP_CHECK(hipStreamSynchronize(P.s));
+        lambdas[k] = v;
+      }
+    }
+    if (levels) *levels = k;
+  });
+}
+
 int topohip_set_cancel_flag(topohip_problem* p, const int* flag) {
   return guarded([&] {
     need(p, "problem");

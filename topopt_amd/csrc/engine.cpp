@@ -30,6 +30,12 @@ constexpr double kOmega = 0.6;       // damped Jacobi weight (multigrid.cpp:72)
 constexpr int kMinCoarseElems = 2;   // coarsen.hpp:76
 constexpr int kCoarseDofCap = 6000;  // coarsen.hpp:80
 constexpr int kMgIterBudget = 300;   // multigrid.cpp:108
+// Chebyshev smoother option: lambda_max(D^-1 A) from kChebPowerSteps power
+// steps per smoothed level at every hierarchy build, smoothing the interval
+// [kChebLow, kChebHigh] * lambda_max (measured on the oracle: 0.3 keeps the MBB
+// and cantilever counts lowest; 0.1 and 1/30 smooth too little of the top).
+constexpr int kChebPowerSteps = 10;
+constexpr double kChebLow = 0.3, kChebHigh = 1.1;
 
 double now_ms() {
   using namespace std::chrono;
@@ -344,6 +350,9 @@ template <> double* Xv<double>(Level& l) { return l.x64.p; }
 template <typename T> T* Tv(Level& l);
 template <> float* Tv<float>(Level& l) { return l.t32.p; }
 template <> double* Tv<double>(Level& l) { return l.t64.p; }
+template <typename T> T* Dv(Level& l);
+template <> float* Dv<float>(Level& l) { return l.d32.p; }
+template <> double* Dv<double>(Level& l) { return l.d64.p; }
 template <typename T> const T* Ev(Problem& P);
 template <> const float* Ev<float>(Problem& P) { return P.E32.p; }
 template <> const double* Ev<double>(Problem& P) { return P.E64.p; }
@@ -429,7 +438,8 @@ void alloc_levels(Problem& P) {
 }
 
 int max_partials(const Problem& P) {
-  return std::max(ku_partials(P.g), std::max(dot_blocks(P.g.ndof()), sens_blocks(P.g)));
+  return std::max(std::max(ku_partials(P.g), 2 * cheb_blocks(P.g.ndof())),
+                  std::max(dot_blocks(P.g.ndof()), sens_blocks(P.g)));
 }
 
 // ---- V-cycle ----------------------------------------------------------------
@@ -437,6 +447,26 @@ int max_partials(const Problem& P) {
 // every launch covers the nc cases, and a case whose PCG is done is skipped.
 // Coarse levels l >= 1: restrict the level-(l-1) residual rprev, smooth,
 // recurse, prolong, smooth. Result lives in t_l (x_l at the coarsest).
+// Chebyshev smoothing of one level (degree operator applies from x, one
+// fewer from zero): res(x, t) forms t = b - A x; the last step writes xo
+// (which may be t: each entry is read before it is overwritten).
+template <typename T, class Res>
+void cheb_smooth(Problem& P, Level& lv, T* x, T* xo, T* t, const T* b, bool zero_start,
+                 const int* done, int nc, Res res) {
+  const long long n = lv.g.ndof();
+  T* d = Dv<T>(lv);
+  const T* dinv = Dinv<T>(lv);
+  const double* coef = lv.cheb.p;
+  const int m = P.cheb_degree;
+  if (!zero_start) res(x, t);
+  launch_cheb_first<T>(x, m == 1 ? xo : x, d, zero_start ? b : t, dinv, coef, n, zero_start, done,
+                       P.s, nc);
+  for (int k = 1; k < m; ++k) {
+    res(x, t);
+    launch_cheb_step<T>(x, k + 1 == m ? xo : x, d, t, dinv, coef, k, n, done, P.s, nc);
+  }
+}
+
 template <typename T>
 void coarse_cycle(Problem& P, size_t l, const T* rprev, const int* done, int nc) {
   Level& f = *P.L[l - 1];
@@ -448,6 +478,24 @@ void coarse_cycle(Problem& P, size_t l, const T* rprev, const int* done, int nc)
     launch_restrict<T, T>(f.g, c.g, rprev, c.mask.p, static_cast<T*>(nullptr), nullptr, nullptr,
                           0.0, done, P.s, nc, P.cr.rhs_map());
     P.cr.solve_in_place(done, P.s, nc);
+    return;
+  }
+  if (P.smoother == 1) {
+    auto res = [&](const T* x, T* t) {
+      launch_stencil_apply<T>(c.g, Kapp<T>(c), c.mask.p, x, t, Bv<T>(c), nullptr, 0.0, EPI_RES,
+                              done, P.s, nc);
+    };
+    launch_restrict<T, T>(f.g, c.g, rprev, c.mask.p, Bv<T>(c), static_cast<T*>(nullptr), nullptr,
+                          0.0, done, P.s, nc);
+    cheb_smooth<T>(P, c, Xv<T>(c), Xv<T>(c), Tv<T>(c), Bv<T>(c), true, done, nc, res);
+    res(Xv<T>(c), Tv<T>(c));
+    coarse_cycle<T>(P, l + 1, Tv<T>(c), done, nc);
+    Level& n = *P.L[l + 1];
+    const bool ncoarsest = l + 2 == P.L.size();
+    launch_prolong_add<T, T>(n.g, c.g, ncoarsest ? nullptr : Tv<T>(n), c.mask.p, Xv<T>(c), done,
+                             P.s, nc, ncoarsest ? P.cr.sol_map() : CrMap());
+    // the level's result goes to t_l, where the parent's prolongation reads it
+    cheb_smooth<T>(P, c, Xv<T>(c), Tv<T>(c), Tv<T>(c), Bv<T>(c), false, done, nc, res);
     return;
   }
   launch_restrict<T, T>(f.g, c.g, rprev, c.mask.p, Bv<T>(c), Xv<T>(c), Dinv<T>(c), kOmega, done,
@@ -485,6 +533,27 @@ template <typename T>
 void vcycle(Problem& P, const T* b0, T* z, const int* done, int nc, const double* r64,
             double* rz_partials) {
   Level& f = *P.L[0];
+  if (P.smoother == 1) {
+    auto res = [&](const T* x, T* t) {
+      KuOp<T> op = fine_op<T>(P, done, nc);
+      op.x = x;
+      op.y = t;
+      op.b = b0;
+      op.epi = EPI_RES;
+      launch_ku_fine<T>(op, P.s);
+    };
+    cheb_smooth<T>(P, f, Xv<T>(f), Xv<T>(f), Tv<T>(f), b0, true, done, nc, res);
+    res(Xv<T>(f), Tv<T>(f));
+    coarse_cycle<T>(P, 1, Tv<T>(f), done, nc);
+    Level& c = *P.L[1];
+    const bool ccoarsest = P.L.size() == 2;
+    launch_prolong_add<T, T>(c.g, f.g, ccoarsest ? nullptr : Tv<T>(c), f.mask.p, Xv<T>(f), done,
+                             P.s, nc, ccoarsest ? P.cr.sol_map() : CrMap());
+    cheb_smooth<T>(P, f, Xv<T>(f), z, Tv<T>(f), b0, false, done, nc, res);
+    launch_dot<double, T>(r64, z, f.g.ndof(), rz_partials, dot_blocks(f.g.ndof()), done, P.s, nc,
+                          P.pstride);
+    return;
+  }
   KuOp<T> op = fine_op<T>(P, done, nc);
   op.b = b0;
   op.dinv = Dinv<T>(f);
@@ -529,13 +598,14 @@ const int* done_flags(Problem& P) { return &P.st.p->done; }
 // case. The V-cycle's last operator apply forms them; Jacobi uses a dot pass.
 int precondition_rz(Problem& P, Prec pc, const int* done, int nc) {
   const long long nd = P.g.ndof();
+  const int nrz = P.smoother == 1 ? dot_blocks(nd) : ku_partials(P.g);
   if (pc == Prec::MG32) {
     vcycle<float>(P, P.r32.p, P.z32.p, done, nc, P.r.p, P.partials.p);
-    return ku_partials(P.g);
+    return nrz;
   }
   if (pc == Prec::MG64) {
     vcycle<double>(P, P.r.p, P.z64.p, done, nc, P.r.p, P.partials.p);
-    return ku_partials(P.g);
+    return nrz;
   }
   launch_jacobi0<double>(P.z64.p, P.L[0]->dinv64.p, P.r.p, 1.0, nd, done, P.s, nc);
   const int nb = dot_blocks(nd);
@@ -862,19 +932,24 @@ void ensure_cases(Problem& P, int nb) {
   P.b.ensure(nd); P.x.ensure(nd); P.r.ensure(nd); P.p.ensure(nd); P.rt.ensure(nd);
   P.u.ensure(nd); P.Ap64.ensure(nd); P.z64.ensure(nd);
   if (mixed) { P.r32.ensure(nd); P.z32.ensure(nd); }
+  const bool cheb = P.smoother == 1;
   for (size_t l = 0; l < P.L.size(); ++l) {
     Level& lv = *P.L[l];
     const size_t ndl = static_cast<size_t>(lv.g.ndof()) * nb;
+    const bool smoothed = P.mg_plan_ok && l + 1 < P.L.size();
     if (mixed) {
       lv.x32.ensure(ndl);
       lv.t32.ensure(ndl);
       if (l > 0) lv.b32.ensure(ndl);
+      if (cheb && smoothed) lv.d32.ensure(ndl);
     }
     if (!mixed || P.mg64_alloc) {
       lv.x64.ensure(ndl);
       lv.t64.ensure(ndl);
       if (l > 0) lv.b64.ensure(ndl);
+      if (cheb && smoothed) lv.d64.ensure(ndl);
     }
+    if (cheb && smoothed) lv.cheb.ensure(static_cast<size_t>(2 * P.cheb_degree + 2));
   }
   P.st.ensure(static_cast<size_t>(nb));
   P.scal.ensure(static_cast<size_t>(std::max(8, nb)));
@@ -919,6 +994,18 @@ void ensure_mg64(Problem& P) {
     }
     P.mg64_ready = true;
   }
+}
+
+void set_smoother(Problem& P, int kind, int degree) {
+  if (kind != 0 && kind != 1) throw Error(TOPOHIP_EINVAL, "smoother: kind must be 0 or 1");
+  if (degree < 1 || degree > 8) throw Error(TOPOHIP_EINVAL, "smoother: degree must be 1..8");
+  sync(P);
+  drop_graphs(P);  // the captured iteration bodies smooth the old way
+  P.smoother = kind;
+  P.cheb_degree = kind == 1 ? degree : 1;
+  const int nb = P.nb;
+  P.nb = 0;  // re-run the capacity pass: direction vectors and coefficient slots
+  ensure_cases(P, nb);
 }
 
 std::unique_ptr<Problem> create_problem(int nx, int ny, int nz, double h, double nu,
@@ -1063,6 +1150,41 @@ void set_loads(Problem& P, int n_rhs, const int32_t* counts, const int32_t* node
   sync(P);
 }
 
+// lambda_max(D^-1 A) of every smoothed level by kChebPowerSteps power steps
+// (the level's x / t work vectors hold v and A v) and the Chebyshev
+// coefficients derived from it, all on the device: captured PCG graphs read
+// them from there, so a rebuilt hierarchy needs no new capture.
+template <typename T>
+void cheb_estimate(Problem& P) {
+  for (size_t l = 0; l + 1 < P.L.size(); ++l) {
+    Level& lv = *P.L[l];
+    const long long n = lv.g.ndof();
+    T* v = Xv<T>(lv);
+    T* y = Tv<T>(lv);
+    const T* dinv = Dinv<T>(lv);
+    auto apply = [&]() {
+      if (l == 0) {
+        KuOp<T> op = fine_op<T>(P, nullptr, 1);
+        op.x = v;
+        op.y = y;
+        op.epi = EPI_AX;
+        launch_ku_fine<T>(op, P.s);
+      } else {
+        launch_stencil_apply<T>(lv.g, Kapp<T>(lv), lv.mask.p, v, y, nullptr, nullptr, 0.0, EPI_AX,
+                                nullptr, P.s, 1);
+      }
+    };
+    launch_cheb_power<T>(0, v, nullptr, dinv, n, P.partials.p, nullptr, 0, 0.0, 0.0, P.s);
+    for (int k = 0; k < kChebPowerSteps; ++k) {
+      apply();
+      if (k + 1 < kChebPowerSteps)
+        launch_cheb_power<T>(1, v, y, dinv, n, P.partials.p, nullptr, 0, 0.0, 0.0, P.s);
+    }
+    launch_cheb_power<T>(2, v, y, dinv, n, P.partials.p, lv.cheb.p, P.cheb_degree, kChebLow,
+                         kChebHigh, P.s);
+  }
+}
+
 void build_hierarchy(Problem& P, Timers& t) {
   const double t0 = now_ms();
   P.mg64_ready = false;
@@ -1111,6 +1233,10 @@ void build_hierarchy(Problem& P, Timers& t) {
     }
     Level& c = *P.L.back();
     P.cr.build(c.Kem.p, c.mask.p, P.s);
+    if (P.smoother == 1) {
+      if (mixed) cheb_estimate<float>(P);
+      else cheb_estimate<double>(P);
+    }
   }
   sync(P);
   t.mg_build += now_ms() - t0;

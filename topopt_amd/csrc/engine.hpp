@@ -29,6 +29,9 @@ struct Level {
   DBuf<double> dinv64;
   DBuf<float> b32, x32, t32;    // V-cycle work (mixed form), stacked load cases
   DBuf<double> b64, x64, t64;   // V-cycle work (FP64 form), stacked load cases
+  DBuf<float> d32;              // Chebyshev smoother: direction (stacked cases)
+  DBuf<double> d64;
+  DBuf<double> cheb;            // Chebyshev smoother: [lambda, 1/theta, (c1_k, c2_k)...]
 };
 
 struct FilterCache {
@@ -51,6 +54,10 @@ struct Problem {
   double h = 1.0, nu = 0.3;
   int precision = TOPOHIP_PREC_MIXED;
   int tz = 8;   // fine K.u z-chunk (autotuned; a multiple of kSlab)
+  // V-cycle smoother: 0 damped Jacobi (the reference's), 1 Chebyshev of degree
+  // cheb_degree on every smoothed level
+  int smoother = 0;
+  int cheb_degree = 1;
 
   bool all_solid = true;
   std::vector<uint8_t> solid_h;
@@ -203,6 +210,8 @@ std::unique_ptr<FilterCache> make_filter(Problem& P, double radius, const uint8_
 void design_mask_of(const Problem& P, const uint8_t* mask, std::vector<uint8_t>& eff,
                     std::vector<uint8_t>& dset, long long& n_design);
 void hex8_stiffness_host(double E, double nu, double h, double* K576);
+// Select the V-cycle smoother (0 Jacobi, 1 Chebyshev of `degree`) for later solves.
+void set_smoother(Problem& P, int kind, int degree);
 // fea_set_solve_deadline_ms for the calling thread; returns the previous value.
 double set_solve_deadline_ms(double absolute_steady_ms);
 double steady_ms();

@@ -80,6 +80,24 @@ int dot_blocks(long long n);
 template <typename T>
 void launch_jacobi0(T* x, const T* dinv, const T* b, double omega, long long n, const int* done,
                     hipStream_t s, int nr = 1);
+// Chebyshev smoother (coef: device [lambda, 1/theta, (c1_k, c2_k) for k >= 1]):
+// first step d = D^-1 r / theta, xo = d (zero_start) or xo = x + d; step k:
+// d = c1_k d + c2_k D^-1 r, xo = x + d (xo may be x). Stacked cases of stride n.
+template <typename T>
+void launch_cheb_first(const T* x, T* xo, T* d, const T* r, const T* dinv, const double* coef,
+                       long long n, bool zero_start, const int* done, hipStream_t s, int nr = 1);
+template <typename T>
+void launch_cheb_step(const T* x, T* xo, T* d, const T* r, const T* dinv, const double* coef,
+                      int k, long long n, const int* done, hipStream_t s, int nr = 1);
+// Power iteration for lambda_max(D^-1 A) over one vector of length n:
+// step 0 writes the start vector into v; step 1 turns y = A v into
+// v = D^-1 y / ||D^-1 y||; step 2 forms the Rayleigh quotient (v.y)/(v.Dv) and
+// the Chebyshev coefficients of [lo, hi] * lambda into coef (partials: at
+// least 2 * cheb_blocks(n)).
+template <typename T>
+void launch_cheb_power(int step, T* v, const T* y, const T* dinv, long long n, double* partials,
+                       double* coef, int degree, double lo, double hi, hipStream_t s);
+int cheb_blocks(long long n);
 // y = x + up (scatter prescribed values into the solution).
 void launch_add(const double* x, const double* up, double* y, long long n, hipStream_t s);
 template <typename T> void launch_fill(T* p, T v, long long n, hipStream_t s);

@@ -402,6 +402,122 @@ __global__ void cg_p_k(CgState* st, const TZ* __restrict__ z, double* __restrict
   }
 }
 
+// ---- Chebyshev smoother ---------------------------------------------------------
+// coef (device, per level): [0] lambda_max estimate, [1] 1/theta, then for
+// step k = 1..degree-1 the pair (c1_k, c2_k) at [2k], [2k+1].
+// (xo receives the updated x; it may be x itself.)
+template <typename T>
+__global__ void cheb_first_k(const T* x, T* xo, T* __restrict__ d, const T* r,
+                             const T* __restrict__ dinv, const double* __restrict__ coef,
+                             long long n, int zero_start, const int* done) {
+  const int rc = blockIdx.y;
+  if (case_done(done, rc)) return;
+  x += rc * n;
+  xo += rc * n;
+  d += rc * n;
+  r += rc * n;
+  const double it = coef[1];
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const T di = static_cast<T>(static_cast<double>(dinv[i]) * static_cast<double>(r[i]) * it);
+    d[i] = di;
+    xo[i] = zero_start ? di : x[i] + di;
+  }
+}
+
+template <typename T>
+__global__ void cheb_step_k(const T* x, T* xo, T* __restrict__ d, const T* r,
+                            const T* __restrict__ dinv, const double* __restrict__ coef, int k,
+                            long long n, const int* done) {
+  const int rc = blockIdx.y;
+  if (case_done(done, rc)) return;
+  x += rc * n;
+  xo += rc * n;
+  d += rc * n;
+  r += rc * n;
+  const T c1 = static_cast<T>(coef[2 * k]), c2 = static_cast<T>(coef[2 * k + 1]);
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const T di = c1 * d[i] + c2 * (dinv[i] * r[i]);
+    d[i] = di;
+    xo[i] = x[i] + di;
+  }
+}
+
+// Power iteration of lambda_max(D^-1 A). start: v = cheb_start(3 node +
+// comp) on kept DOFs (dinv != 0), 0 elsewhere.
+template <typename T>
+__global__ void cheb_start_k(T* __restrict__ v, const T* __restrict__ dinv, long long n) {
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const unsigned h = static_cast<unsigned>(i + 1) * 2654435761u;
+    const double v0 = static_cast<double>(h >> 8) * (1.0 / 16777216.0) - 0.5;
+    v[i] = dinv[i] != T(0) ? static_cast<T>(v0) : T(0);
+  }
+}
+// w = D^-1 y into v, partials of |w|^2.
+template <typename T>
+__global__ void cheb_scale_k(T* __restrict__ v, const T* __restrict__ y, const T* __restrict__ dinv,
+                             long long n, double* __restrict__ partials) {
+  double s = 0.0;
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const double w = static_cast<double>(dinv[i]) * static_cast<double>(y[i]);
+    v[i] = static_cast<T>(w);
+    s += w * w;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+// v /= ||v|| (every block re-sums the partials in a fixed order).
+template <typename T>
+__global__ void cheb_norm_k(T* __restrict__ v, long long n, const double* __restrict__ partials,
+                            int np) {
+  const double inv = 1.0 / sqrt(fixed_sum_all(partials, np));
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    v[i] = static_cast<T>(static_cast<double>(v[i]) * inv);
+}
+// partials of v.y (first np) and v.D v (next np), D = 1 / dinv on kept DOFs.
+template <typename T>
+__global__ void cheb_rq_k(const T* __restrict__ v, const T* __restrict__ y,
+                          const T* __restrict__ dinv, long long n, double* __restrict__ partials) {
+  double a = 0.0, b = 0.0;
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const double vi = static_cast<double>(v[i]);
+    a += vi * static_cast<double>(y[i]);
+    if (dinv[i] != T(0)) b += vi * vi / static_cast<double>(dinv[i]);
+  }
+  a = block_sum(a);
+  __syncthreads();
+  b = block_sum(b);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = a;
+    partials[gridDim.x + blockIdx.x] = b;
+  }
+}
+// lambda = (v.Av) / (v.Dv); the interval [lo, hi] * lambda and the step
+// coefficients of the degree-`degree` Chebyshev smoother.
+__global__ void cheb_coef_k(const double* __restrict__ partials, int np, double lo, double hi,
+                            int degree, double* __restrict__ coef) {
+  const double vav = fixed_sum_all(partials, np);
+  const double vdv = fixed_sum_all(partials + np, np);
+  if (threadIdx.x != 0) return;
+  const double lam = vav / vdv;
+  const double theta = 0.5 * (hi + lo) * lam, delta = 0.5 * (hi - lo) * lam;
+  coef[0] = lam;
+  coef[1] = 1.0 / theta;
+  const double sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  for (int k = 1; k < degree; ++k) {
+    const double rho_new = 1.0 / (2.0 * sigma - rho);
+    coef[2 * k] = rho_new * rho;
+    coef[2 * k + 1] = 2.0 * rho_new / delta;
+    rho = rho_new;
+  }
+}
+
 int grid_for(long long n, int bs, int cap) {
   long long b = (n + bs - 1) / bs;
   if (b > cap) b = cap;
@@ -562,5 +678,49 @@ template void launch_cg_p<float>(CgState*, const float*, double*, long long, int
                                  const double*, int, hipStream_t, int, int);
 template void launch_cg_p<double>(CgState*, const double*, double*, long long, int,
                                   const double*, int, hipStream_t, int, int);
+
+int cheb_blocks(long long n) { return grid_for(n, RB * 4, 1024); }
+
+template <typename T>
+void launch_cheb_first(const T* x, T* xo, T* d, const T* r, const T* dinv, const double* coef,
+                       long long n, bool zero_start, const int* done, hipStream_t s, int nr) {
+  hipLaunchKernelGGL((cheb_first_k<T>), dim3(grid_for(n, 256, 4096), nr), dim3(256), 0, s, x, xo,
+                     d, r, dinv, coef, n, zero_start ? 1 : 0, done);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_cheb_step(const T* x, T* xo, T* d, const T* r, const T* dinv, const double* coef,
+                      int k, long long n, const int* done, hipStream_t s, int nr) {
+  hipLaunchKernelGGL((cheb_step_k<T>), dim3(grid_for(n, 256, 4096), nr), dim3(256), 0, s, x, xo,
+                     d, r, dinv, coef, k, n, done);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_cheb_power(int step, T* v, const T* y, const T* dinv, long long n, double* partials,
+                       double* coef, int degree, double lo, double hi, hipStream_t s) {
+  const int nb = cheb_blocks(n);
+  const dim3 g(nb), b(RB);
+  if (step == 0) {
+    hipLaunchKernelGGL((cheb_start_k<T>), dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, v, dinv,
+                       n);
+  } else if (step == 1) {
+    hipLaunchKernelGGL((cheb_scale_k<T>), g, b, 0, s, v, y, dinv, n, partials);
+    hipLaunchKernelGGL((cheb_norm_k<T>), g, b, 0, s, v, n, partials, nb);
+  } else {
+    hipLaunchKernelGGL((cheb_rq_k<T>), g, b, 0, s, v, y, dinv, n, partials);
+    hipLaunchKernelGGL(cheb_coef_k, dim3(1), b, 0, s, partials, nb, lo, hi, degree, coef);
+  }
+  TOPOHIP_CHECK(hipGetLastError());
+}
+#define TOPOHIP_CHEB_INST(T)                                                                     \
+  template void launch_cheb_first<T>(const T*, T*, T*, const T*, const T*, const double*,       \
+                                     long long, bool, const int*, hipStream_t, int);             \
+  template void launch_cheb_step<T>(const T*, T*, T*, const T*, const T*, const double*, int,    \
+                                    long long, const int*, hipStream_t, int);                    \
+  template void launch_cheb_power<T>(int, T*, const T*, const T*, long long, double*, double*,   \
+                                     int, double, double, hipStream_t);
+TOPOHIP_CHEB_INST(float)
+TOPOHIP_CHEB_INST(double)
+#undef TOPOHIP_CHEB_INST
 
 }  // namespace topohip
