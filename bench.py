@@ -90,6 +90,29 @@ def measured_traffic(workload, variant, tile):
     return None
 
 
+# Design-loop kernel categories that run in FP64 in both forms (the coarsest
+# factorisation and solve, the outer PCG's vector work, the A.p apply, the
+# SIMP kernels and the hierarchy build): with the mixed form's FP32 parts made
+# free, the mixed/FP64 ratio could not exceed (measured ratio) / share.
+FP64_IN_BOTH = ("coarsest factorisation", "coarsest solve", "PCG vector kernels",
+                "fine K.u FP64", "SIMP", "hierarchy build")
+
+
+def fp64_invariant_share(workload):
+    """Share of the design loop's kernel time that is FP64 in both forms, from
+    the committed rocprofv3 kernel summary of the mixed form's design loop."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_design_loop.md")
+    if not os.path.exists(path):
+        return None
+    share = 0.0
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 4 and cells[3].replace(".", "", 1).isdigit():
+            if any(cells[0].startswith(k) for k in FP64_IN_BOTH):
+                share += float(cells[3]) / 100.0
+    return share or None
+
+
 def run_design(ta, wl, precision, steps, warmup, mma=False, smoother=None):
     """One timed optimisation run (OC, or MMA with the plateau stop off so the
     run has exactly `steps` iterations); smoother = (kind, degree) or None for
@@ -378,9 +401,12 @@ def main():
         out64, dt64 = run_design(ta, wl, ta.PREC_F64, a.steps, min(a.warmup, 1))
         v64 = out64["iterations"] / dt64
         rel = abs(out["compliance"] - out64["compliance"]) / abs(out64["compliance"])
+        share = fp64_invariant_share(a.workload)
         fp64 = {"value": v64, "ratio": value / v64, "compliance": out64["compliance"],
                 "compliance_rel_diff_vs_mixed": rel,
-                "cg_iterations_total": out64["cg_iterations_total"]}
+                "cg_iterations_total": out64["cg_iterations_total"],
+                "fp64_in_both_share": share,
+                "ratio_bound_if_fp32_parts_were_free": None if not share else value / v64 / share}
         log(f"# fp64 form: {v64:.2f} design iters/s")
     mma = None
     if not a.no_mma:
