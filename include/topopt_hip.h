@@ -495,6 +495,41 @@ typedef struct {
 int topohip_write_fields_file(const char* path, const topohip_grid_frame* grid, int n_variants,
                               const topohip_fields_variant* variants);
 
+/* run_info.json, the run version record (RunInfo / run_info_json /
+ * write_run_info, core/include/topopt/observability.hpp:260-268,
+ * core/src/simp/observability.cpp:476-1284, 1761-1767): this library writes
+ * the keys of the solve it runs (identity, solver, multigrid outcome,
+ * precision, warm start, per-rung non-convergence, ladder, timestamp, the
+ * record switches) in the reference's order and number formats (%.10g reals,
+ * true/false, JSON null for a multigrid outcome not yet observed). Keys of
+ * subsystems it does not have (GeneO, Krylov recycling beyond its off echo,
+ * lattices, PLSM, preflight, active domain, print settings) are not written.
+ * NULL strings are written as "". */
+typedef struct {
+  const char *cli_version, *fingerprint, *mode, *material, *source_format;
+  int resolution;
+  const char *load_source, *solver;
+  int cg_multigrid_observed, cg_multigrid, mg_levels;
+  const char* mg_mode;       /* NULL or "" = null */
+  int mixed_precision, matfree_threads, warm_start_inherit;
+  long long solved_grid_dofs;
+  int projection;
+  int n_rungs;               /* rung_* arrays (nullable when 0) */
+  const int* rung_non_convergent;
+  const int* rung_non_convergent_iteration;
+  const double* rung_non_convergent_residual;
+  int n_ladder;
+  const double* ladder;
+  const char* ladder_mode;
+  long long created_wall_ms;
+  int iteration_csv, density_snapshots, snapshot_every, snapshot_cap;
+} topohip_run_info;
+/* The record's text into buf (NUL-terminated); *len (nullable) receives its
+ * length. EINVAL if n is too small. */
+int topohip_run_info_json(const topohip_run_info* info, char* buf, size_t n, size_t* len);
+/* write_run_info: the same text into path (truncating). */
+int topohip_write_run_info(const char* path, const topohip_run_info* info);
+
 /* The CLI's stdout protocol lines the LAN worker parses
  * (core/src/cli/run_job.cpp:10223-10259, tools/topopt-worker/topopt_worker.py:231-266),
  * newline included. */

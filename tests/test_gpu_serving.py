@@ -120,3 +120,8 @@ def test_server_run_records(tmp_path):
         assert len(b) == 64 + 48 + 4 * nv + 4 * nd
         vm = np.frombuffer(b, "<f4", nv, 64 + 48)
         assert np.all(vm >= 0) and vm.max() == np.float32(v["max_von_mises_mpa"])
+        import json
+        ri = json.loads((d / "run_info.json").read_text())
+        assert ri["solver"] == "Hip" and ri["cg_multigrid"] is True and ri["mg_levels"] >= 2
+        assert ri["solved_grid_dofs"] == nd and ri["ladder"] == [pytest.approx(j["volfrac"])]
+        assert ri["rung_non_convergent"] == [False] and ri["mixed_precision"] is True

@@ -279,3 +279,42 @@ def test_reference_written_design_bin_reads_and_round_trips(tmp_path):
     out = tmp_path / "design.bin"
     ta.write_design_file(out, g, back["variants"])
     assert out.read_bytes() == open(src, "rb").read()
+
+
+def test_run_info_json_matches_the_reference_record(tmp_path):
+    """run_info_json's keys as the reference's test checks them
+    (core/tests/unit/test_observability.cpp:270-311): the fingerprint, the
+    solver, the observed multigrid outcome (null until observed), the
+    Galerkin cache and thread echoes, the ladder, the timestamp; the file
+    writer writes the same text; the record is JSON with the reference's key
+    order for the keys it carries."""
+    import json
+    kw = dict(cli_version="topopt_amd", fingerprint="abc123def456", mode="minimize_plastic",
+              material="PLA", source_format="step", resolution=96, load_source="loadcase",
+              solver="MultigridCG_Matfree", cg_multigrid_observed=1, cg_multigrid=1,
+              mg_levels=4, mixed_precision=1, matfree_threads=6, solved_grid_dofs=2446275,
+              rung_non_convergent=[0, 1], rung_non_convergent_iteration=[0, 300],
+              rung_non_convergent_residual=[0.0, 3.5e-4],
+              ladder=[0.68, 0.52, 0.38, 0.26], ladder_mode="reduction",
+              created_wall_ms=1234567890123, iteration_csv=1, snapshot_every=10, snapshot_cap=2)
+    js = ta.run_info_json(**kw)
+    for frag in ['"fingerprint": "abc123def456"', '"solver": "MultigridCG_Matfree"',
+                 '"cg_multigrid": true', '"mg_levels": 4', '"galerkin_block_cache": true',
+                 '"matfree_threads": 6', '"ladder": [0.68, 0.52, 0.38, 0.26]',
+                 '"created_wall_ms": 1234567890123', '"rung_non_convergent": [false, true]',
+                 '"rung_non_convergent_residual": [0, 0.00035]']:
+        assert frag in js, frag
+    pending = ta.run_info_json(**dict(kw, cg_multigrid_observed=0))
+    assert '"cg_multigrid": null' in pending and '"mg_levels": null' in pending
+    d = json.loads(js)
+    order = ["cli_version", "fingerprint", "mode", "material", "source_format", "resolution",
+             "load_source", "solver", "cg_multigrid", "mg_levels", "mg_mode",
+             "galerkin_block_cache", "mixed_precision", "matfree_threads", "krylov_recycling",
+             "krylov_recycle_dim", "warm_start_inherit", "solved_grid_dofs", "projection",
+             "rung_non_convergent", "rung_non_convergent_iteration",
+             "rung_non_convergent_residual", "ladder", "ladder_mode", "created_wall_ms",
+             "iteration_csv", "density_snapshots", "snapshot_every", "snapshot_cap"]
+    assert list(d) == order
+    assert js.endswith('  "snapshot_cap": 2\n}\n')
+    ta.write_run_info(tmp_path / "run_info.json", **kw)
+    assert (tmp_path / "run_info.json").read_text() == js

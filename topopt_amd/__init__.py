@@ -609,4 +609,5 @@ def version():
 from .records import (CsvWriter, DesignVariant, FieldsVariant, GridFrame,  # noqa: E402
                       IterationObs, design_fingerprint, format_progress, format_variant,
                       iterations_csv_header, iterations_csv_row, read_density_f16,
-                      read_design_file, write_design_file, write_fields_file, SnapshotCapture)
+                      read_design_file, write_design_file, write_fields_file, SnapshotCapture,
+                      RunInfo, run_info_json, write_run_info)

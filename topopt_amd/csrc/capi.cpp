@@ -1,6 +1,7 @@
 // capi.cpp — extern "C" boundary (include/topopt_hip.h). Converts exceptions
 // to status codes and keeps the message per calling thread.
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <functional>
 #include <type_traits>
@@ -828,6 +829,33 @@ int copy_line(const std::string& line, char* buf, size_t n) {
 
 int topohip_format_progress(size_t rung, size_t rungs, int iteration, char* buf, size_t n) {
   return copy_line(progress_line(rung, rungs, iteration), buf, n);
+}
+
+int topohip_run_info_json(const topohip_run_info* info, char* buf, size_t n, size_t* len) {
+  return guarded([&] {
+    need(info, "info");
+    if (info->n_rungs < 0 || info->n_ladder < 0 || (info->n_ladder > 0 && !info->ladder))
+      throw Error(TOPOHIP_EINVAL, "run_info: bad ladder or rung counts");
+    const std::string js = run_info_json(*info);
+    if (len) *len = js.size();
+    if (!buf || js.size() + 1 > n) throw Error(TOPOHIP_EINVAL, "buffer too small for run_info");
+    std::memcpy(buf, js.c_str(), js.size() + 1);
+  });
+}
+
+int topohip_write_run_info(const char* path, const topohip_run_info* info) {
+  return guarded([&] {
+    need(path, "path"); need(info, "info");
+    if (info->n_rungs < 0 || info->n_ladder < 0 || (info->n_ladder > 0 && !info->ladder))
+      throw Error(TOPOHIP_EINVAL, "run_info: bad ladder or rung counts");
+    const std::string js = run_info_json(*info);
+    std::FILE* f = std::fopen(path, "wb");
+    if (!f) throw Error(TOPOHIP_EINVAL, std::string("write_run_info: cannot open ") + path);
+    const size_t w = std::fwrite(js.data(), 1, js.size(), f);
+    const int c = std::fclose(f);
+    if (w != js.size() || c != 0)
+      throw Error(TOPOHIP_EINVAL, std::string("write_run_info: failed writing ") + path);
+  });
 }
 
 int topohip_format_variant(double vf, double achieved, double printed, double margin,

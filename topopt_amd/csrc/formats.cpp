@@ -432,6 +432,82 @@ std::string progress_line(std::size_t rung, std::size_t rungs, int iteration) {
   return buf;
 }
 
+// run_info_json (core/src/simp/observability.cpp:476-1284): `"key": value`
+// lines of two-space indent in the reference's order, strings with `"` and `\`
+// escaped, reals as %.10g, 64-bit integers as %lld, the multigrid outcome null
+// until observed; the last key carries no comma.
+std::string run_info_json(const topohip_run_info& in) {
+  std::string s = "{\n";
+  char nb[64];
+  auto esc = [](const char* v) {
+    std::string o;
+    for (const char* c = v ? v : ""; *c; ++c) {
+      if (*c == '"' || *c == '\\') o.push_back('\\');
+      o.push_back(*c);
+    }
+    return o;
+  };
+  auto num = [&](const char* k, const std::string& v, bool comma = true) {
+    s += "  \"";
+    s += k;
+    s += "\": " + v + (comma ? ",\n" : "\n");
+  };
+  auto str = [&](const char* k, const char* v) { num(k, "\"" + esc(v) + "\""); };
+  auto fmt = [&](double d) {
+    std::snprintf(nb, sizeof(nb), "%.10g", d);
+    return std::string(nb);
+  };
+  auto fmt_ll = [&](long long v) {
+    std::snprintf(nb, sizeof(nb), "%lld", v);
+    return std::string(nb);
+  };
+  auto fmt_i = [&](int v) {
+    std::snprintf(nb, sizeof(nb), "%d", v);
+    return std::string(nb);
+  };
+  auto b = [](int v) { return std::string(v ? "true" : "false"); };
+  str("cli_version", in.cli_version);
+  str("fingerprint", in.fingerprint);
+  str("mode", in.mode);
+  str("material", in.material);
+  str("source_format", in.source_format);
+  num("resolution", fmt_i(in.resolution));
+  str("load_source", in.load_source);
+  str("solver", in.solver);
+  num("cg_multigrid", in.cg_multigrid_observed ? b(in.cg_multigrid) : "null");
+  num("mg_levels", in.cg_multigrid_observed ? fmt_i(in.mg_levels) : "null");
+  num("mg_mode", in.mg_mode && *in.mg_mode ? "\"" + esc(in.mg_mode) + "\"" : "null");
+  num("galerkin_block_cache", b(1));  // the level-1 product always uses the 8 S_c blocks
+  num("mixed_precision", b(in.mixed_precision));
+  num("matfree_threads", fmt_i(in.matfree_threads));
+  num("krylov_recycling", b(0));
+  num("krylov_recycle_dim", fmt_i(0));
+  num("warm_start_inherit", b(in.warm_start_inherit));
+  num("solved_grid_dofs", fmt_ll(in.solved_grid_dofs));
+  num("projection", b(in.projection));
+  std::string nc = "[", ni = "[", nr = "[";
+  for (int i = 0; i < in.n_rungs; ++i) {
+    const char* sep = i ? ", " : "";
+    nc += sep + b(in.rung_non_convergent ? in.rung_non_convergent[i] : 0);
+    ni += sep + fmt_i(in.rung_non_convergent_iteration ? in.rung_non_convergent_iteration[i] : 0);
+    nr += sep + fmt(in.rung_non_convergent_residual ? in.rung_non_convergent_residual[i] : 0.0);
+  }
+  num("rung_non_convergent", nc + "]");
+  num("rung_non_convergent_iteration", ni + "]");
+  num("rung_non_convergent_residual", nr + "]");
+  std::string ladder = "[";
+  for (int i = 0; i < in.n_ladder; ++i) ladder += (i ? ", " : "") + fmt(in.ladder[i]);
+  num("ladder", ladder + "]");
+  str("ladder_mode", in.ladder_mode);
+  num("created_wall_ms", fmt_ll(in.created_wall_ms));
+  num("iteration_csv", b(in.iteration_csv));
+  num("density_snapshots", b(in.density_snapshots));
+  num("snapshot_every", fmt_i(in.snapshot_every));
+  num("snapshot_cap", fmt_i(in.snapshot_cap), false);
+  s += "}\n";
+  return s;
+}
+
 std::string variant_line(double vf, double achieved, double printed, double margin,
                          const std::string& mesh) {
   char buf[256];

@@ -94,6 +94,8 @@ void write_fields_file(const std::string& path, const topohip_grid_frame& g, int
                        const topohip_fields_variant* v);
 
 std::string progress_line(std::size_t rung, std::size_t rungs, int iteration);
+// run_info.json (the subset this library's solve knows), reference key order.
+std::string run_info_json(const topohip_run_info& info);
 std::string variant_line(double vf, double achieved, double printed, double margin,
                          const std::string& mesh);
 

@@ -55,6 +55,7 @@ void job_bcs(const topohip_job& j, std::vector<int32_t>& bn, std::vector<int32_t
 struct JobOut {
   std::unique_ptr<CsvWriter> csv;
   std::FILE* log = nullptr;
+  int used_multigrid = 0, mg_levels = 0;  // of the last solve the observer saw
   ~JobOut() {
     if (log) std::fclose(log);
   }
@@ -62,6 +63,8 @@ struct JobOut {
 
 void observe_job(const topohip_iteration_obs* o, void* user) {
   auto* out = static_cast<JobOut*>(user);
+  out->used_multigrid = o->cg_used_multigrid;
+  out->mg_levels = o->cg_mg_levels;
   out->csv->append_at(0, *o, wall_clock_ms());
   const std::string line = progress_line(0, 1, o->iteration);
   std::fputs(line.c_str(), out->log);
@@ -73,7 +76,41 @@ void observe_job(const topohip_iteration_obs* o, void* user) {
 // VARIANT line. No margin gate runs here: the margin fields are 0 and the
 // variant counts as accepted; no mesh is exported.
 void write_job_records(Problem& P, const topohip_job& j, const topohip_result& r,
-                       const std::vector<double>& xphys, const std::string& dir, JobOut& jo) {
+                       const std::vector<double>& xphys, const std::string& dir, JobOut& jo,
+                       long long created_ms) {
+  // run_info.json: the record of how this job was solved
+  {
+    const int nc = r.non_convergent ? 1 : 0;
+    const double res = 0.0;
+    const double ladder = j.volfrac;
+    topohip_run_info ri{};
+    ri.cli_version = topohip_version();
+    ri.fingerprint = "dev";
+    ri.mode = "minimize_plastic";
+    ri.load_source = "loadcase";
+    ri.solver = "Hip";
+    ri.cg_multigrid_observed = r.iterations > 0 ? 1 : 0;
+    ri.cg_multigrid = jo.used_multigrid;
+    ri.mg_levels = jo.mg_levels;
+    ri.mixed_precision = j.precision == TOPOHIP_PREC_MIXED ? 1 : 0;
+    ri.solved_grid_dofs = P.g.ndof();
+    ri.n_rungs = 1;
+    ri.rung_non_convergent = &nc;
+    const int nci = nc ? r.iterations : 0;
+    ri.rung_non_convergent_iteration = &nci;
+    ri.rung_non_convergent_residual = &res;
+    ri.n_ladder = 1;
+    ri.ladder = &ladder;
+    ri.ladder_mode = "reduction";
+    ri.created_wall_ms = created_ms;
+    ri.iteration_csv = 1;
+    const std::string js = run_info_json(ri);
+    std::FILE* f = std::fopen((dir + "/run_info.json").c_str(), "wb");
+    if (f) {
+      std::fwrite(js.data(), 1, js.size(), f);
+      std::fclose(f);
+    }
+  }
   const size_t nv = xphys.size(), nd = static_cast<size_t>(P.g.ndof());
   // A run whose final solve did not converge has no certified field: it gets
   // design.bin only (the reference's run leaves such a rung out of fields.bin).
@@ -165,6 +202,7 @@ struct topohip_server {
         std::lock_guard<std::mutex> lk(mu);
         if (!out_dir.empty()) dir = out_dir + "/job_" + std::to_string(id);
       }
+      const long long created_ms = wall_clock_ms();
       if (!dir.empty()) {
         jo = std::make_unique<JobOut>();
         make_dirs(dir);
@@ -175,7 +213,7 @@ struct topohip_server {
         P->observer_user = jo.get();
       }
       optimize(*P, o, nullptr, xp.data(), nullptr, &r);
-      if (jo) write_job_records(*P, j, r, xp, dir, *jo);
+      if (jo) write_job_records(*P, j, r, xp, dir, *jo, created_ms);
     } catch (const Error& e) {
       status = e.code;
       err = e.what();

@@ -7,6 +7,8 @@
                        core/include/topopt/design_store.hpp:40-100
   fields.bin v1        write_fields_file, core/include/topopt/fields.hpp:28-70
   PROGRESS / VARIANT   the CLI stdout lines, core/src/cli/run_job.cpp:10223-10259
+  run_info.json        run_info_json / write_run_info (the keys of this solve),
+                       core/src/simp/observability.cpp:476-1284, 1761-1767
 
 The writers live in libtopopt_hip.so (formats.cpp); the arrays they carry are
 formed on the device (Problem.fields_f32, Problem.density_f16).
@@ -220,6 +222,61 @@ def write_fields_file(path, grid, variants):
                 setattr(arr[i], k, a.ctypes.data)
     _check(_setup().topohip_write_fields_file(str(path).encode(), C.byref(grid), len(variants),
                                               arr))
+
+
+class RunInfo(C.Structure):
+    """topohip_run_info: the run_info.json keys this library writes."""
+    _fields_ = [
+        ("cli_version", C.c_char_p), ("fingerprint", C.c_char_p), ("mode", C.c_char_p),
+        ("material", C.c_char_p), ("source_format", C.c_char_p), ("resolution", C.c_int),
+        ("load_source", C.c_char_p), ("solver", C.c_char_p),
+        ("cg_multigrid_observed", C.c_int), ("cg_multigrid", C.c_int), ("mg_levels", C.c_int),
+        ("mg_mode", C.c_char_p), ("mixed_precision", C.c_int), ("matfree_threads", C.c_int),
+        ("warm_start_inherit", C.c_int), ("solved_grid_dofs", C.c_longlong),
+        ("projection", C.c_int), ("n_rungs", C.c_int),
+        ("rung_non_convergent", C.c_void_p), ("rung_non_convergent_iteration", C.c_void_p),
+        ("rung_non_convergent_residual", C.c_void_p), ("n_ladder", C.c_int),
+        ("ladder", C.c_void_p), ("ladder_mode", C.c_char_p), ("created_wall_ms", C.c_longlong),
+        ("iteration_csv", C.c_int), ("density_snapshots", C.c_int), ("snapshot_every", C.c_int),
+        ("snapshot_cap", C.c_int),
+    ]
+
+
+def _run_info(kw):
+    """RunInfo from keyword values; the arrays are kept alive on the struct."""
+    ri = RunInfo()
+    keep = []
+    for k, v in kw.items():
+        if k in ("ladder", "rung_non_convergent", "rung_non_convergent_iteration",
+                 "rung_non_convergent_residual"):
+            dt = np.float64 if k in ("ladder", "rung_non_convergent_residual") else np.int32
+            a = np.ascontiguousarray(v, dtype=dt)
+            keep.append(a)
+            setattr(ri, k, a.ctypes.data)
+            if k == "ladder":
+                ri.n_ladder = a.size
+            else:
+                ri.n_rungs = a.size
+        elif isinstance(v, str):
+            setattr(ri, k, v.encode())
+        else:
+            setattr(ri, k, v)
+    ri._keep = keep
+    return ri
+
+
+def run_info_json(**kw):
+    """run_info.json text of the given RunInfo fields (names as topohip_run_info)."""
+    ri = _run_info(kw)
+    n = C.c_size_t(0)
+    buf = C.create_string_buffer(1 << 16)
+    _check(_setup().topohip_run_info_json(C.byref(ri), buf, C.c_size_t(1 << 16), C.byref(n)))
+    return buf.value.decode()
+
+
+def write_run_info(path, **kw):
+    ri = _run_info(kw)
+    _check(_setup().topohip_write_run_info(str(path).encode(), C.byref(ri)))
 
 
 def format_progress(rung, rungs, iteration):
