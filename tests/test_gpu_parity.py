@@ -106,10 +106,10 @@ def test_solve_cantilever_matches_oracle(precision):
     assert info["mg_levels"] == info_ref["mg_levels"] == 3
     assert info["residual"] <= 1e-8
     assert err < 1e-6
-    if precision == ta.PREC_F64:
-        assert abs(info["iterations"] - info_ref["iterations"]) <= 1
-    else:
-        assert info["iterations"] <= info_ref["iterations"] + 8
+    # both forms apply A.p in FP64, so the FP32 V-cycle of the mixed form keeps
+    # the FP64 form's iteration count (the reference's own mixed-precision gate
+    # is the looser iters32 <= iters64 + 8)
+    assert abs(info["iterations"] - info_ref["iterations"]) <= 1
 
 
 def test_solve_soft_void_field():

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <functional>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -876,29 +877,35 @@ void autotune_ku(Problem& P) {
   hipEvent_t e0, e1;
   TOPOHIP_CHECK(hipEventCreate(&e0));
   TOPOHIP_CHECK(hipEventCreate(&e1));
+  // A PCG iteration applies the fine operator once in FP64 (A.p) and, in the
+  // mixed form, twice in FP32 (the V-cycle's sweeps; FP64 in the FP64 form):
+  // each z-chunk is scored by that mix.
   float best = 1e30f;
   const int tzs[4] = {4, 8, 16, 32};
   for (int tz : tzs) {
-    auto run = [&]() {
-      if (mixed) {
-        KuOp<float> op;
-        op.g = P.g; op.E = P.E32.p; op.mg = P.mg32; op.tz = tz;
-        op.x = P.r32.p; op.y = P.z32.p; op.epi = EPI_AX;  // (scratch at set-up)
-        launch_ku_fine<float>(op, P.s);
-      } else {
-        KuOp<double> op;
-        op.g = P.g; op.E = P.E64.p; op.mg = P.mg64; op.tz = tz;
-        op.x = P.x.p; op.y = P.Ap64.p; op.epi = EPI_AX;
-        launch_ku_fine<double>(op, P.s);
-      }
+    auto run32 = [&]() {
+      KuOp<float> op;
+      op.g = P.g; op.E = P.E32.p; op.mg = P.mg32; op.tz = tz;
+      op.x = P.r32.p; op.y = P.z32.p; op.epi = EPI_AX;  // (scratch at set-up)
+      launch_ku_fine<float>(op, P.s);
     };
-    run();
-    TOPOHIP_CHECK(hipEventRecord(e0, P.s));
-    for (int i = 0; i < 5; ++i) run();
-    TOPOHIP_CHECK(hipEventRecord(e1, P.s));
-    TOPOHIP_CHECK(hipEventSynchronize(e1));
-    float ms = 0.f;
-    TOPOHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    auto run64 = [&]() {
+      KuOp<double> op;
+      op.g = P.g; op.E = P.E64.p; op.mg = P.mg64; op.tz = tz;
+      op.x = P.x.p; op.y = P.Ap64.p; op.epi = EPI_AX;
+      launch_ku_fine<double>(op, P.s);
+    };
+    auto timed = [&](const std::function<void()>& run) {
+      run();
+      TOPOHIP_CHECK(hipEventRecord(e0, P.s));
+      for (int i = 0; i < 5; ++i) run();
+      TOPOHIP_CHECK(hipEventRecord(e1, P.s));
+      TOPOHIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      TOPOHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      return ms;
+    };
+    const float ms = mixed ? timed(run64) + 2.0f * timed(run32) : 3.0f * timed(run64);
     if (ms < best) { best = ms; P.tz = tz; }
   }
   (void)hipEventDestroy(e0);
