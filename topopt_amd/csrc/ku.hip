@@ -3,8 +3,9 @@
 // Replaces the colour-ordered element loop of core/src/fea/matfree.cpp:75-123,
 // 535-553 (gather 24 / 24x24 column AXPY / scatter 24). Design for gfx950:
 //
-//  * A workgroup of TX x TY lanes (256 or 512) owns a (TX-1) x (TY-1) tile of
-//    nodes in x-y and marches a z-chunk of node planes. Each lane handles one
+//  * A workgroup of kTX x kTY = 32 x 8 lanes owns a 31 x 7 tile of nodes in
+//    x-y and marches a z-chunk of node planes (a multiple of 4, autotuned per
+//    grid and precision). Each lane handles one
 //    element column (one halo element on the low x and y side); node planes
 //    are staged in LDS, double buffered, with the next plane (and the next
 //    layer's moduli, masks and epilogue operands) prefetched into registers
