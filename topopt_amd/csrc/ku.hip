@@ -161,31 +161,34 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
   const int noff = owner ? nb * g.nnx + na : 0;
 
   T pf[NPF], pd[NPF];  // plane values in flight (pd: D^-1 of EPI_RES0)
+  bool pin = false;    // EPI_RES0: the plane in flight lies inside the grid
   auto fetch_plane = [&](int c, T (&pf)[NPF]) {
-    if (c >= 0 && c < g.nnz) {
-      if constexpr (EPI == EPI_RES0) {
-        // one damped-Jacobi sweep from zero, formed on the fly: x0 = omega D^-1 b
-        const T* bp = bs + c * pn * 3;
-        const T* dp = ds + c * pn * 3;
+    if constexpr (EPI == EPI_RES0) {
+      // one damped-Jacobi sweep from zero, formed on the fly: x0 = omega D^-1 b.
+      // The plane index is clamped into the grid and a plane outside it is
+      // zeroed where the values are used, so pd is written in one place only.
+      pin = c >= 0 && c < g.nnz;
+      const int cc = min(max(c, 0), g.nnz - 1);
+      const T* bp = bs + cc * pn * 3;
+      const T* dp = ds + cc * pn * 3;
 #pragma unroll
-        for (int i = 0; i < NPF; ++i) {
-          pf[i] = bp[foff[i]];
-          pd[i] = dp[foff[i]];
-        }
-      } else {
-        const T* xp = xs + c * pn * 3;
-#pragma unroll
-        for (int i = 0; i < NPF; ++i) pf[i] = (valid >> i) & 1u ? xp[foff[i]] : T(0);
+      for (int i = 0; i < NPF; ++i) {
+        pf[i] = bp[foff[i]];
+        pd[i] = dp[foff[i]];
       }
+    } else if (c >= 0 && c < g.nnz) {
+      const T* xp = xs + c * pn * 3;
+#pragma unroll
+      for (int i = 0; i < NPF; ++i) pf[i] = (valid >> i) & 1u ? xp[foff[i]] : T(0);
     } else {
 #pragma unroll
-      for (int i = 0; i < NPF; ++i) pf[i] = pd[i] = T(0);
+      for (int i = 0; i < NPF; ++i) pf[i] = T(0);
     }
   };
   // The loaded values are first used here, a layer after their loads were
   // issued. EPI_RES0 loads b and D^-1 unconditionally at in-grid addresses
-  // and zeroes halo positions outside the grid here (measured faster than
-  // predicated loads for that variant; the others load predicated).
+  // and zeroes halo positions and planes outside the grid here (measured
+  // faster than predicated loads for that variant; the others load predicated).
   auto store_plane = [&](int buf, const T (&pf)[NPF]) {
 #pragma unroll
     for (int i = 0; i < NPF; ++i) {
@@ -193,7 +196,7 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
       T v = pf[i];
       if constexpr (EPI == EPI_RES0) {
         v = static_cast<T>(a.omega_d * static_cast<double>(pd[i]) * static_cast<double>(v));
-        v = (valid >> i) & 1u ? v : T(0);
+        v = ((valid >> i) & 1u) && pin ? v : T(0);
       }
       if (i + 1 < NPF || q < PLANE) s_u[buf][q] = v;
     }
