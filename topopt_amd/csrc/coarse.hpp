@@ -69,7 +69,12 @@ class CoarseCR {
   // Dense tail: levels >= dl are solved with the explicit inverse of level
   // dl's system (nden x nden at W + xm_dense), formed by the GEMM jobs
   // d_mj (one launch per lv_mj range) from the identity at W + eye_off.
-  static constexpr long long kDenseMax = 2048;
+  // 1024 balances the inverse's formation (once per hierarchy build, its
+  // GEMMs scale with the tail size) against the GEMV launches it saves in
+  // every V-cycle: at 192x64x64 (planes of 243 unknowns) the tail starts at
+  // 4 planes (972 unknowns); 2048 (7 planes) and 512 (2 planes) measured
+  // 2 % slower per design iteration.
+  static constexpr long long kDenseMax = 1024;
   int dl = 0;
   long long nden = 0, xm_dense = 0, eye_off = 0;
   std::vector<int> lv_mj;
