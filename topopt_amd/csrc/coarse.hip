@@ -130,21 +130,28 @@ inv_step_k(double* W, const long long* offs, int m, int step, int nsteps, double
 #pragma unroll
   for (int q = 0; q < 4; ++q) Sw[tr][tc + q] = e[q];
   __syncthreads();
-  // U_I = -A_IB Sw, U_J = -A_JB Sw
-  double si[4] = {0.0, 0.0, 0.0, 0.0}, sj[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int k = 0; k < IV_NB; ++k) {
-    const double a = Ai[tr][k], c = AjT[k][tr];
+  // U_I = -A_IB Sw (every tile but the pivot row's), U_J = -A_JB Sw (the
+  // pivot row's tiles only). The branches are uniform over the workgroup.
+  const bool need_ui = I != step, need_uj = I == step && J != step;
+  if (need_ui) {
+    double si[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < IV_NB; ++k) {
+      const double a = Ai[tr][k];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const double w = Sw[k][tc + q];
-      si[q] -= a * w;
-      sj[q] -= c * w;
+      for (int q = 0; q < 4; ++q) si[q] -= a * Sw[k][tc + q];
     }
-  }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    Ui[tr][tc + q] = si[q];
-    Uj[tr][tc + q] = sj[q];
+    for (int q = 0; q < 4; ++q) Ui[tr][tc + q] = si[q];
+  }
+  if (need_uj) {
+    double sj[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < IV_NB; ++k) {
+      const double c = AjT[k][tr];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sj[q] -= c * Sw[k][tc + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Uj[tr][tc + q] = sj[q];
   }
   __syncthreads();
   double v[4];
