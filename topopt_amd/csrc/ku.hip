@@ -103,6 +103,10 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
   constexpr int PW = TX + 1;                  // staged plane: PW x (TY + 1) nodes
   constexpr int PLANE = PW * (TY + 1) * 3;
   constexpr int NPF = (PLANE + NT - 1) / NT;  // prefetch registers per lane
+  // re-read plane k from LDS (see step): the FP64 residual and smoothing
+  // variants, which gain a wave per SIMD by it (the FP64 plain and A.p
+  // applies measured no gain or slower)
+  constexpr bool RECOMP = sizeof(T) == 8 && EPI != EPI_AX;
   const Geom g = a.g;
   // Logical tile (x fastest, then y, z-chunk, load case) of this workgroup,
   // XCD-contiguous so that tiles sharing halo planes share an L2.
@@ -273,6 +277,10 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
   // plane k's outputs (whose other half came from layer k - 1 through Vt) are
   // finished. Fp / xo / e / oc hold plane k's face, own values, layer k's
   // modulus and plane k's owner operands; the *n arguments receive plane k + 1's.
+  // With RECOMP plane k's face and own values are not carried from the
+  // previous step but formed again from LDS buffer cur, which still holds
+  // plane k: 15 LDS reads and 24 adds more per step for 30 fewer registers
+  // carried through the modal core.
   auto step = [&](int k, int cur, T (&Fp)[12], T (&Fn)[12], const T (&xo)[3], T (&xn)[3],
                   const T& e, T& en, const Own& oc, Own& on) {
     const int nxt = cur ^ 1;
@@ -284,9 +292,9 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
     if constexpr (DOTK != 0) {
       if (pending >= 0) flush_pending();
     }
-    if (k == k0 - 1) load_face(cur, Fp);
+    if (RECOMP || k == k0 - 1) load_face(cur, Fp);
     load_face(nxt, Fn);
-    load_own(nxt, xn);
+    if constexpr (!RECOMP) load_own(nxt, xn);
     T A[24], v[24];
 #pragma unroll
     for (int q = 0; q < 12; ++q) {
@@ -299,6 +307,14 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
     for (int q = 0; q < 12; ++q) {
       G[q] = Vt[q] + (v[q] + v[12 + q]);
       Vt[q] = v[q] - v[12 + q];
+    }
+    T xk[3];  // plane k's own values (read before the barrier below: the next
+              // step's plane store may overwrite buffer cur after it)
+    if constexpr (RECOMP) {
+      load_own(cur, xk);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) xk[c] = xo[c];
     }
     if (k >= k0) {
       // Inverse x-y butterfly of plane k's face, corner results to LDS.
@@ -334,14 +350,14 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
         } else if constexpr (EPI == EPI_RES || EPI == EPI_RES0) {
           out = oc.b[c] - ax[c];
         } else {
-          out = fma(a.omega * oc.d[c], oc.b[c] - ax[c], xo[c]);
+          out = fma(a.omega * oc.d[c], oc.b[c] - ax[c], xk[c]);
         }
         out = kept ? out : T(0);
         if constexpr (DOTK == 1)
-          dotp = fma(static_cast<double>(xo[c]), kept ? static_cast<double>(ax[c]) : 0.0, dotp);
+          dotp = fma(static_cast<double>(xk[c]), kept ? static_cast<double>(ax[c]) : 0.0, dotp);
         if constexpr (DOTK == 2) dotp = fma(oc.r[c], static_cast<double>(out), dotp);
         yp[c] = out;
-        if constexpr (EPI == EPI_RES0) xwp[c] = xo[c];
+        if constexpr (EPI == EPI_RES0) xwp[c] = xk[c];
       }
     }
     if constexpr (DOTK != 0) {
@@ -361,10 +377,12 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
   // per SIMD in FP64.)
   for (int k = k0 - 1, cur = 0; k < kend; ++k, cur ^= 1) {
     step(k, cur, F0, F1, X0, X1, e0, e1, O0, O1);
+    if constexpr (!RECOMP) {
 #pragma unroll
-    for (int q = 0; q < 12; ++q) F0[q] = F1[q];
+      for (int q = 0; q < 12; ++q) F0[q] = F1[q];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) X0[q] = X1[q];
+      for (int q = 0; q < 3; ++q) X0[q] = X1[q];
+    }
     e0 = e1;
     O0 = O1;
   }
