@@ -104,9 +104,10 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
   constexpr int PLANE = PW * (TY + 1) * 3;
   constexpr int NPF = (PLANE + NT - 1) / NT;  // prefetch registers per lane
   // re-read plane k from LDS (see step): the FP64 residual and smoothing
-  // variants, which gain a wave per SIMD by it (the FP64 plain and A.p
-  // applies measured no gain or slower)
-  constexpr bool RECOMP = sizeof(T) == 8 && EPI != EPI_AX;
+  // variants and the FP32 sweep from zero, which gain a wave per SIMD by it
+  // (the FP32 sweep: 24.0 -> 20.9 us at 192x64x64, 574 -> 593 us at
+  // 512x256x256; the FP64 plain and A.p applies measured no gain or slower)
+  constexpr bool RECOMP = (sizeof(T) == 8 && EPI != EPI_AX) || EPI == EPI_RES0;
   const Geom g = a.g;
   // Logical tile (x fastest, then y, z-chunk, load case) of this workgroup,
   // XCD-contiguous so that tiles sharing halo planes share an L2.
