@@ -66,11 +66,12 @@ def ku_bytes(nx, ny, nz, variant):
     ndof = 3 * (nx + 1) * (ny + 1) * (nz + 1)
     nv = nx * ny * nz
     w = 8 if variant in ("f64", "ap64_dot") else 4
-    b = 2 * w * ndof + w * nv                       # x read, y written, E read
     if variant == "res0_f32":
-        b += 3 * w * ndof                           # b, D^-1 read; x0 written
-    if variant == "jac_dot_f32":
-        b += 3 * w * ndof + 8 * ndof                # x, b, D^-1 read; FP64 r read
+        b = 4 * w * ndof + w * nv                   # b, D^-1 read; y, x0 written; E read
+    elif variant == "jac_dot_f32":
+        b = 4 * w * ndof + 8 * ndof + w * nv        # x, b, D^-1 read; y written; FP64 r read; E
+    else:
+        b = 2 * w * ndof + w * nv                   # x read, y written, E read
     return b, ndof
 
 
