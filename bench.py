@@ -349,6 +349,7 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 512x256x256 leg")
     ap.add_argument("--jobs", type=int, default=512)
     ap.add_argument("--rate", type=float, default=4.0)
+    ap.add_argument("--slots", type=int, default=16, help="serving: concurrent jobs admitted")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--ku-only", action="store_true",
                     help="only the K.u microbenchmark, after a 256 MiB device copy that "
@@ -366,7 +367,7 @@ def main():
 
     if a.workload == "serving":
         log(f"# serving: {a.jobs} jobs, Poisson rate {a.rate}/s, seed {a.seed}")
-        sv = run_serving(ta, a.jobs, a.rate, a.seed, iterations=60)
+        sv = run_serving(ta, a.jobs, a.rate, a.seed, iterations=60, slots=a.slots)
         print(json.dumps({
             "metric": "serving: aggregate design iterations/s (job p50/p99 latency in config)",
             "value": sv["design_iters_per_s"], "unit": "design iters/s", "n_gpus": 1,
@@ -442,7 +443,7 @@ def main():
     serving = large = None
     if not a.no_serving and a.workload == "mbb_192x64x64":
         log(f"# serving: {a.jobs} jobs, Poisson rate {a.rate}/s, seed {a.seed}")
-        serving = run_serving(ta, a.jobs, a.rate, a.seed, iterations=60)
+        serving = run_serving(ta, a.jobs, a.rate, a.seed, iterations=60, slots=a.slots)
         log(f"# serving: {serving['design_iters_per_s']:.1f} design iters/s, p50 "
             f"{serving['p50_ms']:.0f} ms, p99 {serving['p99_ms']:.0f} ms")
     if not a.no_large and a.workload == "mbb_192x64x64":
