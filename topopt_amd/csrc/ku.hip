@@ -104,10 +104,16 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
   constexpr int PLANE = PW * (TY + 1) * 3;
   constexpr int NPF = (PLANE + NT - 1) / NT;  // prefetch registers per lane
   // re-read plane k from LDS (see step): the FP64 residual and smoothing
-  // variants and the FP32 sweep from zero, which gain a wave per SIMD by it
-  // (the FP32 sweep: 24.0 -> 20.9 us at 192x64x64, 574 -> 593 us at
+  // variants and the FP32 sweeps, which gain a wave per SIMD by it (the FP32
+  // sweep from zero: 24.0 -> 20.9 us at 192x64x64, 574 -> 593 us at
   // 512x256x256; the FP64 plain and A.p applies measured no gain or slower)
-  constexpr bool RECOMP = (sizeof(T) == 8 && EPI != EPI_AX) || EPI == EPI_RES0;
+  constexpr bool RECOMP = (sizeof(T) == 8 && EPI != EPI_AX) || EPI == EPI_RES0 ||
+                          (sizeof(T) == 4 && EPI == EPI_JAC);
+  // LATE: the FP32 Jacobi sweep loads plane k's owner operands (mask, b, D^-1,
+  // the FP64 r) in step k after the modal core, in flight across the corner
+  // exchange, instead of a step ahead: with RECOMP 120 -> 81 registers, five
+  // waves per SIMD (24.1 -> 20.1 us at 192x64x64, 597 -> 576 us at 512x256x256)
+  constexpr bool LATE = sizeof(T) == 4 && EPI == EPI_JAC;
   const Geom g = a.g;
   // Logical tile (x fastest, then y, z-chunk, load case) of this workgroup,
   // XCD-contiguous so that tiles sharing halo planes share an L2.
@@ -288,7 +294,7 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
     store_plane(nxt, pf);  // plane k + 1
     if (k + 1 < kend) fetch_plane(k + 2, pf);
     en = modulus(k + 1);
-    fetch_owner(k + 1, on);
+    if constexpr (!LATE) fetch_owner(k + 1, on);
     __syncthreads();
     if constexpr (DOTK != 0) {
       if (pending >= 0) flush_pending();
@@ -317,6 +323,11 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) xk[c] = xo[c];
     }
+    Own ol{};
+    if constexpr (LATE) {
+      if (k >= k0) fetch_owner(k, ol);
+    }
+    const Own& ow = LATE ? ol : oc;
     if (k >= k0) {
       // Inverse x-y butterfly of plane k's face, corner results to LDS.
 #pragma unroll
@@ -344,19 +355,19 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
                 s_c[0][t11 * 3 + c];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const bool kept = (oc.mk >> c) & 1;
+        const bool kept = (ow.mk >> c) & 1;
         T out;
         if constexpr (EPI == EPI_AX) {
           out = ax[c];
         } else if constexpr (EPI == EPI_RES || EPI == EPI_RES0) {
-          out = oc.b[c] - ax[c];
+          out = ow.b[c] - ax[c];
         } else {
-          out = fma(a.omega * oc.d[c], oc.b[c] - ax[c], xk[c]);
+          out = fma(a.omega * ow.d[c], ow.b[c] - ax[c], xk[c]);
         }
         out = kept ? out : T(0);
         if constexpr (DOTK == 1)
           dotp = fma(static_cast<double>(xk[c]), kept ? static_cast<double>(ax[c]) : 0.0, dotp);
-        if constexpr (DOTK == 2) dotp = fma(oc.r[c], static_cast<double>(out), dotp);
+        if constexpr (DOTK == 2) dotp = fma(ow.r[c], static_cast<double>(out), dotp);
         yp[c] = out;
         if constexpr (EPI == EPI_RES0) xwp[c] = xk[c];
       }
@@ -385,7 +396,7 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
       for (int q = 0; q < 3; ++q) X0[q] = X1[q];
     }
     e0 = e1;
-    O0 = O1;
+    if constexpr (!LATE) O0 = O1;
   }
   if constexpr (DOTK != 0) {
     __syncthreads();
