@@ -138,18 +138,26 @@ def run_design(ta, wl, precision, steps, warmup, mma=False, smoother=None):
 KU_LOOP = ("ap64_dot", "res0_f32", "jac_dot_f32")
 
 
+def variant_tile(tiles, variant):
+    """Launch tile (tx, ty, tz) of a K.u variant: FP64 variants run on the
+    FP64 applies' z-chunk, FP32 ones on the FP32 applies'."""
+    return tiles["f64"] if variant in ("f64", "ap64_dot") else tiles["f32"]
+
+
 def ku_measure(ta, nx, ny, nz, reps=100, variants=("f32", "f64") + KU_LOOP):
     res = {}
     with ta.Problem(nx, ny, nz) as p:
         p.bench_prepare(0x5EED)
-        tile = p.ku_tile()
+        tx, ty, _ = p.ku_tile()
+        tz32, tz64 = p.ku_tiles()
+        tiles = {"f32": [tx, ty, tz32], "f64": [tx, ty, tz64]}
         for v in variants:
             ms, sets = p.bench_ku_variant(v, reps)
             b, ndof = ku_bytes(nx, ny, nz, v)
             gbs = b / (ms * 1e-3) / 1e9
             res[v] = {"us": ms * 1e3, "GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS, "bytes": b,
                       "gdof_s": ndof / (ms * 1e-3) / 1e9, "rotated_sets": sets}
-    return res, tile
+    return res, tiles
 
 
 def fields_measure(ta, nx, ny, nz, reps=50):
@@ -236,12 +244,12 @@ def run_large(ta, problems, iterations=3):
                          continuation=wl["continuation"])
         dt = time.perf_counter() - t0
         m1 = hip_mem_used()
-    ku, tile = ku_measure(ta, nx, ny, nz, reps=10, variants=("f32", "f64") + KU_LOOP)
+    ku, tiles = ku_measure(ta, nx, ny, nz, reps=10, variants=("f32", "f64") + KU_LOOP)
     return {"workload": "cantilever_512x256x256", "dof": 3 * (nx + 1) * (ny + 1) * (nz + 1),
             "iterations": out["iterations"], "design_iters_per_s": out["iterations"] / dt,
             "cg_iterations_total": out["cg_iterations_total"], "compliance": out["compliance"],
             "device_bytes_held": None if m0 is None or m1 is None else m1 - m0,
-            "ku_tile": list(tile), "ku": ku,
+            "ku_tile": tiles, "ku": ku,
             "note": f"{iterations} iterations from the uniform design (continuation stage p = 1) "
                     f"after one warm-up iteration; the full config runs 150"}
 
@@ -389,8 +397,8 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         variants = (a.variant,) if a.variant else ("f32", "f64") + KU_LOOP
-        ku, tile = ku_measure(ta, nx, ny, nz, reps=10, variants=variants)
-        print(json.dumps({"workload": a.workload, "tile": list(tile), "ku": ku,
+        ku, tiles = ku_measure(ta, nx, ny, nz, reps=10, variants=variants)
+        print(json.dumps({"workload": a.workload, "tile": tiles, "ku": ku,
                           "calibration_copy_bytes": 256 << 20}), flush=True)
         return
     log(f"# workload {a.workload}: warmup {a.warmup}, timed {a.steps} design iterations")
@@ -428,9 +436,9 @@ def main():
                 "cg_iterations_total": outc["cg_iterations_total"]}
         log(f"# chebyshev(2) smoother: {cheb['value']:.2f} design iters/s, "
             f"cg iters {cheb['cg_iterations_total']} (Jacobi {out['cg_iterations_total']})")
-    ku, tile = ku_measure(ta, nx, ny, nz)
+    ku, tiles = ku_measure(ta, nx, ny, nz)
     log("# K.u " + ", ".join(f"{k} {v['us']:.1f} us {v['GBs']:.0f} GB/s" for k, v in ku.items())
-        + f" (tile {tile})")
+        + f" (tiles {tiles})")
     fields = fields_measure(ta, nx, ny, nz)
     log("# fields " + ", ".join(f"{k} {v['us']:.1f} us {v['GBs']:.0f} GB/s"
                                 for k, v in fields.items()))
@@ -471,7 +479,7 @@ def main():
                    "cg_iterations_total": out["cg_iterations_total"],
                    "t_solve_ms": out["t_solve_ms"], "t_build_ms": out["t_build_ms"],
                    "t_oc_ms": out["t_oc_ms"], "t_filter_ms": out["t_filter_ms"],
-                   "ku_tile": list(tile), "oc_decision_gap_min_rel": out["oc_gap"][1],
+                   "ku_tile": tiles, "oc_decision_gap_min_rel": out["oc_gap"][1],
                    "oc_rounding_bound_rel": out["oc_gap"][2]},
         "fp64_form": fp64,
         "vs_oracle": par,
@@ -481,13 +489,13 @@ def main():
         "result_fields": fields,
         "roofline": {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom["hbm_frac"],
-                     "traffic": measured_traffic(a.workload, "ap64_dot", tile),
+                     "traffic": measured_traffic(a.workload, "ap64_dot", tiles["f64"]),
                      "kernel": "ku_face<double> EPI_AX + <p,Ap> partials (the PCG's A.p)",
-                     "tile": list(tile), "bytes_per_launch": dom["bytes"],
+                     "tile": tiles["f64"], "bytes_per_launch": dom["bytes"],
                      "us_per_launch": dom["us"]},
         "roofline_loop": {k: {"frac": ku[k]["hbm_frac"], "GBs": ku[k]["GBs"], "us": ku[k]["us"],
                               "bytes": ku[k]["bytes"],
-                              "traffic": measured_traffic(a.workload, k, tile)}
+                              "traffic": measured_traffic(a.workload, k, variant_tile(tiles, k))}
                           for k in KU_LOOP},
         "cpu_baseline": cpu,
         "serving": serving,

@@ -292,13 +292,19 @@ int topohip_bench_ku(topohip_problem* p, int precision, int reps, double* ms_per
  * damped-Jacobi sweep with the fused <r, z> partials (post-smoothing). Every
  * launch cycles to the next of *sets_used (nullable) operand sets, enough that
  * they span more than twice the 256 MiB last-level cache, so inputs come from
- * HBM. The launch shape is the handle's tile (topohip_ku_tile). No reference
- * counterpart (topohip_bench_ku is variants 0 / 1). */
+ * HBM. The launch shape is the handle's tile of that precision
+ * (topohip_ku_tiles). No reference counterpart (topohip_bench_ku is variants
+ * 0 / 1). */
 int topohip_bench_ku_variant(topohip_problem* p, int variant, int reps, double* ms_per_apply,
                              int* sets_used);
-/* The fine K.u launch shape the handle uses: tx x ty lanes per workgroup over a
- * z-chunk of tz node planes (autotuned at creation, or TOPOHIP_KU_TILE). */
+/* The fine K.u launch shape of the handle's FP64 applies (the PCG's A.p):
+ * tx x ty lanes per workgroup over a z-chunk of tz node planes (autotuned at
+ * creation, or TOPOHIP_KU_TILE). No reference counterpart. */
 int topohip_ku_tile(topohip_problem* p, int* tx, int* ty, int* tz);
+/* The z-chunks of the FP32 and the FP64 applies (multiples of 4 and of 7
+ * planes: the dot-partial slabs of the two precisions). No reference
+ * counterpart. */
+int topohip_ku_tiles(topohip_problem* p, int* tz32, int* tz64);
 /* Average device time (ms) of the result-field kernels over reps launches on
  * the bench inputs (topohip_bench_prepare), rotated over operand sets that
  * span more than twice the last-level cache: ms3[0] von Mises to float32,

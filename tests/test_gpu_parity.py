@@ -45,10 +45,11 @@ def test_ku_apply_matches_oracle(dims):
 
 
 def test_ku_apply_every_tile_shape_is_bit_identical(monkeypatch):
-    """Every launch shape the autotuner may pick (the z-chunk length) gives the
-    same bits: each node sums its 4 face contributions in a fixed order, and
-    the FP64 dot partials are one per fixed slab of planes, so a whole solve
-    (its PCG scalars included) is bit-identical whatever the chunk."""
+    """Every launch shape the autotuner may pick (the z-chunk lengths of the
+    FP32 and the FP64 applies) gives the same bits: each node sums its 4 face
+    contributions in a fixed order, and the FP64 dot partials are one per
+    fixed slab of planes, so a whole solve (its PCG scalars included) is
+    bit-identical whatever the chunks."""
     nx, ny, nz = 70, 20, 19
     rng = np.random.default_rng(11)
     u = rng.uniform(-1, 1, 3 * (nx + 1) * (ny + 1) * (nz + 1))
@@ -57,10 +58,15 @@ def test_ku_apply_every_tile_shape_is_bit_identical(monkeypatch):
     bcs, loads = problems.cantilever(nx, ny, nz)
     Es = rng.uniform(0.2, 1.0, nx * ny * nz) ** 3
     first = None
-    for tile in ["4", "32,8,8", "16", "32"]:
+    # (z-chunk of the FP32 applies : of the FP64 applies); a single value is
+    # taken by each precision whose slab (4 or 7 planes) divides it
+    for tile, want in [("4:7", (4, 7)), ("32,8,8", (8, None)), ("16:21", (16, 21)),
+                       ("32:28", (32, 28)), ("28", (28, 28))]:
         monkeypatch.setenv("TOPOHIP_KU_TILE", tile)
         with ta.Problem(nx, ny, nz) as p:
-            assert p.ku_tile()[2] == int(tile.split(",")[-1])
+            tz32, tz64 = p.ku_tiles()
+            assert tz32 == want[0] and (want[1] is None or tz64 == want[1])
+            assert p.ku_tile()[2] == tz64
             y32 = p.apply(E, u, "f32")
             y64 = p.apply(E, u, "f64")
         with ta.Problem(nx, ny, nz, bcs, loads) as p:

@@ -485,8 +485,15 @@ class Problem:
         return ms.value, ns.value
 
     def ku_tile(self):
+        """(tx, ty, tz) of the FP64 applies (the PCG's A.p)."""
         t = [C.c_int(0), C.c_int(0), C.c_int(0)]
         _check(lib().topohip_ku_tile(self._h, *[C.byref(v) for v in t]))
+        return tuple(v.value for v in t)
+
+    def ku_tiles(self):
+        """(z-chunk of the FP32 applies, z-chunk of the FP64 applies)."""
+        t = [C.c_int(0), C.c_int(0)]
+        _check(lib().topohip_ku_tiles(self._h, *[C.byref(v) for v in t]))
         return tuple(v.value for v in t)
 
     def bench_fields(self, reps=50):

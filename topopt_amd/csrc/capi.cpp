@@ -78,7 +78,7 @@ void apply_host(Problem& P, const double* youngs, const double* u, double* y) {
   TOPOHIP_CHECK(hipMemcpyAsync(u64.p, u, nd * sizeof(double), hipMemcpyHostToDevice, P.s));
   KuOp<T> op;
   op.g = P.g;
-  op.tz = P.tz;
+  op.tz = sizeof(T) == 8 ? P.tz64 : P.tz32;
   op.epi = EPI_AX;
   if constexpr (sizeof(T) == 4) {
     DBuf<float> u32, y32;
@@ -490,7 +490,16 @@ int topohip_ku_tile(topohip_problem* p, int* tx, int* ty, int* tz) {
     const Problem& P = *p->P;
     if (tx) *tx = kTX;
     if (ty) *ty = kTY;
-    if (tz) *tz = P.tz;
+    if (tz) *tz = P.tz64;
+  });
+}
+
+int topohip_ku_tiles(topohip_problem* p, int* tz32, int* tz64) {
+  return guarded([&] {
+    need(p, "problem");
+    const Problem& P = *p->P;
+    if (tz32) *tz32 = P.tz32;
+    if (tz64) *tz64 = P.tz64;
   });
 }
 
@@ -538,7 +547,7 @@ int topohip_bench_ku_variant(topohip_problem* p, int variant, int reps, double* 
     auto op_of = [&](auto& q, auto mg) {
       using T = std::remove_pointer_t<decltype(q.x)>;
       KuOp<T> op;
-      op.g = P.g; op.tz = P.tz; op.mg = mg; op.E = q.E;
+      op.g = P.g; op.tz = sizeof(T) == 8 ? P.tz64 : P.tz32; op.mg = mg; op.E = q.E;
       op.mask = P.L[0]->mask.p;
       op.pstride = P.pstride;
       op.y = q.y;

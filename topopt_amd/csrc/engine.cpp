@@ -439,9 +439,13 @@ void alloc_levels(Problem& P) {
 }
 
 int max_partials(const Problem& P) {
-  return std::max(std::max(ku_partials(P.g), 2 * cheb_blocks(P.g.ndof())),
+  const int ku = std::max(ku_partials<float>(P.g), ku_partials<double>(P.g));
+  return std::max(std::max(ku, 2 * cheb_blocks(P.g.ndof())),
                   std::max(dot_blocks(P.g.ndof()), sens_blocks(P.g)));
 }
+
+template <typename T>
+int ku_tz(const Problem& P) { return sizeof(T) == 8 ? P.tz64 : P.tz32; }
 
 // ---- V-cycle ----------------------------------------------------------------
 // All level vectors hold nc stacked load cases (stride: the level's ndof);
@@ -519,7 +523,7 @@ KuOp<T> fine_op(Problem& P, const int* done, int nc) {
   op.mask = P.L[0]->mask.p;
   op.E = Ev<T>(P);
   op.mg = Mg<T>(P);
-  op.tz = P.tz;
+  op.tz = ku_tz<T>(P);
   op.done = done;
   op.nrhs = nc;
   op.pstride = P.pstride;
@@ -599,7 +603,10 @@ const int* done_flags(Problem& P) { return &P.st.p->done; }
 // case. The V-cycle's last operator apply forms them; Jacobi uses a dot pass.
 int precondition_rz(Problem& P, Prec pc, const int* done, int nc) {
   const long long nd = P.g.ndof();
-  const int nrz = P.smoother == 1 ? dot_blocks(nd) : ku_partials(P.g);
+  // (the V-cycle's last fine apply forms them in the V-cycle's precision)
+  const int nrz = P.smoother == 1   ? dot_blocks(nd)
+                  : pc == Prec::MG32 ? ku_partials<float>(P.g)
+                                     : ku_partials<double>(P.g);
   if (pc == Prec::MG32) {
     vcycle<float>(P, P.r32.p, P.z32.p, done, nc, P.r.p, P.partials.p);
     return nrz;
@@ -631,7 +638,7 @@ void restart(Problem& P, Prec pc, int nc) {
 void iteration(Problem& P, Prec pc, int nc) {
   const long long nd = P.g.ndof();
   const int* done = done_flags(P);
-  const int kb = ku_partials(P.g);
+  const int kb = ku_partials<double>(P.g);  // A.p is FP64
   const int db = dot_blocks(nd);
   // A.p partials go to `partials`, the rr partials of cg_xr to `partials2`
   // (cg_xr re-sums the former in every block while it writes the latter).
@@ -839,36 +846,45 @@ double set_solve_deadline_ms(double absolute_steady_ms) {
 double steady_ms() { return now_ms(); }
 
 // Launch-shape autotuner: time the fine-level apply for each z-chunk length
-// on the problem's own buffers and keep the fastest (override:
-// TOPOHIP_KU_TILE=tz, or 32,8,tz). The choice changes the launch shape only:
-// every z-chunk computes bit-identical applies (each node sums its face
-// contributions in a fixed order) and identical dot partials (one per fixed
-// slab of planes), so results do not depend on what the tuner picks.
+// on the problem's own buffers and keep the fastest, separately for the FP32
+// and the FP64 applies (override: TOPOHIP_KU_TILE=tz or 32,8,tz, taken by each
+// precision whose slab divides it, or tz32:tz64). The choice changes the
+// launch shape only: every z-chunk computes bit-identical applies (each node
+// sums its face contributions in a fixed order) and identical dot partials
+// (one per fixed slab of planes), so results do not depend on what the tuner
+// picks.
 void autotune_ku(Problem& P) {
+  bool have32 = false, have64 = false;
   if (const char* env = std::getenv("TOPOHIP_KU_TILE")) {
     int a = 0, b = 0, c = 0;
-    const int n = std::sscanf(env, "%d,%d,%d", &a, &b, &c);
-    const int tz = n == 3 ? (a == kTX && b == kTY ? c : 0) : (n == 1 ? a : 0);
-    if (tz >= kSlab && tz % kSlab == 0) {
-      P.tz = tz;
-      return;
+    if (std::sscanf(env, "%d:%d", &a, &b) == 2) {
+      if (a >= ku_slab<float>() && a % ku_slab<float>() == 0) { P.tz32 = a; have32 = true; }
+      if (b >= ku_slab<double>() && b % ku_slab<double>() == 0) { P.tz64 = b; have64 = true; }
+    } else {
+      const int n = std::sscanf(env, "%d,%d,%d", &a, &b, &c);
+      const int tz = n == 3 ? (a == kTX && b == kTY ? c : 0) : (n == 1 ? a : 0);
+      if (tz >= ku_slab<float>() && tz % ku_slab<float>() == 0) { P.tz32 = tz; have32 = true; }
+      if (tz >= ku_slab<double>() && tz % ku_slab<double>() == 0) { P.tz64 = tz; have64 = true; }
     }
   }
-  if (P.g.elems() < 4096) return;
+  const bool overridden = have32 || have64;
+  const bool mixed = P.precision == TOPOHIP_PREC_MIXED;
+  if (!mixed) have32 = true;  // the FP64 form launches FP64 applies only
+  if ((have32 && have64) || P.g.elems() < 4096) return;
   // The choice is cached per (device, grid, precision) for the life of the
-  // process, so problems of one shape launch alike.
+  // process, so problems of one shape launch alike (an override bypasses it).
   static std::mutex mu;
-  static std::map<std::array<int, 5>, int> cache;
+  static std::map<std::array<int, 5>, std::array<int, 2>> cache;
   const std::array<int, 5> key{P.device, P.g.ex, P.g.ey, P.g.ez, P.precision};
-  {
+  if (!overridden) {
     std::lock_guard<std::mutex> lk(mu);
     auto it = cache.find(key);
     if (it != cache.end()) {
-      P.tz = it->second;
+      if (!have32) P.tz32 = it->second[0];
+      if (!have64) P.tz64 = it->second[1];
       return;
     }
   }
-  const bool mixed = P.precision == TOPOHIP_PREC_MIXED;
   const long long nd = P.g.ndof();
   launch_fill<double>(P.E64.p, 0.5, P.g.elems(), P.s);
   launch_convert<double, float>(P.E64.p, P.E32.p, P.g.elems(), P.s);
@@ -877,41 +893,48 @@ void autotune_ku(Problem& P) {
   hipEvent_t e0, e1;
   TOPOHIP_CHECK(hipEventCreate(&e0));
   TOPOHIP_CHECK(hipEventCreate(&e1));
-  // A PCG iteration applies the fine operator once in FP64 (A.p) and, in the
-  // mixed form, twice in FP32 (the V-cycle's sweeps; FP64 in the FP64 form):
-  // each z-chunk is scored by that mix.
-  float best = 1e30f;
-  const int tzs[4] = {4, 8, 16, 32};
-  for (int tz : tzs) {
-    auto run32 = [&]() {
-      KuOp<float> op;
-      op.g = P.g; op.E = P.E32.p; op.mg = P.mg32; op.tz = tz;
-      op.x = P.r32.p; op.y = P.z32.p; op.epi = EPI_AX;  // (scratch at set-up)
-      launch_ku_fine<float>(op, P.s);
-    };
-    auto run64 = [&]() {
-      KuOp<double> op;
-      op.g = P.g; op.E = P.E64.p; op.mg = P.mg64; op.tz = tz;
-      op.x = P.x.p; op.y = P.Ap64.p; op.epi = EPI_AX;
-      launch_ku_fine<double>(op, P.s);
-    };
-    auto timed = [&](const std::function<void()>& run) {
-      run();
-      TOPOHIP_CHECK(hipEventRecord(e0, P.s));
-      for (int i = 0; i < 5; ++i) run();
-      TOPOHIP_CHECK(hipEventRecord(e1, P.s));
-      TOPOHIP_CHECK(hipEventSynchronize(e1));
-      float ms = 0.f;
-      TOPOHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-      return ms;
-    };
-    const float ms = mixed ? timed(run64) + 2.0f * timed(run32) : 3.0f * timed(run64);
-    if (ms < best) { best = ms; P.tz = tz; }
+  auto timed = [&](const std::function<void()>& run) {
+    run();
+    TOPOHIP_CHECK(hipEventRecord(e0, P.s));
+    for (int i = 0; i < 5; ++i) run();
+    TOPOHIP_CHECK(hipEventRecord(e1, P.s));
+    TOPOHIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    TOPOHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    return ms;
+  };
+  if (!have32) {
+    float best = 1e30f;
+    for (int tz : {4, 8, 16, 32}) {
+      const float ms = timed([&]() {
+        KuOp<float> op;
+        op.g = P.g; op.E = P.E32.p; op.mg = P.mg32; op.tz = tz;
+        op.x = P.r32.p; op.y = P.z32.p; op.epi = EPI_AX;  // (scratch at set-up)
+        launch_ku_fine<float>(op, P.s);
+      });
+      if (ms < best) { best = ms; P.tz32 = tz; }
+    }
+  }
+  if (!have64) {
+    float best = 1e30f;
+    for (int tz : {7, 14, 21, 28}) {
+      const float ms = timed([&]() {
+        KuOp<double> op;
+        op.g = P.g; op.E = P.E64.p; op.mg = P.mg64; op.tz = tz;
+        op.x = P.x.p; op.y = P.Ap64.p; op.epi = EPI_AX;
+        launch_ku_fine<double>(op, P.s);
+      });
+      if (ms < best) { best = ms; P.tz64 = tz; }
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (overridden) return;
   std::lock_guard<std::mutex> lk(mu);
-  P.tz = cache.emplace(key, P.tz).first->second;  // a concurrent tuner may have won
+  // a concurrent tuner may have won: every problem of the key launches alike
+  const auto& won = cache.emplace(key, std::array<int, 2>{P.tz32, P.tz64}).first->second;
+  if (!have32) P.tz32 = won[0];
+  if (!have64) P.tz64 = won[1];
 }
 
 void drop_graphs(Problem& P) {
@@ -1265,7 +1288,7 @@ void solve_cases(Problem& P, int nc, double tol, int max_it, const double* d_gue
     op.g = P.g;
     op.E = P.E64.p;
     op.mg = P.mg64;
-    op.tz = P.tz;
+    op.tz = P.tz64;
     op.x = P.up.p;
     op.y = P.rt.p;
     op.epi = EPI_AX;

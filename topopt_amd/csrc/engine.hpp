@@ -53,7 +53,9 @@ struct Problem {
   Geom g;
   double h = 1.0, nu = 0.3;
   int precision = TOPOHIP_PREC_MIXED;
-  int tz = 8;   // fine K.u z-chunk (autotuned; a multiple of kSlab)
+  // fine K.u z-chunks of the FP32 and FP64 applies (autotuned; multiples of
+  // ku_slab<float>() and ku_slab<double>())
+  int tz32 = 8, tz64 = 7;
   // V-cycle smoother: 0 damped Jacobi (the reference's), 1 Chebyshev of degree
   // cheb_degree on every smoothed level
   int smoother = 0;

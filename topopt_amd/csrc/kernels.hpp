@@ -39,10 +39,16 @@ struct KuOp {
 
 // Fine K.u launch shape: kTX x kTY lanes per workgroup (one element column
 // each, owning (kTX-1) x (kTY-1) nodes) over a z-chunk of tz node planes, tz a
-// multiple of kSlab; the FP64 dot partials are one per (kSlab-plane slab,
-// x-y tile), ku_partials(g) per load case, whatever tz is.
-constexpr int kTX = 32, kTY = 8, kSlab = 4;
-int ku_partials(const Geom& g);
+// multiple of the precision's slab ku_slab<T>(); the FP64 dot partials are one
+// per (slab, x-y tile), ku_partials<T>(g) per load case, whatever tz is.
+// The FP64 applies hold three waves per SIMD against the FP32 applies' five,
+// and chunks in multiples of 7 planes fill their resident workgroups better
+// (192x64x64: a 7-plane chunk puts the grid's 700 workgroups in one round,
+// where 4 planes need 1190 in two and 8 planes 630 of 9 steps each).
+constexpr int kTX = 32, kTY = 8;
+template <typename T>
+constexpr int ku_slab() { return sizeof(T) == 8 ? 7 : 4; }
+template <typename T> int ku_partials(const Geom& g);
 template <typename T> void launch_ku_fine(const KuOp<T>& op, hipStream_t s);
 
 // ---- transfers between a fine level f and the coincident coarse level c ----

@@ -91,11 +91,11 @@ __device__ inline void modal_core(const T (&A)[24], T (&v)[24], T e, const Modal
 // DOTK: 0 none, 1 <x, Ax> over kept DOFs, 2 <rdot, y> over kept DOFs (FP64 partials).
 //
 // Launch shape: kTX x kTY lanes (one element column each) over a z-chunk of
-// a.tz node planes, a.tz a multiple of kSlab. The FP64 dot partials are one
-// per (kSlab-plane slab, tile): each lane sums its node's terms over the slab's
-// planes in plane order and the workgroup adds its four waves in a fixed
-// order, so the partials (and every PCG scalar formed from them) do not depend
-// on the z-chunk the autotuner picks.
+// a.tz node planes, a.tz a multiple of the slab ku_slab<T>(). The FP64 dot
+// partials are one per (slab, tile): each lane sums its node's terms over the
+// slab's planes in plane order and the workgroup adds its four waves in a
+// fixed order, so the partials (and every PCG scalar formed from them) do not
+// depend on the z-chunk the autotuner picks.
 template <typename T, int EPI, int DOTK>
 __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
   constexpr int TX = kTX, TY = kTY;
@@ -103,6 +103,7 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
   constexpr int PW = TX + 1;                  // staged plane: PW x (TY + 1) nodes
   constexpr int PLANE = PW * (TY + 1) * 3;
   constexpr int NPF = (PLANE + NT - 1) / NT;  // prefetch registers per lane
+  constexpr int SLAB = ku_slab<T>();          // planes per dot partial
   // re-read plane k from LDS (see step): the FP64 residual and smoothing
   // variants and the FP32 sweeps, which gain a wave per SIMD by it (the FP32
   // sweep from zero: 24.0 -> 20.9 us at 192x64x64, 574 -> 593 us at
@@ -375,11 +376,11 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
     if constexpr (DOTK != 0) {
       // end of a slab (or of the grid): wave sums to LDS, written out by
       // thread 0 after the next barrier
-      if (k >= k0 && ((k + 1) % kSlab == 0 || k + 1 == g.nnz)) {
+      if (k >= k0 && ((k + 1) % SLAB == 0 || k + 1 == g.nnz)) {
         const double w = wave_sum(dotp);
         if ((t & 63) == 0) s_red[t >> 6] = w;
         dotp = 0.0;
-        pending = k / kSlab;
+        pending = k / SLAB;
       }
     }
   };
@@ -406,8 +407,9 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
 
 template <typename T>
 void dispatch(const KuOp<T>& op, hipStream_t s) {
-  if (op.tz < kSlab || op.tz % kSlab != 0)
-    throw HipError("K.u z-chunk must be a positive multiple of " + std::to_string(kSlab));
+  constexpr int slab = ku_slab<T>();
+  if (op.tz < slab || op.tz % slab != 0)
+    throw HipError("K.u z-chunk must be a positive multiple of " + std::to_string(slab));
   const long long tiles = static_cast<long long>((op.g.nnx + kTX - 2) / (kTX - 1)) *
                           ((op.g.nny + kTY - 2) / (kTY - 1)) * ((op.g.nnz + op.tz - 1) / op.tz) *
                           op.nrhs;
@@ -432,10 +434,14 @@ void dispatch(const KuOp<T>& op, hipStream_t s) {
 
 }  // namespace
 
+template <typename T>
 int ku_partials(const Geom& g) {
+  constexpr int slab = ku_slab<T>();
   return ((g.nnx + kTX - 2) / (kTX - 1)) * ((g.nny + kTY - 2) / (kTY - 1)) *
-         ((g.nnz + kSlab - 1) / kSlab);
+         ((g.nnz + slab - 1) / slab);
 }
+template int ku_partials<float>(const Geom&);
+template int ku_partials<double>(const Geom&);
 
 template <typename T>
 void launch_ku_fine(const KuOp<T>& op, hipStream_t s) {
