@@ -65,8 +65,10 @@ def ku_bytes(nx, ny, nz, variant):
     """Algorithmic HBM bytes of one fine K.u launch (BC mask bytes counted as 0)."""
     ndof = 3 * (nx + 1) * (ny + 1) * (nz + 1)
     nv = nx * ny * nz
-    w = 8 if variant in ("f64", "ap64_dot") else 4
-    if variant == "res0_f32":
+    w = 8 if variant in ("f64", "ap64_dot", "ap64_pdir") else 4
+    if variant == "ap64_pdir":
+        b = 2 * w * ndof + w * nv + 12 * ndof       # p_old read, Ap written, E; p written, FP32 z read
+    elif variant == "res0_f32":
         b = 4 * w * ndof + w * nv                   # b, D^-1 read; y, x0 written; E read
     elif variant == "jac_dot_f32":
         b = 4 * w * ndof + 8 * ndof + w * nv        # x, b, D^-1 read; y written; FP64 r read; E
@@ -135,16 +137,16 @@ def run_design(ta, wl, precision, steps, warmup, mma=False, smoother=None):
     return out, dt
 
 
-KU_LOOP = ("ap64_dot", "res0_f32", "jac_dot_f32")
+KU_LOOP = ("ap64_pdir", "res0_f32", "jac_dot_f32")
 
 
 def variant_tile(tiles, variant):
     """Launch tile (tx, ty, tz) of a K.u variant: FP64 variants run on the
     FP64 applies' z-chunk, FP32 ones on the FP32 applies'."""
-    return tiles["f64"] if variant in ("f64", "ap64_dot") else tiles["f32"]
+    return tiles["f64"] if variant in ("f64", "ap64_dot", "ap64_pdir") else tiles["f32"]
 
 
-def ku_measure(ta, nx, ny, nz, reps=100, variants=("f32", "f64") + KU_LOOP):
+def ku_measure(ta, nx, ny, nz, reps=100, variants=("f32", "f64", "ap64_dot") + KU_LOOP):
     res = {}
     with ta.Problem(nx, ny, nz) as p:
         p.bench_prepare(0x5EED)
@@ -244,7 +246,7 @@ def run_large(ta, problems, iterations=3):
                          continuation=wl["continuation"])
         dt = time.perf_counter() - t0
         m1 = hip_mem_used()
-    ku, tiles = ku_measure(ta, nx, ny, nz, reps=10, variants=("f32", "f64") + KU_LOOP)
+    ku, tiles = ku_measure(ta, nx, ny, nz, reps=10, variants=("f32", "f64", "ap64_dot") + KU_LOOP)
     return {"workload": "cantilever_512x256x256", "dof": 3 * (nx + 1) * (ny + 1) * (nz + 1),
             "iterations": out["iterations"], "design_iters_per_s": out["iterations"] / dt,
             "cg_iterations_total": out["cg_iterations_total"], "compliance": out["compliance"],
@@ -363,8 +365,8 @@ def main():
                     help="only the K.u microbenchmark, after a 256 MiB device copy that "
                          "calibrates the PMC byte counters (for rocprofv3 --pmc passes)")
     ap.add_argument("--variant", default=None,
-                    help="with --ku-only: one K.u variant (f32, f64, ap64_dot, res0_f32, "
-                         "jac_dot_f32)")
+                    help="with --ku-only: one K.u variant (f32, f64, ap64_dot, ap64_pdir, "
+                         "res0_f32, jac_dot_f32)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this engine has no multi-GPU path (each job and each "
@@ -396,7 +398,7 @@ def main():
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
-        variants = (a.variant,) if a.variant else ("f32", "f64") + KU_LOOP
+        variants = (a.variant,) if a.variant else ("f32", "f64", "ap64_dot") + KU_LOOP
         ku, tiles = ku_measure(ta, nx, ny, nz, reps=10, variants=variants)
         print(json.dumps({"workload": a.workload, "tile": tiles, "ku": ku,
                           "calibration_copy_bytes": 256 << 20}), flush=True)
@@ -458,7 +460,7 @@ def main():
         large = run_large(ta, problems)
         log(f"# 512x256x256: {large['design_iters_per_s']:.2f} design iters/s, "
             f"K.u f32 {large['ku']['f32']['us']:.0f} us")
-    dom = ku["ap64_dot"]
+    dom = ku["ap64_pdir"]
     line = {
         "metric": f"design iterations/s ({a.workload}, FP32 multigrid V-cycle in FP64 PCG)",
         "value": value,
@@ -489,8 +491,9 @@ def main():
         "result_fields": fields,
         "roofline": {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom["hbm_frac"],
-                     "traffic": measured_traffic(a.workload, "ap64_dot", tiles["f64"]),
-                     "kernel": "ku_face<double> EPI_AX + <p,Ap> partials (the PCG's A.p)",
+                     "traffic": measured_traffic(a.workload, "ap64_pdir", tiles["f64"]),
+                     "kernel": "ku_face<double> EPI_AX + <p,Ap> partials with the direction "
+                               "update p = z + beta p fused in (the PCG's A.p)",
                      "tile": tiles["f64"], "bytes_per_launch": dom["bytes"],
                      "us_per_launch": dom["us"]},
         "roofline_loop": {k: {"frac": ku[k]["hbm_frac"], "GBs": ku[k]["GBs"], "us": ku[k]["us"],

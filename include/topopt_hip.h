@@ -289,7 +289,10 @@ int topohip_bench_ku(topohip_problem* p, int precision, int reps, double* ms_per
  * way: variant 0 = FP32 y = K u, 1 = FP64 y = K u, 2 = FP64 A.p with the
  * fused <p, Ap> partials (the PCG's operator apply), 3 = FP32 zero-start
  * damped-Jacobi residual (the V-cycle's pre-smoothing apply), 4 = FP32
- * damped-Jacobi sweep with the fused <r, z> partials (post-smoothing). Every
+ * damped-Jacobi sweep with the fused <r, z> partials (post-smoothing), 5 =
+ * the PCG's FP64 A.p as the design loop launches it in the mixed form: the
+ * direction update p = z + beta p (FP32 z) fused in, p written, <p, Ap>
+ * partials. Every
  * launch cycles to the next of *sets_used (nullable) operand sets, enough that
  * they span more than twice the 256 MiB last-level cache, so inputs come from
  * HBM. The launch shape is the handle's tile of that precision

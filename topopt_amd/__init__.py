@@ -474,7 +474,8 @@ class Problem:
         _check(lib().topohip_bench_ku(self._h, int(precision), int(reps), C.byref(ms)))
         return ms.value
 
-    KU_VARIANTS = {"f32": 0, "f64": 1, "ap64_dot": 2, "res0_f32": 3, "jac_dot_f32": 4}
+    KU_VARIANTS = {"f32": 0, "f64": 1, "ap64_dot": 2, "res0_f32": 3, "jac_dot_f32": 4,
+                   "ap64_pdir": 5}
 
     def bench_ku_variant(self, variant, reps=50):
         """(ms per launch, rotated operand sets) of a fine K.u kernel variant."""

@@ -510,15 +510,26 @@ int topohip_bench_ku_variant(topohip_problem* p, int variant, int reps, double* 
     need(p, "problem"); need(ms_per_apply, "ms_per_apply");
     if (!p->bench_ready) throw Error(TOPOHIP_EINVAL, "call topohip_bench_prepare first");
     if (reps < 1) throw Error(TOPOHIP_EINVAL, "reps must be >= 1");
-    if (variant < 0 || variant > 4) throw Error(TOPOHIP_EINVAL, "unknown K.u variant");
+    if (variant < 0 || variant > 5) throw Error(TOPOHIP_EINVAL, "unknown K.u variant");
     Problem& P = *p->P;
     const long long nd = P.g.ndof(), nv = P.g.elems();
-    const bool f64 = variant == 1 || variant == 2;
+    const bool f64 = variant == 1 || variant == 2 || variant == 5;
     const int w = f64 ? 8 : 4;
     // algorithmic bytes of the variant (the per-launch figures of the bench)
     double bytes = 2.0 * w * nd + static_cast<double>(w) * nv;
     if (variant == 3) bytes += 3.0 * w * nd;              // b, dinv read; x0 written
     if (variant == 4) bytes += 3.0 * w * nd + 8.0 * nd;   // x, b, dinv + FP64 r read
+    if (variant == 5) bytes += 8.0 * nd + 4.0 * nd;       // p written, FP32 z read
+    if (variant == 5) {
+      // the case-0 state the direction update reads: beta = 0.5 from nrz
+      // partials of 1 and rz = 2 nrz, not after a restart
+      const int nrz = ku_partials<float>(P.g);
+      launch_fill<double>(P.rzpart.p, 1.0, nrz, P.s);
+      CgState st{};
+      st.rz = 2.0 * nrz;
+      st.rz_new = st.rz;
+      TOPOHIP_CHECK(hipMemcpyAsync(P.st.p, &st, sizeof(st), hipMemcpyHostToDevice, P.s));
+    }
     const int ns = rotation_sets(bytes);
     if (sets_used) *sets_used = ns;
     // one arena: per set x, y, b, dinv, xw, E in T and r in FP64
@@ -541,6 +552,7 @@ int topohip_bench_ku_variant(topohip_problem* p, int variant, int reps, double* 
         TOPOHIP_CHECK(hipMemcpyAsync(q.E, e, nv * sizeof(T), hipMemcpyDeviceToDevice, P.s));
         launch_fill<T>(q.d, T(0.5), nd, P.s);
         TOPOHIP_CHECK(hipMemcpyAsync(q.r, p->bu64.p, nd * sizeof(double), hipMemcpyDeviceToDevice, P.s));
+        if (variant == 5) launch_fill<float>(reinterpret_cast<float*>(q.xw), 0.25f, nd, P.s);
       }
       return S;
     };
@@ -558,7 +570,17 @@ int topohip_bench_ku_variant(topohip_problem* p, int variant, int reps, double* 
         op.partials = P.partials.p; op.rdot = q.r;
       } else {
         op.epi = EPI_AX; op.x = q.x;
-        if (variant == 2) op.partials = P.partials.p;
+        if (variant == 2 || variant == 5) op.partials = P.partials.p;
+        if constexpr (sizeof(T) == 8) {
+          if (variant == 5) {
+            // z (FP32) in the set's x0 space, the new direction to its b
+            op.pz32 = reinterpret_cast<const float*>(q.xw);
+            op.pw = q.b;
+            op.st = P.st.p;
+            op.rzp = P.rzpart.p;
+            op.nrz = ku_partials<float>(P.g);
+          }
+        }
       }
       return op;
     };

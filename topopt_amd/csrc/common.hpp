@@ -89,6 +89,8 @@ struct CgState {
   int fail;       // breakdown: p^T A p <= 0 or non-finite
   int iters;      // completed iterations
   int cap;        // iteration cap (fail = 2 when reached unconverged)
+  int pfresh;     // set at a restart: the next A.p takes p = z (beta = 0)
+  int pad;
 };
 
 // MI355X places consecutive workgroup ids on its 8 XCDs round-robin, and each

@@ -601,41 +601,49 @@ const int* done_flags(Problem& P) { return &P.st.p->done; }
 // Preconditioner application z = M^{-1} r (mixed: r32 must hold float(r)),
 // with the partials of <r, z> written to P.partials; returns their count per
 // case. The V-cycle's last operator apply forms them; Jacobi uses a dot pass.
-int precondition_rz(Problem& P, Prec pc, const int* done, int nc) {
+// Count of <r, z> partials per case the preconditioner writes (the V-cycle's
+// last fine apply forms them in the V-cycle's precision; Jacobi and the
+// Chebyshev smoother use a dot pass).
+int rz_count(const Problem& P, Prec pc) {
   const long long nd = P.g.ndof();
-  // (the V-cycle's last fine apply forms them in the V-cycle's precision)
-  const int nrz = P.smoother == 1   ? dot_blocks(nd)
-                  : pc == Prec::MG32 ? ku_partials<float>(P.g)
-                                     : ku_partials<double>(P.g);
+  if (pc == Prec::JACOBI || P.smoother == 1) return dot_blocks(nd);
+  return pc == Prec::MG32 ? ku_partials<float>(P.g) : ku_partials<double>(P.g);
+}
+
+// Preconditioner application z = M^{-1} r (mixed: r32 must hold float(r)),
+// with the partials of <r, z> written to P.rzpart (rz_count per case).
+void precondition_rz(Problem& P, Prec pc, const int* done, int nc) {
+  const long long nd = P.g.ndof();
   if (pc == Prec::MG32) {
-    vcycle<float>(P, P.r32.p, P.z32.p, done, nc, P.r.p, P.partials.p);
-    return nrz;
+    vcycle<float>(P, P.r32.p, P.z32.p, done, nc, P.r.p, P.rzpart.p);
+    return;
   }
   if (pc == Prec::MG64) {
-    vcycle<double>(P, P.r.p, P.z64.p, done, nc, P.r.p, P.partials.p);
-    return nrz;
+    vcycle<double>(P, P.r.p, P.z64.p, done, nc, P.r.p, P.rzpart.p);
+    return;
   }
   launch_jacobi0<double>(P.z64.p, P.L[0]->dinv64.p, P.r.p, 1.0, nd, done, P.s, nc);
-  const int nb = dot_blocks(nd);
-  launch_dot<double, double>(P.r.p, P.z64.p, nd, P.partials.p, nb, done, P.s, nc, P.pstride);
-  return nb;
+  launch_dot<double, double>(P.r.p, P.z64.p, nd, P.rzpart.p, dot_blocks(nd), done, P.s, nc,
+                             P.pstride);
 }
 
 // Restart the recurrence of every case that is not done from its current r:
-// z = M^-1 r, rz = r.z, p = z.
+// z = M^-1 r, rz = r.z, and the next A.p takes p = z (cg_set_rz marks it).
 void restart(Problem& P, Prec pc, int nc) {
   const long long nd = P.g.ndof();
   const int* done = done_flags(P);
   if (pc == Prec::MG32) launch_convert<double, float>(P.r.p, P.r32.p, nd * nc, P.s);
-  const int nrz = precondition_rz(P, pc, done, nc);
-  launch_cg_set_rz(P.st.p, P.partials.p, nrz, P.s, nc, P.pstride);
-  if (pc == Prec::MG32)
-    launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, nd, 1, nullptr, 0, P.s, nc, P.pstride);
-  else
-    launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nd, 1, nullptr, 0, P.s, nc, P.pstride);
+  precondition_rz(P, pc, done, nc);
+  launch_cg_set_rz(P.st.p, P.rzpart.p, rz_count(P, pc), P.s, nc, P.pstride);
 }
 
-void iteration(Problem& P, Prec pc, int nc) {
+double* direction(Problem& P, int i) { return i == 0 ? P.p.p : P.p2.p; }
+
+// One PCG iteration whose current direction is in buffer par (the next one
+// goes to buffer par ^ 1): p = z + beta p fused into the FP64 A.p (beta from
+// the previous preconditioner's <r, z> partials; p = z after a restart), then
+// x, r and the stopping test, then z = M^-1 r with its <r, z> partials.
+void iteration(Problem& P, Prec pc, int nc, int par) {
   const long long nd = P.g.ndof();
   const int* done = done_flags(P);
   const int kb = ku_partials<double>(P.g);  // A.p is FP64
@@ -645,18 +653,22 @@ void iteration(Problem& P, Prec pc, int nc) {
   // A.p is FP64 in every form (it defines the residual, as mf_mgpcg's
   // mf_fine_matvec, core/src/fea/multigrid.cpp:1286); the mixed form's FP32
   // V-cycle reads float(r), which cg_xr writes alongside r.
-  fine_apply<double>(P, P.p.p, P.Ap64.p, nullptr, EPI_AX, P.partials.p, done, nc);
-  launch_cg_xr<double, double>(P.st.p, P.x.p, P.r.p, P.p.p, P.Ap64.p,
+  KuOp<double> op = fine_op<double>(P, done, nc);
+  op.x = direction(P, par);
+  op.pw = direction(P, par ^ 1);
+  op.y = P.Ap64.p;
+  op.epi = EPI_AX;
+  op.partials = P.partials.p;
+  if (pc == Prec::MG32) op.pz32 = P.z32.p; else op.pz64 = P.z64.p;
+  op.st = P.st.p;
+  op.rzp = P.rzpart.p;
+  op.nrz = rz_count(P, pc);
+  launch_ku_fine<double>(op, P.s);
+  launch_cg_xr<double, double>(P.st.p, P.x.p, P.r.p, direction(P, par ^ 1), P.Ap64.p,
                                pc == Prec::MG32 ? P.r32.p : nullptr, nd, P.partials.p, kb,
                                P.partials2.p, db, P.s, nc, P.pstride);
   launch_cg_check(P.st.p, P.partials2.p, db, P.s, nc, P.pstride);
-  const int nrz = precondition_rz(P, pc, done, nc);
-  if (pc == Prec::MG32)
-    launch_cg_p<float>(P.st.p, P.z32.p, P.p.p, nd, 0, P.partials.p, nrz, P.s, nc,
-                       P.pstride);
-  else
-    launch_cg_p<double>(P.st.p, P.z64.p, P.p.p, nd, 0, P.partials.p, nrz, P.s, nc,
-                        P.pstride);
+  precondition_rz(P, pc, done, nc);
 }
 
 void read_state(Problem& P, int nc) { d2h(P.st_h, P.st.p, static_cast<size_t>(nc), P.s); }
@@ -723,38 +735,78 @@ void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool
       if (!P.st_h[r].done) return false;
     return true;
   };
-  const int batch = 4;
-  // The iteration body is captured once per (preconditioner kind, case
-  // count) into a HIP graph (all pointers are fixed while the capacity
-  // stands) and replayed; every kernel in it skips cases whose done flag is set.
+  // The iteration body is captured into HIP graphs (all pointers are fixed
+  // while the capacity stands) and replayed; every kernel in them skips cases
+  // whose done flag is set. Per preconditioner kind there are four: a batch of
+  // kBatch iterations and a single one, each starting from either direction
+  // buffer (a batch keeps the buffer parity, a single iteration flips it).
   // Relaxed capture: the body makes no synchronising calls, and other threads
   // (other problems of the serving loop) keep allocating and copying meanwhile.
+  constexpr int kBatch = 4;
   const int gi = static_cast<int>(pc);
-  hipGraphExec_t& gx = P.cg_graph[gi];
-  if (gx && P.cg_graph_nc[gi] != nc) {
-    std::lock_guard<std::recursive_mutex> lk(device_api_mutex());
-    (void)hipGraphExecDestroy(gx);
-    gx = nullptr;
-  }
-  if (!gx) {
-    std::lock_guard<std::recursive_mutex> lk(device_api_mutex());
-    hipGraph_t gr;
-    TOPOHIP_CHECK(hipStreamBeginCapture(P.s, hipStreamCaptureModeRelaxed));
-    for (int i = 0; i < batch; ++i) iteration(P, pc, nc);
-    TOPOHIP_CHECK(hipStreamEndCapture(P.s, &gr));
-    TOPOHIP_CHECK(hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0));
-    TOPOHIP_CHECK(hipGraphDestroy(gr));
-    P.cg_graph_nc[gi] = nc;
-  }
+  for (int par = 0; par < 2; ++par)
+    for (int len = 0; len < 2; ++len) {
+      const int k = (gi * 2 + par) * 2 + len;
+      hipGraphExec_t& gx = P.cg_graph[k];
+      if (gx && P.cg_graph_nc[k] == nc) continue;
+      std::lock_guard<std::recursive_mutex> lk(device_api_mutex());
+      if (gx) (void)hipGraphExecDestroy(gx);
+      gx = nullptr;
+      hipGraph_t gr;
+      TOPOHIP_CHECK(hipStreamBeginCapture(P.s, hipStreamCaptureModeRelaxed));
+      const int n = len == 0 ? kBatch : 1;
+      for (int i = 0; i < n; ++i) iteration(P, pc, nc, par ^ (i & 1));
+      TOPOHIP_CHECK(hipStreamEndCapture(P.s, &gr));
+      TOPOHIP_CHECK(hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0));
+      TOPOHIP_CHECK(hipGraphDestroy(gr));
+      P.cg_graph_nc[k] = nc;
+    }
+  auto launch_iterations = [&](int n) {
+    for (; n >= kBatch; n -= kBatch)
+      TOPOHIP_CHECK(hipGraphLaunch(P.cg_graph[(gi * 2 + P.pcur) * 2], P.s));
+    for (; n > 0; --n) {
+      TOPOHIP_CHECK(hipGraphLaunch(P.cg_graph[(gi * 2 + P.pcur) * 2 + 1], P.s));
+      P.pcur ^= 1;
+    }
+  };
   const long long per_it = pc == Prec::JACOBI ? 1 : 3;
-  // The first read-back waits for as many batches as the previous solve of
-  // this kind needed, less a margin (consecutive design iterations take
-  // similar counts); later ones follow every batch. Batches past convergence
-  // are no-ops (every kernel checks the done flag), so this only saves syncs.
-  int ahead = std::min(8, std::max(1, (P.pcg_hint[gi] - 2 * batch) / batch));
+  // Iterations launched before each read-back of the states. The first group
+  // is as many as the previous solve of this kind needed, less a margin
+  // (consecutive design iterations take similar counts); after each read-back
+  // the next group is the count at which the slowest unfinished case reaches
+  // its threshold if its residual keeps falling at the rate seen since the
+  // previous read-back. Iterations past convergence are no-ops (every kernel
+  // checks the done flag) but still cost their launches, and a read-back
+  // costs a round trip, so the prediction saves both.
+  std::vector<double> rr_seen(nc);
+  std::vector<int> it_seen(nc);
+  auto remember = [&]() {
+    for (int r = 0; r < nc; ++r) {
+      rr_seen[r] = P.st_h[r].rr;
+      it_seen[r] = P.st_h[r].iters;
+    }
+  };
+  auto predict = [&]() {
+    int need = 1;
+    for (int r = 0; r < nc; ++r) {
+      const CgState& st = P.st_h[r];
+      if (st.done) continue;
+      const int di = st.iters - it_seen[r];
+      if (di <= 0 || !(st.rr > 0.0) || !(st.rr < rr_seen[r]) || !(st.thresh2 > 0.0)) {
+        need = std::max(need, kBatch);
+        continue;
+      }
+      const double rate = std::log(st.rr / rr_seen[r]) / di;  // < 0
+      const double k = std::ceil(std::log(st.thresh2 / st.rr) / rate);
+      need = std::max(need, static_cast<int>(std::min(64.0, std::max(1.0, k))));
+    }
+    return need;
+  };
+  remember();
+  int next = std::max(kBatch, std::min(64, P.pcg_hint[gi] - 3));
   for (long long guard = 0;; ++guard) {
     if (guard > 4LL * cap + 64) throw Error(TOPOHIP_EINTERNAL, "pcg: iteration state stalled");
-    // An armed solve deadline is polled before every graph batch (the
+    // An armed solve deadline is polled before every group of iterations (the
     // reference polls every 256 iterations, matfree.cpp:988-1001); when it has
     // passed the unfinished cases stop with the state they reached.
     if (deadline_passed()) {
@@ -769,13 +821,16 @@ void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool
     if (!all_done()) {
       long long before = 0;
       for (int r = 0; r < nc; ++r) before += P.st_h[r].iters;
-      for (int k = 0; k < ahead; ++k) TOPOHIP_CHECK(hipGraphLaunch(gx, P.s));
-      ahead = 1;
+      launch_iterations(next);
       read_state(P, nc);
       long long after = 0;
       for (int r = 0; r < nc; ++r) after += P.st_h[r].iters;
       P.matvecs += (after - before) * per_it;
-      if (!all_done()) continue;
+      if (!all_done()) {
+        next = predict();
+        remember();
+        continue;
+      }
     }
     // Every recurrence has stopped: confirm with the FP64 true residuals.
     bool any_pending = false;
@@ -820,6 +875,8 @@ void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool
     if (!restarted) break;
     restart(P, pc, nc);
     read_state(P, nc);
+    remember();
+    next = 2;  // a residual replacement near the threshold needs a few more
   }
   int most = 0;
   for (int r = 0; r < nc; ++r) {
@@ -959,7 +1016,7 @@ void ensure_cases(Problem& P, int nb) {
   P.pstride = std::max(pstride, P.pstride);
   const bool mixed = P.precision == TOPOHIP_PREC_MIXED;
   const size_t nd = static_cast<size_t>(P.g.ndof()) * nb;
-  P.b.ensure(nd); P.x.ensure(nd); P.r.ensure(nd); P.p.ensure(nd); P.rt.ensure(nd);
+  P.b.ensure(nd); P.x.ensure(nd); P.r.ensure(nd); P.p.ensure(nd); P.p2.ensure(nd); P.rt.ensure(nd);
   P.u.ensure(nd); P.Ap64.ensure(nd); P.z64.ensure(nd);
   if (mixed) { P.r32.ensure(nd); P.z32.ensure(nd); }
   const bool cheb = P.smoother == 1;
@@ -994,6 +1051,7 @@ void ensure_cases(Problem& P, int nb) {
   }
   P.partials.ensure(static_cast<size_t>(nb) * P.pstride);
   P.partials2.ensure(static_cast<size_t>(nb) * P.pstride);
+  P.rzpart.ensure(static_cast<size_t>(nb) * P.pstride);
   if (P.mg_plan_ok) P.cr.reserve_cases(nb);
   P.nb = nb;
 }

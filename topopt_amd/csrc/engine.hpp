@@ -101,9 +101,15 @@ struct Problem {
   // PCG vectors
   // PCG vectors, nb stacked load cases each (stride ndof)
   DBuf<double> b, x, r, p, Ap64, z64, rt, u;
+  // The A.p forms the next search direction from the current one (p, p2
+  // alternate: neighbouring tiles still read the current one as their halo);
+  // pcur is the buffer that holds the current direction.
+  DBuf<double> p2;
+  int pcur = 0;
   bool u_valid = false;  // P.u holds the displacements of a completed solve
   DBuf<float> r32, z32;  // mixed form: the V-cycle's FP32 input and output
   DBuf<double> partials, partials2;  // nb cases x pstride block partial sums
+  DBuf<double> rzpart;               // the <r,z> partials the next A.p re-sums
   DBuf<CgState> st;
   CgState* st_h = nullptr;
   DBuf<double> scal;
@@ -142,8 +148,11 @@ struct Problem {
   int replacements = 0;
   bool deadline_hit = false;      // the last solve stopped at the thread's solve deadline
   const int* cancel = nullptr;    // caller-owned cancel flag of the design loops (nullable)
-  hipGraphExec_t cg_graph[3] = {nullptr, nullptr, nullptr};  // per preconditioner kind
-  int cg_graph_nc[3] = {0, 0, 0};                             // case count captured
+  // Captured PCG iterations per (preconditioner kind, direction buffer at the
+  // start, length: kCgBatch iterations or one), index (kind * 2 + parity) * 2 + len.
+  static constexpr int kCgGraphs = 12;
+  hipGraphExec_t cg_graph[kCgGraphs] = {};
+  int cg_graph_nc[kCgGraphs] = {};  // case count captured
   int pcg_hint[3] = {0, 0, 0};  // iterations of the last solve per preconditioner kind
 
   Problem() = default;

@@ -35,6 +35,20 @@ struct KuOp {
   const int* done = nullptr;      // case r exits early when done[r * kDoneStride] != 0
   int nrhs = 1;                   // stacked load cases (x, y, b stride g.ndof())
   int pstride = 0;                // partials of case r at partials + r * pstride
+  // Fused PCG direction update (FP64 EPI_AX with partials only): the applied
+  // vector is p = z + beta * x, formed where each node value is staged, with
+  // x the previous direction, z the preconditioned residual (pz32 or pz64) and
+  // beta = rz_new / st->rz, rz_new the fixed-order sum of the case's nrz <r,z>
+  // partials at rzp (every workgroup forms the same value; the case's first
+  // tile records rz_new and beta, and a non-finite rz_new is a breakdown).
+  // After a restart (st->pfresh) p = z. The owners write p to pw, which must
+  // not alias x (neighbouring tiles still read x as their halo).
+  const float* pz32 = nullptr;
+  const double* pz64 = nullptr;
+  double* pw = nullptr;
+  CgState* st = nullptr;
+  const double* rzp = nullptr;
+  int nrz = 0;
 };
 
 // Fine K.u launch shape: kTX x kTY lanes per workgroup (one element column
@@ -123,17 +137,13 @@ void launch_cg_xr(CgState* st, double* x, double* r, const TP* p, const TA* Ap, 
 // rr = sum(partials); iters++; done if rr <= thresh2; rz = rz_new
 void launch_cg_check(CgState* st, const double* partials, int n, hipStream_t s, int nr = 1,
                      int pstride = 0);
-// rz_new = fixed-order sum of the nrz <r,z> partials at rzp; beta = rz_new / rz;
-// p = z + beta*p (restart: p = z, partials unused)
-template <typename TZ>
-void launch_cg_p(CgState* st, const TZ* z, double* p, long long n, int restart,
-                 const double* rzp, int nrz, hipStream_t s, int nr = 1, int pstride = 0);
 // start: bnorm2, thresh2, rz from partials etc.
 void launch_cg_init(CgState* st, const double* bb_partials, int nb, double tol, int cap,
                     hipStream_t s, int nr = 1, int pstride = 0);
 // rr = sum(partials) of a recomputed residual; done = rr <= thresh2 (resume).
 void launch_cg_resume(CgState* st, const double* partials, int n, hipStream_t s, int nr = 1,
                       int pstride = 0);
+// restart: rz = rz_new = sum(partials); the next A.p takes p = z (pfresh)
 void launch_cg_set_rz(CgState* st, const double* partials, int n, hipStream_t s, int nr = 1,
                       int pstride = 0);
 

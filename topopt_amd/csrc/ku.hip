@@ -23,6 +23,7 @@
 //    corner results to LDS and the owning lane sums its 4 face contributions
 //    in a fixed order. No atomics, bit-identical run to run.
 #include <string>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -34,6 +35,14 @@ __device__ inline double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   return v;
+}
+
+// A value every lane holds equally, moved to scalar registers.
+__device__ inline double uniform(double v) {
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+  const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(b));
+  const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(b >> 32));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
 
 // Forward x-y butterfly of a 4-node face (corner q = bx + 2 by, 3 components).
@@ -96,8 +105,10 @@ __device__ inline void modal_core(const T (&A)[24], T (&v)[24], T e, const Modal
 // slab's planes in plane order and the workgroup adds its four waves in a
 // fixed order, so the partials (and every PCG scalar formed from them) do not
 // depend on the z-chunk the autotuner picks.
-template <typename T, int EPI, int DOTK>
-__global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
+// PU: the fused PCG direction update of the FP64 A.p (KuOp::pw), with z in
+// FP32 (1) or FP64 (2); 0 applies x as it is.
+template <typename T, int EPI, int DOTK, int PU = 0>
+__global__ void __launch_bounds__(kTX * kTY, PU != 0 ? 3 : 1) ku_face(KuOp<T> a) {
   constexpr int TX = kTX, TY = kTY;
   constexpr int NT = TX * TY;
   constexpr int PW = TX + 1;                  // staged plane: PW x (TY + 1) nodes
@@ -134,6 +145,47 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
 
   const int t = threadIdx.x;
   const int tx = t % TX, ty = t / TX;
+  // Direction update p = z + beta * x: beta from the <r,z> partials, summed in
+  // the same fixed order by every workgroup (lanes stride the partials, waves
+  // reduce by shuffles, the four wave sums are added in wave order). Formed
+  // after the first plane's loads are issued, so its latency overlaps theirs;
+  // false: a breakdown (non-finite <r,z>), recorded by the case's first tile.
+  double beta = 0.0;
+  bool fresh = false;
+  if constexpr (PU != 0) fresh = a.st[rc].pfresh != 0;
+  auto form_beta = [&]() -> bool {
+    if constexpr (PU != 0) {
+      if (fresh) return true;
+      CgState* cs = a.st + rc;
+      const double* rp = a.rzp + static_cast<long long>(rc) * a.pstride;
+      double s = 0.0;
+      for (int i = t; i < a.nrz; i += NT) s += rp[i];
+      s = wave_sum(s);
+      if ((t & 63) == 0) s_red[t >> 6] = s;
+      __syncthreads();
+      double rzn = 0.0;
+#pragma unroll
+      for (int w = 0; w < NT / 64; ++w) rzn += s_red[w];
+      rzn = uniform(rzn);
+      __syncthreads();  // s_red is reused by the dot partials
+      const bool lead = bx == 0 && by == 0 && chunk == 0 && t == 0;
+      if (!isfinite(rzn)) {
+        if (lead) {
+          cs->rz_new = rzn;
+          cs->beta = 0.0;
+          cs->fail = 1;
+          cs->done = 1;
+        }
+        return false;
+      }
+      beta = uniform(rzn / cs->rz);
+      if (lead) {
+        cs->rz_new = rzn;
+        cs->beta = beta;
+      }
+    }
+    return true;
+  };
   const int i0 = bx * (TX - 1), j0 = by * (TY - 1);
   const int k0 = chunk * a.tz;
   const int kend = min(k0 + a.tz, g.nnz);
@@ -152,6 +204,12 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
   const double* __restrict__ rd = a.rdot ? a.rdot + vo : nullptr;
   const uint8_t* __restrict__ mask = a.mask;
   const T* __restrict__ E = a.E;
+  using TZ = typename std::conditional<PU == 1, float, double>::type;
+  const TZ* __restrict__ zs = nullptr;
+  double* __restrict__ pws = nullptr;
+  if constexpr (PU == 1) zs = a.pz32 + vo;
+  if constexpr (PU == 2) zs = a.pz64 + vo;
+  if constexpr (PU != 0) pws = a.pw + vo;
   // Loop-invariant in-plane offsets (node 0 for a halo position outside the
   // grid, whose value is zeroed through `valid`).
   int foff[NPF];
@@ -173,9 +231,27 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
   const int noff = owner ? nb * g.nnx + na : 0;
 
   T pf[NPF], pd[NPF];  // plane values in flight (pd: D^-1 of EPI_RES0)
+  TZ pz[NPF];          // PU: z of the plane in flight
   bool pin = false;    // EPI_RES0: the plane in flight lies inside the grid
   auto fetch_plane = [&](int c, T (&pf)[NPF]) {
-    if constexpr (EPI == EPI_RES0) {
+    if constexpr (PU != 0) {
+      // the previous direction (not read after a restart) and z
+      if (c >= 0 && c < g.nnz) {
+        const T* xp = xs + c * pn * 3;
+        const TZ* zp = zs + c * pn * 3;
+#pragma unroll
+        for (int i = 0; i < NPF; ++i) {
+          pf[i] = (valid >> i) & 1u && !fresh ? xp[foff[i]] : T(0);
+          pz[i] = (valid >> i) & 1u ? zp[foff[i]] : TZ(0);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < NPF; ++i) {
+          pf[i] = T(0);
+          pz[i] = TZ(0);
+        }
+      }
+    } else if constexpr (EPI == EPI_RES0) {
       // one damped-Jacobi sweep from zero, formed on the fly: x0 = omega D^-1 b.
       // The plane index is clamped into the grid and a plane outside it is
       // zeroed where the values are used, so pd is written in one place only.
@@ -206,6 +282,11 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
     for (int i = 0; i < NPF; ++i) {
       const int q = t + i * NT;
       T v = pf[i];
+      if constexpr (PU != 0) {
+        // p = z + beta * x (zero off the grid, where both loads were skipped)
+        const T z = static_cast<T>(pz[i]);
+        v = fresh ? z : fma(beta, v, z);
+      }
       if constexpr (EPI == EPI_RES0) {
         v = static_cast<T>(a.omega_d * static_cast<double>(pd[i]) * static_cast<double>(v));
         v = ((valid >> i) & 1u) && pin ? v : T(0);
@@ -276,6 +357,7 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
   };
 
   fetch_plane(k0 - 1, pf);
+  if (!form_beta()) return;
   store_plane(0, pf);
   fetch_plane(k0, pf);
   e0 = modulus(k0 - 1);
@@ -371,6 +453,7 @@ __global__ void __launch_bounds__(kTX * kTY) ku_face(KuOp<T> a) {
         if constexpr (DOTK == 2) dotp = fma(ow.r[c], static_cast<double>(out), dotp);
         yp[c] = out;
         if constexpr (EPI == EPI_RES0) xwp[c] = xk[c];
+        if constexpr (PU != 0) pws[(k * pn + noff) * 3 + c] = xk[c];
       }
     }
     if constexpr (DOTK != 0) {
@@ -415,6 +498,22 @@ void dispatch(const KuOp<T>& op, hipStream_t s) {
                           op.nrhs;
   const dim3 grid(xcd_grid(tiles));
   const bool dot = op.partials != nullptr;
+  if (op.pw) {
+    // fused PCG direction update: FP64 A.p with <p, Ap> partials only
+    if (sizeof(T) != 8 || op.epi != EPI_AX || !dot || !op.st || !op.rzp || !op.x ||
+        (op.pz32 == nullptr) == (op.pz64 == nullptr) ||
+        static_cast<const void*>(op.pw) == static_cast<const void*>(op.x))
+      throw HipError("K.u direction update needs the FP64 A.p with partials, the state, "
+                     "the <r,z> partials, the previous direction, one z and a separate output");
+    if constexpr (sizeof(T) == 8) {
+      if (op.pz32)
+        hipLaunchKernelGGL((ku_face<T, EPI_AX, 1, 1>), grid, dim3(kTX * kTY), 0, s, op);
+      else
+        hipLaunchKernelGGL((ku_face<T, EPI_AX, 1, 2>), grid, dim3(kTX * kTY), 0, s, op);
+    }
+    TOPOHIP_CHECK(hipGetLastError());
+    return;
+  }
 #define TOPOHIP_L(E, D) hipLaunchKernelGGL((ku_face<T, E, D>), grid, dim3(kTX * kTY), 0, s, op)
   if (op.epi == EPI_AX) {
     if (dot) TOPOHIP_L(EPI_AX, 1); else TOPOHIP_L(EPI_AX, 0);

@@ -340,11 +340,14 @@ __global__ void cg_init_k(CgState* st, const double* p, int n, int pstride, doub
     st->iters = 0;
     st->cap = cap;
     st->rr = bb;
+    st->pfresh = 1;
   }
 }
 
-// Restart: rz = rz_new = <r, z>. (In the iteration, cg_xr uses rz_new for
-// alpha, cg_check moves it to rz, and cg_p forms beta = <r,z>_new / rz.)
+// Restart: rz = rz_new = <r, z>, and the next A.p takes p = z. (In the
+// iteration, the A.p forms beta = <r,z>_new / rz from the preconditioner's
+// partials and records <r,z>_new, cg_xr uses it for alpha and clears the
+// restart mark, cg_check moves it to rz.)
 __global__ void cg_set_rz_k(CgState* st, const double* p, int n, int pstride) {
   st += blockIdx.x;
   p += static_cast<long long>(blockIdx.x) * pstride;
@@ -353,6 +356,7 @@ __global__ void cg_set_rz_k(CgState* st, const double* p, int n, int pstride) {
   if (threadIdx.x == 0) {
     st->rz = rz;
     st->rz_new = rz;
+    st->pfresh = 1;
     if (!isfinite(rz)) { st->fail = 1; st->done = 1; }
   }
 }
@@ -415,6 +419,7 @@ __global__ void cg_xr_k(CgState* st, double* __restrict__ x, double* __restrict_
   if (first) {
     st->pAp = pAp;
     st->alpha = alpha;
+    st->pfresh = 0;
   }
   double s = 0.0;
   for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
@@ -427,42 +432,6 @@ __global__ void cg_xr_k(CgState* st, double* __restrict__ x, double* __restrict_
   }
   s = block_sum(s);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
-}
-
-// p = z + beta p with beta = <r,z>_new / rz, <r,z>_new the fixed-order sum of
-// the case's partials (every block forms the same value; block 0 records it
-// and the breakdown test). restart: p = z.
-template <typename TZ>
-__global__ void cg_p_k(CgState* st, const TZ* __restrict__ z, double* __restrict__ p,
-                       long long n, int restart, const double* rzp, int nrz, int pstride) {
-  const int rc = blockIdx.y;
-  st += rc;
-  if (st->done) return;
-  z += rc * n;
-  p += rc * n;
-  double beta = 0.0;
-  if (!restart) {
-    const double rzn = fixed_sum_all(rzp + static_cast<long long>(rc) * pstride, nrz);
-    const bool first = blockIdx.x == 0 && threadIdx.x == 0;
-    if (!isfinite(rzn)) {
-      if (first) {
-        st->rz_new = rzn;
-        st->beta = 0.0;
-        st->fail = 1;
-        st->done = 1;
-      }
-      return;
-    }
-    beta = rzn / st->rz;
-    if (first) {
-      st->rz_new = rzn;
-      st->beta = beta;
-    }
-  }
-  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
-       i += static_cast<long long>(gridDim.x) * blockDim.x) {
-    p[i] = static_cast<double>(z[i]) + (restart ? 0.0 : beta * p[i]);
-  }
 }
 
 // ---- Chebyshev smoother ---------------------------------------------------------
@@ -734,19 +703,6 @@ template void launch_cg_xr<double, double>(CgState*, double*, double*, const dou
 template void launch_cg_xr<float, float>(CgState*, double*, double*, const float*, const float*,
                                          float*, long long, const double*, int, double*, int,
                                          hipStream_t, int, int);
-
-template <typename TZ>
-void launch_cg_p(CgState* st, const TZ* z, double* p, long long n, int restart,
-                 const double* rzp, int nrz, hipStream_t s, int nr, int pstride) {
-  // every block re-sums the <r,z> partials, so the grid stays at 1024 blocks
-  hipLaunchKernelGGL((cg_p_k<TZ>), dim3(grid_for(n, 256, 1024), nr), dim3(256), 0, s, st, z, p,
-                     n, restart, rzp, nrz, pstride);
-  TOPOHIP_CHECK(hipGetLastError());
-}
-template void launch_cg_p<float>(CgState*, const float*, double*, long long, int,
-                                 const double*, int, hipStream_t, int, int);
-template void launch_cg_p<double>(CgState*, const double*, double*, long long, int,
-                                  const double*, int, hipStream_t, int, int);
 
 int cheb_blocks(long long n) { return grid_for(n, RB * 4, 1024); }
 
