@@ -97,6 +97,24 @@ stencil_apply_k(Geom g, const T* __restrict__ S, const uint8_t* __restrict__ mas
 #pragma unroll
     for (int dx = 0; dx < 3; ++dx) mm[dx] = mask[m[dx]];
   }
+  // The three waves that finish the outputs load their epilogue operands
+  // now, so those loads overlap the row gathers instead of following the
+  // barrier as a second memory round trip (except the FP64 sweep, where the
+  // held operands would cost a wave per SIMD).
+  constexpr bool EARLY = sizeof(T) == 4 || EPI != EPI_JAC;
+  const int pe = threadIdx.x / SA_NODES;  // output component of a finishing wave
+  const bool fin = threadIdx.x < 3 * SA_NODES && live;
+  int mk_e = 7;
+  T b_e = T(0), x_e = T(0), d_e = T(0);
+  auto epilogue_operands = [&]() {
+    if (mask) mk_e = mask[n];
+    if constexpr (EPI != EPI_AX) b_e = b[n * 3 + pe];
+    if constexpr (EPI == EPI_JAC) {
+      x_e = x[n * 3 + pe];
+      d_e = dinv[n * 3 + pe];
+    }
+  };
+  if (EARLY && fin) epilogue_operands();
   T acc[3] = {T(0), T(0), T(0)};
 #pragma unroll
   for (int dx = 0; dx < 3; ++dx) {
@@ -126,18 +144,17 @@ stencil_apply_k(Geom g, const T* __restrict__ S, const uint8_t* __restrict__ mas
 #pragma unroll
   for (int p = 0; p < 3; ++p) s_acc[row][p][lane] = acc[p];
   __syncthreads();
-  if (threadIdx.x >= 3 * SA_NODES) return;
-  const int p = threadIdx.x / SA_NODES;
-  if (!live) return;
+  if (!fin) return;
+  if (!EARLY) epilogue_operands();
+  const int p = pe;
   T sum = s_acc[0][p][lane];
 #pragma unroll
   for (int r = 1; r < SA_ROWS; ++r) sum += s_acc[r][p][lane];
-  const int mk = mask ? mask[n] : 7;
   T out = T(0);
-  if ((mk >> p) & 1) {
+  if ((mk_e >> p) & 1) {
     if constexpr (EPI == EPI_AX) out = sum;
-    else if constexpr (EPI == EPI_RES) out = b[n * 3 + p] - sum;
-    else out = x[n * 3 + p] + omega * dinv[n * 3 + p] * (b[n * 3 + p] - sum);
+    else if constexpr (EPI == EPI_RES) out = b_e - sum;
+    else out = x_e + omega * d_e * (b_e - sum);
   }
   y[n * 3 + p] = out;
 }

@@ -121,8 +121,9 @@ __global__ void prolong_add_k(Geom gc, Geom gf, const Tc* __restrict__ xc,
   const int a = static_cast<int>(n % gf.nnx);
   const int b = static_cast<int>((n / gf.nnx) % gf.nny);
   const int c = static_cast<int>(n / (static_cast<long long>(gf.nnx) * gf.nny));
+  // (the mask only gates the stores: a branch on it would hold every load
+  // behind the mask's)
   const int mk = mask_f ? mask_f[n] : 7;
-  if (mk == 0) return;
   // The 1, 2, 4 or 8 coarse nodes of fine node (a, b, c), in a fixed order;
   // the second neighbour along an even axis is absent: its (in-grid) load is
   // weighted by zero, so every load is used and issued up front.
@@ -205,8 +206,7 @@ __global__ void prolong_add_pair_k(Geom gc, Geom gf, const Tc* __restrict__ xc,
   }
   auto node = [&](int a, bool odd) {
     const long long n = (static_cast<long long>(c) * gf.nny + b) * gf.nnx + a;
-    const int mk = mask_f ? mask_f[n] : 7;
-    if (mk == 0) return;
+    const int mk = mask_f ? mask_f[n] : 7;  // gates the stores only (see prolong_add_k)
     const double wa = odd ? 0.5 : 1.0;
     double s[3] = {0.0, 0.0, 0.0};
 #pragma unroll
