@@ -216,3 +216,37 @@ def test_solver_fallback_chain_follows_reference(case):
     assert abs(info["iterations"] - info_ref["iterations"]) <= 0.1 * info_ref["iterations"]
     if case == 0:
         assert info["mg_cycles_attempted"] > 300  # the FP32 V-cycle ran out first
+
+
+@pytest.mark.parametrize("precision", [ta.PREC_MIXED, ta.PREC_F64])
+def test_direction_update_paths_give_identical_bits(monkeypatch, precision):
+    """The PCG's direction update p = z + beta p runs inside the FP64 A.p on
+    grids whose apply fits in about one round of workgroups and as a separate
+    pass on larger ones; both form the same fixed-order beta and the same fma,
+    so forcing either path on one grid gives the same bits: a four-case block
+    solve (the bridge's loads) and a design loop whose solves restart on
+    residual replacement. The separate path is also checked against the
+    oracle, as the fused one is elsewhere."""
+    nx, ny, nz = 24, 8, 12
+    bcs, cases = problems.bridge(nx, ny, nz)
+    cases.append(problems.self_weight(nx, ny, nz))
+    x = np.random.default_rng(7).uniform(0.2, 1.0, nx * ny * nz)
+    x = oracle.filter(oracle.Setup(nx, ny, nz, bcs=bcs, loads=cases[0]), 1.5, x)
+    runs = {}
+    for mode in ("fused", "separate"):
+        monkeypatch.setenv("TOPOHIP_PCG_DIRECTION", mode)
+        with ta.Problem(nx, ny, nz, bcs, cases, precision=precision) as p:
+            c, dc, u, info = p.compliance(x, tol=1e-8, want_u=True)
+        cbcs, cload = problems.cantilever(32, 12, 12)
+        with ta.Problem(32, 12, 12, cbcs, cload, precision=precision) as p:
+            out = p.optimize(0.3, 1.5, 4, cg_tol=1e-8)
+        runs[mode] = (c, dc, u, info["iterations"], out["history"], out["compliance"])
+        print(mode, f"block c={c:.12g} its={info['iterations']} loop c={out['compliance']:.12g}",
+              flush=True)
+    f, sep = runs["fused"], runs["separate"]
+    assert f[0] == sep[0] and np.array_equal(f[1], sep[1]) and np.array_equal(f[2], sep[2])
+    assert f[3] == sep[3]
+    assert np.array_equal(f[4], sep[4]) and f[5] == sep[5]
+    c_ref = sum(oracle.simp_compliance(oracle.Setup(nx, ny, nz, bcs=bcs, loads=cs), x,
+                                       tol=1e-8)[0] for cs in cases)
+    assert abs(sep[0] - c_ref) / abs(c_ref) < 1e-6

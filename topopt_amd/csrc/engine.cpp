@@ -627,22 +627,52 @@ void precondition_rz(Problem& P, Prec pc, const int* done, int nc) {
                              P.pstride);
 }
 
+double* direction(Problem& P, int i) { return i == 0 ? P.p.p : P.p2.p; }
+
+// The PCG's A.p carries the direction update p = z + beta p while its grid
+// fits in about one round of resident workgroups (the FP64 apply holds three
+// waves per SIMD: three workgroups per CU). Past that the separate update is
+// cheaper: the launches then overlap one another's latency, and the apply's
+// extra z and p streams cost it more than the update kernel's own pass
+// (192x64x64: 700 tiles on 768 slots, fused 29.5 us against 22.4 + 11.2 us;
+// 512x256x256: 8177 tiles, fused 1184 us against 601 us + the update's pass).
+// TOPOHIP_PCG_DIRECTION=fused or =separate overrides the choice (both give
+// the same bits: the same fixed-order beta and the same fma).
+bool fused_direction(const Problem& P, int nc) {
+  if (const char* env = std::getenv("TOPOHIP_PCG_DIRECTION")) {
+    if (std::strcmp(env, "fused") == 0) return true;
+    if (std::strcmp(env, "separate") == 0) return false;
+  }
+  const Geom& g = P.g;
+  const long long tiles = static_cast<long long>((g.nnx + kTX - 2) / (kTX - 1)) *
+                          ((g.nny + kTY - 2) / (kTY - 1)) * ((g.nnz + P.tz64 - 1) / P.tz64) * nc;
+  return 2 * tiles <= 3LL * 3 * P.cus;
+}
+
 // Restart the recurrence of every case that is not done from its current r:
-// z = M^-1 r, rz = r.z, and the next A.p takes p = z (cg_set_rz marks it).
+// z = M^-1 r, rz = r.z, and p = z: the next A.p takes it (cg_set_rz marks it),
+// or, without the fused update, it is written to the current direction buffer.
 void restart(Problem& P, Prec pc, int nc) {
   const long long nd = P.g.ndof();
   const int* done = done_flags(P);
   if (pc == Prec::MG32) launch_convert<double, float>(P.r.p, P.r32.p, nd * nc, P.s);
   precondition_rz(P, pc, done, nc);
   launch_cg_set_rz(P.st.p, P.rzpart.p, rz_count(P, pc), P.s, nc, P.pstride);
+  if (fused_direction(P, nc)) return;
+  double* p = direction(P, P.pcur);
+  if (pc == Prec::MG32)
+    launch_cg_p<float>(P.st.p, P.z32.p, p, p, nd, 1, nullptr, 0, P.s, nc, P.pstride);
+  else
+    launch_cg_p<double>(P.st.p, P.z64.p, p, p, nd, 1, nullptr, 0, P.s, nc, P.pstride);
 }
 
-double* direction(Problem& P, int i) { return i == 0 ? P.p.p : P.p2.p; }
-
-// One PCG iteration whose current direction is in buffer par (the next one
-// goes to buffer par ^ 1): p = z + beta p fused into the FP64 A.p (beta from
-// the previous preconditioner's <r, z> partials; p = z after a restart), then
-// x, r and the stopping test, then z = M^-1 r with its <r, z> partials.
+// One PCG iteration at direction-buffer parity par (the next iteration runs
+// at par ^ 1). Fused (fused_direction): buffer par holds the previous
+// direction and the FP64 A.p forms p = z + beta p into buffer par ^ 1 (beta
+// from the previous preconditioner's <r, z> partials; p = z after a restart).
+// Separate: buffer par holds p, and cg_p writes the next direction into buffer
+// par ^ 1 after the preconditioner. Between them x, r and the stopping test,
+// then z = M^-1 r with its <r, z> partials.
 void iteration(Problem& P, Prec pc, int nc, int par) {
   const long long nd = P.g.ndof();
   const int* done = done_flags(P);
@@ -653,22 +683,38 @@ void iteration(Problem& P, Prec pc, int nc, int par) {
   // A.p is FP64 in every form (it defines the residual, as mf_mgpcg's
   // mf_fine_matvec, core/src/fea/multigrid.cpp:1286); the mixed form's FP32
   // V-cycle reads float(r), which cg_xr writes alongside r.
+  const bool fused = fused_direction(P, nc);
+  // the direction this iteration applies: formed by the A.p into buffer
+  // par ^ 1, or (separate update) already in buffer par
+  double* p = direction(P, fused ? par ^ 1 : par);
   KuOp<double> op = fine_op<double>(P, done, nc);
-  op.x = direction(P, par);
-  op.pw = direction(P, par ^ 1);
   op.y = P.Ap64.p;
   op.epi = EPI_AX;
   op.partials = P.partials.p;
-  if (pc == Prec::MG32) op.pz32 = P.z32.p; else op.pz64 = P.z64.p;
-  op.st = P.st.p;
-  op.rzp = P.rzpart.p;
-  op.nrz = rz_count(P, pc);
+  if (fused) {
+    op.x = direction(P, par);
+    op.pw = p;
+    if (pc == Prec::MG32) op.pz32 = P.z32.p; else op.pz64 = P.z64.p;
+    op.st = P.st.p;
+    op.rzp = P.rzpart.p;
+    op.nrz = rz_count(P, pc);
+  } else {
+    op.x = p;
+  }
   launch_ku_fine<double>(op, P.s);
-  launch_cg_xr<double, double>(P.st.p, P.x.p, P.r.p, direction(P, par ^ 1), P.Ap64.p,
+  launch_cg_xr<double, double>(P.st.p, P.x.p, P.r.p, p, P.Ap64.p,
                                pc == Prec::MG32 ? P.r32.p : nullptr, nd, P.partials.p, kb,
                                P.partials2.p, db, P.s, nc, P.pstride);
   launch_cg_check(P.st.p, P.partials2.p, db, P.s, nc, P.pstride);
   precondition_rz(P, pc, done, nc);
+  if (fused) return;
+  // the next direction z + beta p into buffer par ^ 1, as the fused A.p would
+  if (pc == Prec::MG32)
+    launch_cg_p<float>(P.st.p, P.z32.p, p, direction(P, par ^ 1), nd, 0, P.rzpart.p,
+                       rz_count(P, pc), P.s, nc, P.pstride);
+  else
+    launch_cg_p<double>(P.st.p, P.z64.p, p, direction(P, par ^ 1), nd, 0, P.rzpart.p,
+                        rz_count(P, pc), P.s, nc, P.pstride);
 }
 
 void read_state(Problem& P, int nc) { d2h(P.st_h, P.st.p, static_cast<size_t>(nc), P.s); }
@@ -1141,6 +1187,7 @@ std::unique_ptr<Problem> create_problem(int nx, int ny, int nz, double h, double
 
   TOPOHIP_CHECK(hipSetDevice(device));
   P->device = device;
+  TOPOHIP_CHECK(hipDeviceGetAttribute(&P->cus, hipDeviceAttributeMultiprocessorCount, device));
   P->scratch = noalloc_scratch(device);  // set-up of the allocation-free entry point
   {
     std::lock_guard<std::recursive_mutex> lk(device_api_mutex());

@@ -48,6 +48,7 @@ struct Timers {
 
 struct Problem {
   int device = 0;
+  int cus = 256;  // compute units of the device
   hipStream_t s = nullptr;
   void* scratch = nullptr;  // this device's allocation-free work array (noalloc_scratch)
   Geom g;

@@ -143,6 +143,13 @@ void launch_cg_init(CgState* st, const double* bb_partials, int nb, double tol, 
 // rr = sum(partials) of a recomputed residual; done = rr <= thresh2 (resume).
 void launch_cg_resume(CgState* st, const double* partials, int n, hipStream_t s, int nr = 1,
                       int pstride = 0);
+// p_out = z + beta*p_in with rz_new = fixed-order sum of the nrz <r,z> partials
+// at rzp and beta = rz_new / rz (restart: p_out = z, partials unused); the
+// direction update of grids whose A.p does not carry it (p_in != p_out).
+template <typename TZ>
+void launch_cg_p(CgState* st, const TZ* z, const double* p_in, double* p_out, long long n,
+                 int restart, const double* rzp, int nrz, hipStream_t s, int nr = 1,
+                 int pstride = 0);
 // restart: rz = rz_new = sum(partials); the next A.p takes p = z (pfresh)
 void launch_cg_set_rz(CgState* st, const double* partials, int n, hipStream_t s, int nr = 1,
                       int pstride = 0);

@@ -434,6 +434,44 @@ __global__ void cg_xr_k(CgState* st, double* __restrict__ x, double* __restrict_
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// The separate direction update (grids too large for the A.p to carry it):
+// p_out = z + beta p_in with beta = <r,z>_new / rz, <r,z>_new the fixed-order
+// sum of the case's partials (every block forms the same value; block 0
+// records it and the breakdown test). restart: p_out = z.
+template <typename TZ>
+__global__ void cg_p_k(CgState* st, const TZ* __restrict__ z, const double* __restrict__ p_in,
+                       double* __restrict__ p_out, long long n, int restart, const double* rzp,
+                       int nrz, int pstride) {
+  const int rc = blockIdx.y;
+  st += rc;
+  if (st->done) return;
+  z += rc * n;
+  p_in += rc * n;
+  p_out += rc * n;
+  double beta = 0.0;
+  if (!restart) {
+    const double rzn = fixed_sum_all(rzp + static_cast<long long>(rc) * pstride, nrz);
+    const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+    if (!isfinite(rzn)) {
+      if (first) {
+        st->rz_new = rzn;
+        st->beta = 0.0;
+        st->fail = 1;
+        st->done = 1;
+      }
+      return;
+    }
+    beta = rzn / st->rz;
+    if (first) {
+      st->rz_new = rzn;
+      st->beta = beta;
+    }
+  }
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    p_out[i] = restart ? static_cast<double>(z[i]) : fma(beta, p_in[i], static_cast<double>(z[i]));
+}
+
 // ---- Chebyshev smoother ---------------------------------------------------------
 // coef (device, per level): [0] lambda_max estimate, [1] 1/theta, then for
 // step k = 1..degree-1 the pair (c1_k, c2_k) at [2k], [2k+1].
@@ -703,6 +741,19 @@ template void launch_cg_xr<double, double>(CgState*, double*, double*, const dou
 template void launch_cg_xr<float, float>(CgState*, double*, double*, const float*, const float*,
                                          float*, long long, const double*, int, double*, int,
                                          hipStream_t, int, int);
+
+template <typename TZ>
+void launch_cg_p(CgState* st, const TZ* z, const double* p_in, double* p_out, long long n,
+                 int restart, const double* rzp, int nrz, hipStream_t s, int nr, int pstride) {
+  // every block re-sums the <r,z> partials, so the grid stays at 1024 blocks
+  hipLaunchKernelGGL((cg_p_k<TZ>), dim3(grid_for(n, 256, 1024), nr), dim3(256), 0, s, st, z, p_in,
+                     p_out, n, restart, rzp, nrz, pstride);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+template void launch_cg_p<float>(CgState*, const float*, const double*, double*, long long, int,
+                                 const double*, int, hipStream_t, int, int);
+template void launch_cg_p<double>(CgState*, const double*, const double*, double*, long long, int,
+                                  const double*, int, hipStream_t, int, int);
 
 int cheb_blocks(long long n) { return grid_for(n, RB * 4, 1024); }
 
