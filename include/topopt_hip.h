@@ -573,6 +573,20 @@ int topohip_smoother_lambdas(topohip_problem* p, int n, double* lambdas, int* le
  * bit-identical to a disarmed one. */
 double topohip_set_solve_deadline_ms(double absolute_steady_ms);
 double topohip_steady_ms(void);
+/* The per-run multigrid stagnation latch of the CALLING THREAD:
+ * fea_matfree_reset_mg_stagnation_latch / fea_matfree_mg_stagnation_latched
+ * (core/include/topopt/fea.hpp:1081-1091; core/src/fea/multigrid.cpp:123-150,
+ * 2138-2155, 2261-2275). A solve that builds a hierarchy and does not converge
+ * with it (FP32 V-cycle, then the FP64 retry, each within the 300-cycle
+ * budget) is a stagnation; 3 consecutive stagnations latch multigrid off, so
+ * later solves of the thread skip the hierarchy build and go straight to
+ * Jacobi-CG (hier_built = 0, used_multigrid = 0), and a converging solve
+ * clears the count. A block solve counts its load cases in order, as one-case
+ * solves would, and the latch holds from the next solve on. Sticky: the
+ * driver resets it at the start of each run; the query returns 1 when
+ * latched. */
+void topohip_reset_mg_stagnation_latch(void);
+int topohip_mg_stagnation_latched(void);
 /* SimpOptions::cancel (core/include/topopt/simp.hpp:1505-1521): a caller-owned
  * flag (NULL clears) polled once per design iteration, at its start, by
  * topohip_optimize, topohip_optimize_masked, topohip_optimize_mma and

@@ -23,3 +23,14 @@ def _built_libraries():
         from topopt_amd import build
         build.build()
     yield
+
+
+@pytest.fixture(autouse=True)
+def _fresh_mg_latch(request):
+    """Each GPU test starts as a new run: the multigrid stagnation latch is
+    per thread and sticky (the reference's driver resets it at run start), and
+    every test runs on the same thread."""
+    if request.node.get_closest_marker("gpu") is not None:
+        import topopt_amd
+        topopt_amd.reset_mg_stagnation_latch()
+    yield

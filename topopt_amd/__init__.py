@@ -610,6 +610,18 @@ def steady_ms():
     return f()
 
 
+def reset_mg_stagnation_latch():
+    """fea_matfree_reset_mg_stagnation_latch for the calling thread: the run's
+    driver calls it at run start (the latch is sticky otherwise)."""
+    lib().topohip_reset_mg_stagnation_latch()
+
+
+def mg_stagnation_latched():
+    """fea_matfree_mg_stagnation_latched: whether the calling thread's solves
+    skip multigrid after 3 consecutive stagnated solves."""
+    return bool(lib().topohip_mg_stagnation_latched())
+
+
 def version():
     return lib().topohip_version().decode()
 

@@ -334,6 +334,10 @@ double topohip_set_solve_deadline_ms(double absolute_steady_ms) {
 
 double topohip_steady_ms(void) { return steady_ms(); }
 
+void topohip_reset_mg_stagnation_latch(void) { reset_mg_stagnation_latch(); }
+
+int topohip_mg_stagnation_latched(void) { return mg_stagnation_latched() ? 1 : 0; }
+
 int topohip_set_smoother(topohip_problem* p, int kind, int degree) {
   return guarded([&] {
     need(p, "problem");

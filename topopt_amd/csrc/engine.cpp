@@ -54,6 +54,18 @@ bool deadline_passed() {
   return t_solve_deadline_ms > 0.0 && now_ms() >= t_solve_deadline_ms;
 }
 
+// The per-run multigrid stagnation latch of the calling thread
+// (core/src/fea/multigrid.cpp:123-150, 2138-2155, 2261-2275; reset and query
+// core/include/topopt/fea.hpp:1081-1091): a solve that builds a hierarchy and
+// does not converge with it (FP32 V-cycle, then the FP64 retry) is a
+// stagnation; kMgLatchThreshold consecutive ones latch multigrid off, so
+// later solves of the thread skip the hierarchy and go straight to Jacobi-CG,
+// and a solve that converges clears the count. Sticky until the caller
+// resets it at the start of a run.
+constexpr int kMgLatchThreshold = 3;
+thread_local int t_mg_stagnations = 0;
+thread_local bool t_mg_latched = false;
+
 template <typename T>
 void h2d(T* d, const T* h, size_t n, hipStream_t s) {
   TOPOHIP_CHECK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s));
@@ -940,6 +952,13 @@ void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool
 
 }  // namespace
 
+void reset_mg_stagnation_latch() {
+  t_mg_stagnations = 0;
+  t_mg_latched = false;
+}
+
+bool mg_stagnation_latched() { return t_mg_latched; }
+
 double set_solve_deadline_ms(double absolute_steady_ms) {
   const double prev = t_solve_deadline_ms;
   t_solve_deadline_ms = absolute_steady_ms > 0.0 ? absolute_steady_ms : 0.0;
@@ -1329,7 +1348,8 @@ void build_hierarchy(Problem& P, Timers& t) {
   Level& f = *P.L[0];
   launch_diag_fine<double>(f.g, P.E64.p, P.dkdiag.p, f.mask.p, f.dinv64.p, P.s);
   if (mixed) launch_diag_fine<float>(f.g, P.E64.p, P.dkdiag.p, f.mask.p, f.dinv32.p, P.s);
-  if (P.mg_plan_ok) {
+  // (a latched thread skips the levels: the next solve goes straight to Jacobi-CG)
+  if (P.mg_plan_ok && !t_mg_latched) {
     for (size_t l = 1; l < P.L.size(); ++l) {
       Level& c = *P.L[l];
       Level& pf = *P.L[l - 1];
@@ -1458,7 +1478,8 @@ void solve_cases(Problem& P, int nc, double tol, int max_it, const double* d_gue
     std::vector<int> it(static_cast<size_t>(nc), 0);
     std::vector<double> res(static_cast<size_t>(nc), 0.0);
     std::unique_ptr<bool[]> ok(new bool[nc]());
-    if (P.mg_plan_ok) {
+    const bool try_mg = P.mg_plan_ok && !t_mg_latched;
+    if (try_mg) {
       const double tm = now_ms();
       const Prec pc = P.precision == TOPOHIP_PREC_MIXED ? Prec::MG32 : Prec::MG64;
       pcg(P, pc, nc, tol, std::min(cap, kMgIterBudget), warm, nullptr, it.data(), res.data(),
@@ -1480,7 +1501,7 @@ void solve_cases(Problem& P, int nc, double tol, int max_it, const double* d_gue
     }
     bool all_ok = true;
     for (int r = 0; r < nc; ++r) all_ok = all_ok && ok[r];
-    if (!all_ok && P.mg_plan_ok && P.precision == TOPOHIP_PREC_MIXED) {
+    if (!all_ok && try_mg && P.precision == TOPOHIP_PREC_MIXED) {
       // FP64 V-cycle retry of the cases the mixed solve did not finish.
       ensure_mg64(P);
       const double tm = now_ms();
@@ -1509,6 +1530,19 @@ void solve_cases(Problem& P, int nc, double tol, int max_it, const double* d_gue
           d[r].used_multigrid = 1;
           d[r].mg_levels = static_cast<int>(P.L.size());
           d[r].coarse_dofs = static_cast<int>(P.L.back()->nact);
+        }
+      }
+    }
+    // Latch bookkeeping when a hierarchy was built and attempted: each load
+    // case counts as the reference's one-case solve would, in case order (a
+    // converging case clears the count, a stagnating one advances it); the
+    // latch then holds from the next solve on.
+    if (try_mg) {
+      for (int r = 0; r < nc; ++r) {
+        if (ok[r]) {
+          t_mg_stagnations = 0;
+        } else if (++t_mg_stagnations >= kMgLatchThreshold) {
+          t_mg_latched = true;
         }
       }
     }
@@ -1680,16 +1714,23 @@ void oc_search(Problem& P, FilterCache& fc, const uint8_t* dset, double n_design
   const double target = volfrac * n_design;
   launch_oc_init_multi(w.st, target, P.s);
   const int ob = oc_blocks(nv);
-  // Each pass tries up to 7 lambdas (~100 bracket + 100 bisection trials at most).
+  // Each pass tries up to 7 lambdas (~100 bracket + 100 bisection trials at
+  // most). The first round launches as many passes as the previous search
+  // needed (consecutive design iterations bisect to similar depths), later
+  // rounds 3; passes after the search ends are no-ops, each round ends with
+  // a read-back of the state.
+  int passes = P.oc_pass_hint > 0 ? P.oc_pass_hint : 6;
   for (int round = 0;; ++round) {
-    if (round > 40) throw Error(TOPOHIP_EINTERNAL, "oc_update: lambda search did not finish");
-    for (int i = 0; i < 6; ++i) {
+    if (round > 80) throw Error(TOPOHIP_EINTERNAL, "oc_update: lambda search did not finish");
+    for (int i = 0; i < passes; ++i) {
       launch_oc_trial_multi(d_x, w.dcf, fc.dv.p, sol, nv, move, rho_min, w.st, w.lam, ob, P.s);
       launch_oc_decide_multi(w.st, w.lam, ob, P.s);
     }
     d2h(w.st_h, w.st, 1, P.s);
     if (w.st_h->phase == 2) break;
+    passes = 3;
   }
+  P.oc_pass_hint = w.st_h->passes;
   launch_oc_final(d_x, w.dcf, fc.dv.p, sol, nv, move, rho_min, w.st, d_xnew, d_active, w.chg, ob,
                   P.s, keep ? 1 : 0);
   launch_max_partials(w.chg, ob, w.scal, P.s);

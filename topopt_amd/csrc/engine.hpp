@@ -155,6 +155,7 @@ struct Problem {
   hipGraphExec_t cg_graph[kCgGraphs] = {};
   int cg_graph_nc[kCgGraphs] = {};  // case count captured
   int pcg_hint[3] = {0, 0, 0};  // iterations of the last solve per preconditioner kind
+  int oc_pass_hint = 0;         // multi-lambda passes the last OC search needed
 
   Problem() = default;
   Problem(const Problem&) = delete;
@@ -226,6 +227,10 @@ void hex8_stiffness_host(double E, double nu, double h, double* K576);
 void set_smoother(Problem& P, int kind, int degree);
 // fea_set_solve_deadline_ms for the calling thread; returns the previous value.
 double set_solve_deadline_ms(double absolute_steady_ms);
+// fea_matfree_reset_mg_stagnation_latch / fea_matfree_mg_stagnation_latched
+// for the calling thread.
+void reset_mg_stagnation_latch();
+bool mg_stagnation_latched();
 double steady_ms();
 // fea_von_mises_field of u_host (3 * nodes; null = P.u of the last solve) for
 // Young's modulus E: FP64 field into vm64 and/or its float narrowing into

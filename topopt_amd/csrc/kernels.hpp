@@ -224,7 +224,8 @@ struct OcState {
   double l1, l2, lmid, target, vol;
   int phase;      // 0 = bracketing, 1 = bisecting, 2 = done
   int steps;      // bisection steps taken
-  int nlam, pad;
+  int nlam;
+  int passes;     // multi-lambda passes that did work (oc_decide_multi)
   double lam[kLamPass];
   double gap;     // smallest |vol - target| over the search's volume decisions
 };

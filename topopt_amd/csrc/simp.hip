@@ -306,6 +306,7 @@ __global__ void oc_init_multi_k(OcState* st, double target) {
   st->vol = 0.0;
   st->phase = 0;
   st->steps = 0;
+  st->passes = 0;
   st->gap = INFINITY;
   oc_plan(st);
 }
@@ -355,6 +356,7 @@ __global__ void oc_decide_multi_k(OcState* st, const double* __restrict__ partia
   }
   block_sum_n<kLamPass>(S);
   if (threadIdx.x != 0) return;
+  st->passes += 1;
   auto stop = [&] { return !(st->steps < 100 && (st->l2 - st->l1) > 1e-6 * (st->l1 + st->l2)); };
   if (st->phase == 0) {
     for (int j = 0; j < nl; ++j) {
