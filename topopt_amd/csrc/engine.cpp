@@ -714,11 +714,15 @@ void iteration(Problem& P, Prec pc, int nc, int par) {
     op.x = p;
   }
   launch_ku_fine<double>(op, P.s);
+  // (Jacobi-CG: cg_xr also forms z = D^-1 r and the <r, z> partials)
+  const bool jac = pc == Prec::JACOBI;
   launch_cg_xr<double, double>(P.st.p, P.x.p, P.r.p, p, P.Ap64.p,
                                pc == Prec::MG32 ? P.r32.p : nullptr, nd, P.partials.p, kb,
-                               P.partials2.p, db, P.s, nc, P.pstride);
+                               P.partials2.p, db, P.s, nc, P.pstride,
+                               jac ? P.L[0]->dinv64.p : nullptr, jac ? P.z64.p : nullptr,
+                               jac ? P.rzpart.p : nullptr);
   launch_cg_check(P.st.p, P.partials2.p, db, P.s, nc, P.pstride);
-  precondition_rz(P, pc, done, nc);
+  if (!jac) precondition_rz(P, pc, done, nc);
   if (fused) return;
   // the next direction z + beta p into buffer par ^ 1, as the fused A.p would
   if (pc == Prec::MG32)

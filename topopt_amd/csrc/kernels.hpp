@@ -130,10 +130,14 @@ template <typename TI, typename TO> void launch_convert(const TI* in, TO* out, l
 // alpha = rz_new / pAp (pAp = fixed-order sum of the npap A.p partials at
 // pap, breakdown sets fail/done); x += alpha*p; r -= alpha*Ap; rr partials
 // to `partials` (a different buffer); r32 = float(r) (nullable).
+// With jz set (the Jacobi-preconditioned CG): also z = jdinv .* r into jz and the
+// <r, z> partials into rzp (case stride pstride), over nblocks = dot_blocks(n),
+// the partition launch_dot uses, so the sums are dot_k's.
 template <typename TP, typename TA>
 void launch_cg_xr(CgState* st, double* x, double* r, const TP* p, const TA* Ap, float* r32,
                   long long n, const double* pap, int npap, double* partials, int nblocks,
-                  hipStream_t s, int nr = 1, int pstride = 0);
+                  hipStream_t s, int nr = 1, int pstride = 0, const double* jdinv = nullptr,
+                  double* jz = nullptr, double* rzp = nullptr);
 // rr = sum(partials); iters++; done if rr <= thresh2; rz = rz_new
 void launch_cg_check(CgState* st, const double* partials, int n, hipStream_t s, int nr = 1,
                      int pstride = 0);

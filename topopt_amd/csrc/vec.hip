@@ -394,7 +394,8 @@ template <typename TP, typename TA>
 __global__ void cg_xr_k(CgState* st, double* __restrict__ x, double* __restrict__ r,
                         const TP* __restrict__ p, const TA* __restrict__ Ap, float* r32,
                         long long n, const double* pap, int npap, double* partials,
-                        int pstride) {
+                        int pstride, const double* __restrict__ jdinv, double* __restrict__ jz,
+                        double* __restrict__ rzp) {
   const int rc = blockIdx.y;
   st += rc;
   if (st->done) return;
@@ -403,6 +404,7 @@ __global__ void cg_xr_k(CgState* st, double* __restrict__ x, double* __restrict_
   p += rc * n;
   Ap += rc * n;
   if (r32) r32 += rc * n;
+  if (jz) jz += rc * n;
   const double pAp = fixed_sum_all(pap + static_cast<long long>(rc) * pstride, npap);
   partials += static_cast<long long>(rc) * pstride;
   const bool first = blockIdx.x == 0 && threadIdx.x == 0;
@@ -421,7 +423,7 @@ __global__ void cg_xr_k(CgState* st, double* __restrict__ x, double* __restrict_
     st->alpha = alpha;
     st->pfresh = 0;
   }
-  double s = 0.0;
+  double s = 0.0, s2 = 0.0;
   for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<long long>(gridDim.x) * blockDim.x) {
     x[i] += alpha * static_cast<double>(p[i]);
@@ -429,9 +431,21 @@ __global__ void cg_xr_k(CgState* st, double* __restrict__ x, double* __restrict_
     r[i] = ri;
     if (r32) r32[i] = static_cast<float>(ri);
     s += ri * ri;
+    if (jz) {
+      // the Jacobi preconditioner z = D^-1 r and <r, z>, as jacobi0_k (omega 1)
+      // and dot_k form them over the same block partition
+      const double zi = 1.0 * jdinv[i] * ri;
+      jz[i] = zi;
+      s2 += ri * zi;
+    }
   }
   s = block_sum(s);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
+  if (jz) {
+    __syncthreads();  // block_sum's LDS is reused
+    s2 = block_sum(s2);
+    if (threadIdx.x == 0) rzp[static_cast<long long>(rc) * pstride + blockIdx.x] = s2;
+  }
 }
 
 // The separate direction update (grids too large for the A.p to carry it):
@@ -730,17 +744,21 @@ TOPOHIP_CG_SCALAR(launch_cg_check, cg_check_k)
 template <typename TP, typename TA>
 void launch_cg_xr(CgState* st, double* x, double* r, const TP* p, const TA* Ap, float* r32,
                   long long n, const double* pap, int npap, double* partials, int nblocks,
-                  hipStream_t s, int nr, int pstride) {
+                  hipStream_t s, int nr, int pstride, const double* jdinv, double* jz,
+                  double* rzp) {
+  if (jz && (nblocks != dot_blocks(n) || !jdinv || !rzp))
+    throw HipError("cg_xr: the Jacobi step needs D^-1, the <r,z> partials and dot_blocks(n) blocks");
   hipLaunchKernelGGL((cg_xr_k<TP, TA>), dim3(nblocks, nr), dim3(RB), 0, s, st, x, r, p, Ap, r32,
-                     n, pap, npap, partials, pstride);
+                     n, pap, npap, partials, pstride, jdinv, jz, rzp);
   TOPOHIP_CHECK(hipGetLastError());
 }
 template void launch_cg_xr<double, double>(CgState*, double*, double*, const double*,
                                            const double*, float*, long long, const double*, int,
-                                           double*, int, hipStream_t, int, int);
+                                           double*, int, hipStream_t, int, int, const double*,
+                                           double*, double*);
 template void launch_cg_xr<float, float>(CgState*, double*, double*, const float*, const float*,
                                          float*, long long, const double*, int, double*, int,
-                                         hipStream_t, int, int);
+                                         hipStream_t, int, int, const double*, double*, double*);
 
 template <typename TZ>
 void launch_cg_p(CgState* st, const TZ* z, const double* p_in, double* p_out, long long n,
