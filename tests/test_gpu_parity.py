@@ -250,3 +250,48 @@ def test_direction_update_paths_give_identical_bits(monkeypatch, precision):
     c_ref = sum(oracle.simp_compliance(oracle.Setup(nx, ny, nz, bcs=bcs, loads=cs), x,
                                        tol=1e-8)[0] for cs in cases)
     assert abs(sep[0] - c_ref) / abs(c_ref) < 1e-6
+
+
+def test_paired_solution_updates_settle_every_stop():
+    """cg_xr applies the solution update x += alpha p every other iteration,
+    two directions at a time; a recurrence that stops after an odd count owes
+    the last one, applied once after the loop. In the FP64 form the device runs
+    the oracle's recurrence (core/src/fea/multigrid.cpp:1250-1300), so across a
+    ladder of tolerances that stops at odd and even counts the counts match and
+    the displacement agrees far below the tolerance: a dropped or doubled
+    update moves it by about the tolerance. The four-case block (cases stopping
+    at different iterations) is checked through the summed sensitivities."""
+    nx, ny, nz = 32, 16, 16
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    E = np.full(nx * ny * nz, 0.3 ** 3)
+    s = oracle.Setup(nx, ny, nz, bcs=bcs, loads=loads)
+    parities = set()
+    with ta.Problem(nx, ny, nz, bcs, loads, precision=ta.PREC_F64) as p:
+        for tol in (1e-2, 3e-3, 1e-3, 3e-4, 1e-4, 3e-5, 1e-5, 3e-6, 1e-6):
+            u_ref, info_ref = oracle.solve(s, E, tol=tol)
+            u, info = p.solve(E, tol=tol)
+            err = np.abs(u - u_ref).max() / np.abs(u_ref).max()
+            print(f"tol={tol:.0e} its={info['iterations']} (oracle {info_ref['iterations']}) "
+                  f"repl={info['residual_replacements']} err={err:.3e}", flush=True)
+            assert info["iterations"] == info_ref["iterations"]
+            assert info["residual"] <= tol
+            assert err < 1e-3 * tol
+            parities.add(info["iterations"] % 2)
+    assert parities == {0, 1}
+
+    nx, ny, nz = 24, 8, 12
+    bcs, cases = problems.bridge(nx, ny, nz)
+    cases.append(problems.self_weight(nx, ny, nz))
+    x = np.random.default_rng(3).uniform(0.2, 1.0, nx * ny * nz)
+    x = oracle.filter(oracle.Setup(nx, ny, nz, bcs=bcs, loads=cases[0]), 1.5, x)
+    for tol in (1e-3, 1e-4, 1e-5):
+        with ta.Problem(nx, ny, nz, bcs, cases, precision=ta.PREC_F64) as p:
+            c, dc, _, info = p.compliance(x, tol=tol)
+        refs = [oracle.simp_compliance(oracle.Setup(nx, ny, nz, bcs=bcs, loads=cs), x, tol=tol)
+                for cs in cases]
+        its = [r[3]["iterations"] for r in refs]
+        dc_ref = sum(r[1] for r in refs)
+        derr = np.abs(dc - dc_ref).max() / np.abs(dc_ref).max()
+        print(f"block tol={tol:.0e} its={info['iterations']} oracle its={its} "
+              f"dc err={derr:.3e}", flush=True)
+        assert derr < 1e-3 * tol

@@ -85,12 +85,13 @@ constexpr int kRefOfBit[8] = {0, 1, 3, 2, 4, 5, 7, 6};
 struct CgState {
   double rz, rz_new, pAp, rr, alpha, beta;
   double bnorm2, thresh2;
+  double alpha_prev;  // alpha of the last even iteration (its x update deferred)
   int done;       // recurrence residual below threshold (or failure)
   int fail;       // breakdown: p^T A p <= 0 or non-finite
   int iters;      // completed iterations
   int cap;        // iteration cap (fail = 2 when reached unconverged)
   int pfresh;     // set at a restart: the next A.p takes p = z (beta = 0)
-  int pad;
+  int it0;        // iters at the last restart: iteration iters - it0 is even or odd
 };
 
 // MI355X places consecutive workgroup ids on its 8 XCDs round-robin, and each

@@ -641,6 +641,10 @@ void precondition_rz(Problem& P, Prec pc, const int* done, int nc) {
 
 double* direction(Problem& P, int i) { return i == 0 ? P.p.p : P.p2.p; }
 
+// Paired solution updates in cg_xr (up to 32 cases: the width of the masks
+// that settle a stopped case's deferred update).
+bool paired_updates(int nc) { return nc <= 32; }
+
 // The PCG's A.p carries the direction update p = z + beta p while its grid
 // fits in about one round of resident workgroups (the FP64 apply holds three
 // waves per SIMD: three workgroups per CU). Past that the separate update is
@@ -714,13 +718,16 @@ void iteration(Problem& P, Prec pc, int nc, int par) {
     op.x = p;
   }
   launch_ku_fine<double>(op, P.s);
-  // (Jacobi-CG: cg_xr also forms z = D^-1 r and the <r, z> partials)
+  // (Jacobi-CG: cg_xr also forms z = D^-1 r and the <r, z> partials; the
+  // solution updates are paired over two iterations: the previous direction
+  // is the other buffer, fused or not)
   const bool jac = pc == Prec::JACOBI;
   launch_cg_xr<double, double>(P.st.p, P.x.p, P.r.p, p, P.Ap64.p,
                                pc == Prec::MG32 ? P.r32.p : nullptr, nd, P.partials.p, kb,
                                P.partials2.p, db, P.s, nc, P.pstride,
                                jac ? P.L[0]->dinv64.p : nullptr, jac ? P.z64.p : nullptr,
-                               jac ? P.rzpart.p : nullptr);
+                               jac ? P.rzpart.p : nullptr,
+                               paired_updates(nc) ? direction(P, fused ? par : par ^ 1) : nullptr);
   launch_cg_check(P.st.p, P.partials2.p, db, P.s, nc, P.pstride);
   if (!jac) precondition_rz(P, pc, done, nc);
   if (fused) return;
@@ -865,6 +872,25 @@ void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool
     return need;
   };
   remember();
+  // Paired solution updates: case r's recurrence restarted at iteration it0
+  // from direction parity par0; its iteration i ran at parity par0 ^ (i & 1)
+  // and applied the direction in buffer (that parity) ^ 1 when the A.p forms it
+  // (fused), in buffer (that parity) otherwise. A recurrence that stopped after
+  // an odd count of iterations owes its last, deferred update.
+  const bool fusedd = fused_direction(P, nc);
+  int par0 = P.pcur;
+  auto finish_x = [&]() {
+    if (!paired_updates(nc)) return;
+    unsigned pend = 0, par = 0;
+    for (int r = 0; r < nc; ++r) {
+      const int c = P.st_h[r].iters - P.st_h[r].it0;
+      if ((c & 1) == 0) continue;
+      pend |= 1u << r;
+      if ((par0 ^ ((c - 1) & 1) ^ (fusedd ? 1 : 0)) != 0) par |= 1u << r;
+      P.st_h[r].it0 = P.st_h[r].iters;  // the kernel does the same on the device
+    }
+    launch_cg_xfinish(P.st.p, P.x.p, P.p.p, P.p2.p, nd, pend, par, P.s, nc);
+  };
   int next = std::max(kBatch, std::min(64, P.pcg_hint[gi] - 3));
   for (long long guard = 0;; ++guard) {
     if (guard > 4LL * cap + 64) throw Error(TOPOHIP_EINTERNAL, "pcg: iteration state stalled");
@@ -872,6 +898,7 @@ void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool
     // reference polls every 256 iterations, matfree.cpp:988-1001); when it has
     // passed the unfinished cases stop with the state they reached.
     if (deadline_passed()) {
+      finish_x();
       for (int r = 0; r < nc; ++r) {
         if (ok[r] || (skip && skip[r])) continue;
         iters[r] = P.st_h[r].iters;
@@ -895,6 +922,7 @@ void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool
       }
     }
     // Every recurrence has stopped: confirm with the FP64 true residuals.
+    finish_x();
     bool any_pending = false;
     for (int r = 0; r < nc; ++r)
       if (!ok[r] && !P.st_h[r].fail) any_pending = true;
@@ -938,6 +966,7 @@ void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool
     restart(P, pc, nc);
     read_state(P, nc);
     remember();
+    par0 = P.pcur;
     next = 2;  // a residual replacement near the threshold needs a few more
   }
   int most = 0;

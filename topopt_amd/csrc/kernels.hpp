@@ -133,11 +133,20 @@ template <typename TI, typename TO> void launch_convert(const TI* in, TO* out, l
 // With jz set (the Jacobi-preconditioned CG): also z = jdinv .* r into jz and the
 // <r, z> partials into rzp (case stride pstride), over nblocks = dot_blocks(n),
 // the partition launch_dot uses, so the sums are dot_k's.
+// With p_prev set the solution updates are paired: iteration j = iters - it0
+// since the last restart defers its x += alpha p when j is even (recording
+// alpha in alpha_prev) and applies x += alpha_prev p_prev, then x += alpha p,
+// when j is odd (p_prev: the previous iteration's direction). A recurrence
+// that stops after an even iteration owes its last update: launch_cg_xfinish.
 template <typename TP, typename TA>
 void launch_cg_xr(CgState* st, double* x, double* r, const TP* p, const TA* Ap, float* r32,
                   long long n, const double* pap, int npap, double* partials, int nblocks,
                   hipStream_t s, int nr = 1, int pstride = 0, const double* jdinv = nullptr,
-                  double* jz = nullptr, double* rzp = nullptr);
+                  double* jz = nullptr, double* rzp = nullptr, const TP* p_prev = nullptr);
+// x += alpha_prev * p for the cases r with bit r of pend set, p from p1 where bit
+// r of par is set (else p0); then it0 = iters for them (nr <= 32 cases).
+void launch_cg_xfinish(CgState* st, double* x, const double* p0, const double* p1, long long n,
+                       unsigned pend, unsigned par, hipStream_t s, int nr = 1);
 // rr = sum(partials); iters++; done if rr <= thresh2; rz = rz_new
 void launch_cg_check(CgState* st, const double* partials, int n, hipStream_t s, int nr = 1,
                      int pstride = 0);

@@ -341,6 +341,7 @@ __global__ void cg_init_k(CgState* st, const double* p, int n, int pstride, doub
     st->cap = cap;
     st->rr = bb;
     st->pfresh = 1;
+    st->it0 = 0;
   }
 }
 
@@ -357,6 +358,7 @@ __global__ void cg_set_rz_k(CgState* st, const double* p, int n, int pstride) {
     st->rz = rz;
     st->rz_new = rz;
     st->pfresh = 1;
+    st->it0 = st->iters;
     if (!isfinite(rz)) { st->fail = 1; st->done = 1; }
   }
 }
@@ -395,13 +397,20 @@ __global__ void cg_xr_k(CgState* st, double* __restrict__ x, double* __restrict_
                         const TP* __restrict__ p, const TA* __restrict__ Ap, float* r32,
                         long long n, const double* pap, int npap, double* partials,
                         int pstride, const double* __restrict__ jdinv, double* __restrict__ jz,
-                        double* __restrict__ rzp) {
+                        double* __restrict__ rzp, const TP* __restrict__ p_prev) {
   const int rc = blockIdx.y;
   st += rc;
   if (st->done) return;
+  // Paired solution updates (p_prev set): an even iteration since the last
+  // restart defers its x += alpha p, the next one applies both in sequence.
+  const int j = st->iters - st->it0;
+  const bool defer = p_prev && (j & 1) == 0;
+  const bool pair = p_prev && (j & 1) == 1;
+  const double alpha_prev = pair ? st->alpha_prev : 0.0;
   x += rc * n;
   r += rc * n;
   p += rc * n;
+  if (p_prev) p_prev += rc * n;
   Ap += rc * n;
   if (r32) r32 += rc * n;
   if (jz) jz += rc * n;
@@ -422,11 +431,17 @@ __global__ void cg_xr_k(CgState* st, double* __restrict__ x, double* __restrict_
     st->pAp = pAp;
     st->alpha = alpha;
     st->pfresh = 0;
+    if (defer) st->alpha_prev = alpha;
   }
   double s = 0.0, s2 = 0.0;
   for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<long long>(gridDim.x) * blockDim.x) {
-    x[i] += alpha * static_cast<double>(p[i]);
+    if (pair) {
+      const double xp = x[i] + alpha_prev * static_cast<double>(p_prev[i]);
+      x[i] = xp + alpha * static_cast<double>(p[i]);
+    } else if (!defer) {
+      x[i] += alpha * static_cast<double>(p[i]);
+    }
     const double ri = r[i] - alpha * static_cast<double>(Ap[i]);
     r[i] = ri;
     if (r32) r32[i] = static_cast<float>(ri);
@@ -446,6 +461,26 @@ __global__ void cg_xr_k(CgState* st, double* __restrict__ x, double* __restrict_
     s2 = block_sum(s2);
     if (threadIdx.x == 0) rzp[static_cast<long long>(rc) * pstride + blockIdx.x] = s2;
   }
+}
+
+// The deferred solution update a stopped recurrence owes (an odd count of
+// iterations since its restart): x += alpha_prev p for the cases in the
+// pending mask, p from buffer p1 for the cases in the parity mask (else p0);
+// each case's first block then moves it0 to iters, so nothing is owed twice.
+__global__ void cg_xfinish_k(CgState* st, double* __restrict__ x, const double* __restrict__ p0,
+                             const double* __restrict__ p1, long long n, unsigned pend,
+                             unsigned par) {
+  const int rc = blockIdx.y;
+  if (!((pend >> rc) & 1u)) return;
+  st += rc;
+  const double a = st->alpha_prev;
+  const double* p = ((par >> rc) & 1u) ? p1 : p0;
+  x += rc * n;
+  p += rc * n;
+  for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    x[i] += a * p[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->it0 = st->iters;
 }
 
 // The separate direction update (grids too large for the A.p to carry it):
@@ -745,20 +780,29 @@ template <typename TP, typename TA>
 void launch_cg_xr(CgState* st, double* x, double* r, const TP* p, const TA* Ap, float* r32,
                   long long n, const double* pap, int npap, double* partials, int nblocks,
                   hipStream_t s, int nr, int pstride, const double* jdinv, double* jz,
-                  double* rzp) {
+                  double* rzp, const TP* p_prev) {
   if (jz && (nblocks != dot_blocks(n) || !jdinv || !rzp))
     throw HipError("cg_xr: the Jacobi step needs D^-1, the <r,z> partials and dot_blocks(n) blocks");
   hipLaunchKernelGGL((cg_xr_k<TP, TA>), dim3(nblocks, nr), dim3(RB), 0, s, st, x, r, p, Ap, r32,
-                     n, pap, npap, partials, pstride, jdinv, jz, rzp);
+                     n, pap, npap, partials, pstride, jdinv, jz, rzp, p_prev);
   TOPOHIP_CHECK(hipGetLastError());
 }
 template void launch_cg_xr<double, double>(CgState*, double*, double*, const double*,
                                            const double*, float*, long long, const double*, int,
                                            double*, int, hipStream_t, int, int, const double*,
-                                           double*, double*);
+                                           double*, double*, const double*);
 template void launch_cg_xr<float, float>(CgState*, double*, double*, const float*, const float*,
                                          float*, long long, const double*, int, double*, int,
-                                         hipStream_t, int, int, const double*, double*, double*);
+                                         hipStream_t, int, int, const double*, double*, double*,
+                                         const float*);
+
+void launch_cg_xfinish(CgState* st, double* x, const double* p0, const double* p1, long long n,
+                       unsigned pend, unsigned par, hipStream_t s, int nr) {
+  if (!pend) return;
+  hipLaunchKernelGGL(cg_xfinish_k, dim3(grid_for(n, 256, 2048), nr), dim3(256), 0, s, st, x, p0,
+                     p1, n, pend, par);
+  TOPOHIP_CHECK(hipGetLastError());
+}
 
 template <typename TZ>
 void launch_cg_p(CgState* st, const TZ* z, const double* p_in, double* p_out, long long n,
