@@ -273,6 +273,24 @@ int topohip_optimize_projected(topohip_problem* p, const topohip_options* opt, i
 int topohip_optimize_masked(topohip_problem* p, const uint8_t* mask, const topohip_options* opt,
                             double* design_out, double* xphys_out, double* history,
                             topohip_result* result);
+/* The masked loop with Heaviside projection stages: simp_optimize(grid,
+ * params, bcs, loads, options, mask) with options.projection
+ * (core/src/simp/simp.cpp:2989-3689; the projected branch :3196-3204,
+ * :3306-3313, :3411-3413, :3609-3611). Each iteration filters with the
+ * Active-only filter, projects the Active voxels (project_active,
+ * :2789-2794), pins, solves, and takes oc_update_masked_projected
+ * (:2801-2855: the chain rule through the projection on the Active set, the
+ * Active-only filter transposes, unclamped dv with 0/0 holding the voxel,
+ * frozen voxels keep x, the volume constraint on the projected Active density
+ * with target volfrac * n_active); the reported volume fraction is the
+ * projected Active density's. Stages, history and result as
+ * topohip_optimize_projected; the handle and mask as topohip_optimize_masked
+ * (xphys_out pinned). */
+int topohip_optimize_masked_projected(topohip_problem* p, const uint8_t* mask,
+                                      const topohip_options* opt, int n_stages, const double* beta,
+                                      const double* move, const int* iterations, double eta,
+                                      double* design_out, double* xphys_out, double* history,
+                                      topohip_result* result);
 /* make_density_filter(grid, radius, mask) applied to x (adjoint = 0:
  * filter_density; 1: filter_sensitivity), core/src/simp/simp.cpp:834-944. */
 int topohip_filter_masked(topohip_problem* p, double radius, const uint8_t* mask,

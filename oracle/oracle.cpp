@@ -1683,45 +1683,68 @@ SimpResult simp_optimize_masked(const Grid& g, const SimpParams& p,
   SimpResult res;
   std::vector<std::uint8_t> active(g.voxels(), 0);
   for (std::size_t e = 0; e < eff.size(); ++e) active[e] = eff[e] == kActive;
+  // Stage plan (build_stage_plan, simp.cpp:1053-1068), as in simp_optimize.
+  struct Stage { bool project; double beta, move; int iterations; };
+  std::vector<Stage> plan;
+  if (o.projection.empty()) plan.push_back({false, 0.0, o.move, o.max_iterations});
+  else
+    for (const ProjStage& ps : o.projection) plan.push_back({true, ps.beta, ps.move, ps.iterations});
+  if (o.mma && !o.projection.empty())
+    throw std::invalid_argument("simp_optimize: projection requires the OC updater");
+  const double eta = o.projection_eta;
   MmaState mma_state;
   int mma_iter = 0;
   std::vector<double> curve;
-  for (int it = 0; it < o.max_iterations; ++it) {
-    std::vector<double> xphys = f.filter_density(x);
-    apply_mask_pins(eff, xphys);
-    SimpParams tp = p;
-    if (!o.penalty_continuation.empty())
-      tp.penalty = penalty_at_iteration(o.penalty_continuation, res.iterations + 1);
-    SimpCompliance c;
-    try {
-      c = simp_compliance(analysis, tp, xphys, bcs, loads, o.cg_tolerance, o.cg_max_iterations,
-                          o.use_mg);
-    } catch (const SolverNonConvergence&) {
-      res.non_convergent = true;
-      break;
+  for (const Stage& st : plan) {
+    bool stage_converged = false;
+    for (int it = 0; it < st.iterations; ++it) {
+      // physical density: filter, project the Active voxels, pin (:3196-3204)
+      std::vector<double> xphys = f.filter_density(x);
+      if (st.project) project_active(eff, xphys, st.beta, eta);
+      apply_mask_pins(eff, xphys);
+      SimpParams tp = p;
+      if (!o.penalty_continuation.empty())
+        tp.penalty = penalty_at_iteration(o.penalty_continuation, res.iterations + 1);
+      SimpCompliance c;
+      try {
+        c = simp_compliance(analysis, tp, xphys, bcs, loads, o.cg_tolerance, o.cg_max_iterations,
+                            o.use_mg);
+      } catch (const SolverNonConvergence&) {
+        res.non_convergent = true;
+        break;
+      }
+      // core/src/simp/simp.cpp:3306-3313 (projected OC), 3351-3357 (MMA) or the OC update
+      std::vector<double> xn;
+      if (st.project)
+        xn = oc_update_masked_projected(g, f, eff, x, f.filter_density(x), c.dc, st.beta, eta,
+                                        vf_target, st.move, p.density_min);
+      else if (o.mma)
+        xn = mma_update(mma_state, ++mma_iter, g, f, x, c.dc, vf_target, st.move, p.density_min,
+                        nullptr, &active);
+      else
+        xn = oc_update_masked(g, f, eff, x, c.dc, vf_target, st.move, p.density_min);
+      double change = 0.0;
+      for (std::size_t e = 0; e < x.size(); ++e) change = std::max(change, std::fabs(xn[e] - x[e]));
+      x = xn;
+      if (res.iterations == 0) res.initial_compliance = c.compliance;
+      ++res.iterations;
+      std::vector<double> xa = f.filter_density(x);
+      if (st.project) project_active(eff, xa, st.beta, eta);
+      res.history.push_back({c.compliance, change, active_volfrac(xa), c.cg.iterations});
+      curve.push_back(c.compliance);
+      const bool stop = (o.mma && o.mma_plateau_window > 0)
+                            ? mma_objective_plateau(curve, o.mma_plateau_window, o.mma_plateau_tol,
+                                                    o.mma_plateau_min_drop,
+                                                    o.mma_plateau_flat_windows)
+                            : change < o.change_tol;
+      if (stop) { stage_converged = true; break; }
     }
-    // core/src/simp/simp.cpp:3351-3357 (MMA) or the OC update
-    const std::vector<double> xn =
-        o.mma ? mma_update(mma_state, ++mma_iter, g, f, x, c.dc, vf_target, o.move,
-                           p.density_min, nullptr, &active)
-              : oc_update_masked(g, f, eff, x, c.dc, vf_target, o.move, p.density_min);
-    double change = 0.0;
-    for (std::size_t e = 0; e < x.size(); ++e) change = std::max(change, std::fabs(xn[e] - x[e]));
-    x = xn;
-    if (res.iterations == 0) res.initial_compliance = c.compliance;
-    ++res.iterations;
-    const std::vector<double> xa = f.filter_density(x);
-    res.history.push_back({c.compliance, change, active_volfrac(xa), c.cg.iterations});
-    curve.push_back(c.compliance);
-    const bool stop = (o.mma && o.mma_plateau_window > 0)
-                          ? mma_objective_plateau(curve, o.mma_plateau_window, o.mma_plateau_tol,
-                                                  o.mma_plateau_min_drop,
-                                                  o.mma_plateau_flat_windows)
-                          : change < o.change_tol;
-    if (stop) { res.converged = true; break; }
+    res.converged = stage_converged;
+    if (res.non_convergent) break;
   }
   res.design = x;
   res.physical_density = f.filter_density(x);
+  if (plan.back().project) project_active(eff, res.physical_density, plan.back().beta, eta);
   const std::vector<double> unpinned = res.physical_density;
   apply_mask_pins(eff, res.physical_density);
   if (res.non_convergent) {
@@ -1749,6 +1772,68 @@ double heaviside_project_derivative(double rt, double beta, double eta) {
   const double den = std::tanh(beta * eta) + std::tanh(beta * (1.0 - eta));
   const double ch = std::cosh(beta * (rt - eta));
   return beta / (ch * ch * den);
+}
+
+void project_active(const std::vector<std::uint8_t>& eff, std::vector<double>& x, double beta,
+                    double eta) {
+  for (std::size_t e = 0; e < x.size(); ++e)
+    if (eff[e] == kActive) x[e] = heaviside_project(x[e], beta, eta);
+}
+
+std::vector<double> oc_update_masked_projected(const Grid& g, const DensityFilter& f,
+                                               const std::vector<std::uint8_t>& eff,
+                                               const std::vector<double>& x,
+                                               const std::vector<double>& xtilde,
+                                               const std::vector<double>& dcomp, double beta,
+                                               double eta, double volfrac, double move,
+                                               double density_min) {
+  const std::size_t N = g.voxels();
+  if (x.size() != N || xtilde.size() != N || dcomp.size() != N || eff.size() != N)
+    throw std::invalid_argument("oc_update_masked_projected: vector size != voxel_count");
+  // chain rule through the projection on the Active set, 0 elsewhere
+  std::vector<double> dcp(N, 0.0), dpr(N, 0.0);
+  std::size_t n_active = 0;
+  for (std::size_t e = 0; e < N; ++e) {
+    if (eff[e] != kActive) continue;
+    dpr[e] = heaviside_project_derivative(xtilde[e], beta, eta);
+    dcp[e] = dcomp[e] * dpr[e];
+    ++n_active;
+  }
+  const std::vector<double> dc = f.filter_sensitivity(dcp);
+  const std::vector<double> dv = f.filter_sensitivity(dpr);
+  const double target = volfrac * static_cast<double>(n_active);
+  auto candidate = [&](double lambda) {
+    std::vector<double> xn = x;  // FrozenSolid keeps 1, FrozenVoid 0
+    for (std::size_t e = 0; e < N; ++e) {
+      if (eff[e] != kActive) continue;
+      const double xe = x[e];
+      double be = -dc[e] / (lambda * dv[e]);  // unclamped dv
+      if (!std::isfinite(be)) be = 1.0;       // 0/0 holds the voxel
+      if (be < 0.0) be = 0.0;
+      double xt = xe * std::sqrt(be);
+      const double lo = std::max(density_min, xe - move);
+      const double hi = std::min(1.0, xe + move);
+      if (xt < lo) xt = lo;
+      if (xt > hi) xt = hi;
+      xn[e] = xt;
+    }
+    return xn;
+  };
+  // projected Active volume of a candidate (the filter is 0 off the Active set)
+  auto proj_vol = [&](const std::vector<double>& xn) {
+    const std::vector<double> xp = f.filter_density(xn);
+    double v = 0.0;
+    for (std::size_t e = 0; e < N; ++e)
+      if (eff[e] == kActive) v += heaviside_project(xp[e], beta, eta);
+    return v;
+  };
+  double l1 = 0.0, l2 = 1.0;
+  while (l2 < 1e30 && proj_vol(candidate(l2)) > target) l2 *= 2.0;
+  for (int it = 0; it < 100 && (l2 - l1) > 1e-6 * (l1 + l2); ++it) {
+    const double lmid = 0.5 * (l1 + l2);
+    if (proj_vol(candidate(lmid)) > target) l1 = lmid; else l2 = lmid;
+  }
+  return candidate(0.5 * (l1 + l2));
 }
 
 }  // namespace oracle

@@ -368,8 +368,9 @@ std::vector<double> oc_update_masked(const Grid& g, const DensityFilter& f,
                                      double move, double density_min, int* bisections = nullptr,
                                      double* lambda = nullptr);
 // simp_optimize(grid, params, bcs, loads, options, mask) (simp.cpp:2989-3689)
-// restricted like simp_optimize above: OC updater, one unprojected stage, no
-// SEMDOT / lattice coupling / adaptive move, cold solves.
+// restricted like simp_optimize above: OC (or MMA, unprojected) updater, one
+// unprojected stage or the projected stages of options.projection (OC only,
+// :3160-3440), no SEMDOT / lattice coupling / adaptive move, cold solves.
 SimpResult simp_optimize_masked(const Grid& g, const SimpParams& p,
                                 const std::vector<DirichletBC>& bcs,
                                 const std::vector<NodalLoad>& loads, const SimpOptions& o,
@@ -408,6 +409,20 @@ bool mma_objective_plateau(const std::vector<double>& c, int window, double rel_
 
 double heaviside_project(double rt, double beta, double eta);
 double heaviside_project_derivative(double rt, double beta, double eta);
+// project_active (simp.cpp:2789-2794): the projection on Active voxels only.
+void project_active(const std::vector<std::uint8_t>& eff, std::vector<double>& x, double beta,
+                    double eta);
+// oc_update_masked_projected (simp.cpp:2801-2855): oc_update_projected over the
+// Active set with the Active-only filter; frozen voxels keep their x and the
+// volume constraint is on the projected Active density (target volfrac *
+// n_active). xtilde = filter_density(x) of that filter.
+std::vector<double> oc_update_masked_projected(const Grid& g, const DensityFilter& f,
+                                               const std::vector<std::uint8_t>& eff,
+                                               const std::vector<double>& x,
+                                               const std::vector<double>& xtilde,
+                                               const std::vector<double>& dcompliance,
+                                               double beta, double eta, double volfrac,
+                                               double move, double density_min);
 // oc_update_projected (simp.cpp:1647-1716): sensitivities chained through the
 // projection derivative before the filter transpose, UNCLAMPED dv in the OC
 // ratio (a non-finite ratio holds the voxel), volume on the projected

@@ -275,8 +275,9 @@ GATE_V2_SCHEDULE = [(1.0, 0.2, 50), (2.0, 0.2, 50), (4.0, 0.2, 50), (8.0, 0.2, 5
 
 
 def optimize_projected(s, volfrac, radius, stages, eta=0.5, E0=1.0, nu=0.3, penalty=3.0,
-                       rho_min=1e-3, change_tol=0.0, cg_tol=1e-8, use_mg=True):
-    """The OC loop with projected stages [(beta, move, iterations), ...]."""
+                       rho_min=1e-3, change_tol=0.0, cg_tol=1e-8, use_mg=True, mask=None):
+    """The OC loop with projected stages [(beta, move, iterations), ...]; with a
+    design mask the masked loop's projected stages (simp.cpp:2989-3689)."""
     n_it = sum(int(st[2]) for st in stages)
     betas = np.array([st[0] for st in stages], dtype=np.float64)
     moves = np.array([st[1] for st in stages], dtype=np.float64)
@@ -287,8 +288,11 @@ def optimize_projected(s, volfrac, radius, stages, eta=0.5, E0=1.0, nu=0.3, pena
     it = C.c_int(0)
     comp = C.c_double(0)
     vf = C.c_double(0)
+    mp = None
+    if mask is not None:
+        m, mp = _mask(s, mask)
     _check(lib().oracle_optimize_projected(
-        C.byref(s.p), C.c_double(E0), C.c_double(nu), C.c_double(penalty), C.c_double(rho_min),
+        C.byref(s.p), mp, C.c_double(E0), C.c_double(nu), C.c_double(penalty), C.c_double(rho_min),
         C.c_double(volfrac), C.c_double(radius), len(stages), _ptr(betas), _ptr(moves),
         iters.ctypes.data_as(C.c_void_p), C.c_double(eta), C.c_double(change_tol),
         C.c_double(cg_tol), int(use_mg), _ptr(design), _ptr(xphys), _ptr(hist), C.byref(it),

@@ -376,7 +376,8 @@ int oracle_mma_plateau(const double* c, int n, int window, double rel_tol, doubl
 
 // Projected (Heaviside beta-continuation) OC loop: n_stages stages of
 // (beta, move, iterations), threshold eta; other arguments as oracle_optimize.
-int oracle_optimize_projected(const OracleProblem* p, double E0, double nu, double penalty,
+int oracle_optimize_projected(const OracleProblem* p, const uint8_t* mask, double E0, double nu,
+                              double penalty,
                               double rho_min, double volfrac, double radius, int n_stages,
                               const double* betas, const double* moves, const int* iters,
                               double eta, double change_tol, double cg_tol, int use_mg,
@@ -393,7 +394,10 @@ int oracle_optimize_projected(const OracleProblem* p, double E0, double nu, doub
     o.use_mg = use_mg != 0;
     o.projection_eta = eta;
     for (int s = 0; s < n_stages; ++s) o.projection.push_back({betas[s], moves[s], iters[s]});
-    const oracle::SimpResult r = oracle::simp_optimize(g, sp, bcs_of(p), loads_of(p), o);
+    const oracle::SimpResult r =
+        mask ? oracle::simp_optimize_masked(g, sp, bcs_of(p), loads_of(p), o,
+                                            std::vector<uint8_t>(mask, mask + g.voxels()))
+             : oracle::simp_optimize(g, sp, bcs_of(p), loads_of(p), o);
     if (design) std::memcpy(design, r.design.data(), r.design.size() * sizeof(double));
     if (xphys)
       std::memcpy(xphys, r.physical_density.data(), r.physical_density.size() * sizeof(double));

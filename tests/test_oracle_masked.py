@@ -161,3 +161,44 @@ def test_masked_loop_pins_and_analysis_grid():
     # ... and not the solve with the channel filled at rho_min
     c_full, _, _, _ = oracle.simp_compliance(s, xp, tol=1e-9)
     assert abs(c_full - c) > 1e-9 * c
+
+
+STAGES = [(1.0, 0.2, 3), (4.0, 0.1, 3)]
+
+
+def test_all_active_mask_is_the_plain_projected_loop():
+    """With every voxel Active the masked loop's projected stages are the
+    plain projected loop: the Active-only filter is the plain one,
+    project_active projects every voxel, oc_update_masked_projected is
+    oc_update_projected (simp.cpp:2801-2855 against :1665-1716), same sums."""
+    nx, ny, nz = 12, 4, 4
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    s = oracle.Setup(nx, ny, nz, bcs=bcs, loads=loads)
+    plain = oracle.optimize_projected(s, 0.3, 1.5, STAGES, cg_tol=1e-9)
+    masked = oracle.optimize_projected(s, 0.3, 1.5, STAGES, cg_tol=1e-9,
+                                       mask=np.zeros(nx * ny * nz, np.uint8))
+    assert plain["iterations"] == masked["iterations"] == 6
+    for k in ("history", "design", "xphys"):
+        assert np.array_equal(plain[k], masked[k])
+    assert plain["compliance"] == masked["compliance"]
+
+
+def test_masked_projected_stages_keep_frozen_and_meet_the_projected_budget():
+    """Frozen voxels keep their design value through the projected OC steps
+    (FrozenSolid 1, FrozenVoid 0) and are pinned in the physical density; the
+    projected Active volume each step reports sits at the target (the
+    bisection's bracket is 1e-6 relative in lambda)."""
+    nx, ny, nz = 16, 6, 6
+    bcs, loads, mask, _ = frozen_box_problem(nx, ny, nz)
+    s = oracle.Setup(nx, ny, nz, bcs=bcs, loads=loads)
+    out = oracle.optimize_projected(s, 0.3, 1.5, STAGES, cg_tol=1e-9, mask=mask)
+    assert out["iterations"] == 6
+    xp, d = out["xphys"], out["design"]
+    assert np.all(d[mask == FS] == 1.0) and np.all(d[mask == FV] == 0.0)
+    assert np.all(xp[mask == FS] == 1.0) and np.all(xp[mask == FV] == 0.0)
+    act = mask == A
+    assert np.all((d[act] >= 1e-3) & (d[act] <= 1.0))
+    assert np.max(np.abs(out["history"][:, 2] - 0.3)) < 1e-4
+    # the first solve already sees the projected density (H(0.3) < 0.3 at beta 1)
+    plain = oracle.optimize(s, 0.3, 1.5, 1, cg_tol=1e-9, mask=mask)
+    assert out["history"][0, 0] > plain["history"][0, 0]

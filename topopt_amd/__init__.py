@@ -445,8 +445,11 @@ class Problem:
         return out
 
     def optimize_projected(self, volfrac, rmin, stages, eta=0.5, penalty=3.0, E0=1.0,
-                           rho_min=1e-3, change_tol=0.0, cg_tol=1e-8, cg_max_iterations=0):
-        """The design loop with Heaviside projection stages [(beta, move, iterations), ...]."""
+                           rho_min=1e-3, change_tol=0.0, cg_tol=1e-8, cg_max_iterations=0,
+                           mask=None):
+        """The design loop with Heaviside projection stages [(beta, move, iterations), ...];
+        with a design mask, the masked loop's projected stages (the handle on the
+        analysis grid, as optimize(mask=...))."""
         n_it = sum(int(st[2]) for st in stages)
         o = Options(volfrac, rmin, 0.2, penalty, E0, rho_min, change_tol, cg_tol, max(n_it, 1),
                     int(cg_max_iterations), 0.0, 0.0, 0.0, 0)
@@ -457,9 +460,16 @@ class Problem:
         xphys = np.zeros(self.nvox)
         hist = np.zeros(4 * n_it)
         res = Result()
-        _check(lib().topohip_optimize_projected(self._h, C.byref(o), len(stages), _p(betas),
-                                                _p(moves), _p(iters), C.c_double(eta), _p(design),
-                                                _p(xphys), _p(hist), C.byref(res)))
+        if mask is None:
+            _check(lib().topohip_optimize_projected(self._h, C.byref(o), len(stages), _p(betas),
+                                                    _p(moves), _p(iters), C.c_double(eta),
+                                                    _p(design), _p(xphys), _p(hist),
+                                                    C.byref(res)))
+        else:
+            m = self._mask(mask)
+            _check(lib().topohip_optimize_masked_projected(
+                self._h, _p(m), C.byref(o), len(stages), _p(betas), _p(moves), _p(iters),
+                C.c_double(eta), _p(design), _p(xphys), _p(hist), C.byref(res)))
         n = res.iterations
         out = res.as_dict()
         out.update(design=design, xphys=xphys, history=hist[: 4 * n].reshape(n, 4))

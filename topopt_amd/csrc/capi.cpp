@@ -445,6 +445,21 @@ int topohip_optimize_projected(topohip_problem* p, const topohip_options* opt, i
   });
 }
 
+int topohip_optimize_masked_projected(topohip_problem* p, const uint8_t* mask,
+                                      const topohip_options* opt, int n_stages, const double* beta,
+                                      const double* move, const int* iterations, double eta,
+                                      double* design_out, double* xphys_out, double* history,
+                                      topohip_result* result) {
+  return guarded([&] {
+    need(p, "problem"); need(mask, "mask"); need(opt, "options");
+    if (n_stages < 1) throw Error(TOPOHIP_EINVAL, "simp_optimize: projection needs >= 1 stage");
+    need(beta, "beta"); need(move, "move"); need(iterations, "iterations");
+    std::vector<ProjStage> st;
+    for (int i = 0; i < n_stages; ++i) st.push_back({beta[i], move[i], iterations[i]});
+    optimize(*p->P, *opt, design_out, xphys_out, history, result, mask, &st, eta);
+  });
+}
+
 int topohip_filter_masked(topohip_problem* p, double radius, const uint8_t* mask,
                           const double* x, double* out, int adjoint) {
   return guarded([&] {

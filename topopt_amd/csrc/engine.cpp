@@ -1940,9 +1940,11 @@ void design_mask_of(const Problem& P, const uint8_t* mask, std::vector<uint8_t>&
 }
 
 // oc_update_projected (core/src/simp/simp.cpp:1647-1716) over the design set
-// dset with its filter fc; d_xt = filter_density(x).
+// dset with its filter fc; d_xt = filter_density(x). Masked (keep), it is
+// oc_update_masked_projected (:2801-2855): dset the Active voxels, fc their
+// Active-only filter, and the frozen voxels keep their x.
 void oc_update_projected_core(Problem& P, FilterCache& fc, const uint8_t* dset, double n_design,
-                              const double* d_x, const double* d_xt, const double* d_dc,
+                              bool keep, const double* d_x, const double* d_xt, const double* d_dc,
                               double beta, double eta, double volfrac, double move,
                               double rho_min, double* d_xnew, double* change) {
   const long long nv = P.g.elems();
@@ -1965,7 +1967,7 @@ void oc_update_projected_core(Problem& P, FilterCache& fc, const uint8_t* dset, 
     if (P.oc_h->phase == 2) break;
   }
   launch_oc_final_proj(d_x, P.dcf.p, P.dvf.p, dset, nv, move, rho_min, P.oc.p, d_xnew,
-                       P.partials.p, ob, P.s);
+                       P.partials.p, ob, P.s, keep);
   launch_max_partials(P.partials.p, ob, P.scal.p, P.s);
   d2h(P.scal_h, P.scal.p, 1, P.s);
   if (change) *change = P.scal_h[0];
@@ -1979,7 +1981,6 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
   if (!(o.move > 0.0)) throw Error(TOPOHIP_EINVAL, "simp_optimize: move must be > 0");
   if (o.max_iterations < 1) throw Error(TOPOHIP_EINVAL, "simp_optimize: max_iterations must be >= 1");
   if (proj && !proj->empty()) {
-    if (mask) throw Error(TOPOHIP_EINVAL, "simp_optimize: projected stages take no design mask here");
     if (!(eta > 0.0 && eta < 1.0)) throw Error(TOPOHIP_EINVAL, "heaviside_project: eta must be in (0, 1)");
     for (const ProjStage& ps : *proj)
       if (!(std::isfinite(ps.beta) && ps.beta > 0.0) || !(ps.move > 0.0) || ps.iterations < 1)
@@ -2113,8 +2114,8 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
       ta = now_ms();
       double change = 0.0;
       if (st.project)
-        oc_update_projected_core(P, fc, sol, nd, P.xd.p, P.xtil.p, P.dc.p, st.beta, eta, o.volfrac,
-                                 st.move, o.rho_min, P.xnew.p, &change);
+        oc_update_projected_core(P, fc, sol, nd, mask != nullptr, P.xd.p, P.xtil.p, P.dc.p,
+                                 st.beta, eta, o.volfrac, st.move, o.rho_min, P.xnew.p, &change);
       else if (mma)
         mma_update_core(P, fc, sol, nd, ++mma_iter, P.xd.p, P.dc.p, o.volfrac, st.move, o.rho_min,
                         P.xnew.p, nullptr, &change);

@@ -294,10 +294,11 @@ void launch_oc_cand_proj(const double* x, const double* dcf, const double* dvf,
                          const OcState* st, double* xc, hipStream_t s);
 void launch_proj_sum(const double* xtc, const uint8_t* solid, long long n, double beta, double eta,
                      const OcState* st, double* partials, int nblocks, hipStream_t s);
+// keep: voxels off the design set keep x (the masked loop), else they are 0.
 void launch_oc_final_proj(const double* x, const double* dcf, const double* dvf,
                           const uint8_t* solid, long long n, double move, double rho_min,
                           const OcState* st, double* xnew, double* partials, int nblocks,
-                          hipStream_t s);
+                          hipStream_t s, bool keep = false);
 // MMA: per-voxel state and coefficients (each n long) and the dual search.
 struct MmaArgs {
   double *low, *upp, *p0, *q0, *p1, *q1, *al, *be;

@@ -603,17 +603,20 @@ __global__ void proj_sum_k(const double* __restrict__ xtc, const uint8_t* __rest
   s = block_sum(s);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
+// (keep: voxels off the design set keep their x, as the masked loop's frozen
+// voxels do; otherwise they are 0, as off the solid set)
 __global__ void oc_final_proj_k(const double* __restrict__ x, const double* __restrict__ dcf,
                                 const double* __restrict__ dvf, const uint8_t* __restrict__ solid,
                                 long long n, double move, double rho_min, const OcState* st,
-                                double* __restrict__ xnew, double* __restrict__ partials) {
+                                double* __restrict__ xnew, double* __restrict__ partials,
+                                int keep) {
   const double lambda = 0.5 * (st->l1 + st->l2);
   double m = 0.0;
   for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
        e += static_cast<long long>(gridDim.x) * blockDim.x) {
     const double xt = (!solid || solid[e])
                           ? oc_candidate_proj(x[e], dcf[e], dvf[e], lambda, move, rho_min)
-                          : 0.0;
+                          : (keep ? x[e] : 0.0);
     xnew[e] = xt;
     m = fmax(m, fabs(xt - x[e]));
   }
@@ -650,9 +653,9 @@ void launch_proj_sum(const double* xtc, const uint8_t* solid, long long n, doubl
 void launch_oc_final_proj(const double* x, const double* dcf, const double* dvf,
                           const uint8_t* solid, long long n, double move, double rho_min,
                           const OcState* st, double* xnew, double* partials, int nblocks,
-                          hipStream_t s) {
+                          hipStream_t s, bool keep) {
   hipLaunchKernelGGL(oc_final_proj_k, dim3(nblocks), dim3(RB), 0, s, x, dcf, dvf, solid, n, move,
-                     rho_min, st, xnew, partials);
+                     rho_min, st, xnew, partials, keep ? 1 : 0);
   TOPOHIP_CHECK(hipGetLastError());
 }
 
