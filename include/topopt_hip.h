@@ -220,6 +220,27 @@ typedef struct {
 int topohip_optimize_mma(topohip_problem* p, const topohip_options* opt,
                          const topohip_mma_options* mma, const uint8_t* mask, double* design_out,
                          double* xphys_out, double* history, topohip_result* result);
+/* simp_optimize with options.updater = MMA and options.mma_projection
+ * (core/include/topopt/simp.hpp:1488-1500; one dynamic stage, build_stage_plan
+ * core/src/simp/simp.cpp:1036-1068): each iteration projects the filtered
+ * design at the current beta (project_solid, or project_active with a mask),
+ * solves, and takes mma_update_projected (:1509-1636; masked
+ * mma_update_masked_projected :2861-2990: both sensitivities chained through
+ * the projection derivative before the filter transpose, dc normalised, the
+ * constraint on the projected volume) with the move damped to
+ * move * min(1, 8 / beta) (mma_continuation_move, :251-254). A plateau of the
+ * current beta stage's compliance (mma_objective_plateau, the progress gate on
+ * the first stage only) doubles beta, capped at beta_max, and restarts the MMA
+ * asymptotes; a plateau at the cap ends the run (:2461-2491); opt->max_iterations
+ * is the safety cap. The reported per-iteration volume fraction is the filtered
+ * design's; the final physical density is projected at the last beta.
+ * TOPOHIP_EINVAL: beta0 not finite and > 0, beta_max < beta0 or not finite,
+ * eta outside (0, 1). mma, mask and outputs as topohip_optimize_mma. */
+int topohip_optimize_mma_projected(topohip_problem* p, const topohip_options* opt,
+                                   const topohip_mma_options* mma, double beta0,
+                                   double beta_max, double eta, const uint8_t* mask,
+                                   double* design_out, double* xphys_out, double* history,
+                                   topohip_result* result);
 
 /* One mma_update (core/src/simp/simp.cpp:1433-1488, core/src/simp/mma_joint.cpp:
  * 18-118) on the handle's solid voxels with the density filter of `radius`:

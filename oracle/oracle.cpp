@@ -1476,6 +1476,23 @@ std::vector<double> oc_update_projected(const Grid& g, const DensityFilter& f,
   return candidate(0.5 * (l1 + l2));
 }
 
+// validate_mma_projection_options (simp.cpp:112-135).
+void validate_mma_projection(const SimpOptions& o) {
+  if (!o.mma_projection) return;
+  if (!o.mma) throw std::invalid_argument("simp_optimize: mma_projection requires the MMA updater");
+  if (!o.projection.empty())
+    throw std::invalid_argument("simp_optimize: mma_projection and a Heaviside projection "
+                                "schedule are mutually exclusive");
+  if (!(o.projection_eta > 0.0 && o.projection_eta < 1.0))
+    throw std::invalid_argument("simp_optimize: projection_eta must be in (0, 1)");
+  if (!(o.mma_projection_beta0 > 0.0) || !std::isfinite(o.mma_projection_beta0))
+    throw std::invalid_argument("simp_optimize: mma_projection_beta0 must be finite and > 0");
+  if (!(o.mma_projection_beta_max >= o.mma_projection_beta0) ||
+      !std::isfinite(o.mma_projection_beta_max))
+    throw std::invalid_argument(
+        "simp_optimize: mma_projection_beta_max must be finite and >= mma_projection_beta0");
+}
+
 SimpResult simp_optimize(const Grid& g, const SimpParams& p,
                          const std::vector<DirichletBC>& bcs,
                          const std::vector<NodalLoad>& loads, const SimpOptions& o) {
@@ -1491,11 +1508,17 @@ SimpResult simp_optimize(const Grid& g, const SimpParams& p,
     return n_design > 0.0 ? v / n_design : 0.0;
   };
   // Stage plan (build_stage_plan, simp.cpp:1053-1068).
-  struct Stage { bool project; double beta, move; int iterations; };
+  // (cont: the single dynamic stage of MMA with projection, :1036-1068)
+  struct Stage { bool project; double beta, move; int iterations; bool cont; };
   std::vector<Stage> plan;
-  if (o.projection.empty()) plan.push_back({false, 0.0, o.move, o.max_iterations});
+  validate_mma_projection(o);
+  if (o.mma_projection) plan.push_back({false, 0.0, o.move, o.max_iterations, true});
+  else if (o.projection.empty()) plan.push_back({false, 0.0, o.move, o.max_iterations, false});
   else
-    for (const ProjStage& ps : o.projection) plan.push_back({true, ps.beta, ps.move, ps.iterations});
+    for (const ProjStage& ps : o.projection)
+      plan.push_back({true, ps.beta, ps.move, ps.iterations, false});
+  double mma_beta = o.mma_projection_beta0;
+  std::size_t stage_start = 0;  // history index where the current beta stage began
   const double eta = o.projection_eta;
   std::vector<double> x(g.voxels(), 0.0);
   for (int k = 0; k < g.nz; ++k)
@@ -1511,8 +1534,10 @@ SimpResult simp_optimize(const Grid& g, const SimpParams& p,
   for (const Stage& st : plan) {
     bool stage_converged = false;
     for (int it = 0; it < st.iterations; ++it) {
+      const double cur_beta = st.cont ? mma_beta : st.beta;
+      const double cur_move = st.cont ? mma_continuation_move(st.move, cur_beta) : st.move;
       std::vector<double> xphys = f.filter_density(x);
-      if (st.project) project_solid(g, xphys, st.beta, eta);
+      if (st.project || st.cont) project_solid(g, xphys, cur_beta, eta);
       SimpParams tp = p;
       if (!o.penalty_continuation.empty())
         tp.penalty = penalty_at_iteration(o.penalty_continuation, res.iterations + 1);
@@ -1525,7 +1550,12 @@ SimpResult simp_optimize(const Grid& g, const SimpParams& p,
         break;
       }
       std::vector<double> xn;
-      if (st.project) {
+      if (st.cont) {
+        // mma_update_projected at the current continuation beta (:2283-2293)
+        const std::vector<double> xtilde = f.filter_density(x);
+        xn = mma_update(mma_state, ++mma_iter, g, f, x, c.dc, o.volume_fraction, cur_move,
+                        p.density_min, nullptr, nullptr, &xtilde, cur_beta, eta);
+      } else if (st.project) {
         const std::vector<double> xtilde = f.filter_density(x);
         xn = oc_update_projected(g, f, x, xtilde, c.dc, st.beta, eta, o.volume_fraction, st.move,
                                  p.density_min);
@@ -1545,6 +1575,27 @@ SimpResult simp_optimize(const Grid& g, const SimpParams& p,
       res.history.push_back({c.compliance, change, volfrac(xa), c.cg.iterations});
       curve.push_back(c.compliance);
       // stage_should_stop, core/src/simp/simp.cpp:1219-1232
+      if (st.cont) {
+        // beta continuation (:2461-2491): a plateau of the current beta stage
+        // doubles beta (capped) and restarts the MMA state; a plateau at the
+        // cap ends the run. The progress gate guards the first stage only.
+        const std::vector<double> sc(curve.begin() + static_cast<std::ptrdiff_t>(stage_start),
+                                     curve.end());
+        const double min_drop = stage_start == 0 ? o.mma_plateau_min_drop : 0.0;
+        if (mma_objective_plateau(sc, o.mma_plateau_window, o.mma_plateau_tol, min_drop,
+                                  o.mma_plateau_flat_windows)) {
+          if (mma_beta < o.mma_projection_beta_max) {
+            mma_beta = std::min(mma_beta * 2.0, o.mma_projection_beta_max);
+            stage_start = curve.size();
+            mma_state = MmaState{};
+            mma_iter = 0;
+          } else {
+            stage_converged = true;
+            break;
+          }
+        }
+        continue;
+      }
       const bool stop = (o.mma && o.mma_plateau_window > 0)
                             ? mma_objective_plateau(curve, o.mma_plateau_window, o.mma_plateau_tol,
                                                     o.mma_plateau_min_drop,
@@ -1558,6 +1609,7 @@ SimpResult simp_optimize(const Grid& g, const SimpParams& p,
   res.design = x;
   res.physical_density = f.filter_density(x);
   if (plan.back().project) project_solid(g, res.physical_density, plan.back().beta, eta);
+  else if (plan.back().cont) project_solid(g, res.physical_density, mma_beta, eta);
   if (res.non_convergent) {
     res.converged = false;
     res.compliance = res.history.empty() ? 0.0 : res.history.back().compliance;
@@ -1684,11 +1736,17 @@ SimpResult simp_optimize_masked(const Grid& g, const SimpParams& p,
   std::vector<std::uint8_t> active(g.voxels(), 0);
   for (std::size_t e = 0; e < eff.size(); ++e) active[e] = eff[e] == kActive;
   // Stage plan (build_stage_plan, simp.cpp:1053-1068), as in simp_optimize.
-  struct Stage { bool project; double beta, move; int iterations; };
+  // (cont: the single dynamic stage of MMA with projection, :1036-1068)
+  struct Stage { bool project; double beta, move; int iterations; bool cont; };
   std::vector<Stage> plan;
-  if (o.projection.empty()) plan.push_back({false, 0.0, o.move, o.max_iterations});
+  validate_mma_projection(o);
+  if (o.mma_projection) plan.push_back({false, 0.0, o.move, o.max_iterations, true});
+  else if (o.projection.empty()) plan.push_back({false, 0.0, o.move, o.max_iterations, false});
   else
-    for (const ProjStage& ps : o.projection) plan.push_back({true, ps.beta, ps.move, ps.iterations});
+    for (const ProjStage& ps : o.projection)
+      plan.push_back({true, ps.beta, ps.move, ps.iterations, false});
+  double mma_beta = o.mma_projection_beta0;
+  std::size_t stage_start = 0;  // history index where the current beta stage began
   if (o.mma && !o.projection.empty())
     throw std::invalid_argument("simp_optimize: projection requires the OC updater");
   const double eta = o.projection_eta;
@@ -1699,8 +1757,10 @@ SimpResult simp_optimize_masked(const Grid& g, const SimpParams& p,
     bool stage_converged = false;
     for (int it = 0; it < st.iterations; ++it) {
       // physical density: filter, project the Active voxels, pin (:3196-3204)
+      const double cur_beta = st.cont ? mma_beta : st.beta;
+      const double cur_move = st.cont ? mma_continuation_move(st.move, cur_beta) : st.move;
       std::vector<double> xphys = f.filter_density(x);
-      if (st.project) project_active(eff, xphys, st.beta, eta);
+      if (st.project || st.cont) project_active(eff, xphys, cur_beta, eta);
       apply_mask_pins(eff, xphys);
       SimpParams tp = p;
       if (!o.penalty_continuation.empty())
@@ -1715,7 +1775,12 @@ SimpResult simp_optimize_masked(const Grid& g, const SimpParams& p,
       }
       // core/src/simp/simp.cpp:3306-3313 (projected OC), 3351-3357 (MMA) or the OC update
       std::vector<double> xn;
-      if (st.project)
+      if (st.cont) {
+        // mma_update_masked_projected at the current continuation beta (:3296-3307)
+        const std::vector<double> xtilde = f.filter_density(x);
+        xn = mma_update(mma_state, ++mma_iter, g, f, x, c.dc, vf_target, cur_move,
+                        p.density_min, nullptr, &active, &xtilde, cur_beta, eta);
+      } else if (st.project)
         xn = oc_update_masked_projected(g, f, eff, x, f.filter_density(x), c.dc, st.beta, eta,
                                         vf_target, st.move, p.density_min);
       else if (o.mma)
@@ -1732,6 +1797,27 @@ SimpResult simp_optimize_masked(const Grid& g, const SimpParams& p,
       if (st.project) project_active(eff, xa, st.beta, eta);
       res.history.push_back({c.compliance, change, active_volfrac(xa), c.cg.iterations});
       curve.push_back(c.compliance);
+      if (st.cont) {
+        // beta continuation (:2461-2491): a plateau of the current beta stage
+        // doubles beta (capped) and restarts the MMA state; a plateau at the
+        // cap ends the run. The progress gate guards the first stage only.
+        const std::vector<double> sc(curve.begin() + static_cast<std::ptrdiff_t>(stage_start),
+                                     curve.end());
+        const double min_drop = stage_start == 0 ? o.mma_plateau_min_drop : 0.0;
+        if (mma_objective_plateau(sc, o.mma_plateau_window, o.mma_plateau_tol, min_drop,
+                                  o.mma_plateau_flat_windows)) {
+          if (mma_beta < o.mma_projection_beta_max) {
+            mma_beta = std::min(mma_beta * 2.0, o.mma_projection_beta_max);
+            stage_start = curve.size();
+            mma_state = MmaState{};
+            mma_iter = 0;
+          } else {
+            stage_converged = true;
+            break;
+          }
+        }
+        continue;
+      }
       const bool stop = (o.mma && o.mma_plateau_window > 0)
                             ? mma_objective_plateau(curve, o.mma_plateau_window, o.mma_plateau_tol,
                                                     o.mma_plateau_min_drop,
@@ -1745,6 +1831,7 @@ SimpResult simp_optimize_masked(const Grid& g, const SimpParams& p,
   res.design = x;
   res.physical_density = f.filter_density(x);
   if (plan.back().project) project_active(eff, res.physical_density, plan.back().beta, eta);
+  else if (plan.back().cont) project_active(eff, res.physical_density, mma_beta, eta);
   const std::vector<double> unpinned = res.physical_density;
   apply_mask_pins(eff, res.physical_density);
   if (res.non_convergent) {
@@ -1907,14 +1994,18 @@ namespace oracle {
 // core/src/simp/simp.cpp:1433-1488 and core/src/simp/mma_joint.cpp:18-118
 // (asyinit 0.5, asyincr 1.2, asydecr 0.7, albefa 0.1, raa0 1e-5, move as a
 // fraction of xrange, the 1-D dual bisection).
+double mma_continuation_move(double move, double beta) {
+  return beta > 8.0 ? move * (8.0 / beta) : move;
+}
+
 std::vector<double> mma_update(MmaState& st, int mma_iter, const Grid& g, const DensityFilter& f,
                                const std::vector<double>& x, const std::vector<double>& dcomp,
                                double volfrac, double move, double density_min,
-                               double* lambda_out, const std::vector<std::uint8_t>* dset) {
+                               double* lambda_out, const std::vector<std::uint8_t>* dset,
+                               const std::vector<double>* xtilde, double beta, double eta) {
   const std::size_t N = g.voxels();
-  if (x.size() != N || dcomp.size() != N)
+  if (x.size() != N || dcomp.size() != N || (xtilde && xtilde->size() != N))
     throw std::invalid_argument("mma_update: vector size != voxel_count");
-  std::vector<double> dc = f.filter_sensitivity(dcomp);
   std::vector<double> ones(N, 0.0);
   std::vector<std::size_t> dof;
   for (int k = 0; k < g.nz; ++k)
@@ -1926,15 +2017,30 @@ std::vector<double> mma_update(MmaState& st, int mma_iter, const Grid& g, const 
           dof.push_back(e);
         }
       }
-  const std::vector<double> dv = f.filter_sensitivity(ones);
+  std::vector<double> dc, dv;
+  double vol = 0.0;
+  if (xtilde) {
+    // projected: dproj on the design set, dc * dproj and dproj through the
+    // filter transpose, volume = sum of the projected filtered design
+    std::vector<double> dcp(N, 0.0), dpr(N, 0.0);
+    for (std::size_t e : dof) {
+      dpr[e] = heaviside_project_derivative((*xtilde)[e], beta, eta);
+      dcp[e] = dcomp[e] * dpr[e];
+      vol += heaviside_project((*xtilde)[e], beta, eta);
+    }
+    dc = f.filter_sensitivity(dcp);
+    dv = f.filter_sensitivity(dpr);
+  } else {
+    dc = f.filter_sensitivity(dcomp);
+    dv = f.filter_sensitivity(ones);
+    const std::vector<double> xphys = f.filter_density(x);
+    for (double v : xphys) vol += v;
+  }
   const double target = volfrac * static_cast<double>(dof.size());
   double scale = 0.0;
   for (std::size_t e : dof) scale = std::max(scale, std::fabs(dc[e]));
   if (scale > 0.0)
     for (std::size_t e : dof) dc[e] /= scale;
-  const std::vector<double> xphys = f.filter_density(x);
-  double vol = 0.0;
-  for (double v : xphys) vol += v;
   const double g0 = vol - target;
 
   const double asyinit = 0.5, asyincr = 1.2, asydecr = 0.7, albefa = 0.1, raa0 = 1e-5;

@@ -329,6 +329,13 @@ struct SimpOptions {
   double mma_plateau_tol = 1e-3;
   double mma_plateau_min_drop = 0.05;
   int mma_plateau_flat_windows = 3;
+  // SimpOptions::mma_projection (core/include/topopt/simp.hpp:1488-1500): MMA
+  // with Heaviside projection and beta continuation, one dynamic stage
+  // (build_stage_plan, simp.cpp:1036-1068): beta starts at beta0 and doubles
+  // (capped at beta_max) on each plateau of the current beta stage.
+  bool mma_projection = false;
+  double mma_projection_beta0 = 1.0;
+  double mma_projection_beta_max = 32.0;
 };
 struct SimpIteration { double compliance, change, volume_fraction; int cg_iterations; };
 struct SimpResult {
@@ -398,11 +405,19 @@ std::uint64_t fnv1a64(const void* data, std::size_t n,
 struct MmaState { std::vector<double> low, upp, xold1, xold2; };
 // dset (nullable): the design set of mma_update_masked (:2736-2783, the Active
 // voxels; f then the Active-only filter); null = the solid voxels.
+// xtilde (nullable) = filter_density(x) makes it mma_update_projected
+// (simp.cpp:1509-1636; masked :2861-2990): both sensitivities chained through
+// the projection derivative at (beta, eta) before the filter transpose, and
+// the constraint on the projected volume sum H(xtilde) over the design set.
 std::vector<double> mma_update(MmaState& st, int mma_iter, const Grid& g, const DensityFilter& f,
                                const std::vector<double>& x, const std::vector<double>& dcomp,
                                double volfrac, double move, double density_min,
                                double* lambda_out = nullptr,
-                               const std::vector<std::uint8_t>* dset = nullptr);
+                               const std::vector<std::uint8_t>* dset = nullptr,
+                               const std::vector<double>* xtilde = nullptr, double beta = 1.0,
+                               double eta = 0.5);
+// mma_continuation_move (simp.cpp:251-254): move * min(1, 8 / beta).
+double mma_continuation_move(double move, double beta);
 // core/src/simp/simp.cpp:1081-1134.
 bool mma_objective_plateau(const std::vector<double>& c, int window, double rel_tol,
                            double min_drop, int min_flat_windows);

@@ -365,9 +365,12 @@ def fnv1a64(data, basis=None):
 
 def optimize_mma(s, volfrac, radius, iterations, E0=1.0, nu=0.3, penalty=3.0, rho_min=1e-3,
                  move=0.2, change_tol=0.0, cg_tol=1e-8, plateau_window=10, plateau_tol=1e-3,
-                 plateau_min_drop=0.05, plateau_flat_windows=3, mask=None):
+                 plateau_min_drop=0.05, plateau_flat_windows=3, mask=None, projection=None,
+                 eta=0.5):
     """simp_optimize with SimpUpdater::MMA (plateau stop); with a design mask
-    the masked loop with mma_update_masked."""
+    the masked loop with mma_update_masked. projection = (beta0, beta_max)
+    turns on SimpOptions::mma_projection (beta continuation)."""
+    b0, bmax = projection if projection else (1.0, 32.0)
     design = np.zeros(s.nvox)
     xphys = np.zeros(s.nvox)
     hist = np.zeros(4 * iterations)
@@ -380,11 +383,19 @@ def optimize_mma(s, volfrac, radius, iterations, E0=1.0, nu=0.3, penalty=3.0, rh
         C.c_double(volfrac), C.c_double(radius), C.c_double(move), int(iterations),
         C.c_double(change_tol), C.c_double(cg_tol), int(plateau_window), C.c_double(plateau_tol),
         C.c_double(plateau_min_drop), int(plateau_flat_windows),
+        int(projection is not None), C.c_double(b0), C.c_double(bmax), C.c_double(eta),
         mp, _ptr(design), _ptr(xphys),
         _ptr(hist), C.byref(it), C.byref(comp), C.byref(vf)))
     n = it.value
     return {"design": design, "xphys": xphys, "history": hist[: 4 * n].reshape(n, 4),
             "iterations": n, "compliance": comp.value, "volume_fraction": vf.value}
+
+
+def mma_continuation_move(move, beta):
+    """mma_continuation_move (core/src/simp/simp.cpp:251-254)."""
+    f = lib().oracle_mma_continuation_move
+    f.restype = C.c_double
+    return f(C.c_double(move), C.c_double(beta))
 
 
 def mma_steps(s, radius, xs, dcs, volfrac, move=0.2, rho_min=1e-3):

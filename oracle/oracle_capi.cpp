@@ -303,10 +303,15 @@ int oracle_optimize(const OracleProblem* p, double E0, double nu, double penalty
 
 // The plain loop with the MMA updater and its plateau stop (window <= 0: the
 // change_tol test); other arguments as oracle_optimize.
+double oracle_mma_continuation_move(double move, double beta) {
+  return oracle::mma_continuation_move(move, beta);
+}
+
 int oracle_optimize_mma(const OracleProblem* p, double E0, double nu, double penalty,
                         double rho_min, double volfrac, double radius, double move,
                         int max_iterations, double change_tol, double cg_tol, int window,
                         double plateau_tol, double min_drop, int flat_windows,
+                        int mma_projection, double beta0, double beta_max, double eta,
                         const std::uint8_t* mask, double* design,
                         double* xphys, double* hist, int* iterations, double* compliance,
                         double* final_volfrac) {
@@ -325,6 +330,10 @@ int oracle_optimize_mma(const OracleProblem* p, double E0, double nu, double pen
     o.mma_plateau_tol = plateau_tol;
     o.mma_plateau_min_drop = min_drop;
     o.mma_plateau_flat_windows = flat_windows;
+    o.mma_projection = mma_projection != 0;
+    o.mma_projection_beta0 = beta0;
+    o.mma_projection_beta_max = beta_max;
+    o.projection_eta = eta;
     const oracle::SimpResult r =
         mask ? oracle::simp_optimize_masked(g, sp, bcs_of(p), loads_of(p), o,
                                             std::vector<std::uint8_t>(mask, mask + g.voxels()))

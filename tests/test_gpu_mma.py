@@ -127,3 +127,54 @@ def test_masked_mma_all_active_equals_plain_on_device():
                            mask=np.zeros(nx * ny * nz, np.uint8))
     assert np.array_equal(a["history"], b["history"])
     assert np.array_equal(a["design"], b["design"])
+
+
+def test_mma_projection_continuation_matches_oracle():
+    """SimpOptions::mma_projection on the device against the oracle: the
+    projected MMA step at each beta, the beta advances on plateaus and the
+    stop at the capped beta (core/src/simp/simp.cpp:1509-1636, 2461-2491)."""
+    nx, ny, nz = 16, 6, 6
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    s = oracle.Setup(nx, ny, nz, bcs=bcs, loads=loads)
+    ref = oracle.optimize_mma(s, 0.3, 1.5, 200, cg_tol=1e-10, plateau_window=5,
+                              projection=(1.0, 8.0))
+    betas = []
+    with ta.Problem(nx, ny, nz, bcs, loads, precision=ta.PREC_F64) as p:
+        p.set_observer(lambda o: betas.append(o.beta))
+        out = p.optimize_mma(0.3, 1.5, 200, cg_tol=1e-10, plateau_window=5,
+                             projection=(1.0, 8.0))
+        with pytest.raises(ValueError):
+            p.optimize_mma(0.3, 1.5, 5, projection=(4.0, 2.0))
+    hc, hr = out["history"], ref["history"]
+    print("iterations", out["iterations"], ref["iterations"], "betas",
+          sorted(set(betas)), flush=True)
+    assert out["iterations"] == ref["iterations"] < 200
+    assert sorted(set(betas)) == [1.0, 2.0, 4.0, 8.0] and betas == sorted(betas)
+    assert out["converged"] == 1
+    assert np.max(np.abs(hc[:, 0] - hr[:, 0]) / hr[:, 0]) < 1e-5
+    assert abs(out["compliance"] - ref["compliance"]) / ref["compliance"] < 1e-5
+    assert abs(out["volume_fraction"] - ref["volume_fraction"]) < 1e-6
+    assert np.abs(out["xphys"] - ref["xphys"]).max() < 1e-4
+
+
+def test_masked_mma_projection_matches_oracle():
+    nx, ny, nz = 16, 6, 6
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    mask = np.zeros((nz, ny, nx), np.uint8)
+    mask[:, :, -2:] = oracle.FROZEN_SOLID
+    mask[: nz // 2, : ny // 2, nx // 2] = oracle.FROZEN_VOID
+    m = mask.ravel()
+    solid = (m != oracle.FROZEN_VOID).astype(np.uint8)
+    s = oracle.Setup(nx, ny, nz, bcs=bcs, loads=loads)
+    ref = oracle.optimize_mma(s, 0.3, 1.5, 12, cg_tol=1e-10, plateau_window=0, mask=m,
+                              projection=(2.0, 8.0))
+    with ta.Problem(nx, ny, nz, bcs, loads, solid=solid, precision=ta.PREC_F64) as p:
+        out = p.optimize_mma(0.3, 1.5, 12, cg_tol=1e-10, plateau_window=0, mask=m,
+                             projection=(2.0, 8.0))
+    hc, hr = out["history"], ref["history"]
+    assert out["iterations"] == ref["iterations"] == 12
+    assert np.max(np.abs(hc[:, 0] - hr[:, 0]) / hr[:, 0]) < 1e-6
+    assert np.max(np.abs(hc[:, 1:3] - hr[:, 1:3])) < 1e-6
+    assert np.abs(out["design"] - ref["design"]).max() < 1e-5
+    assert np.all(out["xphys"][m == oracle.FROZEN_SOLID] == 1.0)
+    assert np.all(out["xphys"][m == oracle.FROZEN_VOID] == 0.0)

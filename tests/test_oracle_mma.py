@@ -119,3 +119,66 @@ def test_masked_mma_moves_only_active_voxels():
     assert abs(out["volume_fraction"] - 0.3) < 5e-3
     h = out["history"]
     assert h[0, 1] == 1.0 and np.all(h[1:, 1] < 1.0)   # the first step zeroes the frozen x
+
+
+# ---- MMA with Heaviside projection and beta continuation -------------------
+# SimpOptions::mma_projection (simp.cpp:1036-1068 one dynamic stage, :2461-2491
+# continuation, mma_update_projected :1509-1636). The reference's validation
+# test (core/tests/validation/test_mma_projection.cpp) pins behaviour, not
+# vectors: the projected run is near 0/1 where plain MMA leaves a gray field,
+# meets the volume on the projected density, advances beta through several
+# stages and stops on a plateau at the cap before the safety cap, and repeats
+# bit for bit.
+
+def _mnd(x):
+    return float(np.mean(4.0 * x * (1.0 - x)))
+
+
+@pytest.fixture(scope="module")
+def mma_projection_runs():
+    nx, ny, nz = 16, 6, 6
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    s = oracle.Setup(nx, ny, nz, bcs=bcs, loads=loads)
+    gray = oracle.optimize_mma(s, 0.3, 1.5, 80, cg_tol=1e-8, plateau_window=5)
+    proj = oracle.optimize_mma(s, 0.3, 1.5, 200, cg_tol=1e-8, plateau_window=5,
+                               projection=(1.0, 8.0))
+    return s, gray, proj
+
+
+def test_mma_projection_is_crisp_and_meets_the_projected_volume(mma_projection_runs):
+    _, gray, proj = mma_projection_runs
+    print("Mnd gray", _mnd(gray["xphys"]), "projected", _mnd(proj["xphys"]),
+          "its", gray["iterations"], proj["iterations"], flush=True)
+    assert _mnd(gray["xphys"]) > 0.3
+    assert _mnd(proj["xphys"]) < 0.35 * _mnd(gray["xphys"])
+    assert abs(proj["volume_fraction"] - 0.3) < 3e-2
+    # the printed fraction meets the continuous one once the field is crisp
+    printed = float(np.mean(proj["xphys"] > 0.5))
+    assert abs(printed - proj["volume_fraction"]) < abs(
+        float(np.mean(gray["xphys"] > 0.5)) - gray["volume_fraction"]) + 3e-2
+
+
+def test_mma_projection_continues_and_stops_on_a_plateau(mma_projection_runs):
+    s, _, proj = mma_projection_runs
+    n = proj["iterations"]
+    # at least one stage per beta (1, 2, 4, 8), each longer than the window,
+    # and the run ends on the capped beta's plateau, not the safety cap
+    assert 4 * 6 <= n < 200
+    # the unprojected step sizes of the first stage vs the damped last ones
+    # (move * 8 / beta only above beta 8, so here the move stays 0.2)
+    assert np.max(proj["history"][:, 1]) <= 0.2 + 1e-12
+
+
+def test_mma_projection_is_deterministic_and_validated(mma_projection_runs):
+    s, _, proj = mma_projection_runs
+    again = oracle.optimize_mma(s, 0.3, 1.5, 200, cg_tol=1e-8, plateau_window=5,
+                                projection=(1.0, 8.0))
+    assert np.array_equal(again["history"], proj["history"])
+    assert np.array_equal(again["xphys"], proj["xphys"])
+    with pytest.raises(Exception, match="beta_max"):
+        oracle.optimize_mma(s, 0.3, 1.5, 5, projection=(4.0, 2.0))
+    with pytest.raises(Exception, match="beta0"):
+        oracle.optimize_mma(s, 0.3, 1.5, 5, projection=(0.0, 2.0))
+    assert oracle.mma_continuation_move(0.2, 4.0) == 0.2
+    assert oracle.mma_continuation_move(0.2, 16.0) == 0.1
+    assert oracle.mma_continuation_move(0.2, 32.0) == 0.05

@@ -400,9 +400,11 @@ class Problem:
     def optimize_mma(self, volfrac, rmin, iterations, penalty=3.0, move=0.2, E0=1.0, rho_min=1e-3,
                      change_tol=0.0, cg_tol=1e-8, cg_max_iterations=0, continuation=None,
                      plateau_window=10, plateau_tol=1e-3, plateau_min_drop=0.05,
-                     plateau_flat_windows=3, mask=None):
+                     plateau_flat_windows=3, mask=None, projection=None, eta=0.5):
         """simp_optimize with SimpUpdater::MMA (plateau stop); with a design
-        mask the masked loop (mma_update_masked)."""
+        mask the masked loop (mma_update_masked). projection = (beta0,
+        beta_max) turns on SimpOptions::mma_projection: Heaviside projection
+        with beta continuation, iterations the safety cap."""
         o = Options(volfrac, rmin, move, penalty, E0, rho_min, change_tol, cg_tol, int(iterations),
                     int(cg_max_iterations), 0.0, 0.0, 0.0, 0)
         if continuation:
@@ -413,8 +415,14 @@ class Problem:
         hist = np.zeros(4 * iterations)
         res = Result()
         mk = None if mask is None else self._mask(mask)
-        _check(lib().topohip_optimize_mma(self._h, C.byref(o), C.byref(m), _p(mk), _p(design),
-                                          _p(xphys), _p(hist), C.byref(res)))
+        if projection is None:
+            _check(lib().topohip_optimize_mma(self._h, C.byref(o), C.byref(m), _p(mk),
+                                              _p(design), _p(xphys), _p(hist), C.byref(res)))
+        else:
+            b0, bmax = projection
+            _check(lib().topohip_optimize_mma_projected(
+                self._h, C.byref(o), C.byref(m), C.c_double(b0), C.c_double(bmax),
+                C.c_double(eta), _p(mk), _p(design), _p(xphys), _p(hist), C.byref(res)))
         n = res.iterations
         out = res.as_dict()
         out.update(design=design, xphys=xphys, history=hist[: 4 * n].reshape(n, 4))

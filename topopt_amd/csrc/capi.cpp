@@ -422,6 +422,27 @@ int topohip_mma_plateau(const double* c, int n, int window, double rel_tol, doub
   });
 }
 
+int topohip_optimize_mma_projected(topohip_problem* p, const topohip_options* opt,
+                                   const topohip_mma_options* mma, double beta0,
+                                   double beta_max, double eta, const uint8_t* mask,
+                                   double* design_out, double* xphys_out, double* history,
+                                   topohip_result* result) {
+  return guarded([&] {
+    need(p, "problem"); need(opt, "options");
+    MmaOpts m;
+    if (mma) {
+      m.plateau_window = mma->plateau_window;
+      m.plateau_tol = mma->plateau_tol;
+      m.plateau_min_drop = mma->plateau_min_drop;
+      m.plateau_flat_windows = mma->plateau_flat_windows;
+    }
+    m.projection = true;
+    m.beta0 = beta0;
+    m.beta_max = beta_max;
+    optimize(*p->P, *opt, design_out, xphys_out, history, result, mask, nullptr, eta, &m);
+  });
+}
+
 int topohip_optimize_masked(topohip_problem* p, const uint8_t* mask, const topohip_options* opt,
                             double* design_out, double* xphys_out, double* history,
                             topohip_result* result) {

@@ -1847,7 +1847,7 @@ void oc_update_noalloc(Problem& P, double radius, const double* d_x, const doubl
 void mma_update_core(Problem& P, FilterCache& fc, const uint8_t* dset, double n_design,
                      int mma_iter, const double* d_x, const double* d_dc, double volfrac,
                      double move, double rho_min, double* d_xnew, double* lambda,
-                     double* change) {
+                     double* change, const double* d_xt, double beta, double eta) {
   if (!(volfrac > 0.0 && volfrac <= 1.0))
     throw Error(TOPOHIP_EINVAL, "mma_update: volume_fraction must be in (0, 1]");
   if (!(move > 0.0)) throw Error(TOPOHIP_EINVAL, "mma_update: move must be > 0");
@@ -1863,13 +1863,25 @@ void mma_update_core(Problem& P, FilterCache& fc, const uint8_t* dset, double n_
   MmaArgs a{P.mma_low.p, P.mma_upp.p, P.mma_c.p, P.mma_c.p + n, P.mma_c.p + 2 * n,
             P.mma_c.p + 3 * n, P.mma_c.p + 4 * n, P.mma_c.p + 5 * n};
   const int ob = oc_blocks(nv), vb = dot_blocks(nv);
-  launch_filter(P.g, fc.st, dset, fc.hs.p, d_dc, P.dcf.p, 1, P.s);
+  const double* dv = fc.dv.p;
+  if (d_xt) {
+    // projected: dproj and dc * dproj on the design set through the filter
+    // transpose; the volume is the projected filtered design's (0 elsewhere)
+    for (auto* b : {&P.dpr, &P.dcp, &P.dvf}) b->ensure(n);
+    launch_proj_chain(d_xt, d_dc, dset, nv, beta, eta, P.dcp.p, P.dpr.p, P.s);
+    launch_filter(P.g, fc.st, dset, fc.hs.p, P.dcp.p, P.dcf.p, 1, P.s);
+    launch_filter(P.g, fc.st, dset, fc.hs.p, P.dpr.p, P.dvf.p, 1, P.s);
+    launch_project(d_xt, P.xcand.p, dset, nv, beta, eta, P.s);
+    dv = P.dvf.p;
+  } else {
+    launch_filter(P.g, fc.st, dset, fc.hs.p, d_dc, P.dcf.p, 1, P.s);
+    launch_filter(P.g, fc.st, dset, fc.hs.p, d_x, P.xcand.p, 0, P.s);
+  }
   launch_mma_scale(P.dcf.p, dset, nv, P.partials.p, ob, P.s);
   launch_max_partials(P.partials.p, ob, P.scal.p, P.s);
-  launch_filter(P.g, fc.st, dset, fc.hs.p, d_x, P.xcand.p, 0, P.s);
   launch_sum_vec<double>(P.xcand.p, nv, P.partials.p, vb, P.s);
   launch_sum_partials(P.partials.p, vb, P.scal.p + 1, P.s);
-  launch_mma_coef(a, d_x, P.mma_x1.p, P.mma_x2.p, P.dcf.p, fc.dv.p, dset, nv, mma_iter, move,
+  launch_mma_coef(a, d_x, P.mma_x1.p, P.mma_x2.p, P.dcf.p, dv, dset, nv, mma_iter, move,
                   rho_min, P.scal.p, P.partials2.p, ob, P.s);
   launch_mma_init(P.mma_st.p, P.partials2.p, ob, P.scal.p + 1, volfrac * n_design, P.s);
   P.lam_partials.ensure(static_cast<size_t>(kLamPass) * ob);
@@ -1993,6 +2005,18 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
   // (mma_update_masked, core/src/simp/simp.cpp:2736-2783).
   if (mma && proj && !proj->empty())
     throw Error(TOPOHIP_EINVAL, "simp_optimize: projection stages need the OC updater");
+  // SimpOptions::mma_projection (validate_mma_projection_options,
+  // core/src/simp/simp.cpp:112-135)
+  const bool mma_cont = mma && mma->projection;
+  if (mma_cont) {
+    if (!(eta > 0.0 && eta < 1.0))
+      throw Error(TOPOHIP_EINVAL, "simp_optimize: projection_eta must be in (0, 1)");
+    if (!(mma->beta0 > 0.0) || !std::isfinite(mma->beta0))
+      throw Error(TOPOHIP_EINVAL, "simp_optimize: mma_projection_beta0 must be finite and > 0");
+    if (!(mma->beta_max >= mma->beta0) || !std::isfinite(mma->beta_max))
+      throw Error(TOPOHIP_EINVAL, "simp_optimize: mma_projection_beta_max must be finite and "
+                                  ">= mma_projection_beta0");
+  }
   int mma_iter = 0;
   P.oc_gap_min = 1e300;
   std::vector<double> curve;  // compliance history for the MMA plateau stop
@@ -2055,13 +2079,21 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
   const int vb = dot_blocks(nv);
   // Stage plan (build_stage_plan, core/src/simp/simp.cpp:1053-1068): one
   // unprojected stage, or the projected stages in order.
+  // (cont: the single dynamic stage of MMA with projection, whose beta starts
+  // at beta0 and doubles on plateaus, :1036-1068, :2461-2491)
   struct Stage {
     bool project;
     double beta, move;
     int iterations;
+    bool cont;
   };
   std::vector<Stage> plan;
-  if (proj && !proj->empty()) {
+  double mma_beta = mma_cont ? mma->beta0 : 0.0;
+  size_t stage_start = 0;  // curve index where the current beta stage began
+  if (mma_cont) {
+    plan.push_back({false, 0.0, o.move, o.max_iterations, true});
+    P.xtil.ensure(static_cast<size_t>(nv));
+  } else if (proj && !proj->empty()) {
     for (const ProjStage& ps : *proj) plan.push_back({true, ps.beta, ps.move, ps.iterations});
     P.xtil.ensure(static_cast<size_t>(nv));
   } else {
@@ -2069,10 +2101,10 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
   }
   // filtered (and, in a projected stage, projected) density of the design
   // into P.xphys; the filtered one stays in P.xtil for the OC chain rule
-  auto physical = [&](const Stage& st) {
-    if (st.project) {
+  auto physical = [&](bool project, double beta) {
+    if (project) {
       launch_filter(P.g, fc.st, sol, fc.hs.p, P.xd.p, P.xtil.p, 0, P.s);
-      launch_project(P.xtil.p, P.xphys.p, sol, nv, st.beta, eta, P.s);
+      launch_project(P.xtil.p, P.xphys.p, sol, nv, beta, eta, P.s);
     } else {
       launch_filter(P.g, fc.st, sol, fc.hs.p, P.xd.p, P.xphys.p, 0, P.s);
     }
@@ -2090,7 +2122,10 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
       }
       const double t_top = now_ms();
       double ta = t_top;
-      physical(st);
+      const double cur_beta = st.cont ? mma_beta : st.beta;
+      // mma_continuation_move (:251-254): move * min(1, 8 / beta)
+      const double cur_move = st.cont && cur_beta > 8.0 ? st.move * (8.0 / cur_beta) : st.move;
+      physical(st.project || st.cont, cur_beta);
       if (mask) launch_mask_pins(P.xphys.p, d_eff.p, nv, P.s);
       sync(P);
       const double ms_filter = now_ms() - ta;
@@ -2113,7 +2148,10 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
       t_build += ci.t_mg_build_ms;
       ta = now_ms();
       double change = 0.0;
-      if (st.project)
+      if (st.cont)
+        mma_update_core(P, fc, sol, nd, ++mma_iter, P.xd.p, P.dc.p, o.volfrac, cur_move,
+                        o.rho_min, P.xnew.p, nullptr, &change, P.xtil.p, cur_beta, eta);
+      else if (st.project)
         oc_update_projected_core(P, fc, sol, nd, mask != nullptr, P.xd.p, P.xtil.p, P.dc.p,
                                  st.beta, eta, o.volfrac, st.move, o.rho_min, P.xnew.p, &change);
       else if (mma)
@@ -2129,7 +2167,7 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
       ++r.iterations;
       r.cg_iterations_total += ci.iterations;
       ta = now_ms();
-      physical(st);
+      physical(st.project, st.beta);  // (the continuation stage reports the filtered volume)
       launch_sum_vec<double>(P.xphys.p, nv, P.partials.p, vb, P.s);
       launch_sum_partials(P.partials.p, vb, P.scal.p, P.s);
       d2h(P.scal_h, P.scal.p, 1, P.s);
@@ -2146,11 +2184,32 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
       // detector on the compliance history, else the change test
       curve.push_back(c);
       const bool use_plateau = mma && mma->plateau_window > 0;
-      const bool plateau =
-          use_plateau && mma_objective_plateau(curve.data(), static_cast<int>(curve.size()),
-                                               mma->plateau_window, mma->plateau_tol,
-                                               mma->plateau_min_drop, mma->plateau_flat_windows);
-      const bool stop = use_plateau ? plateau : change < o.change_tol;
+      bool plateau = false, stop = false;
+      if (st.cont) {
+        // beta continuation: a plateau of the current beta stage (the progress
+        // gate on the first stage only) doubles beta, capped, and restarts the
+        // MMA asymptotes; a plateau at the cap ends the run
+        const int n0 = static_cast<int>(stage_start);
+        plateau = mma_objective_plateau(curve.data() + n0, static_cast<int>(curve.size()) - n0,
+                                        mma->plateau_window, mma->plateau_tol,
+                                        n0 == 0 ? mma->plateau_min_drop : 0.0,
+                                        mma->plateau_flat_windows);
+        if (plateau) {
+          if (mma_beta < mma->beta_max) {
+            mma_beta = std::min(mma_beta * 2.0, mma->beta_max);
+            stage_start = curve.size();
+            mma_iter = 0;
+          } else {
+            stop = true;
+          }
+        }
+      } else {
+        plateau = use_plateau &&
+                  mma_objective_plateau(curve.data(), static_cast<int>(curve.size()),
+                                        mma->plateau_window, mma->plateau_tol,
+                                        mma->plateau_min_drop, mma->plateau_flat_windows);
+        stop = use_plateau ? plateau : change < o.change_tol;
+      }
       if (P.observer) {
         // SimpIterationObservation + IterationPhaseTimes
         // (core/include/topopt/simp.hpp:870-1000). The projection and mask
@@ -2167,7 +2226,7 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
         ob.cg_mg_levels = ci.mg_levels;
         ob.cg_hier_built = ci.hier_built;
         ob.cg_mg_cycles_attempted = ci.mg_cycles_attempted;
-        ob.beta = st.project ? st.beta : 0.0;
+        ob.beta = st.project || st.cont ? cur_beta : 0.0;
         ob.plateau = plateau ? 1 : 0;
         ob.active_fraction = 1.0;
         ob.tail_prev_ms = t_prev_obs < 0.0 ? 0.0 : t_top - t_prev_obs;
@@ -2200,7 +2259,8 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
     r.converged = stage_converged && !r.cancelled ? 1 : 0;
     if (r.non_convergent || r.cancelled) break;
   }
-  physical(plan.back());
+  physical(plan.back().project || plan.back().cont,
+           plan.back().cont ? mma_beta : plan.back().beta);
   // volume fraction of the unpinned physical density (active_volfrac)
   launch_sum_vec<double>(P.xphys.p, nv, P.partials.p, vb, P.s);
   launch_sum_partials(P.partials.p, vb, P.scal.p, P.s);

@@ -200,14 +200,23 @@ struct MmaOpts {
   double plateau_tol = 1e-3;
   double plateau_min_drop = 0.05;
   int plateau_flat_windows = 3;
+  // SimpOptions::mma_projection: Heaviside projection with beta continuation
+  // (one dynamic stage, beta0 doubling to beta_max on plateaus)
+  bool projection = false;
+  double beta0 = 1.0, beta_max = 32.0;
 };
 // One MMA step (mma_update, core/src/simp/simp.cpp:1433-1488) on the design
 // set dset with its filter: mma_iter is 1-based (the asymptotes restart at 1
 // and 2); the state in P carries over between calls of one run.
+// d_xt (nullable) = filter_density(x) makes it mma_update_projected
+// (:1509-1636, masked :2861-2990): dc and dv chained through the projection
+// derivative at (beta, eta) before the filter transpose, the constraint on the
+// projected volume sum H(xtilde) over the design set.
 void mma_update_core(Problem& P, FilterCache& fc, const uint8_t* dset, double n_design,
                      int mma_iter, const double* d_x, const double* d_dc, double volfrac,
                      double move, double rho_min, double* d_xnew, double* lambda,
-                     double* change);
+                     double* change, const double* d_xt = nullptr, double beta = 1.0,
+                     double eta = 0.5);
 // mma_objective_plateau, core/src/simp/simp.cpp:1081-1134.
 bool mma_objective_plateau(const double* c, int n, int window, double rel_tol, double min_drop,
                            int min_flat_windows);
