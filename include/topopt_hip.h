@@ -204,11 +204,23 @@ int topohip_optimize(topohip_problem* p, const topohip_options* opt, double* des
 
 /* The plateau stop of an MMA run (SimpOptions::mma_plateau_*,
  * core/include/topopt/simp.hpp:1256-1276); plateau_window <= 0 keeps the
- * change_tol test. Defaults: 10, 1e-3, 0.05, 3. */
+ * change_tol test. Defaults: 10, 1e-3, 0.05, 3.
+ * adaptive_move != 0 turns on SimpOptions::adaptive_move (simp.hpp:1108-1118;
+ * core/src/simp/simp.cpp:140-163, 262-300, 2179-2194): from the third MMA step
+ * of a stage the move limit is multiplied by adaptive_move_shrink when the
+ * fraction of design voxels whose last two steps reversed direction (among
+ * those that moved on both) exceeds adaptive_move_osc_hi, by
+ * adaptive_move_grow when it is below adaptive_move_osc_lo, and clamped to
+ * [adaptive_move_min, adaptive_move_max]; the first two steps use the stage's
+ * move. Reference defaults 1.2, 0.7, 0.15, 0.30, 0.5, 0.02; invalid values are
+ * TOPOHIP_EINVAL. Zero-filled = off. */
 typedef struct {
   int plateau_window;
   double plateau_tol, plateau_min_drop;
   int plateau_flat_windows;
+  int adaptive_move;
+  double adaptive_move_grow, adaptive_move_shrink, adaptive_move_osc_lo, adaptive_move_osc_hi;
+  double adaptive_move_max, adaptive_move_min;
 } topohip_mma_options;
 
 /* simp_optimize with options.updater = MMA (core/src/simp/simp.cpp:2305-2311):

@@ -1478,6 +1478,22 @@ std::vector<double> oc_update_projected(const Grid& g, const DensityFilter& f,
 
 // validate_mma_projection_options (simp.cpp:112-135).
 void validate_mma_projection(const SimpOptions& o) {
+  // validate_adaptive_move_options (simp.cpp:140-163)
+  if (o.adaptive_move) {
+    if (!o.mma) throw std::invalid_argument("simp_optimize: adaptive_move requires the MMA updater");
+    if (!(o.adaptive_move_grow > 1.0) || !std::isfinite(o.adaptive_move_grow))
+      throw std::invalid_argument("simp_optimize: adaptive_move_grow must be finite and > 1");
+    if (!(o.adaptive_move_shrink > 0.0 && o.adaptive_move_shrink < 1.0))
+      throw std::invalid_argument("simp_optimize: adaptive_move_shrink must be in (0, 1)");
+    if (!(o.adaptive_move_osc_lo >= 0.0 && o.adaptive_move_osc_lo <= o.adaptive_move_osc_hi &&
+          o.adaptive_move_osc_hi <= 1.0))
+      throw std::invalid_argument(
+          "simp_optimize: adaptive_move_osc thresholds must satisfy 0 <= lo <= hi <= 1");
+    if (!(o.adaptive_move_min > 0.0) || !(o.adaptive_move_max >= o.adaptive_move_min) ||
+        !(o.adaptive_move_max < 1.0))
+      throw std::invalid_argument(
+          "simp_optimize: adaptive_move bounds must satisfy 0 < min <= max < 1");
+  }
   if (!o.mma_projection) return;
   if (!o.mma) throw std::invalid_argument("simp_optimize: mma_projection requires the MMA updater");
   if (!o.projection.empty())
@@ -1519,6 +1535,12 @@ SimpResult simp_optimize(const Grid& g, const SimpParams& p,
       plan.push_back({true, ps.beta, ps.move, ps.iterations, false});
   double mma_beta = o.mma_projection_beta0;
   std::size_t stage_start = 0;  // history index where the current beta stage began
+  double adaptive_move_val = 0.0;
+  std::vector<std::size_t> design_dof;  // the adaptive move's voxels: the solid ones
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i)
+        if (g.is_solid(i, j, k)) design_dof.push_back(g.index(i, j, k));
   const double eta = o.projection_eta;
   std::vector<double> x(g.voxels(), 0.0);
   for (int k = 0; k < g.nz; ++k)
@@ -1535,7 +1557,17 @@ SimpResult simp_optimize(const Grid& g, const SimpParams& p,
     bool stage_converged = false;
     for (int it = 0; it < st.iterations; ++it) {
       const double cur_beta = st.cont ? mma_beta : st.beta;
-      const double cur_move = st.cont ? mma_continuation_move(st.move, cur_beta) : st.move;
+      double cur_move = st.cont ? mma_continuation_move(st.move, cur_beta) : st.move;
+      if (o.adaptive_move) {
+        // seeded at the reference move for the first two MMA steps (and after a
+        // beta advance restarts the state), then evolved (:2179-2194)
+        if (mma_iter < 2 || mma_state.xold2.empty()) adaptive_move_val = cur_move;
+        else
+          adaptive_move_val = adapt_move_value(
+              adaptive_move_val,
+              oscillation_fraction(x, mma_state.xold1, mma_state.xold2, design_dof), o);
+        cur_move = adaptive_move_val;
+      }
       std::vector<double> xphys = f.filter_density(x);
       if (st.project || st.cont) project_solid(g, xphys, cur_beta, eta);
       SimpParams tp = p;
@@ -1560,7 +1592,7 @@ SimpResult simp_optimize(const Grid& g, const SimpParams& p,
         xn = oc_update_projected(g, f, x, xtilde, c.dc, st.beta, eta, o.volume_fraction, st.move,
                                  p.density_min);
       } else if (o.mma) {
-        xn = mma_update(mma_state, ++mma_iter, g, f, x, c.dc, o.volume_fraction, st.move,
+        xn = mma_update(mma_state, ++mma_iter, g, f, x, c.dc, o.volume_fraction, cur_move,
                         p.density_min);
       } else {
         xn = oc_update(g, f, x, c.dc, o.volume_fraction, st.move, p.density_min);
@@ -1747,6 +1779,10 @@ SimpResult simp_optimize_masked(const Grid& g, const SimpParams& p,
       plan.push_back({true, ps.beta, ps.move, ps.iterations, false});
   double mma_beta = o.mma_projection_beta0;
   std::size_t stage_start = 0;  // history index where the current beta stage began
+  double adaptive_move_val = 0.0;
+  std::vector<std::size_t> design_dof;  // the adaptive move's voxels: the Active ones
+  for (std::size_t e = 0; e < eff.size(); ++e)
+    if (eff[e] == kActive) design_dof.push_back(e);
   if (o.mma && !o.projection.empty())
     throw std::invalid_argument("simp_optimize: projection requires the OC updater");
   const double eta = o.projection_eta;
@@ -1758,7 +1794,17 @@ SimpResult simp_optimize_masked(const Grid& g, const SimpParams& p,
     for (int it = 0; it < st.iterations; ++it) {
       // physical density: filter, project the Active voxels, pin (:3196-3204)
       const double cur_beta = st.cont ? mma_beta : st.beta;
-      const double cur_move = st.cont ? mma_continuation_move(st.move, cur_beta) : st.move;
+      double cur_move = st.cont ? mma_continuation_move(st.move, cur_beta) : st.move;
+      if (o.adaptive_move) {
+        // seeded at the reference move for the first two MMA steps (and after a
+        // beta advance restarts the state), then evolved (:2179-2194)
+        if (mma_iter < 2 || mma_state.xold2.empty()) adaptive_move_val = cur_move;
+        else
+          adaptive_move_val = adapt_move_value(
+              adaptive_move_val,
+              oscillation_fraction(x, mma_state.xold1, mma_state.xold2, design_dof), o);
+        cur_move = adaptive_move_val;
+      }
       std::vector<double> xphys = f.filter_density(x);
       if (st.project || st.cont) project_active(eff, xphys, cur_beta, eta);
       apply_mask_pins(eff, xphys);
@@ -1784,7 +1830,7 @@ SimpResult simp_optimize_masked(const Grid& g, const SimpParams& p,
         xn = oc_update_masked_projected(g, f, eff, x, f.filter_density(x), c.dc, st.beta, eta,
                                         vf_target, st.move, p.density_min);
       else if (o.mma)
-        xn = mma_update(mma_state, ++mma_iter, g, f, x, c.dc, vf_target, st.move, p.density_min,
+        xn = mma_update(mma_state, ++mma_iter, g, f, x, c.dc, vf_target, cur_move, p.density_min,
                         nullptr, &active);
       else
         xn = oc_update_masked(g, f, eff, x, c.dc, vf_target, st.move, p.density_min);
@@ -1996,6 +2042,25 @@ namespace oracle {
 // fraction of xrange, the 1-D dual bisection).
 double mma_continuation_move(double move, double beta) {
   return beta > 8.0 ? move * (8.0 / beta) : move;
+}
+
+double oscillation_fraction(const std::vector<double>& x, const std::vector<double>& xold1,
+                            const std::vector<double>& xold2,
+                            const std::vector<std::size_t>& dof) {
+  long long moved = 0, reversed = 0;
+  for (std::size_t e : dof) {
+    const double d1 = x[e] - xold1[e], d2 = xold1[e] - xold2[e];
+    if (d1 == 0.0 || d2 == 0.0) continue;  // at a bound: no direction
+    ++moved;
+    if (d1 * d2 < 0.0) ++reversed;
+  }
+  return moved > 0 ? static_cast<double>(reversed) / static_cast<double>(moved) : 0.0;
+}
+
+double adapt_move_value(double cur, double f_osc, const SimpOptions& o) {
+  if (f_osc > o.adaptive_move_osc_hi) cur *= o.adaptive_move_shrink;
+  else if (f_osc < o.adaptive_move_osc_lo) cur *= o.adaptive_move_grow;
+  return std::min(std::max(cur, o.adaptive_move_min), o.adaptive_move_max);
 }
 
 std::vector<double> mma_update(MmaState& st, int mma_iter, const Grid& g, const DensityFilter& f,

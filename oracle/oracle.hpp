@@ -336,6 +336,12 @@ struct SimpOptions {
   bool mma_projection = false;
   double mma_projection_beta0 = 1.0;
   double mma_projection_beta_max = 32.0;
+  // SimpOptions::adaptive_move (simp.hpp:1108-1118, MMA only): the move limit
+  // evolves from the oscillation fraction of the design voxels' last two steps.
+  bool adaptive_move = false;
+  double adaptive_move_grow = 1.2, adaptive_move_shrink = 0.7;
+  double adaptive_move_osc_lo = 0.15, adaptive_move_osc_hi = 0.30;
+  double adaptive_move_max = 0.5, adaptive_move_min = 0.02;
 };
 struct SimpIteration { double compliance, change, volume_fraction; int cg_iterations; };
 struct SimpResult {
@@ -418,6 +424,13 @@ std::vector<double> mma_update(MmaState& st, int mma_iter, const Grid& g, const 
                                double eta = 0.5);
 // mma_continuation_move (simp.cpp:251-254): move * min(1, 8 / beta).
 double mma_continuation_move(double move, double beta);
+// oscillation_fraction / adapt_move_value (simp.cpp:262-300): among the design
+// voxels that moved on both of the last two steps, the fraction whose direction
+// reversed; the move shrinks above osc_hi, grows below osc_lo, clamped.
+double oscillation_fraction(const std::vector<double>& x, const std::vector<double>& xold1,
+                            const std::vector<double>& xold2,
+                            const std::vector<std::size_t>& dof);
+double adapt_move_value(double cur, double f_osc, const SimpOptions& o);
 // core/src/simp/simp.cpp:1081-1134.
 bool mma_objective_plateau(const std::vector<double>& c, int window, double rel_tol,
                            double min_drop, int min_flat_windows);

@@ -366,10 +366,12 @@ def fnv1a64(data, basis=None):
 def optimize_mma(s, volfrac, radius, iterations, E0=1.0, nu=0.3, penalty=3.0, rho_min=1e-3,
                  move=0.2, change_tol=0.0, cg_tol=1e-8, plateau_window=10, plateau_tol=1e-3,
                  plateau_min_drop=0.05, plateau_flat_windows=3, mask=None, projection=None,
-                 eta=0.5):
+                 eta=0.5, adaptive=None):
     """simp_optimize with SimpUpdater::MMA (plateau stop); with a design mask
     the masked loop with mma_update_masked. projection = (beta0, beta_max)
-    turns on SimpOptions::mma_projection (beta continuation)."""
+    turns on SimpOptions::mma_projection (beta continuation); adaptive =
+    (grow, shrink, osc_lo, osc_hi, max, min) turns on SimpOptions::adaptive_move."""
+    ad = None if adaptive is None else np.ascontiguousarray(adaptive, dtype=np.float64)
     b0, bmax = projection if projection else (1.0, 32.0)
     design = np.zeros(s.nvox)
     xphys = np.zeros(s.nvox)
@@ -384,7 +386,7 @@ def optimize_mma(s, volfrac, radius, iterations, E0=1.0, nu=0.3, penalty=3.0, rh
         C.c_double(change_tol), C.c_double(cg_tol), int(plateau_window), C.c_double(plateau_tol),
         C.c_double(plateau_min_drop), int(plateau_flat_windows),
         int(projection is not None), C.c_double(b0), C.c_double(bmax), C.c_double(eta),
-        mp, _ptr(design), _ptr(xphys),
+        _ptr(ad), mp, _ptr(design), _ptr(xphys),
         _ptr(hist), C.byref(it), C.byref(comp), C.byref(vf)))
     n = it.value
     return {"design": design, "xphys": xphys, "history": hist[: 4 * n].reshape(n, 4),

@@ -312,7 +312,7 @@ int oracle_optimize_mma(const OracleProblem* p, double E0, double nu, double pen
                         int max_iterations, double change_tol, double cg_tol, int window,
                         double plateau_tol, double min_drop, int flat_windows,
                         int mma_projection, double beta0, double beta_max, double eta,
-                        const std::uint8_t* mask, double* design,
+                        const double* adaptive, const std::uint8_t* mask, double* design,
                         double* xphys, double* hist, int* iterations, double* compliance,
                         double* final_volfrac) {
   return guarded([&] {
@@ -334,6 +334,15 @@ int oracle_optimize_mma(const OracleProblem* p, double E0, double nu, double pen
     o.mma_projection_beta0 = beta0;
     o.mma_projection_beta_max = beta_max;
     o.projection_eta = eta;
+    if (adaptive) {  // grow, shrink, osc_lo, osc_hi, max, min
+      o.adaptive_move = true;
+      o.adaptive_move_grow = adaptive[0];
+      o.adaptive_move_shrink = adaptive[1];
+      o.adaptive_move_osc_lo = adaptive[2];
+      o.adaptive_move_osc_hi = adaptive[3];
+      o.adaptive_move_max = adaptive[4];
+      o.adaptive_move_min = adaptive[5];
+    }
     const oracle::SimpResult r =
         mask ? oracle::simp_optimize_masked(g, sp, bcs_of(p), loads_of(p), o,
                                             std::vector<std::uint8_t>(mask, mask + g.voxels()))

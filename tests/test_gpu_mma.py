@@ -178,3 +178,23 @@ def test_masked_mma_projection_matches_oracle():
     assert np.abs(out["design"] - ref["design"]).max() < 1e-5
     assert np.all(out["xphys"][m == oracle.FROZEN_SOLID] == 1.0)
     assert np.all(out["xphys"][m == oracle.FROZEN_VOID] == 0.0)
+
+
+def test_adaptive_move_matches_oracle():
+    """SimpOptions::adaptive_move on the device (oscillation counts over the
+    design set, the seed / shrink / grow / clamp rule) against the oracle."""
+    nx, ny, nz = 16, 6, 6
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    ad = (1.2, 0.7, 0.15, 0.30, 0.5, 0.02)
+    s = oracle.Setup(nx, ny, nz, bcs=bcs, loads=loads)
+    ref = oracle.optimize_mma(s, 0.3, 1.5, 15, cg_tol=1e-10, plateau_window=0, adaptive=ad)
+    with ta.Problem(nx, ny, nz, bcs, loads, precision=ta.PREC_F64) as p:
+        out = p.optimize_mma(0.3, 1.5, 15, cg_tol=1e-10, plateau_window=0, adaptive=ad)
+        with pytest.raises(ValueError):
+            p.optimize_mma(0.3, 1.5, 3, adaptive=(1.2, 0.7, 0.15, 0.30, 1.0, 0.02))
+    hc, hr = out["history"], ref["history"]
+    print("change gpu", hc[:, 1], "\noracle", hr[:, 1], flush=True)
+    assert out["iterations"] == ref["iterations"] == 15
+    assert np.max(np.abs(hc[:, 0] - hr[:, 0]) / hr[:, 0]) < 1e-6
+    assert np.max(np.abs(hc[:, 1:3] - hr[:, 1:3])) < 1e-6
+    assert np.abs(out["design"] - ref["design"]).max() < 1e-5

@@ -182,3 +182,29 @@ def test_mma_projection_is_deterministic_and_validated(mma_projection_runs):
     assert oracle.mma_continuation_move(0.2, 4.0) == 0.2
     assert oracle.mma_continuation_move(0.2, 16.0) == 0.1
     assert oracle.mma_continuation_move(0.2, 32.0) == 0.05
+
+
+# ---- adaptive move limit (SimpOptions::adaptive_move) ----------------------
+AD = (1.2, 0.7, 0.15, 0.30, 0.5, 0.02)  # the reference defaults, simp.hpp:1112-1118
+
+
+def test_adaptive_move_is_seeded_then_evolves():
+    """The first two MMA steps use the stage's move (the seed), so the run
+    matches the fixed-move run until the third step; after it the move
+    follows the oscillation fraction, so the trajectories part (the reference's
+    adaptive-move contract, simp.cpp:2179-2194)."""
+    nx, ny, nz = 16, 6, 6
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    s = oracle.Setup(nx, ny, nz, bcs=bcs, loads=loads)
+    fixed = oracle.optimize_mma(s, 0.3, 1.5, 12, cg_tol=1e-9, plateau_window=0)
+    adapt = oracle.optimize_mma(s, 0.3, 1.5, 12, cg_tol=1e-9, plateau_window=0, adaptive=AD)
+    hf, ha = fixed["history"], adapt["history"]
+    assert np.array_equal(hf[:3, 0], ha[:3, 0]) and np.array_equal(hf[:2, 1], ha[:2, 1])
+    assert not np.array_equal(hf[3:, 0], ha[3:, 0])
+    # a grown move (monotone early steps) lets a step move further than 0.2
+    assert np.max(ha[:, 1]) > 0.2 + 1e-9 or np.max(ha[2:, 1]) < np.max(hf[2:, 1])
+    for bad in [(1.0, 0.7, 0.15, 0.30, 0.5, 0.02), (1.2, 1.0, 0.15, 0.30, 0.5, 0.02),
+                (1.2, 0.7, 0.4, 0.30, 0.5, 0.02), (1.2, 0.7, 0.15, 0.30, 1.0, 0.02),
+                (1.2, 0.7, 0.15, 0.30, 0.5, 0.0)]:
+        with pytest.raises(Exception, match="adaptive_move"):
+            oracle.optimize_mma(s, 0.3, 1.5, 3, adaptive=bad)

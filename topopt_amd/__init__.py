@@ -77,7 +77,11 @@ class Options(C.Structure):
 
 class MmaOptions(C.Structure):
     _fields_ = [("plateau_window", C.c_int), ("plateau_tol", C.c_double),
-                ("plateau_min_drop", C.c_double), ("plateau_flat_windows", C.c_int)]
+                ("plateau_min_drop", C.c_double), ("plateau_flat_windows", C.c_int),
+                ("adaptive_move", C.c_int), ("adaptive_move_grow", C.c_double),
+                ("adaptive_move_shrink", C.c_double), ("adaptive_move_osc_lo", C.c_double),
+                ("adaptive_move_osc_hi", C.c_double), ("adaptive_move_max", C.c_double),
+                ("adaptive_move_min", C.c_double)]
 
 
 def mma_plateau(c, window=10, rel_tol=1e-3, min_drop=0.05, flat_windows=3):
@@ -400,16 +404,22 @@ class Problem:
     def optimize_mma(self, volfrac, rmin, iterations, penalty=3.0, move=0.2, E0=1.0, rho_min=1e-3,
                      change_tol=0.0, cg_tol=1e-8, cg_max_iterations=0, continuation=None,
                      plateau_window=10, plateau_tol=1e-3, plateau_min_drop=0.05,
-                     plateau_flat_windows=3, mask=None, projection=None, eta=0.5):
+                     plateau_flat_windows=3, mask=None, projection=None, eta=0.5,
+                     adaptive=None):
         """simp_optimize with SimpUpdater::MMA (plateau stop); with a design
         mask the masked loop (mma_update_masked). projection = (beta0,
         beta_max) turns on SimpOptions::mma_projection: Heaviside projection
-        with beta continuation, iterations the safety cap."""
+        with beta continuation, iterations the safety cap. adaptive = (grow,
+        shrink, osc_lo, osc_hi, max, min) turns on SimpOptions::adaptive_move."""
         o = Options(volfrac, rmin, move, penalty, E0, rho_min, change_tol, cg_tol, int(iterations),
                     int(cg_max_iterations), 0.0, 0.0, 0.0, 0)
         if continuation:
             o.pc_p0, o.pc_p1, o.pc_step, o.pc_per_stage = continuation
         m = MmaOptions(int(plateau_window), plateau_tol, plateau_min_drop, int(plateau_flat_windows))
+        if adaptive is not None:
+            m.adaptive_move = 1
+            (m.adaptive_move_grow, m.adaptive_move_shrink, m.adaptive_move_osc_lo,
+             m.adaptive_move_osc_hi, m.adaptive_move_max, m.adaptive_move_min) = adaptive
         design = np.zeros(self.nvox)
         xphys = np.zeros(self.nvox)
         hist = np.zeros(4 * iterations)

@@ -397,18 +397,32 @@ int topohip_mma_update(topohip_problem* p, double radius, int mma_iter, const do
   });
 }
 
+// topohip_mma_options -> MmaOpts (NULL = the reference's defaults, adaptive move off).
+MmaOpts mma_opts_of(const topohip_mma_options* mma) {
+  MmaOpts m;
+  if (!mma) return m;
+  m.plateau_window = mma->plateau_window;
+  m.plateau_tol = mma->plateau_tol;
+  m.plateau_min_drop = mma->plateau_min_drop;
+  m.plateau_flat_windows = mma->plateau_flat_windows;
+  m.adaptive = mma->adaptive_move != 0;
+  if (m.adaptive) {
+    m.ad_grow = mma->adaptive_move_grow;
+    m.ad_shrink = mma->adaptive_move_shrink;
+    m.ad_osc_lo = mma->adaptive_move_osc_lo;
+    m.ad_osc_hi = mma->adaptive_move_osc_hi;
+    m.ad_max = mma->adaptive_move_max;
+    m.ad_min = mma->adaptive_move_min;
+  }
+  return m;
+}
+
 int topohip_optimize_mma(topohip_problem* p, const topohip_options* opt,
                          const topohip_mma_options* mma, const uint8_t* mask, double* design_out,
                          double* xphys_out, double* history, topohip_result* result) {
   return guarded([&] {
     need(p, "problem"); need(opt, "options");
-    MmaOpts m;
-    if (mma) {
-      m.plateau_window = mma->plateau_window;
-      m.plateau_tol = mma->plateau_tol;
-      m.plateau_min_drop = mma->plateau_min_drop;
-      m.plateau_flat_windows = mma->plateau_flat_windows;
-    }
+    MmaOpts m = mma_opts_of(mma);
     optimize(*p->P, *opt, design_out, xphys_out, history, result, mask, nullptr, 0.5, &m);
   });
 }
@@ -429,13 +443,7 @@ int topohip_optimize_mma_projected(topohip_problem* p, const topohip_options* op
                                    topohip_result* result) {
   return guarded([&] {
     need(p, "problem"); need(opt, "options");
-    MmaOpts m;
-    if (mma) {
-      m.plateau_window = mma->plateau_window;
-      m.plateau_tol = mma->plateau_tol;
-      m.plateau_min_drop = mma->plateau_min_drop;
-      m.plateau_flat_windows = mma->plateau_flat_windows;
-    }
+    MmaOpts m = mma_opts_of(mma);
     m.projection = true;
     m.beta0 = beta0;
     m.beta_max = beta_max;

@@ -2008,6 +2008,20 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
   // SimpOptions::mma_projection (validate_mma_projection_options,
   // core/src/simp/simp.cpp:112-135)
   const bool mma_cont = mma && mma->projection;
+  // validate_adaptive_move_options (core/src/simp/simp.cpp:140-163)
+  if (mma && mma->adaptive) {
+    if (!(mma->ad_grow > 1.0) || !std::isfinite(mma->ad_grow))
+      throw Error(TOPOHIP_EINVAL, "simp_optimize: adaptive_move_grow must be finite and > 1");
+    if (!(mma->ad_shrink > 0.0 && mma->ad_shrink < 1.0))
+      throw Error(TOPOHIP_EINVAL, "simp_optimize: adaptive_move_shrink must be in (0, 1)");
+    if (!(mma->ad_osc_lo >= 0.0 && mma->ad_osc_lo <= mma->ad_osc_hi && mma->ad_osc_hi <= 1.0))
+      throw Error(TOPOHIP_EINVAL, "simp_optimize: adaptive_move_osc thresholds must satisfy "
+                                  "0 <= lo <= hi <= 1");
+    if (!(mma->ad_min > 0.0) || !(mma->ad_max >= mma->ad_min) || !(mma->ad_max < 1.0))
+      throw Error(TOPOHIP_EINVAL, "simp_optimize: adaptive_move bounds must satisfy "
+                                  "0 < min <= max < 1");
+  }
+  double adaptive_move_val = 0.0;
   if (mma_cont) {
     if (!(eta > 0.0 && eta < 1.0))
       throw Error(TOPOHIP_EINVAL, "simp_optimize: projection_eta must be in (0, 1)");
@@ -2124,7 +2138,29 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
       double ta = t_top;
       const double cur_beta = st.cont ? mma_beta : st.beta;
       // mma_continuation_move (:251-254): move * min(1, 8 / beta)
-      const double cur_move = st.cont && cur_beta > 8.0 ? st.move * (8.0 / cur_beta) : st.move;
+      double cur_move = st.cont && cur_beta > 8.0 ? st.move * (8.0 / cur_beta) : st.move;
+      if (mma && mma->adaptive) {
+        // adaptive move (:2179-2194, oscillation_fraction / adapt_move_value
+        // :262-300): seeded at the reference move for the first two MMA steps
+        // (again after a beta advance restarts them), then shrunk above osc_hi,
+        // grown below osc_lo and clamped; the reading is over the design set
+        if (mma_iter < 2) {
+          adaptive_move_val = cur_move;
+        } else {
+          const int ob = oc_blocks(nv);
+          launch_osc_count(P.xd.p, P.mma_x1.p, P.mma_x2.p, sol, nv, P.partials.p,
+                           P.partials2.p, ob, P.s);
+          launch_sum_partials(P.partials.p, ob, P.scal.p, P.s);
+          launch_sum_partials(P.partials2.p, ob, P.scal.p + 1, P.s);
+          d2h(P.scal_h, P.scal.p, 2, P.s);
+          const double osc = P.scal_h[0] > 0.0 ? P.scal_h[1] / P.scal_h[0] : 0.0;
+          double v = adaptive_move_val;
+          if (osc > mma->ad_osc_hi) v *= mma->ad_shrink;
+          else if (osc < mma->ad_osc_lo) v *= mma->ad_grow;
+          adaptive_move_val = std::min(std::max(v, mma->ad_min), mma->ad_max);
+        }
+        cur_move = adaptive_move_val;
+      }
       physical(st.project || st.cont, cur_beta);
       if (mask) launch_mask_pins(P.xphys.p, d_eff.p, nv, P.s);
       sync(P);
@@ -2155,7 +2191,7 @@ void optimize(Problem& P, const topohip_options& o, double* design_out, double* 
         oc_update_projected_core(P, fc, sol, nd, mask != nullptr, P.xd.p, P.xtil.p, P.dc.p,
                                  st.beta, eta, o.volfrac, st.move, o.rho_min, P.xnew.p, &change);
       else if (mma)
-        mma_update_core(P, fc, sol, nd, ++mma_iter, P.xd.p, P.dc.p, o.volfrac, st.move, o.rho_min,
+        mma_update_core(P, fc, sol, nd, ++mma_iter, P.xd.p, P.dc.p, o.volfrac, cur_move, o.rho_min,
                         P.xnew.p, nullptr, &change);
       else
         oc_update_core(P, fc, sol, nd, mask != nullptr, P.xd.p, P.dc.p, o.volfrac, st.move,

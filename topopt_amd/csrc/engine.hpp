@@ -204,6 +204,10 @@ struct MmaOpts {
   // (one dynamic stage, beta0 doubling to beta_max on plateaus)
   bool projection = false;
   double beta0 = 1.0, beta_max = 32.0;
+  // SimpOptions::adaptive_move: the move evolves from the oscillation fraction
+  bool adaptive = false;
+  double ad_grow = 1.2, ad_shrink = 0.7, ad_osc_lo = 0.15, ad_osc_hi = 0.30;
+  double ad_max = 0.5, ad_min = 0.02;
 };
 // One MMA step (mma_update, core/src/simp/simp.cpp:1433-1488) on the design
 // set dset with its filter: mma_iter is 1-based (the asymptotes restart at 1

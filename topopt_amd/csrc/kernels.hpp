@@ -326,6 +326,10 @@ void launch_mma_decide(MmaSearch* st, const double* partials, int n, hipStream_t
 void launch_mma_final(const MmaArgs& a, const double* x, const uint8_t* dset, long long n,
                       const MmaSearch* st, double* xnew, double* partials, int nblocks,
                       hipStream_t s);
+// oscillation_fraction's per-block counts over the design set dset (null = all):
+// voxels that moved on both steps x2 -> x1 -> x, and those that reversed.
+void launch_osc_count(const double* x, const double* x1, const double* x2, const uint8_t* dset,
+                      long long n, double* moved, double* reversed, int nblocks, hipStream_t s);
 // Design-mask pins on a physical density: eff 1 (FrozenSolid) -> 1, 2 (FrozenVoid) -> 0.
 void launch_mask_pins(double* xphys, const uint8_t* eff, long long n, hipStream_t s);
 void launch_max_partials(const double* partials, int n, double* out, hipStream_t s);

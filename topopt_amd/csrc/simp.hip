@@ -944,6 +944,40 @@ void launch_mma_decide(MmaSearch* st, const double* partials, int n, hipStream_t
   hipLaunchKernelGGL(mma_decide_k, dim3(1), dim3(RB), 0, s, st, partials, n);
   TOPOHIP_CHECK(hipGetLastError());
 }
+namespace {
+// oscillation_fraction's counts (core/src/simp/simp.cpp:262-276) per block:
+// design voxels that moved on both of the last two steps, and those whose
+// direction reversed (integers, summed exactly as doubles).
+__global__ void osc_count_k(const double* __restrict__ x, const double* __restrict__ x1,
+                            const double* __restrict__ x2, const uint8_t* __restrict__ dset,
+                            long long n, double* __restrict__ moved,
+                            double* __restrict__ reversed) {
+  double m = 0.0, r = 0.0;
+  for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < n;
+       e += static_cast<long long>(gridDim.x) * blockDim.x) {
+    if (dset && !dset[e]) continue;
+    const double d1 = x[e] - x1[e], d2 = x1[e] - x2[e];
+    if (d1 == 0.0 || d2 == 0.0) continue;
+    m += 1.0;
+    if (d1 * d2 < 0.0) r += 1.0;
+  }
+  m = block_sum(m);
+  __syncthreads();  // block_sum's LDS is reused
+  r = block_sum(r);
+  if (threadIdx.x == 0) {
+    moved[blockIdx.x] = m;
+    reversed[blockIdx.x] = r;
+  }
+}
+}  // namespace
+
+void launch_osc_count(const double* x, const double* x1, const double* x2, const uint8_t* dset,
+                      long long n, double* moved, double* reversed, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(osc_count_k, dim3(nblocks), dim3(RB), 0, s, x, x1, x2, dset, n, moved,
+                     reversed);
+  TOPOHIP_CHECK(hipGetLastError());
+}
+
 void launch_mma_final(const MmaArgs& a, const double* x, const uint8_t* dset, long long n,
                       const MmaSearch* st, double* xnew, double* partials, int nblocks,
                       hipStream_t s) {
