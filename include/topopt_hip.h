@@ -638,6 +638,21 @@ double topohip_steady_ms(void);
  * latched. */
 void topohip_reset_mg_stagnation_latch(void);
 int topohip_mg_stagnation_latched(void);
+/* The latch's measurement-only re-arm period of the CALLING THREAD:
+ * fea_matfree_set_mg_rearm_period / _mg_rearm_period / _mg_rearm_attempts /
+ * _mg_rearm_carries (core/include/topopt/fea.hpp:1093-1116;
+ * core/src/fea/multigrid.cpp:152-180, 2144-2156, 2248-2262, 2436-2454).
+ * 0 (the default) keeps the latch for the run. n > 0: once latched, the n-th
+ * latched solve call clears the latch and attempts multigrid again, with the
+ * stagnation count one short of the threshold, so a stagnating retry
+ * re-latches at once; attempts counts those solves, carries the ones that
+ * converged (a block solve counts once, and carries when all its load cases
+ * converged). The period survives topohip_reset_mg_stagnation_latch, which
+ * zeroes the counters; setting it restarts the latched-solve count. */
+void topohip_set_mg_rearm_period(int period);
+int topohip_mg_rearm_period(void);
+long long topohip_mg_rearm_attempts(void);
+long long topohip_mg_rearm_carries(void);
 /* SimpOptions::cancel (core/include/topopt/simp.hpp:1505-1521): a caller-owned
  * flag (NULL clears) polled once per design iteration, at its start, by
  * topohip_optimize, topohip_optimize_masked, topohip_optimize_mma and

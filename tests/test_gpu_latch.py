@@ -104,3 +104,34 @@ def test_latch_is_per_thread():
         assert seen == {"latched": False, "mg": 1}
         assert ta.mg_stagnation_latched()
     ta.reset_mg_stagnation_latch()
+
+
+def test_rearm_period_retries_multigrid():
+    """The measurement-only re-arm period (multigrid.cpp:152-180, 2144-2156,
+    2248-2262): 0 keeps the latch; 1 lets every latched solve retry multigrid,
+    a stagnating retry re-latches at once (one stagnation, not three) and a
+    converging one clears the latch and counts as a carry. The period survives
+    the reset, which zeroes the counters."""
+    nx, ny, nz = DIMS
+    bcs, loads = problems.cantilever(nx, ny, nz)
+    E = np.full(nx * ny * nz, 0.4 ** 3)
+    ta.reset_mg_stagnation_latch()
+    try:
+        with ta.Problem(nx, ny, nz, bcs, loads) as p:
+            for _ in range(3):
+                stagnate(p, E)
+            assert ta.mg_stagnation_latched()
+            _, info = p.solve(E, tol=1e-8)  # period 0: stays latched, no hierarchy
+            assert info["hier_built"] == 0 and ta.mg_rearm_counts() == (0, 0, 0)
+            ta.set_mg_rearm_period(1)
+            stagnate(p, E)  # re-armed retry stagnates: latched again at once
+            assert ta.mg_stagnation_latched() and ta.mg_rearm_counts() == (1, 1, 0)
+            _, info = p.solve(E, tol=1e-8)  # re-armed retry converges with multigrid
+            print("rearmed solve", info["used_multigrid"], ta.mg_rearm_counts(), flush=True)
+            assert info["hier_built"] == 1 and info["used_multigrid"] == 1
+            assert not ta.mg_stagnation_latched() and ta.mg_rearm_counts() == (1, 2, 1)
+            ta.reset_mg_stagnation_latch()
+            assert ta.mg_rearm_counts() == (1, 0, 0)
+    finally:
+        ta.set_mg_rearm_period(0)
+        ta.reset_mg_stagnation_latch()

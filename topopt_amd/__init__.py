@@ -650,6 +650,21 @@ def mg_stagnation_latched():
     return bool(lib().topohip_mg_stagnation_latched())
 
 
+def set_mg_rearm_period(period):
+    """fea_matfree_set_mg_rearm_period for the calling thread (measurement only;
+    0 = off, the default)."""
+    lib().topohip_set_mg_rearm_period(int(period))
+
+
+def mg_rearm_counts():
+    """(period, attempts, carries) of the calling thread's latch re-arm."""
+    L = lib()
+    L.topohip_mg_rearm_attempts.restype = C.c_longlong
+    L.topohip_mg_rearm_carries.restype = C.c_longlong
+    return (int(L.topohip_mg_rearm_period()), int(L.topohip_mg_rearm_attempts()),
+            int(L.topohip_mg_rearm_carries()))
+
+
 def version():
     return lib().topohip_version().decode()
 

@@ -338,6 +338,14 @@ void topohip_reset_mg_stagnation_latch(void) { reset_mg_stagnation_latch(); }
 
 int topohip_mg_stagnation_latched(void) { return mg_stagnation_latched() ? 1 : 0; }
 
+void topohip_set_mg_rearm_period(int period) { set_mg_rearm_period(period); }
+
+int topohip_mg_rearm_period(void) { return mg_rearm_period(); }
+
+long long topohip_mg_rearm_attempts(void) { return mg_rearm_attempts(); }
+
+long long topohip_mg_rearm_carries(void) { return mg_rearm_carries(); }
+
 int topohip_set_smoother(topohip_problem* p, int kind, int degree) {
   return guarded([&] {
     need(p, "problem");

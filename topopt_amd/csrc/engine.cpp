@@ -65,6 +65,15 @@ bool deadline_passed() {
 constexpr int kMgLatchThreshold = 3;
 thread_local int t_mg_stagnations = 0;
 thread_local bool t_mg_latched = false;
+// The measurement-only re-arm period (multigrid.cpp:152-180, 2144-2156,
+// 2248-2262): 0 (the default) keeps the latch for the run; n > 0 clears it
+// after n latched solves so one solve attempts multigrid again, with the
+// stagnation count one short of the threshold so a stagnating retry re-latches
+// at once. Attempts and carries (re-armed solves that converged) are counted.
+thread_local int t_mg_rearm_period = 0;
+thread_local int t_mg_latched_solves = 0;
+thread_local long long t_mg_rearm_attempts = 0, t_mg_rearm_carries = 0;
+thread_local bool t_mg_rearmed = false;  // the current solve re-armed the latch
 
 template <typename T>
 void h2d(T* d, const T* h, size_t n, hipStream_t s) {
@@ -988,7 +997,19 @@ void pcg(Problem& P, Prec pc, int nc, double tol, int cap, bool warm, const bool
 void reset_mg_stagnation_latch() {
   t_mg_stagnations = 0;
   t_mg_latched = false;
+  t_mg_latched_solves = 0;
+  t_mg_rearm_attempts = 0;
+  t_mg_rearm_carries = 0;
 }
+
+// (the period survives the reset, as the reference's harness posture does)
+void set_mg_rearm_period(int period) {
+  t_mg_rearm_period = period > 0 ? period : 0;
+  t_mg_latched_solves = 0;
+}
+int mg_rearm_period() { return t_mg_rearm_period; }
+long long mg_rearm_attempts() { return t_mg_rearm_attempts; }
+long long mg_rearm_carries() { return t_mg_rearm_carries; }
 
 bool mg_stagnation_latched() { return t_mg_latched; }
 
@@ -1379,6 +1400,16 @@ void cheb_estimate(Problem& P) {
 
 void build_hierarchy(Problem& P, Timers& t) {
   const double t0 = now_ms();
+  // re-arm check, once per solve call before its build (a block solve counts
+  // once and re-arms all its load cases together)
+  t_mg_rearmed = false;
+  if (t_mg_latched && t_mg_rearm_period > 0 && ++t_mg_latched_solves >= t_mg_rearm_period) {
+    t_mg_latched = false;
+    t_mg_latched_solves = 0;
+    t_mg_stagnations = kMgLatchThreshold - 1;
+    t_mg_rearmed = true;
+    ++t_mg_rearm_attempts;
+  }
   P.mg64_ready = false;
   const long long nv = P.g.elems();
   const bool mixed = P.precision == TOPOHIP_PREC_MIXED;
@@ -1583,6 +1614,9 @@ void solve_cases(Problem& P, int nc, double tol, int max_it, const double* d_gue
           t_mg_latched = true;
         }
       }
+      // a re-armed attempt carries when every case converged with multigrid
+      if (t_mg_rearmed && all_ok) ++t_mg_rearm_carries;
+      t_mg_rearmed = false;
     }
     if (!all_ok) {
       // Exact Jacobi-PCG fallback (core/src/fea/matfree.cpp:841-1076), FP64, for

@@ -244,6 +244,12 @@ double set_solve_deadline_ms(double absolute_steady_ms);
 // for the calling thread.
 void reset_mg_stagnation_latch();
 bool mg_stagnation_latched();
+// fea_matfree_set_mg_rearm_period and its counters (core/include/topopt/
+// fea.hpp:1093-1116): measurement-only, per thread, 0 = off (the default).
+void set_mg_rearm_period(int period);
+int mg_rearm_period();
+long long mg_rearm_attempts();
+long long mg_rearm_carries();
 double steady_ms();
 // fea_von_mises_field of u_host (3 * nodes; null = P.u of the last solve) for
 // Young's modulus E: FP64 field into vm64 and/or its float narrowing into
